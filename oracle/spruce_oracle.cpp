@@ -1,0 +1,707 @@
+// CPU FP64 restatement of the SPRUCE time step -- TEST INFRASTRUCTURE ONLY (the checker of the
+// GPU path; see spruce_oracle.h).  Parity pins: tests/golden/*.npz, produced by the compiled
+// reference, must be reproduced bit-for-bit (tests/test_oracle_golden.py).
+//
+// Every whole-array operation below is one separately rounded elementwise pass, evaluated left to
+// right in the reference's expression order (Grid operators, source/mhd/grid.cpp:528-582); no
+// fused multiply-add is used (the reference Makefile builds baseline x86-64, Makefile:34), and
+// min/max follow std::min/std::max, i.e. NaN survives only as the first operand
+// (grid.cpp:110-113, 161-163).
+#include "spruce_oracle.h"
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace orc {
+
+static thread_local std::string g_err;
+
+constexpr double K_B = 1.3807e-16;        // source/constants.hpp:8
+constexpr double M_ELECTRON = 9.1094e-28; // constants.hpp:9
+constexpr double KAPPA_0 = 1.0e-6;        // constants.hpp:14
+constexpr double PI = 3.14159265358979323846;  // constants.hpp:15
+constexpr int N_GHOST = 2;                // constants.hpp:4
+
+struct fail : std::runtime_error { using std::runtime_error::runtime_error; };
+static void require(bool c, const std::string& msg) { if (!c) throw fail(msg); }
+
+// ---------------------------------------------------------------------------------------------
+// Array type: row-major (x outer, y contiguous) like Grid (grid.cpp:516-526).
+// ---------------------------------------------------------------------------------------------
+struct A {
+  int r = 1, c = 1;
+  std::vector<double> v = std::vector<double>(1, 0.0);
+  A() = default;
+  A(int r_, int c_, double x = 0.0) : r(r_), c(c_), v((size_t)r_ * c_, x) {}
+  double& operator()(int i, int j) { return v[(size_t)i * c + j]; }
+  double operator()(int i, int j) const { return v[(size_t)i * c + j]; }
+  size_t size() const { return v.size(); }
+};
+
+template <class F> static A map1(const A& a, F f) {
+  A o(a.r, a.c);
+  const size_t n = a.size();
+#pragma omp parallel for if (n > 65536)
+  for (size_t k = 0; k < n; ++k) o.v[k] = f(a.v[k]);
+  return o;
+}
+template <class F> static A map2(const A& a, const A& b, F f) {
+  require(a.r == b.r && a.c == b.c, "grid shape mismatch");
+  A o(a.r, a.c);
+  const size_t n = a.size();
+#pragma omp parallel for if (n > 65536)
+  for (size_t k = 0; k < n; ++k) o.v[k] = f(a.v[k], b.v[k]);
+  return o;
+}
+static A operator+(const A& a, const A& b) { return map2(a, b, [](double x, double y) { return x + y; }); }
+static A operator-(const A& a, const A& b) { return map2(a, b, [](double x, double y) { return x - y; }); }
+static A operator*(const A& a, const A& b) { return map2(a, b, [](double x, double y) { return x * y; }); }
+static A operator/(const A& a, const A& b) { return map2(a, b, [](double x, double y) { return x / y; }); }
+static A operator+(const A& a, double s) { return map1(a, [s](double x) { return x + s; }); }
+static A operator+(double s, const A& a) { return a + s; }
+static A operator-(const A& a, double s) { return map1(a, [s](double x) { return x - s; }); }
+static A operator-(double s, const A& a) { return map1(a, [s](double x) { return s - x; }); }
+static A operator*(const A& a, double s) { return map1(a, [s](double x) { return x * s; }); }
+static A operator*(double s, const A& a) { return a * s; }
+static A operator/(const A& a, double s) { return map1(a, [s](double x) { return x / s; }); }
+static A operator/(double s, const A& a) { return map1(a, [s](double x) { return s / x; }); }
+static A operator-(const A& a) { return -1.0 * a; }
+static A& operator+=(A& a, const A& b) { a = a + b; return a; }
+static A& operator*=(A& a, const A& b) { a = a * b; return a; }
+static A vmax(const A& a, double s) { return map1(a, [s](double x) { return (x < s) ? s : x; }); }
+static A vmin(const A& a, double s) { return map1(a, [s](double x) { return (s < x) ? s : x; }); }
+static A vmax(const A& a, const A& b) { return map2(a, b, [](double x, double y) { return (x < y) ? y : x; }); }
+static A vmin(const A& a, const A& b) { return map2(a, b, [](double x, double y) { return (y < x) ? y : x; }); }
+static A vsqrt(const A& a) { return map1(a, [](double x) { return std::sqrt(x); }); }
+static A vabs(const A& a) { return map1(a, [](double x) { return std::fabs(x); }); }
+static A vpow(const A& a, double p) { return map1(a, [p](double x) { return std::pow(x, p); }); }
+static A sq(const A& a) { return a * a; }
+
+// Range minimum with std::min semantics; the reference asserts the result is below DBL_MAX
+// (grid.cpp:71-82).
+static double range_min(const A& a, int il, int jl, int iu, int ju) {
+  double m = DBL_MAX;
+  for (int i = il; i <= iu; ++i)
+    for (int j = jl; j <= ju; ++j) { double x = a(i, j); m = (x < m) ? x : m; }
+  require(m < DBL_MAX, "min reduction found no finite value");
+  return m;
+}
+static double whole_min(const A& a) { return range_min(a, 0, 0, a.r - 1, a.c - 1); }
+static double whole_max(const A& a) {  // grid.cpp:115-123
+  double m = -DBL_MAX;
+  for (double x : a.v) m = (m < x) ? x : m;
+  require(m > -DBL_MAX, "max reduction found no finite value");
+  return m;
+}
+
+enum BC { Periodic = 0, Open = 1, Fixed = 2, Reflect = 3, OpenMoC = 4, OpenUCNP = 5 };
+enum Integ { Euler = 0, RK2 = 1, RK4 = 2 };
+enum PDG { d_x = 0, d_y, pos_x, pos_y, be_x, be_y, be_z };
+static const char* PD_NAMES[7] = {"d_x", "d_y", "pos_x", "pos_y", "be_x", "be_y", "be_z"};
+
+struct Domain;
+using Grids = std::vector<A>;
+
+// ---------------------------------------------------------------------------------------------
+// EquationSet and Module interfaces (equationset.hpp:18-171; module.hpp:15-58)
+// ---------------------------------------------------------------------------------------------
+struct EqSet {
+  Domain& pd;
+  std::vector<std::string> names;
+  Grids g;
+  explicit EqSet(Domain& d, std::vector<std::string> n) : pd(d), names(std::move(n)), g(names.size()) {}
+  virtual ~EqSet() = default;
+  int idx(const std::string& n) const {
+    for (size_t i = 0; i < names.size(); ++i) if (names[i] == n) return (int)i;
+    throw fail("variable <" + n + "> not recognized");
+  }
+  bool has(const std::string& n) const { for (auto& s : names) if (s == n) return true; return false; }
+  virtual std::vector<int> state_vars() const = 0;
+  virtual std::vector<int> evolved() const = 0;
+  virtual std::vector<int> densities() const = 0;
+  virtual std::vector<int> thermal_energies() const = 0;
+  virtual std::vector<std::vector<int>> momenta() const = 0;
+  virtual std::vector<int> fields() const = 0;
+  virtual int num_species() const { return (int)densities().size(); }
+  virtual void parse(const std::vector<std::string>& k, const std::vector<std::string>& v) = 0;
+  virtual Grids rhs(const Grids& grids) const = 0;
+  virtual void evolved_from_state(Grids& grids) const = 0;
+  virtual void derived(Grids& grids) const = 0;
+  virtual void dt(Grids& grids) const = 0;
+  virtual void minimums(Grids& grids) const = 0;
+  virtual void setup_derived() {}
+
+  Grids time_derivs(const Grids& grids) const;        // equationset.cpp:204-210
+  void propagate(Grids& grids) const;                 // equationset.cpp:212-220
+  void apply(Grids& grids, const Grids& k, double step) const;  // equationset.cpp:222-230
+  void populate(Grids& grids) const;                  // equationset.cpp:96-104
+  int evolved_index(const std::string& n) const {
+    auto ev = evolved();
+    for (size_t i = 0; i < ev.size(); ++i) if (names[ev[i]] == n) return (int)i;
+    throw fail("<" + n + "> is not an evolved variable");
+  }
+};
+
+struct Module {
+  Domain& pd;
+  explicit Module(Domain& d) : pd(d) {}
+  virtual ~Module() = default;
+  virtual void parse(const std::vector<std::string>& k, const std::vector<std::string>& v) = 0;
+  virtual void setup() {}
+  virtual void pre(double) {}
+  virtual void iterate(double) {}
+  virtual void post(double) {}
+  virtual void rhs_add(const Grids&, Grids&) {}
+  virtual void pre_recompute(Grids&) const {}
+  virtual int subcycles() const { return -1; }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Domain: bounds, ghost fill, differential operators, time step
+// ---------------------------------------------------------------------------------------------
+struct Domain {
+  int nx = 0, ny = 0;
+  int bc[4] = {0, 0, 0, 0};
+  int integ = Euler;
+  double eps = 0, nmin = 0, tmin = 0, temin = 0, obs = 0, obdb = 1, m_ion = 0, gamma = 0;
+  int xl = 0, xu = 0, yl = 0, yu = 0, xl_dt = 0, xu_dt = 0, yl_dt = 0, yu_dt = 0;
+  A mask;
+  A pd[7];
+  bool pd_set[7] = {false, false, false, false, false, false, false};
+  std::unique_ptr<EqSet> eqs;
+  std::vector<std::unique_ptr<Module>> mods;
+  double time = 0.0;
+  int iter = 0;
+
+  bool xper() const { return bc[0] == Periodic && bc[1] == Periodic; }
+  bool yper() const { return bc[2] == Periodic && bc[3] == Periodic; }
+
+  // plasmadomain.cpp:138-161
+  void iteration_bounds() {
+    require(nx > 2 * N_GHOST && ny > 2 * N_GHOST, "grid too small for ghost zones");
+    for (int k = 0; k < 4; ++k) require(bc[k] != OpenMoC, "open_moc boundaries are not supported");
+    xl = bc[0] == Periodic ? 0 : N_GHOST;
+    xu = bc[1] == Periodic ? nx - 1 : nx - N_GHOST - 1;
+    yl = bc[2] == Periodic ? 0 : N_GHOST;
+    yu = bc[3] == Periodic ? ny - 1 : ny - N_GHOST - 1;
+    require((bc[0] == Periodic) == (bc[1] == Periodic) && (bc[2] == Periodic) == (bc[3] == Periodic),
+            "an axis must be periodic on both sides or on neither");
+    mask = A(nx, ny, 0.0);
+    for (int i = xl; i <= xu; ++i) for (int j = yl; j <= yu; ++j) mask(i, j) = 1.0;
+    xl_dt = xl; xu_dt = xu; yl_dt = yl; yu_dt = yu;
+  }
+
+  // derivs.cpp:477-487: linear interpolation to the face between two cells
+  double interp(const A& q, int i1, int j1, int i2, int j2) const {
+    double a = q(i1, j1), b = q(i2, j2), da, db;
+    if (i1 == i2) { da = 0.5 * pd[d_y](i1, j1); db = 0.5 * pd[d_y](i2, j2); }
+    else { da = 0.5 * pd[d_x](i1, j1); db = 0.5 * pd[d_x](i2, j2); }
+    return (a * db + b * da) / (db + da);
+  }
+  // derivs.cpp:490-499: linear extrapolation through two cells to the next face
+  double extrap(const A& q, int i1, int j1, int i2, int j2) const {
+    double a = q(i1, j1), b = q(i2, j2), da, db;
+    if (i1 == i2) { da = 0.5 * pd[d_y](i1, j1); db = 0.5 * pd[d_y](i2, j2); }
+    else { da = 0.5 * pd[d_x](i1, j1); db = 0.5 * pd[d_x](i2, j2); }
+    return a + (b - a) * (da + 2.0 * db) / (da + db);
+  }
+
+  // Barton upwind face values, derivs.cpp:10-73.  Face (i,j) lies between (i-1,j) and (i,j)
+  // for axis 0, between (i,j-1) and (i,j) for axis 1.
+  A upwind(const A& q, const A& vel, int axis, int il, int jl, int iu, int ju) const {
+    A s(nx + 1 - axis, ny + axis, 0.0);
+    const bool per = axis == 0 ? xper() : yper();
+    const int n = axis == 0 ? nx : ny;
+#pragma omp parallel for collapse(2) if (nx * ny > 65536)
+    for (int i = il; i <= iu + 1 - axis; ++i) {
+      for (int j = jl; j <= ju + axis; ++j) {
+        int c = axis == 0 ? i : j;  // index of the cell above the face along the axis
+        int m2 = c - 2, m1 = c - 1, p1 = c + 1;
+        if (per) {
+          if (c == n) continue;
+          m2 = (m2 + n) % n; m1 = (m1 + n) % n; p1 = (p1 + n) % n;
+        }
+        auto at = [&](int k, int& ii, int& jj) { if (axis == 0) { ii = k; jj = j; } else { ii = i; jj = k; } };
+        int i0, j0, i1, j1, i2, j2, i3, j3;
+        at(m2, i0, j0); at(m1, i1, j1); at(c, i2, j2); at(p1, i3, j3);
+        double d2 = interp(q, i1, j1, i2, j2);
+        double vf = interp(vel, i1, j1, i2, j2);
+        double out;
+        if (vf > 0.0) {
+          double d3 = q(i1, j1), d1 = extrap(q, i0, j0, i1, j1);
+          if (q(i2, j2) <= q(i1, j1)) out = std::min(d3, std::max(d1, d2));
+          else out = std::max(d3, std::min(d1, d2));
+        } else if (vf < 0.0) {
+          double d3 = q(i2, j2), d1 = extrap(q, i3, j3, i2, j2);
+          if (q(i2, j2) <= q(i1, j1)) out = std::max(d3, std::min(d1, d2));
+          else out = std::min(d3, std::max(d1, d2));
+        } else {
+          out = d2;
+        }
+        s(i2, j2) = out;
+      }
+    }
+    return s;
+  }
+
+  // derivs.cpp:122-162 (multiply_vel = true)
+  A tderiv(const A& q, const A& vel, int axis) const {
+    A s = upwind(q, vel, axis, xl, yl, xu, yu);
+    A out(nx, ny, 0.0);
+    const bool per = axis == 0 ? xper() : yper();
+    const int n = axis == 0 ? nx : ny;
+    // denom = d_x*(1-axis) + d_y*axis, built as whole-grid operations (derivs.cpp:128)
+    A denom = pd[d_x] * (double)(1 - axis) + pd[d_y] * (double)axis;
+#pragma omp parallel for collapse(2) if (nx * ny > 65536)
+    for (int i = xl; i <= xu; ++i) {
+      for (int j = yl; j <= yu; ++j) {
+        int c = axis == 0 ? i : j, m = c - 1, p = c + 1;
+        if (per) { m = (m + n) % n; p = (p + n) % n; }
+        int im = axis == 0 ? m : i, jm = axis == 0 ? j : m, ip = axis == 0 ? p : i, jp = axis == 0 ? j : p;
+        out(i, j) = (s(ip, jp) * interp(vel, i, j, ip, jp) - s(i, j) * interp(vel, im, jm, i, j)) / denom(i, j);
+      }
+    }
+    return out;
+  }
+  A tdiv(const A& q, const A& vx, const A& vy) const { return tderiv(q, vx, 0) + tderiv(q, vy, 1); }  // derivs.cpp:216-220
+
+  // derivs.cpp:223-264: central derivative through face interpolants
+  A deriv(const A& q, int axis) const {
+    A out(nx, ny, 0.0);
+    const bool per = axis == 0 ? xper() : yper();
+    const int n = axis == 0 ? nx : ny;
+#pragma omp parallel for collapse(2) if (nx * ny > 65536)
+    for (int i = xl; i <= xu; ++i) {
+      for (int j = yl; j <= yu; ++j) {
+        int c = axis == 0 ? i : j, m = c - 1, p = c + 1;
+        if (per) { m = (m + n) % n; p = (p + n) % n; }
+        int im = axis == 0 ? m : i, jm = axis == 0 ? j : m, ip = axis == 0 ? p : i, jp = axis == 0 ? j : p;
+        double den = axis == 0 ? pd[d_x](i, j) : pd[d_y](i, j);
+        out(i, j) = (interp(q, i, j, ip, jp) - interp(q, im, jm, i, j)) / den;
+      }
+    }
+    return out;
+  }
+  // derivs.cpp:417-455
+  A second(const A& q, int axis) const {
+    A out(nx, ny, 0.0);
+    A denom = sq(0.5 * pd[d_x] * (double)(1 - axis) + 0.5 * pd[d_y] * (double)axis);
+    const bool per = axis == 0 ? xper() : yper();
+    const int n = axis == 0 ? nx : ny;
+#pragma omp parallel for collapse(2) if (nx * ny > 65536)
+    for (int i = xl; i <= xu; ++i) {
+      for (int j = yl; j <= yu; ++j) {
+        int c = axis == 0 ? i : j, m = c - 1, p = c + 1;
+        if (per) { m = (m + n) % n; p = (p + n) % n; }
+        int im = axis == 0 ? m : i, jm = axis == 0 ? j : m, ip = axis == 0 ? p : i, jp = axis == 0 ? j : p;
+        out(i, j) = (interp(q, i, j, ip, jp) - 2.0 * q(i, j) + interp(q, im, jm, i, j)) / denom(i, j);
+      }
+    }
+    return out;
+  }
+  A laplacian(const A& q) const { A rx = second(q, 0); A ry = second(q, 1); return rx + ry; }   // derivs.cpp:458-462
+  std::vector<A> curlZ(const A& z) const { A rx = deriv(z, 1); A ry = -deriv(z, 0); return {rx, ry}; }  // 465-469
+  A curl2D(const A& x, const A& y) const { return deriv(y, 0) - deriv(x, 1); }  // 472-474
+  A div2D(const A& x, const A& y) const { return deriv(x, 0) + deriv(y, 1); }   // 407-414
+
+  // ----- ghost-cell fill, evolution.cpp:126-334 -----
+  void open_fill(Grids& grids, int i1, int i2, int i3, int j1, int j2, int j3) const {  // 158-224
+    (void)grids;
+    Grids& L = eqs->g;  // writes the live grids (evolution.cpp:169-223)
+    const bool xb = (j1 == j2);
+    const A& dd = xb ? pd[d_x] : pd[d_y];
+    double delta_last = dd(i3, j3);
+    double dist23 = 0.5 * (dd(i2, j2) + dd(i3, j3));
+    double dist12 = 0.5 * (dd(i1, j1) + dd(i2, j2));
+    double s2 = std::pow(obdb, dist23 / delta_last);
+    double s1 = std::pow(obdb, dist12 / delta_last);
+    for (int v : eqs->densities()) { L[v](i1, j1) = s1 * L[v](i3, j3); L[v](i2, j2) = s2 * L[v](i3, j3); }
+    for (int v : eqs->thermal_energies()) { L[v](i1, j1) = s1 * L[v](i3, j3); L[v](i2, j2) = s2 * L[v](i3, j3); }
+    double cs = 0.0;
+    const int ns = eqs->num_species();
+    for (int s = 0; s < ns; ++s) {
+      double p3 = L[eqs->thermal_energies()[s]](i3, j3) * (gamma - 1.0);
+      double r3 = L[eqs->densities()[s]](i3, j3);
+      double c = std::sqrt(gamma * p3 / r3);
+      if (c > cs) cs = c;
+    }
+    for (int s = 0; s < ns; ++s) {
+      A& mx = L[eqs->momenta()[s][0]];
+      A& my = L[eqs->momenta()[s][1]];
+      A& rho = L[eqs->densities()[s]];
+      double vx = mx(i3, j3) / rho(i3, j3);
+      double vy = my(i3, j3) / rho(i3, j3);
+      double boost = obs * cs;
+      if (i2 > i1 || j2 > j1) boost *= -1.0;
+      double dist = dist23;
+      if (xb) {
+        double bv = (i1 > i2) ? std::max(0.0, vx + boost) : std::min(0.0, vx + boost);
+        double gv = (dist * bv - 0.5 * dd(i2, j2) * vx) / (0.5 * dd(i3, j3));
+        mx(i1, j1) = rho(i1, j1) * gv; mx(i2, j2) = rho(i2, j2) * gv;
+        my(i1, j1) = rho(i1, j1) * vy; my(i2, j2) = rho(i2, j2) * vy;
+      } else {
+        double bv = (j1 > j2) ? std::max(0.0, vy + boost) : std::min(0.0, vy + boost);
+        double gv = (dist * bv - 0.5 * dd(i2, j2) * vy) / (0.5 * dd(i3, j3));
+        mx(i1, j1) = rho(i1, j1) * vx; mx(i2, j2) = rho(i2, j2) * vx;
+        my(i1, j1) = rho(i1, j1) * gv; my(i2, j2) = rho(i2, j2) * gv;
+      }
+    }
+  }
+  void reflect_fill(int i1, int i2, int i3, int j1, int j2, int j3) const {  // 231-266
+    Grids& L = eqs->g;
+    for (int v : eqs->thermal_energies()) { L[v](i1, j1) = L[v](i3, j3); L[v](i2, j2) = L[v](i3, j3); }
+    for (int v : eqs->densities()) { L[v](i1, j1) = L[v](i3, j3); L[v](i2, j2) = L[v](i3, j3); }
+    for (auto& mom : eqs->momenta())
+      for (int v : mom) { L[v](i1, j1) = 0.0; L[v](i2, j2) = 0.0; L[v](i3, j3) = 0.0; }
+  }
+  void fixed_fill(int i1, int i2, int i3, int j1, int j2, int j3) const {  // 272-282
+    Grids& L = eqs->g;
+    for (auto& mom : eqs->momenta())
+      for (int v : mom) { L[v](i1, j1) = 0.0; L[v](i2, j2) = 0.0; L[v](i3, j3) = 0.0; }
+  }
+  void ucnp_fill(Grids& grids, int side, int index) const {  // 290-334 (writes `grids`)
+    for (int k = 0; k < N_GHOST; ++k) {
+      int ix, iy, bx, by;
+      switch (side) {
+        case 0: ix = xl; iy = index; bx = k; by = index; break;
+        case 1: ix = xu; iy = index; bx = xu + 1 + k; by = index; break;
+        case 2: ix = index; iy = yl; bx = index; by = k; break;
+        default: ix = index; iy = yu; bx = index; by = yu + 1 + k; break;
+      }
+      for (int v : eqs->densities()) grids[v](bx, by) = grids[v](ix, iy);
+      for (int v : eqs->thermal_energies()) grids[v](bx, by) = grids[v](ix, iy);
+      for (int v : eqs->fields()) grids[v](bx, by) = grids[v](ix, iy);
+      const auto moms = eqs->momenta();
+      for (int s = 0; s < eqs->num_species(); ++s)
+        for (int v : moms[s]) grids[v](bx, by) = grids[v](ix, iy);
+    }
+  }
+  void ghost(Grids& grids) const {  // updateGhostZones, evolution.cpp:126-152
+    auto side = [&](int b, bool xaxis, bool upper) {
+      int kind = bc[b];
+      if (kind == Periodic || kind == OpenMoC) return;
+      const int n = xaxis ? nx : ny;
+      const int e1 = upper ? n - 1 : 0, e2 = upper ? n - 2 : 1, e3 = upper ? n - 3 : 2;
+      int lo = xaxis ? yl : xl, hi = xaxis ? yu : xu;
+      if (kind == Fixed) { lo = 0; hi = (xaxis ? ny : nx) - 1; }
+      for (int t = lo; t <= hi; ++t) {
+        int i1, i2, i3, j1, j2, j3;
+        if (xaxis) { i1 = e1; i2 = e2; i3 = e3; j1 = j2 = j3 = t; }
+        else { j1 = e1; j2 = e2; j3 = e3; i1 = i2 = i3 = t; }
+        if (kind == Open) open_fill(grids, i1, i2, i3, j1, j2, j3);
+        else if (kind == Reflect) reflect_fill(i1, i2, i3, j1, j2, j3);
+        else if (kind == Fixed) fixed_fill(i1, i2, i3, j1, j2, j3);
+        else if (kind == OpenUCNP) ucnp_fill(grids, b, t);
+      }
+    };
+    side(0, true, false);
+    side(1, true, true);
+    side(2, false, false);
+    side(3, false, true);
+  }
+
+  // ----- time step, evolution.cpp:59-124 -----
+  void advance() {
+    int dti = eqs->idx("dt");
+    double step = eps * range_min(eqs->g[dti], xl_dt, yl_dt, xu_dt, yu_dt);
+    for (auto& m : mods) m->pre(step);
+    for (auto& m : mods) m->iterate(step);
+    if (integ == RK2) rk2(step);
+    else if (integ == RK4) rk4(step);
+    else euler(step);
+    for (auto& m : mods) m->post(step);
+    last_dt = step;
+    time += step;
+    iter++;
+  }
+  double last_dt = 0.0;
+  void euler(double h) { Grids k = eqs->time_derivs(eqs->g); eqs->apply(eqs->g, k, h); }
+  void rk2(double h) {
+    Grids k = eqs->time_derivs(eqs->g);
+    Grids mid = eqs->g;
+    eqs->apply(mid, k, 0.5 * h);
+    k = eqs->time_derivs(mid);
+    eqs->apply(eqs->g, k, h);
+  }
+  void rk4(double h) {
+    Grids k1 = eqs->time_derivs(eqs->g);
+    Grids r = eqs->g;
+    eqs->apply(r, k1, 0.5 * h);
+    Grids k2 = eqs->time_derivs(r);
+    r = eqs->g;
+    eqs->apply(r, k2, 0.5 * h);
+    Grids k3 = eqs->time_derivs(r);
+    r = eqs->g;
+    eqs->apply(r, k3, h);
+    Grids k4 = eqs->time_derivs(r);
+    Grids kf = k1;
+    for (size_t i = 0; i < eqs->evolved().size(); ++i) kf[i] = (k1[i] + k4[i]) / 6.0 + (k2[i] + k3[i]) / 3.0;
+    eqs->apply(eqs->g, kf, h);
+  }
+};
+
+Grids EqSet::time_derivs(const Grids& grids) const {
+  Grids k = rhs(grids);
+  for (auto& m : pd.mods) m->rhs_add(grids, k);
+  return k;
+}
+void EqSet::propagate(Grids& grids) const {
+  minimums(grids);
+  pd.ghost(grids);
+  for (auto& m : pd.mods) m->pre_recompute(grids);
+  derived(grids);
+  dt(grids);
+}
+void EqSet::apply(Grids& grids, const Grids& k, double step) const {
+  auto ev = evolved();
+  for (size_t i = 0; i < ev.size(); ++i) grids[ev[i]] += step * k[i];
+  propagate(grids);
+}
+void EqSet::populate(Grids& grids) const {
+  evolved_from_state(grids);
+  minimums(grids);
+  pd.ghost(grids);
+  derived(grids);
+  dt(grids);
+}
+
+static double to_d(const std::string& s) { return std::stod(s); }
+
+// ---------------------------------------------------------------------------------------------
+// IdealMHD (source/equationsets/idealmhd.{hpp,cpp})
+// ---------------------------------------------------------------------------------------------
+struct IdealMHD : EqSet {
+  enum V { rho, temp, mom_x, mom_y, mom_z, bi_x, bi_y, bi_z, grav_x, grav_y, n, press, thermal_energy,
+           v_x, v_y, v_z, kinetic_energy, b_x, b_y, b_z, b_mag, b_hat_x, b_hat_y, b_hat_z, dt_ };
+  double global_viscosity = 0.0;
+  explicit IdealMHD(Domain& d)
+      : EqSet(d, {"rho", "temp", "mom_x", "mom_y", "mom_z", "bi_x", "bi_y", "bi_z", "grav_x", "grav_y",
+                  "n", "press", "thermal_energy", "v_x", "v_y", "v_z", "kinetic_energy",
+                  "b_x", "b_y", "b_z", "b_mag", "b_hat_x", "b_hat_y", "b_hat_z", "dt"}) {}
+  std::vector<int> state_vars() const override { return {rho, temp, mom_x, mom_y, mom_z, bi_x, bi_y, bi_z, grav_x, grav_y}; }
+  std::vector<int> evolved() const override { return {rho, mom_x, mom_y, mom_z, thermal_energy, bi_x, bi_y, bi_z}; }
+  std::vector<int> densities() const override { return {rho}; }
+  std::vector<int> thermal_energies() const override { return {thermal_energy}; }
+  std::vector<std::vector<int>> momenta() const override { return {{mom_x, mom_y, mom_z}}; }
+  std::vector<int> fields() const override { return {bi_x, bi_y, bi_z}; }
+  void parse(const std::vector<std::string>& k, const std::vector<std::string>& v) override {  // idealmhd.cpp:12-40
+    for (size_t i = 0; i < k.size(); ++i) {
+      if (k[i] == "global_viscosity") global_viscosity = to_d(v[i]);
+      else if (k[i] == "viscosity_opt" || k[i] == "moc_b_limiting" || k[i] == "moc_mom_limiting" ||
+               k[i] == "moc_b_lower_lim" || k[i] == "moc_b_upper_lim" || k[i] == "moc_mom_lower_lim" ||
+               k[i] == "moc_mom_upper_lim") {}
+      else throw fail(k[i] + " is not recognized for this equation set.");
+    }
+  }
+  // idealmhd.cpp:42-105
+  Grids rhs(const Grids& G) const override {
+    const Domain& D = pd;
+    const A &bex = D.pd[be_x], &bey = D.pd[be_y], &bez = D.pd[be_z];
+    const A &vx = G[v_x], &vy = G[v_y];
+    A d_rho = -D.tdiv(G[rho], vx, vy);
+    A curl_db = D.curl2D(G[bi_x], G[bi_y]) / (4.0 * PI);
+    std::vector<A> ext = {-curl_db * bey, curl_db * bex};
+    std::vector<A> in = {-curl_db * G[bi_y], curl_db * G[bi_x]};
+    std::vector<A> cz = D.curlZ(G[bi_z]);
+    // CrossProduct2DZ(a, b_z) = {b_z * a_y, -b_z * a_x} (grid.cpp:463-468)
+    A bzi = G[bi_z] / (4.0 * PI), bze = bez / (4.0 * PI);
+    std::vector<A> in_z = {-(-1.0 * bzi) * cz[1], (-1.0 * bzi) * cz[0]};
+    std::vector<A> ext_z = {-(-1.0 * bze) * cz[1], (-1.0 * bze) * cz[0]};
+    A fz_ext = (cz[0] * bey - cz[1] * bex) / (4.0 * PI);
+    A fz_in = (cz[0] * G[bi_y] - cz[1] * G[bi_x]) / (4.0 * PI);
+    A d_mx = -D.tdiv(G[mom_x], vx, vy) - D.deriv(G[press], 0) + G[rho] * G[grav_x] + ext[0] + in[0] + in_z[0] + ext_z[0];
+    A d_my = -D.tdiv(G[mom_y], vx, vy) - D.deriv(G[press], 1) + G[rho] * G[grav_y] + ext[1] + in[1] + in_z[1] + ext_z[1];
+    A d_mz = -D.tdiv(G[mom_z], vx, vy) + fz_ext + fz_in;
+    A d_te = -D.tdiv(G[thermal_energy], vx, vy) - G[press] * D.div2D(vx, vy);
+    A d_bx = -D.tdiv(G[bi_x], vx, vy) - D.tdiv(bex, vx, vy) + (G[bi_x] + bex) * D.deriv(G[v_x], 0) + (G[bi_y] + bey) * D.deriv(G[v_x], 1);
+    A d_by = -D.tdiv(G[bi_y], vx, vy) - D.tdiv(bey, vx, vy) + (G[bi_x] + bex) * D.deriv(G[v_y], 0) + (G[bi_y] + bey) * D.deriv(G[v_y], 1);
+    A d_bz = -D.tdiv(G[bi_z], vx, vy) - D.tdiv(bez, vx, vy) + (G[bi_x] + bex) * D.deriv(G[v_z], 0) + (G[bi_y] + bey) * D.deriv(G[v_z], 1);
+    // global_visc_coeff (idealmhd.cpp:90) only feeds the characteristic-boundary terms, which are
+    // identically zero without open_moc boundaries (idealmhd.cpp:308-332): rho*0 + v*0 for momenta.
+    A zero(D.nx, D.ny, 0.0);
+    Grids ch(8, zero);
+    ch[1] = G[rho] * zero + G[v_x] * zero;
+    ch[2] = G[rho] * zero + G[v_y] * zero;
+    ch[3] = G[rho] * zero + G[v_z] * zero;
+    Grids k = {d_rho, d_mx, d_my, d_mz, d_te, d_bx, d_by, d_bz};
+    for (int i = 0; i < 8; ++i) { k[i] *= D.mask; k[i] += ch[i]; }
+    return k;
+  }
+  void evolved_from_state(Grids& G) const override {  // idealmhd.cpp:226-232
+    G[n] = vmax(G[rho] / pd.m_ion, pd.nmin);
+    G[press] = 2 * G[n] * K_B * vmax(G[temp], pd.tmin);
+    G[thermal_energy] = G[press] / (pd.gamma - 1.0);
+  }
+  void minimums(Grids& G) const override {  // idealmhd.cpp:234-239
+    G[rho] = pd.m_ion * vmax(G[rho] / pd.m_ion, pd.nmin);
+    G[thermal_energy] = vmax(G[thermal_energy], pd.temin);
+  }
+  void derived(Grids& G) const override {  // idealmhd.cpp:241-277
+    G[n] = vmax(G[rho] / pd.m_ion, pd.nmin);
+    G[rho] = G[n] * pd.m_ion;
+    G[v_x] = G[mom_x] / G[rho];
+    G[v_y] = G[mom_y] / G[rho];
+    G[v_z] = G[mom_z] / G[rho];
+    G[kinetic_energy] = 0.5 * G[rho] * (sq(G[v_x]) + sq(G[v_y]));
+    G[thermal_energy] = vmax(G[thermal_energy], pd.temin);
+    G[press] = (pd.gamma - 1.0) * G[thermal_energy];
+    G[temp] = vmax(G[press] / (2 * K_B * G[n]), pd.tmin);
+    G[b_x] = pd.pd[be_x] + G[bi_x];
+    G[b_y] = pd.pd[be_y] + G[bi_y];
+    G[b_z] = pd.pd[be_z] + G[bi_z];
+    G[b_mag] = vsqrt(sq(G[b_x]) + sq(G[b_y]) + sq(G[b_z]));
+    G[b_hat_x] = G[b_x] / G[b_mag];
+    G[b_hat_y] = G[b_y] / G[b_mag];
+    G[b_hat_z] = G[b_z] / G[b_mag];
+    for (size_t c = 0; c < G[b_mag].size(); ++c)
+      if (G[b_mag].v[c] == 0.0) { G[b_hat_x].v[c] = 0.0; G[b_hat_y].v[c] = 0.0; G[b_hat_z].v[c] = 0.0; }
+  }
+  void dt(Grids& G) const override {  // idealmhd.cpp:279-304
+    A c_s = vsqrt(pd.gamma * G[press] / G[rho]);
+    A c_s_sq = sq(c_s);
+    A v_a = G[b_mag] / vsqrt(4.0 * PI * G[rho]);
+    A v_a_sq = sq(v_a);
+    A one(pd.nx, pd.ny, 1.0);
+    A delta = vsqrt(one - 4.0 * c_s_sq * v_a_sq / sq(c_s_sq + v_a_sq));
+    A v_fast = vsqrt(0.5 * (c_s_sq + v_a_sq) * (one + delta));
+    A v_slow = vsqrt(0.5 * (c_s_sq + v_a_sq) * (one - delta));
+    A vmx = vsqrt(sq(G[v_x]));
+    A vmy = vsqrt(sq(G[v_y]));
+    A cmax = vmax(vmax(vmax(c_s, v_a), v_fast), v_slow);
+    G[dt_] = one / ((vmx + cmax) / pd.pd[d_x] + (vmy + cmax) / pd.pd[d_y]);
+  }
+};
+
+static std::unique_ptr<EqSet> make_eqset(Domain& d, const std::string& name) {
+  if (name == "ideal_mhd") return std::make_unique<IdealMHD>(d);
+  throw fail("equation set <" + name + "> is not available in the restatement");
+}
+
+static std::unique_ptr<Module> make_module(Domain& d, const std::string& name);
+
+}  // namespace orc
+
+// =============================================================================================
+// Module implementations live in spruce_oracle_modules.inc so the core above stays readable.
+// =============================================================================================
+#include "spruce_oracle_modules.inc"
+
+// =============================================================================================
+// C API
+// =============================================================================================
+using namespace orc;
+
+extern "C" {
+
+const char* oracle_last_error(void) { return g_err.c_str(); }
+
+#define GUARD_BEGIN try {
+#define GUARD_END(ret) } catch (const std::exception& e) { g_err = e.what(); return ret; }
+
+void* oracle_create(int xdim, int ydim, const int bc[4], int integrator, const double scal[8],
+                    const char* eqset, int nopt, const char* const* keys, const char* const* vals) {
+  GUARD_BEGIN
+  auto* d = new Domain();
+  d->nx = xdim; d->ny = ydim;
+  for (int k = 0; k < 4; ++k) d->bc[k] = bc[k];
+  d->integ = integrator;
+  d->eps = scal[0]; d->nmin = scal[1]; d->tmin = scal[2]; d->temin = scal[3];
+  d->obs = scal[4]; d->obdb = scal[5]; d->m_ion = scal[6]; d->gamma = scal[7];
+  d->eqs = make_eqset(*d, eqset);
+  std::vector<std::string> k, v;
+  for (int i = 0; i < nopt; ++i) { k.emplace_back(keys[i]); v.emplace_back(vals[i]); }
+  d->eqs->parse(k, v);
+  return d;
+  GUARD_END(nullptr)
+}
+
+int oracle_add_module(void* h, const char* name, int nopt, const char* const* keys, const char* const* vals) {
+  GUARD_BEGIN
+  auto* d = static_cast<Domain*>(h);
+  auto m = make_module(*d, name);
+  std::vector<std::string> k, v;
+  for (int i = 0; i < nopt; ++i) { k.emplace_back(keys[i]); v.emplace_back(vals[i]); }
+  m->parse(k, v);
+  d->mods.push_back(std::move(m));
+  return 0;
+  GUARD_END(-1)
+}
+
+int oracle_set_grid(void* h, const char* name, const double* data) {
+  GUARD_BEGIN
+  auto* d = static_cast<Domain*>(h);
+  A a(d->nx, d->ny);
+  std::memcpy(a.v.data(), data, sizeof(double) * a.size());
+  for (int k = 0; k < 7; ++k)
+    if (std::strcmp(name, PD_NAMES[k]) == 0) { d->pd[k] = a; d->pd_set[k] = true; return 0; }
+  d->eqs->g[d->eqs->idx(name)] = a;
+  return 0;
+  GUARD_END(-1)
+}
+
+int oracle_setup(void* h) {
+  GUARD_BEGIN
+  auto* d = static_cast<Domain*>(h);
+  for (int k = 0; k < 7; ++k) require(d->pd_set[k], std::string("grid ") + PD_NAMES[k] + " not set");
+  for (int v : d->eqs->state_vars())
+    require(d->eqs->g[v].size() > 1, "state variable " + d->eqs->names[v] + " not set");
+  d->iteration_bounds();
+  d->eqs->populate(d->eqs->g);
+  d->eqs->setup_derived();
+  for (auto& m : d->mods) m->setup();
+  return 0;
+  GUARD_END(-1)
+}
+
+int oracle_step(void* h, int nsteps, double* dt_out) {
+  GUARD_BEGIN
+  auto* d = static_cast<Domain*>(h);
+  for (int s = 0; s < nsteps; ++s) {
+    d->advance();
+    if (dt_out) dt_out[s] = d->last_dt;
+  }
+  return 0;
+  GUARD_END(-1)
+}
+
+int oracle_get_grid(void* h, const char* name, double* out) {
+  GUARD_BEGIN
+  auto* d = static_cast<Domain*>(h);
+  const A* a = nullptr;
+  for (int k = 0; k < 7; ++k) if (std::strcmp(name, PD_NAMES[k]) == 0) a = &d->pd[k];
+  if (!a) a = &d->eqs->g[d->eqs->idx(name)];
+  require((int)a->size() == d->nx * d->ny, std::string("grid ") + name + " is not initialised");
+  std::memcpy(out, a->v.data(), sizeof(double) * a->size());
+  return 0;
+  GUARD_END(-1)
+}
+
+int oracle_rhs(void* h, double* out) {
+  GUARD_BEGIN
+  auto* d = static_cast<Domain*>(h);
+  Grids k = d->eqs->time_derivs(d->eqs->g);
+  size_t n = (size_t)d->nx * d->ny;
+  for (size_t i = 0; i < k.size(); ++i) std::memcpy(out + i * n, k[i].v.data(), sizeof(double) * n);
+  return 0;
+  GUARD_END(-1)
+}
+
+int oracle_num_evolved(void* h) { return (int)static_cast<Domain*>(h)->eqs->evolved().size(); }
+double oracle_time(void* h) { return static_cast<Domain*>(h)->time; }
+
+int oracle_subcycles(void* h, int* out, int nmax) {
+  auto* d = static_cast<Domain*>(h);
+  int n = 0;
+  for (auto& m : d->mods) { if (n < nmax) out[n] = m->subcycles(); ++n; }
+  return n;
+}
+
+void oracle_destroy(void* h) { delete static_cast<Domain*>(h); }
+
+}  // extern "C"

@@ -1,0 +1,102 @@
+"""Parity cases: small ``.state`` + ``.config`` pairs that exercise every path of the time step.
+
+Each case is run by the reference executable (``make_golden.py``) to produce the golden vectors, and
+by the CPU restatement and the GPU path in the tests.  Grids are small and non-square so that a
+transposed index fails loudly; step counts are small so the reference finishes in well under a
+second per case.
+"""
+from __future__ import annotations
+
+import os
+import sys
+from typing import Callable, Dict, List, Sequence, Tuple
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from spruce_amd import workloads as W  # noqa: E402
+from spruce_amd.stateio import ConfigFile, StateFile  # noqa: E402
+
+MHD_OUT = ["rho", "temp", "mom_x", "mom_y", "mom_z", "bi_x", "bi_y", "bi_z", "n", "thermal_energy", "dt"]
+
+
+def random_mhd_state(nx: int, ny: int, seed: int, stretch: bool = False, with_be: bool = True,
+                     with_grav: bool = True, zero_mom: bool = False) -> StateFile:
+    """Perturbed uniform plasma: every evolved field carries seeded noise so no term vanishes."""
+    u = [W.uniform(seed * 16 + k, (nx, ny)) for k in range(10)]
+    if stretch:
+        dx = 1e7 * (1.01 ** np.arange(nx))
+        dy = 8e6 * (1.02 ** np.arange(ny))
+    else:
+        dx = np.full(nx, 1e7)
+        dy = np.full(ny, 8e6)
+    rho = 1e-15 * (1.0 + 0.3 * (2 * u[0] - 1))
+    temp = 1e6 * (1.0 + 0.3 * (2 * u[1] - 1))
+    v0 = 0.0 if zero_mom else 1e7
+    mom = tuple(rho * v0 * (2 * u[k] - 1) for k in (2, 3, 4))
+    bi = tuple(1.0 * (2 * u[k] - 1) for k in (5, 6, 7))
+    z = np.zeros((nx, ny))
+    be = (2.0 + 0.5 * (2 * u[8] - 1), 1.0 + 0.5 * (2 * u[9] - 1), 0.3 * np.ones((nx, ny))) if with_be else None
+    grav = (z, np.full((nx, ny), -2.74e4)) if with_grav else None
+    return W.mhd_state(nx, ny, dx, dy, rho, temp, mom, bi, be=be, grav=grav,
+                       comments=["#random ideal-MHD case, seed %d" % seed])
+
+
+def _mhd(bc, integ, steps=6, **kw) -> ConfigFile:
+    return W.base_config("ideal_mhd", bc, integ, epsilon=0.2, max_iterations=steps,
+                         density_min=1.0, temp_min=1.0, thermal_energy_min=1e-12,
+                         output_flags=MHD_OUT, **kw)
+
+
+P4 = ("periodic",) * 4
+
+
+def _cases() -> Dict[str, Tuple[Callable[[], StateFile], Callable[[], ConfigFile]]]:
+    c: Dict[str, Tuple[Callable[[], StateFile], Callable[[], ConfigFile]]] = {}
+    wave = lambda: W.periodic_wave_state(24, 20, seed=0)  # noqa: E731
+    c["mhd_periodic_rk2"] = (wave, lambda: _mhd(P4, "rk2", 8))
+    c["mhd_periodic_euler"] = (wave, lambda: _mhd(P4, "euler", 8))
+    c["mhd_periodic_rk4"] = (wave, lambda: _mhd(P4, "rk4", 8))
+    c["mhd_random_periodic_rk2"] = (lambda: random_mhd_state(20, 26, 1, with_grav=True),
+                                    lambda: _mhd(P4, "rk2", 6))
+    c["mhd_open_rk2"] = (lambda: random_mhd_state(22, 18, 2),
+                         lambda: _mhd(("open",) * 4, "rk2", 6, open_boundary_strength=0.5,
+                                      open_boundary_decay_base=0.7))
+    c["mhd_open_rk4"] = (lambda: random_mhd_state(18, 22, 3, stretch=True),
+                         lambda: _mhd(("open",) * 4, "rk4", 5, open_boundary_strength=0.3,
+                                      open_boundary_decay_base=0.5))
+    c["mhd_open_euler"] = (lambda: random_mhd_state(18, 22, 4),
+                           lambda: _mhd(("open",) * 4, "euler", 6, open_boundary_strength=1.0))
+    c["mhd_fixed_rk2"] = (lambda: random_mhd_state(22, 18, 5),
+                          lambda: _mhd(("fixed",) * 4, "rk2", 6))
+    c["mhd_reflect_rk2"] = (lambda: random_mhd_state(22, 18, 6),
+                            lambda: _mhd(("reflect",) * 4, "rk2", 6))
+    c["mhd_reflect_rk4"] = (lambda: random_mhd_state(20, 16, 7, stretch=True),
+                            lambda: _mhd(("reflect",) * 4, "rk4", 5))
+    c["mhd_ucnp_rk2"] = (lambda: random_mhd_state(22, 18, 8),
+                         lambda: _mhd(("open_ucnp",) * 4, "rk2", 6))
+    c["mhd_mixed_rk4"] = (lambda: random_mhd_state(16, 24, 9, stretch=True),
+                          lambda: _mhd(("periodic", "periodic", "fixed", "open"), "rk4", 5,
+                                       open_boundary_strength=0.2, open_boundary_decay_base=0.8))
+    c["mhd_mixed_euler"] = (lambda: random_mhd_state(24, 16, 10),
+                            lambda: _mhd(("open", "reflect", "periodic", "periodic"), "euler", 6,
+                                         open_boundary_strength=0.4))
+    c["mhd_floors_rk2"] = (lambda: random_mhd_state(20, 20, 11),
+                           lambda: W.base_config("ideal_mhd", ("fixed", "open", "reflect", "open"), "rk2",
+                                                 epsilon=0.3, max_iterations=6, density_min=5.5e8,
+                                                 temp_min=9.5e5, thermal_energy_min=0.21,
+                                                 output_flags=MHD_OUT))
+    c["mhd_static_stretch_rk2"] = (lambda: random_mhd_state(18, 20, 12, stretch=True, zero_mom=True),
+                                   lambda: _mhd(("fixed", "fixed", "open", "open"), "rk2", 6))
+    return c
+
+
+CASES = _cases()
+
+
+def build_case(name: str) -> Tuple[StateFile, ConfigFile]:
+    st, cf = CASES[name]
+    return st(), cf()

@@ -1,0 +1,93 @@
+"""Write the golden vectors: run the compiled reference on every parity case.
+
+Usage (in a container that has /root/reference):
+    make -C oracle ref                 # builds oracle/_ref/run from the reference sources
+    python tests/golden/make_golden.py [case ...]
+
+For each case the reference executable runs in input mode with ``write_precision = 17`` (which
+round-trips every double), and the frames after setup, after step 1 and after the last step are
+kept for the variables listed in the case's ``output_flags``.  The step sizes of every step are
+reconstructed exactly from the ``dt`` grid of the preceding frame,
+``dt_step = epsilon * min(dt over the interior)`` (evolution.cpp:62; grid.cpp:71-82).
+
+A successful reference run ends in SIGABRT after its success message (evolution.cpp:54-56); that
+exit is accepted.  Outputs: ``tests/golden/<case>.npz`` holding the input texts and the vectors.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+from cases import CASES, ROOT, build_case  # noqa: E402
+
+from spruce_amd.stateio import parse_config, read_out  # noqa: E402
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "run")
+
+
+def interior_bounds(cfg, xdim, ydim):
+    bc = [cfg.settings[k] for k in ("x_bound_1", "x_bound_2", "y_bound_1", "y_bound_2")]
+    xl = 0 if bc[0] == "periodic" else 2
+    xu = xdim - 1 if bc[1] == "periodic" else xdim - 3
+    yl = 0 if bc[2] == "periodic" else 2
+    yu = ydim - 1 if bc[3] == "periodic" else ydim - 3
+    return xl, xu, yl, yu
+
+
+def run_case(name: str) -> str:
+    st, cfg = build_case(name)
+    state_text = st.to_text(17)
+    config_text = cfg.to_text()
+    with tempfile.TemporaryDirectory() as tmp:
+        sp = os.path.join(tmp, "case.state")
+        cp = os.path.join(tmp, "case.config")
+        with open(sp, "w") as f:
+            f.write(state_text)
+        with open(cp, "w") as f:
+            f.write(config_text)
+        out_dir = os.path.join(tmp, "out")
+        r = subprocess.run([REF_BIN, "-m", "input", "-o", out_dir, "-s", sp, "-c", cp],
+                           capture_output=True, text=True, timeout=600, cwd=tmp)
+        ok = r.returncode in (-6, 134) and "successfully reached" in r.stderr
+        if not ok:
+            raise RuntimeError("reference failed on %s (rc=%d):\n%s\n%s" % (name, r.returncode, r.stdout[-2000:],
+                                                                            r.stderr[-2000:]))
+        out = read_out(os.path.join(out_dir, "mhd.out"))
+    parsed = parse_config(config_text)
+    eps = float(parsed.settings["epsilon"])
+    xl, xu, yl, yu = interior_bounds(parsed, st.xdim, st.ydim)
+    nframes = len(out.frames)
+    names = list(parsed.output_flags)
+    dts = []
+    for k in range(nframes - 1):
+        g = out.frames[k]["dt"][xl:xu + 1, yl:yu + 1]
+        g = g[~np.isnan(g)]
+        dts.append(eps * float(g.min()))
+    keep = sorted({0, 1, nframes - 1})
+    data = np.stack([np.stack([out.frames[k][v] for v in names]) for k in keep])
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path,
+                        state=np.frombuffer(state_text.encode(), dtype=np.uint8),
+                        config=np.frombuffer(config_text.encode(), dtype=np.uint8),
+                        names=np.array(names), frames=np.array(keep), data=data,
+                        dt_steps=np.array(dts), bounds=np.array([xl, xu, yl, yu]))
+    return path
+
+
+def main(argv):
+    if not os.path.exists(REF_BIN):
+        sys.exit("build the reference first: make -C oracle ref")
+    names = argv[1:] or sorted(CASES)
+    for n in names:
+        p = run_case(n)
+        print("%-28s -> %s (%d bytes)" % (n, os.path.relpath(p, ROOT), os.path.getsize(p)))
+
+
+if __name__ == "__main__":
+    main(sys.argv)
