@@ -1,0 +1,170 @@
+"""Time-step throughput of the MI355X path: cell-updates/s and the HBM roofline of the stage kernel.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload arcade|periodic|arcade2048]
+
+One "step" is one PlasmaDomain::advanceTime (evolution.cpp:59-82) over the whole grid, ghost cells
+included: every integrator stage, every module subcycle, the ghost fill and the dt reduction.
+Inputs are synthetic (spruce_amd.workloads) and resident in HBM before timing starts; the timed
+region is K steps bracketed by device synchronisation.  Rank 0 prints one JSON line.
+
+The CPU baseline is the reference executable compiled from its own sources (oracle/_ref/run),
+timed on this host over a bounded number of steps of the same workload, with a zero-step run
+subtracted to remove setup and file output.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+# Algorithmic bytes per cell and per stage launch (FP64 = 8 B): the stage reads the evolved fields
+# E of its source set (with halo), the E base values (later stages), the S static fields, and
+# writes E.  IdealMHD: E = 8 (rho, mom_x/y/z, thermal_energy, bi_x/y/z), S = 5 (be_x/y/z, grav_x/y).
+E_MHD, S_MHD = 8, 5
+
+
+def stage_bytes_per_cell(integrator: str) -> float:
+    """Average algorithmic bytes per cell of one stage launch."""
+    if integrator == "rk2":     # (E+S read, E write) + (2E+S read, E write) over 2 launches
+        return 8.0 * (5 * E_MHD + 2 * S_MHD) / 2
+    if integrator == "euler":
+        return 8.0 * (2 * E_MHD + S_MHD)
+    raise ValueError(integrator)
+
+
+def make_workload(name: str):
+    from spruce_amd import workloads as W
+    from spruce_amd.spec import spec_from_files
+    if name == "periodic":
+        st = W.periodic_wave_state(256, 256, seed=0)
+        cfg = W.periodic_wave_config(steps=1000)
+        desc = {"workload": "periodic ideal-MHD 256x256 (RK2, FP64)", "grid": [256, 256],
+                "equation_set": "ideal_mhd", "integrator": "rk2", "modules": []}
+    elif name == "periodic2048":
+        st = W.periodic_wave_state(2048, 2048, d=1.25e6, seed=0)
+        cfg = W.periodic_wave_config(steps=1000)
+        desc = {"workload": "periodic ideal-MHD 2048x2048 (RK2, FP64)", "grid": [2048, 2048],
+                "equation_set": "ideal_mhd", "integrator": "rk2", "modules": []}
+    elif name in ("arcade", "arcade2048"):
+        nx, ny = (512, 1024) if name == "arcade" else (2048, 2048)
+        st = W.arcade_state(nx, ny)
+        cfg = W.arcade_config(steps=2000)
+        desc = {"workload": "solar arcade %dx%d ideal-MHD + thermal conduction + radiative losses (RK2, FP64)"
+                % (nx, ny), "grid": [nx, ny], "equation_set": "ideal_mhd", "integrator": "rk2",
+                "modules": ["thermal_conduction", "radiative_losses"]}
+    else:
+        raise SystemExit("unknown workload " + name)
+    return st, cfg, spec_from_files(cfg, st), desc
+
+
+def cpu_baseline(st, cfg, steps: int, threads: int) -> dict:
+    """Time the reference executable: (t(steps) - t(0)) over the same input, OMP_NUM_THREADS=threads."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "run")
+    if not os.path.exists(exe):
+        return None
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    times = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        sp = os.path.join(tmp, "w.state")
+        st.write(sp)
+        for n in (0, steps):
+            c = cfg
+            c.settings["max_iterations"] = str(n)
+            c.settings["iter_output_interval"] = "-1"
+            c.settings["write_interval"] = "-1"
+            c.settings["std_out_interval"] = "-1"
+            cp = os.path.join(tmp, "w%d.config" % n)
+            c.write(cp)
+            t0 = time.perf_counter()
+            r = subprocess.run([exe, "-m", "input", "-o", os.path.join(tmp, "o%d" % n), "-s", sp, "-c", cp],
+                               capture_output=True, text=True, env=env, cwd=tmp)
+            times[n] = time.perf_counter() - t0
+            if r.returncode not in (0, -6, 134):
+                return {"error": "reference run failed rc=%d" % r.returncode}
+    dt = times[steps] - times[0]
+    return {"value": st.xdim * st.ydim * steps / dt, "unit": "cell-updates/s", "cores": threads,
+            "kind": "reference",
+            "sample": "%d steps of the same workload, reference executable (g++ -O3 -fopenmp), "
+                      "t(%d steps) - t(0 steps) = %.2f s" % (steps, steps, dt)}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="periodic2048")
+    ap.add_argument("--cpu-steps", type=int, default=0, help="reference steps for the CPU baseline (0: auto)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from spruce_amd.gpu import GpuDomain
+
+    st, cfg, spec, desc = make_workload(args.workload)
+    cells = spec.xdim * spec.ydim
+    dom = GpuDomain(spec)
+    dom.step(args.warmup)
+    t0 = time.perf_counter()
+    dom.step(args.steps)   # returns after a device synchronisation
+    wall = time.perf_counter() - t0
+    value = cells * args.steps / wall
+
+    # Per-kernel durations from HIP events recorded on the domain's own stream.
+    dom.set_profiling(True)
+    prof_steps = max(5, min(args.steps, 50))
+    dom.step(prof_steps)
+    kt = dom.kernel_times()
+    dom.set_profiling(False)
+    integ = desc["integrator"]
+    stage_ms = kt.get("mhd_stage", (float("nan"), 0))[0]
+    stage_bytes = stage_bytes_per_cell(integ) * cells
+    achieved = stage_bytes / (stage_ms * 1e-3) / 1e9
+    step_bytes = 8.0 * (5 * E_MHD + 2 * S_MHD) * cells
+
+    out = {
+        "metric": "cell-updates/s and % HBM roofline (FP64); adjoint/forward cost ratio",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": desc,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "mhd_stage", "kernel_ms": stage_ms,
+                     "algorithmic_bytes_per_cell_update": 8.0 * (5 * E_MHD + 2 * S_MHD),
+                     "step_frac": step_bytes * value / cells / 1e9 / HBM_PEAK_GBS},
+        "kernels_ms": {k: v[0] for k, v in kt.items()},
+    }
+    if not args.no_cpu:
+        threads = int(os.environ.get("SPRUCE_CPU_THREADS", str(min(16, os.cpu_count() or 1))))
+        n = args.cpu_steps or max(2, int(2e7 / cells))
+        out["cpu_baseline"] = cpu_baseline(st, cfg, n, threads)
+        if out["cpu_baseline"] and "value" in out["cpu_baseline"]:
+            out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,125 @@
+/* spruce_gpu.h -- C ABI of the MI355X (gfx950) SPRUCE time-step library (libspruce_gpu.so).
+ *
+ * The reference exposes its time step as a C++ plugin API: PlasmaDomain owns the grids and calls
+ * EquationSet and Module hooks in a fixed order (source/mhd/plasmadomain.hpp:18-257,
+ * source/equationsets/equationset.hpp:18-171, source/modules/module.hpp:15-58).  This header is the
+ * drop-in boundary a host binding (ctypes, cgo, JNI, ...) would call instead: plain pointers,
+ * sizes, strings and status codes; no C++ or torch types.
+ *
+ * Conventions
+ *   - Every function returns SPRUCE_OK (0) or a negative status; spruce_gpu_last_error() returns a
+ *     message for the calling thread's most recent failure.  Conditions the reference asserts on
+ *     (unknown names, bad shapes, missing grids) become status codes; nothing aborts.
+ *   - Host arrays are borrowed for the duration of the call.  Device memory is owned by the handle.
+ *   - One handle is used by one host thread at a time.  Grids are xdim*ydim doubles, row-major with
+ *     x as the row index and y contiguous (Grid, source/mhd/grid.cpp:516-526).
+ *   - All arithmetic is IEEE FP64 in the reference's operation order.
+ */
+#ifndef SPRUCE_GPU_H
+#define SPRUCE_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum spruce_status {
+  SPRUCE_OK = 0,
+  SPRUCE_E_ARG = -1,      /* invalid argument, unknown name, inconsistent configuration */
+  SPRUCE_E_STATE = -2,    /* call made in the wrong phase (e.g. step before setup) */
+  SPRUCE_E_HIP = -3,      /* HIP runtime error */
+  SPRUCE_E_NUMERIC = -4,  /* a reduction found no finite value (reference: grid.cpp:80 assert) */
+  SPRUCE_E_UNSUPPORTED = -5
+};
+
+/* Boundary kinds in the reference's order (plasmadomain.hpp:22). */
+enum spruce_bc { SPRUCE_BC_PERIODIC = 0, SPRUCE_BC_OPEN = 1, SPRUCE_BC_FIXED = 2, SPRUCE_BC_REFLECT = 3,
+                 SPRUCE_BC_OPEN_MOC = 4, SPRUCE_BC_OPEN_UCNP = 5 };
+/* Time integrators (plasmadomain.hpp:29); the reference default is Euler (plasmadomain.hpp:95). */
+enum spruce_integrator { SPRUCE_EULER = 0, SPRUCE_RK2 = 1, SPRUCE_RK4 = 2 };
+
+/* PlasmaDomain settings (plasmadomain.hpp:93-153) plus the two .state header scalars. */
+typedef struct spruce_domain_desc {
+  int32_t xdim, ydim;
+  int32_t bc[4];                  /* x_bound_1, x_bound_2, y_bound_1, y_bound_2 */
+  int32_t time_integrator;
+  double epsilon;                 /* CFL safety factor (evolution.cpp:62) */
+  double density_min;             /* number-density floor, cm^-3 */
+  double temp_min;
+  double thermal_energy_min;
+  double open_boundary_strength;
+  double open_boundary_decay_base;
+  double ion_mass;                /* g */
+  double adiabatic_index;
+  double time;                    /* initial simulation time */
+} spruce_domain_desc;
+
+typedef struct spruce_gpu_domain spruce_gpu_domain;
+
+/* Replaces PlasmaDomain construction + EquationSet::instantiateWithConfig (fileio.cpp:84-122,
+ * equationset.cpp:24-85): eqset is one of "ideal_mhd", "ideal_mhd_cons", "ideal_mhd_2E",
+ * "ideal_2F"; keys/vals are the lines of its .config block. Returns NULL on failure. */
+spruce_gpu_domain* spruce_gpu_create(const spruce_domain_desc* desc, const char* eqset, int32_t nopts,
+                                     const char* const* keys, const char* const* vals);
+
+/* Replaces ModuleHandler::instantiateModule + Module::configureModule (modulehandler.cpp:77-112,
+ * module.cpp:11-35).  Modules run in the order they are added (modulehandler.cpp:27-65). */
+int spruce_gpu_add_module(spruce_gpu_domain* d, const char* name, int32_t nopts, const char* const* keys,
+                          const char* const* vals);
+
+/* Replaces the grid assignment in readStateFile (fileio.cpp:53-78): a PlasmaDomain grid (d_x, d_y,
+ * pos_x, pos_y, be_x, be_y, be_z) or a state variable of the equation set.  count = xdim*ydim. */
+int spruce_gpu_set_grid(spruce_gpu_domain* d, const char* name, const double* host, int64_t count);
+
+/* Replaces the tail of the PlasmaDomain constructor (plasmadomain.cpp:48-53): validation,
+ * computeIterationBounds, EquationSet::setupEquationSet (populateVariablesFromState) and
+ * ModuleHandler::setupModules. */
+int spruce_gpu_setup(spruce_gpu_domain* d);
+
+/* Replaces nsteps calls of PlasmaDomain::advanceTime (evolution.cpp:59-82).  dt_out (may be NULL)
+ * receives the nsteps step sizes; subcycles_out (may be NULL) receives nsteps*nmodules subcycle
+ * counts, -1 for modules that do not subcycle. */
+int spruce_gpu_step(spruce_gpu_domain* d, int32_t nsteps, double* dt_out, int32_t* subcycles_out);
+
+/* Copies any equation-set variable (state, evolved or derived, including "dt") or PlasmaDomain grid
+ * of the live state to host memory.  count = xdim*ydim. */
+int spruce_gpu_get_grid(spruce_gpu_domain* d, const char* name, double* host, int64_t count);
+
+/* Simulation time and iteration counter (plasmadomain.hpp:96-98). */
+int spruce_gpu_time(spruce_gpu_domain* d, double* t, int32_t* iter);
+
+/* Unit entry points for parity tests.
+ * rhs: EquationSet::computeTimeDerivatives on the live grids (equationset.cpp:204-210), written as
+ *      nevolved consecutive xdim*ydim blocks in evolved-variable order.
+ * dt_min: epsilon * min of the dt grid over the dt bounds, the step size advanceTime would take
+ *      next (evolution.cpp:62; grid.cpp:71-82).
+ * num_evolved: number of evolved variables of the equation set. */
+int spruce_gpu_rhs(spruce_gpu_domain* d, double* host_out, int64_t count);
+int spruce_gpu_dt_min(spruce_gpu_domain* d, double* step_out);
+int spruce_gpu_num_evolved(spruce_gpu_domain* d);
+
+/* Device-resident access for zero-copy callers: the HIP stream the handle launches on and the
+ * device address of a live grid (valid until the next step call). */
+int spruce_gpu_stream(spruce_gpu_domain* d, void** hip_stream);
+int spruce_gpu_device_grid(spruce_gpu_domain* d, const char* name, double** dev_ptr);
+
+/* Average duration (ms) of each kernel of the most recent spruce_gpu_step call, measured with HIP
+ * events on the handle's stream when profiling is enabled (spruce_gpu_set_profiling).  names is a
+ * comma-separated list written to a caller buffer. */
+int spruce_gpu_set_profiling(spruce_gpu_domain* d, int32_t enable);
+int spruce_gpu_kernel_times(spruce_gpu_domain* d, char* names, int32_t names_len, double* ms_avg,
+                            int32_t* launches, int32_t max_kernels);
+
+/* Device arithmetic self-check (no reference counterpart): evaluates a/b, sqrt(a) and the
+ * reciprocal-corrected quotient used by the stencils on n inputs, for comparison with the host. */
+int spruce_gpu_arith_probe(const double* a, const double* b, int64_t n, double* div_out, double* sqrt_out,
+                           double* qdiv_out);
+
+const char* spruce_gpu_last_error(void);
+void spruce_gpu_destroy(spruce_gpu_domain* d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPRUCE_GPU_H */

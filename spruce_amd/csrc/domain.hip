@@ -1,0 +1,684 @@
+// Host driver of the time step and the C ABI declared in include/spruce_gpu.h.
+//
+// The Domain below plays PlasmaDomain's role for one GPU: it owns the device grids, keeps the
+// reference's hook order (dt -> module pre -> module iterate -> integrator -> module post,
+// evolution.cpp:59-82) and launches the kernels on one HIP stream.  The step size never leaves the
+// device between steps: the dt reduction writes it into device memory and every kernel of the next
+// step reads it from there, so a run of steps needs no host synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/spruce_gpu.h"
+#include "arith.hpp"
+#include "mhd.hpp"
+
+namespace spruce {
+
+static thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+static void require(bool c, int code, const std::string& msg) { if (!c) throw Error(code, msg); }
+static void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error(SPRUCE_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Device buffer owned by the domain.
+struct DBuf {
+  double* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  void swap(DBuf& o) { std::swap(p, o.p); std::swap(n, o.n); }
+  void alloc(size_t count) {
+    release();
+    hip_check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(double)), "hipMalloc");
+    n = count;
+    hip_check(hipMemset(p, 0, std::max<size_t>(count, 1) * sizeof(double)), "hipMemset");
+    // the domain's stream does not synchronise with the null stream: finish the clear now
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  }
+  void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+  ~DBuf() { release(); }
+};
+
+static const char* PD_NAMES[7] = {"d_x", "d_y", "pos_x", "pos_y", "be_x", "be_y", "be_z"};
+
+// IdealMHD variable names (idealmhd.hpp:19-26).
+static const char* MHD_NAMES[25] = {"rho", "temp", "mom_x", "mom_y", "mom_z", "bi_x", "bi_y", "bi_z",
+                                    "grav_x", "grav_y", "n", "press", "thermal_energy", "v_x", "v_y",
+                                    "v_z", "kinetic_energy", "b_x", "b_y", "b_z", "b_mag", "b_hat_x",
+                                    "b_hat_y", "b_hat_z", "dt"};
+static const char* MHD_STATE[10] = {"rho", "temp", "mom_x", "mom_y", "mom_z", "bi_x", "bi_y", "bi_z", "grav_x", "grav_y"};
+// Derived variables computed by k_mhd_derived, in its `which` order.
+static const char* MHD_DERIVED[14] = {"temp", "press", "v_x", "v_y", "v_z", "kinetic_energy", "b_x", "b_y",
+                                      "b_z", "b_mag", "b_hat_x", "b_hat_y", "b_hat_z", "dt"};
+
+struct SetBufs {
+  DBuf rho, n, mx, my, mz, te, bx, by, bz, rho_pre;
+  void alloc(size_t N) { for (DBuf* b : all()) b->alloc(N); }
+  std::vector<DBuf*> all() { return {&rho, &n, &mx, &my, &mz, &te, &bx, &by, &bz, &rho_pre}; }
+  MhdSet view() { return MhdSet{rho.p, n.p, mx.p, my.p, mz.p, te.p, bx.p, by.p, bz.p, rho_pre.p}; }
+  void swap(SetBufs& o) {
+    auto a = all(), b = o.all();
+    for (size_t k = 0; k < a.size(); ++k) a[k]->swap(*b[k]);
+  }
+};
+
+struct AxisBufs {
+  DBuf d, rd, h, hsq, rhsq, fden, rfden, gup, gdn;
+  std::vector<double> hd;  // host copy of the cell sizes
+  Axis view(int n, int lo, int hi, int per) const {
+    return Axis{d.p, rd.p, h.p, hsq.p, rhsq.p, fden.p, rfden.p, gup.p, gdn.p, n, lo, hi, per};
+  }
+  // Precomputes the geometry the stencils divide by; every reciprocal is a correctly rounded
+  // IEEE division on the host.
+  void build(const std::vector<double>& dcell, bool per) {
+    const int n = (int)dcell.size();
+    hd = dcell;
+    std::vector<double> vd(n), vrd(n), vh(n), vhsq(n), vrhsq(n), vf(n, 0.0), vrf(n, 0.0), vgu(n, 0.0), vgd(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+      vd[i] = dcell[i];
+      vrd[i] = 1.0 / dcell[i];
+      vh[i] = 0.5 * dcell[i];
+      vhsq[i] = vh[i] * vh[i];
+      vrhsq[i] = 1.0 / vhsq[i];
+    }
+    auto w = [&](int k) { return per ? ((k % n) + n) % n : k; };
+    for (int c = 0; c < n; ++c) {
+      const int cm1 = w(c - 1), cm2 = w(c - 2), cp1 = w(c + 1);
+      if (cm1 >= 0 && cm1 < n) { vf[c] = vh[c] + vh[cm1]; vrf[c] = 1.0 / vf[c]; }
+      if (cm2 >= 0 && cm2 < n && cm1 >= 0) vgu[c] = vh[cm2] + 2.0 * vh[cm1];
+      if (cp1 >= 0 && cp1 < n) vgd[c] = vh[cp1] + 2.0 * vh[c];
+    }
+    auto up = [&](DBuf& b, const std::vector<double>& v) {
+      b.alloc(n);
+      hip_check(hipMemcpy(b.p, v.data(), n * sizeof(double), hipMemcpyHostToDevice), "upload geometry");
+    };
+    up(d, vd); up(rd, vrd); up(h, vh); up(hsq, vhsq); up(rhsq, vrhsq); up(fden, vf); up(rfden, vrf);
+    up(gup, vgu); up(gdn, vgd);
+  }
+};
+
+struct KernelClock {
+  bool enabled = false;
+  struct Pending { std::string name; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::map<std::string, std::pair<double, int>> acc;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t get() {
+    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+  }
+  void collect() {
+    for (auto& p : pending) {
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+      auto& s = acc[p.name];
+      s.first += ms;
+      s.second += 1;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  ~KernelClock() {
+    for (auto& p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+class Domain {
+ public:
+  spruce_domain_desc desc{};
+  std::string eqset;
+  std::vector<std::pair<std::string, std::vector<std::pair<std::string, std::string>>>> modules;
+  std::map<std::string, std::vector<double>> host_grids;
+  bool ready = false;
+  hipStream_t stream = nullptr;
+  int nx = 0, ny = 0;
+  size_t N = 0;
+  int xl = 0, xu = 0, yl = 0, yu = 0;
+  double time = 0.0;
+  int iter = 0;
+
+  AxisBufs ax, ay;
+  Mesh mesh{};
+  MhdConst kc{};
+  SetBufs L, W, W2;
+  DBuf bex, bey, bez, gx, gy;
+  DBuf kbuf[3];
+  DBuf partial, step, history, scratch;
+  int* err_flag = nullptr;
+  int dt_blocks = 0;
+  int kinds_all[4] = {0, 0, 0, 0}, kinds_ucnp[4] = {0, 0, 0, 0}, kinds_live[4] = {0, 0, 0, 0};
+  double s1[4] = {1, 1, 1, 1}, s2[4] = {1, 1, 1, 1};
+  bool any_open = false;
+  KernelClock clock;
+
+  explicit Domain(const spruce_domain_desc& d, const std::string& eq) : desc(d), eqset(eq) {
+    require(d.xdim > 0 && d.ydim > 0, SPRUCE_E_ARG, "xdim and ydim must be positive");
+    for (int k = 0; k < 4; ++k) require(d.bc[k] >= 0 && d.bc[k] <= 5, SPRUCE_E_ARG, "unknown boundary kind");
+    require(d.time_integrator >= 0 && d.time_integrator <= 2, SPRUCE_E_ARG, "unknown time integrator");
+    require(eq == "ideal_mhd" || eq == "ideal_mhd_cons" || eq == "ideal_mhd_2E" || eq == "ideal_2F", SPRUCE_E_ARG,
+            "Equation set name <" + eq + "> not recognized.");
+    require(eq == "ideal_mhd", SPRUCE_E_UNSUPPORTED, "equation set " + eq + " is not available in this build");
+    nx = d.xdim;
+    ny = d.ydim;
+    N = (size_t)nx * ny;
+    time = d.time;
+    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+  }
+  ~Domain() {
+    if (err_flag) (void)hipFree(err_flag);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void parse_eqset(const std::vector<std::string>& k, const std::vector<std::string>& v) {
+    // IdealMHD::parseEquationSetConfigs (idealmhd.cpp:12-40): keys other than the MoC/viscosity
+    // options are an error; global_viscosity only acts on open_moc boundaries.
+    for (size_t i = 0; i < k.size(); ++i) {
+      static const char* ok[] = {"global_viscosity", "viscosity_opt", "moc_b_limiting", "moc_mom_limiting",
+                                 "moc_b_lower_lim", "moc_b_upper_lim", "moc_mom_lower_lim", "moc_mom_upper_lim"};
+      bool found = false;
+      for (auto* s : ok) found = found || k[i] == s;
+      require(found, SPRUCE_E_ARG, k[i] + " is not recognized for this equation set.");
+      (void)v;
+    }
+  }
+
+  void add_module(const std::string& name, const std::vector<std::pair<std::string, std::string>>& opts) {
+    require(!ready, SPRUCE_E_STATE, "modules must be added before setup");
+    throw Error(SPRUCE_E_UNSUPPORTED, "module " + name + " is not available in this build");
+    modules.emplace_back(name, opts);
+  }
+
+  bool is_pd(const std::string& n) const { for (auto* s : PD_NAMES) if (n == s) return true; return false; }
+  bool is_state(const std::string& n) const { for (auto* s : MHD_STATE) if (n == s) return true; return false; }
+
+  void set_grid(const std::string& name, const double* p, int64_t count) {
+    require(count == (int64_t)N, SPRUCE_E_ARG, "grid " + name + ": count must equal xdim*ydim");
+    bool known = is_pd(name);
+    for (auto* s : MHD_NAMES) known = known || name == s;
+    require(known, SPRUCE_E_ARG, "Variable name <" + name + "> not recognized");
+    host_grids[name].assign(p, p + N);
+  }
+
+  void upload(DBuf& b, const std::vector<double>& v) {
+    b.alloc(N);
+    hip_check(hipMemcpy(b.p, v.data(), N * sizeof(double), hipMemcpyHostToDevice), "upload grid");
+  }
+
+  void setup() {
+    require(!ready, SPRUCE_E_STATE, "setup called twice");
+    for (auto* s : PD_NAMES) require(host_grids.count(s), SPRUCE_E_ARG, std::string("PlasmaDomain grid ") + s + " not set");
+    for (auto* s : MHD_STATE) require(host_grids.count(s), SPRUCE_E_ARG, std::string("state variable ") + s + " not set");
+    // computeIterationBounds (plasmadomain.cpp:138-161)
+    require(nx > 2 * N_GHOST && ny > 2 * N_GHOST, SPRUCE_E_ARG, "Grid too small for ghost zones");
+    const int* bc = desc.bc;
+    for (int k = 0; k < 4; ++k) require(bc[k] != SPRUCE_BC_OPEN_MOC, SPRUCE_E_UNSUPPORTED, "open_moc boundaries are not available in this build");
+    require((bc[0] == 0) == (bc[1] == 0) && (bc[2] == 0) == (bc[3] == 0), SPRUCE_E_ARG,
+            "an axis must be periodic on both sides or on neither");
+    xl = bc[0] == 0 ? 0 : N_GHOST;
+    xu = bc[1] == 0 ? nx - 1 : nx - N_GHOST - 1;
+    yl = bc[2] == 0 ? 0 : N_GHOST;
+    yu = bc[3] == 0 ? ny - 1 : ny - N_GHOST - 1;
+    // The mesh must be separable: d_x constant along y, d_y constant along x (README.md:41).
+    const auto& dxg = host_grids["d_x"];
+    const auto& dyg = host_grids["d_y"];
+    std::vector<double> dx(nx), dy(ny);
+    for (int i = 0; i < nx; ++i) dx[i] = dxg[(size_t)i * ny];
+    for (int j = 0; j < ny; ++j) dy[j] = dyg[j];
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j) {
+        require(dxg[(size_t)i * ny + j] == dx[i], SPRUCE_E_UNSUPPORTED, "d_x must not vary along y");
+        require(dyg[(size_t)i * ny + j] == dy[j], SPRUCE_E_UNSUPPORTED, "d_y must not vary along x");
+      }
+    for (double v : dx) require(std::isfinite(v) && v > 0.0, SPRUCE_E_ARG, "d_x must be positive and finite");
+    for (double v : dy) require(std::isfinite(v) && v > 0.0, SPRUCE_E_ARG, "d_y must be positive and finite");
+    // validateCellSizesAndPositions (plasmadomain.cpp:196-211)
+    const auto& px = host_grids["pos_x"];
+    const auto& py = host_grids["pos_y"];
+    for (int i = 0; i + 1 < nx; ++i)
+      for (int j = 0; j + 1 < ny; ++j) {
+        double dpx = px[(size_t)(i + 1) * ny + j] - px[(size_t)i * ny + j];
+        double ddx = 0.5 * dxg[(size_t)i * ny + j] + 0.5 * dxg[(size_t)(i + 1) * ny + j];
+        double dpy = py[(size_t)i * ny + j + 1] - py[(size_t)i * ny + j];
+        double ddy = 0.5 * dyg[(size_t)i * ny + j] + 0.5 * dyg[(size_t)i * ny + j + 1];
+        require(std::abs(1.0 - dpx / ddx) < 1.0e-4 && std::abs(1.0 - dpy / ddy) < 1.0e-4, SPRUCE_E_ARG,
+                "Cell sizes and positions must correspond");
+      }
+    const bool xper = bc[0] == 0, yper = bc[2] == 0;
+    ax.build(dx, xper);
+    ay.build(dy, yper);
+    mesh.x = ax.view(nx, xl, xu, xper);
+    mesh.y = ay.view(ny, yl, yu, yper);
+    mesh.nx = nx;
+    mesh.ny = ny;
+
+    kc.m = desc.ion_mass;
+    kc.rm = 1.0 / desc.ion_mass;
+    kc.gamma = desc.adiabatic_index;
+    kc.gm1 = desc.adiabatic_index - 1.0;
+    kc.rgm1 = 1.0 / kc.gm1;
+    kc.nmin = desc.density_min;
+    kc.temin = desc.thermal_energy_min;
+    kc.tmin = desc.temp_min;
+    kc.eps = desc.epsilon;
+    kc.obs = desc.open_boundary_strength;
+    kc.r4pi = 1.0 / (4.0 * PI);
+    kc.r6 = 1.0 / 6.0;
+    kc.r3 = 1.0 / 3.0;
+    require(std::isfinite(kc.m) && kc.m > 0 && std::isnormal(kc.rm), SPRUCE_E_ARG, "ion_mass must be positive");
+    require(std::isfinite(kc.gm1) && kc.gm1 > 0 && std::isnormal(kc.rgm1), SPRUCE_E_ARG, "adiabatic_index must exceed 1");
+
+    // boundary kinds per ghost-fill call site and the open-boundary decay factors
+    for (int k = 0; k < 4; ++k) {
+      kinds_all[k] = bc[k];
+      kinds_ucnp[k] = bc[k] == SPRUCE_BC_OPEN_UCNP ? bc[k] : 0;
+      kinds_live[k] = (bc[k] == SPRUCE_BC_OPEN || bc[k] == SPRUCE_BC_FIXED || bc[k] == SPRUCE_BC_REFLECT) ? bc[k] : 0;
+      any_open = any_open || bc[k] == SPRUCE_BC_OPEN;
+      const std::vector<double>& d = k < 2 ? dx : dy;
+      const int n = (int)d.size();
+      const bool upper = k & 1;
+      const int e1 = upper ? n - 1 : 0, e2 = upper ? n - 2 : 1, e3 = upper ? n - 3 : 2;
+      const double delta_last = d[e3];
+      const double dist23 = 0.5 * (d[e2] + d[e3]);
+      const double dist12 = 0.5 * (d[e1] + d[e2]);
+      s2[k] = std::pow(desc.open_boundary_decay_base, dist23 / delta_last);   // evolution.cpp:166-167
+      s1[k] = std::pow(desc.open_boundary_decay_base, dist12 / delta_last);
+    }
+
+    L.alloc(N);
+    W.alloc(N);
+    if (desc.time_integrator == SPRUCE_RK4) {
+      W2.alloc(N);
+      for (auto& b : kbuf) b.alloc(8 * N);
+    }
+    upload(bex, host_grids["be_x"]);
+    upload(bey, host_grids["be_y"]);
+    upload(bez, host_grids["be_z"]);
+    upload(gx, host_grids["grav_x"]);
+    upload(gy, host_grids["grav_y"]);
+    hip_check(hipMemcpy(L.rho.p, host_grids["rho"].data(), N * 8, hipMemcpyHostToDevice), "upload rho");
+    hip_check(hipMemcpy(L.mx.p, host_grids["mom_x"].data(), N * 8, hipMemcpyHostToDevice), "upload mom_x");
+    hip_check(hipMemcpy(L.my.p, host_grids["mom_y"].data(), N * 8, hipMemcpyHostToDevice), "upload mom_y");
+    hip_check(hipMemcpy(L.mz.p, host_grids["mom_z"].data(), N * 8, hipMemcpyHostToDevice), "upload mom_z");
+    hip_check(hipMemcpy(L.bx.p, host_grids["bi_x"].data(), N * 8, hipMemcpyHostToDevice), "upload bi_x");
+    hip_check(hipMemcpy(L.by.p, host_grids["bi_y"].data(), N * 8, hipMemcpyHostToDevice), "upload bi_y");
+    hip_check(hipMemcpy(L.bz.p, host_grids["bi_z"].data(), N * 8, hipMemcpyHostToDevice), "upload bi_z");
+    upload(scratch, host_grids["temp"]);
+
+    dt_blocks = mhd_dt_blocks(mesh);
+    partial.alloc(dt_blocks);
+    step.alloc(1);
+    history.alloc(1);
+    hip_check(hipMalloc(&err_flag, sizeof(int)), "hipMalloc");
+    hip_check(hipMemset(err_flag, 0, sizeof(int)), "hipMemset");
+
+    // populateVariablesFromState (equationset.cpp:96-104)
+    launch_mhd_populate(L.view(), scratch.p, kc, nx, ny, stream);
+    launch_mhd_ghost(L.view(), kc, mesh, kinds_all, s1, s2, GHOST_NORMAL, stream);
+    finish_step(-1);
+    sync_and_check();
+    ready = true;
+  }
+
+  MhdStatic stat() { return MhdStatic{bex.p, bey.p, bez.p, gx.p, gy.p}; }
+
+  template <class F> void timed(const char* name, F&& f) {
+    if (!clock.enabled) { f(); return; }
+    hipEvent_t a = clock.get(), b = clock.get();
+    hip_check(hipEventRecord(a, stream), "hipEventRecord");
+    f();
+    hip_check(hipEventRecord(b, stream), "hipEventRecord");
+    clock.pending.push_back({name, a, b});
+  }
+
+  void stage(SetBufs& S, SetBufs& B, SetBufs& T, double mult, int mode, int kslot) {
+    StageArgs a{};
+    a.S = S.view(); a.B = B.view(); a.T = T.view();
+    a.st = stat(); a.k = kc; a.mesh = mesh;
+    a.step = step.p; a.step_mult = mult; a.mode = mode;
+    a.write_pre = any_open ? 1 : 0;
+    a.write_n = 1;
+    if (mode == STAGE_STORE_K) a.kbuf = kbuf[kslot].p;
+    if (mode == STAGE_RK4_FINAL) { a.kbuf = kbuf[0].p; a.k2buf = kbuf[1].p; a.k3buf = kbuf[2].p; }
+    timed("mhd_stage", [&] { launch_mhd_stage(a, stream); });
+  }
+
+  void ghost(SetBufs& G, const int* kinds, int mode) {
+    if (!(kinds[0] || kinds[1] || kinds[2] || kinds[3])) return;
+    timed("mhd_ghost", [&] { launch_mhd_ghost(G.view(), kc, mesh, kinds, s1, s2, mode, stream); });
+  }
+
+  // End of a propagate on the live grids: per-cell dt, then the step size of the next step
+  // (evolution.cpp:62) written on the device.
+  void finish_step(int idx) {
+    timed("mhd_dt", [&] { launch_mhd_dt(L.view(), stat(), kc, mesh, nullptr, partial.p, stream); });
+    timed("reduce_step", [&] {
+      launch_reduce_step(partial.p, dt_blocks, kc.eps, step.p, idx >= 0 ? history.p : nullptr, idx, err_flag, stream);
+    });
+  }
+
+  void sync_and_check() {
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    int e = 0;
+    hip_check(hipMemcpy(&e, err_flag, sizeof(int), hipMemcpyDeviceToHost), "read error flag");
+    if (e) {
+      hip_check(hipMemset(err_flag, 0, sizeof(int)), "hipMemset");
+      throw Error(SPRUCE_E_NUMERIC, "Something went wrong in the min calculation (no finite dt)");
+    }
+  }
+
+  void advance(int s) {
+    const int integ = desc.time_integrator;
+    if (integ == SPRUCE_EULER) {                       // evolution.cpp:84-88
+      stage(L, L, W, 1.0, STAGE_PLAIN, 0);
+      L.swap(W);
+    } else if (integ == SPRUCE_RK2) {                  // evolution.cpp:90-101
+      stage(L, L, W, 0.5, STAGE_PLAIN, 0);
+      ghost(W, kinds_ucnp, GHOST_NORMAL);              // live-grid ghost writes here are overwritten unread
+      stage(W, L, L, 1.0, STAGE_PLAIN, 0);
+    } else {                                           // evolution.cpp:103-124
+      stage(L, L, W, 0.5, STAGE_STORE_K, 0);
+      ghost(W, kinds_ucnp, GHOST_NORMAL);
+      ghost(L, kinds_live, GHOST_LIVE_RAW);            // open/fixed/reflect write the live grids
+      stage(W, L, W2, 0.5, STAGE_STORE_K, 1);
+      ghost(W2, kinds_ucnp, GHOST_NORMAL);
+      ghost(L, kinds_live, GHOST_LIVE_RAW);
+      stage(W2, L, W, 1.0, STAGE_STORE_K, 2);
+      ghost(W, kinds_ucnp, GHOST_NORMAL);
+      ghost(L, kinds_live, GHOST_LIVE_RAW);
+      stage(W, L, L, 1.0, STAGE_RK4_FINAL, 0);
+    }
+    ghost(L, kinds_all, GHOST_NORMAL);
+    finish_step(s + 1);
+  }
+
+  void run(int nsteps, double* dt_out, int32_t* sub_out) {
+    require(ready, SPRUCE_E_STATE, "setup must precede step");
+    require(nsteps >= 0, SPRUCE_E_ARG, "nsteps must be non-negative");
+    if (nsteps == 0) return;
+    if ((int)history.n < nsteps + 1) history.alloc(nsteps + 1);
+    hip_check(hipMemcpyAsync(history.p, step.p, sizeof(double), hipMemcpyDeviceToDevice, stream), "copy step");
+    for (int s = 0; s < nsteps; ++s) advance(s);
+    sync_and_check();
+    if (clock.enabled) clock.collect();
+    std::vector<double> h(nsteps);
+    hip_check(hipMemcpy(h.data(), history.p, nsteps * sizeof(double), hipMemcpyDeviceToHost), "read dt history");
+    for (int s = 0; s < nsteps; ++s) {
+      time += h[s];                                    // evolution.cpp:80
+      iter++;
+      if (dt_out) dt_out[s] = h[s];
+    }
+    (void)sub_out;
+  }
+
+  void get_grid(const std::string& name, double* out, int64_t count) {
+    require(count == (int64_t)N, SPRUCE_E_ARG, "grid " + name + ": count must equal xdim*ydim");
+    if (is_pd(name)) {
+      auto it = host_grids.find(name);
+      require(it != host_grids.end(), SPRUCE_E_ARG, "grid " + name + " not set");
+      std::memcpy(out, it->second.data(), N * 8);
+      return;
+    }
+    require(ready, SPRUCE_E_STATE, "setup must precede get_grid");
+    const double* src = nullptr;
+    if (name == "rho") src = L.rho.p;
+    else if (name == "n") src = L.n.p;
+    else if (name == "mom_x") src = L.mx.p;
+    else if (name == "mom_y") src = L.my.p;
+    else if (name == "mom_z") src = L.mz.p;
+    else if (name == "thermal_energy") src = L.te.p;
+    else if (name == "bi_x") src = L.bx.p;
+    else if (name == "bi_y") src = L.by.p;
+    else if (name == "bi_z") src = L.bz.p;
+    else if (name == "grav_x") src = gx.p;
+    else if (name == "grav_y") src = gy.p;
+    else {
+      int which = -1;
+      for (int k = 0; k < 14; ++k) if (name == MHD_DERIVED[k]) which = k;
+      require(which >= 0, SPRUCE_E_ARG, "Variable name <" + name + "> not recognized");
+      launch_mhd_derived(L.view(), stat(), kc, mesh, which, scratch.p, stream);
+      src = scratch.p;
+    }
+    hip_check(hipMemcpyAsync(out, src, N * 8, hipMemcpyDeviceToHost, stream), "download grid");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  }
+
+  double* device_grid(const std::string& name) {
+    require(ready, SPRUCE_E_STATE, "setup must precede device_grid");
+    if (name == "rho") return L.rho.p;
+    if (name == "n") return L.n.p;
+    if (name == "mom_x") return L.mx.p;
+    if (name == "mom_y") return L.my.p;
+    if (name == "mom_z") return L.mz.p;
+    if (name == "thermal_energy") return L.te.p;
+    if (name == "bi_x") return L.bx.p;
+    if (name == "bi_y") return L.by.p;
+    if (name == "bi_z") return L.bz.p;
+    throw Error(SPRUCE_E_ARG, "no device-resident grid named " + name);
+  }
+
+  void rhs(double* out, int64_t count) {
+    require(ready, SPRUCE_E_STATE, "setup must precede rhs");
+    require(count == (int64_t)(8 * N), SPRUCE_E_ARG, "rhs output must hold nevolved*xdim*ydim doubles");
+    DBuf tmp;
+    tmp.alloc(8 * N);
+    launch_mhd_rhs_full(L.view(), stat(), kc, mesh, tmp.p, stream);
+    hip_check(hipMemcpyAsync(out, tmp.p, 8 * N * 8, hipMemcpyDeviceToHost, stream), "download rhs");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  }
+
+  double dt_min() {
+    require(ready, SPRUCE_E_STATE, "setup must precede dt_min");
+    double v = 0.0;
+    hip_check(hipMemcpy(&v, step.p, sizeof(double), hipMemcpyDeviceToHost), "read step");
+    return v;
+  }
+};
+
+// Device arithmetic probe kernel.
+__global__ void k_arith_probe(const double* a, const double* b, int64_t n, double* dv, double* sq, double* qd) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = a[i], y = b[i];
+  dv[i] = x / y;
+  sq[i] = sqrt(x);
+  const double ry = 1.0 / y;
+  qd[i] = qdiv(x, y, ry);
+}
+
+}  // namespace spruce
+
+using namespace spruce;
+
+struct spruce_gpu_domain {
+  std::unique_ptr<Domain> d;
+};
+
+#define API_BEGIN try {
+#define API_END(ret)                                        \
+  }                                                         \
+  catch (const Error& e) {                                  \
+    g_err = e.what();                                       \
+    return e.code;                                          \
+  }                                                         \
+  catch (const std::exception& e) {                         \
+    g_err = e.what();                                       \
+    return SPRUCE_E_ARG;                                    \
+  }                                                         \
+  return ret;
+
+static std::vector<std::string> strs(int32_t n, const char* const* p) {
+  std::vector<std::string> v;
+  for (int32_t i = 0; i < n; ++i) v.emplace_back(p[i] ? p[i] : "");
+  return v;
+}
+
+static Domain& dom(spruce_gpu_domain* h) {
+  if (!h || !h->d) throw Error(SPRUCE_E_ARG, "null domain handle");
+  return *h->d;
+}
+
+extern "C" {
+
+const char* spruce_gpu_last_error(void) { return g_err.c_str(); }
+
+spruce_gpu_domain* spruce_gpu_create(const spruce_domain_desc* desc, const char* eqset, int32_t nopts,
+                                     const char* const* keys, const char* const* vals) {
+  try {
+    if (!desc || !eqset) throw Error(SPRUCE_E_ARG, "null argument");
+    auto h = std::make_unique<spruce_gpu_domain>();
+    h->d = std::make_unique<Domain>(*desc, eqset);
+    h->d->parse_eqset(strs(nopts, keys), strs(nopts, vals));
+    return h.release();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+int spruce_gpu_add_module(spruce_gpu_domain* h, const char* name, int32_t nopts, const char* const* keys,
+                          const char* const* vals) {
+  API_BEGIN
+  std::vector<std::pair<std::string, std::string>> o;
+  auto k = strs(nopts, keys), v = strs(nopts, vals);
+  for (size_t i = 0; i < k.size(); ++i) o.emplace_back(k[i], v[i]);
+  dom(h).add_module(name ? name : "", o);
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_set_grid(spruce_gpu_domain* h, const char* name, const double* host, int64_t count) {
+  API_BEGIN
+  require(name && host, SPRUCE_E_ARG, "null argument");
+  dom(h).set_grid(name, host, count);
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_setup(spruce_gpu_domain* h) {
+  API_BEGIN
+  dom(h).setup();
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_step(spruce_gpu_domain* h, int32_t nsteps, double* dt_out, int32_t* subcycles_out) {
+  API_BEGIN
+  dom(h).run(nsteps, dt_out, subcycles_out);
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_get_grid(spruce_gpu_domain* h, const char* name, double* host, int64_t count) {
+  API_BEGIN
+  require(name && host, SPRUCE_E_ARG, "null argument");
+  dom(h).get_grid(name, host, count);
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_time(spruce_gpu_domain* h, double* t, int32_t* it) {
+  API_BEGIN
+  Domain& d = dom(h);
+  if (t) *t = d.time;
+  if (it) *it = d.iter;
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_rhs(spruce_gpu_domain* h, double* host_out, int64_t count) {
+  API_BEGIN
+  require(host_out, SPRUCE_E_ARG, "null argument");
+  dom(h).rhs(host_out, count);
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_dt_min(spruce_gpu_domain* h, double* step_out) {
+  API_BEGIN
+  require(step_out, SPRUCE_E_ARG, "null argument");
+  *step_out = dom(h).dt_min();
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_num_evolved(spruce_gpu_domain* h) {
+  API_BEGIN
+  (void)dom(h);
+  return 8;
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_stream(spruce_gpu_domain* h, void** s) {
+  API_BEGIN
+  require(s, SPRUCE_E_ARG, "null argument");
+  *s = (void*)dom(h).stream;
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_device_grid(spruce_gpu_domain* h, const char* name, double** p) {
+  API_BEGIN
+  require(name && p, SPRUCE_E_ARG, "null argument");
+  *p = dom(h).device_grid(name);
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_set_profiling(spruce_gpu_domain* h, int32_t enable) {
+  API_BEGIN
+  Domain& d = dom(h);
+  d.clock.enabled = enable != 0;
+  d.clock.acc.clear();
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_kernel_times(spruce_gpu_domain* h, char* names, int32_t names_len, double* ms_avg, int32_t* launches,
+                            int32_t max_kernels) {
+  API_BEGIN
+  Domain& d = dom(h);
+  std::string all;
+  int k = 0;
+  for (auto& kv : d.clock.acc) {
+    if (k < max_kernels) {
+      if (ms_avg) ms_avg[k] = kv.second.first / std::max(1, kv.second.second);
+      if (launches) launches[k] = kv.second.second;
+      if (!all.empty()) all += ",";
+      all += kv.first;
+    }
+    ++k;
+  }
+  if (names && names_len > 0) {
+    std::strncpy(names, all.c_str(), names_len - 1);
+    names[names_len - 1] = 0;
+  }
+  return k;
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_arith_probe(const double* a, const double* b, int64_t n, double* div_out, double* sqrt_out,
+                           double* qdiv_out) {
+  API_BEGIN
+  require(a && b && div_out && sqrt_out && qdiv_out && n >= 0, SPRUCE_E_ARG, "bad argument");
+  if (n == 0) return SPRUCE_OK;
+  DBuf da, db, d1, d2, d3;
+  da.alloc(n); db.alloc(n); d1.alloc(n); d2.alloc(n); d3.alloc(n);
+  hip_check(hipMemcpy(da.p, a, n * 8, hipMemcpyHostToDevice), "upload");
+  hip_check(hipMemcpy(db.p, b, n * 8, hipMemcpyHostToDevice), "upload");
+  hipLaunchKernelGGL(k_arith_probe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, da.p, db.p, n, d1.p, d2.p, d3.p);
+  hip_check(hipGetLastError(), "launch");
+  hip_check(hipMemcpy(div_out, d1.p, n * 8, hipMemcpyDeviceToHost), "download");
+  hip_check(hipMemcpy(sqrt_out, d2.p, n * 8, hipMemcpyDeviceToHost), "download");
+  hip_check(hipMemcpy(qdiv_out, d3.p, n * 8, hipMemcpyDeviceToHost), "download");
+  API_END(SPRUCE_OK)
+}
+
+void spruce_gpu_destroy(spruce_gpu_domain* h) { delete h; }
+
+}  // extern "C"

@@ -1,0 +1,481 @@
+// IdealMHD time-step kernels for gfx950.
+//
+// Arithmetic follows the reference operation by operation (source/equationsets/idealmhd.cpp,
+// source/mhd/derivs.cpp, source/mhd/evolution.cpp); every quantity is rounded exactly where the
+// reference's whole-grid operations round it, so results are bit-identical to the CPU build.
+// Memory layout: one contiguous FP64 array per variable, x-major with y contiguous, so a wavefront
+// of 64 lanes reads 512 contiguous bytes per variable.
+#include <cfloat>
+
+#include "arith.hpp"
+#include "mhd.hpp"
+
+namespace spruce {
+
+namespace {
+
+// Transported quantities in the order idealmhd.cpp:52-86 transports them.
+enum TQ { T_RHO = 0, T_MX, T_MY, T_MZ, T_TE, T_BIX, T_BEX, T_BIY, T_BEY, T_BIZ, T_BEZ, T_N };
+
+__device__ __forceinline__ int wrapi(int k, int n, int per) {
+  if (per) {
+    if (k < 0) k += n;
+    else if (k >= n) k -= n;
+  }
+  return k;
+}
+
+// Everything a cell needs from one of its faces.
+struct Face {
+  double vf;       // face velocity along the axis (boundaryInterpolate of v, derivs.cpp:50,155)
+  double s[T_N];   // Barton upwind surface values (derivs.cpp:47-68)
+  double iv[3];    // interpolated v_x, v_y, v_z (derivative1D, derivs.cpp:259)
+  double ip;       // interpolated pressure
+  double ib[3];    // interpolated bi_x, bi_y, bi_z (== the upwind d2 of those quantities)
+};
+
+// Face between cells c-1 and c along axis AX; (i, j) is the cell c.
+template <int AX>
+__device__ __forceinline__ void compute_face(const MhdSet& S, const MhdStatic& st, const MhdConst& K,
+                                             const Mesh& M, int i, int j, Face& F) {
+  const Axis& ax = AX == 0 ? M.x : M.y;
+  const int c = AX == 0 ? i : j;
+  const int cm2 = wrapi(c - 2, ax.n, ax.per), cm1 = wrapi(c - 1, ax.n, ax.per), cp1 = wrapi(c + 1, ax.n, ax.per);
+  auto at = [&](int k) -> size_t { return AX == 0 ? (size_t)k * M.ny + j : (size_t)i * M.ny + k; };
+  const size_t i0 = at(cm2), i1 = at(cm1), i2 = at(c), i3 = at(cp1);
+  const double hlo = ax.h[cm1], hup = ax.h[c];
+  const double den = ax.fden[c], rden = ax.rfden[c];
+  // boundaryInterpolate(q, lower, upper) = (q_lo*h_up + q_up*h_lo)/(h_up + h_lo)
+  auto interp = [&](double qlo, double qup) { return qdiv(qlo * hup + qup * hlo, den, rden); };
+
+  const double r1 = S.rho[i1], r2 = S.rho[i2];
+  const double v1[3] = {S.mx[i1] / r1, S.my[i1] / r1, S.mz[i1] / r1};
+  const double v2[3] = {S.mx[i2] / r2, S.my[i2] / r2, S.mz[i2] / r2};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) F.iv[k] = interp(v1[k], v2[k]);
+  F.vf = F.iv[AX];
+  F.ip = interp(S.te[i1] * K.gm1, S.te[i2] * K.gm1);
+
+  const double vf = F.vf;
+  const bool pos = vf > 0.0, neg = vf < 0.0;
+  // boundaryExtrapolate through (c-2, c-1) when the flow is positive, through (c+1, c) otherwise
+  // (derivs.cpp:52,60,490-499).
+  const double g = pos ? ax.gup[c] : ax.gdn[c];
+  const int ce = pos ? cm1 : cp1;
+  const double eden = ax.fden[ce], reden = ax.rfden[ce];
+  const double* src[T_N] = {S.rho, S.mx, S.my, S.mz, S.te, S.bx, st.bex, S.by, st.bey, S.bz, st.bez};
+#pragma unroll
+  for (int k = 0; k < T_N; ++k) {
+    const double q0 = src[k][i0], q1 = src[k][i1], q2 = src[k][i2], q3 = src[k][i3];
+    const double d2 = interp(q1, q2);
+    const double a = pos ? q0 : q3, b = pos ? q1 : q2;
+    const double d1 = a + qdiv((b - a) * g, eden, reden);
+    const double d3 = b;
+    // derivs.cpp:53-65: positive flow takes min(d3, max(d1,d2)) when q_c <= q_{c-1}, else
+    // max(d3, min(d1,d2)); negative flow the mirror image.
+    const bool le = q2 <= q1;
+    const bool outer_min = pos ? le : !le;
+    const double inner = outer_min ? smax(d1, d2) : smin(d1, d2);
+    const double lim = outer_min ? smin(d3, inner) : smax(d3, inner);
+    F.s[k] = (pos || neg) ? lim : d2;
+    if (k == T_BIX) F.ib[0] = d2;
+    if (k == T_BIY) F.ib[1] = d2;
+    if (k == T_BIZ) F.ib[2] = d2;
+  }
+}
+
+// IdealMHD::computeTimeDerivativesDerived at one interior cell (idealmhd.cpp:42-105).
+__device__ __forceinline__ void mhd_rhs_cell(const MhdSet& S, const MhdStatic& st, const MhdConst& K,
+                                             const Mesh& M, int i, int j, double out[8]) {
+  Face xl, xu, yl, yu;
+  compute_face<0>(S, st, K, M, i, j, xl);
+  compute_face<0>(S, st, K, M, wrapi(i + 1, M.x.n, M.x.per), j, xu);
+  compute_face<1>(S, st, K, M, i, j, yl);
+  compute_face<1>(S, st, K, M, i, wrapi(j + 1, M.y.n, M.y.per), yu);
+  const size_t c = (size_t)i * M.ny + j;
+  const double dx = M.x.d[i], rdx = M.x.rd[i], dy = M.y.d[j], rdy = M.y.rd[j];
+
+  // transportDerivative1D (derivs.cpp:155-156) summed over both axes (derivs.cpp:219)
+  double tdiv[T_N];
+#pragma unroll
+  for (int k = 0; k < T_N; ++k) {
+    const double tx = qdiv(xu.s[k] * xu.vf - xl.s[k] * xl.vf, dx, rdx);
+    const double ty = qdiv(yu.s[k] * yu.vf - yl.s[k] * yl.vf, dy, rdy);
+    tdiv[k] = tx + ty;
+  }
+  // derivative1D (derivs.cpp:259)
+  auto ddx = [&](double up, double lo) { return qdiv(up - lo, dx, rdx); };
+  auto ddy = [&](double up, double lo) { return qdiv(up - lo, dy, rdy); };
+  const double dpx = ddx(xu.ip, xl.ip), dpy = ddy(yu.ip, yl.ip);
+  const double dvx_x = ddx(xu.iv[0], xl.iv[0]), dvx_y = ddy(yu.iv[0], yl.iv[0]);
+  const double dvy_x = ddx(xu.iv[1], xl.iv[1]), dvy_y = ddy(yu.iv[1], yl.iv[1]);
+  const double dvz_x = ddx(xu.iv[2], xl.iv[2]), dvz_y = ddy(yu.iv[2], yl.iv[2]);
+  const double dbiy_x = ddx(xu.ib[1], xl.ib[1]), dbix_y = ddy(yu.ib[0], yl.ib[0]);
+  const double dbiz_x = ddx(xu.ib[2], xl.ib[2]), dbiz_y = ddy(yu.ib[2], yl.ib[2]);
+
+  const double rho = S.rho[c], mx = S.mx[c], my = S.my[c], mz = S.mz[c];
+  const double press = S.te[c] * K.gm1;
+  const double vx = mx / rho, vy = my / rho, vz = mz / rho;
+  const double bix = S.bx[c], biy = S.by[c], biz = S.bz[c];
+  const double bex = st.bex[c], bey = st.bey[c], bez = st.bez[c];
+
+  const double curl_db = qdiv(dbiy_x - dbix_y, 4.0 * PI, K.r4pi);     // idealmhd.cpp:54
+  const double ext0 = (-curl_db) * bey, ext1 = curl_db * bex;         // :55 CrossProductZ2D
+  const double in0 = (-curl_db) * biy, in1 = curl_db * bix;           // :56
+  const double cz0 = dbiz_y, cz1 = -dbiz_x;                           // curlZ(bi_z)
+  const double bzi = qdiv(biz, 4.0 * PI, K.r4pi), bze = qdiv(bez, 4.0 * PI, K.r4pi);
+  const double inz0 = bzi * cz1, inz1 = (-bzi) * cz0;                 // :57 CrossProduct2DZ
+  const double extz0 = bze * cz1, extz1 = (-bze) * cz0;               // :58
+  const double fz_ext = qdiv(cz0 * bey - cz1 * bex, 4.0 * PI, K.r4pi);  // :59
+  const double fz_in = qdiv(cz0 * biy - cz1 * bix, 4.0 * PI, K.r4pi);   // :60
+
+  double k[8];
+  k[0] = -tdiv[T_RHO];
+  k[1] = -tdiv[T_MX] - dpx + rho * st.gx[c] + ext0 + in0 + inz0 + extz0;
+  k[2] = -tdiv[T_MY] - dpy + rho * st.gy[c] + ext1 + in1 + inz1 + extz1;
+  k[3] = -tdiv[T_MZ] + fz_ext + fz_in;
+  k[4] = -tdiv[T_TE] - press * (dvx_x + dvy_y);
+  k[5] = -tdiv[T_BIX] - tdiv[T_BEX] + (bix + bex) * dvx_x + (biy + bey) * dvx_y;
+  k[6] = -tdiv[T_BIY] - tdiv[T_BEY] + (bix + bex) * dvy_x + (biy + bey) * dvy_y;
+  k[7] = -tdiv[T_BIZ] - tdiv[T_BEZ] + (bix + bex) * dvz_x + (biy + bey) * dvz_y;
+  // Ghost mask and the characteristic-boundary terms, which are zero grids without open_moc
+  // boundaries; the momentum entries are rho*0 + v*0 (idealmhd.cpp:99-103, 324-329).
+  out[0] = k[0] + 0.0;
+  out[1] = k[1] + (rho * 0.0 + vx * 0.0);
+  out[2] = k[2] + (rho * 0.0 + vy * 0.0);
+  out[3] = k[3] + (rho * 0.0 + vz * 0.0);
+  out[4] = k[4] + 0.0;
+  out[5] = k[5] + 0.0;
+  out[6] = k[6] + 0.0;
+  out[7] = k[7] + 0.0;
+}
+
+__device__ __forceinline__ bool interior(const Mesh& M, int i, int j) {
+  return i >= M.x.lo && i <= M.x.hi && j >= M.y.lo && j <= M.y.hi;
+}
+
+// Stage: T = propagate-minimums-derived(B + step * k(S)) (equationset.cpp:222-230, 212-220;
+// idealmhd.cpp:234-263).  Ghost cells have k = +0.
+__global__ void __launch_bounds__(256) k_mhd_stage(StageArgs a) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  const Mesh& M = a.mesh;
+  if (i >= M.nx || j >= M.ny) return;
+  const size_t N = (size_t)M.nx * M.ny;
+  const size_t c = (size_t)i * M.ny + j;
+  double k[8];
+  if (interior(M, i, j)) {
+    mhd_rhs_cell(a.S, a.st, a.k, M, i, j, k);
+  } else {
+#pragma unroll
+    for (int v = 0; v < 8; ++v) k[v] = 0.0;
+  }
+  if (a.mode == STAGE_STORE_K) {
+#pragma unroll
+    for (int v = 0; v < 8; ++v) a.kbuf[v * N + c] = k[v];
+  } else if (a.mode == STAGE_RK4_FINAL) {
+    // (k1 + k4)/6 + (k2 + k3)/3 (evolution.cpp:119-121)
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const double k1 = a.kbuf[v * N + c], k2 = a.k2buf[v * N + c], k3 = a.k3buf[v * N + c];
+      k[v] = qdiv(k1 + k[v], 6.0, a.k.r6) + qdiv(k2 + k3, 3.0, a.k.r3);
+    }
+  }
+  const double step = a.step_mult * (*a.step);
+  const MhdConst& K = a.k;
+  const double rho_new = a.B.rho[c] + k[0] * step;
+  const double mx = a.B.mx[c] + k[1] * step;
+  const double my = a.B.my[c] + k[2] * step;
+  const double mz = a.B.mz[c] + k[3] * step;
+  const double te_new = a.B.te[c] + k[4] * step;
+  const double bx = a.B.bx[c] + k[5] * step;
+  const double by = a.B.by[c] + k[6] * step;
+  const double bz = a.B.bz[c] + k[7] * step;
+  // enforceMinimums (idealmhd.cpp:237-238)
+  const double rho1 = smax(qdiv(rho_new, K.m, K.rm), K.nmin) * K.m;
+  const double te1 = smax(te_new, K.temin);
+  // recomputeDerivedVarsFromEvolvedVars (idealmhd.cpp:246-252)
+  const double n = smax(qdiv(rho1, K.m, K.rm), K.nmin);
+  a.T.rho[c] = n * K.m;
+  if (a.write_n) a.T.n[c] = n;
+  if (a.write_pre) a.T.rho_pre[c] = rho1;
+  a.T.mx[c] = mx;
+  a.T.my[c] = my;
+  a.T.mz[c] = mz;
+  a.T.te[c] = smax(te1, K.temin);
+  a.T.bx[c] = bx;
+  a.T.by[c] = by;
+  a.T.bz[c] = bz;
+}
+
+// populateVariablesFromState up to the ghost fill (equationset.cpp:96-104; idealmhd.cpp:226-252).
+__global__ void __launch_bounds__(256) k_mhd_populate(MhdSet L, const double* temp, MhdConst K, int nx, int ny) {
+  const size_t N = (size_t)nx * ny;
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const double rho = L.rho[c];
+  const double n0 = smax(qdiv(rho, K.m, K.rm), K.nmin);
+  const double press = n0 * 2.0 * K_B * smax(temp[c], K.tmin);
+  const double te = qdiv(press, K.gm1, K.rgm1);
+  const double rho1 = smax(qdiv(rho, K.m, K.rm), K.nmin) * K.m;
+  const double te1 = smax(te, K.temin);
+  const double n = smax(qdiv(rho1, K.m, K.rm), K.nmin);
+  L.rho_pre[c] = rho1;
+  L.n[c] = n;
+  L.rho[c] = n * K.m;
+  L.te[c] = smax(te1, K.temin);
+}
+
+// Ghost-cell fill of one side for one boundary column/row (evolution.cpp:126-334).
+// side: 0 x lower, 1 x upper, 2 y lower, 3 y upper; t indexes along the boundary.
+__device__ void ghost_side(MhdSet G, const MhdConst& K, const Mesh& M, int side, int kind, int t,
+                           double s1, double s2, int mode) {
+  const bool xside = side < 2, upper = side & 1;
+  const Axis& ax = xside ? M.x : M.y;
+  const Axis& ot = xside ? M.y : M.x;
+  const int n = ax.n;
+  if (kind == 2 /*fixed*/) {
+    if (t < 0 || t >= ot.n) return;
+  } else {
+    if (t < ot.lo || t > ot.hi) return;
+  }
+  const int e1 = upper ? n - 1 : 0, e2 = upper ? n - 2 : 1, e3 = upper ? n - 3 : 2;
+  auto at = [&](int e) -> size_t { return xside ? (size_t)e * M.ny + t : (size_t)t * M.ny + e; };
+  const size_t c1 = at(e1), c2 = at(e2), c3 = at(e3);
+  if (kind == 1 /*open*/) {                                           // evolution.cpp:158-224
+    const double rho3 = (mode == GHOST_NORMAL) ? G.rho_pre[c3] : G.rho[c3];
+    const double te3 = G.te[c3];
+    const double rg1 = s1 * rho3, rg2 = s2 * rho3;
+    const double tg1 = s1 * te3, tg2 = s2 * te3;
+    double cs = 0.0;
+    const double csn = sqrt(K.gamma * (te3 * K.gm1) / rho3);
+    if (csn > cs) cs = csn;
+    const double vx = G.mx[c3] / rho3, vy = G.my[c3] / rho3;
+    double boost = K.obs * cs;
+    if (!upper) boost *= -1.0;
+    const double dist = 0.5 * (ax.d[e2] + ax.d[e3]);
+    const double vn = xside ? vx : vy;
+    const double bv = upper ? smax(0.0, vn + boost) : smin(0.0, vn + boost);
+    const double gv = (dist * bv - 0.5 * ax.d[e2] * vn) / (0.5 * ax.d[e3]);
+    const double mxg1 = rg1 * (xside ? gv : vx), mxg2 = rg2 * (xside ? gv : vx);
+    const double myg1 = rg1 * (xside ? vy : gv), myg2 = rg2 * (xside ? vy : gv);
+    if (mode == GHOST_NORMAL) {
+      const double n1 = smax(qdiv(rg1, K.m, K.rm), K.nmin), n2 = smax(qdiv(rg2, K.m, K.rm), K.nmin);
+      G.n[c1] = n1; G.n[c2] = n2;
+      G.rho[c1] = n1 * K.m; G.rho[c2] = n2 * K.m;
+      G.te[c1] = smax(tg1, K.temin); G.te[c2] = smax(tg2, K.temin);
+    } else {
+      G.rho[c1] = rg1; G.rho[c2] = rg2;
+      G.te[c1] = tg1; G.te[c2] = tg2;
+    }
+    G.mx[c1] = mxg1; G.mx[c2] = mxg2;
+    G.my[c1] = myg1; G.my[c2] = myg2;
+  } else if (kind == 3 /*reflect*/) {                                 // evolution.cpp:231-266
+    const double te3 = G.te[c3], rho3 = G.rho[c3];
+    G.te[c1] = te3; G.te[c2] = te3;
+    G.rho[c1] = rho3; G.rho[c2] = rho3;
+    if (mode == GHOST_NORMAL) { const double n3 = G.n[c3]; G.n[c1] = n3; G.n[c2] = n3; }
+    G.mx[c1] = 0.0; G.mx[c2] = 0.0; G.mx[c3] = 0.0;
+    G.my[c1] = 0.0; G.my[c2] = 0.0; G.my[c3] = 0.0;
+    G.mz[c1] = 0.0; G.mz[c2] = 0.0; G.mz[c3] = 0.0;
+  } else if (kind == 2 /*fixed*/) {                                   // evolution.cpp:272-282
+    G.mx[c1] = 0.0; G.mx[c2] = 0.0; G.mx[c3] = 0.0;
+    G.my[c1] = 0.0; G.my[c2] = 0.0; G.my[c3] = 0.0;
+    G.mz[c1] = 0.0; G.mz[c2] = 0.0; G.mz[c3] = 0.0;
+  } else if (kind == 5 /*open_ucnp*/) {                               // evolution.cpp:290-334
+    const int ein = upper ? ax.hi : ax.lo;
+    const size_t ci = at(ein);
+    for (int g = 0; g < N_GHOST; ++g) {
+      const size_t cb = at(upper ? ax.hi + 1 + g : g);
+      G.rho[cb] = G.rho[ci]; G.n[cb] = G.n[ci]; G.te[cb] = G.te[ci];
+      G.bx[cb] = G.bx[ci]; G.by[cb] = G.by[ci]; G.bz[cb] = G.bz[ci];
+      G.mx[cb] = G.mx[ci]; G.my[cb] = G.my[ci]; G.mz[cb] = G.mz[ci];
+    }
+  }
+}
+
+struct GhostArgs {
+  MhdSet G;
+  MhdConst k;
+  Mesh mesh;
+  int kind[4];
+  double s1[4], s2[4];
+  int mode;
+};
+
+__global__ void __launch_bounds__(256) k_mhd_ghost(GhostArgs a, int xsides) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s0 = xsides ? 0 : 2;
+  for (int s = s0; s < s0 + 2; ++s)
+    if (a.kind[s] != 0 && a.kind[s] != 4) ghost_side(a.G, a.k, a.mesh, s, a.kind[s], t, a.s1[s], a.s2[s], a.mode);
+}
+
+// IdealMHD::recomputeDT at one cell (idealmhd.cpp:279-304).
+__device__ __forceinline__ double mhd_dt_cell(const MhdSet& L, const MhdStatic& st, const MhdConst& K,
+                                              const Mesh& M, int i, int j) {
+  const size_t c = (size_t)i * M.ny + j;
+  const double rho = L.rho[c];
+  const double press = L.te[c] * K.gm1;
+  const double bx = st.bex[c] + L.bx[c], by = st.bey[c] + L.by[c], bz = st.bez[c] + L.bz[c];
+  const double bmag = sqrt(bx * bx + by * by + bz * bz);
+  const double vx = L.mx[c] / rho, vy = L.my[c] / rho;
+  const double cs = sqrt(press * K.gamma / rho);
+  const double cs2 = cs * cs;
+  const double va = bmag / sqrt(rho * (4.0 * PI));
+  const double va2 = va * va;
+  const double sum = cs2 + va2;
+  const double delta = sqrt(1.0 - cs2 * 4.0 * va2 / (sum * sum));
+  const double vfast = sqrt(sum * 0.5 * (1.0 + delta));
+  const double vslow = sqrt(sum * 0.5 * (1.0 - delta));
+  const double vmx = sqrt(vx * vx), vmy = sqrt(vy * vy);
+  const double cmax = smax(smax(smax(cs, va), vfast), vslow);
+  return 1.0 / (qdiv(vmx + cmax, M.x.d[i], M.x.rd[i]) + qdiv(vmy + cmax, M.y.d[j], M.y.rd[j]));
+}
+
+__device__ __forceinline__ double block_min(double v) {
+  // wavefront reduction with std::min semantics (grid.cpp:74-79); inputs are never NaN here
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(v, off, 64);
+    v = (o < v) ? o : v;
+  }
+  __shared__ double part[16];
+  const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+  const int w = lane >> 6;
+  if ((lane & 63) == 0) part[w] = v;
+  __syncthreads();
+  const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+  if (lane == 0) {
+    double m = part[0];
+    for (int k = 1; k < nw; ++k) m = (part[k] < m) ? part[k] : m;
+    v = m;
+  }
+  return v;
+}
+
+__global__ void __launch_bounds__(256) k_mhd_dt(MhdSet L, MhdStatic st, MhdConst K, Mesh M, double* dt_grid,
+                                                double* partial) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  double v = DBL_MAX;
+  if (i < M.nx && j < M.ny) {
+    const double dt = mhd_dt_cell(L, st, K, M, i, j);
+    if (dt_grid) dt_grid[(size_t)i * M.ny + j] = dt;
+    if (interior(M, i, j)) v = (dt < v) ? dt : v;
+  }
+  v = block_min(v);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(1024) k_reduce_step(const double* partial, int n, double eps, double* step,
+                                                      double* history, int idx, int* err) {
+  double v = DBL_MAX;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) { const double o = partial[k]; v = (o < v) ? o : v; }
+  for (int off = 32; off > 0; off >>= 1) { const double o = __shfl_xor(v, off, 64); v = (o < v) ? o : v; }
+  __shared__ double part[16];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = part[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = (part[k] < m) ? part[k] : m;
+    if (!(m < DBL_MAX)) *err = 1;   // reference asserts (grid.cpp:80)
+    const double s = eps * m;
+    *step = s;
+    if (history) history[idx] = s;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_mhd_rhs_full(MhdSet S, MhdStatic st, MhdConst K, Mesh M, double* out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= M.nx || j >= M.ny) return;
+  const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+  double k[8];
+  if (interior(M, i, j)) mhd_rhs_cell(S, st, K, M, i, j, k);
+  else for (int v = 0; v < 8; ++v) k[v] = 0.0;
+  for (int v = 0; v < 8; ++v) out[v * N + c] = k[v];
+}
+
+// Derived variables of idealmhd.cpp:241-277 for download.
+__global__ void __launch_bounds__(256) k_mhd_derived(MhdSet S, MhdStatic st, MhdConst K, Mesh M, int which,
+                                                     double* out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= M.nx || j >= M.ny) return;
+  const size_t c = (size_t)i * M.ny + j;
+  const double rho = S.rho[c];
+  const double press = S.te[c] * K.gm1;
+  const double vx = S.mx[c] / rho, vy = S.my[c] / rho, vz = S.mz[c] / rho;
+  const double bx = st.bex[c] + S.bx[c], by = st.bey[c] + S.by[c], bz = st.bez[c] + S.bz[c];
+  const double bmag = sqrt(bx * bx + by * by + bz * bz);
+  double r = 0.0;
+  switch (which) {
+    case 0: r = smax(press / (S.n[c] * (2.0 * K_B)), K.tmin); break;   // temp
+    case 1: r = press; break;
+    case 2: r = vx; break;
+    case 3: r = vy; break;
+    case 4: r = vz; break;
+    case 5: r = rho * 0.5 * (vx * vx + vy * vy); break;                 // kinetic_energy
+    case 6: r = bx; break;
+    case 7: r = by; break;
+    case 8: r = bz; break;
+    case 9: r = bmag; break;
+    case 10: r = bmag == 0.0 ? 0.0 : bx / bmag; break;
+    case 11: r = bmag == 0.0 ? 0.0 : by / bmag; break;
+    case 12: r = bmag == 0.0 ? 0.0 : bz / bmag; break;
+    case 13: r = mhd_dt_cell(S, st, K, M, i, j); break;
+  }
+  out[c] = r;
+}
+
+inline dim3 grid2d(int nx, int ny, dim3 b) { return dim3((ny + b.x - 1) / b.x, (nx + b.y - 1) / b.y); }
+
+}  // namespace
+
+void launch_mhd_stage(const StageArgs& a, hipStream_t s) {
+  dim3 b(64, 4);
+  hipLaunchKernelGGL(k_mhd_stage, grid2d(a.mesh.nx, a.mesh.ny, b), b, 0, s, a);
+}
+
+void launch_mhd_populate(MhdSet L, const double* temp, MhdConst k, int nx, int ny, hipStream_t s) {
+  const size_t N = (size_t)nx * ny;
+  hipLaunchKernelGGL(k_mhd_populate, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, L, temp, k, nx, ny);
+}
+
+void launch_mhd_ghost(MhdSet G, MhdConst k, Mesh mesh, const int* kinds, const double* s1, const double* s2,
+                      int mode, hipStream_t s) {
+  GhostArgs a;
+  a.G = G; a.k = k; a.mesh = mesh; a.mode = mode;
+  for (int q = 0; q < 4; ++q) { a.kind[q] = kinds[q]; a.s1[q] = s1[q]; a.s2[q] = s2[q]; }
+  if (a.kind[0] || a.kind[1])
+    hipLaunchKernelGGL(k_mhd_ghost, dim3((mesh.ny + 255) / 256), dim3(256), 0, s, a, 1);
+  if (a.kind[2] || a.kind[3])
+    hipLaunchKernelGGL(k_mhd_ghost, dim3((mesh.nx + 255) / 256), dim3(256), 0, s, a, 0);
+}
+
+int mhd_dt_blocks(const Mesh& m) {
+  dim3 b(64, 4);
+  dim3 g = grid2d(m.nx, m.ny, b);
+  return (int)(g.x * g.y);
+}
+
+void launch_mhd_dt(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double* dt_grid, double* partial, hipStream_t s) {
+  dim3 b(64, 4);
+  hipLaunchKernelGGL(k_mhd_dt, grid2d(mesh.nx, mesh.ny, b), b, 0, s, L, st, k, mesh, dt_grid, partial);
+}
+
+void launch_reduce_step(const double* partial, int n, double eps, double* step, double* history, int idx, int* err,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_step, dim3(1), dim3(1024), 0, s, partial, n, eps, step, history, idx, err);
+}
+
+void launch_mhd_rhs_full(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, double* out, hipStream_t s) {
+  dim3 b(64, 4);
+  hipLaunchKernelGGL(k_mhd_rhs_full, grid2d(mesh.nx, mesh.ny, b), b, 0, s, S, st, k, mesh, out);
+}
+
+void launch_mhd_derived(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, int which, double* out, hipStream_t s) {
+  dim3 b(64, 4);
+  hipLaunchKernelGGL(k_mhd_derived, grid2d(mesh.nx, mesh.ny, b), b, 0, s, S, st, k, mesh, which, out);
+}
+
+}  // namespace spruce
