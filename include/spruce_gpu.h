@@ -116,6 +116,10 @@ int spruce_gpu_kernel_times(spruce_gpu_domain* d, char* names, int32_t names_len
 int spruce_gpu_arith_probe(const double* a, const double* b, int64_t n, double* div_out, double* sqrt_out,
                            double* qdiv_out);
 
+/* Device power self-check (no reference counterpart): out[i] = x[i]^y for the exponents the solar
+ * modules use (which: 0 -> 1.5, 1 -> 2.5, 2 -> -1.5, 3 -> 1/3), correctly rounded. */
+int spruce_gpu_pow_probe(const double* x, int64_t n, int32_t which, double* out);
+
 const char* spruce_gpu_last_error(void);
 void spruce_gpu_destroy(spruce_gpu_domain* d);
 

@@ -36,6 +36,9 @@ def lib() -> ctypes.CDLL:
         L.oracle_time.restype = D
         L.oracle_time.argtypes = [P]
         L.oracle_subcycles.argtypes = [P, ctypes.POINTER(I), I]
+        L.oracle_set_pow_mode.argtypes = [I]
+        L.oracle_pow.restype = D
+        L.oracle_pow.argtypes = [D, D, I]
         L.oracle_last_error.restype = ctypes.c_char_p
         L.oracle_destroy.argtypes = [P]
         _LIB = L
@@ -51,6 +54,16 @@ def _strs(items: Sequence[str]):
 
 def _dptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def set_pow_mode(mode: int) -> None:
+    """0: the C library's pow (bit-identical to the reference); 1: correctly rounded pow (as the GPU)."""
+    lib().oracle_set_pow_mode(mode)
+
+
+def cr_pow(x: np.ndarray, y: float) -> np.ndarray:
+    f = lib().oracle_pow
+    return np.array([f(float(v), float(y), 1) for v in np.ravel(x)]).reshape(np.shape(x))
 
 
 class OracleDomain:

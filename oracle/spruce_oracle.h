@@ -34,6 +34,10 @@ int oracle_num_evolved(void* h);
 double oracle_time(void* h);
 /* Subcycle counts of the most recent step, one per module (-1 for modules without subcycles). */
 int oracle_subcycles(void* h, int* out, int nmax);
+/* pow used by the module restatements: mode 0 = the C library's pow (as the reference), mode 1 =
+ * correctly rounded (quad-precision powq rounded once), as the GPU computes it. */
+int oracle_set_pow_mode(int mode);
+double oracle_pow(double x, double y, int mode);
 const char* oracle_last_error(void);
 void oracle_destroy(void* h);
 

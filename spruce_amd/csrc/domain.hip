@@ -7,6 +7,7 @@
 // step reads it from there, so a run of steps needs no host synchronisation.
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -18,6 +19,8 @@
 #include "../../include/spruce_gpu.h"
 #include "arith.hpp"
 #include "mhd.hpp"
+#include "solar.hpp"
+#include "crpow.hpp"
 
 namespace spruce {
 
@@ -140,6 +143,42 @@ struct KernelClock {
   }
 };
 
+
+class Domain;
+
+// One Module of the reference plugin API (module.hpp:15-58) on the GPU.  pre_launch() enqueues the
+// reductions its subcycle count needs; pre_finish() turns their results into the count on the host
+// after the domain's one synchronisation per step; iterate() enqueues the subcycles and the live
+// propagate that ends every reference iterateModule.
+struct GpuModule {
+  std::string name;
+  int nsub = -1;
+  virtual ~GpuModule() = default;
+  virtual void setup(Domain& d) = 0;
+  virtual void pre_launch(Domain& d, double* scal) = 0;
+  virtual void pre_finish(double dt, const double* scal) = 0;
+  virtual void iterate(Domain& d, double dt) = 0;
+  virtual bool needs_bhat() const { return false; }
+  virtual bool needs_dt_grid() const { return false; }
+};
+
+static double opt_d(const std::vector<std::pair<std::string, std::string>>& o, const std::string& k, bool& found) {
+  for (auto& kv : o) if (kv.first == k) { found = true; return std::stod(kv.second); }
+  found = false;
+  return 0.0;
+}
+static std::string opt_s(const std::vector<std::pair<std::string, std::string>>& o, const std::string& k,
+                         const std::string& def) {
+  std::string v = def;
+  for (auto& kv : o) if (kv.first == k) v = kv.second;
+  return v;
+}
+static int subcycle_count(double dt, double m) {
+  const double q = dt / m;
+  if (!(q < 1.0e8)) throw Error(SPRUCE_E_NUMERIC, "subcycle count out of range");
+  return (int)q + 1;
+}
+
 class Domain {
  public:
   spruce_domain_desc desc{};
@@ -199,11 +238,12 @@ class Domain {
     }
   }
 
-  void add_module(const std::string& name, const std::vector<std::pair<std::string, std::string>>& opts) {
-    require(!ready, SPRUCE_E_STATE, "modules must be added before setup");
-    throw Error(SPRUCE_E_UNSUPPORTED, "module " + name + " is not available in this build");
-    modules.emplace_back(name, opts);
-  }
+  std::vector<std::unique_ptr<GpuModule>> mods;
+  DBuf bhx, bhy, dtgrid, mod_partial, mod_scal;
+  std::vector<int32_t> sub_hist;
+  bool need_dt_grid = false;
+
+  void add_module(const std::string& name, const std::vector<std::pair<std::string, std::string>>& opts);
 
   bool is_pd(const std::string& n) const { for (auto* s : PD_NAMES) if (n == s) return true; return false; }
   bool is_state(const std::string& n) const { for (auto* s : MHD_STATE) if (n == s) return true; return false; }
@@ -333,7 +373,48 @@ class Domain {
     launch_mhd_ghost(L.view(), kc, mesh, kinds_all, s1, s2, GHOST_NORMAL, stream);
     finish_step(-1);
     sync_and_check();
+    if (!mods.empty()) {
+      bhx.alloc(N);
+      bhy.alloc(N);
+      mod_partial.alloc(2 * (size_t)solar_blocks(mesh));
+      mod_scal.alloc(2 * mods.size());
+      for (auto& m : mods) {
+        m->setup(*this);
+        need_dt_grid = need_dt_grid || m->needs_dt_grid();
+      }
+      if (need_dt_grid) dtgrid.alloc(N);
+    }
     ready = true;
+  }
+
+  // The live dt grid (recomputeDT after the latest propagate), for modules that read it.
+  void refresh_dt_grid() {
+    launch_mhd_derived(L.view(), stat(), kc, mesh, 13, dtgrid.p, stream);
+  }
+
+  // propagateChanges() on the live grids (equationset.cpp:212-220), as every reference
+  // iterateModule ends (e.g. thermalconduction.cpp:154-155).
+  void propagate_live() {
+    timed("propagate_live", [&] { launch_propagate_live(L.view(), kc, nx, ny, any_open ? 1 : 0, stream); });
+    ghost(L, kinds_all, GHOST_NORMAL);
+  }
+
+  // ModuleHandler pre/iterate loops (modulehandler.cpp:34-50; evolution.cpp:65-66).
+  void run_modules(int s) {
+    bool bh = false;
+    for (auto& m : mods) bh = bh || m->needs_bhat();
+    if (bh) timed("bhat", [&] { launch_bhat(L.view(), stat(), mesh, bhx.p, bhy.p, stream); });
+    for (size_t k = 0; k < mods.size(); ++k) mods[k]->pre_launch(*this, mod_scal.p + 2 * k);
+    std::vector<double> sc(2 * mods.size() + 1);
+    hip_check(hipMemcpyAsync(sc.data(), mod_scal.p, 2 * mods.size() * sizeof(double), hipMemcpyDeviceToHost, stream),
+              "read module scalars");
+    hip_check(hipMemcpyAsync(sc.data() + 2 * mods.size(), step.p, sizeof(double), hipMemcpyDeviceToHost, stream),
+              "read step");
+    sync_and_check();
+    const double dt = sc[2 * mods.size()];
+    for (size_t k = 0; k < mods.size(); ++k) mods[k]->pre_finish(dt, sc.data() + 2 * k);
+    for (size_t k = 0; k < mods.size(); ++k) sub_hist[(size_t)s * mods.size() + k] = mods[k]->nsub;
+    for (auto& m : mods) m->iterate(*this, dt);
   }
 
   MhdStatic stat() { return MhdStatic{bex.p, bey.p, bez.p, gx.p, gy.p}; }
@@ -384,6 +465,7 @@ class Domain {
   }
 
   void advance(int s) {
+    if (!mods.empty()) run_modules(s);
     const int integ = desc.time_integrator;
     if (integ == SPRUCE_EULER) {                       // evolution.cpp:84-88
       stage(L, L, W, 1.0, STAGE_PLAIN, 0);
@@ -414,6 +496,7 @@ class Domain {
     if (nsteps == 0) return;
     if ((int)history.n < nsteps + 1) history.alloc(nsteps + 1);
     hip_check(hipMemcpyAsync(history.p, step.p, sizeof(double), hipMemcpyDeviceToDevice, stream), "copy step");
+    sub_hist.assign((size_t)nsteps * std::max<size_t>(1, mods.size()), -1);
     for (int s = 0; s < nsteps; ++s) advance(s);
     sync_and_check();
     if (clock.enabled) clock.collect();
@@ -424,7 +507,7 @@ class Domain {
       iter++;
       if (dt_out) dt_out[s] = h[s];
     }
-    (void)sub_out;
+    if (sub_out) std::memcpy(sub_out, sub_hist.data(), sub_hist.size() * sizeof(int32_t));
   }
 
   void get_grid(const std::string& name, double* out, int64_t count) {
@@ -491,6 +574,171 @@ class Domain {
   }
 };
 
+
+// ---------------------------------------------------------------------------------------------
+// ThermalConduction (source/modules/solar/thermalconduction.cpp)
+// ---------------------------------------------------------------------------------------------
+struct TcModule : GpuModule {
+  TcParams p{};
+  double eps = 0.0;
+  std::string integ = "euler";
+  bool inactive = false;
+  DBuf ta, tb, tc, k1, k2, k3, rx, ry, coef;
+  explicit TcModule(const std::vector<std::pair<std::string, std::string>>& o) {
+    name = "thermal_conduction";
+    bool f1, f2, f3;
+    const std::string sat = opt_s(o, "flux_saturation", "");
+    eps = opt_d(o, "epsilon", f1);
+    p.dsm = opt_d(o, "dt_subcycle_min", f2);
+    require(!sat.empty() && f1 && f2, SPRUCE_E_ARG,
+            "thermal_conduction needs flux_saturation, epsilon and dt_subcycle_min (uninitialised otherwise)");
+    p.saturation = sat == "true";
+    double wf = opt_d(o, "weakening_factor", f3);
+    if (!f3) wf = 1.0;
+    integ = opt_s(o, "time_integrator", "euler");
+    if (integ.empty()) integ = "euler";
+    inactive = opt_s(o, "inactive_mode", "false") == "true";
+    require(integ == "euler" || integ == "rk2" || integ == "rk4", SPRUCE_E_ARG,
+            "Invalid time integrator given for Thermal Conduction module");
+    p.k0 = wf * KAPPA_0;
+    p.c1 = K_B / (wf * KAPPA_0);
+    p.c_sat = (1.0 / 6.0) * (3.0 / 2.0);
+    p.sqrt_me = std::sqrt(M_ELECTRON);
+  }
+  bool needs_bhat() const override { return true; }
+  void setup(Domain& d) override {
+    ta.alloc(d.N); tb.alloc(d.N); tc.alloc(d.N);
+    if (integ == "rk4") { k1.alloc(d.N); k2.alloc(d.N); k3.alloc(d.N); }
+    if (p.saturation) { rx.alloc(d.N); ry.alloc(d.N); coef.alloc(d.N); }
+  }
+  void pre_launch(Domain& d, double* scal) override {
+    d.timed("tc_pre", [&] {
+      launch_tc_pre(d.L.view(), d.bhx.p, d.bhy.p, p, d.kc, d.mesh, d.mod_partial.p, solar_blocks(d.mesh), scal,
+                    d.stream);
+    });
+  }
+  void pre_finish(double dt, const double* scal) override {  // thermalconduction.cpp:179-193
+    if (p.saturation && scal[1] == 0.0) { nsub = 0; return; }
+    require(scal[0] < DBL_MAX, SPRUCE_E_NUMERIC, "Something went wrong in the min calculation (conduction)");
+    const double m = std::max(eps * scal[0], p.dsm);
+    nsub = subcycle_count(dt, m);
+  }
+  void step(Domain& d, const double* src, const double* base, double* out, double mult, int mode, double* kout,
+            double h) {
+    if (p.saturation)
+      d.timed("tc_sat", [&] {
+        launch_tc_sat(src, d.L.n.p, d.L.rho.p, d.bhx.p, d.bhy.p, p, d.kc, d.mesh, rx.p, ry.p, coef.p, d.stream);
+      });
+    TcStepArgs a{};
+    a.te_src = src; a.te_base = base; a.n = d.L.n.p; a.bhx = d.bhx.p; a.bhy = d.bhy.p;
+    a.rx = rx.p; a.ry = ry.p; a.coef = coef.p; a.te_out = out; a.k_out = kout;
+    a.k1 = k1.p; a.k2 = k2.p; a.k3 = k3.p; a.h = h; a.mult = mult; a.mode = mode; a.p = p; a.k = d.kc;
+    a.mesh = d.mesh;
+    d.timed("tc_step", [&] { launch_tc_step(a, d.stream); });
+  }
+  void iterate(Domain& d, double dt) override {  // thermalconduction.cpp:91-156
+    const double h = dt / (double)nsub;
+    const double* cur = d.L.te.p;
+    double* bufs[3] = {ta.p, tb.p, tc.p};
+    auto other = [&](const double* a, const double* b) {
+      for (double* q : bufs) if (q != a && q != b) return q;
+      return bufs[0];
+    };
+    for (int s = 0; s < nsub; ++s) {
+      if (integ == "euler") {
+        double* out = other(cur, nullptr);
+        step(d, cur, cur, out, 1.0, TC_UPDATE, nullptr, h);
+        cur = out;
+      } else if (integ == "rk2") {
+        double* half = other(cur, nullptr);
+        step(d, cur, cur, half, 0.5, TC_UPDATE, nullptr, h);
+        double* out = other(cur, half);
+        step(d, half, cur, out, 1.0, TC_UPDATE, nullptr, h);
+        cur = out;
+      } else {
+        double* i1 = other(cur, nullptr);
+        step(d, cur, cur, i1, 0.5, TC_STORE_K, k1.p, h);
+        double* i2 = other(cur, i1);
+        step(d, i1, cur, i2, 0.5, TC_STORE_K, k2.p, h);
+        step(d, i2, cur, i1, 1.0, TC_STORE_K, k3.p, h);
+        step(d, i1, cur, i2, 1.0, TC_RK4_FINAL, nullptr, h);
+        cur = i2;
+      }
+    }
+    if (inactive) return;
+    if (cur != d.L.te.p)
+      hip_check(hipMemcpyAsync(d.L.te.p, cur, d.N * 8, hipMemcpyDeviceToDevice, d.stream), "copy te");
+    d.propagate_live();
+  }
+};
+
+// ---------------------------------------------------------------------------------------------
+// RadiativeLosses (source/modules/solar/radiativelosses.cpp)
+// ---------------------------------------------------------------------------------------------
+struct RlModule : GpuModule {
+  RlParams p{};
+  std::string integ = "euler";
+  bool inactive = false;
+  DBuf tout;
+  explicit RlModule(const std::vector<std::pair<std::string, std::string>>& o) {
+    name = "radiative_losses";
+    bool f1, f2, f3;
+    p.ramp = opt_d(o, "cutoff_ramp", f1);
+    p.cut = opt_d(o, "cutoff_temp", f2);
+    p.eps = opt_d(o, "epsilon", f3);
+    require(f1 && f2 && f3, SPRUCE_E_ARG,
+            "radiative_losses needs cutoff_ramp, cutoff_temp and epsilon (uninitialised otherwise)");
+    p.cut_plus_ramp = p.cut + p.ramp;
+    p.prevent = opt_s(o, "prevent_subcycling", "false") == "true";
+    integ = opt_s(o, "time_integrator", "euler");
+    if (integ.empty()) integ = "euler";
+    inactive = opt_s(o, "inactive_mode", "false") == "true";
+    require(integ == "euler" || integ == "rk2" || integ == "rk4", SPRUCE_E_ARG,
+            "Invalid time integrator given for radiative losses module");
+  }
+  bool needs_dt_grid() const override { return p.prevent != 0; }
+  void setup(Domain& d) override {
+    tout.alloc(d.N);
+    p.pd_eps = d.kc.eps;
+  }
+  void pre_launch(Domain& d, double* scal) override {
+    if (p.prevent) d.refresh_dt_grid();
+    d.timed("rl_pre", [&] {
+      launch_rl_pre(d.L.view(), d.dtgrid.p, p, d.kc, d.mesh, d.mod_partial.p, solar_blocks(d.mesh), scal, d.stream);
+    });
+  }
+  void pre_finish(double dt, const double* scal) override {  // radiativelosses.cpp:200-205
+    if (scal[1] == 0.0) { nsub = 0; return; }
+    require(scal[0] < DBL_MAX, SPRUCE_E_NUMERIC, "Something went wrong in the min calculation (radiation)");
+    nsub = subcycle_count(dt, p.eps * scal[0]);
+  }
+  void iterate(Domain& d, double dt) override {  // radiativelosses.cpp:84-140
+    const double h = dt / (double)nsub;
+    if (p.prevent) d.refresh_dt_grid();
+    const int ig = integ == "euler" ? 0 : (integ == "rk2" ? 1 : 2);
+    d.timed("rl_iterate", [&] { launch_rl_iterate(d.L.view(), d.dtgrid.p, p, d.kc, d.mesh, nsub, h, ig, tout.p, d.stream); });
+    if (inactive) return;
+    hip_check(hipMemcpyAsync(d.L.te.p, tout.p, d.N * 8, hipMemcpyDeviceToDevice, d.stream), "copy te");
+    d.propagate_live();
+  }
+};
+
+void Domain::add_module(const std::string& name, const std::vector<std::pair<std::string, std::string>>& opts) {
+  require(!ready, SPRUCE_E_STATE, "modules must be added before setup");
+  static const char* known[] = {"ambient_heating", "radiative_losses", "thermal_conduction", "localized_heating",
+                                "field_heating", "sg_filtering", "coulomb_explosion", "eic_thermalization",
+                                "artificial_viscosity", "global_temperature", "anomalous_resistivity",
+                                "momentum_injection", "tracer_particles", "mass_injection", "ambient_heating_sink",
+                                "physical_viscosity", "div_cleaning", "boundary_outflow"};
+  bool ok = false;
+  for (auto* k : known) ok = ok || name == k;
+  require(ok, SPRUCE_E_ARG, "Module name was not recognized");
+  if (name == "thermal_conduction") mods.push_back(std::make_unique<TcModule>(opts));
+  else if (name == "radiative_losses") mods.push_back(std::make_unique<RlModule>(opts));
+  else throw Error(SPRUCE_E_UNSUPPORTED, "module " + name + " is not available in this build");
+  modules.emplace_back(name, opts);
+}
+
 // Device arithmetic probe kernel.
 __global__ void k_arith_probe(const double* a, const double* b, int64_t n, double* dv, double* sq, double* qd) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -500,6 +748,13 @@ __global__ void k_arith_probe(const double* a, const double* b, int64_t n, doubl
   sq[i] = sqrt(x);
   const double ry = 1.0 / y;
   qd[i] = qdiv(x, y, ry);
+}
+
+__global__ void k_pow_probe(const double* x, int64_t n, int which, double* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double v = x[i];
+  out[i] = which == 0 ? pow_1p5(v) : which == 1 ? pow_2p5(v) : which == 2 ? pow_m1p5(v) : pow_third(v);
 }
 
 }  // namespace spruce
@@ -676,6 +931,19 @@ int spruce_gpu_arith_probe(const double* a, const double* b, int64_t n, double* 
   hip_check(hipMemcpy(div_out, d1.p, n * 8, hipMemcpyDeviceToHost), "download");
   hip_check(hipMemcpy(sqrt_out, d2.p, n * 8, hipMemcpyDeviceToHost), "download");
   hip_check(hipMemcpy(qdiv_out, d3.p, n * 8, hipMemcpyDeviceToHost), "download");
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_pow_probe(const double* x, int64_t n, int32_t which, double* out) {
+  API_BEGIN
+  require(x && out && n >= 0 && which >= 0 && which <= 3, SPRUCE_E_ARG, "bad argument");
+  if (n == 0) return SPRUCE_OK;
+  DBuf dx, dout;
+  dx.alloc(n); dout.alloc(n);
+  hip_check(hipMemcpy(dx.p, x, n * 8, hipMemcpyHostToDevice), "upload");
+  hipLaunchKernelGGL(k_pow_probe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, dx.p, n, which, dout.p);
+  hip_check(hipGetLastError(), "launch");
+  hip_check(hipMemcpy(out, dout.p, n * 8, hipMemcpyDeviceToHost), "download");
   API_END(SPRUCE_OK)
 }
 

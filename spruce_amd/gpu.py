@@ -68,6 +68,7 @@ def lib() -> ctypes.CDLL:
     L.spruce_gpu_set_profiling.argtypes = [P, I32]
     L.spruce_gpu_kernel_times.argtypes = [P, ctypes.c_char_p, I32, DP, ctypes.POINTER(I32), I32]
     L.spruce_gpu_arith_probe.argtypes = [DP, DP, I64, DP, DP, DP]
+    L.spruce_gpu_pow_probe.argtypes = [DP, I64, I32, DP]
     L.spruce_gpu_last_error.restype = ctypes.c_char_p
     L.spruce_gpu_destroy.argtypes = [P]
     _LIB = L
@@ -102,6 +103,16 @@ def arith_probe(a: np.ndarray, b: np.ndarray) -> Tuple[np.ndarray, np.ndarray, n
     d, s, q = np.empty(n), np.empty(n), np.empty(n)
     _chk(lib().spruce_gpu_arith_probe(_dp(a), _dp(b), n, _dp(d), _dp(s), _dp(q)))
     return d, s, q
+
+
+POW_EXPONENTS = (1.5, 2.5, -1.5, 1.0 / 3.0)
+
+
+def pow_probe(x: np.ndarray, which: int) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.empty(x.size)
+    _chk(lib().spruce_gpu_pow_probe(_dp(x), x.size, which, _dp(out)))
+    return out
 
 
 class GpuDomain:

@@ -54,6 +54,41 @@ def _mhd(bc, integ, steps=6, **kw) -> ConfigFile:
 P4 = ("periodic",) * 4
 
 
+def mini_arcade(nx: int, ny: int, seed: int, hot: float = 1e6) -> StateFile:
+    """A small stratified arcade with seeded velocity and field perturbations."""
+    st = W.arcade_state(nx, ny, dx=5e6, dy0=5e6, stretch=1.03, t_corona=hot, depth=4e7)
+    u = [W.uniform(seed * 16 + k, (nx, ny)) for k in range(4)]
+    rho = st.grids["rho"]
+    st.grids["mom_x"] = rho * 2e6 * (2 * u[0] - 1)
+    st.grids["mom_y"] = rho * 2e6 * (2 * u[1] - 1)
+    st.grids["bi_x"] = 0.5 * (2 * u[2] - 1)
+    st.grids["bi_y"] = 0.5 * (2 * u[3] - 1)
+    st.grids["temp"] = st.grids["temp"] * (1.0 + 0.1 * (2 * W.uniform(seed * 16 + 9, (nx, ny)) - 1))
+    return st
+
+
+def _tc(sat="false", eps="0.01", integ="euler", wf=None):
+    o = [("flux_saturation", sat), ("epsilon", eps), ("dt_subcycle_min", "1e-9"), ("time_integrator", integ)]
+    if wf:
+        o.append(("weakening_factor", wf))
+    return ("thermal_conduction", o)
+
+
+def _rl(eps="0.5", integ="euler", ramp="2e4", cut="3e4", prevent=None):
+    o = [("cutoff_ramp", ramp), ("cutoff_temp", cut), ("epsilon", eps), ("time_integrator", integ)]
+    if prevent:
+        o.append(("prevent_subcycling", prevent))
+    return ("radiative_losses", o)
+
+
+def _solar(bc, integ, steps, mods, **kw) -> ConfigFile:
+    return W.base_config("ideal_mhd", bc, integ, epsilon=0.2, max_iterations=steps, density_min=1.0,
+                         temp_min=1e4, thermal_energy_min=1e-8, output_flags=MHD_OUT, modules=mods, **kw)
+
+
+SOLAR_BC = ("periodic", "periodic", "fixed", "open")
+
+
 def _cases() -> Dict[str, Tuple[Callable[[], StateFile], Callable[[], ConfigFile]]]:
     c: Dict[str, Tuple[Callable[[], StateFile], Callable[[], ConfigFile]]] = {}
     wave = lambda: W.periodic_wave_state(24, 20, seed=0)  # noqa: E731
@@ -91,6 +126,18 @@ def _cases() -> Dict[str, Tuple[Callable[[], StateFile], Callable[[], ConfigFile
                                                  output_flags=MHD_OUT))
     c["mhd_static_stretch_rk2"] = (lambda: random_mhd_state(18, 20, 12, stretch=True, zero_mom=True),
                                    lambda: _mhd(("fixed", "fixed", "open", "open"), "rk2", 6))
+    # thermal conduction and radiative losses (configs 1 and 4)
+    c["solar_tc_rl_rk2"] = (lambda: mini_arcade(20, 28, 21),
+                            lambda: _solar(SOLAR_BC, "rk2", 4, [_tc(wf="30"), _rl(eps="1e-3")]))
+    c["solar_tcsat_rk2"] = (lambda: mini_arcade(18, 26, 22, hot=3e6),
+                            lambda: _solar(SOLAR_BC, "rk2", 3, [_tc(sat="true", eps="0.05", integ="rk2", wf="50")]))
+    c["solar_rl_tc_rk4"] = (lambda: mini_arcade(16, 30, 23, hot=4e6),
+                            lambda: _solar(("open", "open", "reflect", "open"), "rk4", 3,
+                                           [_rl(eps="2e-4", integ="rk4", ramp="1e4", cut="2.5e4"),
+                                            _tc(eps="0.5", integ="rk4", wf="2")], open_boundary_strength=0.1))
+    c["solar_rl_prevent_euler"] = (lambda: mini_arcade(20, 24, 24, hot=2e7),
+                                   lambda: _solar(SOLAR_BC, "euler", 4,
+                                                  [_rl(eps="1e-3", integ="rk2", prevent="true")]))
     return c
 
 
