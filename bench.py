@@ -107,6 +107,8 @@ def main() -> None:
     ap.add_argument("--workload", default="arcade")
     ap.add_argument("--cpu-steps", type=int, default=0, help="reference steps for the CPU baseline (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stage-kernel", default="cell", choices=["strip", "cell"])
+    ap.add_argument("--strip-rows", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -117,7 +119,10 @@ def main() -> None:
 
     st, cfg, spec, desc = make_workload(args.workload)
     cells = spec.xdim * spec.ydim
-    dom = GpuDomain(spec)
+    opts = {"stage_kernel": args.stage_kernel}
+    if args.strip_rows:
+        opts["strip_rows"] = args.strip_rows
+    dom = GpuDomain(spec, options=opts)
     dom.step(args.warmup)
     t0 = time.perf_counter()
     _, subs = dom.step(args.steps, subcycles=True)   # returns after a device synchronisation
@@ -155,7 +160,7 @@ def main() -> None:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": desc,
+        "config": dict(desc, stage_kernel=args.stage_kernel, strip_rows=args.strip_rows or "auto"),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "mhd_stage", "kernel_ms": stage_ms,

@@ -104,6 +104,12 @@ int spruce_gpu_num_evolved(spruce_gpu_domain* d);
 int spruce_gpu_stream(spruce_gpu_domain* d, void** hip_stream);
 int spruce_gpu_device_grid(spruce_gpu_domain* d, const char* name, double** dev_ptr);
 
+/* Execution options (no reference counterpart; results are identical for every setting):
+ *   "stage_kernel" = "strip" (default: wavefront strips marching along x) | "cell" (one thread per
+ *                    cell, faces recomputed per cell);
+ *   "strip_rows"   = rows per wavefront strip, a multiple of 4 (default: chosen from the grid size). */
+int spruce_gpu_set_option(spruce_gpu_domain* d, const char* key, const char* value);
+
 /* Average duration (ms) of each kernel of the most recent spruce_gpu_step call, measured with HIP
  * events on the handle's stream when profiling is enabled (spruce_gpu_set_profiling).  names is a
  * comma-separated list written to a caller buffer. */

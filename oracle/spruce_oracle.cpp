@@ -578,8 +578,161 @@ struct IdealMHD : EqSet {
   }
 };
 
+// ---------------------------------------------------------------------------------------------
+// Ideal2F: two fluids with electric and magnetic field evolution (source/equationsets/ideal2F.{hpp,cpp})
+// ---------------------------------------------------------------------------------------------
+constexpr double E_CH = 4.80320425e-10;   // constants.hpp:19
+constexpr double C_LIGHT = 29979245800.0;  // constants.hpp:18 (an integer literal there)
+
+struct Ideal2F : EqSet {
+  enum V { i_rho, e_rho, i_mom_x, i_mom_y, e_mom_x, e_mom_y, i_temp, e_temp, bi_x, bi_y, bi_z, E_x, E_y, E_z,
+           grav_x, grav_y, i_n, e_n, i_v_x, i_v_y, e_v_x, e_v_y, j_x, j_y, i_press, e_press, press,
+           i_thermal_energy, e_thermal_energy, rho, rho_c, n, dn, dt_, dt_i, b_x, b_y, b_z, b_mag, b_mag_xy,
+           b_hat_x, b_hat_y, curlE_z, divE, divB, i_dPdx, e_dPdx };
+  bool use_sub_cycling = true, remove_curl_terms = false;
+  explicit Ideal2F(Domain& d)
+      : EqSet(d, {"i_rho", "e_rho", "i_mom_x", "i_mom_y", "e_mom_x", "e_mom_y", "i_temp", "e_temp", "bi_x", "bi_y",
+                  "bi_z", "E_x", "E_y", "E_z", "grav_x", "grav_y", "i_n", "e_n", "i_v_x", "i_v_y", "e_v_x", "e_v_y",
+                  "j_x", "j_y", "i_press", "e_press", "press", "i_thermal_energy", "e_thermal_energy", "rho",
+                  "rho_c", "n", "dn", "dt", "dt_i", "b_x", "b_y", "b_z", "b_mag", "b_mag_xy", "b_hat_x", "b_hat_y",
+                  "curlE_z", "divE", "divB", "i_dPdx", "e_dPdx"}) {}
+  std::vector<int> state_vars() const override {
+    return {i_rho, e_rho, i_mom_x, i_mom_y, e_mom_x, e_mom_y, i_temp, e_temp, bi_x, bi_y, bi_z, E_x, E_y, E_z, grav_x, grav_y};
+  }
+  std::vector<int> evolved() const override {
+    return {i_rho, e_rho, i_mom_x, i_mom_y, e_mom_x, e_mom_y, i_thermal_energy, e_thermal_energy, E_x, E_y, E_z,
+            bi_x, bi_y, bi_z};
+  }
+  std::vector<int> densities() const override { return {i_rho, e_rho}; }
+  std::vector<int> thermal_energies() const override { return {i_thermal_energy, e_thermal_energy}; }
+  std::vector<std::vector<int>> momenta() const override { return {{i_mom_x, i_mom_y}, {e_mom_x, e_mom_y}}; }
+  std::vector<int> fields() const override { return {E_x, E_y, E_z}; }
+  void parse(const std::vector<std::string>& k, const std::vector<std::string>& v) override {  // ideal2F.cpp:10-22
+    for (size_t i = 0; i < k.size(); ++i) {
+      if (k[i] == "use_sub_cycling") use_sub_cycling = v[i] == "true";
+      else if (k[i] == "epsilon_courant" || k[i] == "verbose_2F") {}
+      else if (k[i] == "remove_curl_terms") remove_curl_terms = v[i] == "true";
+      else throw fail(k[i] + " is not recognized for this equation set.");
+    }
+  }
+  void setup_derived() override { if (remove_curl_terms) use_sub_cycling = false; }  // :24-28
+  // ideal2F.cpp:30-96
+  Grids rhs(const Grids& G) const override {
+    const Domain& D = pd;
+    require(!use_sub_cycling, "ideal_2F with use_sub_cycling = true leaves 1x1 field derivatives (ideal2F.cpp:73-95)");
+    const A &ivx = G[i_v_x], &ivy = G[i_v_y], &evx = G[e_v_x], &evy = G[e_v_y];
+    A d_irho = -D.tdiv(G[i_rho], ivx, ivy);
+    A d_erho = -D.tdiv(G[e_rho], evx, evy);
+    std::vector<A> ivb = {G[i_v_y] * G[b_z], -G[i_v_x] * G[b_z]};
+    std::vector<A> evb = {G[e_v_y] * G[b_z], -G[e_v_x] * G[b_z]};
+    A iFx = E_CH * G[i_n] * (G[E_x] + ivb[0] / C_LIGHT);
+    A iFy = E_CH * G[i_n] * (G[E_y] + ivb[1] / C_LIGHT);
+    A eFx = -E_CH * G[e_n] * (G[E_x] + evb[0] / C_LIGHT);
+    A eFy = -E_CH * G[e_n] * (G[E_y] + evb[1] / C_LIGHT);
+    A d_imx = -D.tdiv(G[i_mom_x], ivx, ivy) - D.deriv(G[i_press], 0) + G[i_rho] * G[grav_x] + iFx;
+    A d_imy = -D.tdiv(G[i_mom_y], ivx, ivy) - D.deriv(G[i_press], 1) + G[i_rho] * G[grav_y] + iFy;
+    A d_emx = -D.tdiv(G[e_mom_x], evx, evy) - D.deriv(G[e_press], 0) + G[e_rho] * G[grav_x] + eFx;
+    A d_emy = -D.tdiv(G[e_mom_y], evx, evy) - D.deriv(G[e_press], 1) + G[e_rho] * G[grav_y] + eFy;
+    A d_ite = -D.tdiv(G[i_thermal_energy], ivx, ivy) - G[i_press] * D.div2D(ivx, ivy);
+    A d_ete = -D.tdiv(G[e_thermal_energy], evx, evy) - G[e_press] * D.div2D(evx, evy);
+    A dEx, dEy, dEz, dbx, dby, dbz;
+    if (!remove_curl_terms) {
+      dEx = (+C_LIGHT) * D.deriv(G[b_z], 1) - 4. * PI * G[j_x];
+      dEy = (-C_LIGHT) * D.deriv(G[b_z], 0) - 4. * PI * G[j_y];
+      dEz = (+C_LIGHT) * (D.deriv(G[b_y], 0) - D.deriv(G[b_x], 1));
+      dbx = (-C_LIGHT) * D.deriv(G[E_z], 1);
+      dby = (+C_LIGHT) * D.deriv(G[E_z], 0);
+      dbz = (+C_LIGHT) * (D.deriv(G[E_x], 1) - D.deriv(G[E_y], 0));
+    } else {
+      dEx = (-4.) * PI * G[j_x];
+      dEy = (-4.) * PI * G[j_y];
+      A z(D.nx, D.ny, 0.0);
+      dEz = z; dbx = z; dby = z; dbz = z;
+    }
+    Grids k = {d_irho, d_erho, d_imx, d_imy, d_emx, d_emy, d_ite, d_ete, dEx, dEy, dEz, dbx, dby, dbz};
+    for (auto& g : k) g *= D.mask;
+    return k;
+  }
+  void minimums(Grids& G) const override {  // :98-105
+    G[i_rho] = pd.m_ion * vmax(G[i_rho] / pd.m_ion, pd.nmin);
+    G[e_rho] = M_ELECTRON * vmax(G[e_rho] / M_ELECTRON, pd.nmin);
+    G[i_thermal_energy] = vmax(G[i_thermal_energy], pd.temin);
+    G[e_thermal_energy] = vmax(G[e_thermal_energy], pd.temin);
+  }
+  void evolved_from_state(Grids& G) const override {  // :107-117
+    G[i_n] = G[i_rho] / pd.m_ion;
+    G[e_n] = G[e_rho] / M_ELECTRON;
+    G[i_press] = G[i_n] * K_B * G[i_temp];
+    G[e_press] = G[e_n] * K_B * G[e_temp];
+    G[press] = G[i_press] + G[e_press];
+    G[i_thermal_energy] = G[i_press] / (pd.gamma - 1.0);
+    G[e_thermal_energy] = G[e_press] / (pd.gamma - 1.0);
+  }
+  void derived(Grids& G) const override {  // :119-167
+    const Domain& D = pd;
+    G[i_n] = G[i_rho] / pd.m_ion;
+    G[e_n] = G[e_rho] / M_ELECTRON;
+    G[i_v_x] = G[i_mom_x] / G[i_rho];
+    G[i_v_y] = G[i_mom_y] / G[i_rho];
+    G[e_v_x] = G[e_mom_x] / G[e_rho];
+    G[e_v_y] = G[e_mom_y] / G[e_rho];
+    G[j_x] = G[i_n] * E_CH * G[i_v_x] - G[e_n] * E_CH * G[e_v_x];
+    G[j_y] = G[i_n] * E_CH * G[i_v_y] - G[e_n] * E_CH * G[e_v_y];
+    G[i_press] = (pd.gamma - 1.0) * G[i_thermal_energy];
+    G[e_press] = (pd.gamma - 1.0) * G[e_thermal_energy];
+    G[press] = G[i_press] + G[e_press];
+    G[i_temp] = G[i_press] / (K_B * G[i_n]);
+    G[e_temp] = G[e_press] / (K_B * G[e_n]);
+    G[b_x] = D.pd[be_x] + G[bi_x];
+    G[b_y] = D.pd[be_y] + G[bi_y];
+    G[b_z] = G[bi_z];
+    G[b_mag] = vsqrt(sq(G[b_x]) + sq(G[b_y]) + sq(G[b_z]));
+    G[b_mag_xy] = vsqrt(sq(G[b_x]) + sq(G[b_y]));
+    G[b_hat_x] = G[b_x] / G[b_mag_xy];
+    G[b_hat_y] = G[b_y] / G[b_mag_xy];
+    for (size_t c = 0; c < G[b_mag_xy].size(); ++c)
+      if (G[b_mag_xy].v[c] == 0.0) { G[b_hat_x].v[c] = 0.0; G[b_hat_y].v[c] = 0.0; }
+    G[rho] = G[i_rho] + G[e_rho];
+    G[rho_c] = E_CH * (G[i_n] - G[e_n]);
+    G[n] = G[i_n];
+    G[dn] = G[i_n] - G[e_n];
+    G[divE] = D.div2D(G[E_x], G[E_y]);
+    G[divB] = D.div2D(G[b_x], G[b_y]);
+    G[curlE_z] = -(D.deriv(G[E_x], 1) - D.deriv(G[E_y], 0));
+    G[i_dPdx] = D.deriv(G[i_press], 0);
+    G[e_dPdx] = D.deriv(G[e_press], 0);
+  }
+  void dt(Grids& G) const override {  // :169-208
+    const A& dx = pd.pd[d_x];
+    const A& dy = pd.pd[d_y];
+    A lx = dx * 2;
+    A kx = 2. * PI / lx;
+    A ly = dy * 2;
+    A ky = 2. * PI / ly;
+    A vth = vsqrt(K_B * G[e_temp] / M_ELECTRON);
+    A wpe = vsqrt(4 * PI * G[e_n] * E_CH * E_CH / M_ELECTRON);
+    A wthx = vsqrt(3 * sq(kx) * sq(vth));
+    A wthy = vsqrt(3 * sq(ky) * sq(vth));
+    A wLx = vsqrt(sq(wpe) + sq(wthx));
+    A wLy = vsqrt(sq(wpe) + sq(wthy));
+    A vLx = wLx / kx;
+    A vLy = wLy / ky;
+    A vmx = vabs(G[e_v_x]) + vLx;
+    A vmy = vabs(G[e_v_y]) + vLy;
+    A dtv = 1. / (vmx / dx + vmy / dy);
+    A dtem = dx * dy / (dx + dy) / C_LIGHT;
+    G[dt_] = dtv;
+    if (!use_sub_cycling && !remove_curl_terms) G[dt_] = vmin(G[dt_], dtem);
+    A cs = vsqrt(pd.gamma * G[i_press] / G[i_rho]);
+    A ivm = vabs(G[i_v_x]) + cs;
+    A ivmy = vabs(G[i_v_y]) + cs;
+    G[dt_i] = 1. / (ivm / dx + ivmy / dy);
+  }
+};
+
 static std::unique_ptr<EqSet> make_eqset(Domain& d, const std::string& name) {
   if (name == "ideal_mhd") return std::make_unique<IdealMHD>(d);
+  if (name == "ideal_2F") return std::make_unique<Ideal2F>(d);
   throw fail("equation set <" + name + "> is not available in the restatement");
 }
 

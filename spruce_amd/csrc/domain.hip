@@ -206,6 +206,21 @@ class Domain {
   double s1[4] = {1, 1, 1, 1}, s2[4] = {1, 1, 1, 1};
   bool any_open = false;
   KernelClock clock;
+  int stage_kernel = 1;   // 0: wavefront strips (mhd2.hip), 1: one thread per cell (mhd.hip)
+  int strip_rows = 0;     // rows per wavefront strip; 0 = chosen from the grid size
+
+  void set_option(const std::string& k, const std::string& v) {
+    if (k == "stage_kernel") {
+      require(v == "strip" || v == "cell", SPRUCE_E_ARG, "stage_kernel must be strip or cell");
+      stage_kernel = v == "cell" ? 1 : 0;
+    } else if (k == "strip_rows") {
+      const int r = std::stoi(v);
+      require(r >= 4 && r <= 256 && r % 4 == 0, SPRUCE_E_ARG, "strip_rows must be a multiple of 4 in [4, 256]");
+      strip_rows = r;
+    } else {
+      throw Error(SPRUCE_E_ARG, "unknown option " + k);
+    }
+  }
 
   explicit Domain(const spruce_domain_desc& d, const std::string& eq) : desc(d), eqset(eq) {
     require(d.xdim > 0 && d.ydim > 0, SPRUCE_E_ARG, "xdim and ydim must be positive");
@@ -361,6 +376,7 @@ class Domain {
     hip_check(hipMemcpy(L.bz.p, host_grids["bi_z"].data(), N * 8, hipMemcpyHostToDevice), "upload bi_z");
     upload(scratch, host_grids["temp"]);
 
+    if (strip_rows == 0) strip_rows = mhd_strip_rows_default(nx, ny);
     dt_blocks = mhd_dt_blocks(mesh);
     partial.alloc(dt_blocks);
     step.alloc(1);
@@ -437,7 +453,8 @@ class Domain {
     a.write_n = 1;
     if (mode == STAGE_STORE_K) a.kbuf = kbuf[kslot].p;
     if (mode == STAGE_RK4_FINAL) { a.kbuf = kbuf[0].p; a.k2buf = kbuf[1].p; a.k3buf = kbuf[2].p; }
-    timed("mhd_stage", [&] { launch_mhd_stage(a, stream); });
+    if (stage_kernel == 1) timed("mhd_stage", [&] { launch_mhd_stage(a, stream); });
+    else timed("mhd_stage", [&] { launch_mhd_stage_strip(a, strip_rows, stream); });
   }
 
   void ghost(SetBufs& G, const int* kinds, int mode) {
@@ -883,6 +900,13 @@ int spruce_gpu_device_grid(spruce_gpu_domain* h, const char* name, double** p) {
   API_BEGIN
   require(name && p, SPRUCE_E_ARG, "null argument");
   *p = dom(h).device_grid(name);
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_set_option(spruce_gpu_domain* h, const char* key, const char* value) {
+  API_BEGIN
+  require(key && value, SPRUCE_E_ARG, "null argument");
+  dom(h).set_option(key, value);
   API_END(SPRUCE_OK)
 }
 

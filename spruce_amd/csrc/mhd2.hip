@@ -1,0 +1,332 @@
+// IdealMHD stage kernel, wavefront-strip form (the production path of launch_mhd_stage).
+//
+// Each 64-lane wavefront owns a strip of 64 columns (y, contiguous in memory) and marches down R
+// rows (x).  Lanes 2..61 produce outputs; lanes 0, 1, 62, 63 only supply the y-halo.  Along x the
+// four rows a face needs (c-2 .. c+1) stay in a rotating register window, so every x-face is
+// computed once and reused by the two cells that share it; along y the neighbouring columns are
+// read from the neighbouring lanes with DPP wave shifts, so every y-face is computed once too.
+// Every cell-centred derived value (v = mom/rho, press) is computed once per row.  The arithmetic
+// of each value is the reference's (derivs.cpp, idealmhd.cpp), operation for operation.
+#include <cfloat>
+#include <type_traits>
+
+#include "arith.hpp"
+#include "mhd.hpp"
+
+namespace spruce {
+namespace {
+
+constexpr int NQ = 11;  // transported quantities, in idealmhd.cpp:52-86 order
+enum { Q_RHO = 0, Q_MX, Q_MY, Q_MZ, Q_TE, Q_BIX, Q_BEX, Q_BIY, Q_BEY, Q_BIZ, Q_BEZ };
+constexpr int OUT_COLS = 60;
+
+// lane l receives the value of lane l-1 (wave_shr:1) / lane l+1 (wave_shl:1)
+__device__ __forceinline__ double shr1(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x138, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x138, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double shl1(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x130, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x130, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// qdiv for a data-dependent divisor: the reciprocal is exact-rounded only if it is a normal number.
+__device__ __forceinline__ double qdiv_d(double a, double b, double rb) {
+  const double ab = __builtin_fabs(b);
+  if (__builtin_expect(ab > 0x1p-900 && ab < 0x1p+900, 1)) return qdiv(a, b, rb);
+  return a / b;
+}
+
+struct Face {
+  double P[NQ];   // surf * face velocity (derivs.cpp:155)
+  double iv[3];   // interpolated v_x, v_y, v_z
+  double ip;      // interpolated press
+  double ib0, ib1; // interpolated bi (the other two components than the axis's curl needs)
+};
+
+// Geometry of one face: interpolation weights and denominators, and the extrapolation geometry on
+// both sides (derivs.cpp:477-499).
+struct FaceGeom {
+  double hlo, hup, den, rden;
+  double gup, denm, rdenm;   // extrapolation through (c-2, c-1)
+  double gdn, denp, rdenp;   // extrapolation through (c+1, c)
+};
+
+__device__ __forceinline__ int wrapn(int k, int n) {
+  k %= n;
+  return k < 0 ? k + n : k;
+}
+__device__ __forceinline__ int clampn(int k, int n) { return k < 0 ? 0 : (k >= n ? n - 1 : k); }
+
+__device__ __forceinline__ FaceGeom face_geom(const Axis& ax, int c) {
+  const int n = ax.n;
+  int cm1 = c - 1, cp1 = c + 1, cc = c;
+  if (ax.per) { cc = wrapn(c, n); cm1 = wrapn(c - 1, n); cp1 = wrapn(c + 1, n); }
+  else { cc = clampn(c, n); cm1 = clampn(c - 1, n); cp1 = clampn(c + 1, n); }
+  FaceGeom g;
+  g.hlo = ax.h[cm1];
+  g.hup = ax.h[cc];
+  g.den = ax.fden[cc];
+  g.rden = ax.rfden[cc];
+  g.gup = ax.gup[cc];
+  g.denm = ax.fden[cm1];
+  g.rdenm = ax.rfden[cm1];
+  g.gdn = ax.gdn[cc];
+  g.denp = ax.fden[cp1];
+  g.rdenp = ax.rfden[cp1];
+  return g;
+}
+
+// One face from the four cells c-2, c-1 (lo), c (up), c+1 along its axis (derivs.cpp:10-73, 155).
+template <int AX>
+__device__ __forceinline__ void make_face(const double* qm2, const double* qlo, const double* qup, const double* qp1,
+                                          const double* vlo, const double* vup, double plo, double pup,
+                                          const FaceGeom& g, Face& F) {
+  auto interp = [&](double a, double b) { return qdiv(a * g.hup + b * g.hlo, g.den, g.rden); };
+#pragma unroll
+  for (int k = 0; k < 3; ++k) F.iv[k] = interp(vlo[k], vup[k]);
+  F.ip = interp(plo, pup);
+  const double vf = F.iv[AX];
+  const bool pos = vf > 0.0, neg = vf < 0.0;
+  const double ge = pos ? g.gup : g.gdn;
+  const double de = pos ? g.denm : g.denp;
+  const double rde = pos ? g.rdenm : g.rdenp;
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    const double q0 = qm2[k], q1 = qlo[k], q2 = qup[k], q3 = qp1[k];
+    const double d2 = interp(q1, q2);
+    const double ea = pos ? q0 : q3, eb = pos ? q1 : q2;
+    const double d1 = ea + qdiv((eb - ea) * ge, de, rde);
+    const bool le = q2 <= q1;
+    const bool outer_min = pos ? le : !le;
+    const double inner = outer_min ? smax(d1, d2) : smin(d1, d2);
+    const double lim = outer_min ? smin(eb, inner) : smax(eb, inner);
+    F.P[k] = ((pos || neg) ? lim : d2) * vf;
+    if (AX == 0) {            // x-faces feed derivative1D(bi_y, 0) and derivative1D(bi_z, 0)
+      if (k == Q_BIY) F.ib0 = d2;
+      if (k == Q_BIZ) F.ib1 = d2;
+    } else {                  // y-faces feed derivative1D(bi_x, 1) and derivative1D(bi_z, 1)
+      if (k == Q_BIX) F.ib0 = d2;
+      if (k == Q_BIZ) F.ib1 = d2;
+    }
+  }
+}
+
+__device__ __forceinline__ void load_row(const StageArgs& a, int i, bool ivalid, int j, bool jvalid, double* q) {
+  if (ivalid && jvalid) {
+    const size_t c = (size_t)i * a.mesh.ny + j;
+    q[Q_RHO] = a.S.rho[c];
+    q[Q_MX] = a.S.mx[c];
+    q[Q_MY] = a.S.my[c];
+    q[Q_MZ] = a.S.mz[c];
+    q[Q_TE] = a.S.te[c];
+    q[Q_BIX] = a.S.bx[c];
+    q[Q_BEX] = a.st.bex[c];
+    q[Q_BIY] = a.S.by[c];
+    q[Q_BEY] = a.st.bey[c];
+    q[Q_BIZ] = a.S.bz[c];
+    q[Q_BEZ] = a.st.bez[c];
+  } else {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) q[k] = 0.0;
+    q[Q_RHO] = 1.0;
+  }
+}
+
+// v = mom/rho (idealmhd.cpp:248-250) and press = (gamma-1) te (idealmhd.cpp:253)
+__device__ __forceinline__ void derive(const double* q, const MhdConst& K, double* v, double& p) {
+  const double rho = q[Q_RHO];
+  const double rr = 1.0 / rho;
+  v[0] = qdiv_d(q[Q_MX], rho, rr);
+  v[1] = qdiv_d(q[Q_MY], rho, rr);
+  v[2] = qdiv_d(q[Q_MZ], rho, rr);
+  p = q[Q_TE] * K.gm1;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(64) k_mhd_stage_strip(StageArgs a, int R) {
+  const Mesh& M = a.mesh;
+  const MhdConst& K = a.k;
+  const int nx = M.nx, ny = M.ny;
+  const int lane = threadIdx.x;
+  const int jraw = blockIdx.x * OUT_COLS + lane - 2;
+  int j;
+  bool jvalid;
+  if (M.y.per) { j = wrapn(jraw, ny); jvalid = true; }
+  else { jvalid = jraw >= 0 && jraw < ny; j = clampn(jraw, ny); }
+  const bool out_lane = lane >= 2 && lane < 62 && jraw >= 0 && jraw < ny;
+  const bool yint = jvalid && j >= M.y.lo && j <= M.y.hi;
+  const FaceGeom gy = face_geom(M.y, j);          // this lane's lower y-face (between j-1 and j)
+  const double dy = M.y.d[j], rdy = M.y.rd[j];
+  const double stepv = a.step_mult * (*a.step);
+  const size_t N = (size_t)nx * ny;
+  const int r0 = blockIdx.y * R;
+
+  auto row_of = [&](int r, bool& ok) -> int {
+    if (M.x.per) { ok = true; return wrapn(r, nx); }
+    ok = r >= 0 && r < nx;
+    return clampn(r, nx);
+  };
+
+  double q[4][NQ];
+  double v[2][3], p[2];
+  Face f[2];
+
+  {  // prologue: rows r0-2 .. r0+1 and the lower face of row r0
+    bool ok;
+    int i;
+    i = row_of(r0 - 2, ok); load_row(a, i, ok, j, jvalid, q[3]);
+    i = row_of(r0 - 1, ok); load_row(a, i, ok, j, jvalid, q[0]);
+    i = row_of(r0, ok);     load_row(a, i, ok, j, jvalid, q[1]);
+    i = row_of(r0 + 1, ok); load_row(a, i, ok, j, jvalid, q[2]);
+    double vt[3], pt;
+    derive(q[0], K, vt, pt);
+    derive(q[1], K, v[0], p[0]);
+    const FaceGeom gx = face_geom(M.x, r0);
+    make_face<0>(q[3], q[0], q[1], q[2], vt, v[0], pt, p[0], gx, f[0]);
+  }
+
+  auto step = [&](auto BC, int r) {
+    constexpr int B = decltype(BC)::value;
+    constexpr int SM = B % 4, S0 = (B + 1) % 4, S1 = (B + 2) % 4, S2 = (B + 3) % 4;
+    constexpr int D0 = B % 2, D1 = (B + 1) % 2;
+    bool ok;
+    const int i2 = row_of(r + 2, ok);
+    load_row(a, i2, ok, j, jvalid, q[S2]);
+    derive(q[S1], K, v[D1], p[D1]);
+    const FaceGeom gx = face_geom(M.x, r + 1);
+    make_face<0>(q[SM], q[S0], q[S1], q[S2], v[D0], v[D1], p[D0], p[D1], gx, f[D1]);
+    if (r >= nx) return;
+    // lower y-face of this lane's column in row r: cells j-2, j-1, j, j+1 from neighbouring lanes
+    double qm2[NQ], qm1[NQ], qp1[NQ], vm1[3];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      qm1[k] = shr1(q[S0][k]);
+      qm2[k] = shr1(qm1[k]);
+      qp1[k] = shl1(q[S0][k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vm1[k] = shr1(v[D0][k]);
+    const double pm1 = shr1(p[D0]);
+    Face fy;
+    make_face<1>(qm2, qm1, q[S0], qp1, vm1, v[D0], pm1, p[D0], gy, fy);
+    // the upper y-face of this column is the lower face of the next lane
+    double Pu[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) Pu[k] = shl1(fy.P[k]);
+    const double ivu0 = shl1(fy.iv[0]), ivu1 = shl1(fy.iv[1]), ivu2 = shl1(fy.iv[2]);
+    const double ipu = shl1(fy.ip), ibu0 = shl1(fy.ib0), ibu1 = shl1(fy.ib1);
+    if (!out_lane) return;
+    const int i = row_of(r, ok);
+    const size_t c = (size_t)i * ny + j;
+    double k8[8];
+    if (i >= M.x.lo && i <= M.x.hi && yint) {
+      const double dx = M.x.d[i], rdx = M.x.rd[i];
+      const Face& xl = f[D0];
+      const Face& xu = f[D1];
+      double tdiv[NQ];
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) {
+        const double tx = qdiv(xu.P[k] - xl.P[k], dx, rdx);
+        const double ty = qdiv(Pu[k] - fy.P[k], dy, rdy);
+        tdiv[k] = tx + ty;
+      }
+      auto ddx = [&](double up, double lo) { return qdiv(up - lo, dx, rdx); };
+      auto ddy = [&](double up, double lo) { return qdiv(up - lo, dy, rdy); };
+      const double dpx = ddx(xu.ip, xl.ip), dpy = ddy(ipu, fy.ip);
+      const double dvx_x = ddx(xu.iv[0], xl.iv[0]), dvx_y = ddy(ivu0, fy.iv[0]);
+      const double dvy_x = ddx(xu.iv[1], xl.iv[1]), dvy_y = ddy(ivu1, fy.iv[1]);
+      const double dvz_x = ddx(xu.iv[2], xl.iv[2]), dvz_y = ddy(ivu2, fy.iv[2]);
+      const double dbiy_x = ddx(xu.ib0, xl.ib0), dbix_y = ddy(ibu0, fy.ib0);
+      const double dbiz_x = ddx(xu.ib1, xl.ib1), dbiz_y = ddy(ibu1, fy.ib1);
+      const double* qc = q[S0];
+      const double rho = qc[Q_RHO];
+      const double press = p[D0];
+      const double vx = v[D0][0], vy = v[D0][1], vz = v[D0][2];
+      const double bix = qc[Q_BIX], biy = qc[Q_BIY], biz = qc[Q_BIZ];
+      const double bex = qc[Q_BEX], bey = qc[Q_BEY], bez = qc[Q_BEZ];
+      const double curl_db = qdiv(dbiy_x - dbix_y, 4.0 * PI, K.r4pi);
+      const double ext0 = (-curl_db) * bey, ext1 = curl_db * bex;
+      const double in0 = (-curl_db) * biy, in1 = curl_db * bix;
+      const double cz0 = dbiz_y, cz1 = -dbiz_x;
+      const double bzi = qdiv(biz, 4.0 * PI, K.r4pi), bze = qdiv(bez, 4.0 * PI, K.r4pi);
+      const double inz0 = bzi * cz1, inz1 = (-bzi) * cz0;
+      const double extz0 = bze * cz1, extz1 = (-bze) * cz0;
+      const double fz_ext = qdiv(cz0 * bey - cz1 * bex, 4.0 * PI, K.r4pi);
+      const double fz_in = qdiv(cz0 * biy - cz1 * bix, 4.0 * PI, K.r4pi);
+      k8[0] = -tdiv[Q_RHO] + 0.0;
+      k8[1] = (-tdiv[Q_MX] - dpx + rho * a.st.gx[c] + ext0 + in0 + inz0 + extz0) + (rho * 0.0 + vx * 0.0);
+      k8[2] = (-tdiv[Q_MY] - dpy + rho * a.st.gy[c] + ext1 + in1 + inz1 + extz1) + (rho * 0.0 + vy * 0.0);
+      k8[3] = (-tdiv[Q_MZ] + fz_ext + fz_in) + (rho * 0.0 + vz * 0.0);
+      k8[4] = (-tdiv[Q_TE] - press * (dvx_x + dvy_y)) + 0.0;
+      k8[5] = (-tdiv[Q_BIX] - tdiv[Q_BEX] + (bix + bex) * dvx_x + (biy + bey) * dvx_y) + 0.0;
+      k8[6] = (-tdiv[Q_BIY] - tdiv[Q_BEY] + (bix + bex) * dvy_x + (biy + bey) * dvy_y) + 0.0;
+      k8[7] = (-tdiv[Q_BIZ] - tdiv[Q_BEZ] + (bix + bex) * dvz_x + (biy + bey) * dvz_y) + 0.0;
+    } else {
+#pragma unroll
+      for (int w = 0; w < 8; ++w) k8[w] = 0.0;
+    }
+    if (MODE == STAGE_STORE_K) {
+#pragma unroll
+      for (int w = 0; w < 8; ++w) a.kbuf[w * N + c] = k8[w];
+    } else if (MODE == STAGE_RK4_FINAL) {
+#pragma unroll
+      for (int w = 0; w < 8; ++w) {
+        const double k1 = a.kbuf[w * N + c], k2 = a.k2buf[w * N + c], k3 = a.k3buf[w * N + c];
+        k8[w] = qdiv(k1 + k8[w], 6.0, K.r6) + qdiv(k2 + k3, 3.0, K.r3);
+      }
+    }
+    double b[8];
+    if (a.B.rho == a.S.rho) {
+      b[0] = q[S0][Q_RHO]; b[1] = q[S0][Q_MX]; b[2] = q[S0][Q_MY]; b[3] = q[S0][Q_MZ];
+      b[4] = q[S0][Q_TE]; b[5] = q[S0][Q_BIX]; b[6] = q[S0][Q_BIY]; b[7] = q[S0][Q_BIZ];
+    } else {
+      b[0] = a.B.rho[c]; b[1] = a.B.mx[c]; b[2] = a.B.my[c]; b[3] = a.B.mz[c];
+      b[4] = a.B.te[c]; b[5] = a.B.bx[c]; b[6] = a.B.by[c]; b[7] = a.B.bz[c];
+    }
+    const double rho_new = b[0] + k8[0] * stepv;
+    const double rho1 = smax(qdiv(rho_new, K.m, K.rm), K.nmin) * K.m;
+    const double te1 = smax(b[4] + k8[4] * stepv, K.temin);
+    const double n = smax(qdiv(rho1, K.m, K.rm), K.nmin);
+    a.T.rho[c] = n * K.m;
+    if (a.write_n) a.T.n[c] = n;
+    if (a.write_pre) a.T.rho_pre[c] = rho1;
+    a.T.mx[c] = b[1] + k8[1] * stepv;
+    a.T.my[c] = b[2] + k8[2] * stepv;
+    a.T.mz[c] = b[3] + k8[3] * stepv;
+    a.T.te[c] = smax(te1, K.temin);
+    a.T.bx[c] = b[5] + k8[5] * stepv;
+    a.T.by[c] = b[6] + k8[6] * stepv;
+    a.T.bz[c] = b[7] + k8[7] * stepv;
+  };
+
+  for (int r = r0; r < r0 + R; r += 4) {
+    step(std::integral_constant<int, 0>{}, r);
+    step(std::integral_constant<int, 1>{}, r + 1);
+    step(std::integral_constant<int, 2>{}, r + 2);
+    step(std::integral_constant<int, 3>{}, r + 3);
+  }
+}
+
+}  // namespace
+
+int mhd_strip_rows_default(int nx, int ny) {
+  // enough wavefronts to give every CU several, with at most one strip chunk of 4-row steps
+  const long tiles = (ny + OUT_COLS - 1) / OUT_COLS;
+  for (int R : {32, 16, 8}) {
+    const long waves = tiles * ((nx + R - 1) / R);
+    if (waves >= 4096) return R;
+  }
+  return 4;
+}
+
+void launch_mhd_stage_strip(const StageArgs& a, int R, hipStream_t s) {
+  dim3 grid((a.mesh.ny + OUT_COLS - 1) / OUT_COLS, (a.mesh.nx + R - 1) / R);
+  if (a.mode == STAGE_STORE_K) hipLaunchKernelGGL(k_mhd_stage_strip<STAGE_STORE_K>, grid, dim3(64), 0, s, a, R);
+  else if (a.mode == STAGE_RK4_FINAL) hipLaunchKernelGGL(k_mhd_stage_strip<STAGE_RK4_FINAL>, grid, dim3(64), 0, s, a, R);
+  else hipLaunchKernelGGL(k_mhd_stage_strip<STAGE_PLAIN>, grid, dim3(64), 0, s, a, R);
+}
+
+}  // namespace spruce

@@ -65,6 +65,7 @@ def lib() -> ctypes.CDLL:
     L.spruce_gpu_num_evolved.argtypes = [P]
     L.spruce_gpu_stream.argtypes = [P, ctypes.POINTER(P)]
     L.spruce_gpu_device_grid.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(DP)]
+    L.spruce_gpu_set_option.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p]
     L.spruce_gpu_set_profiling.argtypes = [P, I32]
     L.spruce_gpu_kernel_times.argtypes = [P, ctypes.c_char_p, I32, DP, ctypes.POINTER(I32), I32]
     L.spruce_gpu_arith_probe.argtypes = [DP, DP, I64, DP, DP, DP]
@@ -118,7 +119,7 @@ def pow_probe(x: np.ndarray, which: int) -> np.ndarray:
 class GpuDomain:
     """One PlasmaDomain advanced on the GPU through the C ABI."""
 
-    def __init__(self, spec: DomainSpec):
+    def __init__(self, spec: DomainSpec, options: Optional[Dict[str, object]] = None):
         L = lib()
         self.spec = spec
         desc = DomainDesc()
@@ -139,6 +140,8 @@ class GpuDomain:
             k = [a for a, _ in opts]
             v = [b for _, b in opts]
             _chk(L.spruce_gpu_add_module(self.h, name.encode(), len(k), _strs(k), _strs(v)))
+        for k, v in (options or {}).items():
+            _chk(L.spruce_gpu_set_option(self.h, k.encode(), str(v).encode()))
         n = spec.xdim * spec.ydim
         for name, g in spec.grids.items():
             a = np.ascontiguousarray(g, dtype=np.float64)
@@ -176,6 +179,9 @@ class GpuDomain:
         it = ctypes.c_int32()
         _chk(lib().spruce_gpu_time(self.h, ctypes.byref(t), ctypes.byref(it)))
         return t.value
+
+    def set_option(self, key: str, value) -> None:
+        _chk(lib().spruce_gpu_set_option(self.h, key.encode(), str(value).encode()))
 
     def set_profiling(self, on: bool) -> None:
         _chk(lib().spruce_gpu_set_profiling(self.h, 1 if on else 0))

@@ -181,3 +181,49 @@ def arcade_config(steps: int = 2000, integrator: str = "rk2", conduction: bool =
     return base_config("ideal_mhd", ("periodic", "periodic", "fixed", "open"), integrator, epsilon=0.2,
                        max_iterations=steps, density_min=1.0, temp_min=1e4, thermal_energy_min=1e-8,
                        modules=mods, **kw)
+
+
+M_ELECTRON = 9.1094e-28  # g (constants.hpp:9)
+
+
+def ucnp_state(nx: int, ny: int, n_peak: float = 5e9, sigma: float = 0.1, te: float = 100.0, ti: float = 1.0,
+               half_width_sigmas: float = 4.0, n_floor_frac: float = 1e-4, seed: Optional[int] = None,
+               v_noise: float = 0.0, e_noise: float = 0.0) -> StateFile:
+    """Two-fluid Gaussian plasma (ideal_2F state variables, ideal2F.hpp:36-38): ions of mass M_SR.
+
+    n = n_peak exp(-r^2 / (2 sigma^2)) + n_floor on a uniform mesh spanning +-half_width_sigmas*sigma;
+    electrons at temperature te, ions at ti; fields and momenta zero unless noise amplitudes are given
+    (seeded), which makes every term of the equations non-zero in tests.
+    """
+    L = 2 * half_width_sigmas * sigma
+    dx, dy = L / nx, L / ny
+    x = (np.arange(nx) + 0.5) * dx - 0.5 * L
+    y = (np.arange(ny) + 0.5) * dy - 0.5 * L
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    n = n_peak * np.exp(-(X * X + Y * Y) / (2 * sigma * sigma)) + n_floor_frac * n_peak
+    z = np.zeros((nx, ny))
+    g = _pd_grids(np.full(nx, dx), np.full(ny, dy))
+    grids = dict(g)
+    grids.update({"be_x": z, "be_y": z, "be_z": z, "i_rho": n * M_SR, "e_rho": n * M_ELECTRON,
+                  "i_mom_x": z, "i_mom_y": z, "e_mom_x": z, "e_mom_y": z,
+                  "i_temp": np.full((nx, ny), ti), "e_temp": np.full((nx, ny), te),
+                  "bi_x": z, "bi_y": z, "bi_z": z, "E_x": z, "E_y": z, "E_z": z, "grav_x": z, "grav_y": z})
+    if seed is not None:
+        u = [uniform(seed * 16 + k, (nx, ny)) for k in range(9)]
+        if v_noise:
+            grids["i_mom_x"] = n * M_SR * v_noise * 1e-2 * (2 * u[0] - 1)
+            grids["i_mom_y"] = n * M_SR * v_noise * 1e-2 * (2 * u[1] - 1)
+            grids["e_mom_x"] = n * M_ELECTRON * v_noise * (2 * u[2] - 1)
+            grids["e_mom_y"] = n * M_ELECTRON * v_noise * (2 * u[3] - 1)
+        if e_noise:
+            for k, name in enumerate(("E_x", "E_y", "E_z", "bi_x", "bi_y", "bi_z")):
+                grids[name] = e_noise * (2 * u[4 + k % 5] - 1) * (1.0 + 0.1 * k)
+    return StateFile(nx, ny, M_SR, GAMMA, 0.0, grids, ["#two-fluid Gaussian plasma"])
+
+
+def ucnp_config(steps: int = 5000, integrator: str = "rk2", remove_curl_terms: bool = True,
+                bc: Sequence[str] = ("open_ucnp",) * 4, **kw) -> ConfigFile:
+    return base_config("ideal_2F", bc, integrator, epsilon=0.2, max_iterations=steps, density_min=1e2,
+                       temp_min=1e-3, thermal_energy_min=1e-30,
+                       equation_set_options=[("use_sub_cycling", "false"),
+                                             ("remove_curl_terms", "true" if remove_curl_terms else "false")], **kw)

@@ -138,6 +138,17 @@ def _cases() -> Dict[str, Tuple[Callable[[], StateFile], Callable[[], ConfigFile
     c["solar_rl_prevent_euler"] = (lambda: mini_arcade(20, 24, 24, hot=2e7),
                                    lambda: _solar(SOLAR_BC, "euler", 4,
                                                   [_rl(eps="1e-3", integ="rk2", prevent="true")]))
+    # two-fluid equation set (config 2): Gaussian plasma, with and without the Maxwell curl terms
+    F2 = ["i_rho", "e_rho", "i_mom_x", "i_mom_y", "e_mom_x", "e_mom_y", "i_thermal_energy", "e_thermal_energy",
+          "E_x", "E_y", "E_z", "bi_x", "bi_y", "bi_z", "i_temp", "e_temp", "dt", "dt_i", "j_x", "divE", "curlE_z"]
+    c["tf_ucnp_rk2"] = (lambda: W.ucnp_state(24, 20, seed=31, v_noise=3e5, e_noise=1e-3),
+                        lambda: W.ucnp_config(steps=6, output_flags=F2))
+    c["tf_curl_periodic_rk4"] = (lambda: W.ucnp_state(20, 26, seed=32, v_noise=3e5, e_noise=1e-3),
+                                 lambda: W.ucnp_config(steps=5, integrator="rk4", remove_curl_terms=False,
+                                                       bc=("periodic",) * 4, output_flags=F2))
+    c["tf_open_euler"] = (lambda: W.ucnp_state(22, 18, seed=33, v_noise=3e5, e_noise=1e-3),
+                          lambda: W.ucnp_config(steps=6, integrator="euler", bc=("open", "reflect", "fixed", "open"),
+                                                open_boundary_strength=0.2, output_flags=F2))
     return c
 
 

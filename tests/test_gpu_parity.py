@@ -80,11 +80,15 @@ BIG = [
 ]
 
 
+KERNELS = [("strip", None), ("strip", 4), ("strip", 12), ("cell", None)]
+
+
+@pytest.mark.parametrize("kernel,rows", KERNELS, ids=["strip", "strip4", "strip12", "cell"])
 @pytest.mark.parametrize("name,bc,integ,kw", BIG, ids=[b[0] for b in BIG])
-def test_gpu_matches_oracle_multi_block(name, bc, integ, kw):
+def test_gpu_matches_oracle_multi_block(name, bc, integ, kw, kernel, rows):
     spec = _random_case(133, 197, 7, bc, integ, **dict(kw))
     ora = OracleDomain(spec)
-    gpu = GpuDomain(spec)
+    gpu = GpuDomain(spec, options={"stage_kernel": kernel, **({"strip_rows": rows} if rows else {})})
     try:
         rg, ro = gpu.rhs(), ora.rhs()
         assert same_bits(rg, ro), "rhs: " + describe(rg, ro)
