@@ -107,7 +107,8 @@ int spruce_gpu_device_grid(spruce_gpu_domain* d, const char* name, double** dev_
 /* Execution options (no reference counterpart; results are identical for every setting):
  *   "stage_kernel" = "strip" (default: wavefront strips marching along x) | "cell" (one thread per
  *                    cell, faces recomputed per cell);
- *   "strip_rows"   = rows per wavefront strip, a multiple of 4 (default: chosen from the grid size). */
+ *   "strip_rows"   = rows per wavefront strip, a multiple of 4 (default: chosen from the grid size);
+ *   "block"        = "<columns>x<rows>" thread block of the ideal_2F kernels (default 64x4). */
 int spruce_gpu_set_option(spruce_gpu_domain* d, const char* key, const char* value);
 
 /* Average duration (ms) of each kernel of the most recent spruce_gpu_step call, measured with HIP
@@ -128,6 +129,25 @@ int spruce_gpu_pow_probe(const double* x, int64_t n, int32_t which, double* out)
 
 const char* spruce_gpu_last_error(void);
 void spruce_gpu_destroy(spruce_gpu_domain* d);
+
+/* Batched mode (no reference counterpart: the reference runs one PlasmaDomain per process,
+ * main.cpp:17-94).  A batch advances several set-up ideal_2F domains of any sizes with one launch
+ * per kernel of the step (evolution.cpp:59-124 for each domain), each domain keeping its own step
+ * size; the members must share the time integrator.  While in a batch a domain is stepped only
+ * through it (spruce_gpu_step returns SPRUCE_E_STATE); get_grid/rhs/dt_min stay available.
+ * Destroy the batch before its domains. */
+typedef struct spruce_gpu_batch spruce_gpu_batch;
+spruce_gpu_batch* spruce_gpu_batch_create(spruce_gpu_domain* const* domains, int32_t n);
+/* "graph" = "true" (default: one step is captured once into a HIP graph and replayed) | "false";
+ * "block" = "<columns>x<rows>" thread block of every launch. */
+int spruce_gpu_batch_set_option(spruce_gpu_batch* b, const char* key, const char* value);
+/* Times trial_steps steps for each candidate block shape, keeps the fastest and restores the state
+ * of every member, so tuning does not advance the simulation.  chosen receives "<cols>x<rows>". */
+int spruce_gpu_batch_autotune(spruce_gpu_batch* b, int32_t trial_steps, char* chosen, int32_t chosen_len,
+                              double* ms_per_step);
+/* nsteps steps of every member; dt_out (may be NULL) receives n*nsteps step sizes, member-major. */
+int spruce_gpu_batch_step(spruce_gpu_batch* b, int32_t nsteps, double* dt_out);
+void spruce_gpu_batch_destroy(spruce_gpu_batch* b);
 
 #ifdef __cplusplus
 }

@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libspruce_gpu.so")
-SOURCES = ["domain.hip", "mhd.hip", "mhd2.hip", "solar.hip"]
+SOURCES = ["domain.hip", "mhd.hip", "mhd2.hip", "solar.hip", "twofluid.hip", "tf_host.hip"]
 
 # -ffp-contract=off: no implicit multiply-add fusion (the reference rounds every product).
 # -packed-fp32-ops off: packed FP32 instructions are not used on this hardware pool.
@@ -19,15 +19,33 @@ FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared", "-ffp
 
 
 def build(verbose: bool = False) -> str:
+    """Compiles every source to an object in parallel (objects under spruce_amd/_build), then links."""
+    from concurrent.futures import ThreadPoolExecutor
+
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    inc = "-I" + os.path.join(ROOT, "include")
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+
+    def comp(src: str) -> str:
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc] + compile_flags + [inc, "-c", "-o", obj, os.path.join(CSRC, src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("hipcc failed on %s:\n%s%s" % (src, r.stdout, r.stderr))
+        return obj
+
+    jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", "8"))))
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(comp, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [hipcc] + FLAGS + ["-I" + os.path.join(ROOT, "include"), "-o", tmp] + srcs
-    if verbose:
-        print(" ".join(cmd), flush=True)
+    cmd = [hipcc, "--offload-arch=gfx950", "-fPIC", "-shared", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+        raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
     return LIB
 

@@ -19,41 +19,14 @@
 #include "../../include/spruce_gpu.h"
 #include "arith.hpp"
 #include "mhd.hpp"
+#include "host_common.hpp"
 #include "solar.hpp"
 #include "crpow.hpp"
+#include "tf_host.hpp"
 
 namespace spruce {
 
-static thread_local std::string g_err;
-
-struct Error : std::runtime_error {
-  int code;
-  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-static void require(bool c, int code, const std::string& msg) { if (!c) throw Error(code, msg); }
-static void hip_check(hipError_t e, const char* what) {
-  if (e != hipSuccess) throw Error(SPRUCE_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-// Device buffer owned by the domain.
-struct DBuf {
-  double* p = nullptr;
-  size_t n = 0;
-  DBuf() = default;
-  DBuf(const DBuf&) = delete;
-  DBuf& operator=(const DBuf&) = delete;
-  void swap(DBuf& o) { std::swap(p, o.p); std::swap(n, o.n); }
-  void alloc(size_t count) {
-    release();
-    hip_check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(double)), "hipMalloc");
-    n = count;
-    hip_check(hipMemset(p, 0, std::max<size_t>(count, 1) * sizeof(double)), "hipMemset");
-    // the domain's stream does not synchronise with the null stream: finish the clear now
-    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
-  }
-  void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
-  ~DBuf() { release(); }
-};
+thread_local std::string g_err;
 
 static const char* PD_NAMES[7] = {"d_x", "d_y", "pos_x", "pos_y", "be_x", "be_y", "be_z"};
 
@@ -78,73 +51,6 @@ struct SetBufs {
   }
 };
 
-struct AxisBufs {
-  DBuf d, rd, h, hsq, rhsq, fden, rfden, gup, gdn;
-  std::vector<double> hd;  // host copy of the cell sizes
-  Axis view(int n, int lo, int hi, int per) const {
-    return Axis{d.p, rd.p, h.p, hsq.p, rhsq.p, fden.p, rfden.p, gup.p, gdn.p, n, lo, hi, per};
-  }
-  // Precomputes the geometry the stencils divide by; every reciprocal is a correctly rounded
-  // IEEE division on the host.
-  void build(const std::vector<double>& dcell, bool per) {
-    const int n = (int)dcell.size();
-    hd = dcell;
-    std::vector<double> vd(n), vrd(n), vh(n), vhsq(n), vrhsq(n), vf(n, 0.0), vrf(n, 0.0), vgu(n, 0.0), vgd(n, 0.0);
-    for (int i = 0; i < n; ++i) {
-      vd[i] = dcell[i];
-      vrd[i] = 1.0 / dcell[i];
-      vh[i] = 0.5 * dcell[i];
-      vhsq[i] = vh[i] * vh[i];
-      vrhsq[i] = 1.0 / vhsq[i];
-    }
-    auto w = [&](int k) { return per ? ((k % n) + n) % n : k; };
-    for (int c = 0; c < n; ++c) {
-      const int cm1 = w(c - 1), cm2 = w(c - 2), cp1 = w(c + 1);
-      if (cm1 >= 0 && cm1 < n) { vf[c] = vh[c] + vh[cm1]; vrf[c] = 1.0 / vf[c]; }
-      if (cm2 >= 0 && cm2 < n && cm1 >= 0) vgu[c] = vh[cm2] + 2.0 * vh[cm1];
-      if (cp1 >= 0 && cp1 < n) vgd[c] = vh[cp1] + 2.0 * vh[c];
-    }
-    auto up = [&](DBuf& b, const std::vector<double>& v) {
-      b.alloc(n);
-      hip_check(hipMemcpy(b.p, v.data(), n * sizeof(double), hipMemcpyHostToDevice), "upload geometry");
-    };
-    up(d, vd); up(rd, vrd); up(h, vh); up(hsq, vhsq); up(rhsq, vrhsq); up(fden, vf); up(rfden, vrf);
-    up(gup, vgu); up(gdn, vgd);
-  }
-};
-
-struct KernelClock {
-  bool enabled = false;
-  struct Pending { std::string name; hipEvent_t a, b; };
-  std::vector<Pending> pending;
-  std::map<std::string, std::pair<double, int>> acc;
-  std::vector<hipEvent_t> pool;
-  hipEvent_t get() {
-    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
-    hipEvent_t e;
-    hip_check(hipEventCreate(&e), "hipEventCreate");
-    return e;
-  }
-  void collect() {
-    for (auto& p : pending) {
-      float ms = 0.f;
-      hip_check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
-      auto& s = acc[p.name];
-      s.first += ms;
-      s.second += 1;
-      pool.push_back(p.a);
-      pool.push_back(p.b);
-    }
-    pending.clear();
-  }
-  ~KernelClock() {
-    for (auto& p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
-    for (auto e : pool) (void)hipEventDestroy(e);
-  }
-};
-
-
-class Domain;
 
 // One Module of the reference plugin API (module.hpp:15-58) on the GPU.  pre_launch() enqueues the
 // reductions its subcycle count needs; pre_finish() turns their results into the count on the host
@@ -179,19 +85,16 @@ static int subcycle_count(double dt, double m) {
   return (int)q + 1;
 }
 
-class Domain {
+class Domain : public DomainBase {
  public:
   spruce_domain_desc desc{};
   std::string eqset;
   std::vector<std::pair<std::string, std::vector<std::pair<std::string, std::string>>>> modules;
   std::map<std::string, std::vector<double>> host_grids;
   bool ready = false;
-  hipStream_t stream = nullptr;
   int nx = 0, ny = 0;
   size_t N = 0;
   int xl = 0, xu = 0, yl = 0, yu = 0;
-  double time = 0.0;
-  int iter = 0;
 
   AxisBufs ax, ay;
   Mesh mesh{};
@@ -205,11 +108,10 @@ class Domain {
   int kinds_all[4] = {0, 0, 0, 0}, kinds_ucnp[4] = {0, 0, 0, 0}, kinds_live[4] = {0, 0, 0, 0};
   double s1[4] = {1, 1, 1, 1}, s2[4] = {1, 1, 1, 1};
   bool any_open = false;
-  KernelClock clock;
   int stage_kernel = 1;   // 0: wavefront strips (mhd2.hip), 1: one thread per cell (mhd.hip)
   int strip_rows = 0;     // rows per wavefront strip; 0 = chosen from the grid size
 
-  void set_option(const std::string& k, const std::string& v) {
+  void set_option(const std::string& k, const std::string& v) override {
     if (k == "stage_kernel") {
       require(v == "strip" || v == "cell", SPRUCE_E_ARG, "stage_kernel must be strip or cell");
       stage_kernel = v == "cell" ? 1 : 0;
@@ -258,12 +160,13 @@ class Domain {
   std::vector<int32_t> sub_hist;
   bool need_dt_grid = false;
 
-  void add_module(const std::string& name, const std::vector<std::pair<std::string, std::string>>& opts);
+  void add_module(const std::string& name, const std::vector<std::pair<std::string, std::string>>& opts) override;
+  int num_evolved() const override { return 8; }
 
   bool is_pd(const std::string& n) const { for (auto* s : PD_NAMES) if (n == s) return true; return false; }
   bool is_state(const std::string& n) const { for (auto* s : MHD_STATE) if (n == s) return true; return false; }
 
-  void set_grid(const std::string& name, const double* p, int64_t count) {
+  void set_grid(const std::string& name, const double* p, int64_t count) override {
     require(count == (int64_t)N, SPRUCE_E_ARG, "grid " + name + ": count must equal xdim*ydim");
     bool known = is_pd(name);
     for (auto* s : MHD_NAMES) known = known || name == s;
@@ -276,7 +179,7 @@ class Domain {
     hip_check(hipMemcpy(b.p, v.data(), N * sizeof(double), hipMemcpyHostToDevice), "upload grid");
   }
 
-  void setup() {
+  void setup() override {
     require(!ready, SPRUCE_E_STATE, "setup called twice");
     for (auto* s : PD_NAMES) require(host_grids.count(s), SPRUCE_E_ARG, std::string("PlasmaDomain grid ") + s + " not set");
     for (auto* s : MHD_STATE) require(host_grids.count(s), SPRUCE_E_ARG, std::string("state variable ") + s + " not set");
@@ -435,15 +338,6 @@ class Domain {
 
   MhdStatic stat() { return MhdStatic{bex.p, bey.p, bez.p, gx.p, gy.p}; }
 
-  template <class F> void timed(const char* name, F&& f) {
-    if (!clock.enabled) { f(); return; }
-    hipEvent_t a = clock.get(), b = clock.get();
-    hip_check(hipEventRecord(a, stream), "hipEventRecord");
-    f();
-    hip_check(hipEventRecord(b, stream), "hipEventRecord");
-    clock.pending.push_back({name, a, b});
-  }
-
   void stage(SetBufs& S, SetBufs& B, SetBufs& T, double mult, int mode, int kslot) {
     StageArgs a{};
     a.S = S.view(); a.B = B.view(); a.T = T.view();
@@ -507,7 +401,7 @@ class Domain {
     finish_step(s + 1);
   }
 
-  void run(int nsteps, double* dt_out, int32_t* sub_out) {
+  void run(int nsteps, double* dt_out, int32_t* sub_out) override {
     require(ready, SPRUCE_E_STATE, "setup must precede step");
     require(nsteps >= 0, SPRUCE_E_ARG, "nsteps must be non-negative");
     if (nsteps == 0) return;
@@ -527,7 +421,7 @@ class Domain {
     if (sub_out) std::memcpy(sub_out, sub_hist.data(), sub_hist.size() * sizeof(int32_t));
   }
 
-  void get_grid(const std::string& name, double* out, int64_t count) {
+  void get_grid(const std::string& name, double* out, int64_t count) override {
     require(count == (int64_t)N, SPRUCE_E_ARG, "grid " + name + ": count must equal xdim*ydim");
     if (is_pd(name)) {
       auto it = host_grids.find(name);
@@ -559,7 +453,7 @@ class Domain {
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
   }
 
-  double* device_grid(const std::string& name) {
+  double* device_grid(const std::string& name) override {
     require(ready, SPRUCE_E_STATE, "setup must precede device_grid");
     if (name == "rho") return L.rho.p;
     if (name == "n") return L.n.p;
@@ -573,7 +467,7 @@ class Domain {
     throw Error(SPRUCE_E_ARG, "no device-resident grid named " + name);
   }
 
-  void rhs(double* out, int64_t count) {
+  void rhs(double* out, int64_t count) override {
     require(ready, SPRUCE_E_STATE, "setup must precede rhs");
     require(count == (int64_t)(8 * N), SPRUCE_E_ARG, "rhs output must hold nevolved*xdim*ydim doubles");
     DBuf tmp;
@@ -583,7 +477,7 @@ class Domain {
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
   }
 
-  double dt_min() {
+  double dt_min() override {
     require(ready, SPRUCE_E_STATE, "setup must precede dt_min");
     double v = 0.0;
     hip_check(hipMemcpy(&v, step.p, sizeof(double), hipMemcpyDeviceToHost), "read step");
@@ -779,7 +673,11 @@ __global__ void k_pow_probe(const double* x, int64_t n, int which, double* out) 
 using namespace spruce;
 
 struct spruce_gpu_domain {
-  std::unique_ptr<Domain> d;
+  std::unique_ptr<DomainBase> d;
+};
+
+struct spruce_gpu_batch {
+  std::unique_ptr<TfBatchRun> b;
 };
 
 #define API_BEGIN try {
@@ -801,9 +699,15 @@ static std::vector<std::string> strs(int32_t n, const char* const* p) {
   return v;
 }
 
-static Domain& dom(spruce_gpu_domain* h) {
+static DomainBase& dom(spruce_gpu_domain* h) {
   if (!h || !h->d) throw Error(SPRUCE_E_ARG, "null domain handle");
   return *h->d;
+}
+
+static TfBatchRun& bat(spruce_gpu_batch* h) {
+  if (!h || !h->b) throw Error(SPRUCE_E_ARG, "null batch handle");
+  require(!h->b->broken, SPRUCE_E_STATE, "a member domain of this batch was destroyed");
+  return *h->b;
 }
 
 extern "C" {
@@ -815,8 +719,13 @@ spruce_gpu_domain* spruce_gpu_create(const spruce_domain_desc* desc, const char*
   try {
     if (!desc || !eqset) throw Error(SPRUCE_E_ARG, "null argument");
     auto h = std::make_unique<spruce_gpu_domain>();
-    h->d = std::make_unique<Domain>(*desc, eqset);
-    h->d->parse_eqset(strs(nopts, keys), strs(nopts, vals));
+    if (std::string(eqset) == "ideal_2F") {
+      h->d = std::make_unique<TfDomain>(*desc, strs(nopts, keys), strs(nopts, vals));
+    } else {
+      auto d = std::make_unique<Domain>(*desc, eqset);
+      d->parse_eqset(strs(nopts, keys), strs(nopts, vals));
+      h->d = std::move(d);
+    }
     return h.release();
   } catch (const std::exception& e) {
     g_err = e.what();
@@ -862,7 +771,7 @@ int spruce_gpu_get_grid(spruce_gpu_domain* h, const char* name, double* host, in
 
 int spruce_gpu_time(spruce_gpu_domain* h, double* t, int32_t* it) {
   API_BEGIN
-  Domain& d = dom(h);
+  DomainBase& d = dom(h);
   if (t) *t = d.time;
   if (it) *it = d.iter;
   API_END(SPRUCE_OK)
@@ -884,8 +793,7 @@ int spruce_gpu_dt_min(spruce_gpu_domain* h, double* step_out) {
 
 int spruce_gpu_num_evolved(spruce_gpu_domain* h) {
   API_BEGIN
-  (void)dom(h);
-  return 8;
+  return dom(h).num_evolved();
   API_END(SPRUCE_OK)
 }
 
@@ -912,7 +820,7 @@ int spruce_gpu_set_option(spruce_gpu_domain* h, const char* key, const char* val
 
 int spruce_gpu_set_profiling(spruce_gpu_domain* h, int32_t enable) {
   API_BEGIN
-  Domain& d = dom(h);
+  DomainBase& d = dom(h);
   d.clock.enabled = enable != 0;
   d.clock.acc.clear();
   API_END(SPRUCE_OK)
@@ -921,7 +829,7 @@ int spruce_gpu_set_profiling(spruce_gpu_domain* h, int32_t enable) {
 int spruce_gpu_kernel_times(spruce_gpu_domain* h, char* names, int32_t names_len, double* ms_avg, int32_t* launches,
                             int32_t max_kernels) {
   API_BEGIN
-  Domain& d = dom(h);
+  DomainBase& d = dom(h);
   std::string all;
   int k = 0;
   for (auto& kv : d.clock.acc) {
@@ -972,5 +880,82 @@ int spruce_gpu_pow_probe(const double* x, int64_t n, int32_t which, double* out)
 }
 
 void spruce_gpu_destroy(spruce_gpu_domain* h) { delete h; }
+
+spruce_gpu_batch* spruce_gpu_batch_create(spruce_gpu_domain* const* domains, int32_t n) {
+  try {
+    require(domains && n > 0, SPRUCE_E_ARG, "a batch needs at least one domain");
+    std::vector<TfDomain*> m;
+    for (int32_t k = 0; k < n; ++k) {
+      auto* t = dynamic_cast<TfDomain*>(&dom(domains[k]));
+      require(t != nullptr, SPRUCE_E_UNSUPPORTED, "batches hold ideal_2F domains");
+      require(t->ready, SPRUCE_E_STATE, "batch members must be set up");
+      require(t->batch == nullptr, SPRUCE_E_STATE, "domain already belongs to a batch");
+      for (auto* o : m) require(o != t, SPRUCE_E_ARG, "domain listed twice");
+      m.push_back(t);
+    }
+    auto h = std::make_unique<spruce_gpu_batch>();
+    h->b = std::make_unique<TfBatchRun>(m, m[0]->block_x, m[0]->block_y);
+    for (auto* t : m) t->batch = h->b.get();
+    return h.release();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+int spruce_gpu_batch_set_option(spruce_gpu_batch* h, const char* key, const char* value) {
+  API_BEGIN
+  require(key && value, SPRUCE_E_ARG, "null argument");
+  TfBatchRun& b = bat(h);
+  const std::string k = key, v = value;
+  if (k == "graph") {
+    require(v == "true" || v == "false", SPRUCE_E_ARG, "graph must be true or false");
+    b.use_graph = v == "true";
+  } else if (k == "block") {
+    const auto x = v.find('x');
+    require(x != std::string::npos, SPRUCE_E_ARG, "block must be <columns>x<rows>");
+    const int bx = std::stoi(v.substr(0, x)), by = std::stoi(v.substr(x + 1));
+    require(bx > 0 && by > 0 && bx * by <= 256 && (bx * by) % 64 == 0, SPRUCE_E_ARG,
+            "block must hold 64, 128, 192 or 256 threads");
+    b.rebuild(bx, by);
+  } else {
+    throw Error(SPRUCE_E_ARG, "unknown batch option " + k);
+  }
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_batch_autotune(spruce_gpu_batch* h, int32_t trial_steps, char* chosen, int32_t chosen_len,
+                              double* ms_per_step) {
+  API_BEGIN
+  require(trial_steps > 0, SPRUCE_E_ARG, "trial_steps must be positive");
+  std::string c;
+  const double ms = bat(h).autotune(trial_steps, &c);
+  if (ms_per_step) *ms_per_step = ms;
+  if (chosen && chosen_len > 0) {
+    std::strncpy(chosen, c.c_str(), chosen_len - 1);
+    chosen[chosen_len - 1] = 0;
+  }
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_batch_step(spruce_gpu_batch* h, int32_t nsteps, double* dt_out) {
+  API_BEGIN
+  require(nsteps >= 0, SPRUCE_E_ARG, "nsteps must be non-negative");
+  if (nsteps == 0) return SPRUCE_OK;
+  TfBatchRun& b = bat(h);
+  std::vector<std::vector<double>> dts;
+  b.run(nsteps, b.use_graph, dt_out ? &dts : nullptr);
+  if (dt_out)
+    for (size_t k = 0; k < dts.size(); ++k)
+      for (int32_t s = 0; s < nsteps; ++s) dt_out[k * nsteps + s] = dts[k][s];
+  API_END(SPRUCE_OK)
+}
+
+void spruce_gpu_batch_destroy(spruce_gpu_batch* h) {
+  if (!h) return;
+  if (h->b && !h->b->broken)
+    for (auto* t : h->b->members) t->batch = nullptr;
+  delete h;
+}
 
 }  // extern "C"

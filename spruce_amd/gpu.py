@@ -72,6 +72,12 @@ def lib() -> ctypes.CDLL:
     L.spruce_gpu_pow_probe.argtypes = [DP, I64, I32, DP]
     L.spruce_gpu_last_error.restype = ctypes.c_char_p
     L.spruce_gpu_destroy.argtypes = [P]
+    L.spruce_gpu_batch_create.restype = P
+    L.spruce_gpu_batch_create.argtypes = [ctypes.POINTER(P), I32]
+    L.spruce_gpu_batch_set_option.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p]
+    L.spruce_gpu_batch_autotune.argtypes = [P, I32, ctypes.c_char_p, I32, DP]
+    L.spruce_gpu_batch_step.argtypes = [P, I32, DP]
+    L.spruce_gpu_batch_destroy.argtypes = [P]
     _LIB = L
     return L
 
@@ -202,6 +208,48 @@ class GpuDomain:
     def close(self) -> None:
         if getattr(self, "h", None):
             lib().spruce_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuBatch:
+    """Several set-up ideal_2F GpuDomains advanced together (one launch per kernel of the step for all
+    members, optionally replayed from a captured HIP graph).  Close it before its members."""
+
+    def __init__(self, domains: Sequence[GpuDomain], graph: bool = True, block: Optional[str] = None):
+        L = lib()
+        self.domains = list(domains)
+        arr = (ctypes.c_void_p * len(self.domains))(*[d.h for d in self.domains])
+        self.h = L.spruce_gpu_batch_create(arr, len(self.domains))
+        if not self.h:
+            raise SpruceError(-1, _err())
+        self.set_option("graph", "true" if graph else "false")
+        if block:
+            self.set_option("block", block)
+
+    def set_option(self, key: str, value) -> None:
+        _chk(lib().spruce_gpu_batch_set_option(self.h, key.encode(), str(value).encode()))
+
+    def autotune(self, trial_steps: int = 5) -> Tuple[str, float]:
+        buf = ctypes.create_string_buffer(64)
+        ms = ctypes.c_double()
+        _chk(lib().spruce_gpu_batch_autotune(self.h, trial_steps, buf, 64, ctypes.byref(ms)))
+        return buf.value.decode(), ms.value
+
+    def step(self, n: int = 1) -> np.ndarray:
+        """n steps of every member; returns the step sizes as [member, step]."""
+        dts = np.zeros(len(self.domains) * n)
+        _chk(lib().spruce_gpu_batch_step(self.h, n, _dp(dts)))
+        return dts.reshape(len(self.domains), n)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().spruce_gpu_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):
