@@ -1,6 +1,6 @@
 """Time-step throughput of the MI355X path: cell-updates/s and the HBM roofline of the stage kernel.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload arcade|periodic|arcade2048]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload arcade|periodic|periodic2048|arcade2048|batch2f]
 
 One "step" is one PlasmaDomain::advanceTime (evolution.cpp:59-82) over the whole grid, ghost cells
 included: every integrator stage, every module subcycle, the ghost fill and the dt reduction.
@@ -99,6 +99,85 @@ def cpu_baseline(st, cfg, steps: int, threads: int) -> dict:
                       "t(%d steps) - t(0 steps) = %.2f s" % (steps, steps, dt)}
 
 
+def tf_stage_bytes_per_cell(curl: bool) -> float:
+    """Two-fluid RK2: 14 evolved fields, statics grav_x/y (+ be_x/y with the Maxwell curl terms).
+    Stage 1 reads E + S and writes E, stage 2 reads 2E + S and writes E; average of the two."""
+    e, st = 14, 4 if curl else 2
+    return 8.0 * (5 * e + 2 * st) / 2
+
+
+def bench_batch(args) -> dict:
+    """64 two-fluid expansions of uneven sizes stepped as one batch (graph replay, autotuned block),
+    against the same domains stepped one after another through the single-domain entry point."""
+    from spruce_amd import workloads as W
+    from spruce_amd.gpu import GpuBatch, GpuDomain
+    from spruce_amd.spec import spec_from_files
+
+    members = W.batch_2f_members(64)
+    specs = [spec_from_files(c, s) for s, c in members]
+    cells = sum(s.xdim * s.ydim for s in specs)
+    doms = [GpuDomain(s) for s in specs]
+    b = GpuBatch(doms, graph=True)
+    shape, tune_ms = b.autotune(3)
+    b.step(args.warmup)
+    t0 = time.perf_counter()
+    b.step(args.steps)          # returns after a device synchronisation
+    wall = time.perf_counter() - t0
+    value = cells * args.steps / wall
+
+    b.set_profiling(True)
+    b.step(max(5, min(args.steps, 20)))
+    kt = b.kernel_times()
+    b.set_profiling(False)
+    b.close()
+
+    # sequential: each domain advanced alone (one launch sequence per domain and step)
+    seq_steps = max(5, min(args.steps, 100))
+    seq_wall = 0.0
+    for d in doms:
+        d.step(2)
+        t0 = time.perf_counter()
+        d.step(seq_steps)
+        seq_wall += time.perf_counter() - t0
+    seq_value = cells * seq_steps / seq_wall
+    for d in doms:
+        d.close()
+
+    stage_ms = kt.get("tf_stage", (float("nan"), 0))[0]
+    bpc = tf_stage_bytes_per_cell(False)
+    achieved = bpc * cells / (stage_ms * 1e-3) / 1e9
+    edges = {}
+    for s in specs:
+        edges[s.xdim] = edges.get(s.xdim, 0) + 1
+    out = {
+        "metric": "cell-updates/s and % HBM roofline (FP64); adjoint/forward cost ratio",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "64 two-fluid plasma expansions, uneven square grids (ideal_2F, RK2, open_ucnp, "
+                               "FP64), one batch, graph replay, autotuned block",
+                   "domains": len(specs), "cells": cells, "edges": edges, "block": shape},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "tf_stage",
+                     "kernel_ms": stage_ms, "stage_bytes_per_cell": bpc},
+        "sequential": {"value": seq_value, "steps": seq_steps, "batched_over_sequential": value / seq_value},
+        "autotune_ms_per_step": tune_ms,
+        "kernels_ms": {k: v[0] for k, v in kt.items()},
+    }
+    if not args.no_cpu:
+        threads = int(os.environ.get("SPRUCE_CPU_THREADS", str(min(16, os.cpu_count() or 1))))
+        k = next(i for i, s in enumerate(specs) if s.xdim == 256)
+        st, cfg = members[k]
+        n = args.cpu_steps or max(2, int(1.5e6 / (st.xdim * st.ydim)))
+        cb = cpu_baseline(st, cfg, n, threads)
+        if cb and "sample" in cb:
+            cb["sample"] = "member %d (256x256) only: " % k + cb["sample"]
+        out["cpu_baseline"] = cb
+        if cb and "value" in cb:
+            out["speedup_vs_cpu"] = value / cb["value"]
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +196,9 @@ def main() -> None:
 
     from spruce_amd.gpu import GpuDomain
 
+    if args.workload == "batch2f":
+        print(json.dumps(bench_batch(args)))
+        return
     st, cfg, spec, desc = make_workload(args.workload)
     cells = spec.xdim * spec.ydim
     opts = {"stage_kernel": args.stage_kernel}

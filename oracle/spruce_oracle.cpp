@@ -730,9 +730,178 @@ struct Ideal2F : EqSet {
   }
 };
 
+// Magnetoacoustic step-size grid shared by IdealMHD2E (idealmhd2E.cpp:115-137) and IdealMHDCons
+// (idealmhdcons.cpp:54-73); unlike IdealMHD they take |v| with abs rather than sqrt(v^2).
+static A mhd_dt_abs(const Domain& pd, const A& press, const A& rho, const A& bmag, const A& vx, const A& vy);
+
+// ---------------------------------------------------------------------------------------------
+// IdealMHD2E: one fluid, separate ion and electron energies (source/equationsets/idealmhd2E.{hpp,cpp})
+// ---------------------------------------------------------------------------------------------
+struct IdealMHD2E : EqSet {
+  enum V { rho, i_temp, e_temp, mom_x, mom_y, bi_x, bi_y, grav_x, grav_y, n, i_press, e_press, press,
+           i_thermal_energy, e_thermal_energy, v_x, v_y, kinetic_energy, b_x, b_y, b_mag, b_hat_x, b_hat_y, dt_ };
+  explicit IdealMHD2E(Domain& d)
+      : EqSet(d, {"rho", "i_temp", "e_temp", "mom_x", "mom_y", "bi_x", "bi_y", "grav_x", "grav_y", "n", "i_press",
+                  "e_press", "press", "i_thermal_energy", "e_thermal_energy", "v_x", "v_y", "kinetic_energy", "b_x",
+                  "b_y", "b_mag", "b_hat_x", "b_hat_y", "dt"}) {}
+  std::vector<int> state_vars() const override { return {rho, i_temp, e_temp, mom_x, mom_y, bi_x, bi_y, grav_x, grav_y}; }
+  std::vector<int> evolved() const override { return {rho, mom_x, mom_y, i_thermal_energy, e_thermal_energy, bi_x, bi_y}; }
+  // idealmhd2E.hpp:36-44: two species sharing one density and one momentum
+  std::vector<int> densities() const override { return {rho, rho}; }
+  std::vector<int> thermal_energies() const override { return {i_thermal_energy, e_thermal_energy}; }
+  std::vector<std::vector<int>> momenta() const override { return {{mom_x, mom_y}, {mom_x, mom_y}}; }
+  std::vector<int> fields() const override { return {bi_x, bi_y}; }
+  void parse(const std::vector<std::string>& k, const std::vector<std::string>& v) override {  // idealmhd2E.cpp:10-20
+    for (size_t i = 0; i < k.size(); ++i) {
+      if (k[i] == "global_viscosity") (void)std::stod(v[i]);
+      else if (k[i] == "viscosity_opt") {}
+      else throw fail(k[i] + " is not recognized for this equation set.");
+    }
+  }
+  Grids rhs(const Grids& G) const override {  // idealmhd2E.cpp:22-59
+    const Domain& D = pd;
+    const A &bex = D.pd[be_x], &bey = D.pd[be_y];
+    const A &vx = G[v_x], &vy = G[v_y];
+    A d_rho = -D.tdiv(G[rho], vx, vy);
+    A curl_db = D.curl2D(G[bi_x], G[bi_y]) / (4.0 * PI);
+    // CrossProductZ2D(a_z, {b_x, b_y}) = {-a_z * b_y, a_z * b_x} (grid.cpp:455-460)
+    std::vector<A> ext = {-curl_db * bey, curl_db * bex};
+    std::vector<A> in = {-curl_db * G[bi_y], curl_db * G[bi_x]};
+    A d_mx = -D.tdiv(G[mom_x], vx, vy) - D.deriv(G[press], 0) + D.mask * (G[rho] * G[grav_x] + ext[0] + in[0]);
+    A d_my = -D.tdiv(G[mom_y], vx, vy) - D.deriv(G[press], 1) + D.mask * (G[rho] * G[grav_y] + ext[1] + in[1]);
+    A d_ite = -D.tdiv(G[i_thermal_energy], vx, vy) - G[i_press] * D.div2D(vx, vy);
+    A d_ete = -D.tdiv(G[e_thermal_energy], vx, vy) - G[e_press] * D.div2D(vx, vy);
+    // curlZ(CrossProduct2D(v, b)) with CrossProduct2D(a, b) = a_x b_y - a_y b_x (grid.cpp:448-452)
+    std::vector<A> ind_ext = D.curlZ(vx * bey - vy * bex);
+    std::vector<A> ind_in = D.curlZ(vx * G[bi_y] - vy * G[bi_x]);
+    A d_bx = ind_ext[0] + ind_in[0];
+    A d_by = ind_ext[1] + ind_in[1];
+    Grids k = {d_rho, d_mx, d_my, d_ite, d_ete, d_bx, d_by};
+    for (auto& g : k) g *= D.mask;
+    return k;
+  }
+  void minimums(Grids& G) const override {  // idealmhd2E.cpp:61-67
+    G[rho] = pd.m_ion * vmax(G[rho] / pd.m_ion, pd.nmin);
+    G[i_thermal_energy] = vmax(G[i_thermal_energy], pd.temin);
+    G[e_thermal_energy] = vmax(G[e_thermal_energy], pd.temin);
+  }
+  void evolved_from_state(Grids& G) const override {  // idealmhd2E.cpp:69-77
+    G[n] = vmax(G[rho] / pd.m_ion, pd.nmin);
+    G[i_press] = G[n] * K_B * vmax(G[i_temp], pd.tmin);
+    G[e_press] = G[n] * K_B * vmax(G[e_temp], pd.tmin);
+    G[i_thermal_energy] = vmax(G[i_press] / (pd.gamma - 1.0), pd.temin);
+    G[e_thermal_energy] = vmax(G[e_press] / (pd.gamma - 1.0), pd.temin);
+  }
+  void derived(Grids& G) const override {  // idealmhd2E.cpp:79-113
+    G[n] = vmax(G[rho] / pd.m_ion, pd.nmin);
+    G[rho] = G[n] * pd.m_ion;
+    G[v_x] = G[mom_x] / G[rho];
+    G[v_y] = G[mom_y] / G[rho];
+    G[kinetic_energy] = 0.5 * G[rho] * (sq(G[v_x]) + sq(G[v_y]));
+    G[i_thermal_energy] = vmax(G[i_thermal_energy], pd.temin);
+    G[e_thermal_energy] = vmax(G[e_thermal_energy], pd.temin);
+    G[i_press] = (pd.gamma - 1.0) * G[i_thermal_energy];
+    G[e_press] = (pd.gamma - 1.0) * G[e_thermal_energy];
+    G[press] = G[i_press] + G[e_press];
+    G[i_temp] = vmax(G[i_press] / (K_B * G[n]), pd.tmin);
+    G[e_temp] = vmax(G[e_press] / (K_B * G[n]), pd.tmin);
+    G[b_x] = pd.pd[be_x] + G[bi_x];
+    G[b_y] = pd.pd[be_y] + G[bi_y];
+    G[b_mag] = vsqrt(sq(G[b_x]) + sq(G[b_y]));
+    G[b_hat_x] = G[b_x] / G[b_mag];
+    G[b_hat_y] = G[b_y] / G[b_mag];
+    for (size_t c = 0; c < G[b_mag].size(); ++c)
+      if (G[b_mag].v[c] == 0.0) { G[b_hat_x].v[c] = 0.0; G[b_hat_y].v[c] = 0.0; }
+  }
+  void dt(Grids& G) const override { G[dt_] = mhd_dt_abs(pd, G[press], G[rho], G[b_mag], G[v_x], G[v_y]); }
+};
+
+// ---------------------------------------------------------------------------------------------
+// IdealMHDCons: conservative total-energy form (source/equationsets/idealmhdcons.{hpp,cpp}).
+// press_tot is read by the RHS but never computed (idealmhdcons.cpp:24-37 vs 96-128): it keeps
+// the values of the .state file, and without them the reference asserts at setup
+// (equationset.cpp:89-90).
+// ---------------------------------------------------------------------------------------------
+struct IdealMHDCons : EqSet {
+  enum V { rho, temp, mom_x, mom_y, bi_x, bi_y, grav_x, grav_y, n, v_x, v_y, kinetic_energy, b_x, b_y, b_mag,
+           mag_energy, b_hat_x, b_hat_y, thermal_energy, press, press_tot, energy, dt_ };
+  explicit IdealMHDCons(Domain& d)
+      : EqSet(d, {"rho", "temp", "mom_x", "mom_y", "bi_x", "bi_y", "grav_x", "grav_y", "n", "v_x", "v_y",
+                  "kinetic_energy", "b_x", "b_y", "b_mag", "mag_energy", "b_hat_x", "b_hat_y", "thermal_energy",
+                  "press", "press_tot", "energy", "dt"}) {}
+  std::vector<int> state_vars() const override { return {rho, temp, mom_x, mom_y, bi_x, bi_y, grav_x, grav_y}; }
+  std::vector<int> evolved() const override { return {rho, mom_x, mom_y, energy, bi_x, bi_y}; }
+  std::vector<int> densities() const override { return {rho}; }
+  std::vector<int> thermal_energies() const override { return {thermal_energy}; }
+  std::vector<std::vector<int>> momenta() const override { return {{mom_x, mom_y}}; }
+  std::vector<int> fields() const override { return {bi_x, bi_y}; }
+  void parse(const std::vector<std::string>&, const std::vector<std::string>&) override {}  // idealmhdcons.cpp:10-11
+  // idealmhdcons.cpp:13-52 builds the derivative list {rho, mom_x, mom_y, bi_x, bi_y} (:48) for
+  // six evolved variables and then indexes it up to 5 (:49): the energy derivative is never
+  // returned and the reference aborts (or reads past the vector) on the first time derivative.
+  // The restatement stops there too.
+  Grids rhs(const Grids&) const override {
+    throw fail("ideal_mhd_cons: the reference time derivative returns 5 grids for 6 evolved variables "
+               "(idealmhdcons.cpp:48-49) and aborts; there is no step to restate");
+  }
+  void minimums(Grids& G) const override {  // idealmhdcons.cpp:89-94
+    G[rho] = pd.m_ion * vmax(G[rho] / pd.m_ion, pd.nmin);
+    G[thermal_energy] = vmax(G[thermal_energy], pd.temin);
+  }
+  void evolved_from_state(Grids& G) const override {  // idealmhdcons.cpp:96-109
+    G[n] = vmax(G[rho] / pd.m_ion, pd.nmin);
+    G[press] = 2 * G[n] * K_B * vmax(G[temp], pd.tmin);
+    G[thermal_energy] = G[press] / (pd.gamma - 1.0);
+    G[v_x] = G[mom_x] / G[rho];
+    G[v_y] = G[mom_y] / G[rho];
+    G[kinetic_energy] = 0.5 * G[rho] * (sq(G[v_x]) + sq(G[v_y]));
+    G[b_x] = pd.pd[be_x] + G[bi_x];
+    G[b_y] = pd.pd[be_y] + G[bi_y];
+    G[b_mag] = vsqrt(sq(G[b_x]) + sq(G[b_y]));
+    G[mag_energy] = sq(G[b_mag]) / (8. * PI);
+    G[energy] = G[thermal_energy] + G[kinetic_energy] + G[mag_energy];
+  }
+  void derived(Grids& G) const override {  // idealmhdcons.cpp:111-128
+    G[n] = vmax(G[rho] / pd.m_ion, pd.nmin);
+    G[rho] = G[n] * pd.m_ion;
+    G[v_x] = G[mom_x] / G[rho];
+    G[v_y] = G[mom_y] / G[rho];
+    G[kinetic_energy] = 0.5 * G[rho] * (sq(G[v_x]) + sq(G[v_y]));
+    G[b_x] = pd.pd[be_x] + G[bi_x];
+    G[b_y] = pd.pd[be_y] + G[bi_y];
+    G[b_mag] = vsqrt(sq(G[b_x]) + sq(G[b_y]));
+    G[mag_energy] = sq(G[b_mag]) / (8. * PI);
+    G[b_hat_x] = G[b_x] / G[b_mag];
+    G[b_hat_y] = G[b_y] / G[b_mag];
+    // the floor applies to the kinetic energy term only (method call binds tighter than '-')
+    G[thermal_energy] = G[energy] - G[mag_energy] - vmax(G[kinetic_energy], pd.temin);
+    G[press] = (pd.gamma - 1.0) * G[thermal_energy];
+    G[temp] = vmax(G[press] / (2 * K_B * G[n]), pd.tmin);
+    for (size_t c = 0; c < G[b_mag].size(); ++c)
+      if (G[b_mag].v[c] == 0.0) { G[b_hat_x].v[c] = 0.0; G[b_hat_y].v[c] = 0.0; }
+  }
+  void dt(Grids& G) const override { G[dt_] = mhd_dt_abs(pd, G[press], G[rho], G[b_mag], G[v_x], G[v_y]); }
+};
+
+static A mhd_dt_abs(const Domain& pd, const A& press, const A& rho, const A& bmag, const A& vx, const A& vy) {
+  A c_s = vsqrt(pd.gamma * press / rho);
+  A c_s_sq = sq(c_s);
+  A v_a = bmag / vsqrt(4.0 * PI * rho);
+  A v_a_sq = sq(v_a);
+  A one(pd.nx, pd.ny, 1.0);
+  A delta = vsqrt(one - 4.0 * c_s_sq * v_a_sq / sq(c_s_sq + v_a_sq));
+  A v_fast = vsqrt(0.5 * (c_s_sq + v_a_sq) * (one + delta));
+  A v_slow = vsqrt(0.5 * (c_s_sq + v_a_sq) * (one - delta));
+  A vxm = vabs(vx) + vmax(vmax(vmax(c_s, v_a), v_fast), v_slow);
+  A vym = vabs(vy) + vmax(vmax(vmax(c_s, v_a), v_fast), v_slow);
+  return 1. / (vxm / pd.pd[d_x] + vym / pd.pd[d_y]);
+}
+
 static std::unique_ptr<EqSet> make_eqset(Domain& d, const std::string& name) {
   if (name == "ideal_mhd") return std::make_unique<IdealMHD>(d);
   if (name == "ideal_2F") return std::make_unique<Ideal2F>(d);
+  if (name == "ideal_mhd_2E") return std::make_unique<IdealMHD2E>(d);
+  if (name == "ideal_mhd_cons") return std::make_unique<IdealMHDCons>(d);
   throw fail("equation set <" + name + "> is not available in the restatement");
 }
 
@@ -806,6 +975,9 @@ int oracle_setup(void* h) {
     require(d->eqs->g[v].size() > 1, "state variable " + d->eqs->names[v] + " not set");
   d->iteration_bounds();
   d->eqs->populate(d->eqs->g);
+  for (size_t i = 0; i < d->eqs->g.size(); ++i)  // equationset.cpp:89-90
+    require(d->eqs->g[i].size() == (size_t)d->nx * d->ny,
+            "All variables must be initialized by <populateVariablesFromState> (" + d->eqs->names[i] + ")");
   d->eqs->setup_derived();
   for (auto& m : d->mods) m->setup();
   return 0;

@@ -829,23 +829,7 @@ int spruce_gpu_set_profiling(spruce_gpu_domain* h, int32_t enable) {
 int spruce_gpu_kernel_times(spruce_gpu_domain* h, char* names, int32_t names_len, double* ms_avg, int32_t* launches,
                             int32_t max_kernels) {
   API_BEGIN
-  DomainBase& d = dom(h);
-  std::string all;
-  int k = 0;
-  for (auto& kv : d.clock.acc) {
-    if (k < max_kernels) {
-      if (ms_avg) ms_avg[k] = kv.second.first / std::max(1, kv.second.second);
-      if (launches) launches[k] = kv.second.second;
-      if (!all.empty()) all += ",";
-      all += kv.first;
-    }
-    ++k;
-  }
-  if (names && names_len > 0) {
-    std::strncpy(names, all.c_str(), names_len - 1);
-    names[names_len - 1] = 0;
-  }
-  return k;
+  return dom(h).clock.report(names, names_len, ms_avg, launches, max_kernels);
   API_END(SPRUCE_OK)
 }
 
@@ -948,6 +932,21 @@ int spruce_gpu_batch_step(spruce_gpu_batch* h, int32_t nsteps, double* dt_out) {
   if (dt_out)
     for (size_t k = 0; k < dts.size(); ++k)
       for (int32_t s = 0; s < nsteps; ++s) dt_out[k * nsteps + s] = dts[k][s];
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_batch_set_profiling(spruce_gpu_batch* h, int32_t enable) {
+  API_BEGIN
+  TfBatchRun& b = bat(h);
+  b.clock.enabled = enable != 0;
+  b.clock.acc.clear();
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_batch_kernel_times(spruce_gpu_batch* h, char* names, int32_t names_len, double* ms_avg,
+                                  int32_t* launches, int32_t max_kernels) {
+  API_BEGIN
+  return bat(h).clock.report(names, names_len, ms_avg, launches, max_kernels);
   API_END(SPRUCE_OK)
 }
 

@@ -107,6 +107,34 @@ struct KernelClock {
     for (auto& p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
     for (auto e : pool) (void)hipEventDestroy(e);
   }
+  // Brackets the launches of f on stream s with events when profiling is on.
+  template <class F> void timed(hipStream_t s, const char* name, F&& f) {
+    if (!enabled) { f(); return; }
+    hipEvent_t a = get(), b = get();
+    hip_check(hipEventRecord(a, s), "hipEventRecord");
+    f();
+    hip_check(hipEventRecord(b, s), "hipEventRecord");
+    pending.push_back({name, a, b});
+  }
+  // Writes the per-kernel averages in the layout of spruce_gpu_kernel_times; returns the count.
+  int report(char* names, int names_len, double* ms_avg, int32_t* launches, int max_kernels) const {
+    std::string all;
+    int k = 0;
+    for (auto& kv : acc) {
+      if (k < max_kernels) {
+        if (ms_avg) ms_avg[k] = kv.second.first / std::max(1, kv.second.second);
+        if (launches) launches[k] = kv.second.second;
+        if (!all.empty()) all += ",";
+        all += kv.first;
+      }
+      ++k;
+    }
+    if (names && names_len > 0) {
+      all.copy(names, names_len - 1);
+      names[std::min<size_t>(all.size(), names_len - 1)] = 0;
+    }
+    return k;
+  }
 };
 
 // The PlasmaDomain role behind the C ABI: one implementation per equation-set family.
@@ -129,15 +157,7 @@ class DomainBase {
   hipStream_t stream = nullptr;
   KernelClock clock;
 
-  // Brackets the launches of f with events when profiling is on.
-  template <class F> void timed(const char* name, F&& f) {
-    if (!clock.enabled) { f(); return; }
-    hipEvent_t a = clock.get(), b = clock.get();
-    hip_check(hipEventRecord(a, stream), "hipEventRecord");
-    f();
-    hip_check(hipEventRecord(b, stream), "hipEventRecord");
-    clock.pending.push_back({name, a, b});
-  }
+  template <class F> void timed(const char* name, F&& f) { clock.timed(stream, name, f); }
 };
 
 class Domain;

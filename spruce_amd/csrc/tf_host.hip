@@ -320,27 +320,33 @@ void TfBatchRun::rebuild(int bx, int by) {
 // HIP graph can capture it.
 void TfBatchRun::enqueue_step(hipStream_t s) {
   const int ucnp = 1 << SPRUCE_BC_OPEN_UCNP;
+  auto stage = [&](int S, int B, int T, double mult, int mode, int kslot) {
+    clock.timed(s, "tf_stage", [&] { launch_tf_stage(b, S, B, T, mult, mode, kslot, s); });
+  };
+  auto ghost = [&](int G, int kinds) {
+    clock.timed(s, "tf_ghost", [&] { launch_tf_ghost(b, G, kinds, maxn, s); });
+  };
   if (integrator == SPRUCE_EULER) {
-    launch_tf_stage(b, TF_SEL_L, TF_SEL_L, TF_SEL_W, 1.0, TF_STAGE_PLAIN, 0, s);
-    launch_tf_copy(b, TF_SEL_W, TF_SEL_L, s);
+    stage(TF_SEL_L, TF_SEL_L, TF_SEL_W, 1.0, TF_STAGE_PLAIN, 0);
+    clock.timed(s, "tf_copy", [&] { launch_tf_copy(b, TF_SEL_W, TF_SEL_L, s); });
   } else if (integrator == SPRUCE_RK2) {
-    launch_tf_stage(b, TF_SEL_L, TF_SEL_L, TF_SEL_W, 0.5, TF_STAGE_PLAIN, 0, s);
-    if (any_ucnp) launch_tf_ghost(b, TF_SEL_W, ucnp, maxn, s);
-    launch_tf_stage(b, TF_SEL_W, TF_SEL_L, TF_SEL_L, 1.0, TF_STAGE_PLAIN, 0, s);
+    stage(TF_SEL_L, TF_SEL_L, TF_SEL_W, 0.5, TF_STAGE_PLAIN, 0);
+    if (any_ucnp) ghost(TF_SEL_W, ucnp);
+    stage(TF_SEL_W, TF_SEL_L, TF_SEL_L, 1.0, TF_STAGE_PLAIN, 0);
   } else {
     // The live-grid ghost writes of the intermediate propagates (evolution.cpp:168-281) rewrite
     // the values the previous final propagate left there (the two-fluid derived step changes no
     // evolved variable), so only open_ucnp, which writes the stage grids, acts here.
-    launch_tf_stage(b, TF_SEL_L, TF_SEL_L, TF_SEL_W, 0.5, TF_STAGE_STORE_K, 0, s);
-    if (any_ucnp) launch_tf_ghost(b, TF_SEL_W, ucnp, maxn, s);
-    launch_tf_stage(b, TF_SEL_W, TF_SEL_L, TF_SEL_W2, 0.5, TF_STAGE_STORE_K, 1, s);
-    if (any_ucnp) launch_tf_ghost(b, TF_SEL_W2, ucnp, maxn, s);
-    launch_tf_stage(b, TF_SEL_W2, TF_SEL_L, TF_SEL_W, 1.0, TF_STAGE_STORE_K, 2, s);
-    if (any_ucnp) launch_tf_ghost(b, TF_SEL_W, ucnp, maxn, s);
-    launch_tf_stage(b, TF_SEL_W, TF_SEL_L, TF_SEL_L, 1.0, TF_STAGE_RK4_FINAL, 0, s);
+    stage(TF_SEL_L, TF_SEL_L, TF_SEL_W, 0.5, TF_STAGE_STORE_K, 0);
+    if (any_ucnp) ghost(TF_SEL_W, ucnp);
+    stage(TF_SEL_W, TF_SEL_L, TF_SEL_W2, 0.5, TF_STAGE_STORE_K, 1);
+    if (any_ucnp) ghost(TF_SEL_W2, ucnp);
+    stage(TF_SEL_W2, TF_SEL_L, TF_SEL_W, 1.0, TF_STAGE_STORE_K, 2);
+    if (any_ucnp) ghost(TF_SEL_W, ucnp);
+    stage(TF_SEL_W, TF_SEL_L, TF_SEL_L, 1.0, TF_STAGE_RK4_FINAL, 0);
   }
-  if (any_ucnp || any_other) launch_tf_ghost(b, TF_SEL_L, 0x3F, maxn, s);
-  launch_tf_dt(b, s);
+  if (any_ucnp || any_other) ghost(TF_SEL_L, 0x3F);
+  clock.timed(s, "tf_dt", [&] { launch_tf_dt(b, s); });
 }
 
 void TfBatchRun::run(int nsteps, bool use_graph, std::vector<std::vector<double>>* dts) {
@@ -357,7 +363,7 @@ void TfBatchRun::run(int nsteps, bool use_graph, std::vector<std::vector<double>
   }
   if (realloc) rebuild(b.bx, b.by);
   launch_tf_reset_counters(b, s);
-  if (use_graph) {
+  if (use_graph && !clock.enabled) {
     if (!gexec) {
       hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeGlobal), "hipStreamBeginCapture");
       enqueue_step(s);
@@ -369,6 +375,7 @@ void TfBatchRun::run(int nsteps, bool use_graph, std::vector<std::vector<double>
     for (int k = 0; k < nsteps; ++k) enqueue_step(s);
   }
   hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (clock.enabled) clock.collect();
   if (dts) dts->clear();
   for (auto* d : members) {
     std::vector<double> h(nsteps + 1);

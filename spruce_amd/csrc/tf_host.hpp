@@ -62,6 +62,7 @@ class TfBatchRun {
   bool use_graph = true;              // replay a captured step (the batch API default)
   bool broken = false;                // a member was destroyed while in the batch
   std::vector<TfDom> uploaded;        // host copy of d_doms, to notice reallocated member buffers
+  KernelClock clock;                  // per-kernel event timing (profiling runs skip the graph)
   TfDom* d_doms = nullptr;
   int* d_tile_dom = nullptr;
   hipGraphExec_t gexec = nullptr;

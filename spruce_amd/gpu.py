@@ -78,6 +78,8 @@ def lib() -> ctypes.CDLL:
     L.spruce_gpu_batch_autotune.argtypes = [P, I32, ctypes.c_char_p, I32, DP]
     L.spruce_gpu_batch_step.argtypes = [P, I32, DP]
     L.spruce_gpu_batch_destroy.argtypes = [P]
+    L.spruce_gpu_batch_set_profiling.argtypes = [P, I32]
+    L.spruce_gpu_batch_kernel_times.argtypes = [P, ctypes.c_char_p, I32, DP, ctypes.POINTER(I32), I32]
     _LIB = L
     return L
 
@@ -246,6 +248,17 @@ class GpuBatch:
         dts = np.zeros(len(self.domains) * n)
         _chk(lib().spruce_gpu_batch_step(self.h, n, _dp(dts)))
         return dts.reshape(len(self.domains), n)
+
+    def set_profiling(self, on: bool) -> None:
+        _chk(lib().spruce_gpu_batch_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_times(self) -> Dict[str, Tuple[float, int]]:
+        names = ctypes.create_string_buffer(4096)
+        ms = (ctypes.c_double * 64)()
+        cnt = (ctypes.c_int32 * 64)()
+        n = _chk(lib().spruce_gpu_batch_kernel_times(self.h, names, 4096, ms, cnt, 64))
+        keys = names.value.decode().split(",") if n else []
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(keys)}
 
     def close(self) -> None:
         if getattr(self, "h", None):

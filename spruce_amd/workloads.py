@@ -227,3 +227,58 @@ def ucnp_config(steps: int = 5000, integrator: str = "rk2", remove_curl_terms: b
                        temp_min=1e-3, thermal_energy_min=1e-30,
                        equation_set_options=[("use_sub_cycling", "false"),
                                              ("remove_curl_terms", "true" if remove_curl_terms else "false")], **kw)
+
+
+BATCH_EDGES = (128, 192, 256, 384, 512)
+
+
+def batch_2f_members(n: int = 64, seed: int = 7, integrator: str = "rk2", steps: int = 5000):
+    """Independent two-fluid plasma expansions of uneven sizes (the batched-mode workload).
+
+    Member k: an edge drawn from BATCH_EDGES, Gaussian peak density 1e9-1e10 cm^-3, width
+    0.05-0.15 cm, electron temperature 50-200 K, ion temperature 1 K, zero fields and momenta, all
+    drawn from a seeded stream.  Returns [(StateFile, ConfigFile)].
+    """
+    u = uniform(seed, (n, 4))
+    out = []
+    for k in range(n):
+        e = BATCH_EDGES[min(len(BATCH_EDGES) - 1, int(u[k, 0] * len(BATCH_EDGES)))]
+        st = ucnp_state(e, e, n_peak=1e9 + 9e9 * u[k, 1], sigma=0.05 + 0.1 * u[k, 2], te=50.0 + 150.0 * u[k, 3],
+                        ti=1.0)
+        out.append((st, ucnp_config(steps=steps, integrator=integrator)))
+    return out
+
+
+def mhd_variant_state(equation_set: str, nx: int, ny: int, seed: int, stretch: bool = False,
+                      zero_field: bool = False) -> StateFile:
+    """Perturbed uniform plasma in the state variables of ideal_mhd_2E (idealmhd2E.hpp:27-29) or
+    ideal_mhd_cons (idealmhdcons.hpp:27-29); every evolved field carries seeded noise.
+
+    ideal_mhd_cons reads a press_tot grid it never computes, so the state supplies it: the thermal
+    plus magnetic pressure of the initial state.
+    """
+    u = [uniform(seed * 16 + k, (nx, ny)) for k in range(10)]
+    dx = 1e7 * (1.01 ** np.arange(nx)) if stretch else np.full(nx, 1e7)
+    dy = 8e6 * (1.02 ** np.arange(ny)) if stretch else np.full(ny, 8e6)
+    rho = 1e-15 * (1.0 + 0.3 * (2 * u[0] - 1))
+    t_i = 1e6 * (1.0 + 0.3 * (2 * u[1] - 1))
+    t_e = 1.5e6 * (1.0 + 0.3 * (2 * u[9] - 1))
+    mom_x = rho * 1e7 * (2 * u[2] - 1)
+    mom_y = rho * 1e7 * (2 * u[3] - 1)
+    z = np.zeros((nx, ny))
+    bi_x = z if zero_field else 1.0 * (2 * u[5] - 1)
+    bi_y = z if zero_field else 1.0 * (2 * u[6] - 1)
+    be_x = z if zero_field else 2.0 + 0.5 * (2 * u[7] - 1)
+    be_y = z if zero_field else 1.0 + 0.5 * (2 * u[8] - 1)
+    grids = dict(_pd_grids(dx, dy))
+    grids.update({"be_x": be_x, "be_y": be_y, "be_z": z, "rho": rho, "mom_x": mom_x, "mom_y": mom_y,
+                  "bi_x": bi_x, "bi_y": bi_y, "grav_x": z, "grav_y": np.full((nx, ny), -2.74e4)})
+    if equation_set == "ideal_mhd_2E":
+        grids.update({"i_temp": t_i, "e_temp": t_e})
+    elif equation_set == "ideal_mhd_cons":
+        n = rho / M_PROTON
+        b2 = (be_x + bi_x) ** 2 + (be_y + bi_y) ** 2
+        grids.update({"temp": t_i, "press_tot": 2 * n * K_B * t_i + b2 / (8 * np.pi)})
+    else:
+        raise ValueError(equation_set)
+    return StateFile(nx, ny, M_PROTON, GAMMA, 0.0, grids, ["#random %s case, seed %d" % (equation_set, seed)])

@@ -149,6 +149,58 @@ def _cases() -> Dict[str, Tuple[Callable[[], StateFile], Callable[[], ConfigFile
     c["tf_open_euler"] = (lambda: W.ucnp_state(22, 18, seed=33, v_noise=3e5, e_noise=1e-3),
                           lambda: W.ucnp_config(steps=6, integrator="euler", bc=("open", "reflect", "fixed", "open"),
                                                 open_boundary_strength=0.2, output_flags=F2))
+    # artificial viscosity (viscosity.cpp): RHS terms (strength <= 1) and hyperviscous subcycling
+    def _visc(opt, strength, diff, evol, length, species, **extra):
+        o = [("visc_opt", opt), ("visc_strength", strength), ("visc_vars_to_diff", diff),
+             ("visc_vars_to_evol", evol), ("visc_length", length), ("visc_species", species)]
+        o += [(k, v) for k, v in extra.items()]
+        return ("artificial_viscosity", o)
+    c["visc_local_global_rk2"] = (lambda: random_mhd_state(22, 18, 51),
+                                  lambda: _mhd(P4, "rk2", 5, modules=[_visc("local,global", "0.1,0.05", "v_x,temp",
+                                                                            "mom_x,thermal_energy", "0,0", "i,i")]))
+    c["visc_boundary_gc_rk4"] = (lambda: random_mhd_state(20, 24, 52, stretch=True),
+                                 lambda: _mhd(("open",) * 4, "rk4", 4, open_boundary_strength=0.3,
+                                              modules=[_visc("boundary,boundary_global", "0.3,0.2", "rho,v_y",
+                                                             "rho,mom_y", "5e7,8e7", "i,i",
+                                                             gradient_correction="true")]))
+    c["visc_hyper_rk2"] = (lambda: random_mhd_state(18, 20, 53),
+                           lambda: _mhd(("fixed", "reflect", "periodic", "periodic"), "rk2", 4,
+                                        modules=[_visc("local,local", "2.5,0.2", "v_y,bi_x", "mom_y,bi_x", "0,0",
+                                                       "i,i", hv_time_integrator="rk2", hv_epsilon="1.0")]))
+    c["visc_hyper_ellip_euler"] = (lambda: random_mhd_state(20, 20, 54),
+                                   lambda: _mhd(("open", "open", "fixed", "open"), "euler", 4,
+                                                open_boundary_strength=0.2,
+                                                modules=[_visc("boundary", "1.7", "temp", "thermal_energy", "4e7", "i",
+                                                               hv_time_integrator="rk4", hv_epsilon="0.5",
+                                                               boundary_falloff_shape="gaussian_elliptical")]))
+    c["visc_exp_tc_rk2"] = (lambda: mini_arcade(18, 26, 55),
+                            lambda: _solar(SOLAR_BC, "rk2", 3, [_tc(wf="30"),
+                                                               _visc("boundary_global,local", "0.4,0.1", "v_x,v_z",
+                                                                     "mom_x,mom_z", "3e7,0", "i,i",
+                                                                     boundary_falloff_shape="exp")]))
+    # single-fluid variants: two energies, and the conservative total-energy form
+    E2 = ["rho", "mom_x", "mom_y", "i_thermal_energy", "e_thermal_energy", "bi_x", "bi_y", "i_temp", "e_temp",
+          "press", "b_hat_x", "dt"]
+    CO = ["rho", "mom_x", "mom_y", "energy", "bi_x", "bi_y", "thermal_energy", "press", "temp", "mag_energy",
+          "b_hat_y", "dt"]
+
+    def _var(eq, bc, integ, steps, outs, **kw):
+        return W.base_config(eq, bc, integ, epsilon=0.2, max_iterations=steps, density_min=1.0, temp_min=1.0,
+                             thermal_energy_min=1e-12, output_flags=outs, **kw)
+    c["e2_periodic_rk2"] = (lambda: W.mhd_variant_state("ideal_mhd_2E", 20, 26, 41),
+                            lambda: _var("ideal_mhd_2E", P4, "rk2", 6, E2))
+    c["e2_open_rk4"] = (lambda: W.mhd_variant_state("ideal_mhd_2E", 22, 18, 42, stretch=True),
+                        lambda: _var("ideal_mhd_2E", ("open", "reflect", "fixed", "open"), "rk4", 5, E2,
+                                     open_boundary_strength=0.4, open_boundary_decay_base=0.7))
+    c["e2_ucnp_euler"] = (lambda: W.mhd_variant_state("ideal_mhd_2E", 18, 24, 43),
+                          lambda: _var("ideal_mhd_2E", ("open_ucnp",) * 4, "euler", 6, E2))
+    # ideal_mhd_cons aborts at its first time derivative in the reference (idealmhdcons.cpp:48-49),
+    # so its cases hold the set-up state only (max_iterations = 0)
+    c["cons_periodic_setup"] = (lambda: W.mhd_variant_state("ideal_mhd_cons", 24, 20, 44),
+                                lambda: _var("ideal_mhd_cons", P4, "rk2", 0, CO))
+    c["cons_open_setup"] = (lambda: W.mhd_variant_state("ideal_mhd_cons", 18, 22, 45, stretch=True),
+                            lambda: _var("ideal_mhd_cons", ("fixed", "open", "reflect", "open_ucnp"), "rk4", 0, CO,
+                                         open_boundary_strength=0.3, open_boundary_decay_base=0.6))
     return c
 
 
