@@ -21,6 +21,7 @@
 #include "mhd.hpp"
 #include "host_common.hpp"
 #include "solar.hpp"
+#include "visc.hpp"
 #include "crpow.hpp"
 #include "tf_host.hpp"
 
@@ -66,6 +67,16 @@ struct GpuModule {
   virtual void iterate(Domain& d, double dt) = 0;
   virtual bool needs_bhat() const { return false; }
   virtual bool needs_dt_grid() const { return false; }
+  // false when pre_launch/pre_finish do nothing (no subcycle-count reduction to read back)
+  virtual bool needs_pre_scalars() const { return true; }
+  // iterate() has work to do (needs the step size on the host)
+  virtual bool iterates() const { return true; }
+  // after every module's iterate, before the integrator (the live grids are final for this step)
+  virtual void before_integrator(Domain&) {}
+  // RHS terms of this module on grid set S (computeTimeDerivativesModule), appended to a stage
+  virtual void rhs_terms(Domain&, SetBufs&, StageArgs&) {}
+  // the same terms added to a full RHS array (nevolved consecutive grids)
+  virtual void add_rhs(Domain&, SetBufs&, double*) {}
 };
 
 static double opt_d(const std::vector<std::pair<std::string, std::string>>& o, const std::string& k, bool& found) {
@@ -323,17 +334,24 @@ class Domain : public DomainBase {
     bool bh = false;
     for (auto& m : mods) bh = bh || m->needs_bhat();
     if (bh) timed("bhat", [&] { launch_bhat(L.view(), stat(), mesh, bhx.p, bhy.p, stream); });
-    for (size_t k = 0; k < mods.size(); ++k) mods[k]->pre_launch(*this, mod_scal.p + 2 * k);
-    std::vector<double> sc(2 * mods.size() + 1);
-    hip_check(hipMemcpyAsync(sc.data(), mod_scal.p, 2 * mods.size() * sizeof(double), hipMemcpyDeviceToHost, stream),
-              "read module scalars");
-    hip_check(hipMemcpyAsync(sc.data() + 2 * mods.size(), step.p, sizeof(double), hipMemcpyDeviceToHost, stream),
-              "read step");
-    sync_and_check();
-    const double dt = sc[2 * mods.size()];
-    for (size_t k = 0; k < mods.size(); ++k) mods[k]->pre_finish(dt, sc.data() + 2 * k);
-    for (size_t k = 0; k < mods.size(); ++k) sub_hist[(size_t)s * mods.size() + k] = mods[k]->nsub;
-    for (auto& m : mods) m->iterate(*this, dt);
+    bool any_iterate = false;
+    for (auto& m : mods) any_iterate = any_iterate || m->needs_pre_scalars() || m->iterates();
+    if (any_iterate) {
+      for (size_t k = 0; k < mods.size(); ++k) mods[k]->pre_launch(*this, mod_scal.p + 2 * k);
+      std::vector<double> sc(2 * mods.size() + 1);
+      hip_check(hipMemcpyAsync(sc.data(), mod_scal.p, 2 * mods.size() * sizeof(double), hipMemcpyDeviceToHost,
+                               stream), "read module scalars");
+      hip_check(hipMemcpyAsync(sc.data() + 2 * mods.size(), step.p, sizeof(double), hipMemcpyDeviceToHost, stream),
+                "read step");
+      sync_and_check();
+      const double dt = sc[2 * mods.size()];
+      for (size_t k = 0; k < mods.size(); ++k) mods[k]->pre_finish(dt, sc.data() + 2 * k);
+      for (size_t k = 0; k < mods.size(); ++k) sub_hist[(size_t)s * mods.size() + k] = mods[k]->nsub;
+      for (auto& m : mods) m->iterate(*this, dt);
+    } else {
+      for (size_t k = 0; k < mods.size(); ++k) sub_hist[(size_t)s * mods.size() + k] = mods[k]->nsub;
+    }
+    for (auto& m : mods) m->before_integrator(*this);
   }
 
   MhdStatic stat() { return MhdStatic{bex.p, bey.p, bez.p, gx.p, gy.p}; }
@@ -347,6 +365,7 @@ class Domain : public DomainBase {
     a.write_n = 1;
     if (mode == STAGE_STORE_K) a.kbuf = kbuf[kslot].p;
     if (mode == STAGE_RK4_FINAL) { a.kbuf = kbuf[0].p; a.k2buf = kbuf[1].p; a.k3buf = kbuf[2].p; }
+    for (auto& m : mods) m->rhs_terms(*this, S, a);
     if (stage_kernel == 1) timed("mhd_stage", [&] { launch_mhd_stage(a, stream); });
     else timed("mhd_stage", [&] { launch_mhd_stage_strip(a, strip_rows, stream); });
   }
@@ -473,6 +492,8 @@ class Domain : public DomainBase {
     DBuf tmp;
     tmp.alloc(8 * N);
     launch_mhd_rhs_full(L.view(), stat(), kc, mesh, tmp.p, stream);
+    for (auto& m : mods) m->before_integrator(*this);
+    for (auto& m : mods) m->add_rhs(*this, L, tmp.p);
     hip_check(hipMemcpyAsync(out, tmp.p, 8 * N * 8, hipMemcpyDeviceToHost, stream), "download rhs");
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
   }
@@ -634,6 +655,277 @@ struct RlModule : GpuModule {
   }
 };
 
+// ---------------------------------------------------------------------------------------------
+// Artificial viscosity (source/modules/viscosity.cpp), module name "artificial_viscosity"
+// ---------------------------------------------------------------------------------------------
+static std::string no_space(std::string v) {  // clearWhitespace (utils.cpp)
+  v.erase(std::remove_if(v.begin(), v.end(), [](unsigned char ch) { return std::isspace(ch); }), v.end());
+  return v;
+}
+static std::vector<std::string> split_list(const std::string& v) {  // splitString(.., ',')
+  std::vector<std::string> out;
+  size_t a = 0;
+  if (v.empty()) return out;
+  while (true) {
+    const size_t b = v.find(',', a);
+    out.push_back(v.substr(a, b == std::string::npos ? std::string::npos : b - a));
+    if (b == std::string::npos) break;
+    a = b + 1;
+  }
+  if (!out.empty() && out.back().empty()) out.pop_back();  // getline-based splitting drops a trailing empty field
+  return out;
+}
+
+struct ViscModule : GpuModule {
+  struct Term {
+    std::string opt, diff, evol, species;
+    double strength = 0.0, length = 0.0;
+    int var = 0, evol_idx = 0, scale = 0;
+    double scale_c = 1.0;
+    std::vector<double> sgrid_host;
+    DBuf sgrid, out;
+    bool hyper() const { return strength > 1.0; }
+  };
+  std::string in_opt, in_strength, in_diff, in_evol, in_length, in_species, shape, hv_integ;
+  bool grad = false;
+  double hv_eps = 1.0;  // viscosity.hpp:68
+  std::vector<Term> terms;
+  DBuf dtg, dtmin, part, init, d1, d2, d3, d4, dq;
+  bool any_rhs = false, any_hyper = false;
+
+  explicit ViscModule(const std::vector<std::pair<std::string, std::string>>& o) {  // viscosity.cpp:6-33
+    name = "artificial_viscosity";
+    for (auto& kv : o) {
+      const std::string& k = kv.first;
+      const std::string& v = kv.second;
+      if (k == "hv_time_integrator") hv_integ = v;
+      else if (k == "gradient_correction") grad = v == "true";
+      else if (k == "hv_epsilon") hv_eps = std::stod(v);
+      else if (k == "visc_opt") in_opt = v;
+      else if (k == "boundary_falloff_shape") shape = v;
+      else if (k == "visc_strength") in_strength = v;
+      else if (k == "visc_vars_to_diff") in_diff = v;
+      else if (k == "visc_vars_to_evol") in_evol = v;
+      else if (k == "visc_length") in_length = v;
+      else if (k == "visc_species") in_species = v;
+      // visc_output_* select file output only; other keys only warn in the reference
+    }
+  }
+  bool needs_pre_scalars() const override { return false; }
+  bool iterates() const override { return any_hyper; }
+  void pre_launch(Domain&, double*) override {}
+  void pre_finish(double, const double*) override {}
+
+  static int var_code(const std::string& n) {
+    static const char* names[VV_COUNT] = {"rho", "n", "mom_x", "mom_y", "mom_z", "thermal_energy", "bi_x", "bi_y",
+                                          "bi_z", "grav_x", "grav_y", "temp", "press", "v_x", "v_y", "v_z",
+                                          "kinetic_energy", "b_x", "b_y", "b_z", "b_mag", "b_hat_x", "b_hat_y",
+                                          "b_hat_z"};
+    for (int k = 0; k < VV_COUNT; ++k) if (n == names[k]) return k;
+    return -1;
+  }
+  static int evolved_code(const std::string& n) {
+    static const char* ev[8] = {"rho", "mom_x", "mom_y", "mom_z", "thermal_energy", "bi_x", "bi_y", "bi_z"};
+    for (int k = 0; k < 8; ++k) if (n == ev[k]) return k;
+    return -1;
+  }
+
+  // getBoundaryViscosity (viscosity.cpp:288-338), evaluated once on the host with the C library's
+  // exp: it depends on the cell positions only.
+  std::vector<double> boundary(const Domain& d, double st, double len) const {
+    const auto& x = d.host_grids.at("pos_x");
+    const auto& y = d.host_grids.at("pos_y");
+    const size_t N = d.N;
+    auto gmax = [](const std::vector<double>& a) {
+      double m = -DBL_MAX;
+      for (double v : a) m = (m < v) ? v : m;
+      require(m > -DBL_MAX, SPRUCE_E_NUMERIC, "Something went wrong in the max calculation");
+      return m;
+    };
+    auto gmin = [](const std::vector<double>& a) {
+      double m = DBL_MAX;
+      for (double v : a) m = (v < m) ? v : m;
+      require(m < DBL_MAX, SPRUCE_E_NUMERIC, "Something went wrong in the min calculation");
+      return m;
+    };
+    const double xmax = gmax(x), ymax = gmax(y), xmin = gmin(x), ymin = gmin(y);
+    std::vector<double> r(N, 0.0);
+    if (shape == "gaussian") {
+      for (size_t c = 0; c < N; ++c) {
+        const double a0 = (x[c] - xmin) / len, a1 = (x[c] - xmax) / len;
+        const double a2 = (y[c] - ymax) / len, a3 = (y[c] - ymin) / len;
+        r[c] = r[c] + std::exp(-2.3 * (a0 * a0)) * st;
+        r[c] = r[c] + std::exp(-2.3 * (a1 * a1)) * st;
+        r[c] = r[c] + std::exp(-2.3 * (a2 * a2)) * st;
+        r[c] = r[c] + std::exp(-2.3 * (a3 * a3)) * st;
+      }
+    } else if (shape == "exp") {
+      for (size_t c = 0; c < N; ++c) {
+        r[c] = r[c] + std::exp(-2.3 * (x[c] - xmin) / len) * st;
+        r[c] = r[c] + std::exp(2.3 * (x[c] - xmax) / len) * st;
+        r[c] = r[c] + std::exp(2.3 * (y[c] - ymax) / len) * st;
+        r[c] = r[c] + std::exp(-2.3 * (y[c] - ymin) / len) * st;
+      }
+    } else {
+      const double xc = 0.5 * (xmin + xmax), yc = 0.5 * (ymin + ymax);
+      const double ax = xmax - xc, ay = ymax - yc;
+      const double sl = len / std::min(ax, ay);
+      for (size_t c = 0; c < N; ++c) {
+        const double dx = x[c] - xc, dy = y[c] - yc;
+        const double rs = dx * dx / (ax * ax) + dy * dy / (ay * ay);
+        const double m = ((rs - 1.0) < 0.0) ? (rs - 1.0) : 0.0;   // min(rs - 1, 0) with std::min semantics
+        if (shape == "exp_elliptical") r[c] = std::exp(2.3 * m / sl) * st;
+        else { const double t = m / sl; r[c] = std::exp(-2.3 * (t * t)) * st; }
+      }
+    }
+    for (auto& v : r) v = (st < v) ? st : v;
+    return r;
+  }
+
+  void setup(Domain& d) override {  // viscosity.cpp:35-108
+    const auto opts = split_list(no_space(in_opt));
+    const auto strs = split_list(no_space(in_strength));
+    const auto diffs = split_list(no_space(in_diff));
+    const auto evols = split_list(no_space(in_evol));
+    const auto lens = split_list(no_space(in_length));
+    const auto specs = split_list(in_species);
+    const size_t n = opts.size();
+    require(n > 0, SPRUCE_E_ARG, "artificial_viscosity: no terms given");
+    require(strs.size() == n && diffs.size() == n && evols.size() == n && lens.size() == n && specs.size() == n,
+            SPRUCE_E_ARG, "artificial_viscosity: every list must have one entry per term");
+    if (hv_integ.empty()) hv_integ = "euler";
+    require(hv_integ == "euler" || hv_integ == "rk2" || hv_integ == "rk4", SPRUCE_E_ARG,
+            "Invalid hyperviscous time integrator given for Viscosity module");
+    if (shape.empty()) shape = "gaussian";
+    require(shape == "gaussian" || shape == "exp" || shape == "exp_elliptical" || shape == "gaussian_elliptical",
+            SPRUCE_E_ARG, "Invalid boundary falloff shape given for Viscosity module");
+    terms.resize(n);
+    int n_rhs = 0;
+    for (size_t i = 0; i < n; ++i) {
+      Term& t = terms[i];
+      t.opt = opts[i]; t.diff = diffs[i]; t.evol = evols[i]; t.species = specs[i];
+      t.strength = std::stod(strs[i]);
+      t.length = std::stod(lens[i]);
+      require(t.opt == "local" || t.opt == "global" || t.opt == "boundary" || t.opt == "boundary_global",
+              SPRUCE_E_ARG, "Viscosity option must be global, local, or boundary.");
+      require(t.length >= 0, SPRUCE_E_ARG, "Length constants must greater than or equal to zero.");
+      t.var = var_code(t.diff);
+      require(t.var >= 0, SPRUCE_E_ARG,
+              "Each variable to differentiate must be a valid variable within the chosen equation set.");
+      t.evol_idx = evolved_code(t.evol);
+      require(t.evol_idx >= 0, SPRUCE_E_ARG,
+              "Each variable to evolve must be a valid evolved variable within the chosen equation set.");
+      const bool mom = t.evol_idx >= 1 && t.evol_idx <= 3, vel = t.var >= VV_VX && t.var <= VV_VZ;
+      if (mom && vel) {            // viscosity.cpp:231-239: ions scale by n*m_i; IdealMHD has no e_n
+        require(t.species == "i", SPRUCE_E_UNSUPPORTED, "electron momentum viscosity needs e_n (not in ideal_mhd)");
+        t.scale = 1;
+        t.scale_c = d.kc.m;
+      }
+      if (t.evol_idx == 4 && t.var == VV_TEMP && (t.species == "i" || t.species == "e")) {  // :244-254
+        t.scale = 2;
+        t.scale_c = K_B / (d.desc.adiabatic_index - 1);
+      }
+      if (t.opt == "boundary" || t.opt == "boundary_global") {
+        t.sgrid_host = boundary(d, t.strength, t.length);
+        t.sgrid.alloc(d.N);
+        hip_check(hipMemcpy(t.sgrid.p, t.sgrid_host.data(), d.N * 8, hipMemcpyHostToDevice), "upload strength");
+      }
+      if (t.hyper()) any_hyper = true;
+      else { any_rhs = true; t.out.alloc(d.N); ++n_rhs; }
+    }
+    require(n_rhs <= MAX_RHS_TERMS, SPRUCE_E_UNSUPPORTED, "at most 4 artificial_viscosity RHS terms");
+    dtg.alloc(d.N);
+    dtmin.alloc(1);
+    part.alloc(d.dt_blocks);
+    if (any_hyper) {
+      init.alloc(d.N); d1.alloc(d.N);
+      if (hv_integ == "rk4") { d2.alloc(d.N); d3.alloc(d.N); d4.alloc(d.N); dq.alloc(d.N); }
+    }
+  }
+
+  // the live dt grid and its minimum over the dt bounds (viscosity.cpp:195-208)
+  void refresh(Domain& d) {
+    d.timed("visc_dt", [&] {
+      launch_mhd_dt(d.L.view(), d.stat(), d.kc, d.mesh, dtg.p, part.p, d.stream);
+      launch_reduce_step(part.p, d.dt_blocks, 1.0, dtmin.p, nullptr, 0, d.err_flag, d.stream);
+    });
+  }
+  void term(Domain& d, const Term& t, SetBufs& S, double* out, bool masked) {
+    ViscTermArgs a{};
+    a.S = S.view(); a.st = d.stat(); a.k = d.kc; a.mesh = d.mesh;
+    a.strength = t.sgrid.p;
+    a.strength_c = t.strength;
+    const bool local = t.opt == "local" || t.opt == "boundary";
+    a.dtgrid = local ? dtg.p : nullptr;
+    a.dtmin = dtmin.p;
+    a.var = t.var; a.scale = t.scale; a.scale_c = t.scale_c; a.grad = grad ? 1 : 0; a.masked = masked ? 1 : 0;
+    a.out = out;
+    d.timed("visc_term", [&] { launch_visc_term(a, d.stream); });
+  }
+  void before_integrator(Domain& d) override { if (any_rhs) refresh(d); }
+  void rhs_terms(Domain& d, SetBufs& S, StageArgs& a) override {
+    for (auto& t : terms) {
+      if (t.hyper()) continue;
+      term(d, t, S, t.out.p, true);
+      a.extra[a.n_extra] = t.out.p;
+      a.extra_var[a.n_extra] = t.evol_idx;
+      a.n_extra++;
+    }
+  }
+  void add_rhs(Domain& d, SetBufs& S, double* out) override {
+    for (auto& t : terms) {
+      if (t.hyper()) continue;
+      term(d, t, S, t.out.p, true);
+      launch_add(out + (size_t)t.evol_idx * d.N, t.out.p, d.N, d.stream);
+    }
+  }
+  static double* evolved_ptr(SetBufs& L, int e) {
+    DBuf* b[8] = {&L.rho, &L.mx, &L.my, &L.mz, &L.te, &L.bx, &L.by, &L.bz};
+    return b[e]->p;
+  }
+  void iterate(Domain& d, double dt) override {  // viscosity.cpp:125-180
+    nsub = -1;
+    for (auto& t : terms) {
+      if (!t.hyper()) continue;
+      const int n = (int)(std::ceil(t.strength / hv_eps) + 0.1);
+      require(n > 0, SPRUCE_E_ARG, "hyperviscous subcycle count must be positive");
+      if (nsub < 0) nsub = n;
+      const double h = dt / (double)n;
+      double* g = evolved_ptr(d.L, t.evol_idx);
+      auto eval = [&](double* out) { refresh(d); term(d, t, d.L, out, false); };
+      auto upd = [&](const double* base, const double* dd, double hh) {
+        d.timed("visc_update", [&] { launch_visc_update(g, base, dd, hh, d.mesh, d.stream); });
+        d.propagate_live();
+      };
+      for (int sc = 0; sc < n; ++sc) {
+        if (hv_integ == "euler") {
+          eval(d1.p);
+          upd(g, d1.p, h);
+        } else {
+          hip_check(hipMemcpyAsync(init.p, g, d.N * 8, hipMemcpyDeviceToDevice, d.stream), "copy");
+          if (hv_integ == "rk2") {
+            eval(d1.p);
+            upd(init.p, d1.p, 0.5 * h);
+            eval(d1.p);
+            upd(init.p, d1.p, h);
+          } else {
+            eval(d1.p);
+            upd(init.p, d1.p, 0.5 * h);
+            eval(d2.p);
+            upd(init.p, d2.p, 0.5 * h);
+            eval(d3.p);
+            upd(init.p, d3.p, h);
+            eval(d4.p);
+            launch_visc_rk4(d1.p, d2.p, d3.p, d4.p, dq.p, d.N, d.stream);
+            upd(init.p, dq.p, h);
+          }
+        }
+      }
+      d.propagate_live();
+    }
+  }
+};
+
 void Domain::add_module(const std::string& name, const std::vector<std::pair<std::string, std::string>>& opts) {
   require(!ready, SPRUCE_E_STATE, "modules must be added before setup");
   static const char* known[] = {"ambient_heating", "radiative_losses", "thermal_conduction", "localized_heating",
@@ -646,6 +938,7 @@ void Domain::add_module(const std::string& name, const std::vector<std::pair<std
   require(ok, SPRUCE_E_ARG, "Module name was not recognized");
   if (name == "thermal_conduction") mods.push_back(std::make_unique<TcModule>(opts));
   else if (name == "radiative_losses") mods.push_back(std::make_unique<RlModule>(opts));
+  else if (name == "artificial_viscosity") mods.push_back(std::make_unique<ViscModule>(opts));
   else throw Error(SPRUCE_E_UNSUPPORTED, "module " + name + " is not available in this build");
   modules.emplace_back(name, opts);
 }

@@ -31,6 +31,11 @@ struct DBuf {
   DBuf() = default;
   DBuf(const DBuf&) = delete;
   DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
   void swap(DBuf& o) { std::swap(p, o.p); std::swap(n, o.n); }
   void alloc(size_t count) {
     release();

@@ -170,6 +170,11 @@ __global__ void __launch_bounds__(256) k_mhd_stage(StageArgs a) {
 #pragma unroll
     for (int v = 0; v < 8; ++v) k[v] = 0.0;
   }
+  for (int e = 0; e < a.n_extra; ++e) {
+#pragma unroll
+    for (int v = 0; v < 8; ++v)
+      if (v == a.extra_var[e]) k[v] = k[v] + a.extra[e][c];
+  }
   if (a.mode == STAGE_STORE_K) {
 #pragma unroll
     for (int v = 0; v < 8; ++v) a.kbuf[v * N + c] = k[v];

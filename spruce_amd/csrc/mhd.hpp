@@ -72,7 +72,13 @@ struct StageArgs {
   int mode;
   int write_pre;         // write rho_pre (open boundaries present)
   int write_n;
+  // Module RHS terms (computeTimeDerivativesModule, equationset.cpp:204-210): k[extra_var[e]] +=
+  // extra[e][c] in order, already multiplied by the ghost mask.
+  const double* extra[4];
+  int extra_var[4];
+  int n_extra;
 };
+constexpr int MAX_RHS_TERMS = 4;
 
 void launch_mhd_stage(const StageArgs& a, hipStream_t s);
 // Wavefront-strip form of the same stage (mhd2.hip); R rows per wavefront, a multiple of 4.

@@ -268,6 +268,11 @@ __global__ void __launch_bounds__(64) k_mhd_stage_strip(StageArgs a, int R) {
 #pragma unroll
       for (int w = 0; w < 8; ++w) k8[w] = 0.0;
     }
+    for (int e = 0; e < a.n_extra; ++e) {
+#pragma unroll
+      for (int w = 0; w < 8; ++w)
+        if (w == a.extra_var[e]) k8[w] = k8[w] + a.extra[e][c];
+    }
     if (MODE == STAGE_STORE_K) {
 #pragma unroll
       for (int w = 0; w < 8; ++w) a.kbuf[w * N + c] = k8[w];
