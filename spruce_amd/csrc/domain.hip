@@ -24,6 +24,7 @@
 #include "visc.hpp"
 #include "crpow.hpp"
 #include "tf_host.hpp"
+#include "e2_host.hpp"
 
 namespace spruce {
 
@@ -1012,8 +1013,13 @@ spruce_gpu_domain* spruce_gpu_create(const spruce_domain_desc* desc, const char*
   try {
     if (!desc || !eqset) throw Error(SPRUCE_E_ARG, "null argument");
     auto h = std::make_unique<spruce_gpu_domain>();
-    if (std::string(eqset) == "ideal_2F") {
+    const std::string eq = eqset;
+    if (eq == "ideal_2F") {
       h->d = std::make_unique<TfDomain>(*desc, strs(nopts, keys), strs(nopts, vals));
+    } else if (eq == "ideal_mhd_2E") {
+      h->d = std::make_unique<E2Domain>(*desc, strs(nopts, keys), strs(nopts, vals));
+    } else if (eq == "ideal_mhd_cons") {
+      h->d = std::make_unique<ConsDomain>(*desc);   // parseEquationSetConfigs accepts any key (idealmhdcons.cpp:10-11)
     } else {
       auto d = std::make_unique<Domain>(*desc, eqset);
       d->parse_eqset(strs(nopts, keys), strs(nopts, vals));

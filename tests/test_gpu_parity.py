@@ -44,7 +44,7 @@ def test_device_division_and_sqrt_are_ieee():
 
 # the solar cases go through libm pow in the reference and are compared within tolerance in
 # test_gpu_modules.py; every other case is bit-identical
-@pytest.mark.parametrize("case", [c for c in golden_names() if c.startswith(("mhd_", "tf_"))])
+@pytest.mark.parametrize("case", [c for c in golden_names() if c.startswith(("mhd_", "tf_", "e2_", "cons_"))])
 def test_gpu_matches_reference_golden(case):
     g = load_golden(case)
     dom = GpuDomain(g.spec())
