@@ -78,6 +78,8 @@ struct GpuModule {
   virtual void rhs_terms(Domain&, SetBufs&, StageArgs&) {}
   // the same terms added to a full RHS array (nevolved consecutive grids)
   virtual void add_rhs(Domain&, SetBufs&, double*) {}
+  // postIterateModule: after the integrator's final propagate, before the next step size
+  virtual void post(Domain&) {}
 };
 
 static double opt_d(const std::vector<std::pair<std::string, std::string>>& o, const std::string& k, bool& found) {
@@ -418,6 +420,7 @@ class Domain : public DomainBase {
       stage(W, L, L, 1.0, STAGE_RK4_FINAL, 0);
     }
     ghost(L, kinds_all, GHOST_NORMAL);
+    for (auto& m : mods) m->post(*this);
     finish_step(s + 1);
   }
 
@@ -927,6 +930,62 @@ struct ViscModule : GpuModule {
   }
 };
 
+// ---------------------------------------------------------------------------------------------
+// AmbientHeating (source/modules/solar/ambientheating.cpp)
+// ---------------------------------------------------------------------------------------------
+struct HeatModule : GpuModule {
+  double rate = 0.0, exp_base = 0.0, exp_h = 0.0, split_h = 0.0, split_start = 0.0;
+  bool exp_mode = false, split_mode = false;
+  DBuf heating;
+  explicit HeatModule(const std::vector<std::pair<std::string, std::string>>& o) {  // :12-26
+    name = "ambient_heating";
+    for (auto& kv : o) {
+      const std::string& k = kv.first;
+      const std::string& v = kv.second;
+      if (k == "heating_rate") rate = std::stod(v);
+      if (k == "exp_mode") exp_mode = v == "true";
+      else if (k == "exp_base_heating_rate") exp_base = std::stod(v);
+      else if (k == "exp_scale_height") exp_h = std::stod(v);
+      else if (k == "split_exp_mode") split_mode = v == "true";
+      else if (k == "split_exp_scale_height") split_h = std::stod(v);
+      else if (k == "split_exp_start_height") split_start = std::stod(v);
+    }
+  }
+  bool needs_pre_scalars() const override { return false; }
+  bool iterates() const override { return false; }
+  void pre_launch(Domain&, double*) override {}
+  void pre_finish(double, const double*) override {}
+  void iterate(Domain&, double) override {}
+  // the heating grid depends on positions only: built once on the host with the C library's exp
+  // (setupModule, :28-40)
+  void setup(Domain& d) override {
+    const auto& y = d.host_grids.at("pos_y");
+    std::vector<double> h(d.N);
+    for (int i = 0; i < d.nx; ++i)
+      for (int j = 0; j < d.ny; ++j) {
+        const size_t c = (size_t)i * d.ny + j;
+        const double mask = (i >= d.xl && i <= d.xu && j >= d.yl && j <= d.yu) ? 1.0 : 0.0;
+        if (exp_mode) {
+          double v = mask * exp_base * std::exp(-1.0 * y[c] / exp_h);
+          if (split_mode) {
+            const double sb = exp_base * std::exp((exp_h - split_h) * split_start / (exp_h * split_h));
+            const double o = mask * sb * std::exp(-1.0 * y[c] / split_h);
+            v = (v < o) ? o : v;
+          }
+          h[c] = v;
+        } else {
+          h[c] = mask * rate;
+        }
+      }
+    heating.alloc(d.N);
+    hip_check(hipMemcpy(heating.p, h.data(), d.N * 8, hipMemcpyHostToDevice), "upload heating");
+  }
+  void post(Domain& d) override {  // postIterateModule (:42-49)
+    d.timed("heating", [&] { launch_add_step_times(d.L.te.p, heating.p, d.step.p, d.N, d.stream); });
+    d.propagate_live();
+  }
+};
+
 void Domain::add_module(const std::string& name, const std::vector<std::pair<std::string, std::string>>& opts) {
   require(!ready, SPRUCE_E_STATE, "modules must be added before setup");
   static const char* known[] = {"ambient_heating", "radiative_losses", "thermal_conduction", "localized_heating",
@@ -940,6 +999,7 @@ void Domain::add_module(const std::string& name, const std::vector<std::pair<std
   if (name == "thermal_conduction") mods.push_back(std::make_unique<TcModule>(opts));
   else if (name == "radiative_losses") mods.push_back(std::make_unique<RlModule>(opts));
   else if (name == "artificial_viscosity") mods.push_back(std::make_unique<ViscModule>(opts));
+  else if (name == "ambient_heating") mods.push_back(std::make_unique<HeatModule>(opts));
   else throw Error(SPRUCE_E_UNSUPPORTED, "module " + name + " is not available in this build");
   modules.emplace_back(name, opts);
 }

@@ -136,6 +136,11 @@ __global__ void __launch_bounds__(256) k_add(double* out, const double* add, siz
   if (c < n) out[c] = out[c] + add[c];
 }
 
+__global__ void __launch_bounds__(256) k_add_step_times(double* y, const double* x, const double* step, size_t n) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) y[c] = y[c] + (*step) * x[c];
+}
+
 inline dim3 g2(const Mesh& m, dim3 b) { return dim3((m.ny + b.x - 1) / b.x, (m.nx + b.y - 1) / b.y); }
 
 }  // namespace
@@ -155,6 +160,10 @@ void launch_visc_rk4(const double* d1, const double* d2, const double* d3, const
 
 void launch_add(double* out, const double* add, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_add, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, add, n);
+}
+
+void launch_add_step_times(double* y, const double* x, const double* step, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_step_times, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, y, x, step, n);
 }
 
 }  // namespace spruce

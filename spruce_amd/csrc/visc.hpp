@@ -38,5 +38,7 @@ void launch_visc_rk4(const double* d1, const double* d2, const double* d3, const
                      hipStream_t s);
 // out += add (EquationSet::computeTimeDerivatives adding a module term, equationset.cpp:204-210).
 void launch_add(double* out, const double* add, size_t n, hipStream_t s);
+// y += (*step) * x with the step size read on the device (AmbientHeating::postIterateModule).
+void launch_add_step_times(double* y, const double* x, const double* step, size_t n, hipStream_t s);
 
 }  // namespace spruce

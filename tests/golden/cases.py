@@ -178,6 +178,18 @@ def _cases() -> Dict[str, Tuple[Callable[[], StateFile], Callable[[], ConfigFile
                                                                _visc("boundary_global,local", "0.4,0.1", "v_x,v_z",
                                                                      "mom_x,mom_z", "3e7,0", "i,i",
                                                                      boundary_falloff_shape="exp")]))
+    # ambient heating (ambientheating.cpp): the post-integrator heating injection
+    c["heat_const_euler"] = (lambda: random_mhd_state(20, 22, 56),
+                             lambda: _mhd(("periodic", "periodic", "fixed", "open"), "euler", 5,
+                                          modules=[("ambient_heating", [("heating_rate", "3e-4")])]))
+    c["heat_split_tc_rk2"] = (lambda: mini_arcade(18, 28, 57),
+                              lambda: _solar(SOLAR_BC, "rk2", 3, [_tc(wf="30"),
+                                                                 ("ambient_heating", [("exp_mode", "true"),
+                                                                                      ("exp_base_heating_rate", "2e-3"),
+                                                                                      ("exp_scale_height", "5e7"),
+                                                                                      ("split_exp_mode", "true"),
+                                                                                      ("split_exp_scale_height", "2e8"),
+                                                                                      ("split_exp_start_height", "6e7")])]))
     # single-fluid variants: two energies, and the conservative total-energy form
     E2 = ["rho", "mom_x", "mom_y", "i_thermal_energy", "e_thermal_energy", "bi_x", "bi_y", "i_temp", "e_temp",
           "press", "b_hat_x", "dt"]

@@ -1,5 +1,6 @@
 """GPU parity for artificial viscosity (module "artificial_viscosity", viscosity.cpp): RHS terms of
-strength <= 1 fused into the stage kernels, hyperviscous subcycling, boundary strength profiles.
+strength <= 1 fused into the stage kernels, hyperviscous subcycling, boundary strength profiles;
+and for ambient heating (ambientheating.cpp), the post-integrator heating injection.
 No libm power is involved, so every case without thermal conduction is bit-identical to the
 reference golden vectors; the case with conduction is bit-identical to the restatement run with
 correctly rounded powers and within 1e-10 of the reference."""
@@ -14,8 +15,8 @@ from spruce_amd.spec import spec_from_files
 
 pytestmark = pytest.mark.gpu
 
-VISC = [c for c in golden_names() if c.startswith("visc_") and "_tc_" not in c]
-VISC_TC = [c for c in golden_names() if c.startswith("visc_") and "_tc_" in c]
+VISC = [c for c in golden_names() if c.startswith(("visc_", "heat_")) and "_tc_" not in c]
+VISC_TC = [c for c in golden_names() if c.startswith(("visc_", "heat_")) and "_tc_" in c]
 VARS = ["rho", "temp", "mom_x", "mom_y", "mom_z", "bi_x", "bi_y", "bi_z", "n", "thermal_energy", "press", "dt"]
 KERNELS = ["cell", "strip"]
 
