@@ -190,6 +190,22 @@ def _cases() -> Dict[str, Tuple[Callable[[], StateFile], Callable[[], ConfigFile
                                                                                       ("split_exp_mode", "true"),
                                                                                       ("split_exp_scale_height", "2e8"),
                                                                                       ("split_exp_start_height", "6e7")])]))
+    # physical (Braginskii) viscosity (physicalviscosity.cpp): heating and force, subcycled; T^2.5 is
+    # a libm pow in the reference
+    c["pv_euler"] = (lambda: random_mhd_state(22, 20, 58),
+                     lambda: _mhd(("periodic", "periodic", "fixed", "open"), "rk2", 4,
+                                  modules=[("physical_viscosity", [("coeff", "2e-16"), ("epsilon", "0.3")])]))
+    c["pv_gc_ramp_rk2"] = (lambda: random_mhd_state(20, 24, 59, stretch=True),
+                           lambda: _mhd(("open",) * 4, "rk2", 3, open_boundary_strength=0.2,
+                                        modules=[("physical_viscosity", [("coeff", "5e-16"), ("ramp_length", "4e7"),
+                                                                         ("gradient_correction", "true"),
+                                                                         ("time_integrator", "rk2"),
+                                                                         ("epsilon", "0.5")])]))
+    c["pv_tc_force_only"] = (lambda: mini_arcade(18, 24, 60),
+                             lambda: _solar(SOLAR_BC, "rk2", 3, [_tc(wf="30"),
+                                                                ("physical_viscosity", [("coeff", "5e-17"),
+                                                                                        ("heating_on", "false"),
+                                                                                        ("epsilon", "0.5")])]))
     # single-fluid variants: two energies, and the conservative total-energy form
     E2 = ["rho", "mom_x", "mom_y", "i_thermal_energy", "e_thermal_energy", "bi_x", "bi_y", "i_temp", "e_temp",
           "press", "b_hat_x", "dt"]
