@@ -22,6 +22,7 @@
 #include "host_common.hpp"
 #include "solar.hpp"
 #include "visc.hpp"
+#include "pvisc.hpp"
 #include "crpow.hpp"
 #include "tf_host.hpp"
 #include "e2_host.hpp"
@@ -986,6 +987,114 @@ struct HeatModule : GpuModule {
   }
 };
 
+// ---------------------------------------------------------------------------------------------
+// PhysicalViscosity (source/modules/solar/physicalviscosity.cpp)
+// ---------------------------------------------------------------------------------------------
+struct PvModule : GpuModule {
+  double coeff = 0.0, ramp = 0.0, epsilon = 1.0;
+  bool heating_on = true, force_on = true, inactive = false, grad_corr = false;
+  std::string integ;
+  DBuf coef, bhx, bhy, bhz, part, tsmin;
+  DBuf sa[8], sb[8], sh[8], dv[7];
+  explicit PvModule(const std::vector<std::pair<std::string, std::string>>& o) {  // :17-34
+    name = "physical_viscosity";
+    for (auto& kv : o) {
+      const std::string& k = kv.first;
+      const std::string& v = kv.second;
+      if (k == "coeff") coeff = std::stod(v);
+      else if (k == "ramp_length") ramp = std::stod(v);
+      else if (k == "buffer_length") (void)std::stod(v);
+      else if (k == "epsilon") epsilon = std::stod(v);
+      else if (k == "heating_on") heating_on = v == "true";
+      else if (k == "force_on") force_on = v == "true";
+      else if (k == "inactive_mode") inactive = v == "true";
+      else if (k == "gradient_correction") grad_corr = v == "true";
+      else if (k == "time_integrator") integ = v;
+    }
+    if (integ.empty()) integ = "euler";
+    require(integ == "euler" || integ == "rk2", SPRUCE_E_ARG, "Invalid time integrator given for Physical Viscosity module");
+  }
+  bool needs_pre_scalars() const override { return false; }
+  void pre_launch(Domain&, double*) override {}
+  void pre_finish(double, const double*) override {}
+  static PvSet view(DBuf* b) { return PvSet{b[0].p, b[1].p, b[2].p, b[3].p, b[4].p, b[5].p, b[6].p, b[7].p}; }
+  // constructCoefficientGrid (:252-270), once on the host with the C library's exp
+  void setup(Domain& d) override {
+    std::vector<double> g(d.N, coeff * 1.0);
+    if (ramp != 0.0) {
+      const auto& x = d.host_grids.at("pos_x");
+      const auto& y = d.host_grids.at("pos_y");
+      double xmax = -DBL_MAX, ymax = -DBL_MAX, xmin = DBL_MAX, ymin = DBL_MAX;
+      for (double v : x) { xmax = (xmax < v) ? v : xmax; xmin = (v < xmin) ? v : xmin; }
+      for (double v : y) { ymax = (ymax < v) ? v : ymax; ymin = (v < ymin) ? v : ymin; }
+      const double xc = 0.5 * (xmin + xmax), yc = 0.5 * (ymin + ymax);
+      const double ax = xmax - xc, ay = ymax - yc;
+      const double sl = ramp / std::min(ax, ay);
+      for (size_t c = 0; c < d.N; ++c) {
+        const double dx = x[c] - xc, dy = y[c] - yc;
+        const double rs = dx * dx / (ax * ax) + dy * dy / (ay * ay);
+        double m = rs + 2.0 * sl - 1.0;
+        m = (m < 0.0) ? 0.0 : m;
+        const double t = m / sl;
+        double e = std::exp(-2.3 * (t * t)) - 0.01;
+        e = (e < 0.0) ? 0.0 : e;
+        g[c] = coeff * (1.0 / 0.99 * e);
+      }
+    }
+    coef.alloc(d.N);
+    hip_check(hipMemcpy(coef.p, g.data(), d.N * 8, hipMemcpyHostToDevice), "upload coefficient");
+    bhx.alloc(d.N); bhy.alloc(d.N); bhz.alloc(d.N);
+    for (auto& b : sa) b.alloc(d.N);
+    for (auto& b : sb) b.alloc(d.N);
+    if (integ == "rk2") for (auto& b : sh) b.alloc(d.N);
+    for (auto& b : dv) b.alloc(d.N);
+    part.alloc(pv_blocks(d.mesh));
+    tsmin.alloc(1);
+  }
+  void iterate(Domain& d, double dt) override {  // :148-250
+    PvFrozen fz{bhx.p, bhy.p, bhz.p, coef.p, d.L.n.p, d.L.rho.p};
+    d.timed("pv_init", [&] { launch_pv_init(d.L.view(), d.stat(), d.kc, d.mesh, bhx.p, bhy.p, bhz.p, view(sa), d.stream); });
+    // computeViscousSubcycles (:67-80): epsilon * min(timescale) over the interior
+    d.timed("pv_timescale", [&] {
+      launch_pv_timescale(view(sa), fz, 0.000001 * coeff, d.mesh, part.p, d.stream);
+      launch_reduce_step(part.p, pv_blocks(d.mesh), 1.0, tsmin.p, nullptr, 0, d.err_flag, d.stream);
+    });
+    double m = 0.0;
+    hip_check(hipMemcpyAsync(&m, tsmin.p, 8, hipMemcpyDeviceToHost, d.stream), "read timescale");
+    hip_check(hipStreamSynchronize(d.stream), "hipStreamSynchronize");
+    require(m < DBL_MAX, SPRUCE_E_NUMERIC, "Something went wrong in the min calculation (physical viscosity)");
+    nsub = subcycle_count(dt, epsilon * m);
+    const double h = dt / nsub;
+    PvDerivs dd{dv[0].p, dv[1].p, dv[2].p, dv[3].p, dv[4].p, dv[5].p, dv[6].p};
+    PvArgs a{};
+    a.d = dd; a.f = fz; a.k = d.kc; a.mesh = d.mesh; a.h = h;
+    a.heating_on = heating_on ? 1 : 0; a.force_on = force_on ? 1 : 0; a.apply = inactive ? 0 : 1;
+    a.coeff_zero = coeff == 0.0 ? 1 : 0; a.grad_corr = grad_corr ? 1 : 0;
+    DBuf* cur = sa;
+    DBuf* nxt = sb;
+    for (int s = 0; s < nsub; ++s) {
+      d.timed("pv_derivs", [&] { launch_pv_derivs(view(cur), d.mesh, dd, d.stream); });
+      if (integ == "euler") {
+        a.eval = view(cur); a.base = view(cur); a.out = view(nxt); a.mode = PV_FULL;
+        d.timed("pv_apply", [&] { launch_pv_apply(a, d.stream); });
+      } else {
+        a.eval = view(cur); a.base = view(cur); a.out = view(sh); a.mode = PV_HALF;
+        d.timed("pv_apply", [&] { launch_pv_apply(a, d.stream); });
+        d.timed("pv_derivs", [&] { launch_pv_derivs(view(sh), d.mesh, dd, d.stream); });
+        a.eval = view(sh); a.base = view(cur); a.out = view(nxt); a.mode = PV_FULL;
+        d.timed("pv_apply", [&] { launch_pv_apply(a, d.stream); });
+      }
+      std::swap(cur, nxt);
+    }
+    const size_t bytes = d.N * 8;
+    hip_check(hipMemcpyAsync(d.L.te.p, cur[0].p, bytes, hipMemcpyDeviceToDevice, d.stream), "copy te");
+    hip_check(hipMemcpyAsync(d.L.mx.p, cur[2].p, bytes, hipMemcpyDeviceToDevice, d.stream), "copy mom_x");
+    hip_check(hipMemcpyAsync(d.L.my.p, cur[3].p, bytes, hipMemcpyDeviceToDevice, d.stream), "copy mom_y");
+    hip_check(hipMemcpyAsync(d.L.mz.p, cur[4].p, bytes, hipMemcpyDeviceToDevice, d.stream), "copy mom_z");
+    d.propagate_live();
+  }
+};
+
 void Domain::add_module(const std::string& name, const std::vector<std::pair<std::string, std::string>>& opts) {
   require(!ready, SPRUCE_E_STATE, "modules must be added before setup");
   static const char* known[] = {"ambient_heating", "radiative_losses", "thermal_conduction", "localized_heating",
@@ -1000,6 +1109,7 @@ void Domain::add_module(const std::string& name, const std::vector<std::pair<std
   else if (name == "radiative_losses") mods.push_back(std::make_unique<RlModule>(opts));
   else if (name == "artificial_viscosity") mods.push_back(std::make_unique<ViscModule>(opts));
   else if (name == "ambient_heating") mods.push_back(std::make_unique<HeatModule>(opts));
+  else if (name == "physical_viscosity") mods.push_back(std::make_unique<PvModule>(opts));
   else throw Error(SPRUCE_E_UNSUPPORTED, "module " + name + " is not available in this build");
   modules.emplace_back(name, opts);
 }

@@ -16,7 +16,8 @@ from spruce_amd.spec import spec_from_files
 pytestmark = pytest.mark.gpu
 
 VISC = [c for c in golden_names() if c.startswith(("visc_", "heat_")) and "_tc_" not in c]
-VISC_TC = [c for c in golden_names() if c.startswith(("visc_", "heat_")) and "_tc_" in c]
+# cases through a libm power in the reference: conduction, physical viscosity (T^2.5)
+VISC_TC = [c for c in golden_names() if (c.startswith(("visc_", "heat_")) and "_tc_" in c) or c.startswith("pv_")]
 VARS = ["rho", "temp", "mom_x", "mom_y", "mom_z", "bi_x", "bi_y", "bi_z", "n", "thermal_energy", "press", "dt"]
 KERNELS = ["cell", "strip"]
 
@@ -128,5 +129,41 @@ def test_viscosity_matches_oracle(case, kernel):
                 a, b = gpu.grid(v), ora.grid(v)
                 assert same_bits(a, b), "%s %s: %s" % (name, v, describe(a, b))
     finally:
+        gpu.close()
+        ora.close()
+
+
+PV_BIG = [
+    ("pv_euler_periodic", ("periodic",) * 4, "rk2", [("coeff", "2e-16"), ("epsilon", "0.3")]),
+    ("pv_rk2_gc_ramp_open", ("open",) * 4, "rk4", [("coeff", "3e-16"), ("ramp_length", "2e8"),
+                                                   ("gradient_correction", "true"), ("time_integrator", "rk2")]),
+    ("pv_force_only_inactive_heat", ("fixed", "reflect", "periodic", "periodic"), "euler",
+     [("coeff", "2e-16"), ("heating_on", "false")]),
+]
+
+
+@pytest.mark.parametrize("case", PV_BIG, ids=[c[0] for c in PV_BIG])
+def test_physical_viscosity_matches_cr_oracle(case):
+    from tests.golden.cases import random_mhd_state
+    name, bc, integ, opts = case
+    print("case", name, flush=True)
+    st = random_mhd_state(72, 64, 81, stretch=True)
+    cfg = W.base_config("ideal_mhd", bc, integ, epsilon=0.2, density_min=1.0, temp_min=1.0, thermal_energy_min=1e-12,
+                        open_boundary_strength=0.3, modules=[("physical_viscosity", opts)])
+    spec = spec_from_files(cfg, st)
+    set_pow_mode(1)
+    gpu = GpuDomain(spec)
+    ora = OracleDomain(spec)
+    try:
+        for _ in range(3):
+            dg, sg = gpu.step(1, subcycles=True)
+            do = ora.step(1)
+            assert same_bits(dg, do), "%s dt %s vs %s" % (name, dg, do)
+            assert list(sg[-1]) == ora.subcycles(), "%s subcycles %s vs %s" % (name, sg[-1], ora.subcycles())
+        for v in VARS:
+            a, b = gpu.grid(v), ora.grid(v)
+            assert same_bits(a, b), "%s %s: %s" % (name, v, describe(a, b))
+    finally:
+        set_pow_mode(0)
         gpu.close()
         ora.close()
