@@ -130,6 +130,20 @@ int spruce_gpu_pow_probe(const double* x, int64_t n, int32_t which, double* out)
 const char* spruce_gpu_last_error(void);
 void spruce_gpu_destroy(spruce_gpu_domain* d);
 
+/* Discrete adjoint (no reference counterpart: the reference has no gradient).  Runs nsteps steps
+ * exactly as spruce_gpu_step does (the handle advances; dt_out, may be NULL, receives the step
+ * sizes), keeping a checkpoint of the evolved grids every checkpoint_interval steps, then sweeps
+ * backward through the hand-written reverse of every stage, minimum/derived update and step-size
+ * reduction, recomputing each segment from its checkpoint.  seed = dJ/d(evolved grids after the
+ * steps); grad_out = dJ/d(evolved grids before them); both 8*xdim*ydim doubles in evolved order
+ * (rho, mom_x, mom_y, mom_z, thermal_energy, bi_x, bi_y, bi_z).  Covers ideal_mhd with periodic
+ * boundaries, euler or rk2, and no modules; SPRUCE_E_UNSUPPORTED otherwise. */
+int spruce_gpu_adjoint(spruce_gpu_domain* d, int32_t nsteps, int32_t checkpoint_interval, const double* seed,
+                       double* grad_out, double* dt_out);
+/* Reverse of populateVariablesFromState (equationset.cpp:96-104): maps dJ/d(evolved grids right
+ * after setup) to dJ/d(.state grids rho, temp, mom_x, mom_y, mom_z, bi_x, bi_y, bi_z), 8*xdim*ydim. */
+int spruce_gpu_adjoint_state(spruce_gpu_domain* d, const double* grad_evolved, double* grad_state);
+
 /* Batched mode (no reference counterpart: the reference runs one PlasmaDomain per process,
  * main.cpp:17-94).  A batch advances several set-up ideal_2F domains of any sizes with one launch
  * per kernel of the step (evolution.cpp:59-124 for each domain), each domain keeping its own step
@@ -147,6 +161,11 @@ int spruce_gpu_batch_autotune(spruce_gpu_batch* b, int32_t trial_steps, char* ch
                               double* ms_per_step);
 /* nsteps steps of every member; dt_out (may be NULL) receives n*nsteps step sizes, member-major. */
 int spruce_gpu_batch_step(spruce_gpu_batch* b, int32_t nsteps, double* dt_out);
+/* Per-kernel event timing of batch steps, as spruce_gpu_set_profiling / spruce_gpu_kernel_times;
+ * while profiling is on, steps are launched one by one instead of replayed from the graph. */
+int spruce_gpu_batch_set_profiling(spruce_gpu_batch* b, int32_t enable);
+int spruce_gpu_batch_kernel_times(spruce_gpu_batch* b, char* names, int32_t names_len, double* ms_avg,
+                                  int32_t* launches, int32_t max_kernels);
 void spruce_gpu_batch_destroy(spruce_gpu_batch* b);
 
 #ifdef __cplusplus

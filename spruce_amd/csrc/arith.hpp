@@ -22,7 +22,7 @@ __device__ __host__ __forceinline__ double smax(double a, double b) { return (a 
 // NaN, the IEEE division is taken instead, so the result always equals a/b bit for bit.
 // b must be a finite positive normal number and rb its correctly rounded reciprocal (both are
 // geometry or physical constants prepared on the host).
-__device__ __forceinline__ double qdiv(double a, double b, double rb) {
+__device__ __host__ __forceinline__ double qdiv(double a, double b, double rb) {
   double q = a * rb;
   double aq = __builtin_fabs(q), aa = __builtin_fabs(a);
   if (__builtin_expect(aq > 0x1p-900 && aq < 0x1p+900 && aa > 0x1p-900, 1)) {

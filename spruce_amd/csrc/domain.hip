@@ -23,6 +23,7 @@
 #include "solar.hpp"
 #include "visc.hpp"
 #include "pvisc.hpp"
+#include "adjoint.hpp"
 #include "crpow.hpp"
 #include "tf_host.hpp"
 #include "e2_host.hpp"
@@ -360,11 +361,11 @@ class Domain : public DomainBase {
 
   MhdStatic stat() { return MhdStatic{bex.p, bey.p, bez.p, gx.p, gy.p}; }
 
-  void stage(SetBufs& S, SetBufs& B, SetBufs& T, double mult, int mode, int kslot) {
+  void stage(SetBufs& S, SetBufs& B, SetBufs& T, double mult, int mode, int kslot, const double* step_ptr = nullptr) {
     StageArgs a{};
     a.S = S.view(); a.B = B.view(); a.T = T.view();
     a.st = stat(); a.k = kc; a.mesh = mesh;
-    a.step = step.p; a.step_mult = mult; a.mode = mode;
+    a.step = step_ptr ? step_ptr : step.p; a.step_mult = mult; a.mode = mode;
     a.write_pre = any_open ? 1 : 0;
     a.write_n = 1;
     if (mode == STAGE_STORE_K) a.kbuf = kbuf[kslot].p;
@@ -500,6 +501,110 @@ class Domain : public DomainBase {
     for (auto& m : mods) m->before_integrator(*this);
     for (auto& m : mods) m->add_rhs(*this, L, tmp.p);
     hip_check(hipMemcpyAsync(out, tmp.p, 8 * N * 8, hipMemcpyDeviceToHost, stream), "download rhs");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  }
+
+  static void copy_set(SetBufs& from, SetBufs& to, size_t N, hipStream_t s) {
+    auto a = from.all(), b = to.all();
+    for (size_t k = 0; k < a.size(); ++k)
+      hip_check(hipMemcpyAsync(b[k]->p, a[k]->p, N * 8, hipMemcpyDeviceToDevice, s), "copy set");
+  }
+
+  // Discrete adjoint of nsteps steps (reverse of advance(): stages, minimums/derived, step size),
+  // with the evolved grids checkpointed every `ckpt` steps and each segment recomputed from its
+  // checkpoint during the backward sweep.  Periodic boundaries, euler or rk2, no modules.
+  void adjoint(int nsteps, int ckpt, const double* seed, double* grad, double* dt_out) override {
+    require(ready, SPRUCE_E_STATE, "setup must precede adjoint");
+    require(mods.empty(), SPRUCE_E_UNSUPPORTED, "the adjoint does not cover modules");
+    for (int k = 0; k < 4; ++k)
+      require(desc.bc[k] == SPRUCE_BC_PERIODIC, SPRUCE_E_UNSUPPORTED,
+              "the adjoint covers periodic boundaries (no ghost-fill adjoint)");
+    const int integ = desc.time_integrator;
+    require(integ == SPRUCE_EULER || integ == SPRUCE_RK2, SPRUCE_E_UNSUPPORTED, "the adjoint covers euler and rk2");
+    require(nsteps >= 1 && ckpt >= 1, SPRUCE_E_ARG, "nsteps and checkpoint_interval must be positive");
+    require(seed && grad, SPRUCE_E_ARG, "null argument");
+    const int nseg = (nsteps + ckpt - 1) / ckpt;
+    std::vector<SetBufs> cps(nseg);
+    for (auto& c : cps) c.alloc(N);
+    if ((int)history.n < nsteps + 1) history.alloc(nsteps + 1);
+    hip_check(hipMemcpyAsync(history.p, step.p, sizeof(double), hipMemcpyDeviceToDevice, stream), "copy step");
+    // forward sweep, exactly the steps of run()
+    for (int n = 0; n < nsteps; ++n) {
+      if (n % ckpt == 0) copy_set(L, cps[n / ckpt], N, stream);
+      advance(n);
+    }
+    sync_and_check();
+    std::vector<double> h(nsteps);
+    hip_check(hipMemcpy(h.data(), history.p, nsteps * sizeof(double), hipMemcpyDeviceToHost), "read dt history");
+    for (int n = 0; n < nsteps; ++n) {
+      time += h[n];
+      iter++;
+      if (dt_out) dt_out[n] = h[n];
+    }
+    // backward sweep
+    std::vector<SetBufs> seg(std::min(ckpt, nsteps));
+    for (auto& q : seg) q.alloc(N);
+    DBuf lam, lamN, lamW, lamk, cx, cy, k1, k2, gh, part, pmin, pidx;
+    lam.alloc(8 * N); lamN.alloc(8 * N); lamW.alloc(8 * N); lamk.alloc(8 * N);
+    cx.alloc(ADJ_COEFS * N); cy.alloc(ADJ_COEFS * N); k1.alloc(8 * N); k2.alloc(8 * N); gh.alloc(1);
+    const int nb = adj_blocks(mesh);
+    part.alloc(nb); pmin.alloc(nb); pidx.alloc(nb);
+    hip_check(hipMemcpy(lam.p, seed, 8 * N * 8, hipMemcpyHostToDevice), "upload seed");
+    const MhdStatic st = stat();
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+      const int n0 = sg * ckpt, n1 = std::min(nsteps, n0 + ckpt);
+      copy_set(cps[sg], seg[0], N, stream);
+      for (int n = n0; n < n1 - 1; ++n) {            // recompute the segment's states
+        SetBufs& A = seg[n - n0];
+        SetBufs& B = seg[n - n0 + 1];
+        if (integ == SPRUCE_EULER) {
+          stage(A, A, B, 1.0, STAGE_PLAIN, 0, history.p + n);
+        } else {
+          stage(A, A, W, 0.5, STAGE_PLAIN, 0, history.p + n);
+          stage(W, A, B, 1.0, STAGE_PLAIN, 0, history.p + n);
+        }
+      }
+      for (int n = n1 - 1; n >= n0; --n) {
+        SetBufs& Ln = seg[n - n0];
+        const double* hp = history.p + n;
+        launch_fill(lamN.p, 0.0, 8 * N, stream);
+        launch_fill(gh.p, 0.0, 1, stream);
+        launch_mhd_rhs_full(Ln.view(), st, kc, mesh, k1.p, stream);
+        if (integ == SPRUCE_RK2) {
+          stage(Ln, Ln, W, 0.5, STAGE_PLAIN, 0, hp);
+          launch_mhd_rhs_full(W.view(), st, kc, mesh, k2.p, stream);
+          timed("adj_phi", [&] { launch_adj_phi(Ln.view(), k2.p, hp, 1.0, kc, mesh, lam.p, lamN.p, lamk.p, part.p, stream); });
+          launch_adj_sum(part.p, nb, gh.p, stream);
+          launch_fill(lamW.p, 0.0, 8 * N, stream);
+          timed("adj_rhs", [&] { launch_adj_rhs(W.view(), st, kc, mesh, lamk.p, cx.p, cy.p, lamW.p, stream); });
+          timed("adj_phi", [&] { launch_adj_phi(Ln.view(), k1.p, hp, 0.5, kc, mesh, lamW.p, lamN.p, lamk.p, part.p, stream); });
+          launch_adj_sum(part.p, nb, gh.p, stream);
+        } else {
+          timed("adj_phi", [&] { launch_adj_phi(Ln.view(), k1.p, hp, 1.0, kc, mesh, lam.p, lamN.p, lamk.p, part.p, stream); });
+          launch_adj_sum(part.p, nb, gh.p, stream);
+        }
+        timed("adj_rhs", [&] { launch_adj_rhs(Ln.view(), st, kc, mesh, lamk.p, cx.p, cy.p, lamN.p, stream); });
+        launch_adj_dt(Ln.view(), st, kc, mesh, kc.eps, gh.p, pmin.p, (int*)pidx.p, lamN.p, stream);
+        lam.swap(lamN);
+      }
+    }
+    hip_check(hipMemcpyAsync(grad, lam.p, 8 * N * 8, hipMemcpyDeviceToHost, stream), "download gradient");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    if (clock.enabled) clock.collect();
+  }
+
+  // reverse of populateVariablesFromState for the .state grids kept from set-up
+  void adjoint_state(const double* grad_evolved, double* grad_state) override {
+    require(ready, SPRUCE_E_STATE, "setup must precede adjoint_state");
+    require(grad_evolved && grad_state, SPRUCE_E_ARG, "null argument");
+    DBuf rho_s, temp_s, lam, g;
+    upload(rho_s, host_grids["rho"]);
+    upload(temp_s, host_grids["temp"]);
+    lam.alloc(8 * N);
+    g.alloc(8 * N);
+    hip_check(hipMemcpy(lam.p, grad_evolved, 8 * N * 8, hipMemcpyHostToDevice), "upload gradient");
+    launch_adj_populate(rho_s.p, temp_s.p, lam.p, kc, N, g.p, stream);
+    hip_check(hipMemcpyAsync(grad_state, g.p, 8 * N * 8, hipMemcpyDeviceToHost, stream), "download gradient");
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
   }
 
@@ -1416,6 +1521,19 @@ int spruce_gpu_batch_kernel_times(spruce_gpu_batch* h, char* names, int32_t name
                                   int32_t* launches, int32_t max_kernels) {
   API_BEGIN
   return bat(h).clock.report(names, names_len, ms_avg, launches, max_kernels);
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_adjoint(spruce_gpu_domain* h, int32_t nsteps, int32_t checkpoint_interval, const double* seed,
+                       double* grad_out, double* dt_out) {
+  API_BEGIN
+  dom(h).adjoint(nsteps, checkpoint_interval, seed, grad_out, dt_out);
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_adjoint_state(spruce_gpu_domain* h, const double* grad_evolved, double* grad_state) {
+  API_BEGIN
+  dom(h).adjoint_state(grad_evolved, grad_state);
   API_END(SPRUCE_OK)
 }
 

@@ -156,6 +156,15 @@ class DomainBase {
   virtual double* device_grid(const std::string& name) = 0;
   virtual void set_option(const std::string& k, const std::string& v) = 0;
   virtual int num_evolved() const = 0;
+  // discrete adjoint of nsteps steps (only some equation sets provide it)
+  virtual void adjoint(int nsteps, int checkpoint, const double* seed, double* grad, double* dt_out) {
+    (void)nsteps; (void)checkpoint; (void)seed; (void)grad; (void)dt_out;
+    throw Error(SPRUCE_E_UNSUPPORTED, "the adjoint is available for ideal_mhd only");
+  }
+  virtual void adjoint_state(const double* grad_evolved, double* grad_state) {
+    (void)grad_evolved; (void)grad_state;
+    throw Error(SPRUCE_E_UNSUPPORTED, "the adjoint is available for ideal_mhd only");
+  }
 
   double time = 0.0;
   int iter = 0;

@@ -78,6 +78,8 @@ def lib() -> ctypes.CDLL:
     L.spruce_gpu_batch_autotune.argtypes = [P, I32, ctypes.c_char_p, I32, DP]
     L.spruce_gpu_batch_step.argtypes = [P, I32, DP]
     L.spruce_gpu_batch_destroy.argtypes = [P]
+    L.spruce_gpu_adjoint.argtypes = [P, I32, I32, DP, DP, DP]
+    L.spruce_gpu_adjoint_state.argtypes = [P, DP, DP]
     L.spruce_gpu_batch_set_profiling.argtypes = [P, I32]
     L.spruce_gpu_batch_kernel_times.argtypes = [P, ctypes.c_char_p, I32, DP, ctypes.POINTER(I32), I32]
     _LIB = L
@@ -201,6 +203,23 @@ class GpuDomain:
         n = _chk(lib().spruce_gpu_kernel_times(self.h, names, 4096, ms, cnt, 64))
         keys = names.value.decode().split(",") if n else []
         return {k: (ms[i], cnt[i]) for i, k in enumerate(keys)}
+
+    def adjoint(self, nsteps: int, checkpoint: int, seed: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+        """Runs nsteps steps and returns (dJ/d evolved grids before them, step sizes), given
+        seed = dJ/d evolved grids after them, both shaped (8, xdim, ydim) in evolved order."""
+        seed = np.ascontiguousarray(seed, np.float64).reshape(8, self.spec.xdim, self.spec.ydim)
+        grad = np.empty_like(seed)
+        dts = np.empty(nsteps)
+        _chk(lib().spruce_gpu_adjoint(self.h, nsteps, checkpoint, _dp(seed), _dp(grad), _dp(dts)))
+        return grad, dts
+
+    def adjoint_state(self, grad_evolved: np.ndarray) -> np.ndarray:
+        """dJ/d(.state grids rho, temp, mom_x, mom_y, mom_z, bi_x, bi_y, bi_z) from dJ/d(evolved grids
+        after setup)."""
+        g = np.ascontiguousarray(grad_evolved, np.float64).reshape(8, self.spec.xdim, self.spec.ydim)
+        out = np.empty_like(g)
+        _chk(lib().spruce_gpu_adjoint_state(self.h, _dp(g), _dp(out)))
+        return out
 
     def stream(self) -> int:
         p = ctypes.c_void_p()
