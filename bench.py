@@ -1,6 +1,6 @@
 """Time-step throughput of the MI355X path: cell-updates/s and the HBM roofline of the stage kernel.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload arcade|periodic|periodic2048|arcade2048|batch2f]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload arcade|periodic|periodic2048|arcade2048|batch2f|adjoint]
 
 One "step" is one PlasmaDomain::advanceTime (evolution.cpp:59-82) over the whole grid, ghost cells
 included: every integrator stage, every module subcycle, the ghost fill and the dt reduction.
@@ -178,6 +178,40 @@ def bench_batch(args) -> dict:
     return out
 
 
+def bench_adjoint(args) -> dict:
+    """The periodic-wave adjoint: 256x256 ideal MHD, RK2, 500 steps, checkpoint every 50; gradient of
+    sum((rho_final - rho_target)^2) with respect to the initial temperature grid and v0.  Reports the
+    cost ratio (checkpointed forward + reverse sweep) / forward and a finite-difference probe."""
+    import numpy as np
+    from spruce_amd.adjoint import PeriodicWaveProblem
+
+    steps = args.steps if args.steps != 200 else 500
+    prob = PeriodicWaveProblem(256, 256, nsteps=steps, checkpoint=50)
+    prob.forward(prob.temp0, prob.v0)                       # warm-up (module load, first allocations)
+    loss, rho, t_fwd = prob.forward(prob.temp0, prob.v0)
+    g_temp, g_v0, t_adj = prob.gradient(prob.temp0, prob.v0, rho)
+    rng = np.random.default_rng(1)
+    dt = np.zeros((256, 256))
+    dt.reshape(-1)[rng.integers(0, 65536, 256)] = rng.standard_normal(256) / 16.0
+    h = 1e-6
+    ad = float(np.sum(g_temp * dt * prob.temp0))
+    lp, _, _ = prob.forward(prob.temp0 * (1 + h * dt), prob.v0)
+    lm, _, _ = prob.forward(prob.temp0 * (1 - h * dt), prob.v0)
+    fd = (lp - lm) / (2 * h)
+    cells = 256 * 256
+    return {
+        "metric": "cell-updates/s and % HBM roofline (FP64); adjoint/forward cost ratio",
+        "value": t_adj / t_fwd, "unit": "adjoint/forward time ratio", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": t_adj / steps * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "adjoint of the periodic ideal-MHD wave 256x256 (RK2, FP64), %d steps, "
+                               "checkpoint every 50; gradient w.r.t. the initial temperature grid and v0" % steps},
+        "forward_s": t_fwd, "adjoint_s": t_adj, "forward_cell_updates_per_s": cells * steps / t_fwd,
+        "loss": loss, "dJ_dv0": g_v0,
+        "fd_probe": {"adjoint": ad, "central_difference": fd, "relative_difference": abs(ad - fd) / (abs(ad) + abs(fd))},
+    }
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,6 +232,9 @@ def main() -> None:
 
     if args.workload == "batch2f":
         print(json.dumps(bench_batch(args)))
+        return
+    if args.workload == "adjoint":
+        print(json.dumps(bench_adjoint(args)))
         return
     st, cfg, spec, desc = make_workload(args.workload)
     cells = spec.xdim * spec.ydim
