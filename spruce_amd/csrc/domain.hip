@@ -625,7 +625,7 @@ struct TcModule : GpuModule {
   double eps = 0.0;
   std::string integ = "euler";
   bool inactive = false;
-  DBuf ta, tb, tc, k1, k2, k3, rx, ry, coef;
+  DBuf ta, tb, tc, k1, k2, k3, rx, ry, coef, dbxx, dbxy, dbyx, dbyy;
   explicit TcModule(const std::vector<std::pair<std::string, std::string>>& o) {
     name = "thermal_conduction";
     bool f1, f2, f3;
@@ -650,10 +650,12 @@ struct TcModule : GpuModule {
   bool needs_bhat() const override { return true; }
   void setup(Domain& d) override {
     ta.alloc(d.N); tb.alloc(d.N); tc.alloc(d.N);
+    dbxx.alloc(d.N); dbxy.alloc(d.N); dbyx.alloc(d.N); dbyy.alloc(d.N);
     if (integ == "rk4") { k1.alloc(d.N); k2.alloc(d.N); k3.alloc(d.N); }
     if (p.saturation) { rx.alloc(d.N); ry.alloc(d.N); coef.alloc(d.N); }
   }
   void pre_launch(Domain& d, double* scal) override {
+    d.timed("bhat_derivs", [&] { launch_bhat_derivs(d.bhx.p, d.bhy.p, d.mesh, dbxx.p, dbxy.p, dbyx.p, dbyy.p, d.stream); });
     d.timed("tc_pre", [&] {
       launch_tc_pre(d.L.view(), d.bhx.p, d.bhy.p, p, d.kc, d.mesh, d.mod_partial.p, solar_blocks(d.mesh), scal,
                     d.stream);
@@ -675,6 +677,7 @@ struct TcModule : GpuModule {
     a.te_src = src; a.te_base = base; a.n = d.L.n.p; a.bhx = d.bhx.p; a.bhy = d.bhy.p;
     a.rx = rx.p; a.ry = ry.p; a.coef = coef.p; a.te_out = out; a.k_out = kout;
     a.k1 = k1.p; a.k2 = k2.p; a.k3 = k3.p; a.h = h; a.mult = mult; a.mode = mode; a.p = p; a.k = d.kc;
+    a.dbxx = dbxx.p; a.dbxy = dbxy.p; a.dbyx = dbyx.p; a.dbyy = dbyy.p;
     a.mesh = d.mesh;
     d.timed("tc_step", [&] { launch_tc_step(a, d.stream); });
   }

@@ -58,6 +58,27 @@ __global__ void __launch_bounds__(256) k_bhat(MhdSet L, MhdStatic st, Mesh M, do
   bhy[c] = bm == 0.0 ? 0.0 : by / bm;
 }
 
+__global__ void __launch_bounds__(256) k_bhat_derivs(const double* bhx, const double* bhy, Mesh M, double* dbxx,
+                                                     double* dbxy, double* dbyx, double* dbyy) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= M.nx || j >= M.ny) return;
+  const size_t c = (size_t)i * M.ny + j;
+  double r[4] = {0.0, 0.0, 0.0, 0.0};
+  if (inside(M, i, j)) {
+    const int ny = M.ny;
+    const int im = wrp(i - 1, M.nx, M.x.per), ip = wrp(i + 1, M.nx, M.x.per);
+    const int jm = wrp(j - 1, M.ny, M.y.per), jp = wrp(j + 1, M.ny, M.y.per);
+    const size_t cxm = (size_t)im * ny + j, cxp = (size_t)ip * ny + j, cym = (size_t)i * ny + jm, cyp = (size_t)i * ny + jp;
+    const double dx = M.x.d[i], rdx = M.x.rd[i], dy = M.y.d[j], rdy = M.y.rd[j];
+    const double bx = bhx[c], by = bhy[c];
+    r[0] = qdiv(fint(M.x, ip, i, bx, bhx[cxp]) - fint(M.x, i, im, bhx[cxm], bx), dx, rdx);
+    r[1] = qdiv(fint(M.y, jp, j, bx, bhx[cyp]) - fint(M.y, j, jm, bhx[cym], bx), dy, rdy);
+    r[2] = qdiv(fint(M.x, ip, i, by, bhy[cxp]) - fint(M.x, i, im, bhy[cxm], by), dx, rdx);
+    r[3] = qdiv(fint(M.y, jp, j, by, bhy[cyp]) - fint(M.y, j, jm, bhy[cym], by), dy, rdy);
+  }
+  dbxx[c] = r[0]; dbxy[c] = r[1]; dbyx[c] = r[2]; dbyy[c] = r[3];
+}
+
 // Field-aligned conductive flux at an interior cell and its saturated magnitude
 // (thermalconduction.cpp:198-232).
 struct SatFlux { double fx, fy, fm, sfm; };
@@ -157,22 +178,13 @@ __global__ void __launch_bounds__(256) k_tc_sat(const double* te, const double* 
   coef[c] = s.fm != 0.0 ? s.sfm / s.fm : 1.0;                  // thermalconduction.cpp:264-268
 }
 
-// ThermalConduction::thermalEnergyDerivative at an interior cell (thermalconduction.cpp:158-176).
-__device__ double tc_energy_derivative(const TcStepArgs& a, int i, int j) {
+// ThermalConduction::thermalEnergyDerivative at an interior cell (thermalconduction.cpp:158-176),
+// from the temperatures of its 3x3 neighbourhood.
+__device__ double tc_energy_derivative(const TcStepArgs& a, int i, int j, const double T[3][3]) {
   const Mesh& M = a.mesh;
-  const MhdConst& K = a.k;
   const int ny = M.ny;
   const int im = wrp(i - 1, M.nx, M.x.per), ip = wrp(i + 1, M.nx, M.x.per);
   const int jm = wrp(j - 1, M.ny, M.y.per), jp = wrp(j + 1, M.ny, M.y.per);
-  double T[3][3];
-#pragma unroll
-  for (int a1 = 0; a1 < 3; ++a1)
-#pragma unroll
-    for (int b1 = 0; b1 < 3; ++b1) {
-      const int ii = a1 == 0 ? im : (a1 == 1 ? i : ip), jj = b1 == 0 ? jm : (b1 == 1 ? j : jp);
-      const size_t q = (size_t)ii * ny + jj;
-      T[a1][b1] = temp_of(a.te_src[q], a.n[q], K);
-    }
   const double Tc = T[1][1];
   // face interpolants of T on the four faces of (i,j)
   const double Txu = fint(M.x, ip, i, Tc, T[2][1]), Txl = fint(M.x, i, im, T[0][1], Tc);
@@ -197,14 +209,11 @@ __device__ double tc_energy_derivative(const TcStepArgs& a, int i, int j) {
   // secondDerivative1D (derivs.cpp:450)
   const double s0 = qdiv(Txu - 2.0 * Tc + Txl, M.x.hsq[i], M.x.rhsq[i]);
   const double s1 = qdiv(Tyu - 2.0 * Tc + Tyl, M.y.hsq[j], M.y.rhsq[j]);
-  // b_hat derivatives
+  // b_hat and its derivatives (frozen for the step, computed once by k_bhat_derivs)
   const size_t c = (size_t)i * ny + j;
   const size_t cxm = (size_t)im * ny + j, cxp = (size_t)ip * ny + j, cym = (size_t)i * ny + jm, cyp = (size_t)i * ny + jp;
   const double bx = a.bhx[c], by = a.bhy[c];
-  const double dbx_x = qdiv(fint(M.x, ip, i, bx, a.bhx[cxp]) - fint(M.x, i, im, a.bhx[cxm], bx), dx, rdx);
-  const double dbx_y = qdiv(fint(M.y, jp, j, bx, a.bhx[cyp]) - fint(M.y, j, jm, a.bhx[cym], bx), dy, rdy);
-  const double dby_x = qdiv(fint(M.x, ip, i, by, a.bhy[cxp]) - fint(M.x, i, im, a.bhy[cxm], by), dx, rdx);
-  const double dby_y = qdiv(fint(M.y, jp, j, by, a.bhy[cyp]) - fint(M.y, j, jm, a.bhy[cym], by), dy, rdy);
+  const double dbx_x = a.dbxx[c], dbx_y = a.dbxy[c], dby_x = a.dbyx[c], dby_y = a.dbyy[c];
 
   const double bdg = bx * gx + by * gy;
   const double t1x = bx * s0 + by * dgx_y, t1y = bx * dgy_x + by * s1;
@@ -226,9 +235,25 @@ __device__ double tc_energy_derivative(const TcStepArgs& a, int i, int j) {
   return result;
 }
 
+// One conduction update.  The block's temperatures and a one-cell halo are computed once into LDS
+// (each temperature is a division), then every cell reads its 3x3 neighbourhood from there.
 __global__ void __launch_bounds__(256) k_tc_step(TcStepArgs a) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  constexpr int BX = 64, BY = 4;
+  __shared__ double Ts[BY + 2][BX + 2];
   const Mesh& M = a.mesh;
+  const int j0 = blockIdx.x * BX, i0 = blockIdx.y * BY;
+  for (int t = threadIdx.y * BX + threadIdx.x; t < (BY + 2) * (BX + 2); t += BX * BY) {
+    const int r = t / (BX + 2), q = t - r * (BX + 2);
+    const int ii = wrp(i0 - 1 + r, M.nx, M.x.per), jj = wrp(j0 - 1 + q, M.ny, M.y.per);
+    double v = 0.0;
+    if (ii >= 0 && ii < M.nx && jj >= 0 && jj < M.ny) {
+      const size_t cc = (size_t)ii * M.ny + jj;
+      v = temp_of(a.te_src[cc], a.n[cc], a.k);
+    }
+    Ts[r][q] = v;
+  }
+  __syncthreads();
+  const int j = j0 + threadIdx.x, i = i0 + threadIdx.y;
   if (i >= M.nx || j >= M.ny) return;
   const size_t c = (size_t)i * M.ny + j;
   const double base = a.te_base[c];
@@ -238,7 +263,12 @@ __global__ void __launch_bounds__(256) k_tc_step(TcStepArgs a) {
     if (a.mode == TC_STORE_K) a.k_out[c] = 0.0;
     return;
   }
-  double dE = tc_energy_derivative(a, i, j);
+  double T[3][3];
+#pragma unroll
+  for (int a1 = 0; a1 < 3; ++a1)
+#pragma unroll
+    for (int b1 = 0; b1 < 3; ++b1) T[a1][b1] = Ts[threadIdx.y + a1][threadIdx.x + b1];
+  double dE = tc_energy_derivative(a, i, j, T);
   double v;
   if (a.mode == TC_RK4_FINAL) {
     const double sum = a.k1[c] + 2.0 * a.k2[c] + 2.0 * a.k3[c] + dE;
@@ -354,6 +384,11 @@ int solar_blocks(const Mesh& m) {
 
 void launch_bhat(MhdSet L, MhdStatic st, Mesh mesh, double* bhx, double* bhy, hipStream_t s) {
   hipLaunchKernelGGL(k_bhat, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, L, st, mesh, bhx, bhy);
+}
+
+void launch_bhat_derivs(const double* bhx, const double* bhy, Mesh mesh, double* dbxx, double* dbxy, double* dbyx,
+                        double* dbyy, hipStream_t s) {
+  hipLaunchKernelGGL(k_bhat_derivs, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, bhx, bhy, mesh, dbxx, dbxy, dbyx, dbyy);
 }
 
 void launch_tc_pre(MhdSet L, const double* bhx, const double* bhy, TcParams p, MhdConst k, Mesh mesh, double* partial,

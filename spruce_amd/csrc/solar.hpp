@@ -41,6 +41,10 @@ struct TcStepArgs {
   const double* k1;
   const double* k2;
   const double* k3;
+  const double* dbxx;     // derivatives of the frozen b_hat (d b_hat_x/dx, d b_hat_x/dy, d b_hat_y/dx,
+  const double* dbxy;     // d b_hat_y/dy), computed once per step
+  const double* dbyx;
+  const double* dbyy;
   double h, mult;
   int mode;
   TcParams p;
@@ -50,6 +54,10 @@ struct TcStepArgs {
 
 // b_hat of the live grids (idealmhd.cpp:259-277).
 void launch_bhat(MhdSet L, MhdStatic st, Mesh mesh, double* bhx, double* bhy, hipStream_t s);
+// derivative1D of the frozen b_hat along both axes (0 outside the interior): the conduction
+// subcycles of a step all use the same values (thermalconduction.cpp:158-176).
+void launch_bhat_derivs(const double* bhx, const double* bhy, Mesh mesh, double* dbxx, double* dbxy, double* dbyx,
+                        double* dbyy, hipStream_t s);
 // TC subcycle count inputs: min over the interior of the conduction time scale and, with saturation,
 // max |b_hat . grad T| (thermalconduction.cpp:179-193).  Writes out[0] = min, out[1] = max.
 void launch_tc_pre(MhdSet L, const double* bhx, const double* bhy, TcParams p, MhdConst k, Mesh mesh,
