@@ -1,6 +1,6 @@
 """Time-step throughput of the MI355X path: cell-updates/s and the HBM roofline of the stage kernel.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload arcade|periodic|periodic2048|arcade2048|batch2f|adjoint]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload arcade|periodic|periodic2048|arcade2048|batch2f|adjoint|copy]
 
 One "step" is one PlasmaDomain::advanceTime (evolution.cpp:59-82) over the whole grid, ghost cells
 included: every integrator stage, every module subcycle, the ghost fill and the dt reduction.
@@ -212,6 +212,34 @@ def bench_adjoint(args) -> dict:
     }
 
 
+def bench_copy(args) -> dict:
+    """HBM calibration: one elementwise kernel (torch.neg) reading and writing 1 GiB of FP64.  Its
+    FETCH_SIZE / WRITE_SIZE counters per dispatch, against the known 1 GiB each way, give the factor
+    that turns those counters into bytes for the stage kernels; its rate is the achievable copy
+    bandwidth of this GPU."""
+    import torch
+
+    n = 1 << 27
+    a = torch.rand(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(args.warmup):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        torch.neg(a, out=b)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    gbs = 2.0 * n * 8 / (ms * 1e-3) / 1e9
+    return {"metric": "HBM copy bandwidth (calibration)", "value": gbs, "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "elementwise FP64 negate, 1 GiB read + 1 GiB write per launch"},
+            "bytes_read": n * 8, "bytes_written": n * 8, "frac_of_peak": gbs / HBM_PEAK_GBS}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,7 +248,7 @@ def main() -> None:
     ap.add_argument("--workload", default="arcade")
     ap.add_argument("--cpu-steps", type=int, default=0, help="reference steps for the CPU baseline (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--stage-kernel", default="cell", choices=["strip", "cell"])
+    ap.add_argument("--stage-kernel", default="strip", choices=["strip", "cell"])
     ap.add_argument("--strip-rows", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -235,6 +263,9 @@ def main() -> None:
         return
     if args.workload == "adjoint":
         print(json.dumps(bench_adjoint(args)))
+        return
+    if args.workload == "copy":
+        print(json.dumps(bench_copy(args)))
         return
     st, cfg, spec, desc = make_workload(args.workload)
     cells = spec.xdim * spec.ydim

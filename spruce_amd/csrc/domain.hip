@@ -124,7 +124,7 @@ class Domain : public DomainBase {
   int kinds_all[4] = {0, 0, 0, 0}, kinds_ucnp[4] = {0, 0, 0, 0}, kinds_live[4] = {0, 0, 0, 0};
   double s1[4] = {1, 1, 1, 1}, s2[4] = {1, 1, 1, 1};
   bool any_open = false;
-  int stage_kernel = 1;   // 0: wavefront strips (mhd2.hip), 1: one thread per cell (mhd.hip)
+  int stage_kernel = 0;   // 0: wavefront strips (mhd2.hip), 1: one thread per cell (mhd.hip)
   int strip_rows = 0;     // rows per wavefront strip; 0 = chosen from the grid size
 
   void set_option(const std::string& k, const std::string& v) override {
@@ -351,8 +351,9 @@ class Domain : public DomainBase {
       sync_and_check();
       const double dt = sc[2 * mods.size()];
       for (size_t k = 0; k < mods.size(); ++k) mods[k]->pre_finish(dt, sc.data() + 2 * k);
-      for (size_t k = 0; k < mods.size(); ++k) sub_hist[(size_t)s * mods.size() + k] = mods[k]->nsub;
       for (auto& m : mods) m->iterate(*this, dt);
+      // counts are recorded after iterate(): viscosity modules fix theirs there (viscosity.cpp:125-180)
+      for (size_t k = 0; k < mods.size(); ++k) sub_hist[(size_t)s * mods.size() + k] = mods[k]->nsub;
     } else {
       for (size_t k = 0; k < mods.size(); ++k) sub_hist[(size_t)s * mods.size() + k] = mods[k]->nsub;
     }

@@ -318,11 +318,12 @@ __global__ void __launch_bounds__(64) k_mhd_stage_strip(StageArgs a, int R) {
 }  // namespace
 
 int mhd_strip_rows_default(int nx, int ny) {
-  // enough wavefronts to give every CU several, with at most one strip chunk of 4-row steps
+  // the tallest strip that still leaves about 32 wavefronts per CU (256 CUs); shorter strips re-read
+  // more halo rows, but on 2048x2048 and 512x1024 the extra waves hide more latency than that costs
   const long tiles = (ny + OUT_COLS - 1) / OUT_COLS;
   for (int R : {32, 16, 8}) {
     const long waves = tiles * ((nx + R - 1) / R);
-    if (waves >= 4096) return R;
+    if (waves >= 8192) return R;
   }
   return 4;
 }
