@@ -34,6 +34,19 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 E_MHD, S_MHD = 8, 5
 
 
+def measured_traffic(workload: str, kernel: str, rows: str):
+    """HBM bytes per stage launch from the committed FETCH_SIZE/WRITE_SIZE counter summary
+    (profiles/hbm_traffic_mhd_stage.md), for the same workload and launch shape; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_mhd_stage.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 5 and f[0] == "traffic" and f[1:4] == [workload, kernel, rows]:
+            return float(f[4])
+    return None
+
+
 def stage_bytes_per_cell(integrator: str) -> float:
     """Average algorithmic bytes per cell of one stage launch."""
     if integrator == "rk2":     # (E+S read, E write) + (2E+S read, E write) over 2 launches
@@ -158,7 +171,8 @@ def bench_batch(args) -> dict:
                                "FP64), one batch, graph replay, autotuned block",
                    "domains": len(specs), "cells": cells, "edges": edges, "block": shape},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "tf_stage",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "tf_stage",
                      "kernel_ms": stage_ms, "stage_bytes_per_cell": bpc},
         "sequential": {"value": seq_value, "steps": seq_steps, "batched_over_sequential": value / seq_value},
         "autotune_ms_per_step": tune_ms,
@@ -312,7 +326,8 @@ def main() -> None:
         "data": "synthetic",
         "config": dict(desc, stage_kernel=args.stage_kernel, strip_rows=args.strip_rows or "auto"),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload, args.stage_kernel, str(args.strip_rows or "auto")),
                      "kernel": "mhd_stage", "kernel_ms": stage_ms,
                      "stage_bytes_per_cell": stage_bytes_per_cell(integ),
                      "step_bytes_per_cell_update": b_alg,
