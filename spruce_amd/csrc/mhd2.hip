@@ -39,13 +39,6 @@ __device__ __forceinline__ double qdiv_d(double a, double b, double rb) {
   return a / b;
 }
 
-struct Face {
-  double P[NQ];   // surf * face velocity (derivs.cpp:155)
-  double iv[3];   // interpolated v_x, v_y, v_z
-  double ip;      // interpolated press
-  double ib0, ib1; // interpolated bi (the other two components than the axis's curl needs)
-};
-
 // Geometry of one face: interpolation weights and denominators, and the extrapolation geometry on
 // both sides (derivs.cpp:477-499).
 struct FaceGeom {
@@ -79,39 +72,40 @@ __device__ __forceinline__ FaceGeom face_geom(const Axis& ax, int c) {
   return g;
 }
 
-// One face from the four cells c-2, c-1 (lo), c (up), c+1 along its axis (derivs.cpp:10-73, 155).
+// Face velocity of one face: interpolated v and press, and the upwind side (derivs.cpp:10-73, 155).
+struct FaceDir {
+  double vf, ge, de, rde;
+  bool pos, moving;
+};
+
 template <int AX>
-__device__ __forceinline__ void make_face(const double* qm2, const double* qlo, const double* qup, const double* qp1,
-                                          const double* vlo, const double* vup, double plo, double pup,
-                                          const FaceGeom& g, Face& F) {
-  auto interp = [&](double a, double b) { return qdiv(a * g.hup + b * g.hlo, g.den, g.rden); };
+__device__ __forceinline__ FaceDir face_velocity(const double* vlo, const double* vup, double plo, double pup,
+                                                 const FaceGeom& g, double* iv, double& ip) {
 #pragma unroll
-  for (int k = 0; k < 3; ++k) F.iv[k] = interp(vlo[k], vup[k]);
-  F.ip = interp(plo, pup);
-  const double vf = F.iv[AX];
-  const bool pos = vf > 0.0, neg = vf < 0.0;
-  const double ge = pos ? g.gup : g.gdn;
-  const double de = pos ? g.denm : g.denp;
-  const double rde = pos ? g.rdenm : g.rdenp;
-#pragma unroll
-  for (int k = 0; k < NQ; ++k) {
-    const double q0 = qm2[k], q1 = qlo[k], q2 = qup[k], q3 = qp1[k];
-    const double d2 = interp(q1, q2);
-    const double ea = pos ? q0 : q3, eb = pos ? q1 : q2;
-    const double d1 = ea + qdiv((eb - ea) * ge, de, rde);
-    const bool le = q2 <= q1;
-    const bool outer_min = pos ? le : !le;
-    const double inner = outer_min ? smax(d1, d2) : smin(d1, d2);
-    const double lim = outer_min ? smin(eb, inner) : smax(eb, inner);
-    F.P[k] = ((pos || neg) ? lim : d2) * vf;
-    if (AX == 0) {            // x-faces feed derivative1D(bi_y, 0) and derivative1D(bi_z, 0)
-      if (k == Q_BIY) F.ib0 = d2;
-      if (k == Q_BIZ) F.ib1 = d2;
-    } else {                  // y-faces feed derivative1D(bi_x, 1) and derivative1D(bi_z, 1)
-      if (k == Q_BIX) F.ib0 = d2;
-      if (k == Q_BIZ) F.ib1 = d2;
-    }
-  }
+  for (int k = 0; k < 3; ++k) iv[k] = qdiv(vlo[k] * g.hup + vup[k] * g.hlo, g.den, g.rden);
+  ip = qdiv(plo * g.hup + pup * g.hlo, g.den, g.rden);
+  FaceDir d;
+  d.vf = iv[AX];
+  d.pos = d.vf > 0.0;
+  d.moving = d.pos || d.vf < 0.0;
+  d.ge = d.pos ? g.gup : g.gdn;
+  d.de = d.pos ? g.denm : g.denp;
+  d.rde = d.pos ? g.rdenm : g.rdenp;
+  return d;
+}
+
+// surf * face velocity * (Barton-limited upwind value) of one transported quantity, from the four
+// cells c-2, c-1 (lo), c (up), c+1 along the face's axis; d2 = the plain interpolation.
+__device__ __forceinline__ double face_flux(double q0, double q1, double q2, double q3, const FaceGeom& g,
+                                            const FaceDir& fd, double& d2) {
+  d2 = qdiv(q1 * g.hup + q2 * g.hlo, g.den, g.rden);
+  const double ea = fd.pos ? q0 : q3, eb = fd.pos ? q1 : q2;
+  const double d1 = ea + qdiv((eb - ea) * fd.ge, fd.de, fd.rde);
+  const bool le = q2 <= q1;
+  const bool outer_min = fd.pos ? le : !le;
+  const double inner = outer_min ? smax(d1, d2) : smin(d1, d2);
+  const double lim = outer_min ? smin(eb, inner) : smax(eb, inner);
+  return (fd.moving ? lim : d2) * fd.vf;
 }
 
 __device__ __forceinline__ void load_row(const StageArgs& a, int i, bool ivalid, int j, bool jvalid, double* q) {
@@ -145,24 +139,37 @@ __device__ __forceinline__ void derive(const double* q, const MhdConst& K, doubl
   p = q[Q_TE] * K.gm1;
 }
 
+// Strip of a workgroup.  Workgroups go to the 8 XCDs round-robin by launch index (index % 8), and
+// each XCD has its own L2: launch index L is mapped so that XCD x takes the x-th eighth of the
+// strips in column-major order, so strips that run together on one XCD are neighbours along x and
+// share their halo rows in that XCD's L2.
+__device__ __forceinline__ void strip_of(int L, int nstrips, int total, int& col, int& strip) {
+  const int xcd = L & 7, k = L >> 3;
+  const int q = total >> 3, rem = total & 7;
+  const int t = xcd * q + (xcd < rem ? xcd : rem) + k;
+  col = t / nstrips;
+  strip = t - col * nstrips;
+}
+
 template <int MODE>
-__global__ void __launch_bounds__(64) k_mhd_stage_strip(StageArgs a, int R) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+k_mhd_stage_strip(StageArgs a, int R, int nstrips) {
   const Mesh& M = a.mesh;
   const MhdConst& K = a.k;
   const int nx = M.nx, ny = M.ny;
   const int lane = threadIdx.x;
-  const int jraw = blockIdx.x * OUT_COLS + lane - 2;
+  int col, strip;
+  strip_of(blockIdx.x, nstrips, gridDim.x, col, strip);
+  const int jraw = col * OUT_COLS + lane - 2;
   int j;
   bool jvalid;
   if (M.y.per) { j = wrapn(jraw, ny); jvalid = true; }
   else { jvalid = jraw >= 0 && jraw < ny; j = clampn(jraw, ny); }
   const bool out_lane = lane >= 2 && lane < 62 && jraw >= 0 && jraw < ny;
   const bool yint = jvalid && j >= M.y.lo && j <= M.y.hi;
-  const FaceGeom gy = face_geom(M.y, j);          // this lane's lower y-face (between j-1 and j)
-  const double dy = M.y.d[j], rdy = M.y.rd[j];
   const double stepv = a.step_mult * (*a.step);
   const size_t N = (size_t)nx * ny;
-  const int r0 = blockIdx.y * R;
+  const int r0 = strip * R;
 
   auto row_of = [&](int r, bool& ok) -> int {
     if (M.x.per) { ok = true; return wrapn(r, nx); }
@@ -170,9 +177,10 @@ __global__ void __launch_bounds__(64) k_mhd_stage_strip(StageArgs a, int R) {
     return clampn(r, nx);
   };
 
-  double q[4][NQ];
-  double v[2][3], p[2];
-  Face f[2];
+  double q[4][NQ];      // rotating window of rows r-1 .. r+2
+  double v[2][3], p[2];  // v and press of rows r and r+1
+  double Px[NQ];         // flux of row r's lower x-face (the upper face of row r-1)
+  double xiv[2][3], xip[2], xib0[2], xib1[2];   // interpolated values of the lower/upper x-face
 
   {  // prologue: rows r0-2 .. r0+1 and the lower face of row r0
     bool ok;
@@ -185,7 +193,14 @@ __global__ void __launch_bounds__(64) k_mhd_stage_strip(StageArgs a, int R) {
     derive(q[0], K, vt, pt);
     derive(q[1], K, v[0], p[0]);
     const FaceGeom gx = face_geom(M.x, r0);
-    make_face<0>(q[3], q[0], q[1], q[2], vt, v[0], pt, p[0], gx, f[0]);
+    const FaceDir fd = face_velocity<0>(vt, v[0], pt, p[0], gx, xiv[0], xip[0]);
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      double d2;
+      Px[k] = face_flux(q[3][k], q[0][k], q[1][k], q[2][k], gx, fd, d2);
+      if (k == Q_BIY) xib0[0] = d2;
+      if (k == Q_BIZ) xib1[0] = d2;
+    }
   }
 
   auto step = [&](auto BC, int r) {
@@ -193,54 +208,68 @@ __global__ void __launch_bounds__(64) k_mhd_stage_strip(StageArgs a, int R) {
     constexpr int SM = B % 4, S0 = (B + 1) % 4, S1 = (B + 2) % 4, S2 = (B + 3) % 4;
     constexpr int D0 = B % 2, D1 = (B + 1) % 2;
     bool ok;
+    // row r+2 is loaded first and arrives while row r's y-faces are computed
     const int i2 = row_of(r + 2, ok);
     load_row(a, i2, ok, j, jvalid, q[S2]);
-    derive(q[S1], K, v[D1], p[D1]);
-    const FaceGeom gx = face_geom(M.x, r + 1);
-    make_face<0>(q[SM], q[S0], q[S1], q[S2], v[D0], v[D1], p[D0], p[D1], gx, f[D1]);
     if (r >= nx) return;
-    // lower y-face of this lane's column in row r: cells j-2, j-1, j, j+1 from neighbouring lanes
-    double qm2[NQ], qm1[NQ], qp1[NQ], vm1[3];
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      qm1[k] = shr1(q[S0][k]);
-      qm2[k] = shr1(qm1[k]);
-      qp1[k] = shl1(q[S0][k]);
-    }
+    const int i = row_of(r, ok);
+    // lower y-face of this lane's column in row r (cells j-2, j-1, j, j+1 from neighbouring lanes),
+    // one transported quantity at a time: the upper y-face of the column is the next lane's lower one
+    double fyiv[3], fyip, fyib0 = 0.0, fyib1 = 0.0;
+    double vm1[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) vm1[k] = shr1(v[D0][k]);
-    const double pm1 = shr1(p[D0]);
-    Face fy;
-    make_face<1>(qm2, qm1, q[S0], qp1, vm1, v[D0], pm1, p[D0], gy, fy);
-    // the upper y-face of this column is the lower face of the next lane
-    double Pu[NQ];
+    // geometry of this lane's lower y-face (between j-1 and j), re-read per row (cached) rather
+    // than held in registers for the whole strip
+    const FaceGeom gy = face_geom(M.y, j);
+    const double dy = M.y.d[j], rdy = M.y.rd[j];
+    const FaceDir fdy = face_velocity<1>(vm1, v[D0], shr1(p[D0]), p[D0], gy, fyiv, fyip);
+    double tdiv[NQ];       // y part first; the x part is added when the upper x-face is known
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) Pu[k] = shl1(fy.P[k]);
-    const double ivu0 = shl1(fy.iv[0]), ivu1 = shl1(fy.iv[1]), ivu2 = shl1(fy.iv[2]);
-    const double ipu = shl1(fy.ip), ibu0 = shl1(fy.ib0), ibu1 = shl1(fy.ib1);
+    for (int k = 0; k < NQ; ++k) {
+      const double qc_k = q[S0][k];
+      const double qm1 = shr1(qc_k);
+      const double qm2 = shr1(qm1);
+      const double qp1 = shl1(qc_k);
+      double d2;
+      const double Pl = face_flux(qm2, qm1, qc_k, qp1, gy, fdy, d2);
+      // y-faces feed derivative1D(bi_x, 1) and derivative1D(bi_z, 1)
+      if (k == Q_BIX) fyib0 = d2;
+      if (k == Q_BIZ) fyib1 = d2;
+      tdiv[k] = qdiv(shl1(Pl) - Pl, dy, rdy);
+    }
+    const double ivu0 = shl1(fyiv[0]), ivu1 = shl1(fyiv[1]), ivu2 = shl1(fyiv[2]);
+    const double ipu = shl1(fyip), ibu0 = shl1(fyib0), ibu1 = shl1(fyib1);
+    // upper x-face of row r from rows r-1 .. r+2; divergence term (up - lo)/dx + y term
+    derive(q[S1], K, v[D1], p[D1]);
+    const double dx = M.x.d[i], rdx = M.x.rd[i];
+    {
+      const FaceGeom gx = face_geom(M.x, r + 1);
+      const FaceDir fd = face_velocity<0>(v[D0], v[D1], p[D0], p[D1], gx, xiv[D1], xip[D1]);
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) {
+        double d2;
+        const double Pn = face_flux(q[SM][k], q[S0][k], q[S1][k], q[S2][k], gx, fd, d2);
+        // x-faces feed derivative1D(bi_y, 0) and derivative1D(bi_z, 0)
+        if (k == Q_BIY) xib0[D1] = d2;
+        if (k == Q_BIZ) xib1[D1] = d2;
+        const double tx = qdiv(Pn - Px[k], dx, rdx);
+        tdiv[k] = tx + tdiv[k];
+        Px[k] = Pn;
+      }
+    }
     if (!out_lane) return;
-    const int i = row_of(r, ok);
     const size_t c = (size_t)i * ny + j;
     double k8[8];
     if (i >= M.x.lo && i <= M.x.hi && yint) {
-      const double dx = M.x.d[i], rdx = M.x.rd[i];
-      const Face& xl = f[D0];
-      const Face& xu = f[D1];
-      double tdiv[NQ];
-#pragma unroll
-      for (int k = 0; k < NQ; ++k) {
-        const double tx = qdiv(xu.P[k] - xl.P[k], dx, rdx);
-        const double ty = qdiv(Pu[k] - fy.P[k], dy, rdy);
-        tdiv[k] = tx + ty;
-      }
       auto ddx = [&](double up, double lo) { return qdiv(up - lo, dx, rdx); };
       auto ddy = [&](double up, double lo) { return qdiv(up - lo, dy, rdy); };
-      const double dpx = ddx(xu.ip, xl.ip), dpy = ddy(ipu, fy.ip);
-      const double dvx_x = ddx(xu.iv[0], xl.iv[0]), dvx_y = ddy(ivu0, fy.iv[0]);
-      const double dvy_x = ddx(xu.iv[1], xl.iv[1]), dvy_y = ddy(ivu1, fy.iv[1]);
-      const double dvz_x = ddx(xu.iv[2], xl.iv[2]), dvz_y = ddy(ivu2, fy.iv[2]);
-      const double dbiy_x = ddx(xu.ib0, xl.ib0), dbix_y = ddy(ibu0, fy.ib0);
-      const double dbiz_x = ddx(xu.ib1, xl.ib1), dbiz_y = ddy(ibu1, fy.ib1);
+      const double dpx = ddx(xip[D1], xip[D0]), dpy = ddy(ipu, fyip);
+      const double dvx_x = ddx(xiv[D1][0], xiv[D0][0]), dvx_y = ddy(ivu0, fyiv[0]);
+      const double dvy_x = ddx(xiv[D1][1], xiv[D0][1]), dvy_y = ddy(ivu1, fyiv[1]);
+      const double dvz_x = ddx(xiv[D1][2], xiv[D0][2]), dvz_y = ddy(ivu2, fyiv[2]);
+      const double dbiy_x = ddx(xib0[D1], xib0[D0]), dbix_y = ddy(ibu0, fyib0);
+      const double dbiz_x = ddx(xib1[D1], xib1[D0]), dbiz_y = ddy(ibu1, fyib1);
       const double* qc = q[S0];
       const double rho = qc[Q_RHO];
       const double press = p[D0];
@@ -329,10 +358,14 @@ int mhd_strip_rows_default(int nx, int ny) {
 }
 
 void launch_mhd_stage_strip(const StageArgs& a, int R, hipStream_t s) {
-  dim3 grid((a.mesh.ny + OUT_COLS - 1) / OUT_COLS, (a.mesh.nx + R - 1) / R);
-  if (a.mode == STAGE_STORE_K) hipLaunchKernelGGL(k_mhd_stage_strip<STAGE_STORE_K>, grid, dim3(64), 0, s, a, R);
-  else if (a.mode == STAGE_RK4_FINAL) hipLaunchKernelGGL(k_mhd_stage_strip<STAGE_RK4_FINAL>, grid, dim3(64), 0, s, a, R);
-  else hipLaunchKernelGGL(k_mhd_stage_strip<STAGE_PLAIN>, grid, dim3(64), 0, s, a, R);
+  const int cols = (a.mesh.ny + OUT_COLS - 1) / OUT_COLS, nstrips = (a.mesh.nx + R - 1) / R;
+  const dim3 grid(cols * nstrips);
+  if (a.mode == STAGE_STORE_K)
+    hipLaunchKernelGGL(k_mhd_stage_strip<STAGE_STORE_K>, grid, dim3(64), 0, s, a, R, nstrips);
+  else if (a.mode == STAGE_RK4_FINAL)
+    hipLaunchKernelGGL(k_mhd_stage_strip<STAGE_RK4_FINAL>, grid, dim3(64), 0, s, a, R, nstrips);
+  else
+    hipLaunchKernelGGL(k_mhd_stage_strip<STAGE_PLAIN>, grid, dim3(64), 0, s, a, R, nstrips);
 }
 
 }  // namespace spruce

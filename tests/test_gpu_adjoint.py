@@ -33,6 +33,11 @@ def _loss(grids, base, cfg, nsteps, w):
 
 @pytest.mark.parametrize("integ,nsteps,ckpt", [("rk2", 24, 5), ("euler", 16, 16), ("rk2", 9, 1)])
 def test_adjoint_matches_finite_differences(integ, nsteps, ckpt):
+    """dJ/dstate . d against central differences at two step sizes (relative 1e-7 and 1e-8 along
+    seeded random directions).  The loss is only piecewise smooth: the limiter's min/max selections
+    switch where two compared values cross, and a direction whose two differences disagree has
+    crossed such a switch inside the difference interval, where no single derivative describes it.
+    Those directions are counted and left out; at least four of six must remain."""
     print("adjoint", integ, nsteps, ckpt, flush=True)
     from tests.golden.cases import random_mhd_state
     base = random_mhd_state(40, 48, 91, stretch=True)
@@ -50,21 +55,23 @@ def test_adjoint_matches_finite_differences(integ, nsteps, ckpt):
         d.close()
     scale = {"rho": 1e-15, "temp": 1e6, "mom_x": 1e-8, "mom_y": 1e-8, "mom_z": 1e-8, "bi_x": 1.0, "bi_y": 1.0,
              "bi_z": 1.0}
-    worst = 0.0
+    smooth, worst = 0, 0.0
     for trial in range(6):
         dirs = {v: rng.uniform(-1, 1, (40, 48)) * scale[v] for v in STATE}
         ad = sum(float(np.sum(g[k] * dirs[v])) for k, v in enumerate(STATE))
-        best = 1.0
-        for eps in (1e-6, 1e-5):
-            gp = dict(base.grids)
-            gm = dict(base.grids)
-            for v in STATE:
-                gp[v] = base.grids[v] + eps * dirs[v]
-                gm[v] = base.grids[v] - eps * dirs[v]
-            fd = (_loss(gp, base, cfg, nsteps, w) - _loss(gm, base, cfg, nsteps, w)) / (2 * eps)
-            best = min(best, abs(ad - fd) / (abs(ad) + abs(fd)))
-        print("direction", trial, "relative difference", best, flush=True)
-        worst = max(worst, best)
+        fds = []
+        for eps in (1e-7, 1e-8):
+            gp = {v: base.grids[v] + eps * dirs[v] if v in dirs else base.grids[v] for v in base.grids}
+            gm = {v: base.grids[v] - eps * dirs[v] if v in dirs else base.grids[v] for v in base.grids}
+            fds.append((_loss(gp, base, cfg, nsteps, w) - _loss(gm, base, cfg, nsteps, w)) / (2 * eps))
+        consistent = abs(fds[0] - fds[1]) <= 1e-6 * abs(fds[0])
+        rel = abs(ad - fds[0]) / (abs(ad) + abs(fds[0]))
+        print("direction", trial, "adjoint %.12e fd(1e-7) %.12e fd(1e-8) %.12e rel %.2e%s"
+              % (ad, fds[0], fds[1], rel, "" if consistent else " (branch crossed)"), flush=True)
+        if consistent:
+            smooth += 1
+            worst = max(worst, rel)
+    assert smooth >= 4
     assert worst < 1e-6
 
 
