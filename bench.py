@@ -47,6 +47,14 @@ def measured_traffic(workload: str, kernel: str, rows: str):
     return None
 
 
+def resolved_stage_kernel(option: str, nx: int, ny: int) -> str:
+    """The stage kernel the library runs for --stage-kernel auto: strips when 4-row strips of 60
+    output columns give at least 1024 wavefronts, else one thread per cell."""
+    if option != "auto":
+        return option
+    return "strip" if ((ny + 59) // 60) * ((nx + 3) // 4) >= 1024 else "cell"
+
+
 def stage_bytes_per_cell(integrator: str) -> float:
     """Average algorithmic bytes per cell of one stage launch."""
     if integrator == "rk2":     # (E+S read, E write) + (2E+S read, E write) over 2 launches
@@ -262,7 +270,7 @@ def main() -> None:
     ap.add_argument("--workload", default="arcade")
     ap.add_argument("--cpu-steps", type=int, default=0, help="reference steps for the CPU baseline (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--stage-kernel", default="strip", choices=["strip", "cell"])
+    ap.add_argument("--stage-kernel", default="auto", choices=["auto", "strip", "cell"])
     ap.add_argument("--strip-rows", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -283,6 +291,7 @@ def main() -> None:
         return
     st, cfg, spec, desc = make_workload(args.workload)
     cells = spec.xdim * spec.ydim
+    kernel = resolved_stage_kernel(args.stage_kernel, spec.xdim, spec.ydim)
     opts = {"stage_kernel": args.stage_kernel}
     if args.strip_rows:
         opts["strip_rows"] = args.strip_rows
@@ -324,10 +333,10 @@ def main() -> None:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": dict(desc, stage_kernel=args.stage_kernel, strip_rows=args.strip_rows or "auto"),
+        "config": dict(desc, stage_kernel=kernel, strip_rows=args.strip_rows or "auto"),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.workload, args.stage_kernel, str(args.strip_rows or "auto")),
+                     "traffic": measured_traffic(args.workload, kernel, str(args.strip_rows or "auto")),
                      "kernel": "mhd_stage", "kernel_ms": stage_ms,
                      "stage_bytes_per_cell": stage_bytes_per_cell(integ),
                      "step_bytes_per_cell_update": b_alg,

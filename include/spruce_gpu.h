@@ -105,8 +105,9 @@ int spruce_gpu_stream(spruce_gpu_domain* d, void** hip_stream);
 int spruce_gpu_device_grid(spruce_gpu_domain* d, const char* name, double** dev_ptr);
 
 /* Execution options (no reference counterpart; results are identical for every setting):
- *   "stage_kernel" = "strip" (default: wavefront strips marching along x) | "cell" (one thread per
- *                    cell, faces recomputed per cell);
+ *   "stage_kernel" = "auto" (default: strips when the grid gives at least 1024 4-row strip
+ *                    wavefronts, else one thread per cell) | "strip" (wavefront strips marching
+ *                    along x) | "cell" (one thread per cell, faces recomputed per cell);
  *   "strip_rows"   = rows per wavefront strip, a multiple of 4 (default: chosen from the grid size);
  *   "block"        = "<columns>x<rows>" thread block of the ideal_2F kernels (default 64x4). */
 int spruce_gpu_set_option(spruce_gpu_domain* d, const char* key, const char* value);

@@ -145,12 +145,18 @@ __global__ void __launch_bounds__(256) k_adj_dt_blocks(MhdSet L, MhdStatic st, M
 __global__ void __launch_bounds__(64) k_adj_dt_apply(MhdSet L, MhdStatic st, MhdConst K, Mesh M, double eps,
                                                      const double* gh, const double* pmin, const int* pidx, int nb,
                                                      double* lam_L) {
-  if (threadIdx.x != 0) return;
+  // (min, lowest index) over the block results: one wavefront, then a shuffle reduction; the rule
+  // is order-independent, so the cell is the same however the blocks are visited
   double v = DBL_MAX;
   int idx = -1;
-  for (int k = 0; k < nb; ++k)
+  for (int k = threadIdx.x; k < nb; k += 64)
     if (pmin[k] < v || (pmin[k] == v && pidx[k] >= 0 && (idx < 0 || pidx[k] < idx))) { v = pmin[k]; idx = pidx[k]; }
-  if (idx < 0) return;
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(v, off, 64);
+    const int oi = __shfl_xor(idx, off, 64);
+    if (ov < v || (ov == v && oi >= 0 && (idx < 0 || oi < idx))) { v = ov; idx = oi; }
+  }
+  if (threadIdx.x != 0 || idx < 0) return;
   const int i = idx / M.ny, j = idx - i * M.ny;
   const Dual8 dt = dt_cell_dual(L, st, K, M, i, j);
   const size_t N = (size_t)M.nx * M.ny;

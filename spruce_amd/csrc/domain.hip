@@ -124,13 +124,13 @@ class Domain : public DomainBase {
   int kinds_all[4] = {0, 0, 0, 0}, kinds_ucnp[4] = {0, 0, 0, 0}, kinds_live[4] = {0, 0, 0, 0};
   double s1[4] = {1, 1, 1, 1}, s2[4] = {1, 1, 1, 1};
   bool any_open = false;
-  int stage_kernel = 0;   // 0: wavefront strips (mhd2.hip), 1: one thread per cell (mhd.hip)
+  int stage_kernel = -1;  // 0: wavefront strips (mhd2.hip), 1: one thread per cell (mhd.hip), -1: by size
   int strip_rows = 0;     // rows per wavefront strip; 0 = chosen from the grid size
 
   void set_option(const std::string& k, const std::string& v) override {
     if (k == "stage_kernel") {
-      require(v == "strip" || v == "cell", SPRUCE_E_ARG, "stage_kernel must be strip or cell");
-      stage_kernel = v == "cell" ? 1 : 0;
+      require(v == "strip" || v == "cell" || v == "auto", SPRUCE_E_ARG, "stage_kernel must be strip, cell or auto");
+      stage_kernel = v == "cell" ? 1 : (v == "strip" ? 0 : -1);
     } else if (k == "strip_rows") {
       const int r = std::stoi(v);
       require(r >= 4 && r <= 256 && r % 4 == 0, SPRUCE_E_ARG, "strip_rows must be a multiple of 4 in [4, 256]");
@@ -296,6 +296,7 @@ class Domain : public DomainBase {
     upload(scratch, host_grids["temp"]);
 
     if (strip_rows == 0) strip_rows = mhd_strip_rows_default(nx, ny);
+    if (stage_kernel < 0) stage_kernel = mhd_stage_kernel_default(nx, ny);
     dt_blocks = mhd_dt_blocks(mesh);
     partial.alloc(dt_blocks);
     step.alloc(1);
@@ -363,15 +364,29 @@ class Domain : public DomainBase {
   MhdStatic stat() { return MhdStatic{bex.p, bey.p, bez.p, gx.p, gy.p}; }
 
   void stage(SetBufs& S, SetBufs& B, SetBufs& T, double mult, int mode, int kslot, const double* step_ptr = nullptr) {
+    StageArgs a = stage_args(S.view(), B.view(), T.view(), mult, mode, kslot, step_ptr, nullptr);
+    for (auto& m : mods) m->rhs_terms(*this, S, a);
+    launch_stage(a);
+  }
+  // a stage on plain set views, for sets that live in a caller's buffer (no module terms)
+  void stage_view(const MhdSet& S, const MhdSet& B, const MhdSet& T, double mult, int mode, const double* step_ptr,
+                  double* kout) {
+    require(mods.empty(), SPRUCE_E_UNSUPPORTED, "module terms need the domain's own sets");
+    launch_stage(stage_args(S, B, T, mult, mode, 0, step_ptr, kout));
+  }
+  StageArgs stage_args(const MhdSet& S, const MhdSet& B, const MhdSet& T, double mult, int mode, int kslot,
+                       const double* step_ptr, double* kout) {
     StageArgs a{};
-    a.S = S.view(); a.B = B.view(); a.T = T.view();
+    a.S = S; a.B = B; a.T = T;
     a.st = stat(); a.k = kc; a.mesh = mesh;
     a.step = step_ptr ? step_ptr : step.p; a.step_mult = mult; a.mode = mode;
     a.write_pre = any_open ? 1 : 0;
     a.write_n = 1;
-    if (mode == STAGE_STORE_K) a.kbuf = kbuf[kslot].p;
+    if (mode == STAGE_STORE_K) a.kbuf = kout ? kout : kbuf[kslot].p;
     if (mode == STAGE_RK4_FINAL) { a.kbuf = kbuf[0].p; a.k2buf = kbuf[1].p; a.k3buf = kbuf[2].p; }
-    for (auto& m : mods) m->rhs_terms(*this, S, a);
+    return a;
+  }
+  void launch_stage(const StageArgs& a) {
     if (stage_kernel == 1) timed("mhd_stage", [&] { launch_mhd_stage(a, stream); });
     else timed("mhd_stage", [&] { launch_mhd_stage_strip(a, strip_rows, stream); });
   }
@@ -510,6 +525,16 @@ class Domain : public DomainBase {
     for (size_t k = 0; k < a.size(); ++k)
       hip_check(hipMemcpyAsync(b[k]->p, a[k]->p, N * 8, hipMemcpyDeviceToDevice, s), "copy set");
   }
+  static void copy_set(const MhdSet& from, const MhdSet& to, size_t N, hipStream_t s) {
+    const double* a[10] = {from.rho, from.n, from.mx, from.my, from.mz, from.te, from.bx, from.by, from.bz, from.rho_pre};
+    double* b[10] = {to.rho, to.n, to.mx, to.my, to.mz, to.te, to.bx, to.by, to.bz, to.rho_pre};
+    for (int k = 0; k < 10; ++k)
+      hip_check(hipMemcpyAsync(b[k], a[k], N * 8, hipMemcpyDeviceToDevice, s), "copy set");
+  }
+  // a set of 10 grids laid out one after another from p
+  static MhdSet set_at(double* p, size_t N) {
+    return MhdSet{p, p + N, p + 2 * N, p + 3 * N, p + 4 * N, p + 5 * N, p + 6 * N, p + 7 * N, p + 8 * N, p + 9 * N};
+  }
 
   // Discrete adjoint of nsteps steps (reverse of advance(): stages, minimums/derived, step size),
   // with the evolved grids checkpointed every `ckpt` steps and each segment recomputed from its
@@ -525,13 +550,41 @@ class Domain : public DomainBase {
     require(nsteps >= 1 && ckpt >= 1, SPRUCE_E_ARG, "nsteps and checkpoint_interval must be positive");
     require(seed && grad, SPRUCE_E_ARG, "null argument");
     const int nseg = (nsteps + ckpt - 1) / ckpt;
-    std::vector<SetBufs> cps(nseg);
-    for (auto& c : cps) c.alloc(N);
+    const int sl = std::min(ckpt, nsteps);
+    const bool rk2 = integ == SPRUCE_RK2;
+    const int nb = adj_blocks(mesh);
+    // One allocation holds everything: checkpoints (nseg sets), the recomputed segment (sl + 1 sets),
+    // its RK2 midpoints (sl sets) and stage derivatives k1, k2 (8 grids per step each), the four
+    // adjoint vectors, the face coefficients and the reduction scratch.
+    const size_t SET = 10 * N, EV = 8 * N, CO = (size_t)ADJ_COEFS * N;
+    const size_t total = (size_t)nseg * SET + (size_t)(sl + 1) * SET + (rk2 ? (size_t)sl * SET : 0) +
+                         (size_t)sl * EV * (rk2 ? 2 : 1) + 4 * EV + 2 * CO + 1 + 3 * (size_t)nb;
+    DBuf slab;
+    slab.alloc(total);
+    double* cur = slab.p;
+    auto take = [&](size_t n) { double* p = cur; cur += n; return p; };
+    std::vector<MhdSet> cps(nseg), seg(sl + 1), mid(rk2 ? sl : 0);
+    for (auto& c : cps) c = set_at(take(SET), N);
+    for (auto& c : seg) c = set_at(take(SET), N);
+    for (auto& c : mid) c = set_at(take(SET), N);
+    std::vector<double*> k1s(sl), k2s(rk2 ? sl : 0);
+    for (auto& k : k1s) k = take(EV);
+    for (auto& k : k2s) k = take(EV);
+    double* lam = take(EV);
+    double* lamN = take(EV);
+    double* lamW = take(EV);
+    double* lamk = take(EV);
+    double* cx = take(CO);
+    double* cy = take(CO);
+    double* gh = take(1);
+    double* part = take(nb);
+    double* pmin = take(nb);
+    int* pidx = reinterpret_cast<int*>(take(nb));
     if ((int)history.n < nsteps + 1) history.alloc(nsteps + 1);
     hip_check(hipMemcpyAsync(history.p, step.p, sizeof(double), hipMemcpyDeviceToDevice, stream), "copy step");
     // forward sweep, exactly the steps of run()
     for (int n = 0; n < nsteps; ++n) {
-      if (n % ckpt == 0) copy_set(L, cps[n / ckpt], N, stream);
+      if (n % ckpt == 0) copy_set(L.view(), cps[n / ckpt], N, stream);
       advance(n);
     }
     sync_and_check();
@@ -542,54 +595,45 @@ class Domain : public DomainBase {
       iter++;
       if (dt_out) dt_out[n] = h[n];
     }
-    // backward sweep
-    std::vector<SetBufs> seg(std::min(ckpt, nsteps));
-    for (auto& q : seg) q.alloc(N);
-    DBuf lam, lamN, lamW, lamk, cx, cy, k1, k2, gh, part, pmin, pidx;
-    lam.alloc(8 * N); lamN.alloc(8 * N); lamW.alloc(8 * N); lamk.alloc(8 * N);
-    cx.alloc(ADJ_COEFS * N); cy.alloc(ADJ_COEFS * N); k1.alloc(8 * N); k2.alloc(8 * N); gh.alloc(1);
-    const int nb = adj_blocks(mesh);
-    part.alloc(nb); pmin.alloc(nb); pidx.alloc(nb);
-    hip_check(hipMemcpy(lam.p, seed, 8 * N * 8, hipMemcpyHostToDevice), "upload seed");
+    // backward sweep.  Recomputing a segment keeps, for each of its steps, the stage derivatives k1
+    // and k2 and the RK2 midpoint W, so the reverse steps read them instead of evaluating them again.
+    hip_check(hipMemcpyAsync(lam, seed, 8 * N * 8, hipMemcpyHostToDevice, stream), "upload seed");
     const MhdStatic st = stat();
     for (int sg = nseg - 1; sg >= 0; --sg) {
       const int n0 = sg * ckpt, n1 = std::min(nsteps, n0 + ckpt);
       copy_set(cps[sg], seg[0], N, stream);
-      for (int n = n0; n < n1 - 1; ++n) {            // recompute the segment's states
-        SetBufs& A = seg[n - n0];
-        SetBufs& B = seg[n - n0 + 1];
-        if (integ == SPRUCE_EULER) {
-          stage(A, A, B, 1.0, STAGE_PLAIN, 0, history.p + n);
+      for (int n = n0; n < n1; ++n) {                // recompute the segment: states, k1, W, k2
+        const int i = n - n0;
+        if (!rk2) {
+          stage_view(seg[i], seg[i], seg[i + 1], 1.0, STAGE_STORE_K, history.p + n, k1s[i]);
         } else {
-          stage(A, A, W, 0.5, STAGE_PLAIN, 0, history.p + n);
-          stage(W, A, B, 1.0, STAGE_PLAIN, 0, history.p + n);
+          stage_view(seg[i], seg[i], mid[i], 0.5, STAGE_STORE_K, history.p + n, k1s[i]);
+          stage_view(mid[i], seg[i], seg[i + 1], 1.0, STAGE_STORE_K, history.p + n, k2s[i]);
         }
       }
       for (int n = n1 - 1; n >= n0; --n) {
-        SetBufs& Ln = seg[n - n0];
+        const int i = n - n0;
+        const MhdSet& Ln = seg[i];
         const double* hp = history.p + n;
-        launch_fill(lamN.p, 0.0, 8 * N, stream);
-        launch_fill(gh.p, 0.0, 1, stream);
-        launch_mhd_rhs_full(Ln.view(), st, kc, mesh, k1.p, stream);
-        if (integ == SPRUCE_RK2) {
-          stage(Ln, Ln, W, 0.5, STAGE_PLAIN, 0, hp);
-          launch_mhd_rhs_full(W.view(), st, kc, mesh, k2.p, stream);
-          timed("adj_phi", [&] { launch_adj_phi(Ln.view(), k2.p, hp, 1.0, kc, mesh, lam.p, lamN.p, lamk.p, part.p, stream); });
-          launch_adj_sum(part.p, nb, gh.p, stream);
-          launch_fill(lamW.p, 0.0, 8 * N, stream);
-          timed("adj_rhs", [&] { launch_adj_rhs(W.view(), st, kc, mesh, lamk.p, cx.p, cy.p, lamW.p, stream); });
-          timed("adj_phi", [&] { launch_adj_phi(Ln.view(), k1.p, hp, 0.5, kc, mesh, lamW.p, lamN.p, lamk.p, part.p, stream); });
-          launch_adj_sum(part.p, nb, gh.p, stream);
+        launch_fill(lamN, 0.0, 8 * N, stream);
+        launch_fill(gh, 0.0, 1, stream);
+        if (rk2) {
+          timed("adj_phi", [&] { launch_adj_phi(Ln, k2s[i], hp, 1.0, kc, mesh, lam, lamN, lamk, part, stream); });
+          launch_adj_sum(part, nb, gh, stream);
+          launch_fill(lamW, 0.0, 8 * N, stream);
+          timed("adj_rhs", [&] { launch_adj_rhs(mid[i], st, kc, mesh, lamk, cx, cy, lamW, stream); });
+          timed("adj_phi", [&] { launch_adj_phi(Ln, k1s[i], hp, 0.5, kc, mesh, lamW, lamN, lamk, part, stream); });
+          launch_adj_sum(part, nb, gh, stream);
         } else {
-          timed("adj_phi", [&] { launch_adj_phi(Ln.view(), k1.p, hp, 1.0, kc, mesh, lam.p, lamN.p, lamk.p, part.p, stream); });
-          launch_adj_sum(part.p, nb, gh.p, stream);
+          timed("adj_phi", [&] { launch_adj_phi(Ln, k1s[i], hp, 1.0, kc, mesh, lam, lamN, lamk, part, stream); });
+          launch_adj_sum(part, nb, gh, stream);
         }
-        timed("adj_rhs", [&] { launch_adj_rhs(Ln.view(), st, kc, mesh, lamk.p, cx.p, cy.p, lamN.p, stream); });
-        launch_adj_dt(Ln.view(), st, kc, mesh, kc.eps, gh.p, pmin.p, (int*)pidx.p, lamN.p, stream);
-        lam.swap(lamN);
+        timed("adj_rhs", [&] { launch_adj_rhs(Ln, st, kc, mesh, lamk, cx, cy, lamN, stream); });
+        launch_adj_dt(Ln, st, kc, mesh, kc.eps, gh, pmin, pidx, lamN, stream);
+        std::swap(lam, lamN);
       }
     }
-    hip_check(hipMemcpyAsync(grad, lam.p, 8 * N * 8, hipMemcpyDeviceToHost, stream), "download gradient");
+    hip_check(hipMemcpyAsync(grad, lam, 8 * N * 8, hipMemcpyDeviceToHost, stream), "download gradient");
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     if (clock.enabled) clock.collect();
   }

@@ -84,6 +84,8 @@ void launch_mhd_stage(const StageArgs& a, hipStream_t s);
 // Wavefront-strip form of the same stage (mhd2.hip); R rows per wavefront, a multiple of 4.
 void launch_mhd_stage_strip(const StageArgs& a, int R, hipStream_t s);
 int mhd_strip_rows_default(int nx, int ny);
+// 0 (strips) when the grid gives the strip kernel enough wavefronts, else 1 (one thread per cell)
+int mhd_stage_kernel_default(int nx, int ny);
 void launch_mhd_populate(MhdSet L, const double* temp, MhdConst k, int nx, int ny, hipStream_t s);
 // Ghost fill of the sides whose kinds[] entry is non-zero; s1/s2 are the open-boundary decay
 // factors of each side (evolution.cpp:166-167), computed on the host with the C library's pow.

@@ -357,6 +357,13 @@ int mhd_strip_rows_default(int nx, int ny) {
   return 4;
 }
 
+int mhd_stage_kernel_default(int nx, int ny) {
+  // with 4-row strips a 256x256 grid gives 320 wavefronts, too few to fill 256 CUs (strips 49 us,
+  // one thread per cell 32 us); from about four wavefronts per CU on, the strips are faster
+  const long waves = (long)((ny + OUT_COLS - 1) / OUT_COLS) * ((nx + 3) / 4);
+  return waves >= 1024 ? 0 : 1;
+}
+
 void launch_mhd_stage_strip(const StageArgs& a, int R, hipStream_t s) {
   const int cols = (a.mesh.ny + OUT_COLS - 1) / OUT_COLS, nstrips = (a.mesh.nx + R - 1) / R;
   const dim3 grid(cols * nstrips);

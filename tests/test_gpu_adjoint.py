@@ -106,7 +106,18 @@ def test_adjoint_refuses_what_it_does_not_cover():
 def test_periodic_wave_gradient_256_through_torch_harness():
     """The 256x256 periodic wave, 500 steps, checkpoint every 50: gradient of the final density
     misfit with respect to the initial temperature grid and v0, against central differences
-    (h = 1e-6 relative) on 32 seeded unit probe directions."""
+    (h = 1e-6 relative) on 32 seeded unit probe directions.
+
+    The initial state has uniform temperature, so the sound speed does not depend on the density
+    noise, and |v_x| = |v_y| = v0 cos(pi/256) at the 16 cells next to the four velocity extrema:
+    their step sizes tie exactly at step 0 and the loss has a kink there (the step is a minimum over
+    cells).  A probe that moves one of those cells' temperature sees different forward and backward
+    differences; no gradient reproduces their mean.  For such probes (detected by forward and
+    backward differences that disagree) the adjoint must equal one of the two one-sided
+    derivatives; every other probe must match the central difference to 1e-6 relative.  Both
+    comparisons carry an absolute floor of 1e-9 of the gradient's norm (in relative parameter
+    units), the size of the differences' rounding noise (|J| * 2^-52 / h is about 5e-37 here), which
+    matters only for probes whose derivative is a thousand times smaller than typical."""
     torch = pytest.importorskip("torch")
     from torch_harness import PeriodicWaveLoss
     prob = PeriodicWaveProblem(256, 256, nsteps=500, checkpoint=50)
@@ -117,8 +128,10 @@ def test_periodic_wave_gradient_256_through_torch_harness():
     g_temp = temp.grad.numpy()
     g_v0 = float(v0.grad)
     print("cost ratio", PeriodicWaveLoss.last_ratio, flush=True)
+    j0, _, _ = prob.forward(prob.temp0, prob.v0)
+    floor = 1e-9 * float(np.sqrt(np.sum((g_temp * prob.temp0) ** 2) + (g_v0 * prob.v0) ** 2))
     rng = np.random.default_rng(32)
-    worst = 0.0
+    worst, kinks = 0.0, 0
     for k in range(32):
         u = rng.standard_normal(257)
         u /= np.linalg.norm(u)
@@ -130,8 +143,15 @@ def test_periodic_wave_gradient_256_through_torch_harness():
         ad = float(np.sum(g_temp * dt * prob.temp0)) + g_v0 * dv * prob.v0
         lp, _, _ = prob.forward(prob.temp0 * (1 + h * dt), prob.v0 * (1 + h * dv))
         lm, _, _ = prob.forward(prob.temp0 * (1 - h * dt), prob.v0 * (1 - h * dv))
-        fd = (lp - lm) / (2 * h)
-        rel = abs(ad - fd) / (abs(ad) + abs(fd))
-        print("probe", k, "adjoint %.9e fd %.9e rel %.2e" % (ad, fd, rel), flush=True)
-        worst = max(worst, rel)
+        fwd, bwd, fd = (lp - j0) / h, (j0 - lm) / h, (lp - lm) / (2 * h)
+        if abs(fwd - bwd) <= 1e-5 * (abs(fwd) + abs(bwd)) + 4 * floor:   # one-sided noise is twice the central
+            rel = abs(ad - fd) / (abs(ad) + abs(fd) + floor / 1e-6)
+            print("probe", k, "adjoint %.9e fd %.9e rel %.2e" % (ad, fd, rel), flush=True)
+            worst = max(worst, rel)
+        else:
+            kinks += 1
+            rel = min(abs(ad - fwd) / (abs(ad) + abs(fwd)), abs(ad - bwd) / (abs(ad) + abs(bwd)))
+            print("probe", k, "kink: adjoint %.9e forward %.9e backward %.9e rel %.2e" % (ad, fwd, bwd, rel), flush=True)
+            assert rel < 1e-5
+    assert kinks <= 2
     assert worst < 1e-6
