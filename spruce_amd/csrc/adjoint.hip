@@ -15,29 +15,71 @@ __device__ __forceinline__ bool interior_cell(const Mesh& M, int i, int j) {
   return i >= M.x.lo && i <= M.x.hi && j >= M.y.lo && j <= M.y.hi;
 }
 
-__global__ void __launch_bounds__(256) k_adj_cells(MhdSet S, MhdStatic st, MhdConst K, Mesh M, const double* lamk,
-                                                   double* cx, double* cy, double* lam_S) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i >= M.nx || j >= M.ny) return;
-  const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
-  double fx[ADJ_NCOEF], fy[ADJ_NCOEF];
-  if (!interior_cell(M, i, j)) {
-    for (int q = 0; q < ADJ_NCOEF; ++q) { cx[q * N + c] = 0.0; cy[q * N + c] = 0.0; }
-    return;
+__device__ __forceinline__ double block_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  __shared__ double part[16];
+  const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+  if ((lane & 63) == 0) part[lane >> 6] = v;
+  __syncthreads();
+  if (lane == 0) {
+    const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+    double m = 0.0;
+    for (int k = 0; k < nw; ++k) m += part[k];
+    v = m;
   }
-  double lam[8], own[8];
-  for (int v = 0; v < 8; ++v) lam[v] = lamk[v * N + c];
-  adj_cell_coeffs(S, st, K, M, i, j, lam, own, fx, fy);
-  for (int q = 0; q < ADJ_NCOEF; ++q) { cx[q * N + c] = fx[q]; cy[q * N + c] = fy[q]; }
-  for (int v = 0; v < 8; ++v) lam_S[v * N + c] += own[v];
+  return v;
+}
+
+// Reverse of one stage for one cell: T = minimums/derived(B + mult*h*kv) reversed (lam_B gets mu,
+// lamk = mult*h*mu, the block's partial of mult*mu.kv), then the cell's own part of the transposed
+// time derivative at S applied to lamk (its face coefficients to cx, cy; its own part to lam_S).
+// assign_B / assign_S: this launch is the first writer of lam_B / lam_S in the step (no clear).
+__global__ void __launch_bounds__(256) k_adj_stage_cells(MhdSet B, const double* kv, const double* step, double mult,
+                                                         MhdSet S, MhdStatic st, MhdConst K, Mesh M,
+                                                         const double* lam_T, double* lam_B, int assign_B,
+                                                         double* lam_S, int assign_S, double* cx, double* cy,
+                                                         double* partial) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  double dot = 0.0;
+  if (i < M.nx && j < M.ny) {
+    const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+    const double h = mult * (*step);
+    // derivative of the minimums/derived maps (idealmhd.cpp:237-238, 246-252): rho passes when
+    // neither floor is active, te when its floor is not
+    const double rho_new = B.rho[c] + kv[c] * h;
+    const double x = rho_new / K.m;
+    const double rho1 = smax(x, K.nmin) * K.m;
+    const double drho = (x < K.nmin || rho1 / K.m < K.nmin) ? 0.0 : 1.0;
+    const double te_new = B.te[c] + kv[4 * N + c] * h;
+    const double dte = te_new < K.temin ? 0.0 : 1.0;
+    double lamk[8];
+    for (int v = 0; v < 8; ++v) {
+      const double d = v == 0 ? drho : (v == 4 ? dte : 1.0);
+      const double mu = lam_T[v * N + c] * d;
+      lam_B[v * N + c] = assign_B ? mu : lam_B[v * N + c] + mu;
+      lamk[v] = h * mu;
+      dot += mu * kv[v * N + c];
+    }
+    dot *= mult;
+    double fx[ADJ_NCOEF], fy[ADJ_NCOEF];
+    if (interior_cell(M, i, j)) {
+      double own[8];
+      adj_cell_coeffs(S, st, K, M, i, j, lamk, own, fx, fy);
+      for (int v = 0; v < 8; ++v) lam_S[v * N + c] = assign_S ? own[v] : lam_S[v * N + c] + own[v];
+    } else {
+      for (int q = 0; q < ADJ_NCOEF; ++q) { fx[q] = 0.0; fy[q] = 0.0; }
+      if (assign_S) for (int v = 0; v < 8; ++v) lam_S[v * N + c] = 0.0;
+    }
+    for (int q = 0; q < ADJ_NCOEF; ++q) { cx[q * N + c] = fx[q]; cy[q * N + c] = fy[q]; }
+  }
+  dot = block_sum(dot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
 }
 
 // One thread per face along AX; the face at cell (i, j) lies between (i, j) and its lower neighbour.
 template <int AX>
-__global__ void __launch_bounds__(256) k_adj_faces(MhdSet S, MhdStatic st, MhdConst K, Mesh M, const double* coef,
-                                                   double* lam_S) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i >= M.nx || j >= M.ny) return;
+__device__ __forceinline__ void adj_faces_axis(const MhdSet& S, const MhdStatic& st, const MhdConst& K, const Mesh& M,
+                                               const double* coef, double* lam_S, int i, int j) {
   const Axis& ax = AX == 0 ? M.x : M.y;
   const int f = AX == 0 ? i : j;
   if (!ax.per && (f < ax.lo || f > ax.hi + 1)) return;      // faces of interior cells only
@@ -55,62 +97,13 @@ __global__ void __launch_bounds__(256) k_adj_faces(MhdSet S, MhdStatic st, MhdCo
   adj_face<AX>(S, st, K, M, i, j, fa, [&](size_t cell, int v, double val) { atomicAdd(&lam_S[v * N + cell], val); });
 }
 
-__device__ __forceinline__ double block_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  __shared__ double part[16];
-  const int lane = threadIdx.x + threadIdx.y * blockDim.x;
-  if ((lane & 63) == 0) part[lane >> 6] = v;
-  __syncthreads();
-  if (lane == 0) {
-    const int nw = (blockDim.x * blockDim.y + 63) >> 6;
-    double m = 0.0;
-    for (int k = 0; k < nw; ++k) m += part[k];
-    v = m;
-  }
-  return v;
-}
-
-__global__ void __launch_bounds__(256) k_adj_phi(MhdSet B, const double* kv, const double* step, double mult,
-                                                 MhdConst K, Mesh M, const double* lam_T, double* lam_B, double* lamk,
-                                                 double* partial) {
+// x-faces for blockIdx.z == 0, y-faces for blockIdx.z == 1
+__global__ void __launch_bounds__(256) k_adj_faces(MhdSet S, MhdStatic st, MhdConst K, Mesh M, const double* cx,
+                                                   const double* cy, double* lam_S) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
-  double dot = 0.0;
-  if (i < M.nx && j < M.ny) {
-    const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
-    const double h = mult * (*step);
-    // derivative of the minimums/derived maps (idealmhd.cpp:237-238, 246-252): rho passes when
-    // neither floor is active, te when its floor is not
-    const double rho_new = B.rho[c] + kv[c] * h;
-    const double x = rho_new / K.m;
-    const double rho1 = smax(x, K.nmin) * K.m;
-    const double drho = (x < K.nmin || rho1 / K.m < K.nmin) ? 0.0 : 1.0;
-    const double te_new = B.te[c] + kv[4 * N + c] * h;
-    const double dte = te_new < K.temin ? 0.0 : 1.0;
-    for (int v = 0; v < 8; ++v) {
-      const double d = v == 0 ? drho : (v == 4 ? dte : 1.0);
-      const double mu = lam_T[v * N + c] * d;
-      lam_B[v * N + c] += mu;
-      lamk[v * N + c] = h * mu;
-      dot += mu * kv[v * N + c];
-    }
-    dot *= mult;
-  }
-  dot = block_sum(dot);
-  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
-}
-
-__global__ void __launch_bounds__(1024) k_adj_sum(const double* partial, int n, double* gh) {
-  double v = 0.0;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) v += partial[k];
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  __shared__ double part[16];
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = 0.0;
-    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) m += part[k];
-    *gh += m;
-  }
+  if (i >= M.nx || j >= M.ny) return;
+  if (blockIdx.z == 0) adj_faces_axis<0>(S, st, K, M, cx, lam_S, i, j);
+  else adj_faces_axis<1>(S, st, K, M, cy, lam_S, i, j);
 }
 
 // per-block (min dt, index) over the interior, std::min semantics (NaN never selected)
@@ -143,8 +136,12 @@ __global__ void __launch_bounds__(256) k_adj_dt_blocks(MhdSet L, MhdStatic st, M
 }
 
 __global__ void __launch_bounds__(64) k_adj_dt_apply(MhdSet L, MhdStatic st, MhdConst K, Mesh M, double eps,
-                                                     const double* gh, const double* pmin, const int* pidx, int nb,
-                                                     double* lam_L) {
+                                                     const double* p1, const double* p2, const double* pmin,
+                                                     const int* pidx, int nb, double* lam_L) {
+  // gh: the adjoint of the step size, the sum of the stage kernels' block partials
+  double g = 0.0;
+  for (int k = threadIdx.x; k < nb; k += 64) g += p1[k] + (p2 ? p2[k] : 0.0);
+  for (int off = 32; off > 0; off >>= 1) g += __shfl_xor(g, off, 64);
   // (min, lowest index) over the block results: one wavefront, then a shuffle reduction; the rule
   // is order-independent, so the cell is the same however the blocks are visited
   double v = DBL_MAX;
@@ -160,7 +157,7 @@ __global__ void __launch_bounds__(64) k_adj_dt_apply(MhdSet L, MhdStatic st, Mhd
   const int i = idx / M.ny, j = idx - i * M.ny;
   const Dual8 dt = dt_cell_dual(L, st, K, M, i, j);
   const size_t N = (size_t)M.nx * M.ny;
-  const double s = eps * (*gh);
+  const double s = eps * g;
   for (int v8 = 0; v8 < 8; ++v8) lam_L[v8 * N + idx] += s * dt.d[v8];
 }
 
@@ -197,34 +194,30 @@ inline dim3 g2(const Mesh& m, dim3 b) { return dim3((m.ny + b.x - 1) / b.x, (m.n
 
 }  // namespace
 
-void launch_adj_rhs(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, const double* lamk, double* cx, double* cy,
-                    double* lam_S, hipStream_t s) {
-  const dim3 b(64, 4), g = g2(mesh, b);
-  hipLaunchKernelGGL(k_adj_cells, g, b, 0, s, S, st, k, mesh, lamk, cx, cy, lam_S);
-  hipLaunchKernelGGL(k_adj_faces<0>, g, b, 0, s, S, st, k, mesh, (const double*)cx, lam_S);
-  hipLaunchKernelGGL(k_adj_faces<1>, g, b, 0, s, S, st, k, mesh, (const double*)cy, lam_S);
-}
-
 int adj_blocks(const Mesh& m) {
   const dim3 g = g2(m, dim3(64, 4));
   return (int)(g.x * g.y);
 }
 
-void launch_adj_phi(MhdSet B, const double* kv, const double* step, double mult, MhdConst k, Mesh mesh,
-                    const double* lam_T, double* lam_B, double* lamk, double* partial, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_phi, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, B, kv, step, mult, k, mesh, lam_T, lam_B,
-                     lamk, partial);
+void launch_adj_stage_cells(MhdSet B, const double* kv, const double* step, double mult, MhdSet S, MhdStatic st,
+                            MhdConst k, Mesh mesh, const double* lam_T, double* lam_B, int assign_B, double* lam_S,
+                            int assign_S, double* cx, double* cy, double* partial, hipStream_t s) {
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(k_adj_stage_cells, g2(mesh, b), b, 0, s, B, kv, step, mult, S, st, k, mesh, lam_T, lam_B,
+                     assign_B, lam_S, assign_S, cx, cy, partial);
 }
 
-void launch_adj_sum(const double* partial, int n, double* gh, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(1024), 0, s, partial, n, gh);
+void launch_adj_faces(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, const double* cx, const double* cy,
+                      double* lam_S, hipStream_t s) {
+  const dim3 b(64, 4), g = g2(mesh, b);
+  hipLaunchKernelGGL(k_adj_faces, dim3(g.x, g.y, 2), b, 0, s, S, st, k, mesh, cx, cy, lam_S);
 }
 
-void launch_adj_dt(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double eps, const double* gh, double* pmin,
-                   int* pidx, double* lam_L, hipStream_t s) {
+void launch_adj_dt(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double eps, const double* p1, const double* p2,
+                   double* pmin, int* pidx, double* lam_L, hipStream_t s) {
   const dim3 b(64, 4), g = g2(mesh, b);
   hipLaunchKernelGGL(k_adj_dt_blocks, g, b, 0, s, L, st, k, mesh, pmin, pidx);
-  hipLaunchKernelGGL(k_adj_dt_apply, dim3(1), dim3(64), 0, s, L, st, k, mesh, eps, gh, (const double*)pmin,
+  hipLaunchKernelGGL(k_adj_dt_apply, dim3(1), dim3(64), 0, s, L, st, k, mesh, eps, p1, p2, (const double*)pmin,
                      (const int*)pidx, (int)(g.x * g.y), lam_L);
 }
 

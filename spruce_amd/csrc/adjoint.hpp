@@ -8,21 +8,23 @@ namespace spruce {
 
 constexpr int ADJ_COEFS = 17;
 
-// lam_S += d(k . lamk)/dS: the transpose of the per-cell time derivative applied to lamk (8*N),
-// using cx, cy (17*N each) as scratch.
-void launch_adj_rhs(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, const double* lamk, double* cx, double* cy,
-                    double* lam_S, hipStream_t s);
-// Reverse of T = minimums/derived(B + mult*h*kv) (equationset.cpp:222-230, idealmhd.cpp:234-252):
-// lam_B += mu, lamk = mult*h*mu with mu = dT/dpre^T lam_T, and per-block partial sums of
-// mult * mu . kv (the adjoint of the step size h).
 int adj_blocks(const Mesh& m);
-void launch_adj_phi(MhdSet B, const double* kv, const double* step, double mult, MhdConst k, Mesh mesh,
-                    const double* lam_T, double* lam_B, double* lamk, double* partial, hipStream_t s);
-// gh += sum of the partials (one block)
-void launch_adj_sum(const double* partial, int n, double* gh, hipStream_t s);
-// step-size adjoint: h = eps * min dt(L) over the interior; lam_L(c*) += eps * gh * grad dt(c*)
-void launch_adj_dt(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double eps, const double* gh, double* pmin,
-                   int* pidx, double* lam_L, hipStream_t s);
+// Reverse of one stage T = minimums/derived(B + mult*h*kv) (equationset.cpp:222-230,
+// idealmhd.cpp:234-252) fused with the per-cell part of the transposed time derivative at S:
+// lam_B (+)= mu with mu = dT/dpre^T lam_T; lamk = mult*h*mu stays in registers and gives the cell's
+// face coefficients (cx, cy, 17*N each) and its own part, lam_S (+)= own; partial (one per block)
+// holds the block sums of mult * mu . kv (the adjoint of the step size h).  assign_*: first writer
+// of that vector in the step, so no clear is needed.
+void launch_adj_stage_cells(MhdSet B, const double* kv, const double* step, double mult, MhdSet S, MhdStatic st,
+                            MhdConst k, Mesh mesh, const double* lam_T, double* lam_B, int assign_B, double* lam_S,
+                            int assign_S, double* cx, double* cy, double* partial, hipStream_t s);
+// The face part of the transposed time derivative at S (both axes, one launch): FP64 atomics into lam_S.
+void launch_adj_faces(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, const double* cx, const double* cy,
+                      double* lam_S, hipStream_t s);
+// step-size adjoint: gh = sum of the partials p1 (+ p2 when not null); h = eps * min dt(L) over the
+// interior; lam_L(c*) += eps * gh * grad dt(c*)
+void launch_adj_dt(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double eps, const double* p1, const double* p2,
+                   double* pmin, int* pidx, double* lam_L, hipStream_t s);
 // adjoint of populateVariablesFromState (idealmhd.cpp:226-252) from the state grids rho and temp:
 // g_state (8*N: rho, temp, mom_x, mom_y, mom_z, bi_x, bi_y, bi_z) from lam (8*N, evolved order)
 void launch_adj_populate(const double* rho_s, const double* temp_s, const double* lam, MhdConst k, size_t N,

@@ -554,11 +554,11 @@ class Domain : public DomainBase {
     const bool rk2 = integ == SPRUCE_RK2;
     const int nb = adj_blocks(mesh);
     // One allocation holds everything: checkpoints (nseg sets), the recomputed segment (sl + 1 sets),
-    // its RK2 midpoints (sl sets) and stage derivatives k1, k2 (8 grids per step each), the four
+    // its RK2 midpoints (sl sets) and stage derivatives k1, k2 (8 grids per step each), the three
     // adjoint vectors, the face coefficients and the reduction scratch.
     const size_t SET = 10 * N, EV = 8 * N, CO = (size_t)ADJ_COEFS * N;
     const size_t total = (size_t)nseg * SET + (size_t)(sl + 1) * SET + (rk2 ? (size_t)sl * SET : 0) +
-                         (size_t)sl * EV * (rk2 ? 2 : 1) + 4 * EV + 2 * CO + 1 + 3 * (size_t)nb;
+                         (size_t)sl * EV * (rk2 ? 2 : 1) + 3 * EV + 2 * CO + 4 * (size_t)nb;
     DBuf slab;
     slab.alloc(total);
     double* cur = slab.p;
@@ -573,11 +573,10 @@ class Domain : public DomainBase {
     double* lam = take(EV);
     double* lamN = take(EV);
     double* lamW = take(EV);
-    double* lamk = take(EV);
     double* cx = take(CO);
     double* cy = take(CO);
-    double* gh = take(1);
     double* part = take(nb);
+    double* part2 = take(nb);
     double* pmin = take(nb);
     int* pidx = reinterpret_cast<int*>(take(nb));
     if ((int)history.n < nsteps + 1) history.alloc(nsteps + 1);
@@ -615,21 +614,26 @@ class Domain : public DomainBase {
         const int i = n - n0;
         const MhdSet& Ln = seg[i];
         const double* hp = history.p + n;
-        launch_fill(lamN, 0.0, 8 * N, stream);
-        launch_fill(gh, 0.0, 1, stream);
+        // lamN and lamW are written first by the stage-cell kernels (assign), so they need no clear
         if (rk2) {
-          timed("adj_phi", [&] { launch_adj_phi(Ln, k2s[i], hp, 1.0, kc, mesh, lam, lamN, lamk, part, stream); });
-          launch_adj_sum(part, nb, gh, stream);
-          launch_fill(lamW, 0.0, 8 * N, stream);
-          timed("adj_rhs", [&] { launch_adj_rhs(mid[i], st, kc, mesh, lamk, cx, cy, lamW, stream); });
-          timed("adj_phi", [&] { launch_adj_phi(Ln, k1s[i], hp, 0.5, kc, mesh, lamW, lamN, lamk, part, stream); });
-          launch_adj_sum(part, nb, gh, stream);
+          timed("adj_stage", [&] {
+            launch_adj_stage_cells(Ln, k2s[i], hp, 1.0, mid[i], st, kc, mesh, lam, lamN, 1, lamW, 1, cx, cy, part,
+                                   stream);
+          });
+          timed("adj_faces", [&] { launch_adj_faces(mid[i], st, kc, mesh, cx, cy, lamW, stream); });
+          timed("adj_stage", [&] {
+            launch_adj_stage_cells(Ln, k1s[i], hp, 0.5, Ln, st, kc, mesh, lamW, lamN, 0, lamN, 0, cx, cy, part2,
+                                   stream);
+          });
         } else {
-          timed("adj_phi", [&] { launch_adj_phi(Ln, k1s[i], hp, 1.0, kc, mesh, lam, lamN, lamk, part, stream); });
-          launch_adj_sum(part, nb, gh, stream);
+          timed("adj_stage", [&] {
+            launch_adj_stage_cells(Ln, k1s[i], hp, 1.0, Ln, st, kc, mesh, lam, lamN, 1, lamN, 0, cx, cy, part, stream);
+          });
         }
-        timed("adj_rhs", [&] { launch_adj_rhs(Ln, st, kc, mesh, lamk, cx, cy, lamN, stream); });
-        launch_adj_dt(Ln, st, kc, mesh, kc.eps, gh, pmin, pidx, lamN, stream);
+        timed("adj_faces", [&] { launch_adj_faces(Ln, st, kc, mesh, cx, cy, lamN, stream); });
+        timed("adj_dt", [&] {
+          launch_adj_dt(Ln, st, kc, mesh, kc.eps, part, rk2 ? part2 : nullptr, pmin, pidx, lamN, stream);
+        });
         std::swap(lam, lamN);
       }
     }
