@@ -1504,7 +1504,7 @@ spruce_gpu_batch* spruce_gpu_batch_create(spruce_gpu_domain* const* domains, int
       m.push_back(t);
     }
     auto h = std::make_unique<spruce_gpu_batch>();
-    h->b = std::make_unique<TfBatchRun>(m, m[0]->block_x, m[0]->block_y);
+    h->b = std::make_unique<TfBatchRun>(m, m[0]->shape);
     for (auto* t : m) t->batch = h->b.get();
     return h.release();
   } catch (const std::exception& e) {
@@ -1522,12 +1522,7 @@ int spruce_gpu_batch_set_option(spruce_gpu_batch* h, const char* key, const char
     require(v == "true" || v == "false", SPRUCE_E_ARG, "graph must be true or false");
     b.use_graph = v == "true";
   } else if (k == "block") {
-    const auto x = v.find('x');
-    require(x != std::string::npos, SPRUCE_E_ARG, "block must be <columns>x<rows>");
-    const int bx = std::stoi(v.substr(0, x)), by = std::stoi(v.substr(x + 1));
-    require(bx > 0 && by > 0 && bx * by <= 256 && (bx * by) % 64 == 0, SPRUCE_E_ARG,
-            "block must hold 64, 128, 192 or 256 threads");
-    b.rebuild(bx, by);
+    b.rebuild(parse_tf_shape(v));
   } else {
     throw Error(SPRUCE_E_ARG, "unknown batch option " + k);
   }

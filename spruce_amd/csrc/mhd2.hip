@@ -12,25 +12,14 @@
 
 #include "arith.hpp"
 #include "mhd.hpp"
+#include "strip.hpp"
 
 namespace spruce {
 namespace {
 
 constexpr int NQ = 11;  // transported quantities, in idealmhd.cpp:52-86 order
 enum { Q_RHO = 0, Q_MX, Q_MY, Q_MZ, Q_TE, Q_BIX, Q_BEX, Q_BIY, Q_BEY, Q_BIZ, Q_BEZ };
-constexpr int OUT_COLS = 60;
-
-// lane l receives the value of lane l-1 (wave_shr:1) / lane l+1 (wave_shl:1)
-__device__ __forceinline__ double shr1(double x) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x138, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x138, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double shl1(double x) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x130, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x130, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
+using namespace strip;
 
 // qdiv for a data-dependent divisor: the reciprocal is exact-rounded only if it is a normal number.
 __device__ __forceinline__ double qdiv_d(double a, double b, double rb) {
@@ -39,73 +28,14 @@ __device__ __forceinline__ double qdiv_d(double a, double b, double rb) {
   return a / b;
 }
 
-// Geometry of one face: interpolation weights and denominators, and the extrapolation geometry on
-// both sides (derivs.cpp:477-499).
-struct FaceGeom {
-  double hlo, hup, den, rden;
-  double gup, denm, rdenm;   // extrapolation through (c-2, c-1)
-  double gdn, denp, rdenp;   // extrapolation through (c+1, c)
-};
-
-__device__ __forceinline__ int wrapn(int k, int n) {
-  k %= n;
-  return k < 0 ? k + n : k;
-}
-__device__ __forceinline__ int clampn(int k, int n) { return k < 0 ? 0 : (k >= n ? n - 1 : k); }
-
-__device__ __forceinline__ FaceGeom face_geom(const Axis& ax, int c) {
-  const int n = ax.n;
-  int cm1 = c - 1, cp1 = c + 1, cc = c;
-  if (ax.per) { cc = wrapn(c, n); cm1 = wrapn(c - 1, n); cp1 = wrapn(c + 1, n); }
-  else { cc = clampn(c, n); cm1 = clampn(c - 1, n); cp1 = clampn(c + 1, n); }
-  FaceGeom g;
-  g.hlo = ax.h[cm1];
-  g.hup = ax.h[cc];
-  g.den = ax.fden[cc];
-  g.rden = ax.rfden[cc];
-  g.gup = ax.gup[cc];
-  g.denm = ax.fden[cm1];
-  g.rdenm = ax.rfden[cm1];
-  g.gdn = ax.gdn[cc];
-  g.denp = ax.fden[cp1];
-  g.rdenp = ax.rfden[cp1];
-  return g;
-}
-
 // Face velocity of one face: interpolated v and press, and the upwind side (derivs.cpp:10-73, 155).
-struct FaceDir {
-  double vf, ge, de, rde;
-  bool pos, moving;
-};
-
 template <int AX>
 __device__ __forceinline__ FaceDir face_velocity(const double* vlo, const double* vup, double plo, double pup,
                                                  const FaceGeom& g, double* iv, double& ip) {
 #pragma unroll
-  for (int k = 0; k < 3; ++k) iv[k] = qdiv(vlo[k] * g.hup + vup[k] * g.hlo, g.den, g.rden);
-  ip = qdiv(plo * g.hup + pup * g.hlo, g.den, g.rden);
-  FaceDir d;
-  d.vf = iv[AX];
-  d.pos = d.vf > 0.0;
-  d.moving = d.pos || d.vf < 0.0;
-  d.ge = d.pos ? g.gup : g.gdn;
-  d.de = d.pos ? g.denm : g.denp;
-  d.rde = d.pos ? g.rdenm : g.rdenp;
-  return d;
-}
-
-// surf * face velocity * (Barton-limited upwind value) of one transported quantity, from the four
-// cells c-2, c-1 (lo), c (up), c+1 along the face's axis; d2 = the plain interpolation.
-__device__ __forceinline__ double face_flux(double q0, double q1, double q2, double q3, const FaceGeom& g,
-                                            const FaceDir& fd, double& d2) {
-  d2 = qdiv(q1 * g.hup + q2 * g.hlo, g.den, g.rden);
-  const double ea = fd.pos ? q0 : q3, eb = fd.pos ? q1 : q2;
-  const double d1 = ea + qdiv((eb - ea) * fd.ge, fd.de, fd.rde);
-  const bool le = q2 <= q1;
-  const bool outer_min = fd.pos ? le : !le;
-  const double inner = outer_min ? smax(d1, d2) : smin(d1, d2);
-  const double lim = outer_min ? smin(eb, inner) : smax(eb, inner);
-  return (fd.moving ? lim : d2) * fd.vf;
+  for (int k = 0; k < 3; ++k) iv[k] = interp(g, vlo[k], vup[k]);
+  ip = interp(g, plo, pup);
+  return face_dir(iv[AX], g);
 }
 
 __device__ __forceinline__ void load_row(const StageArgs& a, int i, bool ivalid, int j, bool jvalid, double* q) {
@@ -139,14 +69,10 @@ __device__ __forceinline__ void derive(const double* q, const MhdConst& K, doubl
   p = q[Q_TE] * K.gm1;
 }
 
-// Strip of a workgroup.  Workgroups go to the 8 XCDs round-robin by launch index (index % 8), and
-// each XCD has its own L2: launch index L is mapped so that XCD x takes the x-th eighth of the
-// strips in column-major order, so strips that run together on one XCD are neighbours along x and
-// share their halo rows in that XCD's L2.
+// Strip of a workgroup: launch index L mapped XCD-aware (xcd_tile) onto the strips in column-major
+// order, so strips that run together on one XCD are x-neighbours and share their halo rows in L2.
 __device__ __forceinline__ void strip_of(int L, int nstrips, int total, int& col, int& strip) {
-  const int xcd = L & 7, k = L >> 3;
-  const int q = total >> 3, rem = total & 7;
-  const int t = xcd * q + (xcd < rem ? xcd : rem) + k;
+  const int t = xcd_tile(L, total);
   col = t / nstrips;
   strip = t - col * nstrips;
 }

@@ -66,16 +66,28 @@ void TfDomain::set_grid(const std::string& name, const double* p, int64_t count)
   host_grids[name].assign(p, p + N);
 }
 
+TfShape parse_tf_shape(const std::string& v) {
+  TfShape sh;
+  if (v.rfind("strip", 0) == 0) {
+    const int r = std::stoi(v.substr(5));
+    require(r >= 4 && r <= 64 && r % 4 == 0, SPRUCE_E_ARG, "strip rows must be a multiple of 4 in [4, 64]");
+    sh.R = r;
+    return sh;
+  }
+  const auto x = v.find('x');
+  require(x != std::string::npos, SPRUCE_E_ARG, "block must be <columns>x<rows> or strip<rows>");
+  const int bx = std::stoi(v.substr(0, x)), by = std::stoi(v.substr(x + 1));
+  require(bx > 0 && by > 0 && bx * by <= 256 && (bx * by) % 64 == 0, SPRUCE_E_ARG,
+          "block must hold 64, 128, 192 or 256 threads");
+  sh.bx = bx;
+  sh.by = by;
+  return sh;
+}
+
 void TfDomain::set_option(const std::string& k, const std::string& v) {
   if (k == "block") {
-    const auto x = v.find('x');
-    require(x != std::string::npos, SPRUCE_E_ARG, "block must be <columns>x<rows>");
-    const int bx = std::stoi(v.substr(0, x)), by = std::stoi(v.substr(x + 1));
-    require(bx > 0 && by > 0 && bx * by <= 256 && (bx * by) % 64 == 0, SPRUCE_E_ARG,
-            "block must hold 64, 128, 192 or 256 threads");
-    block_x = bx;
-    block_y = by;
-    if (self) self->rebuild(bx, by);
+    shape = parse_tf_shape(v);
+    if (self) self->rebuild(shape);
   } else if (k == "stage_kernel" || k == "strip_rows") {
     // IdealMHD options; the two-fluid path has a single stage kernel
   } else {
@@ -179,7 +191,7 @@ void TfDomain::setup() {
   hip_check(hipMemset(counter, 0, sizeof(int)), "hipMemset");
   hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   fill_descriptor();
-  self = std::make_unique<TfBatchRun>(std::vector<TfDomain*>{this}, block_x, block_y);
+  self = std::make_unique<TfBatchRun>(std::vector<TfDomain*>{this}, shape);
   // populateVariablesFromState (equationset.cpp:96-104)
   const double* hp[2] = {itemp.p, etemp.p};
   double** dp = nullptr;
@@ -254,7 +266,7 @@ void TfDomain::rhs(double* out, int64_t count) {
 
 // ---------------------------------------------------------------------------------------------
 
-TfBatchRun::TfBatchRun(const std::vector<TfDomain*>& m, int bx, int by) : members(m) {
+TfBatchRun::TfBatchRun(const std::vector<TfDomain*>& m, TfShape sh) : members(m) {
   require(!members.empty(), SPRUCE_E_ARG, "empty batch");
   integrator = members[0]->desc.time_integrator;
   for (auto* d : members) {
@@ -265,13 +277,14 @@ TfBatchRun::TfBatchRun(const std::vector<TfDomain*>& m, int bx, int by) : member
     }
     maxn = std::max(maxn, std::max(d->nx, d->ny));
   }
-  rebuild(bx, by);
+  rebuild(sh);
 }
 
 TfBatchRun::~TfBatchRun() {
   destroy_graph();
   if (d_doms) (void)hipFree(d_doms);
   if (d_tile_dom) (void)hipFree(d_tile_dom);
+  if (d_strip_dom) (void)hipFree(d_strip_dom);
 }
 
 void TfBatchRun::destroy_graph() {
@@ -281,11 +294,13 @@ void TfBatchRun::destroy_graph() {
   graph = nullptr;
 }
 
-void TfBatchRun::rebuild(int bx, int by) {
+void TfBatchRun::rebuild(TfShape sh) {
   destroy_graph();
+  shape = sh;
+  const int bx = sh.bx, by = sh.by;
   std::vector<TfDom> doms;
-  std::vector<int> tile_dom;
-  int base = 0;
+  std::vector<int> tile_dom, strip_dom;
+  int base = 0, sbase = 0;
   for (size_t k = 0; k < members.size(); ++k) {
     TfDomain* d = members[k];
     TfDom D = d->dom;
@@ -297,23 +312,38 @@ void TfBatchRun::rebuild(int bx, int by) {
     D.partial = d->partial.p;
     D.history = d->history.p;
     d->dom.tiles_x = D.tiles_x; d->dom.tiles_y = D.tiles_y;
-    doms.push_back(D);
     for (int t = 0; t < nt; ++t) tile_dom.push_back((int)k);
     base += nt;
+    D.strip_base = sbase;
+    D.strips_x = sh.R > 0 ? (d->nx + sh.R - 1) / sh.R : 0;
+    const int ns = sh.R > 0 ? D.strips_x * ((d->ny + 59) / 60) : 0;   // 60 output columns per strip
+    for (int t = 0; t < ns; ++t) strip_dom.push_back((int)k);
+    sbase += ns;
+    doms.push_back(D);
   }
   uploaded = doms;
   if (d_doms) (void)hipFree(d_doms);
   if (d_tile_dom) (void)hipFree(d_tile_dom);
+  if (d_strip_dom) (void)hipFree(d_strip_dom);
+  d_strip_dom = nullptr;
   hip_check(hipMalloc(&d_doms, doms.size() * sizeof(TfDom)), "hipMalloc");
   hip_check(hipMalloc(&d_tile_dom, tile_dom.size() * sizeof(int)), "hipMalloc");
   hip_check(hipMemcpy(d_doms, doms.data(), doms.size() * sizeof(TfDom), hipMemcpyHostToDevice), "upload");
   hip_check(hipMemcpy(d_tile_dom, tile_dom.data(), tile_dom.size() * sizeof(int), hipMemcpyHostToDevice), "upload");
+  if (!strip_dom.empty()) {
+    hip_check(hipMalloc(&d_strip_dom, strip_dom.size() * sizeof(int)), "hipMalloc");
+    hip_check(hipMemcpy(d_strip_dom, strip_dom.data(), strip_dom.size() * sizeof(int), hipMemcpyHostToDevice),
+              "upload");
+  }
   b.doms = d_doms;
   b.tile_dom = d_tile_dom;
   b.ndom = (int)members.size();
   b.ntiles = base;
   b.bx = bx;
   b.by = by;
+  b.strip_dom = d_strip_dom;
+  b.nstrip_tiles = sbase;
+  b.strip_rows = sh.R;
 }
 
 // One PlasmaDomain::advanceTime of every domain (evolution.cpp:59-124); no host interaction, so a
@@ -361,7 +391,7 @@ void TfBatchRun::run(int nsteps, bool use_graph, std::vector<std::vector<double>
     // another batch holding this domain may have replaced its history or partial buffers
     realloc = realloc || uploaded[k].history != d->history.p || uploaded[k].partial != d->partial.p;
   }
-  if (realloc) rebuild(b.bx, b.by);
+  if (realloc) rebuild(shape);
   launch_tf_reset_counters(b, s);
   if (use_graph && !clock.enabled) {
     if (!gexec) {
@@ -406,14 +436,26 @@ double TfBatchRun::autotune(int trial_steps, std::string* chosen) {
     saves[k].t = d->time;
     saves[k].it = d->iter;
   }
-  const int shapes[][2] = {{64, 1}, {64, 2}, {64, 4}, {128, 1}, {128, 2}, {32, 2}, {32, 4}, {32, 8}, {16, 4}, {16, 8}};
+  std::vector<TfShape> shapes;
+  for (auto& bxy : std::vector<std::pair<int, int>>{{64, 1}, {64, 2}, {64, 4}, {128, 1}, {128, 2}, {32, 2}, {32, 4},
+                                                    {32, 8}, {16, 4}, {16, 8}}) {
+    TfShape t;
+    t.bx = bxy.first;
+    t.by = bxy.second;
+    shapes.push_back(t);
+  }
+  for (int r : {4, 8, 16}) {
+    TfShape t;
+    t.R = r;
+    shapes.push_back(t);
+  }
   double best = 1e300;
-  int bx = b.bx, by = b.by;
+  TfShape bestsh = shape;
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "hipEventCreate");
   hip_check(hipEventCreate(&e1), "hipEventCreate");
   for (auto& sh : shapes) {
-    rebuild(sh[0], sh[1]);
+    rebuild(sh);
     run(1, false, nullptr);  // warm
     hip_check(hipEventRecord(e0, s), "record");
     for (int k = 0; k < trial_steps; ++k) enqueue_step(s);
@@ -421,7 +463,7 @@ double TfBatchRun::autotune(int trial_steps, std::string* chosen) {
     hip_check(hipEventSynchronize(e1), "sync");
     float ms = 0.f;
     hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
-    if (ms < best) { best = ms; bx = sh[0]; by = sh[1]; }
+    if (ms < best) { best = ms; bestsh = sh; }
     for (size_t k = 0; k < members.size(); ++k)
       for (size_t q = 0; q < saves[k].src.size(); ++q)
         hip_check(hipMemcpy(saves[k].src[q]->p, saves[k].copy[q]->p, saves[k].src[q]->n * 8, hipMemcpyDeviceToDevice),
@@ -430,9 +472,9 @@ double TfBatchRun::autotune(int trial_steps, std::string* chosen) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   for (size_t k = 0; k < members.size(); ++k) { members[k]->time = saves[k].t; members[k]->iter = saves[k].it; }
-  rebuild(bx, by);
-  for (auto* d : members) { d->block_x = bx; d->block_y = by; }
-  if (chosen) *chosen = std::to_string(bx) + "x" + std::to_string(by);
+  rebuild(bestsh);
+  for (auto* d : members) d->shape = bestsh;
+  if (chosen) *chosen = bestsh.str();
   return best / trial_steps;
 }
 

@@ -12,6 +12,15 @@ namespace spruce {
 
 class TfBatchRun;
 
+// Launch shape of a batch: the stage kernel in strips of R rows (R > 0) or one thread per cell in
+// bx x by blocks; the other kernels always use the bx x by cell tiles.
+struct TfShape {
+  int bx = 64, by = 4, R = 0;
+  std::string str() const { return R > 0 ? "strip" + std::to_string(R) : std::to_string(bx) + "x" + std::to_string(by); }
+};
+// "<columns>x<rows>" (64, 128, 192 or 256 threads) or "strip<rows>" (rows a multiple of 4, 4..64)
+TfShape parse_tf_shape(const std::string& v);
+
 class TfDomain : public DomainBase {
  public:
   TfDomain(const spruce_domain_desc& d, const std::vector<std::string>& keys, const std::vector<std::string>& vals);
@@ -34,7 +43,7 @@ class TfDomain : public DomainBase {
   bool use_sub_cycling = true, remove_curl = false;
   int nx = 0, ny = 0;
   size_t N = 0;
-  int block_x = 64, block_y = 4;
+  TfShape shape;
   std::map<std::string, std::vector<double>> host_grids;
   AxisBufs ax, ay;
   DBuf L[TF_NE], W[TF_NE], W2[TF_NE], K[3];
@@ -48,9 +57,9 @@ class TfDomain : public DomainBase {
 // A set of two-fluid domains stepped together: every launch covers the tiles of all domains.
 class TfBatchRun {
  public:
-  TfBatchRun(const std::vector<TfDomain*>& members, int bx, int by);
+  TfBatchRun(const std::vector<TfDomain*>& members, TfShape shape);
   ~TfBatchRun();
-  void rebuild(int bx, int by);   // new block shape (autotuning)
+  void rebuild(TfShape shape);    // new launch shape (autotuning)
   void enqueue_step(hipStream_t s);
   void run(int nsteps, bool graph, std::vector<std::vector<double>>* dts);
   double autotune(int trial_steps, std::string* chosen);
@@ -65,6 +74,8 @@ class TfBatchRun {
   KernelClock clock;                  // per-kernel event timing (profiling runs skip the graph)
   TfDom* d_doms = nullptr;
   int* d_tile_dom = nullptr;
+  int* d_strip_dom = nullptr;
+  TfShape shape;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   void destroy_graph();

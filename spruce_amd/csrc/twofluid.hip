@@ -7,7 +7,10 @@
 // can capture.  Each domain keeps its own step size on the device.
 #include <cfloat>
 
+#include <type_traits>
+
 #include "arith.hpp"
+#include "strip.hpp"
 #include "twofluid.hpp"
 
 namespace spruce {
@@ -521,10 +524,320 @@ __global__ __launch_bounds__(256) void k_tf_rhs(TfBatch b, int d, double* out) {
   for (int v = 0; v < TF_NE; ++v) out[v * N + c] = k[v];
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// Strip form of the stage kernel: one 64-lane wavefront per (domain, 60-column tile, strip of R
+// rows), every face computed once (x-faces carried down the strip in registers, y-faces shared
+// between neighbouring lanes), with the arithmetic of tf_face / tf_axis / tf_rhs_cell above,
+// operation for operation.
+// ---------------------------------------------------------------------------------------------
+// per-cell transported quantities, index s * 4 + q (q: rho, mom_x, mom_y, thermal_energy)
+constexpr int TQ = 8;
+// per-cell field values: bz, Ez, be_y + bi_y, Ey (x-faces), be_x + bi_x, Ex (y-faces)
+enum { TFF_BZ = 0, TFF_EZ, TFF_BYT, TFF_EY, TFF_BXT, TFF_EX, TFF_N };
+// per-cell derived values: v_x, v_y and press per species
+enum { TFD_VX0 = 0, TFD_VY0, TFD_P0, TFD_VX1, TFD_VY1, TFD_P1, TFD_N };
+
+__device__ __forceinline__ void tf_load_q(const TfSet& S, size_t c, bool ok, double* q) {
+  if (ok) {
+    q[0] = S.v[TF_IRHO][c]; q[1] = S.v[TF_IMX][c]; q[2] = S.v[TF_IMY][c]; q[3] = S.v[TF_ITE][c];
+    q[4] = S.v[TF_ERHO][c]; q[5] = S.v[TF_EMX][c]; q[6] = S.v[TF_EMY][c]; q[7] = S.v[TF_ETE][c];
+  } else {
+#pragma unroll
+    for (int k = 0; k < TQ; ++k) q[k] = 0.0;
+    q[0] = 1.0;
+    q[4] = 1.0;
+  }
+}
+__device__ __forceinline__ void tf_load_f(const TfDom& D, const TfSet& S, size_t c, bool ok, double* f) {
+  if (ok) {
+    f[TFF_BZ] = S.v[TF_BZ][c];
+    f[TFF_EZ] = S.v[TF_EZ][c];
+    f[TFF_BYT] = D.bey[c] + S.v[TF_BY][c];
+    f[TFF_EY] = S.v[TF_EY][c];
+    f[TFF_BXT] = D.bex[c] + S.v[TF_BX][c];
+    f[TFF_EX] = S.v[TF_EX][c];
+  } else {
+#pragma unroll
+    for (int k = 0; k < TFF_N; ++k) f[k] = 0.0;
+  }
+}
+// v = mom/rho per species (ideal2F.cpp:124-127), press = (gamma-1) te (:131-132)
+__device__ __forceinline__ void tf_derive(const double* q, const TfConst& K, double* dv) {
+  dv[TFD_VX0] = q[1] / q[0];
+  dv[TFD_VY0] = q[2] / q[0];
+  dv[TFD_P0] = q[3] * K.gm1;
+  dv[TFD_VX1] = q[5] / q[4];
+  dv[TFD_VY1] = q[6] / q[4];
+  dv[TFD_P1] = q[7] * K.gm1;
+}
+
+__device__ __forceinline__ void tf_strip_of(const TfBatch& b, int& d, int& col, int& strip) {
+  const int t = strip::xcd_tile(blockIdx.x, b.nstrip_tiles);
+  d = b.strip_dom[t];
+  const TfDom& D = b.doms[d];
+  const int lt = t - D.strip_base;
+  col = lt / D.strips_x;
+  strip = lt - col * D.strips_x;
+}
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+k_tf_stage_strip(TfBatch b, int Ssel, int Bsel, int Tsel, double mult, int mode, int kslot) {
+  using namespace strip;
+  int d, col, sidx;
+  tf_strip_of(b, d, col, sidx);
+  const TfDom& D = b.doms[d];
+  const TfSet& S = pick(D, Ssel);
+  const TfSet& B = pick(D, Bsel);
+  const TfSet& T = pick(D, Tsel);
+  const TfConst& K = D.c;
+  const Mesh& M = D.mesh;
+  const int nx = M.nx, ny = M.ny, R = b.strip_rows;
+  const size_t N = (size_t)nx * ny;
+  const int lane = threadIdx.x;
+  const int jraw = col * OUT_COLS + lane - 2;
+  int j;
+  bool jvalid;
+  if (M.y.per) { j = wrapn(jraw, ny); jvalid = true; }
+  else { jvalid = jraw >= 0 && jraw < ny; j = clampn(jraw, ny); }
+  const bool out_lane = lane >= 2 && lane < 62 && jraw >= 0 && jraw < ny;
+  const bool yint = jvalid && j >= M.y.lo && j <= M.y.hi;
+  const bool curl = K.curl != 0;
+  const int r0 = sidx * R;
+  auto row_of = [&](int r, bool& ok) -> int {
+    if (M.x.per) { ok = true; return wrapn(r, nx); }
+    ok = r >= 0 && r < nx;
+    return clampn(r, nx);
+  };
+  auto cell = [&](int i) { return (size_t)i * ny + j; };
+
+  // rotating rows r-1 .. r+2 of the transported quantities, one column per lane: kept in LDS
+  // (16 KiB per wavefront) rather than registers, which leaves room for two wavefronts per SIMD
+  __shared__ double qs[4][TQ][64];
+  auto Q = [&](int slot, int k) -> double& { return qs[slot][k][lane]; };
+  auto load_q = [&](size_t c, bool ok, int slot) {
+    double t[TQ];
+    tf_load_q(S, c, ok, t);
+#pragma unroll
+    for (int k = 0; k < TQ; ++k) Q(slot, k) = t[k];
+  };
+  auto derive = [&](int slot, double* dvo) {
+    double t[TQ];
+#pragma unroll
+    for (int k = 0; k < TQ; ++k) t[k] = Q(slot, k);
+    tf_derive(t, K, dvo);
+  };
+  double f[2][TFF_N];         // fields of rows r, r+1
+  double dv[2][TFD_N];        // derived values of rows r, r+1
+  double Px[TQ];              // flux of row r's lower x-face
+  double xvf[2][2], xip[2][2], xfl[2][4];   // lower/upper x-face: face velocity, press, fields
+
+  {  // prologue: rows r0-2 .. r0+1, fields of row r0-1 and r0, lower face of row r0
+    bool ok;
+    int i;
+    i = row_of(r0 - 2, ok); load_q(cell(i), ok && jvalid, 3);
+    i = row_of(r0 - 1, ok); load_q(cell(i), ok && jvalid, 0);
+    double ft[TFF_N], dt[TFD_N];
+    if (curl) tf_load_f(D, S, cell(i), ok && jvalid, ft);
+    i = row_of(r0, ok);     load_q(cell(i), ok && jvalid, 1);
+    if (curl) tf_load_f(D, S, cell(i), ok && jvalid, f[0]);
+    i = row_of(r0 + 1, ok); load_q(cell(i), ok && jvalid, 2);
+    derive(0, dt);
+    derive(1, dv[0]);
+    const FaceGeom gx = face_geom(M.x, r0);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int o = s * 3;
+      xvf[0][s] = interp(gx, dt[o + TFD_VX0], dv[0][o + TFD_VX0]);
+      xip[0][s] = interp(gx, dt[o + TFD_P0], dv[0][o + TFD_P0]);
+      const FaceDir fd = face_dir(xvf[0][s], gx);
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        double d2;
+        const int k = s * 4 + qq;
+        Px[k] = face_flux(Q(3, k), Q(0, k), Q(1, k), Q(2, k), gx, fd, d2);
+      }
+    }
+    if (curl) {
+      xfl[0][0] = interp(gx, ft[TFF_BZ], f[0][TFF_BZ]);
+      xfl[0][1] = interp(gx, ft[TFF_BYT], f[0][TFF_BYT]);
+      xfl[0][2] = interp(gx, ft[TFF_EZ], f[0][TFF_EZ]);
+      xfl[0][3] = interp(gx, ft[TFF_EY], f[0][TFF_EY]);
+    }
+  }
+
+  auto step = [&](auto BC, int r) {
+    constexpr int Bi = decltype(BC)::value;
+    constexpr int SM = Bi % 4, S0 = (Bi + 1) % 4, S1 = (Bi + 2) % 4, S2 = (Bi + 3) % 4;
+    constexpr int D0 = Bi % 2, D1 = (Bi + 1) % 2;
+    bool ok;
+    // rows r+2 (transported) and r+1 (fields) are loaded first and arrive during the y-face work
+    const int i2 = row_of(r + 2, ok);
+    load_q(cell(i2), ok && jvalid, S2);
+    const int i1 = row_of(r + 1, ok);
+    if (curl) tf_load_f(D, S, cell(i1), ok && jvalid, f[D1]);
+    if (r >= nx) return;
+    const int i = row_of(r, ok);
+    const bool in = i >= M.x.lo && i <= M.x.hi && yint;   // derivatives are zero outside (derivs.cpp:229-233)
+    // ---- y-faces of row r: the lower face of this lane's cell, the upper one from the next lane
+    const FaceGeom gy = face_geom(M.y, j);
+    const double dy = M.y.d[j], rdy = M.y.rd[j];
+    double tdiv[TQ], dpy[2], dvy[2], fly[4];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int o = s * 3;
+      const double vf = interp(gy, shr1(dv[D0][o + TFD_VY0]), dv[D0][o + TFD_VY0]);
+      const double ip = interp(gy, shr1(dv[D0][o + TFD_P0]), dv[D0][o + TFD_P0]);
+      dpy[s] = qdiv(shl1(ip) - ip, dy, rdy);
+      dvy[s] = qdiv(shl1(vf) - vf, dy, rdy);
+      const FaceDir fd = face_dir(vf, gy);
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const double qc = Q(S0, s * 4 + qq);
+        const double qm1 = shr1(qc);
+        const double qm2 = shr1(qm1);
+        const double qp1 = shl1(qc);
+        double d2;
+        const double Pl = face_flux(qm2, qm1, qc, qp1, gy, fd, d2);
+        tdiv[s * 4 + qq] = qdiv(shl1(Pl) - Pl, dy, rdy);
+      }
+    }
+    if (curl) {
+      const double a0 = interp(gy, shr1(f[D0][TFF_BZ]), f[D0][TFF_BZ]);
+      const double a1 = interp(gy, shr1(f[D0][TFF_BXT]), f[D0][TFF_BXT]);
+      const double a2 = interp(gy, shr1(f[D0][TFF_EZ]), f[D0][TFF_EZ]);
+      const double a3 = interp(gy, shr1(f[D0][TFF_EX]), f[D0][TFF_EX]);
+      fly[0] = qdiv(shl1(a0) - a0, dy, rdy);
+      fly[1] = qdiv(shl1(a1) - a1, dy, rdy);
+      fly[2] = qdiv(shl1(a2) - a2, dy, rdy);
+      fly[3] = qdiv(shl1(a3) - a3, dy, rdy);
+    }
+    // ---- upper x-face of row r (between r and r+1) from rows r-1 .. r+2
+    derive(S1, dv[D1]);
+    const double dx = M.x.d[i], rdx = M.x.rd[i];
+    double dpx[2], dvx[2], flx[4];
+    {
+      const FaceGeom gx = face_geom(M.x, r + 1);
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int o = s * 3;
+        xvf[D1][s] = interp(gx, dv[D0][o + TFD_VX0], dv[D1][o + TFD_VX0]);
+        xip[D1][s] = interp(gx, dv[D0][o + TFD_P0], dv[D1][o + TFD_P0]);
+        dpx[s] = qdiv(xip[D1][s] - xip[D0][s], dx, rdx);
+        dvx[s] = qdiv(xvf[D1][s] - xvf[D0][s], dx, rdx);
+        const FaceDir fd = face_dir(xvf[D1][s], gx);
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+          const int k = s * 4 + qq;
+          double d2;
+          const double Pn = face_flux(Q(SM, k), Q(S0, k), Q(S1, k), Q(S2, k), gx, fd, d2);
+          const double tx = qdiv(Pn - Px[k], dx, rdx);
+          tdiv[k] = tx + tdiv[k];
+          Px[k] = Pn;
+        }
+      }
+      if (curl) {
+        xfl[D1][0] = interp(gx, f[D0][TFF_BZ], f[D1][TFF_BZ]);
+        xfl[D1][1] = interp(gx, f[D0][TFF_BYT], f[D1][TFF_BYT]);
+        xfl[D1][2] = interp(gx, f[D0][TFF_EZ], f[D1][TFF_EZ]);
+        xfl[D1][3] = interp(gx, f[D0][TFF_EY], f[D1][TFF_EY]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) flx[k] = qdiv(xfl[D1][k] - xfl[D0][k], dx, rdx);
+      }
+    }
+    if (!out_lane) return;
+    // ---- the cell (tf_rhs_cell)
+    const size_t c = cell(i);
+    double td[TQ], dpxv[2] = {0.0, 0.0}, dpyv[2] = {0.0, 0.0}, dvv[2] = {0.0, 0.0};
+    double dbz_x = 0.0, dbz_y = 0.0, dby_x = 0.0, dbx_y = 0.0, dEz_x = 0.0, dEz_y = 0.0, dEy_x = 0.0, dEx_y = 0.0;
+#pragma unroll
+    for (int k = 0; k < TQ; ++k) td[k] = in ? tdiv[k] : 0.0;
+    if (in) {
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        dpxv[s] = dpx[s];
+        dpyv[s] = dpy[s];
+        dvv[s] = dvx[s] + dvy[s];                                     // divergence2D(v)
+      }
+      if (curl) {
+        dbz_x = flx[0]; dby_x = flx[1]; dEz_x = flx[2]; dEy_x = flx[3];
+        dbz_y = fly[0]; dbx_y = fly[1]; dEz_y = fly[2]; dEx_y = fly[3];
+      }
+    }
+    const double irho = Q(S0, 0), erho = Q(S0, 4);
+    const double ivx = dv[D0][TFD_VX0], ivy = dv[D0][TFD_VY0];
+    const double evx = dv[D0][TFD_VX1], evy = dv[D0][TFD_VY1];
+    const double in_ = qdiv(irho, K.mi, K.rmi), en_ = qdiv(erho, K.me, K.rme);
+    const double jx = in_ * E_CH * ivx - en_ * E_CH * evx;          // ideal2F.cpp:128-129
+    const double jy = in_ * E_CH * ivy - en_ * E_CH * evy;
+    const double bz = S.v[TF_BZ][c];
+    const double Ex = S.v[TF_EX][c], Ey = S.v[TF_EY][c];
+    const double ivb0 = ivy * bz, ivb1 = (-ivx) * bz, evb0 = evy * bz, evb1 = (-evx) * bz;
+    const double iFx = E_CH * in_ * (Ex + qdiv(ivb0, C_L, K.rC));
+    const double iFy = E_CH * in_ * (Ey + qdiv(ivb1, C_L, K.rC));
+    const double eFx = -E_CH * en_ * (Ex + qdiv(evb0, C_L, K.rC));
+    const double eFy = -E_CH * en_ * (Ey + qdiv(evb1, C_L, K.rC));
+    const double ipc = dv[D0][TFD_P0], epc = dv[D0][TFD_P1];
+    double k[TF_NE];
+    k[TF_IRHO] = -td[0];
+    k[TF_ERHO] = -td[4];
+    k[TF_IMX] = -td[1] - dpxv[0] + irho * D.gx[c] + iFx;
+    k[TF_IMY] = -td[2] - dpyv[0] + irho * D.gy[c] + iFy;
+    k[TF_EMX] = -td[5] - dpxv[1] + erho * D.gx[c] + eFx;
+    k[TF_EMY] = -td[6] - dpyv[1] + erho * D.gy[c] + eFy;
+    k[TF_ITE] = -td[3] - ipc * dvv[0];
+    k[TF_ETE] = -td[7] - epc * dvv[1];
+    if (curl) {                                                    // ideal2F.cpp:75-80
+      k[TF_EX] = C_L * dbz_y - 4. * PI * jx;
+      k[TF_EY] = -C_L * dbz_x - 4. * PI * jy;
+      k[TF_EZ] = C_L * (dby_x - dbx_y);
+      k[TF_BX] = -C_L * dEz_y;
+      k[TF_BY] = C_L * dEz_x;
+      k[TF_BZ] = C_L * (dEx_y - dEy_x);
+    } else {                                                       // :83-88
+      k[TF_EX] = -4. * PI * jx;
+      k[TF_EY] = -4. * PI * jy;
+      k[TF_EZ] = 0.0;
+      k[TF_BX] = 0.0;
+      k[TF_BY] = 0.0;
+      k[TF_BZ] = 0.0;
+    }
+    const double mask = in ? 1.0 : 0.0;                            // :94
+#pragma unroll
+    for (int v = 0; v < TF_NE; ++v) k[v] *= mask;
+    if (mode == TF_STAGE_STORE_K) {
+      for (int v = 0; v < TF_NE; ++v) D.k[kslot][v * N + c] = k[v];
+    } else if (mode == TF_STAGE_RK4_FINAL) {                       // evolution.cpp:119-121
+      for (int v = 0; v < TF_NE; ++v)
+        k[v] = qdiv(D.k[0][v * N + c] + k[v], 6.0, K.r6) + qdiv(D.k[1][v * N + c] + D.k[2][v * N + c], 3.0, K.r3);
+    }
+    const double stp = mult * (*D.step);
+    double nv[TF_NE];
+#pragma unroll
+    for (int v = 0; v < TF_NE; ++v) nv[v] = B.v[v][c] + k[v] * stp;
+    // enforceMinimums (ideal2F.cpp:98-105)
+    nv[TF_IRHO] = smax(qdiv(nv[TF_IRHO], K.mi, K.rmi), K.nmin) * K.mi;
+    nv[TF_ERHO] = smax(qdiv(nv[TF_ERHO], K.me, K.rme), K.nmin) * K.me;
+    nv[TF_ITE] = smax(nv[TF_ITE], K.temin);
+    nv[TF_ETE] = smax(nv[TF_ETE], K.temin);
+#pragma unroll
+    for (int v = 0; v < TF_NE; ++v) T.v[v][c] = nv[v];
+  };
+
+  for (int r = r0; r < r0 + R; r += 4) {
+    step(std::integral_constant<int, 0>{}, r);
+    step(std::integral_constant<int, 1>{}, r + 1);
+    step(std::integral_constant<int, 2>{}, r + 2);
+    step(std::integral_constant<int, 3>{}, r + 3);
+  }
+}
 }  // namespace
 
 void launch_tf_stage(const TfBatch& b, int S, int B, int T, double mult, int mode, int kslot, hipStream_t s) {
-  hipLaunchKernelGGL(k_tf_stage, dim3(b.ntiles), dim3(b.bx, b.by), 0, s, b, S, B, T, mult, mode, kslot);
+  if (b.strip_rows > 0)
+    hipLaunchKernelGGL(k_tf_stage_strip, dim3(b.nstrip_tiles), dim3(64), 0, s, b, S, B, T, mult, mode, kslot);
+  else
+    hipLaunchKernelGGL(k_tf_stage, dim3(b.ntiles), dim3(b.bx, b.by), 0, s, b, S, B, T, mult, mode, kslot);
 }
 
 void launch_tf_ghost(const TfBatch& b, int G, int kinds_mask, int maxn, hipStream_t s) {

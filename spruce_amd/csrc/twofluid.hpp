@@ -44,6 +44,7 @@ struct TfDom {
   int* counter;
   double* partial;         // one dt minimum per tile
   int tile_base, tiles_x, tiles_y;
+  int strip_base, strips_x;  // strip kernel: first strip tile of this domain, strips per column tile
 };
 
 // tile -> domain map and block shape shared by every launch of a batch
@@ -52,6 +53,9 @@ struct TfBatch {
   const int* tile_dom;     // device: domain of each tile
   int ndom, ntiles;
   int bx, by;              // block shape: bx columns (y) x by rows (x) per tile
+  const int* strip_dom;    // device: domain of each strip tile (strip stage kernel)
+  int nstrip_tiles;
+  int strip_rows;          // rows per strip of the stage kernel; 0: one thread per cell
 };
 
 enum TfStage { TF_STAGE_PLAIN = 0, TF_STAGE_STORE_K = 1, TF_STAGE_RK4_FINAL = 2 };

@@ -47,13 +47,15 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("block", ["64x4", "strip4", "strip8"])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_twofluid_matches_oracle(case):
+def test_twofluid_matches_oracle(case, block):
+    """Both stage kernels: one thread per cell (64x4 blocks) and wavefront strips of 4 and 8 rows."""
     name, nx, ny, bc, integ, curl, kw = case
-    print("case", name, nx, ny, flush=True)
+    print("case", name, nx, ny, block, flush=True)
     spec = tf_spec(nx, ny, 11, bc, integ, curl, **kw)
     ref = OracleDomain(spec)
-    gpu = GpuDomain(spec)
+    gpu = GpuDomain(spec, options={"block": block})
     try:
         for v in EVOLVED + DERIVED:
             a, b = gpu.grid(v), ref.grid(v)
@@ -130,7 +132,7 @@ def test_batch_mixed_boundaries_and_blocks():
              tf_spec(72, 48, 9, ("fixed",) * 4, "rk2", True),
              tf_spec(40, 96, 10, ("open",) * 4, "rk2", False, open_boundary_strength=0.4)]
     alone, dts_a = _run_alone(specs, [5])
-    for block in ("16x4", "32x8", "128x2"):
+    for block in ("16x4", "32x8", "128x2", "strip4", "strip8", "strip16"):
         doms = [GpuDomain(s) for s in specs]
         b = GpuBatch(doms, graph=True, block=block)
         try:
