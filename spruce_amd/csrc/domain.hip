@@ -120,6 +120,7 @@ class Domain : public DomainBase {
   DBuf kbuf[3];
   DBuf partial, step, history, scratch;
   int* err_flag = nullptr;
+  double* hscal = nullptr;   // pinned host copy of the module scalars, step size and error flag
   int dt_blocks = 0;
   int kinds_all[4] = {0, 0, 0, 0}, kinds_ucnp[4] = {0, 0, 0, 0}, kinds_live[4] = {0, 0, 0, 0};
   double s1[4] = {1, 1, 1, 1}, s2[4] = {1, 1, 1, 1};
@@ -155,6 +156,7 @@ class Domain : public DomainBase {
   }
   ~Domain() {
     if (err_flag) (void)hipFree(err_flag);
+    if (hscal) (void)hipHostFree(hscal);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -314,6 +316,7 @@ class Domain : public DomainBase {
       bhy.alloc(N);
       mod_partial.alloc(2 * (size_t)solar_blocks(mesh));
       mod_scal.alloc(2 * mods.size());
+      hip_check(hipHostMalloc((void**)&hscal, (2 * mods.size() + 2) * sizeof(double)), "hipHostMalloc");
       for (auto& m : mods) {
         m->setup(*this);
         need_dt_grid = need_dt_grid || m->needs_dt_grid();
@@ -344,13 +347,15 @@ class Domain : public DomainBase {
     for (auto& m : mods) any_iterate = any_iterate || m->needs_pre_scalars() || m->iterates();
     if (any_iterate) {
       for (size_t k = 0; k < mods.size(); ++k) mods[k]->pre_launch(*this, mod_scal.p + 2 * k);
-      std::vector<double> sc(2 * mods.size() + 1);
-      hip_check(hipMemcpyAsync(sc.data(), mod_scal.p, 2 * mods.size() * sizeof(double), hipMemcpyDeviceToHost,
-                               stream), "read module scalars");
-      hip_check(hipMemcpyAsync(sc.data() + 2 * mods.size(), step.p, sizeof(double), hipMemcpyDeviceToHost, stream),
-                "read step");
-      sync_and_check();
-      const double dt = sc[2 * mods.size()];
+      const int ns = 2 * (int)mods.size();
+      launch_gather_scalars(mod_scal.p, ns, step.p, err_flag, hscal, stream);
+      hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+      if (hscal[ns + 1] != 0.0) {
+        hip_check(hipMemset(err_flag, 0, sizeof(int)), "hipMemset");
+        throw Error(SPRUCE_E_NUMERIC, "Something went wrong in the min calculation (no finite dt)");
+      }
+      const std::vector<double> sc(hscal, hscal + ns + 1);
+      const double dt = sc[ns];
       for (size_t k = 0; k < mods.size(); ++k) mods[k]->pre_finish(dt, sc.data() + 2 * k);
       for (auto& m : mods) m->iterate(*this, dt);
       // counts are recorded after iterate(): viscosity modules fix theirs there (viscosity.cpp:125-180)

@@ -295,6 +295,15 @@ __global__ void __launch_bounds__(256) k_mhd_derived(MhdSet S, MhdStatic st, Mhd
 
 inline dim3 grid2d(int nx, int ny, dim3 b) { return dim3((ny + b.x - 1) / b.x, (nx + b.y - 1) / b.y); }
 
+// n module scalars, the step size and the error flag into host-visible memory in one launch, so the
+// host reads everything a step's subcycle counts need after a single synchronisation
+__global__ void k_gather_scalars(const double* scal, int n, const double* step, const int* err, double* out) {
+  const int t = threadIdx.x;
+  if (t < n) out[t] = scal[t];
+  if (t == n) out[n] = *step;
+  if (t == n + 1) out[n + 1] = (double)*err;
+}
+
 }  // namespace
 
 void launch_mhd_stage(const StageArgs& a, hipStream_t s) {
@@ -327,6 +336,11 @@ int mhd_dt_blocks(const Mesh& m) {
 void launch_mhd_dt(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double* dt_grid, double* partial, hipStream_t s) {
   dim3 b(64, 4);
   hipLaunchKernelGGL(k_mhd_dt, grid2d(mesh.nx, mesh.ny, b), b, 0, s, L, st, k, mesh, dt_grid, partial);
+}
+
+void launch_gather_scalars(const double* scal, int n, const double* step, const int* err, double* host_out,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_scalars, dim3(1), dim3(((n + 2 + 63) / 64) * 64), 0, s, scal, n, step, err, host_out);
 }
 
 void launch_reduce_step(const double* partial, int n, double eps, double* step, double* history, int idx, int* err,

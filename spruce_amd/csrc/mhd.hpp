@@ -95,6 +95,9 @@ void launch_mhd_ghost(MhdSet G, MhdConst k, Mesh mesh, const int* kinds, const d
 // per block (NaN dropped as std::min does); mhd_dt_blocks() blocks are launched.
 int mhd_dt_blocks(const Mesh& m);
 void launch_mhd_dt(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double* dt_grid, double* partial, hipStream_t s);
+// out[0..n) = scal, out[n] = *step, out[n+1] = *err (out: host memory from hipHostMalloc)
+void launch_gather_scalars(const double* scal, int n, const double* step, const int* err, double* host_out,
+                           hipStream_t s);
 void launch_reduce_step(const double* partial, int n, double eps, double* step, double* history, int idx,
                         int* err, hipStream_t s);
 void launch_mhd_rhs_full(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, double* out, hipStream_t s);
