@@ -34,16 +34,18 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 E_MHD, S_MHD = 8, 5
 
 
-def measured_traffic(workload: str, kernel: str, rows: str):
-    """HBM bytes per stage launch from the committed FETCH_SIZE/WRITE_SIZE counter summary
-    (profiles/hbm_traffic_mhd_stage.md), for the same workload and launch shape; None otherwise."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_mhd_stage.md")
+def measured_traffic(workload: str, kernel: str, rows: str, summary: str = "hbm_traffic_mhd_stage.md"):
+    """HBM bytes per stage launch from a committed FETCH_SIZE/WRITE_SIZE counter summary under
+    profiles/ (lines `traffic <workload> <kernel> [<rows>] <bytes>`), for the same workload and
+    launch shape; None otherwise."""
+    path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
+    want = [workload, kernel] + ([rows] if rows else [])
     for line in open(path):
         f = line.split()
-        if len(f) == 5 and f[0] == "traffic" and f[1:4] == [workload, kernel, rows]:
-            return float(f[4])
+        if len(f) == len(want) + 2 and f[0] == "traffic" and f[1:-1] == want:
+            return float(f[-1])
     return None
 
 
@@ -187,7 +189,7 @@ def bench_batch(args) -> dict:
                    "domains": len(specs), "cells": cells, "edges": edges, "block": shape},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "tf_stage",
+                     "traffic": measured_traffic("batch2f", shape, "", "tf_stage_batch64.md"), "kernel": "tf_stage",
                      "kernel_ms": stage_ms, "stage_bytes_per_cell": bpc},
         "sequential": {"value": seq_value, "steps": seq_steps, "by_shape": seq,
                        "batched_over_sequential": value / seq_value},
