@@ -109,7 +109,9 @@ int spruce_gpu_device_grid(spruce_gpu_domain* d, const char* name, double** dev_
  *                    wavefronts, else one thread per cell) | "strip" (wavefront strips marching
  *                    along x) | "cell" (one thread per cell, faces recomputed per cell);
  *   "strip_rows"   = rows per wavefront strip, a multiple of 4 (default: chosen from the grid size);
- *   "block"        = "<columns>x<rows>" thread block of the ideal_2F kernels (default 64x4). */
+ *   "block"        = ideal_2F launch shape: "<columns>x<rows>" thread block of every kernel, stage
+ *                    one thread per cell (default 64x4), or "strip<rows>" (rows a multiple of 4
+ *                    up to 64): the stage kernel in wavefront strips, the other kernels in 64x4. */
 int spruce_gpu_set_option(spruce_gpu_domain* d, const char* key, const char* value);
 
 /* Average duration (ms) of each kernel of the most recent spruce_gpu_step call, measured with HIP
@@ -154,7 +156,7 @@ int spruce_gpu_adjoint_state(spruce_gpu_domain* d, const double* grad_evolved, d
 typedef struct spruce_gpu_batch spruce_gpu_batch;
 spruce_gpu_batch* spruce_gpu_batch_create(spruce_gpu_domain* const* domains, int32_t n);
 /* "graph" = "true" (default: one step is captured once into a HIP graph and replayed) | "false";
- * "block" = "<columns>x<rows>" thread block of every launch. */
+ * "block" = launch shape, as the domain option "block" ("<columns>x<rows>" or "strip<rows>"). */
 int spruce_gpu_batch_set_option(spruce_gpu_batch* b, const char* key, const char* value);
 /* Times trial_steps steps for each candidate block shape, keeps the fastest and restores the state
  * of every member, so tuning does not advance the simulation.  chosen receives "<cols>x<rows>". */
