@@ -47,7 +47,25 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    build_runner(verbose)
     return LIB
+
+
+RUNNER = os.path.join(HERE, "spruce_run")
+
+
+def build_runner(verbose: bool = False) -> str:
+    """The command-line program (csrc/run_main.cpp): host C++ on the C ABI, linked to the library."""
+    rocm_lib = "/opt/rocm/lib"
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "run_main.cpp"),
+           "-L" + HERE, "-lspruce_gpu", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link," + rocm_lib, "-o", RUNNER + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("runner build failed:\n" + r.stdout + r.stderr)
+    os.replace(RUNNER + ".tmp", RUNNER)
+    return RUNNER
 
 
 if __name__ == "__main__":
