@@ -206,6 +206,52 @@ def _cases() -> Dict[str, Tuple[Callable[[], StateFile], Callable[[], ConfigFile
                                                                 ("physical_viscosity", [("coeff", "5e-17"),
                                                                                         ("heating_on", "false"),
                                                                                         ("epsilon", "0.5")])]))
+    # source-term modules of source/modules/solar: localized heating, mass and momentum injection,
+    # ambient heating sink, field heating, divergence cleaning
+    HB = ("periodic", "periodic", "fixed", "open")
+    c["lheat_ramp_rk2"] = (lambda: random_mhd_state(20, 22, 61),
+                           lambda: _mhd(HB, "rk2", 7, modules=[("localized_heating", [
+                               ("start_time", "0.03"), ("duration", "0.4"), ("max_heating_rate", "2.0"),
+                               ("stddev_x", "3.0"), ("stddev_y", "4.0"), ("center_x", "1.5"), ("center_y", "9.0"),
+                               ("ramp_time", "0.1")])]))
+    c["lheat_periodic_euler"] = (lambda: random_mhd_state(22, 18, 62),
+                                 lambda: _mhd(P4, "euler", 6, modules=[("localized_heating", [
+                                     ("start_time", "0.0"), ("duration", "1e9"), ("max_heating_rate", "1.5"),
+                                     ("stddev_x", "2.5"), ("stddev_y", "2.0"), ("center_x", "20.0"),
+                                     ("center_y", "1.0"), ("ramp_time", "0.0")])]))
+    c["mass_inj_rk4"] = (lambda: random_mhd_state(18, 22, 63, stretch=True),
+                         lambda: _mhd(("periodic", "periodic", "open", "open"), "rk4", 5, modules=[("mass_injection", [
+                             ("start_time", "0.02"), ("duration", "0.1"), ("max_injection_rate", "3e9"),
+                             ("stddev_x", "2.0"), ("stddev_y", "3.0"), ("center_x", "16.0"), ("center_y", "11.0")])]))
+    c["mom_inj_rot_rk2"] = (lambda: random_mhd_state(20, 20, 64),
+                            lambda: _mhd(P4, "rk2", 6, modules=[("momentum_injection", [
+                                ("start_time", "0.0"), ("duration", "1e9"), ("max_accel", "1e8"),
+                                ("stddev_x", "3.0"), ("stddev_y", "1.5"), ("center_x", "2.0"), ("center_y", "18.5"),
+                                ("dir_x", "1.0"), ("dir_y", "-0.5"), ("template_angle", "30.0")])]))
+    # With periodic bounds the reference folds the shifted templates in with min() for either sign
+    # (momentuminjection.cpp:50-59), which zeroes a positive component: the case above keeps that (its
+    # x component vanishes), the one below uses closed bounds so that both components act.
+    c["mom_inj_osc_euler"] = (lambda: random_mhd_state(18, 20, 65),
+                              lambda: _mhd(("open", "reflect", "fixed", "open"), "euler", 5, modules=[("momentum_injection", [
+                                  ("start_time", "0.01"), ("duration", "1e9"), ("max_accel", "2e8"),
+                                  ("stddev_x", "2.0"), ("stddev_y", "2.0"), ("center_x", "9.0"), ("center_y", "10.0"),
+                                  ("dir_x", "0.3"), ("dir_y", "-1.0"), ("oscillatory", "true"),
+                                  ("oscillation_period", "0.15")])]))
+    c["sink_exp_heat_rk2"] = (lambda: random_mhd_state(20, 24, 66),
+                              lambda: _mhd(HB, "rk2", 5, modules=[
+                                  ("ambient_heating", [("heating_rate", "1.0")]),
+                                  ("ambient_heating_sink", [("exp_mode", "true"), ("exp_base_heating_rate", "0.8"),
+                                                            ("exp_scale_height", "8e7"), ("center_x", "9e7"),
+                                                            ("half_width", "6e7")])]))
+    c["sink_const_euler"] = (lambda: random_mhd_state(18, 20, 67),
+                             lambda: _mhd(P4, "euler", 5, modules=[("ambient_heating_sink", [("heating_rate", "0.3")])]))
+    c["fheat_rk2"] = (lambda: random_mhd_state(20, 22, 68),
+                      lambda: _mhd(HB, "rk2", 5, modules=[("field_heating", [
+                          ("coeff", "3e3"), ("current_pow", "1.0"), ("b_pow", "0.5"), ("n_pow", "-0.5"),
+                          ("roc_pow", "0.5")])]))
+    c["divclean_rk2"] = (lambda: random_mhd_state(22, 20, 69, stretch=True),
+                         lambda: _mhd(("periodic", "periodic", "reflect", "fixed"), "rk2", 5, modules=[
+                             ("div_cleaning", [("epsilon", "0.1"), ("time_scale", "0.01")])]))
     # single-fluid variants: two energies, and the conservative total-energy form
     E2 = ["rho", "mom_x", "mom_y", "i_thermal_energy", "e_thermal_energy", "bi_x", "bi_y", "i_temp", "e_temp",
           "press", "b_hat_x", "dt"]
