@@ -23,6 +23,7 @@
 #include "solar.hpp"
 #include "visc.hpp"
 #include "pvisc.hpp"
+#include "sources.hpp"
 #include "adjoint.hpp"
 #include "crpow.hpp"
 #include "tf_host.hpp"
@@ -72,8 +73,10 @@ struct GpuModule {
   virtual bool needs_dt_grid() const { return false; }
   // false when pre_launch/pre_finish do nothing (no subcycle-count reduction to read back)
   virtual bool needs_pre_scalars() const { return true; }
-  // iterate() has work to do (needs the step size on the host)
+  // iterate() has work to do
   virtual bool iterates() const { return true; }
+  // the step size (and so the simulation time) is needed on the host during the step
+  virtual bool needs_host_dt() const { return iterates(); }
   // after every module's iterate, before the integrator (the live grids are final for this step)
   virtual void before_integrator(Domain&) {}
   // RHS terms of this module on grid set S (computeTimeDerivativesModule), appended to a stage
@@ -343,10 +346,11 @@ class Domain : public DomainBase {
     bool bh = false;
     for (auto& m : mods) bh = bh || m->needs_bhat();
     if (bh) timed("bhat", [&] { launch_bhat(L.view(), stat(), mesh, bhx.p, bhy.p, stream); });
-    bool any_iterate = false;
-    for (auto& m : mods) any_iterate = any_iterate || m->needs_pre_scalars() || m->iterates();
-    if (any_iterate) {
-      for (size_t k = 0; k < mods.size(); ++k) mods[k]->pre_launch(*this, mod_scal.p + 2 * k);
+    bool host = false;
+    for (auto& m : mods) host = host || m->needs_pre_scalars() || m->needs_host_dt();
+    for (size_t k = 0; k < mods.size(); ++k) mods[k]->pre_launch(*this, mod_scal.p + 2 * k);
+    double dt = std::nan("");
+    if (host) {
       const int ns = 2 * (int)mods.size();
       launch_gather_scalars(mod_scal.p, ns, step.p, err_flag, hscal, stream);
       hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
@@ -355,16 +359,21 @@ class Domain : public DomainBase {
         throw Error(SPRUCE_E_NUMERIC, "Something went wrong in the min calculation (no finite dt)");
       }
       const std::vector<double> sc(hscal, hscal + ns + 1);
-      const double dt = sc[ns];
+      dt = sc[ns];
       for (size_t k = 0; k < mods.size(); ++k) mods[k]->pre_finish(dt, sc.data() + 2 * k);
-      for (auto& m : mods) m->iterate(*this, dt);
-      // counts are recorded after iterate(): viscosity modules fix theirs there (viscosity.cpp:125-180)
-      for (size_t k = 0; k < mods.size(); ++k) sub_hist[(size_t)s * mods.size() + k] = mods[k]->nsub;
-    } else {
-      for (size_t k = 0; k < mods.size(); ++k) sub_hist[(size_t)s * mods.size() + k] = mods[k]->nsub;
     }
+    host_dt = dt;
+    have_host_dt = host;
+    for (auto& m : mods)
+      if (m->iterates()) m->iterate(*this, dt);
+    // counts are recorded after iterate(): viscosity modules fix theirs there (viscosity.cpp:125-180)
+    for (size_t k = 0; k < mods.size(); ++k) sub_hist[(size_t)s * mods.size() + k] = mods[k]->nsub;
     for (auto& m : mods) m->before_integrator(*this);
   }
+  // the step size of the current step and the simulation time at its start, on the host (valid
+  // while have_host_dt: some module asked for them)
+  double host_dt = 0.0, tstep = 0.0;
+  bool have_host_dt = false;
 
   MhdStatic stat() { return MhdStatic{bex.p, bey.p, bez.p, gx.p, gy.p}; }
 
@@ -444,6 +453,7 @@ class Domain : public DomainBase {
     }
     ghost(L, kinds_all, GHOST_NORMAL);
     for (auto& m : mods) m->post(*this);
+    if (have_host_dt) tstep += host_dt;               // evolution.cpp:80, after the post-iterate hooks
     finish_step(s + 1);
   }
 
@@ -454,6 +464,7 @@ class Domain : public DomainBase {
     if ((int)history.n < nsteps + 1) history.alloc(nsteps + 1);
     hip_check(hipMemcpyAsync(history.p, step.p, sizeof(double), hipMemcpyDeviceToDevice, stream), "copy step");
     sub_hist.assign((size_t)nsteps * std::max<size_t>(1, mods.size()), -1);
+    tstep = time;
     for (int s = 0; s < nsteps; ++s) advance(s);
     sync_and_check();
     if (clock.enabled) clock.collect();
@@ -1257,6 +1268,329 @@ struct PvModule : GpuModule {
   }
 };
 
+// ---------------------------------------------------------------------------------------------
+// Source-term modules: localized heating, mass and momentum injection, ambient heating sink,
+// field heating, divergence cleaning (source/modules/solar/*.cpp)
+// ---------------------------------------------------------------------------------------------
+using Opts = std::vector<std::pair<std::string, std::string>>;
+
+// SolarUtils::GaussianGrid (solarutils.cpp:65-82), on the host with the C library's exp and pow
+static std::vector<double> gaussian_grid(int nx, int ny, double mn, double mx, double sx, double sy, double cx,
+                                         double cy) {
+  std::vector<double> gx(nx), gy(ny), r((size_t)nx * ny);
+  for (int i = 0; i < nx; ++i) gx[i] = std::exp(-0.5 * std::pow(((double)i - cx) / sx, 2.0));
+  for (int j = 0; j < ny; ++j) gy[j] = std::exp(-0.5 * std::pow(((double)j - cy) / sy, 2.0));
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) r[(size_t)i * ny + j] = (mx - mn) * gx[i] * gy[j] + mn;
+  return r;
+}
+// SolarUtils::GaussianGridRotated (solarutils.cpp:85-98)
+static std::vector<double> gaussian_grid_rot(int nx, int ny, double mn, double mx, double sx, double sy, double cx,
+                                             double cy, double ang) {
+  require(ang > -90.0 && ang < 90.0, SPRUCE_E_ARG, "GaussianGridRotated requires angles between -90 deg and +90 deg");
+  if (ang == 0.0) return gaussian_grid(nx, ny, mn, mx, sx, sy, cx, cy);
+  std::vector<double> r((size_t)nx * ny);
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      const double xe = ((double)i - cx) * std::cos(ang * PI / 180.0) - ((double)j - cy) * std::sin(ang * PI / 180.0);
+      const double ye = ((double)i - cx) * std::sin(ang * PI / 180.0) + ((double)j - cy) * std::cos(ang * PI / 180.0);
+      r[(size_t)i * ny + j] = (mx - mn) * std::exp(-0.5 * std::pow(xe / sx, 2.0)) * std::exp(-0.5 * std::pow(ye / sy, 2.0)) + mn;
+    }
+  return r;
+}
+static void vmax_into(std::vector<double>& a, const std::vector<double>& b) {
+  for (size_t k = 0; k < a.size(); ++k) a[k] = smax(a[k], b[k]);
+}
+static void vmin_into(std::vector<double>& a, const std::vector<double>& b) {
+  for (size_t k = 0; k < a.size(); ++k) a[k] = smin(a[k], b[k]);
+}
+// The clipped Gaussian template of localized heating and mass injection with its periodic images
+// (localizedheating.cpp:34-49, massinjection.cpp:32-47).
+static std::vector<double> injection_template(const Domain& d, double rate, double sx, double sy, double cx, double cy) {
+  auto g = [&](double x, double y) {
+    std::vector<double> v = gaussian_grid(d.nx, d.ny, -1.0e-2 * rate, rate, sx, sy, x, y);
+    for (double& e : v) e = smax(e, 0.0);
+    return v;
+  };
+  std::vector<double> t = g(cx, cy);
+  if (d.desc.bc[0] == SPRUCE_BC_PERIODIC) {
+    vmax_into(t, g(cx + (double)d.nx, cy));
+    vmax_into(t, g(cx - (double)d.nx, cy));
+  }
+  if (d.desc.bc[2] == SPRUCE_BC_PERIODIC) {
+    vmax_into(t, g(cx, cy + (double)d.ny));
+    vmax_into(t, g(cx, cy - (double)d.ny));
+  }
+  return t;
+}
+static void upload_vec(DBuf& b, const std::vector<double>& v) {
+  b.alloc(v.size());
+  hip_check(hipMemcpy(b.p, v.data(), v.size() * 8, hipMemcpyHostToDevice), "upload template");
+}
+
+// Base of the modules that act only after the integrator, at a simulation time window.
+struct PostSourceModule : GpuModule {
+  bool needs_pre_scalars() const override { return false; }
+  bool iterates() const override { return false; }
+  void pre_launch(Domain&, double*) override {}
+  void pre_finish(double, const double*) override {}
+  void iterate(Domain&, double) override {}
+};
+
+// LocalizedHeating (localizedheating.cpp)
+struct LocalHeatModule : PostSourceModule {
+  double start = 0.0, ramp = 0.0, dur = 0.0, rate = 0.0, sx = 1.0, sy = 1.0, cx = 0.0, cy = 0.0, msf = 0.0;
+  DBuf tmpl;
+  explicit LocalHeatModule(const Opts& o) {  // :14-29
+    name = "localized_heating";
+    for (auto& kv : o) {
+      const std::string& k = kv.first;
+      const std::string& v = kv.second;
+      if (k == "start_time") start = std::stod(v);
+      else if (k == "duration") dur = std::stod(v);
+      else if (k == "max_heating_rate") rate = std::stod(v);
+      else if (k == "stddev_x") sx = std::stod(v);
+      else if (k == "stddev_y") sy = std::stod(v);
+      else if (k == "center_x") cx = std::stod(v);
+      else if (k == "center_y") cy = std::stod(v);
+      else if (k == "ramp_time") ramp = std::stod(v);
+      else if (k == "ms_electron_heating_fraction") msf = std::stod(v);
+    }
+  }
+  bool needs_host_dt() const override { return true; }
+  // the template depends on positions only: the reference builds it at its first preIterateModule
+  // (:31-50), this build at set-up
+  void setup(Domain& d) override {
+    require(msf >= 0.0 && msf <= 1.0, SPRUCE_E_ARG,
+            "Localized Heating MS electron heating fraction must be between 0 and 1");
+    upload_vec(tmpl, injection_template(d, rate, sx, sy, cx, cy));
+  }
+  void post(Domain& d) override {  // :52-66
+    const double now = d.tstep;
+    if (now < start || now > start + dur) return;
+    const double t = now - start;
+    double rf = 1.0;
+    if (t < ramp && t <= 0.5 * dur) rf = t / ramp;
+    else if (t > dur - ramp && t > 0.5 * dur) rf = (dur - t) / ramp;
+    const double a = d.host_dt * rf;
+    d.timed("source", [&] { launch_source(d.L.te.p, tmpl.p, nullptr, a, 0.0, nullptr, SRC_ADD, d.mesh, d.stream); });
+    d.propagate_live();
+  }
+};
+
+// MassInjection (massinjection.cpp)
+struct MassInjModule : PostSourceModule {
+  double start = 0.0, dur = 0.0, rate = 0.0, sx = 1.0, sy = 1.0, cx = 0.0, cy = 0.0;
+  DBuf tmpl;
+  explicit MassInjModule(const Opts& o) {  // :14-27
+    name = "mass_injection";
+    for (auto& kv : o) {
+      const std::string& k = kv.first;
+      if (k == "start_time") start = std::stod(kv.second);
+      else if (k == "duration") dur = std::stod(kv.second);
+      else if (k == "max_injection_rate") rate = std::stod(kv.second);
+      else if (k == "stddev_x") sx = std::stod(kv.second);
+      else if (k == "stddev_y") sy = std::stod(kv.second);
+      else if (k == "center_x") cx = std::stod(kv.second);
+      else if (k == "center_y") cy = std::stod(kv.second);
+    }
+  }
+  bool needs_host_dt() const override { return true; }
+  void setup(Domain& d) override { upload_vec(tmpl, injection_template(d, rate, sx, sy, cx, cy)); }  // :29-48
+  void post(Domain& d) override {  // :49-53
+    if (d.tstep < start || d.tstep > start + dur) return;
+    d.timed("source", [&] {
+      launch_source(d.L.rho.p, tmpl.p, nullptr, d.host_dt, d.kc.m, nullptr, SRC_ADD_SCALED, d.mesh, d.stream);
+    });
+    d.propagate_live();
+  }
+};
+
+// MomentumInjection (momentuminjection.cpp)
+struct MomInjModule : PostSourceModule {
+  double start = 0.0, dur = 0.0, amax = 0.0, sx = 1.0, sy = 1.0, cx = 0.0, cy = 0.0, ang = 0.0, period = 1.0;
+  double dir[2] = {0.0, 0.0};
+  bool osc = false;
+  DBuf tmpl[2];
+  explicit MomInjModule(const Opts& o) {  // :17-35
+    name = "momentum_injection";
+    for (auto& kv : o) {
+      const std::string& k = kv.first;
+      const std::string& v = kv.second;
+      if (k == "start_time") start = std::stod(v);
+      else if (k == "duration") dur = std::stod(v);
+      else if (k == "max_accel") amax = std::stod(v);
+      else if (k == "stddev_x") sx = std::stod(v);
+      else if (k == "stddev_y") sy = std::stod(v);
+      else if (k == "center_x") cx = std::stod(v);
+      else if (k == "center_y") cy = std::stod(v);
+      else if (k == "dir_x") dir[0] = std::stod(v);
+      else if (k == "dir_y") dir[1] = std::stod(v);
+      else if (k == "template_angle") ang = std::stod(v);
+      else if (k == "oscillatory") osc = v == "true";
+      else if (k == "oscillation_period") period = std::stod(v);
+    }
+  }
+  bool needs_host_dt() const override { return true; }
+  void setup(Domain& d) override {  // :37-68
+    require(!(dir[0] == 0.0 && dir[1] == 0.0), SPRUCE_E_ARG,
+            "Momentum Injection module must be given a nonzero acceleration direction");
+    const double mag = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1]);
+    dir[0] = dir[0] / mag;
+    dir[1] = dir[1] / mag;
+    for (int c = 0; c < 2; ++c) {
+      const double m = dir[c];
+      auto g = [&](double x, double y) { return gaussian_grid_rot(d.nx, d.ny, -1.0e-2 * m, m, sx, sy, x, y, ang); };
+      std::vector<double> t = g(cx, cy);
+      // the periodic images are folded in with min() for either sign of the component (:47-64)
+      if (d.desc.bc[0] == SPRUCE_BC_PERIODIC) {
+        vmin_into(t, g(cx + (double)d.nx, cy));
+        vmin_into(t, g(cx - (double)d.nx, cy));
+      }
+      if (d.desc.bc[2] == SPRUCE_BC_PERIODIC) {
+        vmin_into(t, g(cx, cy + (double)d.ny));
+        vmin_into(t, g(cx, cy - (double)d.ny));
+      }
+      for (double& e : t) e = m > 0.0 ? smax(e, 0.0) : smin(e, 0.0);
+      upload_vec(tmpl[c], t);
+    }
+  }
+  void post(Domain& d) override {  // :70-76
+    if (d.tstep < start || d.tstep > start + dur) return;
+    const double of = osc ? std::sin(2.0 * PI * (d.tstep - start) / period) : 1.0;
+    const double a = d.host_dt * (of * amax);
+    d.timed("source", [&] {
+      launch_source(d.L.mx.p, tmpl[0].p, d.L.rho.p, a, 0.0, nullptr, SRC_ADD_WEIGHTED, d.mesh, d.stream);
+      launch_source(d.L.my.p, tmpl[1].p, d.L.rho.p, a, 0.0, nullptr, SRC_ADD_WEIGHTED, d.mesh, d.stream);
+    });
+    d.propagate_live();
+  }
+};
+
+// AmbientHeatingSink (ambientheatingsink.cpp)
+struct HeatSinkModule : PostSourceModule {
+  double rate = 0.0, exp_base = 0.0, exp_h = 1.0, cx = 0.0, hw = 1.0, msf = 0.0;
+  bool exp_mode = false;
+  DBuf red;
+  explicit HeatSinkModule(const Opts& o) {  // :12-25
+    name = "ambient_heating_sink";
+    for (auto& kv : o) {
+      const std::string& k = kv.first;
+      const std::string& v = kv.second;
+      if (k == "heating_rate") rate = std::stod(v);
+      if (k == "exp_mode") exp_mode = v == "true";
+      else if (k == "exp_base_heating_rate") exp_base = std::stod(v);
+      else if (k == "exp_scale_height") exp_h = std::stod(v);
+      else if (k == "center_x") cx = std::stod(v);
+      else if (k == "half_width") hw = std::stod(v);
+      else if (k == "ms_electron_heating_fraction") msf = std::stod(v);
+    }
+  }
+  bool needs_host_dt() const override { return false; }
+  // the reduction depends on positions only: built once on the host with the C library's exp (:27-33)
+  void setup(Domain& d) override {
+    const auto& x = d.host_grids.at("pos_x");
+    const auto& y = d.host_grids.at("pos_y");
+    std::vector<double> r(d.N);
+    for (int i = 0; i < d.nx; ++i)
+      for (int j = 0; j < d.ny; ++j) {
+        const size_t c = (size_t)i * d.ny + j;
+        const double mask = (i >= d.xl && i <= d.xu && j >= d.yl && j <= d.yu) ? 1.0 : 0.0;
+        if (exp_mode) {
+          const double w = (x[c] - cx) / hw;
+          r[c] = mask * exp_base * std::exp(-1.0 * y[c] / exp_h) * smax(1.0 - w * w, 0.0);
+        } else {
+          r[c] = mask * rate;
+        }
+      }
+    require(msf >= 0.0 && msf <= 1.0, SPRUCE_E_ARG,
+            "Ambient Heating Sink MS electron heating fraction must be between 0 and 1");
+    upload_vec(red, r);
+  }
+  void post(Domain& d) override {  // :35-37
+    d.timed("source", [&] { launch_source(d.L.te.p, red.p, nullptr, 0.0, 0.0, d.step.p, SRC_SUB, d.mesh, d.stream); });
+    d.propagate_live();
+  }
+};
+
+// FieldHeating (fieldheating.cpp)
+struct FieldHeatModule : GpuModule {
+  FheatParams p{};
+  bool inactive = false;
+  DBuf heat;
+  explicit FieldHeatModule(const Opts& o) {  // :15-28
+    name = "field_heating";
+    p.cc = C_LIGHT / (4.0 * PI);
+    for (auto& kv : o) {
+      const std::string& k = kv.first;
+      const std::string& v = kv.second;
+      if (k == "output_to_file") require(v != "true", SPRUCE_E_UNSUPPORTED, "field_heating file output is not available in this build");
+      else if (k == "coeff") p.coeff = std::stod(v);
+      else if (k == "current_pow") p.cpow = std::stod(v);
+      else if (k == "b_pow") p.bpow = std::stod(v);
+      else if (k == "n_pow") p.npow = std::stod(v);
+      else if (k == "roc_pow") p.rpow = std::stod(v);
+      else if (k == "inactive_mode") inactive = v == "true";
+    }
+  }
+  bool needs_pre_scalars() const override { return false; }
+  bool needs_host_dt() const override { return false; }   // the step size is read on the device
+  bool needs_bhat() const override { return p.coeff != 0.0 && p.rpow != 0.0; }
+  void setup(Domain& d) override { heat.alloc(d.N); }
+  void pre_launch(Domain& d, double*) override {  // preIterateModule -> computeHeating (:34-54)
+    d.timed("fheat", [&] { launch_fheat_pre(d.L.view(), d.stat(), d.bhx.p, d.bhy.p, p, d.mesh, heat.p, d.stream); });
+  }
+  void pre_finish(double, const double*) override {}
+  void iterate(Domain& d, double) override {  // :56-61
+    d.timed("fheat", [&] { launch_fheat_apply(d.L.te.p, heat.p, d.step.p, d.mesh, inactive ? 0 : 1, d.stream); });
+    if (!inactive) d.propagate_live();
+  }
+};
+
+// DivCleaning (divcleaning.cpp)
+struct DivCleanModule : GpuModule {
+  double eps = 0.1, ts = 1.0, h = 0.0;
+  DBuf coeff, bx[2], by[2], dxb, dyb;
+  explicit DivCleanModule(const Opts& o) {  // :12-20
+    name = "div_cleaning";
+    for (auto& kv : o) {
+      if (kv.first == "epsilon") eps = std::stod(kv.second);
+      if (kv.first == "time_scale") ts = std::stod(kv.second);
+    }
+  }
+  bool needs_pre_scalars() const override { return false; }
+  void pre_launch(Domain&, double*) override {}
+  void pre_finish(double, const double*) override {}
+  // coeff = (1 / (1/dx^2 + 1/dy^2)) / 2 / time_scale, on the host (:22-26)
+  void setup(Domain& d) override {
+    const auto& gx = d.host_grids.at("d_x");
+    const auto& gy = d.host_grids.at("d_y");
+    std::vector<double> c(d.N);
+    for (size_t k = 0; k < d.N; ++k) c[k] = (1.0 / (1.0 / (gx[k] * gx[k]) + 1.0 / (gy[k] * gy[k]))) / 2. / ts;
+    upload_vec(coeff, c);
+    for (auto* b : {&bx[0], &bx[1], &by[0], &by[1], &dxb, &dyb}) b->alloc(d.N);
+  }
+  // the subcycle count is fixed by the step size (computeNumSubcycles, :48-50); the cleaning itself
+  // runs after the integrator
+  void iterate(Domain&, double dt) override {
+    nsub = subcycle_count(dt, eps * ts);
+    h = dt / ((double)nsub);
+  }
+  void post(Domain& d) override {  // :28-46
+    int cur = 0;
+    d.timed("divclean", [&] {
+      launch_field_sum(d.L.bx.p, d.L.by.p, d.bex.p, d.bey.p, d.N, bx[0].p, by[0].p, d.stream);
+      for (int s = 0; s < nsub; ++s) {
+        launch_dc_inner(bx[cur].p, by[cur].p, d.mesh, dxb.p, dyb.p, d.stream);
+        launch_dc_update(bx[cur].p, by[cur].p, dxb.p, dyb.p, d.L.bx.p, d.L.by.p, d.bex.p, d.bey.p, coeff.p, h, d.mesh,
+                         bx[1 - cur].p, by[1 - cur].p, d.stream);
+        cur = 1 - cur;
+      }
+    });
+    d.propagate_live();
+  }
+};
+
 void Domain::add_module(const std::string& name, const std::vector<std::pair<std::string, std::string>>& opts) {
   require(!ready, SPRUCE_E_STATE, "modules must be added before setup");
   static const char* known[] = {"ambient_heating", "radiative_losses", "thermal_conduction", "localized_heating",
@@ -1272,6 +1606,12 @@ void Domain::add_module(const std::string& name, const std::vector<std::pair<std
   else if (name == "artificial_viscosity") mods.push_back(std::make_unique<ViscModule>(opts));
   else if (name == "ambient_heating") mods.push_back(std::make_unique<HeatModule>(opts));
   else if (name == "physical_viscosity") mods.push_back(std::make_unique<PvModule>(opts));
+  else if (name == "localized_heating") mods.push_back(std::make_unique<LocalHeatModule>(opts));
+  else if (name == "mass_injection") mods.push_back(std::make_unique<MassInjModule>(opts));
+  else if (name == "momentum_injection") mods.push_back(std::make_unique<MomInjModule>(opts));
+  else if (name == "ambient_heating_sink") mods.push_back(std::make_unique<HeatSinkModule>(opts));
+  else if (name == "field_heating") mods.push_back(std::make_unique<FieldHeatModule>(opts));
+  else if (name == "div_cleaning") mods.push_back(std::make_unique<DivCleanModule>(opts));
   else throw Error(SPRUCE_E_UNSUPPORTED, "module " + name + " is not available in this build");
   modules.emplace_back(name, opts);
 }

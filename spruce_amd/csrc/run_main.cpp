@@ -158,10 +158,17 @@ const std::vector<std::string> OUTPUT_KEYS = {"output_to_file", "visc_output_vis
 struct Module {
   std::string name;
   std::vector<std::pair<std::string, std::string>> opts;
+  // the last setting of a key wins, as the modules' parsers assign in order
   std::string opt(const std::string& k) const {
+    std::string v;
     for (auto& kv : opts)
-      if (kv.first == k) return kv.second;
-    return "";
+      if (kv.first == k) v = kv.second;
+    return v;
+  }
+  bool has(const std::string& k) const {
+    for (auto& kv : opts)
+      if (kv.first == k) return true;
+    return false;
   }
 };
 
@@ -523,8 +530,8 @@ class Runner {
       f << fmt(n, -1);
     }
   }
-  // PlasmaDomain::printUpdate (fileio.cpp:279-291) with the module messages this build has
-  // (thermalconduction.cpp:242-245, radiativelosses.cpp:169-172, ambientheating.cpp:51-54)
+  // PlasmaDomain::printUpdate (fileio.cpp:283-295): the time printed is that at the start of the
+  // step, and each module adds its commandLineMessage unless it is empty (modulehandler.cpp:114-122)
   void print_update(double dt, const std::vector<int32_t>& subs) {
     std::cout << "Iter: " << iter_;
     if (max_iterations_ > 0) std::cout << "/" << max_iterations_;
@@ -532,14 +539,51 @@ class Runner {
     if (max_time_ > 0.0) std::cout << "/" << max_time_;
     std::cout << "|dt: " << dt;
     for (size_t k = 0; k < cfg_.modules.size(); ++k) {
-      const Module& m = cfg_.modules[k];
-      const std::string na = m.opt("inactive_mode") == "true" ? " (Not Applied)" : "";
-      if (m.name == "thermal_conduction") std::cout << "|Thermal Subcycles: " << std::to_string(subs[k]) << na;
-      else if (m.name == "radiative_losses") std::cout << "|Radiative Subcycles: " << std::to_string(subs[k]) << na;
-      else if (m.name == "ambient_heating")
-        std::cout << "|" << (m.opt("exp_mode") == "true" ? "Ambient Heating On (Exp. Mode)" : "Ambient Heating On");
+      const std::string msg = module_message(cfg_.modules[k], subs[k]);
+      if (!msg.empty()) std::cout << "|" << msg;
     }
     std::cout << std::endl;
+  }
+  static bool opt_flag(const Module& m, const std::string& k, bool def) {
+    return m.has(k) ? m.opt(k) == "true" : def;
+  }
+  static double opt_num(const Module& m, const std::string& k) {
+    const std::string v = m.opt(k);
+    return v.empty() ? 0.0 : std::stod(v);
+  }
+  // commandLineMessage of the modules this build runs
+  std::string module_message(const Module& m, int32_t sub) const {
+    const std::string na = opt_flag(m, "inactive_mode", false) ? " (Not Applied)" : "";
+    if (m.name == "thermal_conduction") return "Thermal Subcycles: " + std::to_string(sub) + na;  // thermalconduction.cpp:242-245
+    if (m.name == "radiative_losses") return "Radiative Subcycles: " + std::to_string(sub) + na;  // radiativelosses.cpp:169-172
+    if (m.name == "ambient_heating" || m.name == "ambient_heating_sink")  // ambientheating.cpp:51-54, ambientheatingsink.cpp:44-47
+      return m.opt("exp_mode") == "true" ? "Ambient Heating On (Exp. Mode)" : "Ambient Heating On";
+    if (m.name == "localized_heating" || m.name == "mass_injection" || m.name == "momentum_injection") {
+      // localizedheating.cpp:69-78, massinjection.cpp:55-64, momentuminjection.cpp:78-87
+      std::ostringstream oss;
+      oss.precision(4);
+      oss << opt_num(m, "center_x") << "," << opt_num(m, "center_y");
+      const char* head = m.name == "localized_heating" ? "Heating at " :
+                         m.name == "mass_injection" ? "Mass injection at " : "Momentum injection at ";
+      const double start = opt_num(m, "start_time"), dur = opt_num(m, "duration");
+      return head + oss.str() + ((time_ < start || time_ > start + dur) ? " Off" : " On");
+    }
+    if (m.name == "field_heating")  // fieldheating.cpp:64-71
+      return std::string("Field Heating") + (opt_num(m, "coeff") == 0.0 ? " Zero" : " On") + na;
+    if (m.name == "div_cleaning") return "Div. Cleaning On";  // divcleaning.cpp:50-53
+    if (m.name == "physical_viscosity") {  // physicalviscosity.cpp:273-290
+      const bool heat = opt_flag(m, "heating_on", true), force = opt_flag(m, "force_on", true);
+      const bool zero = opt_num(m, "coeff") == 0.0;
+      std::string msg;
+      if (heat) {
+        msg += std::string("Viscous Heating") + (zero ? " Zero" : " On");
+        if (force) msg += ", ";
+      }
+      if (force) msg += std::string("Viscous Force") + (zero ? " Zero" : " On");
+      if (force || heat) msg += ", " + std::to_string(sub) + " Subcycle(s)";
+      return msg + na;
+    }
+    return "";
   }
 
   fs::path out_;

@@ -95,6 +95,22 @@ def test_runner_output_schedule_and_continue_mode(tmp_path):
     _same_files(cont)
 
 
+SOURCE_MODULES = ["lheat_ramp_rk2", "mass_inj_rk4", "mom_inj_osc_euler", "sink_exp_heat_rk2", "divclean_rk2"]
+
+
+@pytest.mark.parametrize("name", SOURCE_MODULES)
+def test_runner_source_modules_byte_identical_with_messages(name, tmp_path):
+    """Files and the per-step standard output lines, whose module messages switch from Off to On
+    at the injection start times."""
+    tmp = str(tmp_path)
+    sp, cp = _inputs(tmp, name, std_out_interval=1)
+    o = _both(tmp, lambda d: ["-m", "input", "-o", d, "-s", sp, "-c", cp])
+    _same_files(o)
+    ref_out = o["ref"][2].replace(os.path.join(tmp, "ref"), "<out>")
+    gpu_out = o["gpu"][2].replace(os.path.join(tmp, "gpu"), "<out>")
+    assert ref_out == gpu_out, (ref_out, gpu_out)
+
+
 def test_runner_with_libm_modules_matches_to_1e10(tmp_path):
     tmp = str(tmp_path)
     sp, cp = _inputs(tmp, "solar_tc_rl_rk2")
