@@ -114,15 +114,20 @@ def cpu_baseline(st, cfg, steps: int, threads: int) -> dict:
             c.settings["max_iterations"] = str(n)
             c.settings["iter_output_interval"] = "-1"
             c.settings["write_interval"] = "-1"
-            c.settings["std_out_interval"] = "-1"
+            # one line per reference step on stderr: a sample of a large grid runs for minutes
+            c.settings["std_out_interval"] = "1"
             cp = os.path.join(tmp, "w%d.config" % n)
             c.write(cp)
+            print("cpu baseline: reference run of %d steps" % n, file=sys.stderr, flush=True)
             t0 = time.perf_counter()
-            r = subprocess.run([exe, "-m", "input", "-o", os.path.join(tmp, "o%d" % n), "-s", sp, "-c", cp],
-                               capture_output=True, text=True, env=env, cwd=tmp)
+            p = subprocess.Popen([exe, "-m", "input", "-o", os.path.join(tmp, "o%d" % n), "-s", sp, "-c", cp],
+                                 stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, env=env, cwd=tmp)
+            for line in p.stdout:
+                print("  reference: " + line.rstrip()[:160], file=sys.stderr, flush=True)
+            rc = p.wait()
             times[n] = time.perf_counter() - t0
-            if r.returncode not in (0, -6, 134):
-                return {"error": "reference run failed rc=%d" % r.returncode}
+            if rc not in (0, -6, 134):
+                return {"error": "reference run failed rc=%d" % rc}
     dt = times[steps] - times[0]
     return {"value": st.xdim * st.ydim * steps / dt, "unit": "cell-updates/s", "cores": threads,
             "kind": "reference",

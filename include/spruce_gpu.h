@@ -139,10 +139,19 @@ void spruce_gpu_destroy(spruce_gpu_domain* d);
  * backward through the hand-written reverse of every stage, minimum/derived update and step-size
  * reduction, recomputing each segment from its checkpoint.  seed = dJ/d(evolved grids after the
  * steps); grad_out = dJ/d(evolved grids before them); both 8*xdim*ydim doubles in evolved order
- * (rho, mom_x, mom_y, mom_z, thermal_energy, bi_x, bi_y, bi_z).  Covers ideal_mhd with periodic
- * boundaries, euler or rk2, and no modules; SPRUCE_E_UNSUPPORTED otherwise. */
+ * (rho, mom_x, mom_y, mom_z, thermal_energy, bi_x, bi_y, bi_z).  Covers ideal_mhd, euler or rk2,
+ * with periodic, fixed, reflect and open boundaries (open only with the other axis periodic) and the
+ * modules thermal_conduction (euler subcycles, no flux saturation), radiative_losses (no
+ * prevent_subcycling), ambient_heating and parametric_heating; SPRUCE_E_UNSUPPORTED otherwise.
+ * With modules each step of the backward sweep is replayed once more to keep its intermediate
+ * states (subcycle temperatures, module outputs). */
 int spruce_gpu_adjoint(spruce_gpu_domain* d, int32_t nsteps, int32_t checkpoint_interval, const double* seed,
                        double* grad_out, double* dt_out);
+/* Gradient of the last spruce_gpu_adjoint call with respect to module parameters (no reference
+ * counterpart): the coefficients coeff_0 .. coeff_{K-1} of each parametric_heating module, in module
+ * order.  count_out (may be NULL) receives the number of parameters; grad_out (may be NULL) must
+ * hold count >= that many doubles. */
+int spruce_gpu_adjoint_params(spruce_gpu_domain* d, double* grad_out, int32_t count, int32_t* count_out);
 /* Reverse of populateVariablesFromState (equationset.cpp:96-104): maps dJ/d(evolved grids right
  * after setup) to dJ/d(.state grids rho, temp, mom_x, mom_y, mom_z, bi_x, bi_y, bi_z), 8*xdim*ydim. */
 int spruce_gpu_adjoint_state(spruce_gpu_domain* d, const double* grad_evolved, double* grad_state);

@@ -1,0 +1,431 @@
+// Discrete-adjoint kernels of the ghost fill, live propagate and modules for gfx950 (reverse of
+// mhd.hip's ghost fill, solar.hip's conduction and radiation kernels and the heating update).
+// Local Jacobians of the nonlinear per-cell functions are taken in forward mode (dual.hpp) from the
+// same code as the forward where the function is shared (tc_cell.hpp).  Contributions to
+// neighbouring cells use FP64 atomics, so results are reproducible to rounding.
+#include <cfloat>
+
+#include "adj_modules.hpp"
+#include "arith.hpp"
+#include "dual.hpp"
+#include "tc_cell.hpp"
+
+namespace spruce {
+namespace {
+
+using tcc::inside;
+using tcc::wrp;
+
+__device__ __forceinline__ double block_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  __shared__ double part[16];
+  const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+  if ((lane & 63) == 0) part[lane >> 6] = v;
+  __syncthreads();
+  if (lane == 0) {
+    const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+    double m = 0.0;
+    for (int k = 0; k < nw; ++k) m += part[k];
+    v = m;
+  }
+  return v;
+}
+
+// ---- ghost fill ------------------------------------------------------------------------------
+
+struct AdjGhostArgs {
+  MhdSet P;
+  MhdConst k;
+  Mesh mesh;
+  int kind[4];
+  double s1[4], s2[4];
+};
+
+// Open boundary ghost values of one boundary cell row (mesh.hip ghost_side, kind 1, GHOST_NORMAL)
+// from x = (rho_pre, te, mom_x, mom_y) of the first interior cell: out = rho, te, mom_x, mom_y of
+// the two ghost cells (order c1, c2).
+__device__ void open_ghost_dual(const MhdConst& K, const Axis& ax, bool xside, bool upper, double s1, double s2,
+                                const double x[4], Dn<4> out[8]) {
+  const int n = ax.n;
+  const int e2 = upper ? n - 2 : 1, e3 = upper ? n - 3 : 2;
+  const Dn<4> rho3 = dn_var<4>(x[0], 0), te3 = dn_var<4>(x[1], 1), mx3 = dn_var<4>(x[2], 2), my3 = dn_var<4>(x[3], 3);
+  const Dn<4> rg1 = s1 * rho3, rg2 = s2 * rho3, tg1 = s1 * te3, tg2 = s2 * te3;
+  Dn<4> cs = dn_const<4>(0.0);
+  const Dn<4> csn = dn_sqrt(K.gamma * (te3 * K.gm1) / rho3);
+  if (csn.v > cs.v) cs = csn;
+  const Dn<4> vx = mx3 / rho3, vy = my3 / rho3;
+  Dn<4> boost = K.obs * cs;
+  if (!upper) boost = boost * -1.0;
+  const double dist = 0.5 * (ax.d[e2] + ax.d[e3]);
+  const Dn<4> vn = xside ? vx : vy;
+  const Dn<4> bv = upper ? dn_max(dn_const<4>(0.0), vn + boost) : dn_min(dn_const<4>(0.0), vn + boost);
+  const Dn<4> gv = (dist * bv - (0.5 * ax.d[e2]) * vn) / (0.5 * ax.d[e3]);
+  const Dn<4> n1 = dn_max(rg1 / K.m, K.nmin), n2 = dn_max(rg2 / K.m, K.nmin);
+  out[0] = n1 * K.m;
+  out[1] = n2 * K.m;
+  out[2] = dn_max(tg1, K.temin);
+  out[3] = dn_max(tg2, K.temin);
+  out[4] = rg1 * (xside ? gv : vx);
+  out[5] = rg2 * (xside ? gv : vx);
+  out[6] = rg1 * (xside ? vy : gv);
+  out[7] = rg2 * (xside ? vy : gv);
+}
+
+__device__ void adj_ghost_side(const AdjGhostArgs& a, int side, int t, double* lam) {
+  const int kind = a.kind[side];
+  if (kind != 1 && kind != 2 && kind != 3) return;
+  const Mesh& M = a.mesh;
+  const bool xside = side < 2, upper = side & 1;
+  const Axis& ax = xside ? M.x : M.y;
+  const Axis& ot = xside ? M.y : M.x;
+  if (kind == 2) {
+    if (t < 0 || t >= ot.n) return;
+  } else {
+    if (t < ot.lo || t > ot.hi) return;
+  }
+  const int n = ax.n;
+  const int e1 = upper ? n - 1 : 0, e2 = upper ? n - 2 : 1, e3 = upper ? n - 3 : 2;
+  auto at = [&](int e) -> size_t { return xside ? (size_t)e * M.ny + t : (size_t)t * M.ny + e; };
+  const size_t c1 = at(e1), c2 = at(e2), c3 = at(e3);
+  const size_t N = (size_t)M.nx * M.ny;
+  double* lr = lam + 0 * N; double* lmx = lam + 1 * N; double* lmy = lam + 2 * N; double* lmz = lam + 3 * N;
+  double* lte = lam + 4 * N;
+  if (kind == 2) {                 // fixed: momenta of c1, c2, c3 set to zero
+    lmx[c1] = 0.0; lmx[c2] = 0.0; lmx[c3] = 0.0;
+    lmy[c1] = 0.0; lmy[c2] = 0.0; lmy[c3] = 0.0;
+    lmz[c1] = 0.0; lmz[c2] = 0.0; lmz[c3] = 0.0;
+  } else if (kind == 3) {          // reflect: rho, te (and n) copied from c3, momenta zeroed
+    lte[c3] += lte[c1] + lte[c2];
+    lte[c1] = 0.0; lte[c2] = 0.0;
+    lr[c3] += lr[c1] + lr[c2];
+    lr[c1] = 0.0; lr[c2] = 0.0;
+    lmx[c1] = 0.0; lmx[c2] = 0.0; lmx[c3] = 0.0;
+    lmy[c1] = 0.0; lmy[c2] = 0.0; lmy[c3] = 0.0;
+    lmz[c1] = 0.0; lmz[c2] = 0.0; lmz[c3] = 0.0;
+  } else {                         // open
+    const double x[4] = {a.P.rho_pre[c3], a.P.te[c3], a.P.mx[c3], a.P.my[c3]};
+    Dn<4> o[8];
+    open_ghost_dual(a.k, ax, xside, upper, a.s1[side], a.s2[side], x, o);
+    const double lo[8] = {lr[c1], lr[c2], lte[c1], lte[c2], lmx[c1], lmx[c2], lmy[c1], lmy[c2]};
+    double g[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q = 0; q < 8; ++q)
+      for (int k = 0; k < 4; ++k) g[k] += lo[q] * o[q].d[k];
+    lr[c1] = 0.0; lr[c2] = 0.0; lte[c1] = 0.0; lte[c2] = 0.0;
+    lmx[c1] = 0.0; lmx[c2] = 0.0; lmy[c1] = 0.0; lmy[c2] = 0.0;
+    // rho_pre of c3 and its rho come from the same pre-update density with the same floors
+    lr[c3] += g[0];
+    lte[c3] += g[1];
+    lmx[c3] += g[2];
+    lmy[c3] += g[3];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_adj_ghost_axis(AdjGhostArgs a, int xsides, double* lam) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s0 = xsides ? 0 : 2;
+  // the forward fills the lower side, then the upper one
+  adj_ghost_side(a, s0 + 1, t, lam);
+  adj_ghost_side(a, s0, t, lam);
+}
+
+// ---- minimums / derived ------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_adj_minmask(const double* rho_in, const double* te_base, const double* add,
+                                                     const double* step, MhdConst K, Mesh M, double* lam,
+                                                     double* partial) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  double dot = 0.0;
+  if (i < M.nx && j < M.ny) {
+    const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+    const double x = rho_in[c] / K.m;
+    const double rho1 = smax(x, K.nmin) * K.m;
+    const double drho = (x < K.nmin || rho1 / K.m < K.nmin) ? 0.0 : 1.0;
+    const double te_in = add ? te_base[c] + (*step) * add[c] : te_base[c];
+    const double dte = te_in < K.temin ? 0.0 : 1.0;
+    lam[c] *= drho;
+    const double lt = lam[4 * N + c] * dte;
+    lam[4 * N + c] = lt;
+    if (add) dot = lt * add[c];
+  }
+  if (partial) {
+    dot = block_sum(dot);
+    if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
+  }
+}
+
+// ---- thermal conduction ----------------------------------------------------------------------
+
+__device__ __forceinline__ double temp_of(double te, double n, const MhdConst& K) {
+  return smax(te * K.gm1 / (n * (2.0 * K_B)), K.tmin);
+}
+
+constexpr int TC_PASS = 3;   // derivative components per forward-mode pass (15 inputs, 5 passes)
+
+__global__ void __launch_bounds__(256) k_adj_tc_sub(const double* te_s, const double* n, const double* bhx,
+                                                    const double* bhy, const double* dbxx, const double* dbxy,
+                                                    const double* dbyx, const double* dbyy, double h, double k0,
+                                                    MhdConst K, Mesh M, double* lam_te, double* lam_T,
+                                                    double* lam_fz, double* partial) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  double dot = 0.0;
+  if (i < M.nx && j < M.ny) {
+    const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+    const double base = te_s[c];
+    if (!inside(M, i, j)) {
+      lam_te[c] *= (base + 0.0 < K.temin) ? 0.0 : 1.0;
+    } else {
+      size_t q[3][3];
+      double x[15];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          q[a][b] = (size_t)wrp(i - 1 + a, M.nx, M.x.per) * M.ny + wrp(j - 1 + b, M.ny, M.y.per);
+          x[3 * a + b] = temp_of(te_s[q[a][b]], n[q[a][b]], K);
+        }
+      x[9] = bhx[c]; x[10] = bhy[c];
+      x[11] = dbxx[c]; x[12] = dbxy[c]; x[13] = dbyx[c]; x[14] = dbyy[c];
+      double J[15];
+      double dE = 0.0;
+      for (int p = 0; p < 15 / TC_PASS; ++p) {
+        Dn<TC_PASS> xs[15];
+        for (int k = 0; k < 15; ++k)
+          xs[k] = (k >= p * TC_PASS && k < (p + 1) * TC_PASS) ? dn_var<TC_PASS>(x[k], k - p * TC_PASS)
+                                                              : dn_const<TC_PASS>(x[k]);
+        const Dn<TC_PASS> T[3][3] = {{xs[0], xs[1], xs[2]}, {xs[3], xs[4], xs[5]}, {xs[6], xs[7], xs[8]}};
+        const Dn<TC_PASS> r = tcc::tc_core(M, i, j, T, xs[9], xs[10], xs[11], xs[12], xs[13], xs[14], k0);
+        dE = r.v;
+        for (int k = 0; k < TC_PASS; ++k) J[p * TC_PASS + k] = r.d[k];
+      }
+      const double v = base + 1.0 * (1.0 * h) * dE;
+      const double lv = (v < K.temin) ? 0.0 : lam_te[c];
+      lam_te[c] = lv;
+      dot = lv * dE;
+      const double ld = lv * h;
+      if (ld != 0.0) {
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) atomicAdd(&lam_T[q[a][b]], ld * J[3 * a + b]);
+        for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += ld * J[9 + k];
+      }
+    }
+  }
+  dot = block_sum(dot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
+}
+
+__global__ void __launch_bounds__(256) k_adj_tc_temp(const double* te_s, const double* n, MhdConst K, size_t N,
+                                                     double* lam_T, double* lam_te, double* lam_rho) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const double lT = lam_T[c];
+  if (lT == 0.0) return;
+  lam_T[c] = 0.0;
+  const double te = te_s[c], nn = n[c];
+  const double den = nn * (2.0 * K_B);
+  const double x = te * K.gm1 / den;
+  if (x < K.tmin) return;
+  lam_te[c] += lT * (K.gm1 / den);
+  lam_rho[c] += lT * (-x / nn) * K.rm;
+}
+
+// derivative1D(q, axis) at interior cell (i, j) transposed: d(c) depends on q(c-1), q(c), q(c+1)
+__device__ __forceinline__ void scatter_deriv(const Mesh& M, int axis, int i, int j, double ld, double* lq) {
+  const Axis& ax = axis == 0 ? M.x : M.y;
+  const int c = axis == 0 ? i : j;
+  const int cm = wrp(c - 1, ax.n, ax.per), cp = wrp(c + 1, ax.n, ax.per);
+  auto at = [&](int k) -> size_t { return axis == 0 ? (size_t)k * M.ny + j : (size_t)i * M.ny + k; };
+  const double r = ld / ax.d[c];
+  // upper face (c, cp): (q(c) h[cp] + q(cp) h[c]) / fden[cp]; lower face (cm, c): (q(cm) h[c] + q(c) h[cm]) / fden[c]
+  atomicAdd(&lq[at(cp)], r * (ax.h[c] / ax.fden[cp]));
+  atomicAdd(&lq[at(c)], r * (ax.h[cp] / ax.fden[cp] - ax.h[cm] / ax.fden[c]));
+  atomicAdd(&lq[at(cm)], -r * (ax.h[c] / ax.fden[c]));
+}
+
+__global__ void __launch_bounds__(256) k_adj_bhat_derivs(Mesh M, double* lam_fz) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= M.nx || j >= M.ny || !inside(M, i, j)) return;
+  const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+  double* lbx = lam_fz;
+  double* lby = lam_fz + N;
+  const double lxx = lam_fz[2 * N + c], lxy = lam_fz[3 * N + c], lyx = lam_fz[4 * N + c], lyy = lam_fz[5 * N + c];
+  if (lxx != 0.0) scatter_deriv(M, 0, i, j, lxx, lbx);
+  if (lxy != 0.0) scatter_deriv(M, 1, i, j, lxy, lbx);
+  if (lyx != 0.0) scatter_deriv(M, 0, i, j, lyx, lby);
+  if (lyy != 0.0) scatter_deriv(M, 1, i, j, lyy, lby);
+}
+
+__global__ void __launch_bounds__(256) k_adj_bhat_field(MhdSet L, MhdStatic st, Mesh M, const double* lam_fz,
+                                                        double* lam) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= M.nx || j >= M.ny) return;
+  const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+  const double gx = lam_fz[c], gy = lam_fz[N + c];
+  if (gx == 0.0 && gy == 0.0) return;
+  const double bx = st.bex[c] + L.bx[c], by = st.bey[c] + L.by[c], bz = st.bez[c] + L.bz[c];
+  const double bm = sqrt(bx * bx + by * by + bz * bz);
+  if (bm == 0.0) return;
+  const double hx = bx / bm, hy = by / bm, hz = bz / bm;
+  // d b_hat_a / d b_b = (delta_ab - b_hat_a b_hat_b) / |b|
+  const double s = gx * hx + gy * hy;
+  lam[5 * N + c] += (gx - s * hx) / bm;
+  lam[6 * N + c] += (gy - s * hy) / bm;
+  lam[7 * N + c] += (-s * hz) / bm;
+}
+
+// ---- radiative losses --------------------------------------------------------------------------
+
+template <int K>
+__device__ Dn<K> rl_loss_dn(const Dn<K>& t, const Dn<K>& n, const RlParams& p) {
+  if (t.v < p.cut) return dn_const<K>(0.0);
+  const double lt = log10(t.v);
+  double chi;
+  Dn<K> pw;
+  if (lt <= 4.97) { chi = 1.09e-31; pw = t * t; }
+  else if (lt <= 5.67) { chi = 8.87e-17; pw = 1.0 / t; }
+  else if (lt <= 6.18) { chi = 1.90e-22; pw = dn_const<K>(1.0); }
+  else if (lt <= 6.55) { chi = 3.53e-13; pw = dn_pow_m1p5(t); }
+  else if (lt <= 6.90) { chi = 3.46e-25; pw = dn_pow_third(t); }
+  else if (lt <= 7.63) { chi = 5.49e-16; pw = 1.0 / t; }
+  else { chi = 1.96e-27; pw = dn_sqrt(t); }
+  Dn<K> v = n * n * chi * pw;
+  if (t.v < p.cut_plus_ramp) v = v * ((t - p.cut) / p.ramp);
+  return v;
+}
+
+__global__ void __launch_bounds__(256) k_adj_rl(MhdSet A, RlParams p, MhdConst K, Mesh M, int nsub, double h,
+                                                int integ, double* lam, double* partial) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  double dot = 0.0;
+  if (i < M.nx && j < M.ny) {
+    const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+    // the live propagate after the update: rho floors on A's density, te floor on the output
+    const double x = A.rho[c] / K.m;
+    const double rho1 = smax(x, K.nmin) * K.m;
+    const double drho = (x < K.nmin || rho1 / K.m < K.nmin) ? 0.0 : 1.0;
+    double lrho = lam[c] * drho;
+    const double te0 = A.te[c];
+    if (!inside(M, i, j)) {
+      lam[4 * N + c] *= (te0 < K.temin) ? 0.0 : 1.0;
+    } else {
+      const Dn<3> nn = dn_var<3>(A.n[c], 1), hv = dn_var<3>(h, 2);
+      Dn<3> te = dn_var<3>(te0, 0);
+      const Dn<3> hh = 1.0 * hv, half = 1.0 * (0.5 * hv);
+      auto loss = [&](const Dn<3>& e) {
+        const Dn<3> T = dn_max(e * K.gm1 / (nn * (2.0 * K_B)), K.tmin);
+        return rl_loss_dn(T, nn, p);
+      };
+      for (int s = 0; s < nsub; ++s) {
+        if (integ == 0) {
+          te = dn_max(te - hh * loss(te), K.temin);
+        } else if (integ == 1) {
+          const Dn<3> hs = dn_max(te - half * loss(te), K.temin);
+          te = dn_max(te - hh * loss(hs), K.temin);
+        } else {
+          const Dn<3> k1 = -loss(te);
+          Dn<3> im = dn_max(te + half * k1, K.temin);
+          const Dn<3> k2 = -loss(im);
+          im = dn_max(te + half * k2, K.temin);
+          const Dn<3> k3 = -loss(im);
+          im = dn_max(te + hh * k3, K.temin);
+          const Dn<3> k4 = -loss(im);
+          te = dn_max(te + (hh * (k1 + 2.0 * k2 + 2.0 * k3 + k4)) / 6.0, K.temin);
+        }
+      }
+      const double lo = (te.v < K.temin) ? 0.0 : lam[4 * N + c];
+      lam[4 * N + c] = lo * te.d[0];
+      lrho += lo * te.d[1] * K.rm;
+      dot = lo * te.d[2];
+    }
+    lam[c] = lrho;
+  }
+  dot = block_sum(dot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
+}
+
+// ---- parametrised heating ----------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) k_adj_colsum(const double* lam_te, Mesh M, double* colsum) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M.ny) return;
+  double s = 0.0;
+  if (j >= M.y.lo && j <= M.y.hi)
+    for (int i = M.x.lo; i <= M.x.hi; ++i) s += lam_te[(size_t)i * M.ny + j];
+  colsum[j] = s;
+}
+
+__global__ void __launch_bounds__(256) k_adj_theta(const double* colsum, const double* basis, const double* step,
+                                                   Mesh M, double* g_theta) {
+  const int k = blockIdx.x;
+  double s = 0.0;
+  for (int j = threadIdx.x; j < M.ny; j += blockDim.x) s += basis[(size_t)k * M.ny + j] * colsum[j];
+  s = block_sum(s);
+  if (threadIdx.x == 0) g_theta[k] += (*step) * s;
+}
+
+__global__ void __launch_bounds__(256) k_adj_accum(const double* partial, int n, double scale, double* gh) {
+  double s = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) s += partial[k];
+  s = block_sum(s);
+  if (threadIdx.x == 0) gh[0] += scale * s;
+}
+
+inline dim3 g2(const Mesh& m, dim3 b) { return dim3((m.ny + b.x - 1) / b.x, (m.nx + b.y - 1) / b.y); }
+
+}  // namespace
+
+int adj_mod_blocks(const Mesh& m) {
+  const dim3 g = g2(m, dim3(64, 4));
+  return (int)(g.x * g.y);
+}
+
+void launch_adj_ghost_axis(MhdSet P, MhdConst k, Mesh mesh, const int kinds[4], const double s1[4],
+                           const double s2[4], int axis, double* lam, hipStream_t s) {
+  AdjGhostArgs a;
+  a.P = P; a.k = k; a.mesh = mesh;
+  for (int q = 0; q < 4; ++q) { a.kind[q] = kinds[q]; a.s1[q] = s1[q]; a.s2[q] = s2[q]; }
+  const int s0 = axis == 0 ? 0 : 2;
+  if (!(a.kind[s0] || a.kind[s0 + 1])) return;
+  const int len = axis == 0 ? mesh.ny : mesh.nx;
+  hipLaunchKernelGGL(k_adj_ghost_axis, dim3((len + 255) / 256), dim3(256), 0, s, a, axis == 0 ? 1 : 0, lam);
+}
+
+void launch_adj_minmask(const double* rho_in, const double* te_base, const double* add, const double* step,
+                        MhdConst k, Mesh mesh, double* lam, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_minmask, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, rho_in, te_base, add, step, k, mesh, lam,
+                     partial);
+}
+
+void launch_adj_tc_sub(const double* te_s, const double* n, const double* bhx, const double* bhy,
+                       const double* dbxx, const double* dbxy, const double* dbyx, const double* dbyy, double h,
+                       double k0, MhdConst k, Mesh mesh, double* lam_te, double* lam_T, double* lam_fz,
+                       double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_tc_sub, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, te_s, n, bhx, bhy, dbxx, dbxy, dbyx,
+                     dbyy, h, k0, k, mesh, lam_te, lam_T, lam_fz, partial);
+}
+
+void launch_adj_tc_temp(const double* te_s, const double* n, MhdConst k, size_t N, double* lam_T, double* lam_te,
+                        double* lam_rho, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_tc_temp, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, te_s, n, k, N, lam_T, lam_te,
+                     lam_rho);
+}
+
+void launch_adj_bhat(MhdSet L, MhdStatic st, Mesh mesh, double* lam_fz, double* lam, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_bhat_derivs, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, mesh, lam_fz);
+  hipLaunchKernelGGL(k_adj_bhat_field, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, L, st, mesh, (const double*)lam_fz,
+                     lam);
+}
+
+void launch_adj_rl(MhdSet A, RlParams p, MhdConst k, Mesh mesh, int nsub, double h, int integ, double* lam,
+                   double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_rl, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, A, p, k, mesh, nsub, h, integ, lam, partial);
+}
+
+void launch_adj_heat_params(const double* lam_te, const double* basis, int K, const double* step, Mesh mesh,
+                            double* colsum, double* g_theta, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_colsum, dim3((mesh.ny + 255) / 256), dim3(256), 0, s, lam_te, mesh, colsum);
+  hipLaunchKernelGGL(k_adj_theta, dim3(K), dim3(256), 0, s, (const double*)colsum, basis, step, mesh, g_theta);
+}
+
+void launch_adj_accum(const double* partial, int n, double scale, double* gh, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_accum, dim3(1), dim3(256), 0, s, partial, n, scale, gh);
+}
+
+}  // namespace spruce

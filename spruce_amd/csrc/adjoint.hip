@@ -136,12 +136,13 @@ __global__ void __launch_bounds__(256) k_adj_dt_blocks(MhdSet L, MhdStatic st, M
 }
 
 __global__ void __launch_bounds__(64) k_adj_dt_apply(MhdSet L, MhdStatic st, MhdConst K, Mesh M, double eps,
-                                                     const double* p1, const double* p2, const double* pmin,
-                                                     const int* pidx, int nb, double* lam_L) {
-  // gh: the adjoint of the step size, the sum of the stage kernels' block partials
+                                                     const double* p1, const double* p2, const double* gh,
+                                                     const double* pmin, const int* pidx, int nb, double* lam_L) {
+  // the adjoint of the step size: the sum of the stage kernels' block partials, plus the modules'
   double g = 0.0;
   for (int k = threadIdx.x; k < nb; k += 64) g += p1[k] + (p2 ? p2[k] : 0.0);
   for (int off = 32; off > 0; off >>= 1) g += __shfl_xor(g, off, 64);
+  if (gh) g += gh[0];
   // (min, lowest index) over the block results: one wavefront, then a shuffle reduction; the rule
   // is order-independent, so the cell is the same however the blocks are visited
   double v = DBL_MAX;
@@ -214,10 +215,10 @@ void launch_adj_faces(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, const doubl
 }
 
 void launch_adj_dt(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double eps, const double* p1, const double* p2,
-                   double* pmin, int* pidx, double* lam_L, hipStream_t s) {
+                   const double* gh, double* pmin, int* pidx, double* lam_L, hipStream_t s) {
   const dim3 b(64, 4), g = g2(mesh, b);
   hipLaunchKernelGGL(k_adj_dt_blocks, g, b, 0, s, L, st, k, mesh, pmin, pidx);
-  hipLaunchKernelGGL(k_adj_dt_apply, dim3(1), dim3(64), 0, s, L, st, k, mesh, eps, p1, p2, (const double*)pmin,
+  hipLaunchKernelGGL(k_adj_dt_apply, dim3(1), dim3(64), 0, s, L, st, k, mesh, eps, p1, p2, gh, (const double*)pmin,
                      (const int*)pidx, (int)(g.x * g.y), lam_L);
 }
 

@@ -21,10 +21,10 @@ void launch_adj_stage_cells(MhdSet B, const double* kv, const double* step, doub
 // The face part of the transposed time derivative at S (both axes, one launch): FP64 atomics into lam_S.
 void launch_adj_faces(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, const double* cx, const double* cy,
                       double* lam_S, hipStream_t s);
-// step-size adjoint: gh = sum of the partials p1 (+ p2 when not null); h = eps * min dt(L) over the
-// interior; lam_L(c*) += eps * gh * grad dt(c*)
+// step-size adjoint: g = sum of the partials p1 (+ p2 when not null) (+ gh[0] when not null, the
+// modules' part); h = eps * min dt(L) over the interior; lam_L(c*) += eps * g * grad dt(c*)
 void launch_adj_dt(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double eps, const double* p1, const double* p2,
-                   double* pmin, int* pidx, double* lam_L, hipStream_t s);
+                   const double* gh, double* pmin, int* pidx, double* lam_L, hipStream_t s);
 // adjoint of populateVariablesFromState (idealmhd.cpp:226-252) from the state grids rho and temp:
 // g_state (8*N: rho, temp, mom_x, mom_y, mom_z, bi_x, bi_y, bi_z) from lam (8*N, evolved order)
 void launch_adj_populate(const double* rho_s, const double* temp_s, const double* lam, MhdConst k, size_t N,

@@ -25,6 +25,7 @@
 #include "pvisc.hpp"
 #include "sources.hpp"
 #include "adjoint.hpp"
+#include "adj_modules.hpp"
 #include "crpow.hpp"
 #include "tf_host.hpp"
 #include "e2_host.hpp"
@@ -103,6 +104,15 @@ static int subcycle_count(double dt, double m) {
   if (!(q < 1.0e8)) throw Error(SPRUCE_E_NUMERIC, "subcycle count out of range");
   return (int)q + 1;
 }
+
+// The state snapshots the adjoint keeps while it replays one step, for that step's reverse.
+struct StepTape {
+  std::vector<MhdSet> after_it;    // after each iterating module (its propagate included)
+  MhdSet W{}, A3{};                // RK2 midpoint; after the integrator's final ghost fill
+  std::vector<MhdSet> after_post;  // after each post-iterate module
+  double* k1 = nullptr;
+  double* k2 = nullptr;
+};
 
 class Domain : public DomainBase {
  public:
@@ -364,8 +374,12 @@ class Domain : public DomainBase {
     }
     host_dt = dt;
     have_host_dt = host;
+    size_t ni = 0;
     for (auto& m : mods)
-      if (m->iterates()) m->iterate(*this, dt);
+      if (m->iterates()) {
+        m->iterate(*this, dt);
+        if (rec) copy_set(L.view(), rec->after_it[ni++], N, stream);
+      }
     // counts are recorded after iterate(): viscosity modules fix theirs there (viscosity.cpp:125-180)
     for (size_t k = 0; k < mods.size(); ++k) sub_hist[(size_t)s * mods.size() + k] = mods[k]->nsub;
     for (auto& m : mods) m->before_integrator(*this);
@@ -374,6 +388,10 @@ class Domain : public DomainBase {
   // while have_host_dt: some module asked for them)
   double host_dt = 0.0, tstep = 0.0;
   bool have_host_dt = false;
+  // set while the adjoint replays a step: where the step keeps its intermediate states, and which
+  // modules act after the integrator
+  StepTape* rec = nullptr;
+  std::vector<bool> rec_post;
 
   MhdStatic stat() { return MhdStatic{bex.p, bey.p, bez.p, gx.p, gy.p}; }
 
@@ -433,12 +451,18 @@ class Domain : public DomainBase {
     if (!mods.empty()) run_modules(s);
     const int integ = desc.time_integrator;
     if (integ == SPRUCE_EULER) {                       // evolution.cpp:84-88
-      stage(L, L, W, 1.0, STAGE_PLAIN, 0);
+      if (rec) launch_stage(stage_args(L.view(), L.view(), W.view(), 1.0, STAGE_STORE_K, 0, nullptr, rec->k1));
+      else stage(L, L, W, 1.0, STAGE_PLAIN, 0);
       L.swap(W);
     } else if (integ == SPRUCE_RK2) {                  // evolution.cpp:90-101
-      stage(L, L, W, 0.5, STAGE_PLAIN, 0);
-      ghost(W, kinds_ucnp, GHOST_NORMAL);              // live-grid ghost writes here are overwritten unread
-      stage(W, L, L, 1.0, STAGE_PLAIN, 0);
+      if (rec) {                                       // the same stages, keeping k1, k2 and the midpoint
+        launch_stage(stage_args(L.view(), L.view(), rec->W, 0.5, STAGE_STORE_K, 0, nullptr, rec->k1));
+        launch_stage(stage_args(rec->W, L.view(), L.view(), 1.0, STAGE_STORE_K, 0, nullptr, rec->k2));
+      } else {
+        stage(L, L, W, 0.5, STAGE_PLAIN, 0);
+        ghost(W, kinds_ucnp, GHOST_NORMAL);            // live-grid ghost writes here are overwritten unread
+        stage(W, L, L, 1.0, STAGE_PLAIN, 0);
+      }
     } else {                                           // evolution.cpp:103-124
       stage(L, L, W, 0.5, STAGE_STORE_K, 0);
       ghost(W, kinds_ucnp, GHOST_NORMAL);
@@ -452,7 +476,12 @@ class Domain : public DomainBase {
       stage(W, L, L, 1.0, STAGE_RK4_FINAL, 0);
     }
     ghost(L, kinds_all, GHOST_NORMAL);
-    for (auto& m : mods) m->post(*this);
+    if (rec) copy_set(L.view(), rec->A3, N, stream);
+    size_t np = 0;
+    for (size_t k = 0; k < mods.size(); ++k) {
+      mods[k]->post(*this);
+      if (rec && k < rec_post.size() && rec_post[k]) copy_set(L.view(), rec->after_post[np++], N, stream);
+    }
     if (have_host_dt) tstep += host_dt;               // evolution.cpp:80, after the post-iterate hooks
     finish_step(s + 1);
   }
@@ -552,111 +581,17 @@ class Domain : public DomainBase {
     return MhdSet{p, p + N, p + 2 * N, p + 3 * N, p + 4 * N, p + 5 * N, p + 6 * N, p + 7 * N, p + 8 * N, p + 9 * N};
   }
 
-  // Discrete adjoint of nsteps steps (reverse of advance(): stages, minimums/derived, step size),
-  // with the evolved grids checkpointed every `ckpt` steps and each segment recomputed from its
-  // checkpoint during the backward sweep.  Periodic boundaries, euler or rk2, no modules.
-  void adjoint(int nsteps, int ckpt, const double* seed, double* grad, double* dt_out) override {
-    require(ready, SPRUCE_E_STATE, "setup must precede adjoint");
-    require(mods.empty(), SPRUCE_E_UNSUPPORTED, "the adjoint does not cover modules");
-    for (int k = 0; k < 4; ++k)
-      require(desc.bc[k] == SPRUCE_BC_PERIODIC, SPRUCE_E_UNSUPPORTED,
-              "the adjoint covers periodic boundaries (no ghost-fill adjoint)");
-    const int integ = desc.time_integrator;
-    require(integ == SPRUCE_EULER || integ == SPRUCE_RK2, SPRUCE_E_UNSUPPORTED, "the adjoint covers euler and rk2");
-    require(nsteps >= 1 && ckpt >= 1, SPRUCE_E_ARG, "nsteps and checkpoint_interval must be positive");
-    require(seed && grad, SPRUCE_E_ARG, "null argument");
-    const int nseg = (nsteps + ckpt - 1) / ckpt;
-    const int sl = std::min(ckpt, nsteps);
-    const bool rk2 = integ == SPRUCE_RK2;
-    const int nb = adj_blocks(mesh);
-    // One allocation holds everything: checkpoints (nseg sets), the recomputed segment (sl + 1 sets),
-    // its RK2 midpoints (sl sets) and stage derivatives k1, k2 (8 grids per step each), the three
-    // adjoint vectors, the face coefficients and the reduction scratch.
-    const size_t SET = 10 * N, EV = 8 * N, CO = (size_t)ADJ_COEFS * N;
-    const size_t total = (size_t)nseg * SET + (size_t)(sl + 1) * SET + (rk2 ? (size_t)sl * SET : 0) +
-                         (size_t)sl * EV * (rk2 ? 2 : 1) + 3 * EV + 2 * CO + 4 * (size_t)nb;
-    DBuf slab;
-    slab.alloc(total);
-    double* cur = slab.p;
-    auto take = [&](size_t n) { double* p = cur; cur += n; return p; };
-    std::vector<MhdSet> cps(nseg), seg(sl + 1), mid(rk2 ? sl : 0);
-    for (auto& c : cps) c = set_at(take(SET), N);
-    for (auto& c : seg) c = set_at(take(SET), N);
-    for (auto& c : mid) c = set_at(take(SET), N);
-    std::vector<double*> k1s(sl), k2s(rk2 ? sl : 0);
-    for (auto& k : k1s) k = take(EV);
-    for (auto& k : k2s) k = take(EV);
-    double* lam = take(EV);
-    double* lamN = take(EV);
-    double* lamW = take(EV);
-    double* cx = take(CO);
-    double* cy = take(CO);
-    double* part = take(nb);
-    double* part2 = take(nb);
-    double* pmin = take(nb);
-    int* pidx = reinterpret_cast<int*>(take(nb));
-    if ((int)history.n < nsteps + 1) history.alloc(nsteps + 1);
-    hip_check(hipMemcpyAsync(history.p, step.p, sizeof(double), hipMemcpyDeviceToDevice, stream), "copy step");
-    // forward sweep, exactly the steps of run()
-    for (int n = 0; n < nsteps; ++n) {
-      if (n % ckpt == 0) copy_set(L.view(), cps[n / ckpt], N, stream);
-      advance(n);
-    }
-    sync_and_check();
-    std::vector<double> h(nsteps);
-    hip_check(hipMemcpy(h.data(), history.p, nsteps * sizeof(double), hipMemcpyDeviceToHost), "read dt history");
-    for (int n = 0; n < nsteps; ++n) {
-      time += h[n];
-      iter++;
-      if (dt_out) dt_out[n] = h[n];
-    }
-    // backward sweep.  Recomputing a segment keeps, for each of its steps, the stage derivatives k1
-    // and k2 and the RK2 midpoint W, so the reverse steps read them instead of evaluating them again.
-    hip_check(hipMemcpyAsync(lam, seed, 8 * N * 8, hipMemcpyHostToDevice, stream), "upload seed");
-    const MhdStatic st = stat();
-    for (int sg = nseg - 1; sg >= 0; --sg) {
-      const int n0 = sg * ckpt, n1 = std::min(nsteps, n0 + ckpt);
-      copy_set(cps[sg], seg[0], N, stream);
-      for (int n = n0; n < n1; ++n) {                // recompute the segment: states, k1, W, k2
-        const int i = n - n0;
-        if (!rk2) {
-          stage_view(seg[i], seg[i], seg[i + 1], 1.0, STAGE_STORE_K, history.p + n, k1s[i]);
-        } else {
-          stage_view(seg[i], seg[i], mid[i], 0.5, STAGE_STORE_K, history.p + n, k1s[i]);
-          stage_view(mid[i], seg[i], seg[i + 1], 1.0, STAGE_STORE_K, history.p + n, k2s[i]);
-        }
-      }
-      for (int n = n1 - 1; n >= n0; --n) {
-        const int i = n - n0;
-        const MhdSet& Ln = seg[i];
-        const double* hp = history.p + n;
-        // lamN and lamW are written first by the stage-cell kernels (assign), so they need no clear
-        if (rk2) {
-          timed("adj_stage", [&] {
-            launch_adj_stage_cells(Ln, k2s[i], hp, 1.0, mid[i], st, kc, mesh, lam, lamN, 1, lamW, 1, cx, cy, part,
-                                   stream);
-          });
-          timed("adj_faces", [&] { launch_adj_faces(mid[i], st, kc, mesh, cx, cy, lamW, stream); });
-          timed("adj_stage", [&] {
-            launch_adj_stage_cells(Ln, k1s[i], hp, 0.5, Ln, st, kc, mesh, lamW, lamN, 0, lamN, 0, cx, cy, part2,
-                                   stream);
-          });
-        } else {
-          timed("adj_stage", [&] {
-            launch_adj_stage_cells(Ln, k1s[i], hp, 1.0, Ln, st, kc, mesh, lam, lamN, 1, lamN, 0, cx, cy, part, stream);
-          });
-        }
-        timed("adj_faces", [&] { launch_adj_faces(Ln, st, kc, mesh, cx, cy, lamN, stream); });
-        timed("adj_dt", [&] {
-          launch_adj_dt(Ln, st, kc, mesh, kc.eps, part, rk2 ? part2 : nullptr, pmin, pidx, lamN, stream);
-        });
-        std::swap(lam, lamN);
-      }
-    }
-    hip_check(hipMemcpyAsync(grad, lam, 8 * N * 8, hipMemcpyDeviceToHost, stream), "download gradient");
-    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    if (clock.enabled) clock.collect();
+  // Discrete adjoint of nsteps steps (reverse of advance(): modules, stages, ghost fills,
+  // minimums/derived, step size), with the evolved grids checkpointed every `ckpt` steps and each
+  // segment recomputed from its checkpoint during the backward sweep.  Defined after the modules.
+  void adjoint(int nsteps, int ckpt, const double* seed, double* grad, double* dt_out) override;
+  void replay_step(int n, const MhdSet& start, StepTape& tp, const std::vector<double>& times);
+  void adj_ghost(const MhdSet& P, double* lam) {
+    launch_adj_ghost_axis(P, kc, mesh, kinds_all, s1, s2, 1, lam, stream);   // y sides were filled last
+    launch_adj_ghost_axis(P, kc, mesh, kinds_all, s1, s2, 0, lam, stream);
   }
+  // dJ/dtheta of the parametrised heating modules after the last adjoint call
+  std::vector<double> param_grad() override;
 
   // reverse of populateVariablesFromState for the .state grids kept from set-up
   void adjoint_state(const double* grad_evolved, double* grad_state) override {
@@ -668,6 +603,18 @@ class Domain : public DomainBase {
     lam.alloc(8 * N);
     g.alloc(8 * N);
     hip_check(hipMemcpy(lam.p, grad_evolved, 8 * N * 8, hipMemcpyHostToDevice), "upload gradient");
+    if (kinds_all[0] || kinds_all[1] || kinds_all[2] || kinds_all[3]) {
+      // set-up ends with a ghost fill (equationset.cpp:96-104): rebuild the set-up state and reverse it
+      SetBufs P;
+      P.alloc(N);
+      const char* src[7] = {"rho", "mom_x", "mom_y", "mom_z", "bi_x", "bi_y", "bi_z"};
+      DBuf* dst[7] = {&P.rho, &P.mx, &P.my, &P.mz, &P.bx, &P.by, &P.bz};
+      for (int k = 0; k < 7; ++k)
+        hip_check(hipMemcpy(dst[k]->p, host_grids[src[k]].data(), N * 8, hipMemcpyHostToDevice), "upload grid");
+      launch_mhd_populate(P.view(), temp_s.p, kc, nx, ny, stream);
+      launch_mhd_ghost(P.view(), kc, mesh, kinds_all, s1, s2, GHOST_NORMAL, stream);
+      adj_ghost(P.view(), lam.p);
+    }
     launch_adj_populate(rho_s.p, temp_s.p, lam.p, kc, N, g.p, stream);
     hip_check(hipMemcpyAsync(grad_state, g.p, 8 * N * 8, hipMemcpyDeviceToHost, stream), "download gradient");
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
@@ -691,6 +638,10 @@ struct TcModule : GpuModule {
   std::string integ = "euler";
   bool inactive = false;
   DBuf ta, tb, tc, k1, k2, k3, rx, ry, coef, dbxx, dbxy, dbyx, dbyy;
+  // while recording (the adjoint's replay of a step): te after each Euler subcycle
+  bool recording = false;
+  std::vector<DBuf> tape;
+  double h_last = 0.0;
   explicit TcModule(const std::vector<std::pair<std::string, std::string>>& o) {
     name = "thermal_conduction";
     bool f1, f2, f3;
@@ -748,6 +699,7 @@ struct TcModule : GpuModule {
   }
   void iterate(Domain& d, double dt) override {  // thermalconduction.cpp:91-156
     const double h = dt / (double)nsub;
+    h_last = h;
     const double* cur = d.L.te.p;
     double* bufs[3] = {ta.p, tb.p, tc.p};
     auto other = [&](const double* a, const double* b) {
@@ -759,6 +711,13 @@ struct TcModule : GpuModule {
         double* out = other(cur, nullptr);
         step(d, cur, cur, out, 1.0, TC_UPDATE, nullptr, h);
         cur = out;
+        if (recording) {
+          if ((int)tape.size() <= s) {
+            tape.emplace_back();
+            tape.back().alloc(d.N);
+          }
+          hip_check(hipMemcpyAsync(tape[s].p, out, d.N * 8, hipMemcpyDeviceToDevice, d.stream), "record te");
+        }
       } else if (integ == "rk2") {
         double* half = other(cur, nullptr);
         step(d, cur, cur, half, 0.5, TC_UPDATE, nullptr, h);
@@ -822,8 +781,10 @@ struct RlModule : GpuModule {
     require(scal[0] < DBL_MAX, SPRUCE_E_NUMERIC, "Something went wrong in the min calculation (radiation)");
     nsub = subcycle_count(dt, p.eps * scal[0]);
   }
+  double h_last = 0.0;
   void iterate(Domain& d, double dt) override {  // radiativelosses.cpp:84-140
     const double h = dt / (double)nsub;
+    h_last = h;
     if (p.prevent) d.refresh_dt_grid();
     const int ig = integ == "euler" ? 0 : (integ == "rk2" ? 1 : 2);
     d.timed("rl_iterate", [&] { launch_rl_iterate(d.L.view(), d.dtgrid.p, p, d.kc, d.mesh, nsub, h, ig, tout.p, d.stream); });
@@ -1591,13 +1552,358 @@ struct DivCleanModule : GpuModule {
   }
 };
 
+// ParametricHeating: heating(y) = sum_k theta_k phi_k(y), the parametrised heating profile of the
+// gradient workload (no reference module; injected as AmbientHeating::postIterateModule does,
+// ambientheating.cpp:42-49).  phi_k are Gaussians in height centred on K points spread evenly over
+// [min pos_y, max pos_y], of width basis_width times their spacing.  The adjoint returns dJ/dtheta.
+struct ParamHeatModule : PostSourceModule {
+  int K = 16;
+  double width = 1.0;
+  std::vector<double> theta;
+  std::vector<double> basis_y;   // K x ny: phi_k at the height of row j
+  DBuf heating, basis, g_theta;  // g_theta: dJ/dtheta of the last adjoint
+  explicit ParamHeatModule(const Opts& o) {
+    name = "parametric_heating";
+    std::vector<std::pair<int, double>> th;
+    for (auto& kv : o) {
+      if (kv.first == "num_basis") K = std::stoi(kv.second);
+      else if (kv.first == "basis_width") width = std::stod(kv.second);
+      else if (kv.first.rfind("coeff_", 0) == 0) th.emplace_back(std::stoi(kv.first.substr(6)), std::stod(kv.second));
+    }
+    require(K >= 1 && K <= 64, SPRUCE_E_ARG, "parametric_heating: num_basis must be in [1, 64]");
+    theta.assign(K, 0.0);
+    for (auto& t : th) {
+      require(t.first >= 0 && t.first < K, SPRUCE_E_ARG, "parametric_heating: coefficient index out of range");
+      theta[t.first] = t.second;
+    }
+  }
+  bool needs_host_dt() const override { return false; }
+  void setup(Domain& d) override {
+    const auto& y = d.host_grids.at("pos_y");
+    double lo = y[0], hi = y[0];
+    for (double v : y) { lo = std::min(lo, v); hi = std::max(hi, v); }
+    const double sp = K > 1 ? (hi - lo) / (double)(K - 1) : hi - lo;
+    const double w = width * sp;
+    std::vector<double> h(d.N);
+    basis_y.assign((size_t)K * d.ny, 0.0);
+    for (int i = 0; i < d.nx; ++i)
+      for (int j = 0; j < d.ny; ++j) {
+        const size_t c = (size_t)i * d.ny + j;
+        const double mask = (i >= d.xl && i <= d.xu && j >= d.yl && j <= d.yu) ? 1.0 : 0.0;
+        double hsum = 0.0;
+        for (int q = 0; q < K; ++q) {
+          const double u = (y[c] - (lo + (double)q * sp)) / w;
+          const double phi = std::exp(-0.5 * (u * u));
+          hsum += theta[q] * phi;
+          if (i == 0) basis_y[(size_t)q * d.ny + j] = phi;   // pos_y varies along y only
+        }
+        h[c] = mask * hsum;
+      }
+    upload_vec(heating, h);
+    upload_vec(basis, basis_y);
+    g_theta.alloc(K);
+  }
+  void post(Domain& d) override {
+    d.timed("heating", [&] { launch_add_step_times(d.L.te.p, heating.p, d.step.p, d.N, d.stream); });
+    d.propagate_live();
+  }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Discrete adjoint of the step
+// ---------------------------------------------------------------------------------------------
+
+void Domain::replay_step(int n, const MhdSet& start, StepTape& tp, const std::vector<double>& times) {
+  copy_set(start, L.view(), N, stream);
+  hip_check(hipMemcpyAsync(step.p, history.p + n, sizeof(double), hipMemcpyDeviceToDevice, stream), "copy step");
+  tstep = times[n];
+  for (auto& m : mods)
+    if (auto* tc = dynamic_cast<TcModule*>(m.get())) tc->recording = true;
+  rec = &tp;
+  advance(n);
+  rec = nullptr;
+  for (auto& m : mods)
+    if (auto* tc = dynamic_cast<TcModule*>(m.get())) tc->recording = false;
+}
+
+std::vector<double> Domain::param_grad() {
+  std::vector<double> out;
+  for (auto& m : mods)
+    if (auto* ph = dynamic_cast<ParamHeatModule*>(m.get())) {
+      std::vector<double> g(ph->K);
+      hip_check(hipMemcpy(g.data(), ph->g_theta.p, ph->K * sizeof(double), hipMemcpyDeviceToHost), "read dJ/dtheta");
+      out.insert(out.end(), g.begin(), g.end());
+    }
+  return out;
+}
+
+void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, double* dt_out) {
+  require(ready, SPRUCE_E_STATE, "setup must precede adjoint");
+  const int integ = desc.time_integrator;
+  require(integ == SPRUCE_EULER || integ == SPRUCE_RK2, SPRUCE_E_UNSUPPORTED, "the adjoint covers euler and rk2");
+  const int* bc = desc.bc;
+  for (int k = 0; k < 4; ++k)
+    require(bc[k] == SPRUCE_BC_PERIODIC || bc[k] == SPRUCE_BC_OPEN || bc[k] == SPRUCE_BC_FIXED ||
+                bc[k] == SPRUCE_BC_REFLECT, SPRUCE_E_UNSUPPORTED,
+            "the adjoint covers periodic, open, fixed and reflect boundaries");
+  require(!((bc[0] == SPRUCE_BC_OPEN || bc[1] == SPRUCE_BC_OPEN) && bc[2] != SPRUCE_BC_PERIODIC) &&
+              !((bc[2] == SPRUCE_BC_OPEN || bc[3] == SPRUCE_BC_OPEN) && bc[0] != SPRUCE_BC_PERIODIC),
+          SPRUCE_E_UNSUPPORTED, "the adjoint of an open boundary needs the other axis periodic");
+  // modules: conduction (euler subcycles, no saturation) and radiative losses (no
+  // prevent_subcycling) act in iterateModule; ambient and parametrised heating after the integrator
+  std::vector<int> it_mods, post_mods;
+  rec_post.assign(mods.size(), false);
+  for (size_t k = 0; k < mods.size(); ++k) {
+    GpuModule* m = mods[k].get();
+    if (auto* tc = dynamic_cast<TcModule*>(m)) {
+      require(tc->integ == "euler" && !tc->p.saturation && !tc->inactive, SPRUCE_E_UNSUPPORTED,
+              "the adjoint covers thermal conduction with euler subcycles and no flux saturation");
+      it_mods.push_back((int)k);
+    } else if (auto* rl = dynamic_cast<RlModule*>(m)) {
+      require(!rl->p.prevent && !rl->inactive, SPRUCE_E_UNSUPPORTED,
+              "the adjoint covers radiative losses without prevent_subcycling");
+      it_mods.push_back((int)k);
+    } else if (dynamic_cast<ParamHeatModule*>(m) || dynamic_cast<HeatModule*>(m)) {
+      post_mods.push_back((int)k);
+      rec_post[k] = true;
+    } else {
+      throw Error(SPRUCE_E_UNSUPPORTED, "the adjoint does not cover module " + m->name);
+    }
+  }
+  require(nsteps >= 1 && ckpt >= 1, SPRUCE_E_ARG, "nsteps and checkpoint_interval must be positive");
+  require(seed && grad, SPRUCE_E_ARG, "null argument");
+  const bool with_mods = !mods.empty();
+  const int nseg = (nsteps + ckpt - 1) / ckpt;
+  const int sl = std::min(ckpt, nsteps);
+  const bool rk2 = integ == SPRUCE_RK2;
+  const int nb = adj_blocks(mesh);
+  const int nit = (int)it_mods.size(), npost = (int)post_mods.size();
+  // One allocation: checkpoints (nseg sets), the recomputed segment (sl + 1 sets); without modules
+  // its RK2 midpoints (sl sets) and stage derivatives (8 grids per step each), with modules the
+  // snapshots of one replayed step; the three adjoint vectors, face coefficients, module scratch
+  // (dJ/dT, the six frozen conduction fields, column sums) and the reduction partials.
+  const size_t SET = 10 * N, EV = 8 * N, CO = (size_t)ADJ_COEFS * N;
+  const size_t seg_store = with_mods ? (size_t)(nit + npost + 2) * SET + 2 * EV
+                                     : (rk2 ? (size_t)sl * SET : 0) + (size_t)sl * EV * (rk2 ? 2 : 1);
+  const size_t modscratch = with_mods ? 7 * N + (size_t)ny + (size_t)nb + 1 : 0;
+  const size_t total = (size_t)nseg * SET + (size_t)(sl + 1) * SET + seg_store + 3 * EV + 2 * CO + 4 * (size_t)nb +
+                       modscratch;
+  DBuf slab;
+  slab.alloc(total);
+  double* cur = slab.p;
+  auto take = [&](size_t n) { double* p = cur; cur += n; return p; };
+  std::vector<MhdSet> cps(nseg), seg(sl + 1), mid;
+  for (auto& c : cps) c = set_at(take(SET), N);
+  for (auto& c : seg) c = set_at(take(SET), N);
+  std::vector<double*> k1s, k2s;
+  StepTape tape;
+  if (with_mods) {
+    tape.after_it.resize(nit);
+    tape.after_post.resize(npost);
+    for (auto& c : tape.after_it) c = set_at(take(SET), N);
+    for (auto& c : tape.after_post) c = set_at(take(SET), N);
+    tape.W = set_at(take(SET), N);
+    tape.A3 = set_at(take(SET), N);
+    tape.k1 = take(EV);
+    tape.k2 = take(EV);
+  } else {
+    mid.resize(rk2 ? sl : 0);
+    for (auto& c : mid) c = set_at(take(SET), N);
+    k1s.resize(sl);
+    k2s.resize(rk2 ? sl : 0);
+    for (auto& k : k1s) k = take(EV);
+    for (auto& k : k2s) k = take(EV);
+  }
+  double* lam = take(EV);
+  double* lamN = take(EV);
+  double* lamW = take(EV);
+  double* cx = take(CO);
+  double* cy = take(CO);
+  double* part = take(nb);
+  double* part2 = take(nb);
+  double* pmin = take(nb);
+  int* pidx = reinterpret_cast<int*>(take(nb));
+  double *lamT = nullptr, *lamfz = nullptr, *colsum = nullptr, *partm = nullptr, *gh = nullptr;
+  if (with_mods) {
+    lamT = take(N);
+    lamfz = take(6 * N);
+    colsum = take(ny);
+    partm = take(nb);
+    gh = take(1);
+  }
+  if ((int)history.n < nsteps + 1) history.alloc(nsteps + 1);
+  hip_check(hipMemcpyAsync(history.p, step.p, sizeof(double), hipMemcpyDeviceToDevice, stream), "copy step");
+  sub_hist.assign((size_t)nsteps * std::max<size_t>(1, mods.size()), -1);
+  const double time0 = time;
+  tstep = time;
+  // forward sweep, exactly the steps of run()
+  for (int n = 0; n < nsteps; ++n) {
+    if (n % ckpt == 0) copy_set(L.view(), cps[n / ckpt], N, stream);
+    advance(n);
+  }
+  sync_and_check();
+  std::vector<double> h(nsteps), times(nsteps + 1);
+  hip_check(hipMemcpy(h.data(), history.p, nsteps * sizeof(double), hipMemcpyDeviceToHost), "read dt history");
+  times[0] = time0;
+  for (int n = 0; n < nsteps; ++n) {
+    times[n + 1] = times[n] + h[n];
+    time += h[n];
+    iter++;
+    if (dt_out) dt_out[n] = h[n];
+  }
+  // the handle keeps the state after the steps: the replays of the backward sweep overwrite the
+  // live grids, so a copy is restored at the end
+  DBuf keep;
+  MhdSet kept{};
+  if (with_mods) {
+    keep.alloc(SET);
+    kept = set_at(keep.p, N);
+    copy_set(L.view(), kept, N, stream);
+  }
+  for (auto& m : mods)
+    if (auto* ph = dynamic_cast<ParamHeatModule*>(m.get())) launch_fill(ph->g_theta.p, 0.0, ph->K, stream);
+  if (lamT) launch_fill(lamT, 0.0, N, stream);
+  hip_check(hipMemcpyAsync(lam, seed, 8 * N * 8, hipMemcpyHostToDevice, stream), "upload seed");
+  const MhdStatic st = stat();
+  for (int sg = nseg - 1; sg >= 0; --sg) {
+    const int n0 = sg * ckpt, n1 = std::min(nsteps, n0 + ckpt);
+    copy_set(cps[sg], seg[0], N, stream);
+    if (!with_mods) {
+      // recompute the segment: states, k1, W, k2 (and the final ghost fill of each step)
+      for (int n = n0; n < n1; ++n) {
+        const int i = n - n0;
+        if (!rk2) {
+          stage_view(seg[i], seg[i], seg[i + 1], 1.0, STAGE_STORE_K, history.p + n, k1s[i]);
+        } else {
+          stage_view(seg[i], seg[i], mid[i], 0.5, STAGE_STORE_K, history.p + n, k1s[i]);
+          stage_view(mid[i], seg[i], seg[i + 1], 1.0, STAGE_STORE_K, history.p + n, k2s[i]);
+        }
+        launch_mhd_ghost(seg[i + 1], kc, mesh, kinds_all, s1, s2, GHOST_NORMAL, stream);
+      }
+    } else {
+      copy_set(seg[0], L.view(), N, stream);
+      hip_check(hipMemcpyAsync(step.p, history.p + n0, sizeof(double), hipMemcpyDeviceToDevice, stream), "copy step");
+      tstep = times[n0];
+      for (int n = n0; n < n1; ++n) {
+        advance(n);
+        copy_set(L.view(), seg[n - n0 + 1], N, stream);
+      }
+    }
+    for (int n = n1 - 1; n >= n0; --n) {
+      const int i = n - n0;
+      const MhdSet& Ln = seg[i];
+      const double* hp = history.p + n;
+      if (!with_mods) {
+        adj_ghost(seg[i + 1], lam);
+        // lamN and lamW are written first by the stage-cell kernels (assign), so they need no clear
+        if (rk2) {
+          timed("adj_stage", [&] {
+            launch_adj_stage_cells(Ln, k2s[i], hp, 1.0, mid[i], st, kc, mesh, lam, lamN, 1, lamW, 1, cx, cy, part,
+                                   stream);
+          });
+          timed("adj_faces", [&] { launch_adj_faces(mid[i], st, kc, mesh, cx, cy, lamW, stream); });
+          timed("adj_stage", [&] {
+            launch_adj_stage_cells(Ln, k1s[i], hp, 0.5, Ln, st, kc, mesh, lamW, lamN, 0, lamN, 0, cx, cy, part2,
+                                   stream);
+          });
+        } else {
+          timed("adj_stage", [&] {
+            launch_adj_stage_cells(Ln, k1s[i], hp, 1.0, Ln, st, kc, mesh, lam, lamN, 1, lamN, 0, cx, cy, part, stream);
+          });
+        }
+        timed("adj_faces", [&] { launch_adj_faces(Ln, st, kc, mesh, cx, cy, lamN, stream); });
+        timed("adj_dt", [&] {
+          launch_adj_dt(Ln, st, kc, mesh, kc.eps, part, rk2 ? part2 : nullptr, nullptr, pmin, pidx, lamN, stream);
+        });
+        std::swap(lam, lamN);
+        continue;
+      }
+      // with modules: replay the step keeping its intermediate states, then reverse it
+      replay_step(n, Ln, tape, times);
+      launch_fill(gh, 0.0, 1, stream);
+      for (int p = npost - 1; p >= 0; --p) {              // post-iterate modules (heating)
+        const MhdSet& post = tape.after_post[p];
+        const MhdSet& pre = p ? tape.after_post[p - 1] : tape.A3;
+        GpuModule* m = mods[post_mods[p]].get();
+        adj_ghost(post, lam);
+        auto* ph = dynamic_cast<ParamHeatModule*>(m);
+        const double* heat = ph ? ph->heating.p : static_cast<HeatModule*>(m)->heating.p;
+        timed("adj_heating", [&] {
+          launch_adj_minmask(pre.rho, pre.te, heat, hp, kc, mesh, lam, partm, stream);
+          launch_adj_accum(partm, nb, 1.0, gh, stream);
+          if (ph) launch_adj_heat_params(lam + 4 * N, ph->basis.p, ph->K, hp, mesh, colsum, ph->g_theta.p, stream);
+        });
+      }
+      adj_ghost(tape.A3, lam);                             // the integrator's final ghost fill
+      const MhdSet& A2 = nit ? tape.after_it[nit - 1] : Ln;
+      if (rk2) {
+        timed("adj_stage", [&] {
+          launch_adj_stage_cells(A2, tape.k2, hp, 1.0, tape.W, st, kc, mesh, lam, lamN, 1, lamW, 1, cx, cy, part,
+                                 stream);
+        });
+        timed("adj_faces", [&] { launch_adj_faces(tape.W, st, kc, mesh, cx, cy, lamW, stream); });
+        timed("adj_stage", [&] {
+          launch_adj_stage_cells(A2, tape.k1, hp, 0.5, A2, st, kc, mesh, lamW, lamN, 0, lamN, 0, cx, cy, part2,
+                                 stream);
+        });
+      } else {
+        timed("adj_stage", [&] {
+          launch_adj_stage_cells(A2, tape.k1, hp, 1.0, A2, st, kc, mesh, lam, lamN, 1, lamN, 0, cx, cy, part, stream);
+        });
+      }
+      timed("adj_faces", [&] { launch_adj_faces(A2, st, kc, mesh, cx, cy, lamN, stream); });
+      std::swap(lam, lamN);
+      for (int k = nit - 1; k >= 0; --k) {                 // iterating modules, last first
+        const MhdSet& post = tape.after_it[k];
+        const MhdSet& pre = k ? tape.after_it[k - 1] : Ln;
+        GpuModule* m = mods[it_mods[k]].get();
+        adj_ghost(post, lam);
+        if (auto* tc = dynamic_cast<TcModule*>(m)) {
+          const int ns = tc->nsub;
+          const double* te_last = ns ? tc->tape[ns - 1].p : pre.te;
+          timed("adj_tc", [&] {
+            launch_adj_minmask(pre.rho, te_last, nullptr, nullptr, kc, mesh, lam, nullptr, stream);
+            launch_fill(lamfz, 0.0, 6 * N, stream);
+            for (int q = ns - 1; q >= 0; --q) {
+              const double* te_s = q ? tc->tape[q - 1].p : pre.te;
+              launch_adj_tc_sub(te_s, pre.n, bhx.p, bhy.p, tc->dbxx.p, tc->dbxy.p, tc->dbyx.p, tc->dbyy.p, tc->h_last,
+                                tc->p.k0, kc, mesh, lam + 4 * N, lamT, lamfz, partm, stream);
+              launch_adj_accum(partm, nb, 1.0 / (double)ns, gh, stream);
+              launch_adj_tc_temp(te_s, pre.n, kc, N, lamT, lam + 4 * N, lam, stream);
+            }
+            launch_adj_bhat(Ln, st, mesh, lamfz, lam, stream);
+          });
+        } else {
+          auto* rl = static_cast<RlModule*>(m);
+          const int ig = rl->integ == "euler" ? 0 : (rl->integ == "rk2" ? 1 : 2);
+          timed("adj_rl", [&] {
+            launch_adj_rl(pre, rl->p, kc, mesh, rl->nsub, rl->h_last, ig, lam, partm, stream);
+            if (rl->nsub > 0) launch_adj_accum(partm, nb, 1.0 / (double)rl->nsub, gh, stream);
+          });
+        }
+      }
+      timed("adj_dt", [&] {
+        launch_adj_dt(Ln, st, kc, mesh, kc.eps, part, rk2 ? part2 : nullptr, gh, pmin, pidx, lam, stream);
+      });
+    }
+  }
+  hip_check(hipMemcpyAsync(grad, lam, 8 * N * 8, hipMemcpyDeviceToHost, stream), "download gradient");
+  if (with_mods) {                                      // the state after the steps and its step size
+    copy_set(kept, L.view(), N, stream);
+    finish_step(-1);
+  }
+  hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  tstep = time;
+  if (clock.enabled) clock.collect();
+}
+
 void Domain::add_module(const std::string& name, const std::vector<std::pair<std::string, std::string>>& opts) {
   require(!ready, SPRUCE_E_STATE, "modules must be added before setup");
   static const char* known[] = {"ambient_heating", "radiative_losses", "thermal_conduction", "localized_heating",
                                 "field_heating", "sg_filtering", "coulomb_explosion", "eic_thermalization",
                                 "artificial_viscosity", "global_temperature", "anomalous_resistivity",
                                 "momentum_injection", "tracer_particles", "mass_injection", "ambient_heating_sink",
-                                "physical_viscosity", "div_cleaning", "boundary_outflow"};
+                                "physical_viscosity", "div_cleaning", "boundary_outflow", "parametric_heating"};
   bool ok = false;
   for (auto* k : known) ok = ok || name == k;
   require(ok, SPRUCE_E_ARG, "Module name was not recognized");
@@ -1612,6 +1918,7 @@ void Domain::add_module(const std::string& name, const std::vector<std::pair<std
   else if (name == "ambient_heating_sink") mods.push_back(std::make_unique<HeatSinkModule>(opts));
   else if (name == "field_heating") mods.push_back(std::make_unique<FieldHeatModule>(opts));
   else if (name == "div_cleaning") mods.push_back(std::make_unique<DivCleanModule>(opts));
+  else if (name == "parametric_heating") mods.push_back(std::make_unique<ParamHeatModule>(opts));
   else throw Error(SPRUCE_E_UNSUPPORTED, "module " + name + " is not available in this build");
   modules.emplace_back(name, opts);
 }
@@ -1926,6 +2233,18 @@ int spruce_gpu_adjoint(spruce_gpu_domain* h, int32_t nsteps, int32_t checkpoint_
 int spruce_gpu_adjoint_state(spruce_gpu_domain* h, const double* grad_evolved, double* grad_state) {
   API_BEGIN
   dom(h).adjoint_state(grad_evolved, grad_state);
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_adjoint_params(spruce_gpu_domain* h, double* grad_out, int32_t count, int32_t* count_out) {
+  API_BEGIN
+  const std::vector<double> g = dom(h).param_grad();
+  require(count_out || grad_out, SPRUCE_E_ARG, "null argument");
+  if (count_out) *count_out = (int32_t)g.size();
+  if (grad_out) {
+    require(count >= (int32_t)g.size(), SPRUCE_E_ARG, "grad_out holds fewer values than the parameters");
+    std::memcpy(grad_out, g.data(), g.size() * sizeof(double));
+  }
   API_END(SPRUCE_OK)
 }
 

@@ -165,6 +165,9 @@ class DomainBase {
     (void)grad_evolved; (void)grad_state;
     throw Error(SPRUCE_E_UNSUPPORTED, "the adjoint is available for ideal_mhd only");
   }
+  virtual std::vector<double> param_grad() {
+    throw Error(SPRUCE_E_UNSUPPORTED, "the adjoint is available for ideal_mhd only");
+  }
 
   double time = 0.0;
   int iter = 0;

@@ -150,7 +150,8 @@ const std::vector<std::string> MODULE_NAMES = {
     "radiative_losses", "thermal_conduction", "ambient_heating", "anomalous_resistivity", "momentum_injection",
     "localized_heating", "field_heating", "tracer_particles", "sg_filtering", "coulomb_explosion",
     "eic_thermalization", "artificial_viscosity", "global_temperature", "mass_injection", "ambient_heating_sink",
-    "physical_viscosity", "div_cleaning", "boundary_outflow"};
+    "physical_viscosity", "div_cleaning", "boundary_outflow",
+    "parametric_heating"};  // this build's parametrised heating profile (no reference module)
 // module options that ask for file output (fileOutput hooks)
 const std::vector<std::string> OUTPUT_KEYS = {"output_to_file", "visc_output_visc", "visc_output_lap",
                                               "visc_output_strength", "visc_output_timescale"};

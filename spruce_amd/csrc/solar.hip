@@ -6,6 +6,7 @@
 #include "arith.hpp"
 #include "crpow.hpp"
 #include "solar.hpp"
+#include "tc_cell.hpp"
 
 namespace spruce {
 namespace {
@@ -179,52 +180,19 @@ __global__ void __launch_bounds__(256) k_tc_sat(const double* te, const double* 
 }
 
 // ThermalConduction::thermalEnergyDerivative at an interior cell (thermalconduction.cpp:158-176),
-// from the temperatures of its 3x3 neighbourhood.
+// from the temperatures of its 3x3 neighbourhood: the unsaturated part (tc_cell.hpp), then the
+// saturation factor and correction.
 __device__ double tc_energy_derivative(const TcStepArgs& a, int i, int j, const double T[3][3]) {
   const Mesh& M = a.mesh;
   const int ny = M.ny;
   const int im = wrp(i - 1, M.nx, M.x.per), ip = wrp(i + 1, M.nx, M.x.per);
   const int jm = wrp(j - 1, M.ny, M.y.per), jp = wrp(j + 1, M.ny, M.y.per);
-  const double Tc = T[1][1];
-  // face interpolants of T on the four faces of (i,j)
-  const double Txu = fint(M.x, ip, i, Tc, T[2][1]), Txl = fint(M.x, i, im, T[0][1], Tc);
-  const double Tyu = fint(M.y, jp, j, Tc, T[1][2]), Tyl = fint(M.y, j, jm, T[1][0], Tc);
   const double dx = M.x.d[i], rdx = M.x.rd[i], dy = M.y.d[j], rdy = M.y.rd[j];
-  const double gx = qdiv(Txu - Txl, dx, rdx), gy = qdiv(Tyu - Tyl, dy, rdy);
-  // derivative1D(dtemp_dx, 1): dtemp_dx is zero outside the interior (derivs.cpp:229-233)
-  auto gx_at = [&](int jj, int b1) -> double {
-    if (!inside(M, i, jj)) return 0.0;
-    const double u = fint(M.x, ip, i, T[1][b1], T[2][b1]), l = fint(M.x, i, im, T[0][b1], T[1][b1]);
-    return qdiv(u - l, dx, rdx);
-  };
-  auto gy_at = [&](int ii, int a1) -> double {
-    if (!inside(M, ii, j)) return 0.0;
-    const double u = fint(M.y, jp, j, T[a1][1], T[a1][2]), l = fint(M.y, j, jm, T[a1][0], T[a1][1]);
-    return qdiv(u - l, dy, rdy);
-  };
-  const double gxm = gx_at(jm, 0), gxp = gx_at(jp, 2);
-  const double gym = gy_at(im, 0), gyp = gy_at(ip, 2);
-  const double dgx_y = qdiv(fint(M.y, jp, j, gx, gxp) - fint(M.y, j, jm, gxm, gx), dy, rdy);
-  const double dgy_x = qdiv(fint(M.x, ip, i, gy, gyp) - fint(M.x, i, im, gym, gy), dx, rdx);
-  // secondDerivative1D (derivs.cpp:450)
-  const double s0 = qdiv(Txu - 2.0 * Tc + Txl, M.x.hsq[i], M.x.rhsq[i]);
-  const double s1 = qdiv(Tyu - 2.0 * Tc + Tyl, M.y.hsq[j], M.y.rhsq[j]);
   // b_hat and its derivatives (frozen for the step, computed once by k_bhat_derivs)
   const size_t c = (size_t)i * ny + j;
   const size_t cxm = (size_t)im * ny + j, cxp = (size_t)ip * ny + j, cym = (size_t)i * ny + jm, cyp = (size_t)i * ny + jp;
-  const double bx = a.bhx[c], by = a.bhy[c];
-  const double dbx_x = a.dbxx[c], dbx_y = a.dbxy[c], dby_x = a.dbyx[c], dby_y = a.dbyy[c];
-
-  const double bdg = bx * gx + by * gy;
-  const double t1x = bx * s0 + by * dgx_y, t1y = bx * dgy_x + by * s1;
-  const double t2x = gx * dbx_x + gy * dbx_y, t2y = gx * dby_x + gy * dby_y;
-  const double curl = dby_x - dbx_y;
-  const double t3x = (-(-1.0 * curl)) * gy, t3y = (-1.0 * curl) * gx;
-  const double p15 = pow_1p5(Tc), p25 = pow_2p5(Tc);
-  const double ttx = 5.0 / 2.0 * p15 * gx * bdg + p25 * (t1x + t2x + t3x);
-  const double tty = 5.0 / 2.0 * p15 * gy * bdg + p25 * (t1y + t2y + t3y);
-  const double divb = dbx_x + dby_y;
-  double result = ((p25 * bdg * divb) + (bx * ttx + by * tty)) * a.p.k0;
+  double result = tcc::tc_core<double>(M, i, j, T, a.bhx[c], a.bhy[c], a.dbxx[c], a.dbxy[c], a.dbyx[c], a.dbyy[c],
+                                       a.p.k0);
   if (a.p.saturation) {                                         // thermalconduction.cpp:171-174, 255-271
     const double cc = a.coef[c];
     const double dcx = qdiv(fint(M.x, ip, i, cc, a.coef[cxp]) - fint(M.x, i, im, a.coef[cxm], cc), dx, rdx);

@@ -80,6 +80,7 @@ def lib() -> ctypes.CDLL:
     L.spruce_gpu_batch_destroy.argtypes = [P]
     L.spruce_gpu_adjoint.argtypes = [P, I32, I32, DP, DP, DP]
     L.spruce_gpu_adjoint_state.argtypes = [P, DP, DP]
+    L.spruce_gpu_adjoint_params.argtypes = [P, DP, I32, ctypes.POINTER(I32)]
     L.spruce_gpu_batch_set_profiling.argtypes = [P, I32]
     L.spruce_gpu_batch_kernel_times.argtypes = [P, ctypes.c_char_p, I32, DP, ctypes.POINTER(I32), I32]
     _LIB = L
@@ -212,6 +213,15 @@ class GpuDomain:
         dts = np.empty(nsteps)
         _chk(lib().spruce_gpu_adjoint(self.h, nsteps, checkpoint, _dp(seed), _dp(grad), _dp(dts)))
         return grad, dts
+
+    def adjoint_params(self) -> np.ndarray:
+        """dJ/dtheta of the last adjoint call for the parametric_heating coefficients, in module order."""
+        n = ctypes.c_int32(0)
+        _chk(lib().spruce_gpu_adjoint_params(self.h, None, 0, ctypes.byref(n)))
+        out = np.empty(n.value)
+        if n.value:
+            _chk(lib().spruce_gpu_adjoint_params(self.h, _dp(out), n.value, None))
+        return out
 
     def adjoint_state(self, grad_evolved: np.ndarray) -> np.ndarray:
         """dJ/d(.state grids rho, temp, mom_x, mom_y, mom_z, bi_x, bi_y, bi_z) from dJ/d(evolved grids

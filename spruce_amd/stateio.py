@@ -47,6 +47,8 @@ MODULE_NAMES = (
     "artificial_viscosity", "global_temperature", "anomalous_resistivity",
     "momentum_injection", "tracer_particles", "mass_injection", "ambient_heating_sink",
     "physical_viscosity", "div_cleaning", "boundary_outflow",
+    # the parametrised heating profile of the gradient workload (no reference module)
+    "parametric_heating",
 )
 
 BOUNDARY_NAMES = ("periodic", "open", "fixed", "reflect", "open_moc", "open_ucnp")

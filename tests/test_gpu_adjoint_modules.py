@@ -1,0 +1,259 @@
+"""GPU discrete adjoint with non-periodic boundaries and modules, against central finite
+differences of the GPU forward.
+
+Covered: the ghost-fill reverse of fixed, reflect and open boundaries; thermal conduction (Euler
+subcycles, no flux saturation), radiative losses and the parametrised heating profile of the
+gradient workload, whose coefficients get their own gradient (spruce_gpu_adjoint_params).  The loss
+is a seeded linear functional of the final evolved grids.  It is only piecewise smooth (limiter,
+min/max and floor selections, the argmin of the step size, the integer subcycle counts), so every
+directional check takes two difference steps and leaves out directions whose two differences
+disagree (a branch crossed inside the interval); the rest must agree with the adjoint to 1e-6
+relative, the tolerance of the adjoint requirement."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleDomain, set_pow_mode
+from spruce_amd import workloads as W
+from spruce_amd.gpu import GpuDomain, SpruceError
+from spruce_amd.spec import spec_from_files
+from spruce_amd.stateio import StateFile
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+from cases import mini_arcade, random_mhd_state  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+STATE = ["rho", "temp", "mom_x", "mom_y", "mom_z", "bi_x", "bi_y", "bi_z"]
+SCALE = {"rho": 1e-15, "temp": 1e6, "mom_x": 1e-8, "mom_y": 1e-8, "mom_z": 1e-8, "bi_x": 1.0, "bi_y": 1.0,
+         "bi_z": 1.0}
+TC = ("thermal_conduction", [("flux_saturation", "false"), ("epsilon", "0.5"), ("dt_subcycle_min", "1e-6"),
+                             ("time_integrator", "euler")])
+
+
+def RL(integ="euler"):
+    return ("radiative_losses", [("cutoff_ramp", "2e4"), ("cutoff_temp", "3e4"), ("epsilon", "0.5"),
+                                 ("time_integrator", integ)])
+
+
+def HEAT(theta):
+    return ("parametric_heating", [("num_basis", str(len(theta)))] +
+            [("coeff_%d" % k, repr(float(t))) for k, t in enumerate(theta)])
+
+
+THETA = np.array([1e-2 * (1 + 0.1 * k) for k in range(16)])
+
+
+def _arcade():
+    """A small stratified arcade.  Its density is uniform along x, so at step 0 the x-face limiter
+    of the density compares equal values and selects among equal branches: the loss has a kink at
+    that state in every direction that moves the density unevenly (difference quotients agree with
+    each other there but not with any one gradient).  A 1% seeded density noise removes the ties."""
+    st = mini_arcade(24, 40, 7)
+    u = W.uniform(4242, (24, 40))
+    st.grids["rho"] = st.grids["rho"] * (1.0 + 0.01 * (2 * u - 1))
+    return st
+
+
+def _state(grids, base):
+    return StateFile(base.xdim, base.ydim, base.ion_mass, base.adiabatic_index, base.t, grids, list(base.comments))
+
+
+def _loss(spec, nsteps, w):
+    d = GpuDomain(spec)
+    try:
+        d.step(nsteps)
+        out = [d.grid(v) for v in ("rho", "mom_x", "thermal_energy", "bi_y")]
+    finally:
+        d.close()
+    return sum(float(np.sum(wk * o)) for wk, o in zip(w, out))
+
+
+def _seed(w, shape):
+    seed = np.zeros((8,) + shape)
+    seed[0], seed[1], seed[4], seed[6] = w
+    return seed
+
+
+def _weights(rng, shape, scales):
+    return [rng.uniform(-1, 1, shape) * s for s in scales]
+
+
+def _relative_scale(base):
+    """Direction scales proportional to each grid's own magnitude (with a floor for grids that are
+    zero somewhere): the arcade's density spans three decades, and a perturbation of a few ulps of
+    a cell is swamped by the re-rounding rho = max(rho/m, n_min) * m of every update."""
+    out = {}
+    for v in STATE:
+        a = np.abs(base.grids[v])
+        out[v] = a + 1e-3 * float(np.max(a)) + 1e-3 * SCALE[v]
+    return out
+
+
+def _check_directions(base, make_cfg, nsteps, ckpt, w, rng, trials=6, min_smooth=4, steps=(1e-7, 1e-8),
+                      scale=None):
+    shape = (base.xdim, base.ydim)
+    scale = scale if scale is not None else {v: SCALE[v] for v in STATE}
+    d = GpuDomain(spec_from_files(make_cfg(), base))
+    try:
+        g_ev, _ = d.adjoint(nsteps, ckpt, _seed(w, shape))
+        g = d.adjoint_state(g_ev)
+    finally:
+        d.close()
+    smooth, worst = 0, 0.0
+    for trial in range(trials):
+        dirs = {v: rng.uniform(-1, 1, shape) * scale[v] for v in STATE}
+        ad = sum(float(np.sum(g[k] * dirs[v])) for k, v in enumerate(STATE))
+        fds = []
+        for eps in steps:
+            gp = {v: base.grids[v] + eps * dirs[v] if v in dirs else base.grids[v] for v in base.grids}
+            gm = {v: base.grids[v] - eps * dirs[v] if v in dirs else base.grids[v] for v in base.grids}
+            fds.append((_loss(spec_from_files(make_cfg(), _state(gp, base)), nsteps, w) -
+                        _loss(spec_from_files(make_cfg(), _state(gm, base)), nsteps, w)) / (2 * eps))
+        consistent = abs(fds[0] - fds[1]) <= 1e-6 * abs(fds[0])
+        rel = abs(ad - fds[0]) / (abs(ad) + abs(fds[0]))
+        print("direction", trial, "adjoint %.12e fd %.12e / %.12e rel %.2e%s"
+              % (ad, fds[0], fds[1], rel, "" if consistent else " (branch crossed)"), flush=True)
+        if consistent:
+            smooth += 1
+            worst = max(worst, rel)
+    assert smooth >= min_smooth
+    assert worst < 1e-6
+
+
+BOUNDARIES = [
+    ("fixed_reflect_rk2", ("periodic", "periodic", "fixed", "reflect"), "rk2", 12, 5),
+    ("reflect_fixed_x_euler", ("reflect", "fixed", "periodic", "periodic"), "euler", 10, 4),
+    ("fixed_open_rk2", ("periodic", "periodic", "fixed", "open"), "rk2", 12, 4),
+]
+
+
+@pytest.mark.parametrize("name,bc,integ,nsteps,ckpt", BOUNDARIES, ids=[b[0] for b in BOUNDARIES])
+def test_boundary_adjoint_matches_finite_differences(name, bc, integ, nsteps, ckpt):
+    print("adjoint", name, flush=True)
+    base = random_mhd_state(40, 48, 93, stretch=True)
+    rng = np.random.default_rng(11)
+    w = _weights(rng, (40, 48), (1e15, 1e8, 10.0, 1.0))
+
+    def cfg():
+        return W.base_config("ideal_mhd", bc, integ, epsilon=0.2, density_min=1.0, temp_min=1.0,
+                             thermal_energy_min=1e-12)
+    _check_directions(base, cfg, nsteps, ckpt, w, rng)
+
+
+def _arcade_cfg(theta, bc=("periodic", "periodic", "fixed", "open"), integ="rk2", mods=None):
+    mods = mods if mods is not None else [TC, RL(), HEAT(theta)]
+    return W.base_config("ideal_mhd", bc, integ, epsilon=0.2, density_min=1.0, temp_min=1e4,
+                         thermal_energy_min=1e-8, modules=mods)
+
+
+def test_parametric_heating_forward_matches_restatement():
+    """The gradient workload's physics (conduction, radiation, parametrised heating) on a small
+    arcade: bit-identical to the restatement with correctly rounded powers, step by step."""
+    st = _arcade()
+    spec = spec_from_files(_arcade_cfg(THETA), st)
+    set_pow_mode(1)
+    gpu = GpuDomain(spec)
+    ora = OracleDomain(spec)
+    try:
+        for s in range(6):
+            dg, sg = gpu.step(1, subcycles=True)
+            do = ora.step(1)
+            assert list(sg[0]) == ora.subcycles(), (s, sg[0], ora.subcycles())
+            assert dg[0] == do[0], (s, dg, do)
+        for v in ("rho", "thermal_energy", "mom_x", "mom_y", "bi_x", "bi_y", "temp", "dt"):
+            assert np.array_equal(gpu.grid(v), ora.grid(v)), v
+    finally:
+        set_pow_mode(0)
+        gpu.close()
+        ora.close()
+
+
+MODULE_CASES = [
+    ("tc_rl_heat_rk2", ("periodic", "periodic", "fixed", "open"), "rk2", None, 6, 4),
+    ("tc_rl2_ambient_euler", ("periodic", "periodic", "reflect", "fixed"), "euler",
+     [TC, RL("rk2"), ("ambient_heating", [("heating_rate", "3e-3")]), HEAT(THETA[:8])], 6, 3),
+]
+
+
+@pytest.mark.parametrize("name,bc,integ,mods,nsteps,ckpt", MODULE_CASES, ids=[m[0] for m in MODULE_CASES])
+def test_module_adjoint_parameters_match_finite_differences(name, bc, integ, mods, nsteps, ckpt):
+    """dJ/dtheta_k of the heating coefficients against central differences in theta_k."""
+    print("adjoint params", name, flush=True)
+    st = _arcade()
+    theta = THETA if mods is None else THETA[:8]
+
+    def cfg(th):
+        m = None if mods is None else [x if x[0] != "parametric_heating" else HEAT(th) for x in mods]
+        return _arcade_cfg(th, bc, integ, m)
+    rng = np.random.default_rng(21)
+    w = _weights(rng, (24, 40), (1e13, 1e6, 1.0, 1.0))
+    d = GpuDomain(spec_from_files(cfg(theta), st))
+    try:
+        d.adjoint(nsteps, ckpt, _seed(w, (24, 40)))
+        gt = d.adjoint_params()
+    finally:
+        d.close()
+    assert gt.shape == theta.shape
+    smooth, worst = 0, 0.0
+    for k in range(0, len(theta), 2 if len(theta) <= 8 else 3):
+        fds = []
+        for rel in (1e-5, 1e-6):
+            e = rel * theta[k]
+            tp, tm = theta.copy(), theta.copy()
+            tp[k] += e
+            tm[k] -= e
+            fds.append((_loss(spec_from_files(cfg(tp), st), nsteps, w) -
+                        _loss(spec_from_files(cfg(tm), st), nsteps, w)) / (tp[k] - tm[k]))
+        consistent = abs(fds[0] - fds[1]) <= 1e-5 * abs(fds[0])
+        r = abs(gt[k] - fds[0]) / (abs(gt[k]) + abs(fds[0]))
+        print("theta", k, "adjoint %.12e fd %.12e / %.12e rel %.2e%s"
+              % (gt[k], fds[0], fds[1], r, "" if consistent else " (branch crossed)"), flush=True)
+        if consistent:
+            smooth += 1
+            worst = max(worst, r)
+    assert smooth >= 4
+    assert worst < 1e-6
+
+
+@pytest.mark.parametrize("name,bc,integ,mods,nsteps,ckpt", MODULE_CASES, ids=[m[0] for m in MODULE_CASES])
+def test_module_adjoint_state_matches_finite_differences(name, bc, integ, mods, nsteps, ckpt):
+    print("adjoint state", name, flush=True)
+    st = _arcade()
+    rng = np.random.default_rng(31)
+    w = _weights(rng, (24, 40), (1e13, 1e6, 1.0, 1.0))
+
+    def cfg():
+        return _arcade_cfg(THETA, bc, integ, mods)
+    _check_directions(st, cfg, nsteps, ckpt, w, rng, scale=_relative_scale(st))
+
+
+def test_module_adjoint_leaves_the_same_final_state_as_stepping():
+    st = _arcade()
+    spec = spec_from_files(_arcade_cfg(THETA), st)
+    a = GpuDomain(spec)
+    b = GpuDomain(spec)
+    try:
+        dta = a.step(7)
+        _, dtb = b.adjoint(7, 3, np.zeros((8, 24, 40)))
+        assert np.array_equal(dta, dtb)
+        for v in ("rho", "mom_x", "thermal_energy", "bi_y", "dt"):
+            assert np.array_equal(a.grid(v), b.grid(v)), v
+        assert np.array_equal(a.step(2), b.step(2))
+    finally:
+        a.close()
+        b.close()
+
+
+def test_module_adjoint_refuses_what_it_does_not_cover():
+    st = _arcade()
+    sat = ("thermal_conduction", [("flux_saturation", "true"), ("epsilon", "0.5"), ("dt_subcycle_min", "1e-6")])
+    for mods in ([sat], [("physical_viscosity", [("coeff", "0.1")])]):
+        d = GpuDomain(spec_from_files(_arcade_cfg(THETA, mods=mods), st))
+        try:
+            with pytest.raises(SpruceError):
+                d.adjoint(2, 1, np.zeros((8, 24, 40)))
+        finally:
+            d.close()
