@@ -1,6 +1,6 @@
 """Time-step throughput of the MI355X path: cell-updates/s and the HBM roofline of the stage kernel.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload arcade|periodic|periodic2048|arcade2048|batch2f|adjoint|copy]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload arcade|periodic|periodic2048|arcade2048|batch2f|adjoint|adjoint_arcade|copy]
 
 One "step" is one PlasmaDomain::advanceTime (evolution.cpp:59-82) over the whole grid, ghost cells
 included: every integrator stage, every module subcycle, the ghost fill and the dt reduction.
@@ -257,6 +257,61 @@ def bench_adjoint(args) -> dict:
     }
 
 
+def bench_adjoint_arcade(args) -> dict:
+    """The gradient workload with modules: the 2048x2048 arcade (conduction, radiative losses and
+    the 16-coefficient parametrised heating), RK2, checkpoint every 50 steps; gradient of the
+    coronal temperature misfit with respect to the 16 heating coefficients.  Reports the cost ratio
+    (checkpointed forward + per-step replays + reverse sweep) / forward, the device memory the
+    adjoint allocated, and central differences on 4 coefficients."""
+    import numpy as np
+    from spruce_amd.adjoint import ArcadeHeatingProblem
+
+    steps = args.steps if args.steps != 200 else 1000
+    n = args.grid or 2048
+    prob = ArcadeHeatingProblem(n, n, nsteps=steps, checkpoint=50)
+    th = prob.theta0
+    print("adjoint_arcade: forward warm-up", file=sys.stderr, flush=True)
+    prob.forward(th)
+    loss, final, t_fwd = prob.forward(th)
+    print("adjoint_arcade: forward %.2f s, loss %.6e; adjoint" % (t_fwd, loss), file=sys.stderr, flush=True)
+    g, t_adj, mem = prob.gradient(th, final)
+    print("adjoint_arcade: adjoint %.2f s" % t_adj, file=sys.stderr, flush=True)
+    # Central differences need a loss that is smooth in theta over the difference interval.  Over
+    # tens of steps this one is not: the conduction subcycle count is an integer set by the
+    # smallest time scale over the whole grid, which the cells cooling at the open top move, so the
+    # loss jumps wherever that count changes.  The probes use the same grid over 5 steps.
+    short = ArcadeHeatingProblem(n, n, nsteps=5, checkpoint=50)
+    _, fin5, _ = short.forward(th)
+    g5, _, _ = short.gradient(th, fin5)
+    fd = []
+    for k in (0, 5, 10, 15):
+        ds = []
+        for rel in (1e-5, 1e-6):
+            e = rel * th[k]
+            tp, tm = th.copy(), th.copy()
+            tp[k] += e
+            tm[k] -= e
+            ds.append((short.forward(tp)[0] - short.forward(tm)[0]) / (tp[k] - tm[k]))
+        fd.append({"k": k, "adjoint": float(g5[k]), "central_difference": ds[0], "central_difference_1e-6": ds[1],
+                   "relative_difference": abs(g5[k] - ds[0]) / (abs(g5[k]) + abs(ds[0]))})
+        print("adjoint_arcade: 5 steps, theta_%d adjoint %.9e fd %.9e %.9e" % (k, g5[k], ds[0], ds[1]),
+              file=sys.stderr, flush=True)
+    cells = n * n
+    return {
+        "metric": "cell-updates/s and % HBM roofline (FP64); adjoint/forward cost ratio",
+        "value": t_adj / t_fwd, "unit": "adjoint/forward time ratio", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": t_adj / steps * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "adjoint of the solar arcade %dx%d (ideal MHD + thermal conduction + radiative "
+                               "losses + 16-coefficient parametric heating, RK2, FP64), %d steps, checkpoint "
+                               "every 50; gradient of the coronal temperature misfit w.r.t. the heating "
+                               "coefficients" % (n, n, steps)},
+        "forward_s": t_fwd, "adjoint_s": t_adj, "forward_cell_updates_per_s": cells * steps / t_fwd,
+        "adjoint_device_bytes": mem, "loss": loss, "dJ_dtheta": [float(x) for x in g],
+        "fd_probes": {"steps": 5, "relative_step": 1e-5, "probes": fd},
+    }
+
+
 def bench_copy(args) -> dict:
     """HBM calibration: one elementwise kernel (torch.neg) reading and writing 1 GiB of FP64.  Its
     FETCH_SIZE / WRITE_SIZE counters per dispatch, against the known 1 GiB each way, give the factor
@@ -295,6 +350,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-kernel", default="auto", choices=["auto", "strip", "cell"])
     ap.add_argument("--strip-rows", type=int, default=0)
+    ap.add_argument("--grid", type=int, default=0, help="edge of the square arcade of the adjoint_arcade workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -308,6 +364,9 @@ def main() -> None:
         return
     if args.workload == "adjoint":
         print(json.dumps(bench_adjoint(args)))
+        return
+    if args.workload == "adjoint_arcade":
+        print(json.dumps(bench_adjoint_arcade(args)))
         return
     if args.workload == "copy":
         print(json.dumps(bench_copy(args)))

@@ -152,6 +152,9 @@ int spruce_gpu_adjoint(spruce_gpu_domain* d, int32_t nsteps, int32_t checkpoint_
  * order.  count_out (may be NULL) receives the number of parameters; grad_out (may be NULL) must
  * hold count >= that many doubles. */
 int spruce_gpu_adjoint_params(spruce_gpu_domain* d, double* grad_out, int32_t count, int32_t* count_out);
+/* Device memory (bytes) the last spruce_gpu_adjoint call allocated beyond the domain's own grids
+ * (no reference counterpart). */
+int spruce_gpu_adjoint_device_bytes(spruce_gpu_domain* d, int64_t* bytes);
 /* Reverse of populateVariablesFromState (equationset.cpp:96-104): maps dJ/d(evolved grids right
  * after setup) to dJ/d(.state grids rho, temp, mom_x, mom_y, mom_z, bi_x, bi_y, bi_z), 8*xdim*ydim. */
 int spruce_gpu_adjoint_state(spruce_gpu_domain* d, const double* grad_evolved, double* grad_state);

@@ -1,5 +1,10 @@
 """Gradient of a misfit of the final state through many IdealMHD steps, by the GPU discrete adjoint.
 
+``ArcadeHeatingProblem`` is the gradient workload with modules: the solar arcade with thermal
+conduction and radiative losses, heated by ``parametric_heating`` (heating(y) = sum_k theta_k
+phi_k(y)); the loss is the coronal temperature misfit and the gradient is taken with respect to the
+coefficients theta_k (``spruce_gpu_adjoint_params``).
+
 ``PeriodicWaveProblem`` is the adjoint workload on the periodic ideal-MHD wave: the loss is
 ``sum((rho_final - rho_target)^2)`` after ``nsteps`` steps, and its gradient is taken with respect to
 the initial temperature grid and the flow amplitude v0 (mom_x = -rho v0 sin(2 pi y/L),
@@ -72,3 +77,67 @@ class PeriodicWaveProblem:
         g_temp = g_state[1]
         g_v0 = float(np.sum(g_state[2] * self.sx) + np.sum(g_state[3] * self.sy))
         return g_temp, g_v0, sec
+
+
+class ArcadeHeatingProblem:
+    """Arcade (default 2048x2048, rows stretched by 1.0005, dipole depth 6.4e8 cm), ideal MHD with
+    thermal conduction and radiative losses (RK2, FP64), heated by parametric_heating with K
+    coefficients.  J = sum over coronal cells (initial T above 0.9 of the coronal temperature, below
+    the top eighth of the rows) of (T_final - T_initial)^2 after nsteps steps."""
+
+    def __init__(self, nx: int = 2048, ny: int = 2048, stretch: float = 1.0005, depth: float = 6.4e8,
+                 nsteps: int = 1000, checkpoint: int = 50, K: int = 16, theta_scale: float = 1e-4,
+                 bc=("periodic", "periodic", "fixed", "open")):
+        self.nx, self.ny, self.nsteps, self.checkpoint, self.K = nx, ny, nsteps, checkpoint, K
+        self.bc = tuple(bc)
+        self.base = W.arcade_state(nx, ny, stretch=stretch, depth=depth)
+        self.t0 = self.base.grids["temp"].copy()
+        # coronal cells below the top eighth of the domain: the cells next to the open top boundary
+        # cool to the temperature floor within tens of steps, where the misfit has no derivative
+        self.mask = (self.t0 > 0.9 * float(np.max(self.t0))) & (np.arange(ny)[None, :] < ny - ny // 8)
+        self.theta0 = theta_scale * (1.0 + 0.5 * np.sin(np.arange(K)))
+        self.m = self.base.ion_mass
+        self.gm1 = self.base.adiabatic_index - 1.0
+        self.tmin = float(W.arcade_config(steps=1).settings["temp_min"])
+
+    def spec(self, theta: np.ndarray):
+        cfg = W.arcade_config(steps=self.nsteps)
+        for k, v in zip(("x_bound_1", "x_bound_2", "y_bound_1", "y_bound_2"), self.bc):
+            cfg.settings[k] = v
+        cfg.modules.append(("parametric_heating", [("num_basis", str(self.K))] +
+                            [("coeff_%d" % k, repr(float(t))) for k, t in enumerate(theta)]))
+        return spec_from_files(cfg, self.base)
+
+    def forward(self, theta: np.ndarray):
+        """(loss, final (temp, n), seconds of the nsteps steps)."""
+        dom = GpuDomain(self.spec(theta))
+        try:
+            t0 = time.perf_counter()
+            dom.step(self.nsteps)
+            sec = time.perf_counter() - t0
+            T, n = dom.grid("temp"), dom.grid("n")
+        finally:
+            dom.close()
+        return float(np.sum(((T - self.t0) * self.mask) ** 2)), (T, n), sec
+
+    def seed(self, final) -> np.ndarray:
+        """dJ/d(evolved grids after the steps) from T = max((gamma-1) te / (2 K_B n), T_min), n = rho/m."""
+        T, n = final
+        dJ = 2.0 * (T - self.t0) * self.mask * (T > self.tmin)
+        seed = np.zeros((8, self.nx, self.ny))
+        seed[4] = dJ * (self.gm1 / (2.0 * W.K_B * n))
+        seed[0] = dJ * (-T / n) / self.m
+        return seed
+
+    def gradient(self, theta: np.ndarray, final):
+        """(dJ/dtheta, seconds of the adjoint call, device bytes the adjoint allocated)."""
+        dom = GpuDomain(self.spec(theta))
+        try:
+            t0 = time.perf_counter()
+            dom.adjoint(self.nsteps, self.checkpoint, self.seed(final))
+            sec = time.perf_counter() - t0
+            g = dom.adjoint_params()
+            mem = dom.adjoint_device_bytes()
+        finally:
+            dom.close()
+        return g, sec, mem

@@ -1,8 +1,9 @@
 // Discrete-adjoint kernels of the ghost fill, live propagate and modules for gfx950 (reverse of
 // mhd.hip's ghost fill, solar.hip's conduction and radiation kernels and the heating update).
-// Local Jacobians of the nonlinear per-cell functions are taken in forward mode (dual.hpp) from the
-// same code as the forward where the function is shared (tc_cell.hpp).  Contributions to
-// neighbouring cells use FP64 atomics, so results are reproducible to rounding.
+// Conduction is reversed by hand (tc_cell.hpp, checked against the same function in forward mode);
+// the ghost fill of open sides and the radiative-loss subcycles take their local Jacobians in
+// forward mode (dual.hpp).  Contributions to neighbouring cells use FP64 atomics, so results are
+// reproducible to rounding.
 #include <cfloat>
 
 #include "adj_modules.hpp"
@@ -159,8 +160,6 @@ __device__ __forceinline__ double temp_of(double te, double n, const MhdConst& K
   return smax(te * K.gm1 / (n * (2.0 * K_B)), K.tmin);
 }
 
-constexpr int TC_PASS = 3;   // derivative components per forward-mode pass (15 inputs, 5 passes)
-
 __global__ void __launch_bounds__(256) k_adj_tc_sub(const double* te_s, const double* n, const double* bhx,
                                                     const double* bhy, const double* dbxx, const double* dbxy,
                                                     const double* dbyx, const double* dbyy, double h, double k0,
@@ -175,26 +174,18 @@ __global__ void __launch_bounds__(256) k_adj_tc_sub(const double* te_s, const do
       lam_te[c] *= (base + 0.0 < K.temin) ? 0.0 : 1.0;
     } else {
       size_t q[3][3];
-      double x[15];
+      double T[3][3];
       for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) {
           q[a][b] = (size_t)wrp(i - 1 + a, M.nx, M.x.per) * M.ny + wrp(j - 1 + b, M.ny, M.y.per);
-          x[3 * a + b] = temp_of(te_s[q[a][b]], n[q[a][b]], K);
+          T[a][b] = temp_of(te_s[q[a][b]], n[q[a][b]], K);
         }
-      x[9] = bhx[c]; x[10] = bhy[c];
-      x[11] = dbxx[c]; x[12] = dbxy[c]; x[13] = dbyx[c]; x[14] = dbyy[c];
-      double J[15];
-      double dE = 0.0;
-      for (int p = 0; p < 15 / TC_PASS; ++p) {
-        Dn<TC_PASS> xs[15];
-        for (int k = 0; k < 15; ++k)
-          xs[k] = (k >= p * TC_PASS && k < (p + 1) * TC_PASS) ? dn_var<TC_PASS>(x[k], k - p * TC_PASS)
-                                                              : dn_const<TC_PASS>(x[k]);
-        const Dn<TC_PASS> T[3][3] = {{xs[0], xs[1], xs[2]}, {xs[3], xs[4], xs[5]}, {xs[6], xs[7], xs[8]}};
-        const Dn<TC_PASS> r = tcc::tc_core(M, i, j, T, xs[9], xs[10], xs[11], xs[12], xs[13], xs[14], k0);
-        dE = r.v;
-        for (int k = 0; k < TC_PASS; ++k) J[p * TC_PASS + k] = r.d[k];
-      }
+      // the clamp of the update needs dE first; its reverse is linear in the seed, so it runs with
+      // a unit seed and is scaled afterwards
+      double gT[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+      double gfz[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      const double dE = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0,
+                                             1.0, gT, gfz);
       const double v = base + 1.0 * (1.0 * h) * dE;
       const double lv = (v < K.temin) ? 0.0 : lam_te[c];
       lam_te[c] = lv;
@@ -202,8 +193,8 @@ __global__ void __launch_bounds__(256) k_adj_tc_sub(const double* te_s, const do
       const double ld = lv * h;
       if (ld != 0.0) {
         for (int a = 0; a < 3; ++a)
-          for (int b = 0; b < 3; ++b) atomicAdd(&lam_T[q[a][b]], ld * J[3 * a + b]);
-        for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += ld * J[9 + k];
+          for (int b = 0; b < 3; ++b) atomicAdd(&lam_T[q[a][b]], ld * gT[a][b]);
+        for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += ld * gfz[k];
       }
     }
   }

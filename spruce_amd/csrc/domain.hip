@@ -592,6 +592,10 @@ class Domain : public DomainBase {
   }
   // dJ/dtheta of the parametrised heating modules after the last adjoint call
   std::vector<double> param_grad() override;
+  // device memory the last adjoint call allocated for itself (checkpoints, segment, replay
+  // snapshots, adjoint vectors, conduction subcycle record)
+  int64_t adj_bytes = 0;
+  int64_t adjoint_bytes() const override { return adj_bytes; }
 
   // reverse of populateVariablesFromState for the .state grids kept from set-up
   void adjoint_state(const double* grad_evolved, double* grad_state) override {
@@ -1690,6 +1694,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
                        modscratch;
   DBuf slab;
   slab.alloc(total);
+  adj_bytes = (int64_t)(total * sizeof(double));
   double* cur = slab.p;
   auto take = [&](size_t n) { double* p = cur; cur += n; return p; };
   std::vector<MhdSet> cps(nseg), seg(sl + 1), mid;
@@ -1759,6 +1764,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
     keep.alloc(SET);
     kept = set_at(keep.p, N);
     copy_set(L.view(), kept, N, stream);
+    adj_bytes += (int64_t)(SET * sizeof(double));
   }
   for (auto& m : mods)
     if (auto* ph = dynamic_cast<ParamHeatModule*>(m.get())) launch_fill(ph->g_theta.p, 0.0, ph->K, stream);
@@ -1891,6 +1897,8 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   if (with_mods) {                                      // the state after the steps and its step size
     copy_set(kept, L.view(), N, stream);
     finish_step(-1);
+    for (auto& m : mods)                                  // conduction's subcycle record
+      if (auto* tc = dynamic_cast<TcModule*>(m.get())) adj_bytes += (int64_t)(tc->tape.size() * N * sizeof(double));
   }
   hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
   tstep = time;
@@ -2233,6 +2241,13 @@ int spruce_gpu_adjoint(spruce_gpu_domain* h, int32_t nsteps, int32_t checkpoint_
 int spruce_gpu_adjoint_state(spruce_gpu_domain* h, const double* grad_evolved, double* grad_state) {
   API_BEGIN
   dom(h).adjoint_state(grad_evolved, grad_state);
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_adjoint_device_bytes(spruce_gpu_domain* h, int64_t* bytes) {
+  API_BEGIN
+  require(bytes != nullptr, SPRUCE_E_ARG, "null argument");
+  *bytes = dom(h).adjoint_bytes();
   API_END(SPRUCE_OK)
 }
 

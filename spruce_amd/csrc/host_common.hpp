@@ -168,6 +168,7 @@ class DomainBase {
   virtual std::vector<double> param_grad() {
     throw Error(SPRUCE_E_UNSUPPORTED, "the adjoint is available for ideal_mhd only");
   }
+  virtual int64_t adjoint_bytes() const { return 0; }
 
   double time = 0.0;
   int iter = 0;

@@ -86,5 +86,135 @@ __host__ __device__ inline S tc_core(const Mesh& M, int i, int j, const S T[3][3
   return ((q25 * bdg * divb) + (bx * ttx + by * tty)) * k0;
 }
 
+// Reverse mode of tc_core<double> at interior cell (i, j): returns the value and adds ld * dR/dT to
+// gT[3][3] and ld * dR/d(b_hat_x, b_hat_y, d b_hat_x/dx, /dy, d b_hat_y/dx, /dy) to gfz[6].  The
+// partial derivatives are those of the exact function (interpolation weights and divisions taken
+// in exact arithmetic); the value is tc_core's.
+__host__ __device__ inline double tc_core_reverse(const Mesh& M, int i, int j, const double T[3][3], double bx,
+                                                  double by, double dbx_x, double dbx_y, double dby_x, double dby_y,
+                                                  double k0, double ld, double gT[3][3], double gfz[6]) {
+  const int im = wrp(i - 1, M.nx, M.x.per), ip = wrp(i + 1, M.nx, M.x.per);
+  const int jm = wrp(j - 1, M.ny, M.y.per), jp = wrp(j + 1, M.ny, M.y.per);
+  const Axis& X = M.x;
+  const Axis& Y = M.y;
+  const double dx = X.d[i], rdx = X.rd[i], dy = Y.d[j], rdy = Y.rd[j];
+  // interpolation weights: upper face (c, c+1) and lower face (c-1, c) on each axis
+  const double xu_c = X.h[ip] / X.fden[ip], xu_p = X.h[i] / X.fden[ip];
+  const double xl_m = X.h[i] / X.fden[i], xl_c = X.h[im] / X.fden[i];
+  const double yu_c = Y.h[jp] / Y.fden[jp], yu_p = Y.h[j] / Y.fden[jp];
+  const double yl_m = Y.h[j] / Y.fden[j], yl_c = Y.h[jm] / Y.fden[j];
+  // forward, as tc_core
+  const double Tc = T[1][1];
+  const double Txu = fint(X, ip, i, Tc, T[2][1]), Txl = fint(X, i, im, T[0][1], Tc);
+  const double Tyu = fint(Y, jp, j, Tc, T[1][2]), Tyl = fint(Y, j, jm, T[1][0], Tc);
+  const double gx = qdiv(Txu - Txl, dx, rdx), gy = qdiv(Tyu - Tyl, dy, rdy);
+  const bool in_jm = inside(M, i, jm), in_jp = inside(M, i, jp), in_im = inside(M, im, j), in_ip = inside(M, ip, j);
+  auto gxc = [&](int b1) {
+    return qdiv(fint(X, ip, i, T[1][b1], T[2][b1]) - fint(X, i, im, T[0][b1], T[1][b1]), dx, rdx);
+  };
+  auto gyr = [&](int a1) {
+    return qdiv(fint(Y, jp, j, T[a1][1], T[a1][2]) - fint(Y, j, jm, T[a1][0], T[a1][1]), dy, rdy);
+  };
+  const double gxm = in_jm ? gxc(0) : 0.0, gxp = in_jp ? gxc(2) : 0.0;
+  const double gym = in_im ? gyr(0) : 0.0, gyp = in_ip ? gyr(2) : 0.0;
+  const double dgx_y = qdiv(fint(Y, jp, j, gx, gxp) - fint(Y, j, jm, gxm, gx), dy, rdy);
+  const double dgy_x = qdiv(fint(X, ip, i, gy, gyp) - fint(X, i, im, gym, gy), dx, rdx);
+  const double s0 = qdiv(Txu - 2.0 * Tc + Txl, X.hsq[i], X.rhsq[i]);
+  const double s1 = qdiv(Tyu - 2.0 * Tc + Tyl, Y.hsq[j], Y.rhsq[j]);
+  const double bdg = bx * gx + by * gy;
+  const double t1x = bx * s0 + by * dgx_y, t1y = bx * dgy_x + by * s1;
+  const double t2x = gx * dbx_x + gy * dbx_y, t2y = gx * dby_x + gy * dby_y;
+  const double curl = dby_x - dbx_y;
+  const double t3x = (-(-1.0 * curl)) * gy, t3y = (-1.0 * curl) * gx;
+  const double q15 = pow_1p5(Tc), q25 = pow_2p5(Tc);
+  const double ttx = 5.0 / 2.0 * q15 * gx * bdg + q25 * (t1x + t2x + t3x);
+  const double tty = 5.0 / 2.0 * q15 * gy * bdg + q25 * (t1y + t2y + t3y);
+  const double divb = dbx_x + dby_y;
+  const double R = ((q25 * bdg * divb) + (bx * ttx + by * tty)) * k0;
+  // reverse
+  const double gR = ld * k0;
+  double g_q25 = gR * bdg * divb, g_bdg = gR * q25 * divb;
+  const double g_divb = gR * q25 * bdg;
+  double g_bx = gR * ttx, g_by = gR * tty;
+  const double g_ttx = gR * bx, g_tty = gR * by;
+  double g_q15 = g_ttx * 2.5 * gx * bdg + g_tty * 2.5 * gy * bdg;
+  double g_gx = g_ttx * 2.5 * q15 * bdg, g_gy = g_tty * 2.5 * q15 * bdg;
+  g_bdg += g_ttx * 2.5 * q15 * gx + g_tty * 2.5 * q15 * gy;
+  g_q25 += g_ttx * (t1x + t2x + t3x) + g_tty * (t1y + t2y + t3y);
+  const double gSx = g_ttx * q25, gSy = g_tty * q25;
+  double g_dbx_x = g_divb, g_dby_y = g_divb, g_dbx_y = 0.0, g_dby_x = 0.0;
+  // t3x = curl gy, t3y = -curl gx; curl = dby_x - dbx_y
+  const double g_curl = gSx * gy - gSy * gx;
+  g_gy += gSx * curl;
+  g_gx -= gSy * curl;
+  g_dby_x += g_curl;
+  g_dbx_y -= g_curl;
+  // t2
+  g_gx += gSx * dbx_x + gSy * dby_x;
+  g_gy += gSx * dbx_y + gSy * dby_y;
+  g_dbx_x += gSx * gx;
+  g_dbx_y += gSx * gy;
+  g_dby_x += gSy * gx;
+  g_dby_y += gSy * gy;
+  // t1
+  g_bx += gSx * s0 + gSy * dgy_x;
+  g_by += gSx * dgx_y + gSy * s1;
+  const double g_s0 = gSx * bx, g_dgx_y = gSx * by, g_dgy_x = gSy * bx, g_s1 = gSy * by;
+  // bdg
+  g_bx += g_bdg * gx;
+  g_by += g_bdg * gy;
+  g_gx += g_bdg * bx;
+  g_gy += g_bdg * by;
+  // powers of Tc
+  double g_Tc = g_q15 * 1.5 * sqrt(Tc) + g_q25 * 2.5 * q15;
+  // second derivatives
+  double g_Txu = g_s0 / X.hsq[i], g_Txl = g_s0 / X.hsq[i];
+  g_Tc -= 2.0 * g_s0 / X.hsq[i];
+  double g_Tyu = g_s1 / Y.hsq[j], g_Tyl = g_s1 / Y.hsq[j];
+  g_Tc -= 2.0 * g_s1 / Y.hsq[j];
+  // nested derivatives
+  const double ry = g_dgx_y / dy;
+  g_gx += ry * (yu_c - yl_c);
+  const double g_gxp = ry * yu_p, g_gxm = -ry * yl_m;
+  const double rx = g_dgy_x / dx;
+  g_gy += rx * (xu_c - xl_c);
+  const double g_gyp = rx * xu_p, g_gym = -rx * xl_m;
+  auto rev_gxc = [&](int b1, double g) {   // gx of column b1 (cells (i-1..i+1, j-1+b1))
+    const double r = g / dx;
+    gT[1][b1] += r * (xu_c - xl_c);
+    gT[2][b1] += r * xu_p;
+    gT[0][b1] -= r * xl_m;
+  };
+  auto rev_gyr = [&](int a1, double g) {   // gy of row a1 (cells (i-1+a1, j-1..j+1))
+    const double r = g / dy;
+    gT[a1][1] += r * (yu_c - yl_c);
+    gT[a1][2] += r * yu_p;
+    gT[a1][0] -= r * yl_m;
+  };
+  if (in_jm) rev_gxc(0, g_gxm);
+  if (in_jp) rev_gxc(2, g_gxp);
+  if (in_im) rev_gyr(0, g_gym);
+  if (in_ip) rev_gyr(2, g_gyp);
+  // first derivatives at the cell
+  g_Txu += g_gx / dx;
+  g_Txl -= g_gx / dx;
+  g_Tyu += g_gy / dy;
+  g_Tyl -= g_gy / dy;
+  // face interpolants
+  g_Tc += g_Txu * xu_c + g_Txl * xl_c + g_Tyu * yu_c + g_Tyl * yl_c;
+  gT[2][1] += g_Txu * xu_p;
+  gT[0][1] += g_Txl * xl_m;
+  gT[1][2] += g_Tyu * yu_p;
+  gT[1][0] += g_Tyl * yl_m;
+  gT[1][1] += g_Tc;
+  gfz[0] += g_bx;
+  gfz[1] += g_by;
+  gfz[2] += g_dbx_x;
+  gfz[3] += g_dbx_y;
+  gfz[4] += g_dby_x;
+  gfz[5] += g_dby_y;
+  return R;
+}
+
 }  // namespace tcc
 }  // namespace spruce

@@ -81,6 +81,7 @@ def lib() -> ctypes.CDLL:
     L.spruce_gpu_adjoint.argtypes = [P, I32, I32, DP, DP, DP]
     L.spruce_gpu_adjoint_state.argtypes = [P, DP, DP]
     L.spruce_gpu_adjoint_params.argtypes = [P, DP, I32, ctypes.POINTER(I32)]
+    L.spruce_gpu_adjoint_device_bytes.argtypes = [P, ctypes.POINTER(ctypes.c_int64)]
     L.spruce_gpu_batch_set_profiling.argtypes = [P, I32]
     L.spruce_gpu_batch_kernel_times.argtypes = [P, ctypes.c_char_p, I32, DP, ctypes.POINTER(I32), I32]
     _LIB = L
@@ -222,6 +223,12 @@ class GpuDomain:
         if n.value:
             _chk(lib().spruce_gpu_adjoint_params(self.h, _dp(out), n.value, None))
         return out
+
+    def adjoint_device_bytes(self) -> int:
+        """Device memory the last adjoint call allocated beyond the domain's own grids."""
+        n = ctypes.c_int64(0)
+        _chk(lib().spruce_gpu_adjoint_device_bytes(self.h, ctypes.byref(n)))
+        return int(n.value)
 
     def adjoint_state(self, grad_evolved: np.ndarray) -> np.ndarray:
         """dJ/d(.state grids rho, temp, mom_x, mom_y, mom_z, bi_x, bi_y, bi_z) from dJ/d(evolved grids

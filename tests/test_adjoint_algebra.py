@@ -32,3 +32,19 @@ def test_adjoint_algebra_matches_finite_differences(checker, field):
     assert r.returncode == 0, r.stdout
     worst = float(r.stdout.strip().splitlines()[-1].split()[1])
     assert worst < 1e-6
+
+
+def test_conduction_reverse_matches_forward_mode_and_differences(tmp_path):
+    """The hand-written reverse of the conduction energy derivative (tc_cell.hpp) against the same
+    function evaluated in forward mode (all 15 partials at once) and against central differences,
+    at every interior cell of periodic and ghost-bounded non-uniform meshes."""
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not available")
+    out = str(tmp_path / "tc_reverse_check")
+    cmd = [HIPCC, "-O1", "-std=c++17", "-ffp-contract=off", "--offload-arch=gfx950",
+           "-I" + os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "tc_reverse_check.cpp"), "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    r = subprocess.run([out], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout
