@@ -1,6 +1,8 @@
 // CPU FP64 restatement of the SPRUCE time step -- TEST INFRASTRUCTURE ONLY (the checker of the
 // GPU path; see spruce_oracle.h).  Parity pins: tests/golden/*.npz, produced by the compiled
-// reference, must be reproduced bit-for-bit (tests/test_oracle_golden.py).
+// reference, must be reproduced bit-for-bit (tests/test_oracle_golden.py).  ParametricHeating
+// (spruce_oracle_modules.inc) has no reference counterpart: its restatement is its specification,
+// so it has no reference pin.
 //
 // Every whole-array operation below is one separately rounded elementwise pass, evaluated left to
 // right in the reference's expression order (Grid operators, source/mhd/grid.cpp:528-582); no
