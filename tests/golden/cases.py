@@ -252,6 +252,31 @@ def _cases() -> Dict[str, Tuple[Callable[[], StateFile], Callable[[], ConfigFile
     c["divclean_rk2"] = (lambda: random_mhd_state(22, 20, 69, stretch=True),
                          lambda: _mhd(("periodic", "periodic", "reflect", "fixed"), "rk2", 5, modules=[
                              ("div_cleaning", [("epsilon", "0.1"), ("time_scale", "0.01")])]))
+    # anomalous resistivity (anomalousresistivity.cpp): flood-fill templates on non-periodic x (the
+    # localized argmin box can leave a periodic grid), masks with xdim <= ydim (the masks loop
+    # columns over the row count), frobenius templates; the three resistivity models
+    c["ar_flood_time_euler"] = (lambda: random_mhd_state(24, 24, 71),
+                                lambda: _mhd(("reflect", "open", "fixed", "fixed"), "euler", 4, modules=[
+                                    ("anomalous_resistivity", [
+                                        ("resistivity_model", "time_scale"), ("time_scale", "1e-2"),
+                                        ("template_mode", "flood_fill"), ("flood_fill_threshold", "1.9"),
+                                        ("smoothing_sigma", "1.0"), ("flood_fill_argmin_radius", "6e7")])]))
+    c["ar_flood_masks_rk2"] = (lambda: random_mhd_state(22, 26, 81),
+                               lambda: _mhd(("open", "reflect", "fixed", "open"), "rk2", 4, modules=[
+                                   ("anomalous_resistivity", [
+                                       ("resistivity_model", "syntelis_19"),
+                                       ("resistivity_model_params", "1e13,5e13,150.0"),
+                                       ("template_mode", "flood_fill"), ("flood_fill_threshold", "2.2"),
+                                       ("flood_fill_max_radius", "8e7"), ("flood_fill_min_current", "100.0"),
+                                       ("flood_fill_current_ramp_length", "200.0"), ("smoothing_sigma", "0.8"),
+                                       ("gradient_correction", "true"), ("time_integrator", "rk2")])]))
+    c["ar_frob_ys_rk4"] = (lambda: random_mhd_state(20, 20, 73),
+                           lambda: _mhd(P4, "rk4", 4, modules=[
+                               ("anomalous_resistivity", [
+                                   ("resistivity_model", "ys_94"), ("resistivity_model_params", "0.5,3e11,2e12"),
+                                   ("template_mode", "frobenius"), ("frobenius_metric_coeff", "1e41"),
+                                   ("smoothing_sigma", "0.6"), ("time_integrator", "rk4"),
+                                   ("safety_factor", "0.5")])]))
     # single-fluid variants: two energies, and the conservative total-energy form
     E2 = ["rho", "mom_x", "mom_y", "i_thermal_energy", "e_thermal_energy", "bi_x", "bi_y", "i_temp", "e_temp",
           "press", "b_hat_x", "dt"]
