@@ -13,6 +13,7 @@
 #include <map>
 #include <memory>
 #include <stdexcept>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -1613,6 +1614,254 @@ struct ParamHeatModule : PostSourceModule {
   }
 };
 
+// AnomalousResistivity (anomalousresistivity.cpp)
+struct ResistModule : GpuModule {
+  std::string integ = "euler", tmode = "flood_fill", model = "time_scale";
+  double safety = 1.0, frob_coeff = 1.0e50, ff_threshold = 1.0, ff_ramp = 1.0e-5, ff_argmin_radius = 5.0e9;
+  double sigma = 3.0, ts = -1.0, ff_max_radius = -1.0, ff_min_current = -1.0;
+  bool smoothing = true, grad_corr = false;
+  std::vector<double> params;
+  int null_i = 0, null_j = 0, kr = 0;
+  int nx = 0, ny = 0;
+  std::vector<double> px, py;                           // host positions (the argmin box)
+  DBuf dpx, dpy, kern, J, bx, by, bm, ratio, tmpl, tmp, diff, coeff, td, lap[3], work[4], d1[4], d2[4], d3[4], d4[4],
+      mid[3], part, scal, argout;
+  int* changed = nullptr;
+  static constexpr double CC = C_LIGHT / (4.0 * PI);
+  explicit ResistModule(const Opts& o) {  // :51-77
+    name = "anomalous_resistivity";
+    for (auto& kv : o) {
+      const std::string& a = kv.first;
+      const std::string& v = kv.second;
+      if (a == "time_scale") ts = std::stod(v);
+      else if (a == "output_to_file") require(v != "true", SPRUCE_E_UNSUPPORTED, "anomalous_resistivity file output is not available in this build");
+      else if (a == "frobenius_metric_coeff") frob_coeff = std::stod(v);
+      else if (a == "smoothing_sigma") sigma = std::stod(v);
+      else if (a == "safety_factor") safety = std::stod(v);
+      else if (a == "metric_smoothing") smoothing = v == "true";
+      else if (a == "time_integrator") integ = v;
+      else if (a == "template_mode") tmode = v;
+      else if (a == "flood_fill_max_radius") ff_max_radius = std::stod(v);
+      else if (a == "flood_fill_argmin_radius") ff_argmin_radius = std::stod(v);
+      else if (a == "flood_fill_min_current") ff_min_current = std::stod(v);
+      else if (a == "flood_fill_current_ramp_length") ff_ramp = std::stod(v);
+      else if (a == "resistivity_model") model = v;
+      else if (a == "gradient_correction") grad_corr = v == "true";
+      else if (a == "resistivity_model_params") {
+        params.clear();
+        std::stringstream ss(v);
+        std::string t;
+        while (std::getline(ss, t, ',')) params.push_back(std::stod(t));
+      } else if (a == "flood_fill_threshold") ff_threshold = std::stod(v);
+    }
+    if (integ.empty()) integ = "euler";
+  }
+  ~ResistModule() override { if (changed) (void)hipFree(changed); }
+  bool needs_pre_scalars() const override { return false; }
+  void pre_launch(Domain&, double*) override {}
+  void pre_finish(double, const double*) override {}
+
+  ArParams ar_params() const {
+    ArParams p{};
+    p.model = model == "time_scale" ? AR_TIME_SCALE : (model == "syntelis_19" ? AR_SYNTELIS : AR_YS94);
+    p.ts = ts;
+    if (p.model != AR_TIME_SCALE) { p.p0 = params[0]; p.p1 = params[1]; p.p2 = params[2]; }
+    return p;
+  }
+  // argminLocalized (:202-225): box on the host from the positions, minimum on the device
+  void argmin_local(Domain& d) {
+    const int ci = null_i, cj = null_j;
+    auto X = [&](int i, int j) { return px[(size_t)i * ny + j]; };
+    auto Y = [&](int i, int j) { return py[(size_t)i * ny + j]; };
+    const double r = ff_argmin_radius;
+    int il = ci, iu = ci, jl = cj, ju = cj;
+    for (; il >= d.xl; il--) if (std::abs(X(ci, cj) - X(il, cj)) > r) break;
+    for (; iu <= d.xu; iu++) if (std::abs(X(ci, cj) - X(iu, cj)) > r) break;
+    for (; jl >= d.yl; jl--) if (std::abs(Y(ci, cj) - Y(ci, jl)) > r) break;
+    for (; ju <= d.yu; ju++) if (std::abs(Y(ci, cj) - Y(ci, ju)) > r) break;
+    require(il >= 0 && jl >= 0 && iu < nx && ju < ny, SPRUCE_E_UNSUPPORTED,
+            "anomalous_resistivity: the localized argmin box leaves the grid (undefined in the reference)");
+    launch_ar_argmin_box(bm.p, dpx.p, dpy.p, ci, cj, r, il, iu, jl, ju, ny, argout.p, d.stream);
+    double o[3];
+    hip_check(hipMemcpyAsync(o, argout.p, sizeof o, hipMemcpyDeviceToHost, d.stream), "read argmin");
+    hip_check(hipStreamSynchronize(d.stream), "hipStreamSynchronize");
+    require(std::min(o[0], o[2]) < DBL_MAX, SPRUCE_E_NUMERIC, "Something went wrong in the localized argmin calculation");
+    if (o[1] >= 0.0 && o[0] < o[2]) {
+      const long long c = (long long)o[1];
+      null_i = (int)(c / ny);
+      null_j = (int)(c % ny);
+    }
+  }
+  // computeTemplate (:158-199) for the field be + bi and the current density Jc
+  void compute_template(Domain& d, const double* bix, const double* biy, const double* Jc) {
+    launch_ar_field(d.bex.p, d.bey.p, bix, biy, d.N, bx.p, by.p, bm.p, d.stream);
+    if (tmode == "frobenius") {
+      launch_ar_frob(bx.p, by.p, bm.p, frob_coeff, d.mesh, ratio.p, tmpl.p, d.stream);
+    } else {
+      argmin_local(d);
+      const size_t start = (size_t)null_i * ny + null_j;
+      launch_ar_flood_seed(bm.p, d.N, start, ff_threshold, tmpl.p, d.stream);
+      for (;;) {                                        // sweeps until the region stops growing
+        hip_check(hipMemsetAsync(changed, 0, sizeof(int), d.stream), "hipMemsetAsync");
+        for (int k = 0; k < 8; ++k) launch_ar_flood_sweep(bm.p, ff_threshold, d.mesh, tmpl.p, changed, d.stream);
+        int c = 0;
+        hip_check(hipMemcpyAsync(&c, changed, sizeof(int), hipMemcpyDeviceToHost, d.stream), "read flood flag");
+        hip_check(hipStreamSynchronize(d.stream), "hipStreamSynchronize");
+        if (!c) break;
+      }
+      if (ff_max_radius > 0.0 || ff_min_current > 0.0)
+        launch_ar_masks(tmpl.p, Jc, dpx.p, dpy.p, start, ff_max_radius, ff_min_current, ff_ramp, d.mesh, d.stream);
+    }
+    if (smoothing) {
+      launch_ar_smooth(tmpl.p, kern.p, kr, d.mesh, tmp.p, d.stream);
+      tmpl.swap(tmp);
+    }
+    if (tmode == "frobenius") launch_ar_frob_final(tmpl.p, d.N, d.stream);
+  }
+  void compute_diffusion(Domain& d, const double* bix, const double* biy, const double* biz) {  // :124-156
+    if (model != "time_scale") launch_ar_current(bix, biy, biz, CC, d.mesh, J.p, d.stream);
+    launch_ar_diffusion(J.p, d.L.n.p, ar_params(), d.mesh, diff.p, d.stream);
+  }
+  // computeTimeDerivatives (:79-102) into out[4]
+  void derivs(Domain& d, const double* bix, const double* biy, const double* biz, DBuf* out) {
+    launch_ar_current(bix, biy, biz, CC, d.mesh, J.p, d.stream);
+    compute_template(d, bix, biy, J.p);
+    compute_diffusion(d, bix, biy, biz);                  // the same field: J is recomputed unchanged
+    launch_ar_coeff(tmpl.p, diff.p, d.mesh, coeff.p, td.p, d.stream);
+    ArRhsArgs a{};
+    a.bix = bix; a.biy = biy; a.biz = biz; a.coeff = coeff.p; a.J = J.p;
+    a.lbx = lap[0].p; a.lby = lap[1].p; a.lbz = lap[2].p;
+    a.d0 = out[0].p; a.d1 = out[1].p; a.d2 = out[2].p; a.d3 = out[3].p;
+    a.eta_factor = 4.0 * PI / C_LIGHT / C_LIGHT;
+    a.grad_corr = grad_corr ? 1 : 0;
+    a.mesh = d.mesh;
+    launch_ar_rhs(a, d.stream);
+  }
+  void setup(Domain& d) override {  // :22-49
+    require(model != "time_scale" || ts > 0.0, SPRUCE_E_ARG,
+            "Anomalous Resistivity time_scale must be specified, and positive, if run in time_scale mode");
+    require(integ == "euler" || integ == "rk2" || integ == "rk4", SPRUCE_E_ARG,
+            "Invalid time integrator given for Anomalous Resistivity module");
+    require(tmode == "frobenius" || tmode == "flood_fill", SPRUCE_E_ARG, "Invalid template mode given for Anomalous Resistivity module");
+    require(model == "time_scale" || model == "syntelis_19" || model == "ys_94", SPRUCE_E_ARG,
+            "Invalid resistivity model given for Anomalous Resistivity module");
+    require(model == "time_scale" || params.size() == 3, SPRUCE_E_ARG, "the resistivity model requires three parameters");
+    nx = d.nx;
+    ny = d.ny;
+    if (tmode == "flood_fill" && (ff_max_radius > 0.0 || ff_min_current > 0.0))
+      require(nx <= ny, SPRUCE_E_UNSUPPORTED,
+              "anomalous_resistivity masks need xdim <= ydim (the reference loops columns over the row count)");
+    px = d.host_grids.at("pos_x");
+    py = d.host_grids.at("pos_y");
+    upload_vec(dpx, px);
+    upload_vec(dpy, py);
+    for (DBuf* b : {&J, &bx, &by, &bm, &ratio, &tmpl, &tmp, &diff, &coeff, &td}) b->alloc(d.N);
+    for (int k = 0; k < 3; ++k) { lap[k].alloc(d.N); mid[k].alloc(d.N); }
+    for (int k = 0; k < 4; ++k) {
+      work[k].alloc(d.N); d1[k].alloc(d.N);
+      if (integ != "euler") d2[k].alloc(d.N);
+      if (integ == "rk4") { d3[k].alloc(d.N); d4[k].alloc(d.N); }
+    }
+    part.alloc(2 * (size_t)ar_scale_blocks(d.mesh));
+    scal.alloc(4);
+    argout.alloc(3);
+    hip_check(hipMalloc(&changed, sizeof(int)), "hipMalloc");
+    launch_ar_laplacian(d.bex.p, d.mesh, lap[0].p, d.stream);
+    launch_ar_laplacian(d.bey.p, d.mesh, lap[1].p, d.stream);
+    launch_ar_laplacian(d.bez.p, d.mesh, lap[2].p, d.stream);
+    // null_location = argmin of |b| in the plane over the interior (Grid::argmin, grid.cpp:86-100)
+    launch_ar_field(d.bex.p, d.bey.p, d.L.bx.p, d.L.by.p, d.N, bx.p, by.p, bm.p, d.stream);
+    std::vector<double> hb(d.N);
+    hip_check(hipMemcpyAsync(hb.data(), bm.p, d.N * 8, hipMemcpyDeviceToHost, d.stream), "read |b|");
+    hip_check(hipStreamSynchronize(d.stream), "hipStreamSynchronize");
+    int bi = (int)(0.5 * nx), bj = (int)(0.5 * ny);
+    double m = hb[(size_t)bi * ny + bj];
+    for (int i = d.xl; i <= d.xu; ++i)
+      for (int j = d.yl; j <= d.yu; ++j)
+        if (hb[(size_t)i * ny + j] < m) { m = hb[(size_t)i * ny + j]; bi = i; bj = j; }
+    require(m < DBL_MAX, SPRUCE_E_NUMERIC, "Something went wrong in the argmin calculation");
+    null_i = bi;
+    null_j = bj;
+    if (smoothing) {
+      require(sigma > 0.0, SPRUCE_E_ARG, "smoothing_sigma must be a positive number");
+      kr = (int)std::nearbyint(4.0 * sigma);
+      const int ks = 2 * kr + 1;
+      upload_vec(kern, gaussian_grid(ks, ks, 0.0, 1.0 / (2.0 * PI * sigma * sigma), sigma, sigma, (double)kr, (double)kr));
+    }
+    launch_ar_current(d.L.bx.p, d.L.by.p, d.L.bz.p, CC, d.mesh, J.p, d.stream);
+    compute_template(d, d.L.bx.p, d.L.by.p, J.p);
+    hip_check(hipStreamSynchronize(d.stream), "hipStreamSynchronize");
+  }
+  void iterate(Domain& d, double dt) override {  // :104-122, 170-220
+    const size_t bytes = d.N * 8;
+    double* L4[4] = {d.L.bx.p, d.L.by.p, d.L.bz.p, d.L.te.p};
+    for (int k = 0; k < 4; ++k)
+      hip_check(hipMemcpyAsync(work[k].p, L4[k], bytes, hipMemcpyDeviceToDevice, d.stream), "copy field");
+    d.timed("resist", [&] {
+      launch_ar_current(work[0].p, work[1].p, work[2].p, CC, d.mesh, J.p, d.stream);
+      compute_template(d, work[0].p, work[1].p, J.p);
+      compute_diffusion(d, work[0].p, work[0].p, work[2].p);   // the reference passes bi_x for bi_y (:107)
+      launch_ar_coeff(tmpl.p, diff.p, d.mesh, coeff.p, td.p, d.stream);
+      // computeNumSubcycles (:105-122)
+      launch_ar_scales(td.p, d.mesh, d.xl, d.xu, d.yl, d.yu, part.p, scal.p, d.stream);
+      launch_mhd_dt(d.L.view(), d.stat(), d.kc, d.mesh, nullptr, d.partial.p, d.stream);
+      launch_reduce_step(d.partial.p, d.dt_blocks, d.kc.eps, scal.p + 2, nullptr, 0, d.err_flag, d.stream);
+    });
+    double sc[3];
+    hip_check(hipMemcpyAsync(sc, scal.p, sizeof sc, hipMemcpyDeviceToHost, d.stream), "read scales");
+    hip_check(hipStreamSynchronize(d.stream), "hipStreamSynchronize");
+    if (!(sc[0] > 0.0)) {
+      nsub = 1;
+    } else {
+      if (model != "time_scale") {
+        require(sc[1] < DBL_MAX, SPRUCE_E_NUMERIC, "Something went wrong in the min calculation (anomalous resistivity)");
+        ts = sc[1];
+      }
+      const double q = sc[2] / (safety * ts);
+      require(q < 1.0e8, SPRUCE_E_NUMERIC, "subcycle count out of range");
+      nsub = (int)(1.0 + q);
+    }
+    const double h = dt / ((double)nsub);
+    auto axpy = [&](DBuf* base, double sfac, DBuf* k1, DBuf* out, int nf) {
+      ArAxpyArgs a{};
+      for (int q = 0; q < nf; ++q) { a.base[q] = base[q].p; a.k1[q] = k1[q].p; a.out[q] = out[q].p; }
+      a.s = sfac; a.nf = nf; a.rk4 = 0; a.n = d.N;
+      launch_ar_axpy(a, d.stream);
+    };
+    d.timed("resist", [&] {
+      for (int s = 0; s < nsub; ++s) {
+        if (integ == "euler") {
+          derivs(d, work[0].p, work[1].p, work[2].p, d1);
+          axpy(work, h, d1, work, 4);
+        } else if (integ == "rk2") {
+          derivs(d, work[0].p, work[1].p, work[2].p, d1);
+          axpy(work, 0.5 * h, d1, mid, 3);
+          derivs(d, mid[0].p, mid[1].p, mid[2].p, d2);
+          axpy(work, h, d2, work, 4);
+        } else {
+          derivs(d, work[0].p, work[1].p, work[2].p, d1);
+          axpy(work, 0.5 * h, d1, mid, 3);
+          derivs(d, mid[0].p, mid[1].p, mid[2].p, d2);
+          axpy(work, 0.5 * h, d2, mid, 3);
+          derivs(d, mid[0].p, mid[1].p, mid[2].p, d3);
+          axpy(work, h, d3, mid, 3);
+          derivs(d, mid[0].p, mid[1].p, mid[2].p, d4);
+          ArAxpyArgs a{};
+          for (int q = 0; q < 4; ++q) {
+            a.base[q] = work[q].p; a.k1[q] = d1[q].p; a.k2[q] = d2[q].p; a.k3[q] = d3[q].p; a.k4[q] = d4[q].p;
+            a.out[q] = work[q].p;
+          }
+          a.s = h; a.nf = 4; a.rk4 = 1; a.n = d.N;
+          launch_ar_axpy(a, d.stream);
+        }
+      }
+    });
+    for (int k = 0; k < 4; ++k)
+      hip_check(hipMemcpyAsync(L4[k], work[k].p, bytes, hipMemcpyDeviceToDevice, d.stream), "copy field");
+    d.propagate_live();
+  }
+};
+
 // ---------------------------------------------------------------------------------------------
 // Discrete adjoint of the step
 // ---------------------------------------------------------------------------------------------
@@ -1927,6 +2176,7 @@ void Domain::add_module(const std::string& name, const std::vector<std::pair<std
   else if (name == "field_heating") mods.push_back(std::make_unique<FieldHeatModule>(opts));
   else if (name == "div_cleaning") mods.push_back(std::make_unique<DivCleanModule>(opts));
   else if (name == "parametric_heating") mods.push_back(std::make_unique<ParamHeatModule>(opts));
+  else if (name == "anomalous_resistivity") mods.push_back(std::make_unique<ResistModule>(opts));
   else throw Error(SPRUCE_E_UNSUPPORTED, "module " + name + " is not available in this build");
   modules.emplace_back(name, opts);
 }

@@ -572,6 +572,8 @@ class Runner {
     if (m.name == "field_heating")  // fieldheating.cpp:64-71
       return std::string("Field Heating") + (opt_num(m, "coeff") == 0.0 ? " Zero" : " On") + na;
     if (m.name == "div_cleaning") return "Div. Cleaning On";  // divcleaning.cpp:50-53
+    if (m.name == "anomalous_resistivity")  // anomalousresistivity.cpp:315-318
+      return "Anomalous Resistivity Subcycles: " + std::to_string(sub);
     if (m.name == "physical_viscosity") {  // physicalviscosity.cpp:273-290
       const bool heat = opt_flag(m, "heating_on", true), force = opt_flag(m, "force_on", true);
       const bool zero = opt_num(m, "coeff") == 0.0;

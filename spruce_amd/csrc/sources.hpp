@@ -49,4 +49,55 @@ void launch_dc_update(const double* bx, const double* by, const double* dx_bx, c
                       double* biy, const double* bex, const double* bey, const double* coeff, double h, Mesh mesh,
                       double* nbx, double* nby, hipStream_t s);
 
+// ---- anomalous resistivity (anomalousresistivity.cpp) ----
+enum ArModel : int { AR_TIME_SCALE = 0, AR_SYNTELIS = 1, AR_YS94 = 2 };
+struct ArParams {
+  int model;
+  double ts;          // time_scale (AR_TIME_SCALE)
+  double p0, p1, p2;  // eta_0, eta_1, J_crit (Syntelis-19) or v_c, alpha, eta_max (YS-94)
+};
+struct ArRhsArgs {
+  const double *bix, *biy, *biz, *coeff, *J, *lbx, *lby, *lbz;
+  double *d0, *d1, *d2, *d3;
+  double eta_factor;  // 4 pi / C / C
+  int grad_corr;
+  Mesh mesh;
+};
+struct ArAxpyArgs {
+  const double* base[4];
+  const double* k1[4];
+  const double* k2[4];
+  const double* k3[4];
+  const double* k4[4];
+  double* out[4];
+  double s;
+  int nf, rk4;
+  size_t n;
+};
+void launch_ar_current(const double* bix, const double* biy, const double* biz, double cc, Mesh mesh, double* J,
+                       hipStream_t s);
+void launch_ar_field(const double* bex, const double* bey, const double* bix, const double* biy, size_t n, double* bx,
+                     double* by, double* bm, hipStream_t s);
+void launch_ar_frob(const double* bx, const double* by, const double* bm, double coeff, Mesh mesh, double* ratio,
+                    double* tmpl, hipStream_t s);
+// out[0] = smallest value within the radius in the box, out[1] = its cell index (-1: none),
+// out[2] = the value at the centre
+void launch_ar_argmin_box(const double* q, const double* px, const double* py, int ci, int cj, double radius, int il,
+                          int iu, int jl, int ju, int ny, double* out, hipStream_t s);
+void launch_ar_flood_seed(const double* q, size_t n, size_t start, double thr, double* res, hipStream_t s);
+void launch_ar_flood_sweep(const double* q, double thr, Mesh mesh, double* res, int* changed, hipStream_t s);
+void launch_ar_masks(double* tmpl, const double* J, const double* px, const double* py, size_t cnull, double max_radius,
+                     double min_current, double ramp, Mesh mesh, hipStream_t s);
+void launch_ar_smooth(const double* t, const double* kern, int kr, Mesh mesh, double* out, hipStream_t s);
+void launch_ar_frob_final(double* t, size_t n, hipStream_t s);
+void launch_ar_diffusion(const double* J, const double* n, ArParams p, Mesh mesh, double* diff, hipStream_t s);
+void launch_ar_coeff(const double* tmpl, const double* diff, Mesh mesh, double* coeff, double* td, hipStream_t s);
+void launch_ar_rhs(const ArRhsArgs& a, hipStream_t s);
+void launch_ar_laplacian(const double* q, Mesh mesh, double* out, hipStream_t s);
+void launch_ar_axpy(const ArAxpyArgs& a, hipStream_t s);
+int ar_scale_blocks(const Mesh& m);
+// out[0] = max over every cell of td, out[1] = min over [xl..xu] x [yl..yu] of (1/(1/dx^2+1/dy^2))/2/td
+void launch_ar_scales(const double* td, Mesh mesh, int xl, int xu, int yl, int yu, double* partial, double* out,
+                      hipStream_t s);
+
 }  // namespace spruce

@@ -95,7 +95,8 @@ def test_runner_output_schedule_and_continue_mode(tmp_path):
     _same_files(cont)
 
 
-SOURCE_MODULES = ["lheat_ramp_rk2", "mass_inj_rk4", "mom_inj_osc_euler", "sink_exp_heat_rk2", "divclean_rk2"]
+SOURCE_MODULES = ["lheat_ramp_rk2", "mass_inj_rk4", "mom_inj_osc_euler", "sink_exp_heat_rk2", "divclean_rk2",
+                  "ar_flood_time_euler", "ar_flood_masks_rk2"]
 
 
 @pytest.mark.parametrize("name", SOURCE_MODULES)

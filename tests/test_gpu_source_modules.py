@@ -1,13 +1,15 @@
 """GPU parity for the source-term modules: localized heating, mass and momentum injection, the
-ambient heating sink, field heating and divergence cleaning.
+ambient heating sink, field heating, divergence cleaning and anomalous resistivity.
 
 Their templates depend on positions only and are built on the host with the C library's exp, sin,
-cos and pow, as the reference builds them, so every case except field heating is bit-identical to
-the reference golden vectors (grids after set-up, after step 1 and after the last step, and the
-step-size sequence) and to the CPU restatement step by step, subcycle counts included.  Field
-heating raises grids to arbitrary powers with the device pow, which is not correctly rounded and
-need not agree with the C library's in the last place: it is compared within 1e-10 relative
-L-infinity, the tolerance of the plain path, with step sizes within 1e-12.
+cos and pow, as the reference builds them, so every case without a power of a grid is
+bit-identical to the reference golden vectors (grids after set-up, after step 1 and after the last
+step, and the step-size sequence) and to the CPU restatement step by step, subcycle counts
+included.  Field heating raises grids to arbitrary powers with the device pow, and the frobenius
+template of anomalous resistivity raises it to 1.5 correctly rounded; the reference's C library pow
+need not agree in the last place, so those are compared within 1e-10 relative L-infinity, the
+tolerance of the plain path, with step sizes within 1e-12 (and the x^1.5 case bit for bit against
+the restatement with correctly rounded powers).
 """
 import os
 import sys
@@ -16,7 +18,7 @@ import numpy as np
 import pytest
 
 from golden_util import load_golden
-from oracle.oracle import OracleDomain
+from oracle.oracle import OracleDomain, set_pow_mode
 from spruce_amd import workloads as W
 from spruce_amd.gpu import GpuDomain
 from spruce_amd.spec import spec_from_files
@@ -27,7 +29,7 @@ from cases import random_mhd_state  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 EXACT = ["lheat_ramp_rk2", "lheat_periodic_euler", "mass_inj_rk4", "mom_inj_rot_rk2", "mom_inj_osc_euler",
-         "sink_exp_heat_rk2", "sink_const_euler", "divclean_rk2"]
+         "sink_exp_heat_rk2", "sink_const_euler", "divclean_rk2", "ar_flood_time_euler", "ar_flood_masks_rk2"]
 VARS = ["rho", "temp", "mom_x", "mom_y", "mom_z", "bi_x", "bi_y", "bi_z", "n", "thermal_energy", "press", "dt"]
 REL_TOL = 1e-10
 
@@ -81,8 +83,11 @@ def test_source_modules_match_restatement_each_step(case):
         ora.close()
 
 
-def test_field_heating_matches_reference_within_tolerance():
-    g = load_golden("fheat_rk2")
+@pytest.mark.parametrize("case", ["fheat_rk2", "ar_frob_ys_rk4"])
+def test_pow_modules_match_reference_within_tolerance(case):
+    """Field heating (arbitrary powers) and the frobenius template of anomalous resistivity (x^1.5)
+    go through the C library's pow in the reference."""
+    g = load_golden(case)
     dom = GpuDomain(g.spec())
     try:
         step = 0
@@ -161,6 +166,55 @@ def test_field_heating_multi_block_matches_restatement_within_tolerance():
         base = OracleDomain(spec0)
         base.step(4)
         assert rel_linf(gpu.grid("thermal_energy"), base.grid("thermal_energy")) > 1e-6
+        base.close()
+    finally:
+        gpu.close()
+        ora.close()
+
+
+def test_resistivity_frobenius_matches_cr_restatement_bitwise():
+    spec = load_golden("ar_frob_ys_rk4").spec()
+    set_pow_mode(1)
+    gpu = GpuDomain(spec)
+    ora = OracleDomain(spec)
+    try:
+        for s in range(4):
+            dg, sg = gpu.step(1, subcycles=True)
+            do = ora.step(1)
+            assert list(sg[0]) == ora.subcycles(), (s, sg[0], ora.subcycles())
+            assert same_bits(dg, do), (s, dg, do)
+        for v in VARS:
+            assert same_bits(gpu.grid(v), ora.grid(v)), v
+    finally:
+        set_pow_mode(0)
+        gpu.close()
+        ora.close()
+
+
+def test_resistivity_multi_block_matches_restatement():
+    """Flood-fill template with both masks, Syntelis-19 resistivity, gradient correction and RK4
+    subcycles on a 133x197 stretched mesh (many blocks, a large flood region)."""
+    mods = [("anomalous_resistivity", [
+        ("resistivity_model", "syntelis_19"), ("resistivity_model_params", "2e13,5e13,150.0"),
+        ("template_mode", "flood_fill"), ("flood_fill_threshold", "2.4"), ("flood_fill_argmin_radius", "3e8"),
+        ("flood_fill_max_radius", "6e8"), ("flood_fill_min_current", "80.0"),
+        ("flood_fill_current_ramp_length", "150.0"), ("smoothing_sigma", "1.5"), ("gradient_correction", "true"),
+        ("time_integrator", "rk4")])]
+    spec = _big_spec(("open", "fixed", "reflect", "open"), "rk2", mods, 93)
+    gpu = GpuDomain(spec)
+    ora = OracleDomain(spec)
+    try:
+        for s in range(3):
+            dg, sg = gpu.step(1, subcycles=True)
+            do = ora.step(1)
+            assert list(sg[0]) == ora.subcycles(), (s, sg[0], ora.subcycles())
+            assert same_bits(dg, do), (s, dg, do)
+        for v in VARS:
+            assert same_bits(gpu.grid(v), ora.grid(v)), v
+        spec0 = _big_spec(("open", "fixed", "reflect", "open"), "rk2", [], 93)
+        base = OracleDomain(spec0)
+        base.step(3)
+        assert rel_linf(gpu.grid("bi_x"), base.grid("bi_x")) > 1e-6
         base.close()
     finally:
         gpu.close()
