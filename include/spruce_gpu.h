@@ -89,6 +89,13 @@ int spruce_gpu_get_grid(spruce_gpu_domain* d, const char* name, double* host, in
 /* Simulation time and iteration counter (plasmadomain.hpp:96-98). */
 int spruce_gpu_time(spruce_gpu_domain* d, double* t, int32_t* iter);
 
+/* The device-side values that Module::commandLineMessage (module.hpp:45; boundaryoutflow.cpp:69-74)
+ * prints for module `module` (index in the order modules were added) after the latest step:
+ * boundary_outflow reports {largest outflow speed, applied acceleration}; other modules report
+ * nothing.  count_out (may be NULL) receives the number of values; out (may be NULL) must hold
+ * count >= that many doubles. */
+int spruce_gpu_module_report(spruce_gpu_domain* d, int32_t module, double* out, int32_t count, int32_t* count_out);
+
 /* Unit entry points for parity tests.
  * rhs: EquationSet::computeTimeDerivatives on the live grids (equationset.cpp:204-210), written as
  *      nevolved consecutive xdim*ydim blocks in evolved-variable order.

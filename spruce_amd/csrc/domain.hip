@@ -87,6 +87,8 @@ struct GpuModule {
   virtual void add_rhs(Domain&, SetBufs&, double*) {}
   // postIterateModule: after the integrator's final propagate, before the next step size
   virtual void post(Domain&) {}
+  // the values commandLineMessage prints that live on the device (empty for most modules)
+  virtual std::vector<double> report(Domain&) { return {}; }
 };
 
 static double opt_d(const std::vector<std::pair<std::string, std::string>>& o, const std::string& k, bool& found) {
@@ -593,6 +595,10 @@ class Domain : public DomainBase {
   }
   // dJ/dtheta of the parametrised heating modules after the last adjoint call
   std::vector<double> param_grad() override;
+  std::vector<double> module_report(int k) override {
+    require(k >= 0 && k < (int)mods.size(), SPRUCE_E_ARG, "module index out of range");
+    return mods[k]->report(*this);
+  }
   // device memory the last adjoint call allocated for itself (checkpoints, segment, replay
   // snapshots, adjoint vectors, conduction subcycle record)
   int64_t adj_bytes = 0;
@@ -1862,6 +1868,119 @@ struct ResistModule : GpuModule {
   }
 };
 
+// BoundaryOutflow (boundaryoutflow.cpp)
+struct OutflowModule : PostSourceModule {
+  OutflowParams p{};
+  double fall = 0.0, feather = 0.0;
+  std::string side = "y_bound_2", shape = "exp";
+  DBuf tmpl, part, sc;
+  explicit OutflowModule(const Opts& o) {  // :16-31
+    name = "boundary_outflow";
+    p.dtime = 1.0;
+    for (auto& kv : o) {
+      const std::string& k = kv.first;
+      const std::string& v = kv.second;
+      if (k == "max_accel") p.amax = std::stod(v);
+      else if (k == "falloff_length") fall = std::stod(v);
+      else if (k == "boundary") side = v;
+      else if (k == "falloff_shape") shape = v;
+      else if (k == "feather_length") feather = std::stod(v);
+      else if (k == "field_aligned_mode") p.aligned = v == "true" ? 1 : 0;
+      else if (k == "dynamic_mode") p.dynamic = v == "true" ? 1 : 0;
+      else if (k == "dynamic_time") p.dtime = std::stod(v);
+      else if (k == "dynamic_target_speed") p.target = std::stod(v);
+    }
+  }
+  bool needs_host_dt() const override { return false; }  // the step size is read on the device
+  bool xside() const { return p.side == OF_X1 || p.side == OF_X2; }
+  // constructBoundaryAccel (:78-138): depends on positions only, built once with the C library's exp
+  std::vector<double> build(const Domain& d) const {
+    const auto& x = d.host_grids.at("pos_x");
+    const auto& y = d.host_grids.at("pos_y");
+    auto lo = [](const std::vector<double>& a) { double m = a[0]; for (double e : a) m = (e < m) ? e : m; return m; };
+    auto hi = [](const std::vector<double>& a) { double m = a[0]; for (double e : a) m = (m < e) ? e : m; return m; };
+    const double xmin = lo(x), xmax = hi(x), ymin = lo(y), ymax = hi(y), len = fall, fe = feather;
+    std::vector<double> r(d.N, 0.0);
+    for (int i = 0; i < d.nx; ++i)
+      for (int j = 0; j < d.ny; ++j) {
+        const size_t c = (size_t)i * d.ny + j;
+        const double X = x[c], Y = y[c];
+        const bool in = i >= d.xl && i <= d.xu && j >= d.yl && j <= d.yu;
+        double t = 0.0;
+        if (shape == "exp") {
+          if (p.side == OF_X1) t = std::exp(-2.3 * (X - xmin) / len);
+          else if (p.side == OF_X2) t = std::exp(2.3 * (X - xmax) / len);
+          else if (p.side == OF_Y2) t = std::exp(2.3 * (Y - ymax) / len);
+          else t = std::exp(-2.3 * (Y - ymin) / len);
+        } else if (shape == "gaussian") {
+          const double u = p.side == OF_X1 ? (X - xmin) / len : p.side == OF_X2 ? (-X + xmax) / len
+                         : p.side == OF_Y2 ? (-Y + ymax) / len : (Y - ymin) / len;
+          t = std::exp(-2.3 * (u * u));
+        } else {  // flat: 1 within the falloff length of the boundary, on the interior
+          const double e = p.side == OF_X1 ? xmin : p.side == OF_X2 ? xmax : p.side == OF_Y2 ? ymax : ymin;
+          if (in && std::fabs((xside() ? X : Y) - e) <= len) t = 1.0;
+        }
+        if (fe > 0.0) {  // feathering along the boundary (:128-135)
+          const double q = xside() ? Y : X, qmax = xside() ? ymax : xmax, qmin = xside() ? ymin : xmin;
+          const double a = smax(q - (qmax - 2.0 * fe), 0.0) / fe;
+          const double b = smin(q - (qmin + 2.0 * fe), 0.0) / fe;
+          t = t * smax(std::exp(-2.3 * (a * a)) * std::exp(-2.3 * (b * b)) - 0.01, 0.0);
+        }
+        r[c] = (in ? 1.0 : 0.0) * t;
+      }
+    return r;
+  }
+  // the cells computeMeanOutflow scans (:142-212): the interior, less the feathered ends along the
+  // boundary and cut off at the falloff length from it
+  void window(const Domain& d) {
+    const auto& x = d.host_grids.at("pos_x");
+    const auto& y = d.host_grids.at("pos_y");
+    auto X = [&](int i) { return x[(size_t)i * d.ny]; };
+    auto Y = [&](int j) { return y[(size_t)j]; };
+    int xl = d.xl, xu = d.xu, yl = d.yl, yu = d.yu;
+    if (xside()) {
+      for (int j = yl; j <= yu; ++j) if (Y(j) - Y(yl) >= feather) { yl = j; break; }
+      for (int j = yu; j >= yl; --j) if (Y(yu) - Y(j) >= feather) { yu = j; break; }
+      if (p.side == OF_X1) { for (int i = xl; i <= xu; ++i) if (X(i) - X(xl) >= fall) { xu = i; break; } }
+      else { for (int i = xu; i >= xl; --i) if (X(xu) - X(i) >= fall) { xl = i; break; } }
+    } else {
+      for (int i = xl; i <= xu; ++i) if (X(i) - X(xl) >= feather) { xl = i; break; }
+      for (int i = xu; i >= xl; --i) if (X(xu) - X(i) >= feather) { xu = i; break; }
+      if (p.side == OF_Y1) { for (int j = yl; j <= yu; ++j) if (Y(j) - Y(yl) >= fall) { yu = j; break; } }
+      else { for (int j = yu; j >= yl; --j) if (Y(yu) - Y(j) >= fall) { yl = j; break; } }
+    }
+    p.wxl = xl; p.wxu = xu; p.wyl = yl; p.wyu = yu;
+  }
+  void setup(Domain& d) override {  // :33-37
+    require(shape == "exp" || shape == "gaussian" || shape == "flat", SPRUCE_E_ARG,
+            "BoundaryOutflow shape must be exp or gaussian or flat");
+    if (side == "x_bound_1") p.side = OF_X1;
+    else if (side == "x_bound_2") p.side = OF_X2;
+    else if (side == "y_bound_1") p.side = OF_Y1;
+    else if (side == "y_bound_2") p.side = OF_Y2;
+    else throw Error(SPRUCE_E_ARG, "BoundaryOutflow boundary config must be {x,y}_bound_{1,2}");
+    upload_vec(tmpl, build(d));
+    window(d);
+    require(p.wxl <= p.wxu && p.wyl <= p.wyu, SPRUCE_E_ARG, "boundary_outflow: empty interior");
+    part.alloc(2 * (size_t)outflow_blocks(p));
+    sc.alloc(2);   // mean outflow, acceleration (0 until the first step)
+    d.timed("outflow", [&] { launch_outflow_max(d.L.view(), d.stat(), p, d.mesh, part.p, sc.p, 0, d.stream); });
+  }
+  void post(Domain& d) override {  // :39-67
+    d.timed("outflow", [&] {
+      launch_outflow_max(d.L.view(), d.stat(), p, d.mesh, part.p, sc.p, 1, d.stream);
+      launch_outflow_apply(d.L.view(), d.stat(), tmpl.p, sc.p, d.step.p, p, d.mesh, d.stream);
+    });
+    d.propagate_live();
+  }
+  std::vector<double> report(Domain& d) override {  // commandLineMessage (:69-74)
+    std::vector<double> v(2);
+    hip_check(hipMemcpyAsync(v.data(), sc.p, 2 * sizeof(double), hipMemcpyDeviceToHost, d.stream), "read outflow");
+    hip_check(hipStreamSynchronize(d.stream), "hipStreamSynchronize");
+    return v;
+  }
+};
+
 // ---------------------------------------------------------------------------------------------
 // Discrete adjoint of the step
 // ---------------------------------------------------------------------------------------------
@@ -2177,6 +2296,7 @@ void Domain::add_module(const std::string& name, const std::vector<std::pair<std
   else if (name == "div_cleaning") mods.push_back(std::make_unique<DivCleanModule>(opts));
   else if (name == "parametric_heating") mods.push_back(std::make_unique<ParamHeatModule>(opts));
   else if (name == "anomalous_resistivity") mods.push_back(std::make_unique<ResistModule>(opts));
+  else if (name == "boundary_outflow") mods.push_back(std::make_unique<OutflowModule>(opts));
   else throw Error(SPRUCE_E_UNSUPPORTED, "module " + name + " is not available in this build");
   modules.emplace_back(name, opts);
 }
@@ -2498,6 +2618,18 @@ int spruce_gpu_adjoint_device_bytes(spruce_gpu_domain* h, int64_t* bytes) {
   API_BEGIN
   require(bytes != nullptr, SPRUCE_E_ARG, "null argument");
   *bytes = dom(h).adjoint_bytes();
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_module_report(spruce_gpu_domain* h, int32_t module, double* out, int32_t count, int32_t* count_out) {
+  API_BEGIN
+  require(count_out || out, SPRUCE_E_ARG, "null argument");
+  const std::vector<double> v = dom(h).module_report(module);
+  if (count_out) *count_out = (int32_t)v.size();
+  if (out) {
+    require(count >= (int32_t)v.size(), SPRUCE_E_ARG, "out holds fewer values than the module reports");
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+  }
   API_END(SPRUCE_OK)
 }
 

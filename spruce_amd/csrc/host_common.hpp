@@ -169,6 +169,8 @@ class DomainBase {
     throw Error(SPRUCE_E_UNSUPPORTED, "the adjoint is available for ideal_mhd only");
   }
   virtual int64_t adjoint_bytes() const { return 0; }
+  // the device values module k's command-line message reports (empty when it reports none)
+  virtual std::vector<double> module_report(int k) { (void)k; return {}; }
 
   double time = 0.0;
   int iter = 0;

@@ -540,7 +540,7 @@ class Runner {
     if (max_time_ > 0.0) std::cout << "/" << max_time_;
     std::cout << "|dt: " << dt;
     for (size_t k = 0; k < cfg_.modules.size(); ++k) {
-      const std::string msg = module_message(cfg_.modules[k], subs[k]);
+      const std::string msg = module_message(cfg_.modules[k], subs[k], (int)k);
       if (!msg.empty()) std::cout << "|" << msg;
     }
     std::cout << std::endl;
@@ -553,7 +553,7 @@ class Runner {
     return v.empty() ? 0.0 : std::stod(v);
   }
   // commandLineMessage of the modules this build runs
-  std::string module_message(const Module& m, int32_t sub) const {
+  std::string module_message(const Module& m, int32_t sub, int k) const {
     const std::string na = opt_flag(m, "inactive_mode", false) ? " (Not Applied)" : "";
     if (m.name == "thermal_conduction") return "Thermal Subcycles: " + std::to_string(sub) + na;  // thermalconduction.cpp:242-245
     if (m.name == "radiative_losses") return "Radiative Subcycles: " + std::to_string(sub) + na;  // radiativelosses.cpp:169-172
@@ -572,6 +572,15 @@ class Runner {
     if (m.name == "field_heating")  // fieldheating.cpp:64-71
       return std::string("Field Heating") + (opt_num(m, "coeff") == 0.0 ? " Zero" : " On") + na;
     if (m.name == "div_cleaning") return "Div. Cleaning On";  // divcleaning.cpp:50-53
+    if (m.name == "boundary_outflow") {  // boundaryoutflow.cpp:69-74
+      double v[2] = {0.0, 0.0};
+      int32_t cnt = 0;
+      gpu(spruce_gpu_module_report(dom_, k, v, 2, &cnt), "module report");
+      check(cnt == 2, "boundary_outflow reports two values");
+      const std::string side = m.has("boundary") ? m.opt("boundary") : "y_bound_2";
+      return side + " boundary outflow enforced (max " + std::to_string(v[0]) + " cm/s outflow) (accel. " +
+             std::to_string(v[1]) + " cm/s^2)";
+    }
     if (m.name == "anomalous_resistivity")  // anomalousresistivity.cpp:315-318
       return "Anomalous Resistivity Subcycles: " + std::to_string(sub);
     if (m.name == "physical_viscosity") {  // physicalviscosity.cpp:273-290

@@ -3,6 +3,7 @@
 // divcleaning.cpp).  Each whole-grid expression of the reference is evaluated per cell with the
 // same operation order; derivatives are the reference's derivative1D / secondDerivative1D
 // (derivs.cpp:376-455): zero off the interior, face values by boundaryInterpolate.
+#include <algorithm>
 #include <cfloat>
 
 #include "arith.hpp"
@@ -517,6 +518,110 @@ __global__ void __launch_bounds__(256) k_ar_scales_final(const double* pmax, con
 
 inline dim3 g2(const Mesh& m, dim3 b) { return dim3((m.ny + b.x - 1) / b.x, (m.nx + b.y - 1) / b.y); }
 
+
+// b_hat of one cell as IdealMHD::recomputeDerivedVarsFromEvolvedVars builds it
+// (idealmhd.cpp:255-262): b = be + bi, b_hat = b / |b|, and 0 where |b| is 0
+__device__ inline void of_bhat(const MhdSet& L, const MhdStatic& st, size_t c, double& hx, double& hy) {
+  const double bx = st.bex[c] + L.bx[c], by = st.bey[c] + L.by[c], bz = st.bez[c] + L.bz[c];
+  const double bm = sqrt(bx * bx + by * by + bz * bz);
+  hx = bm == 0.0 ? 0.0 : bx / bm;
+  hy = bm == 0.0 ? 0.0 : by / bm;
+}
+
+// the better of two (value, row-order index) pairs for a left fold with std::max: the larger
+// value, and of equal values the earlier one; NaN never wins
+__device__ inline void of_pick(double& v, long long& k, double ov, long long ok) {
+  if (ok >= 0 && (ov > v || (ov == v && (k < 0 || ok < k)))) {
+    v = ov;
+    k = ok;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_outflow_max(MhdSet L, MhdStatic st, OutflowParams p, int ny,
+                                                     double* pv, double* pk) {
+  const int w = p.wyu - p.wyl + 1;
+  const long long cnt = (long long)(p.wxu - p.wxl + 1) * w;
+  double v = -INFINITY;
+  long long k = -1;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += (long long)gridDim.x * blockDim.x) {
+    const int i = p.wxl + (int)(t / w), j = p.wyl + (int)(t % w);
+    const size_t c = (size_t)i * ny + j;
+    double hx, hy;
+    of_bhat(L, st, c, hx, hy);
+    const double vx = L.mx[c] / L.rho[c], vy = L.my[c] / L.rho[c];
+    double cur = hx * vx + hy * vy;
+    // orient towards the boundary (boundaryoutflow.cpp:225-229)
+    if (p.side == OF_X1 && hx > 0.0) cur *= -1.0;
+    else if (p.side == OF_X2 && hx < 0.0) cur *= -1.0;
+    else if (p.side == OF_Y2 && hy < 0.0) cur *= -1.0;
+    else if (p.side == OF_Y1 && hy > 0.0) cur *= -1.0;
+    of_pick(v, k, cur, (long long)c);
+  }
+  __shared__ double sv[256];
+  __shared__ long long sk[256];
+  sv[threadIdx.x] = v;
+  sk[threadIdx.x] = k;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) of_pick(sv[threadIdx.x], sk[threadIdx.x], sv[threadIdx.x + off], sk[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pv[blockIdx.x] = sv[0];
+    pk[blockIdx.x] = (double)sk[0];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_outflow_final(const double* pv, const double* pk, int nb, OutflowParams p,
+                                                       int set_acc, double* sc) {
+  double v = -INFINITY;
+  long long k = -1;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) of_pick(v, k, pv[b], (long long)pk[b]);
+  __shared__ double sv[256];
+  __shared__ long long sk[256];
+  sv[threadIdx.x] = v;
+  sk[threadIdx.x] = k;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) of_pick(sv[threadIdx.x], sk[threadIdx.x], sv[threadIdx.x + off], sk[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    // the fold starts from -dynamic_target_speed (:217) and takes a cell only when it is larger
+    const double init = -1.0 * p.target;
+    const double m = (sk[0] >= 0 && init < sv[0]) ? sv[0] : init;
+    sc[0] = m;
+    if (set_acc) {
+      double a = p.amax;
+      if (p.dynamic) {  // :41-45
+        a = (p.target - m) / p.dtime;
+        a = (p.amax < a) ? p.amax : a;
+        a = (a < 0.0) ? 0.0 : a;
+      }
+      sc[1] = a;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_outflow_apply(MhdSet L, MhdStatic st, const double* tmpl, const double* sc,
+                                                       const double* step, OutflowParams p, Mesh M) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= M.nx || j >= M.ny) return;
+  const size_t c = (size_t)i * M.ny + j;
+  const double a = *step * (sc[1] * tmpl[c]);
+  const double rho = L.rho[c];
+  if (p.aligned) {
+    double hx, hy;
+    of_bhat(L, st, c, hx, hy);
+    L.mx[c] = L.mx[c] + (a * hx) * rho;
+    L.my[c] = L.my[c] + (a * hy) * rho;
+  } else if (p.side == OF_X1 || p.side == OF_X2) {
+    L.mx[c] = L.mx[c] + a * rho;
+  } else {
+    L.my[c] = L.my[c] + a * rho;
+  }
+}
+
 }  // namespace
 
 void launch_source(double* dst, const double* t, const double* w, double a, double b, const double* a_dev, int mode,
@@ -611,6 +716,23 @@ void launch_ar_scales(const double* td, Mesh mesh, int xl, int xu, int yl, int y
   hipLaunchKernelGGL(k_ar_scales, ar_blocks(mesh), dim3(64, 4), 0, s, td, mesh, xl, xu, yl, yu, partial, partial + nb);
   hipLaunchKernelGGL(k_ar_scales_final, dim3(1), dim3(256), 0, s, (const double*)partial, (const double*)(partial + nb),
                      nb, out);
+}
+
+
+int outflow_blocks(const OutflowParams& p) {
+  const long long cnt = (long long)(p.wxu - p.wxl + 1) * (p.wyu - p.wyl + 1);
+  return (int)std::max(1LL, std::min(256LL, (cnt + 255) / 256));
+}
+void launch_outflow_max(MhdSet L, MhdStatic st, OutflowParams p, Mesh mesh, double* partial, double* sc, int set_acc,
+                        hipStream_t s) {
+  const int nb = outflow_blocks(p);
+  hipLaunchKernelGGL(k_outflow_max, dim3(nb), dim3(256), 0, s, L, st, p, mesh.ny, partial, partial + nb);
+  hipLaunchKernelGGL(k_outflow_final, dim3(1), dim3(256), 0, s, (const double*)partial,
+                     (const double*)(partial + nb), nb, p, set_acc, sc);
+}
+void launch_outflow_apply(MhdSet L, MhdStatic st, const double* tmpl, const double* sc, const double* step,
+                          OutflowParams p, Mesh mesh, hipStream_t s) {
+  hipLaunchKernelGGL(k_outflow_apply, ar_blocks(mesh), dim3(64, 4), 0, s, L, st, tmpl, sc, step, p, mesh);
 }
 
 }  // namespace spruce

@@ -100,4 +100,28 @@ int ar_scale_blocks(const Mesh& m);
 void launch_ar_scales(const double* td, Mesh mesh, int xl, int xu, int yl, int yu, double* partial, double* out,
                       hipStream_t s);
 
+// ---- boundary outflow (boundaryoutflow.cpp) ----
+enum OutflowSide : int { OF_X1 = 0, OF_X2 = 1, OF_Y1 = 2, OF_Y2 = 3 };
+struct OutflowParams {
+  int side;           // which boundary the outflow leaves through
+  int aligned;        // field_aligned_mode: accelerate along b_hat instead of the boundary normal
+  int dynamic;        // dynamic_mode: throttle towards dynamic_target_speed
+  double amax;        // max_accel
+  double dtime;       // dynamic_time
+  double target;      // dynamic_target_speed
+  int wxl, wxu, wyl, wyu;  // the cells computeMeanOutflow scans (inclusive)
+};
+int outflow_blocks(const OutflowParams& p);
+// computeMeanOutflow (boundaryoutflow.cpp:140-236) and the acceleration of postIterateModule
+// (:39-46): sc[0] = the largest outflow speed along the field over the window, starting the fold
+// from -dynamic_target_speed (ties keep the first value in row order, as std::max does), and with
+// set_acc != 0, sc[1] = max_accel or the throttled acceleration.  partial holds 2 * outflow_blocks.
+void launch_outflow_max(MhdSet L, MhdStatic st, OutflowParams p, Mesh mesh, double* partial, double* sc, int set_acc,
+                        hipStream_t s);
+// postIterateModule's momentum update (:47-66): a = dt * (sc[1] * template), then
+// mom_x += (a * b_hat_x) * rho and mom_y += (a * b_hat_y) * rho along the field, or the momentum
+// normal to the boundary += a * rho.  dt is read from the device.
+void launch_outflow_apply(MhdSet L, MhdStatic st, const double* tmpl, const double* sc, const double* step,
+                          OutflowParams p, Mesh mesh, hipStream_t s);
+
 }  // namespace spruce

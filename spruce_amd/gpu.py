@@ -81,6 +81,7 @@ def lib() -> ctypes.CDLL:
     L.spruce_gpu_adjoint.argtypes = [P, I32, I32, DP, DP, DP]
     L.spruce_gpu_adjoint_state.argtypes = [P, DP, DP]
     L.spruce_gpu_adjoint_params.argtypes = [P, DP, I32, ctypes.POINTER(I32)]
+    L.spruce_gpu_module_report.argtypes = [P, I32, DP, I32, ctypes.POINTER(I32)]
     L.spruce_gpu_adjoint_device_bytes.argtypes = [P, ctypes.POINTER(ctypes.c_int64)]
     L.spruce_gpu_batch_set_profiling.argtypes = [P, I32]
     L.spruce_gpu_batch_kernel_times.argtypes = [P, ctypes.c_char_p, I32, DP, ctypes.POINTER(I32), I32]
@@ -222,6 +223,16 @@ class GpuDomain:
         out = np.empty(n.value)
         if n.value:
             _chk(lib().spruce_gpu_adjoint_params(self.h, _dp(out), n.value, None))
+        return out
+
+    def module_report(self, k: int) -> np.ndarray:
+        """The device values module k's command-line message prints after the latest step
+        (boundary_outflow: largest outflow speed, applied acceleration)."""
+        n = ctypes.c_int32(0)
+        _chk(lib().spruce_gpu_module_report(self.h, k, None, 0, ctypes.byref(n)))
+        out = np.empty(n.value)
+        if n.value:
+            _chk(lib().spruce_gpu_module_report(self.h, k, _dp(out), n.value, None))
         return out
 
     def adjoint_device_bytes(self) -> int:

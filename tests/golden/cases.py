@@ -277,7 +277,32 @@ def _cases() -> Dict[str, Tuple[Callable[[], StateFile], Callable[[], ConfigFile
                                    ("template_mode", "frobenius"), ("frobenius_metric_coeff", "1e41"),
                                    ("smoothing_sigma", "0.6"), ("time_integrator", "rk4"),
                                    ("safety_factor", "0.5")])]))
-    # single-fluid variants (SURVEY a28, a29): two energies, and the conservative total-energy form
+    # boundary outflow (boundaryoutflow.cpp): the three falloff shapes, feathering, acceleration
+    # along an axis or along the field, and the dynamic throttle towards a target outflow speed
+    c["outflow_exp_y2_rk2"] = (lambda: random_mhd_state(20, 22, 101),
+                               lambda: _mhd(HB, "rk2", 5, modules=[("boundary_outflow", [
+                                   ("max_accel", "1e8"), ("falloff_length", "4e7"), ("boundary", "y_bound_2"),
+                                   ("falloff_shape", "exp")])]))
+    c["outflow_gauss_x1_dyn_euler"] = (lambda: random_mhd_state(22, 20, 102, stretch=True),
+                                       lambda: _mhd(("open", "reflect", "fixed", "open"), "euler", 5, modules=[
+                                           ("boundary_outflow", [
+                                               ("max_accel", "1e9"), ("falloff_length", "5e7"),
+                                               ("boundary", "x_bound_1"), ("falloff_shape", "gaussian"),
+                                               ("feather_length", "2e7"), ("field_aligned_mode", "true"),
+                                               ("dynamic_mode", "true"), ("dynamic_time", "1.0"),
+                                               ("dynamic_target_speed", "3e7")])]))
+    c["outflow_flat_y1_rk4"] = (lambda: random_mhd_state(20, 24, 103),
+                                lambda: _mhd(("periodic", "periodic", "open", "fixed"), "rk4", 4, modules=[
+                                    ("boundary_outflow", [
+                                        ("max_accel", "5e7"), ("falloff_length", "3e7"), ("boundary", "y_bound_1"),
+                                        ("falloff_shape", "flat"), ("feather_length", "2e7")])]))
+    c["outflow_exp_x2_aligned_rk2"] = (lambda: random_mhd_state(22, 20, 104),
+                                       lambda: _mhd(("reflect", "open", "periodic", "periodic"), "rk2", 5, modules=[
+                                           ("boundary_outflow", [
+                                               ("max_accel", "2e8"), ("falloff_length", "3e7"),
+                                               ("boundary", "x_bound_2"), ("falloff_shape", "exp"),
+                                               ("feather_length", "1.5e7"), ("field_aligned_mode", "true")])]))
+    # single-fluid variants: two energies, and the conservative total-energy form
     E2 = ["rho", "mom_x", "mom_y", "i_thermal_energy", "e_thermal_energy", "bi_x", "bi_y", "i_temp", "e_temp",
           "press", "b_hat_x", "dt"]
     CO = ["rho", "mom_x", "mom_y", "energy", "bi_x", "bi_y", "thermal_energy", "press", "temp", "mag_energy",

@@ -1,5 +1,6 @@
 """GPU parity for the source-term modules: localized heating, mass and momentum injection, the
-ambient heating sink, field heating, divergence cleaning and anomalous resistivity.
+ambient heating sink, field heating, divergence cleaning, anomalous resistivity and boundary
+outflow.
 
 Their templates depend on positions only and are built on the host with the C library's exp, sin,
 cos and pow, as the reference builds them, so every case without a power of a grid is
@@ -29,7 +30,8 @@ from cases import random_mhd_state  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 EXACT = ["lheat_ramp_rk2", "lheat_periodic_euler", "mass_inj_rk4", "mom_inj_rot_rk2", "mom_inj_osc_euler",
-         "sink_exp_heat_rk2", "sink_const_euler", "divclean_rk2", "ar_flood_time_euler", "ar_flood_masks_rk2"]
+         "sink_exp_heat_rk2", "sink_const_euler", "divclean_rk2", "ar_flood_time_euler", "ar_flood_masks_rk2",
+         "outflow_exp_y2_rk2", "outflow_gauss_x1_dyn_euler", "outflow_flat_y1_rk4", "outflow_exp_x2_aligned_rk2"]
 VARS = ["rho", "temp", "mom_x", "mom_y", "mom_z", "bi_x", "bi_y", "bi_z", "n", "thermal_energy", "press", "dt"]
 REL_TOL = 1e-10
 
@@ -127,6 +129,14 @@ MULTI = [
                               ("template_angle", "-20.0"), ("oscillatory", "true"),
                               ("oscillation_period", "0.3")]),
       ("ambient_heating_sink", [("heating_rate", "0.2")])]),
+    # a window of thousands of cells (several reduction blocks), along the field, throttled
+    ("outflow_dynamic", ("open", "fixed", "reflect", "open"), "rk2",
+     [("boundary_outflow", [("max_accel", "1e9"), ("falloff_length", "6e9"), ("boundary", "y_bound_2"),
+                            ("falloff_shape", "gaussian"), ("feather_length", "1e8"), ("field_aligned_mode", "true"),
+                            ("dynamic_mode", "true"), ("dynamic_time", "0.5"), ("dynamic_target_speed", "2e7")])]),
+    ("outflow_flat_x2", ("periodic", "periodic", "open", "open"), "euler",
+     [("boundary_outflow", [("max_accel", "3e8"), ("falloff_length", "4e8"), ("boundary", "x_bound_2"),
+                            ("falloff_shape", "flat"), ("feather_length", "5e8")])]),
 ]
 
 
@@ -146,6 +156,30 @@ def test_source_modules_multi_block_match_restatement(name, bc, integ, modules):
     finally:
         gpu.close()
         ora.close()
+
+
+def test_boundary_outflow_report_follows_the_throttle():
+    """The reported outflow speed and acceleration: max_accel without dynamic mode, and with it
+    (target - outflow) / dynamic_time clipped to [0, max_accel]; the acceleration is 0 before the
+    first step."""
+    g = load_golden("outflow_gauss_x1_dyn_euler")
+    dom = GpuDomain(g.spec())
+    try:
+        r0 = dom.module_report(0)
+        assert r0.shape == (2,) and r0[1] == 0.0 and np.isfinite(r0[0])
+        for _ in range(3):
+            dom.step(1)
+            mean, acc = dom.module_report(0)
+            assert acc == min((3e7 - mean) / 1.0, 1e9) and 0.0 < acc < 1e9, (mean, acc)
+    finally:
+        dom.close()
+    g = load_golden("outflow_exp_y2_rk2")
+    dom = GpuDomain(g.spec())
+    try:
+        dom.step(2)
+        assert dom.module_report(0)[1] == 1e8
+    finally:
+        dom.close()
 
 
 def test_field_heating_multi_block_matches_restatement_within_tolerance():
