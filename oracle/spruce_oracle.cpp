@@ -11,6 +11,8 @@
 // (grid.cpp:110-113, 161-163).
 #include "spruce_oracle.h"
 
+#include <algorithm>
+#include <array>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -187,7 +189,6 @@ struct Domain {
   // plasmadomain.cpp:138-161
   void iteration_bounds() {
     require(nx > 2 * N_GHOST && ny > 2 * N_GHOST, "grid too small for ghost zones");
-    for (int k = 0; k < 4; ++k) require(bc[k] != OpenMoC, "open_moc boundaries are not supported");
     xl = bc[0] == Periodic ? 0 : N_GHOST;
     xu = bc[1] == Periodic ? nx - 1 : nx - N_GHOST - 1;
     yl = bc[2] == Periodic ? 0 : N_GHOST;
@@ -197,7 +198,13 @@ struct Domain {
     mask = A(nx, ny, 0.0);
     for (int i = xl; i <= xu; ++i) for (int j = yl; j <= yu; ++j) mask(i, j) = 1.0;
     xl_dt = xl; xu_dt = xu; yl_dt = yl; yu_dt = yu;
+    // open_moc sides evolve their ghost cells and take part in the step size (plasmadomain.cpp:155-159)
+    if (bc[0] == OpenMoC) xl_dt -= N_GHOST;
+    if (bc[1] == OpenMoC) xu_dt += N_GHOST;
+    if (bc[2] == OpenMoC) yl_dt -= N_GHOST;
+    if (bc[3] == OpenMoC) yu_dt += N_GHOST;
   }
+  bool any_moc() const { return bc[0] == OpenMoC || bc[1] == OpenMoC || bc[2] == OpenMoC || bc[3] == OpenMoC; }
 
   // derivs.cpp:477-487: linear interpolation to the face between two cells
   double interp(const A& q, int i1, int j1, int i2, int j2) const {
@@ -311,6 +318,118 @@ struct Domain {
   std::vector<A> curlZ(const A& z) const { A rx = deriv(z, 1); A ry = -deriv(z, 0); return {rx, ry}; }  // 465-469
   A curl2D(const A& x, const A& y) const { return deriv(y, 0) - deriv(x, 1); }  // 472-474
   A div2D(const A& x, const A& y) const { return deriv(x, 0) + deriv(y, 1); }   // 407-414
+
+  // ----- the same operators over explicit ranges, and the one-sided operators of the method of
+  // characteristics (derivs.cpp:77-117, 167-214, 223-264, 317-404, 417-455) -----
+  struct R4 { int il, jl, iu, ju; };
+  // neighbour c+off along the axis, wrapped only on a periodic axis
+  int nb(int c, int off, int axis) const {
+    const bool per = axis == 0 ? xper() : yper();
+    const int n = axis == 0 ? nx : ny;
+    return per ? (c + off + n) % n : c + off;
+  }
+  A deriv_r(const A& q, int axis, R4 r) const {  // derivative1D
+    A out(nx, ny, 0.0);
+    for (int i = r.il; i <= r.iu; ++i)
+      for (int j = r.jl; j <= r.ju; ++j) {
+        const int c = axis == 0 ? i : j, m = nb(c, -1, axis), p = nb(c, 1, axis);
+        const int im = axis == 0 ? m : i, jm = axis == 0 ? j : m, ip = axis == 0 ? p : i, jp = axis == 0 ? j : p;
+        const double den = axis == 0 ? pd[d_x](i, j) : pd[d_y](i, j);
+        out(i, j) = (interp(q, i, j, ip, jp) - interp(q, im, jm, i, j)) / den;
+      }
+    return out;
+  }
+  A second_r(const A& q, int axis, R4 r) const {  // secondDerivative1D
+    A out(nx, ny, 0.0);
+    A denom = sq(0.5 * pd[d_x] * (double)(1 - axis) + 0.5 * pd[d_y] * (double)axis);
+    for (int i = r.il; i <= r.iu; ++i)
+      for (int j = r.jl; j <= r.ju; ++j) {
+        const int c = axis == 0 ? i : j, m = nb(c, -1, axis), p = nb(c, 1, axis);
+        const int im = axis == 0 ? m : i, jm = axis == 0 ? j : m, ip = axis == 0 ? p : i, jp = axis == 0 ? j : p;
+        out(i, j) = (interp(q, i, j, ip, jp) - 2.0 * q(i, j) + interp(q, im, jm, i, j)) / denom(i, j);
+      }
+    return out;
+  }
+  A tderiv_r(const A& q, const A& vel, int axis, R4 r) const {  // transportDerivative1D
+    A sf = upwind(q, vel, axis, r.il, r.jl, r.iu, r.ju);
+    A out(nx, ny, 0.0);
+    A denom = pd[d_x] * (double)(1 - axis) + pd[d_y] * (double)axis;
+    for (int i = r.il; i <= r.iu; ++i)
+      for (int j = r.jl; j <= r.ju; ++j) {
+        const int c = axis == 0 ? i : j, m = nb(c, -1, axis), p = nb(c, 1, axis);
+        const int im = axis == 0 ? m : i, jm = axis == 0 ? j : m, ip = axis == 0 ? p : i, jp = axis == 0 ? j : p;
+        out(i, j) = (sf(ip, jp) * interp(vel, i, j, ip, jp) - sf(i, j) * interp(vel, im, jm, i, j)) / denom(i, j);
+      }
+    return out;
+  }
+  // upwindSurfaceDirected: Barton face values with the upwind side fixed
+  A upwind_dir(const A& q, bool vpos, int axis, R4 r) const {
+    A sf(nx + 1 - axis, ny + axis, 0.0);
+    const int sg = vpos ? 1 : -1;
+    for (int i = r.il; i <= r.iu; ++i)
+      for (int j = r.jl; j <= r.ju; ++j) {
+        int i0 = i, i1 = i, j0 = j, j1 = j;
+        if (axis == 0) { i0 = i - 2 * sg; i1 = i - sg; } else { j0 = j - 2 * sg; j1 = j - sg; }
+        const double d2 = interp(q, i1, j1, i, j), d3 = q(i1, j1), d1 = extrap(q, i0, j0, i1, j1);
+        const int is = i + ((axis == 0 && !vpos) ? 1 : 0), js = j + ((axis == 1 && !vpos) ? 1 : 0);
+        if (q(i, j) <= q(i1, j1)) sf(is, js) = std::min(d3, std::max(d1, d2));
+        else sf(is, js) = std::max(d3, std::min(d1, d2));
+      }
+    return sf;
+  }
+  // characteristicBartonDerivative1D
+  A char_barton(const A& q, bool pf, int axis, R4 r) const {
+    A sf = upwind_dir(q, pf, axis, r);
+    if (axis == 0) {
+      for (int j = r.jl; j <= r.ju; ++j) {
+        if (pf) sf(sf.r - 1, j) = q(q.r - 1, j) + (q(q.r - 1, j) - sf(sf.r - 2, j));
+        else sf(0, j) = q(0, j) - (sf(1, j) - q(0, j));
+      }
+    } else {
+      for (int i = r.il; i <= r.iu; ++i) {
+        if (pf) sf(i, sf.c - 1) = q(i, q.c - 1) + (q(i, q.c - 1) - sf(i, sf.c - 2));
+        else sf(i, 0) = q(i, 0) - (sf(i, 1) - q(i, 0));
+      }
+    }
+    A denom = pd[d_x] * (double)(1 - axis) + pd[d_y] * (double)axis;
+    A out(nx, ny, 0.0);
+    for (int i = r.il; i <= r.iu; ++i)
+      for (int j = r.jl; j <= r.ju; ++j) {
+        const int i2 = axis == 0 ? i + 1 : i, j2 = axis == 0 ? j : j + 1;
+        out(i, j) = (sf(i2, j2) - sf(i, j)) / denom(i, j);
+      }
+    return out;
+  }
+  // derivative1DBackward (first order, one-sided)
+  A back1(const A& q, bool pf, int axis, R4 r) const {
+    A out(nx, ny, 0.0);
+    for (int i = r.il; i <= r.iu; ++i)
+      for (int j = r.jl; j <= r.ju; ++j) {
+        const int c = axis == 0 ? i : j, o = nb(c, pf ? -1 : 1, axis);
+        const int i0 = axis == 0 ? o : i, j0 = axis == 0 ? j : o;
+        const A& dd = axis == 0 ? pd[d_x] : pd[d_y];
+        const double den = 0.5 * (dd(i, j) + dd(i0, j0));
+        out(i, j) = (pf ? 1.0 : -1.0) * (q(i, j) - q(i0, j0)) / den;
+      }
+    return out;
+  }
+  // secondDerivative1DBackward
+  A second_back(const A& q, bool pf, int axis, R4 r) const {
+    A out(nx, ny, 0.0);
+    for (int i = r.il; i <= r.iu; ++i)
+      for (int j = r.jl; j <= r.ju; ++j) {
+        const int c = axis == 0 ? i : j;
+        const int c1 = nb(c, pf ? -1 : 1, axis), c0 = nb(c1, pf ? -1 : 1, axis);
+        const int i1 = axis == 0 ? c1 : i, j1 = axis == 0 ? j : c1, i0 = axis == 0 ? c0 : i, j0 = axis == 0 ? j : c0;
+        const A& dd = axis == 0 ? pd[d_x] : pd[d_y];
+        const double den01 = 0.5 * (dd(i1, j1) + dd(i0, j0));
+        const double den12 = 0.5 * (dd(i1, j1) + dd(i, j));
+        const double t12 = q(std::max(i1, i), std::max(j1, j)) - q(std::min(i1, i), std::min(j1, j));
+        const double t01 = q(std::max(i1, i0), std::max(j1, j0)) - q(std::min(i1, i0), std::min(j1, j0));
+        out(i, j) = (pf ? 1.0 : -1.0) * (t12 / den12 - t01 / den01) / den12;
+      }
+    return out;
+  }
 
   // ----- ghost-cell fill, evolution.cpp:126-334 -----
   void open_fill(Grids& grids, int i1, int i2, int i3, int j1, int j2, int j3) const {  // 158-224
@@ -482,6 +601,8 @@ struct IdealMHD : EqSet {
   enum V { rho, temp, mom_x, mom_y, mom_z, bi_x, bi_y, bi_z, grav_x, grav_y, n, press, thermal_energy,
            v_x, v_y, v_z, kinetic_energy, b_x, b_y, b_z, b_mag, b_hat_x, b_hat_y, b_hat_z, dt_ };
   double global_viscosity = 0.0;
+  bool moc_b = false, moc_mom = false;
+  double moc_b_lo = 0.1, moc_b_hi = 10.0, moc_mom_lo = 0.1, moc_mom_hi = 10.0;  // idealmhd.hpp:59-64
   explicit IdealMHD(Domain& d)
       : EqSet(d, {"rho", "temp", "mom_x", "mom_y", "mom_z", "bi_x", "bi_y", "bi_z", "grav_x", "grav_y",
                   "n", "press", "thermal_energy", "v_x", "v_y", "v_z", "kinetic_energy",
@@ -495,9 +616,13 @@ struct IdealMHD : EqSet {
   void parse(const std::vector<std::string>& k, const std::vector<std::string>& v) override {  // idealmhd.cpp:12-40
     for (size_t i = 0; i < k.size(); ++i) {
       if (k[i] == "global_viscosity") global_viscosity = to_d(v[i]);
-      else if (k[i] == "viscosity_opt" || k[i] == "moc_b_limiting" || k[i] == "moc_mom_limiting" ||
-               k[i] == "moc_b_lower_lim" || k[i] == "moc_b_upper_lim" || k[i] == "moc_mom_lower_lim" ||
-               k[i] == "moc_mom_upper_lim") {}
+      else if (k[i] == "viscosity_opt") {}
+      else if (k[i] == "moc_b_limiting") moc_b = v[i] == "true";
+      else if (k[i] == "moc_mom_limiting") moc_mom = v[i] == "true";
+      else if (k[i] == "moc_b_lower_lim") { moc_b_lo = to_d(v[i]); require(moc_b_lo <= 1.0, "MoC B field lower threshold should be <=1.0"); }
+      else if (k[i] == "moc_b_upper_lim") { moc_b_hi = to_d(v[i]); require(moc_b_hi >= 1.0, "MoC B field upper threshold should be >=1.0"); }
+      else if (k[i] == "moc_mom_lower_lim") { moc_mom_lo = to_d(v[i]); require(moc_mom_lo <= 1.0, "MoC momentum threshold should be <=1.0"); }
+      else if (k[i] == "moc_mom_upper_lim") { moc_mom_hi = to_d(v[i]); require(moc_mom_hi >= 1.0, "MoC momentum threshold should be >=1.0"); }
       else throw fail(k[i] + " is not recognized for this equation set.");
     }
   }
@@ -526,14 +651,272 @@ struct IdealMHD : EqSet {
     A d_bz = -D.tdiv(G[bi_z], vx, vy) - D.tdiv(bez, vx, vy) + (G[bi_x] + bex) * D.deriv(G[v_z], 0) + (G[bi_y] + bey) * D.deriv(G[v_z], 1);
     // global_visc_coeff (idealmhd.cpp:90) only feeds the characteristic-boundary terms, which are
     // identically zero without open_moc boundaries (idealmhd.cpp:308-332): rho*0 + v*0 for momenta.
-    A zero(D.nx, D.ny, 0.0);
-    Grids ch(8, zero);
-    ch[1] = G[rho] * zero + G[v_x] * zero;
-    ch[2] = G[rho] * zero + G[v_y] * zero;
-    ch[3] = G[rho] * zero + G[v_z] * zero;
+    Grids ch;
+    if (D.any_moc()) {
+      A one(D.nx, D.ny, 1.0);
+      const double gvc = global_viscosity * 0.5 *
+          range_min(one / (one / sq(D.pd[d_x]) + one / sq(D.pd[d_y])) / G[dt_], D.xl_dt, D.yl_dt, D.xu_dt, D.yu_dt);
+      ch = moc_boundary(G, gvc);
+    } else {
+      A zero(D.nx, D.ny, 0.0);
+      ch.assign(8, zero);
+      ch[1] = G[rho] * zero + G[v_x] * zero;
+      ch[2] = G[rho] * zero + G[v_y] * zero;
+      ch[3] = G[rho] * zero + G[v_z] * zero;
+    }
     Grids k = {d_rho, d_mx, d_my, d_mz, d_te, d_bx, d_by, d_bz};
     for (int i = 0; i < 8; ++i) { k[i] *= D.mask; k[i] += ch[i]; }
     return k;
+  }
+  // computeTimeDerivativesCharacteristicBoundary (idealmhd.cpp:308-332): each open_moc side's
+  // normal + parallel terms summed into zero grids in side order, then the momentum form
+  Grids moc_boundary(const Grids& G, double vc) const {
+    A zero(pd.nx, pd.ny, 0.0);
+    Grids res(8, zero);
+    const int sides[4][2] = {{0, 1}, {0, 0}, {1, 1}, {1, 0}};  // (axis, lower) of x_bound_1 .. y_bound_2
+    for (int b = 0; b < 4; ++b) {
+      if (pd.bc[b] != OpenMoC) continue;
+      auto t = moc_side(G, sides[b][0], sides[b][1] != 0, vc);
+      for (int i = 0; i < 8; ++i) res[i] += t[i][0] + t[i][1];
+    }
+    A mx = G[rho] * res[1] + G[v_x] * res[0];
+    A my = G[rho] * res[2] + G[v_y] * res[0];
+    A mz = G[rho] * res[3] + G[v_z] * res[0];
+    res[1] = mx;
+    res[2] = my;
+    res[3] = mz;
+    return res;
+  }
+  // singleBoundaryTermsMOC (idealmhd.cpp:336-615): {normal, parallel} terms of one open_moc side for
+  // {rho, v_x, v_y, v_z, thermal_energy, b_x, b_y, b_z}
+  std::vector<std::array<A, 2>> moc_side(const Grids& G, int bi, bool lower, double vc) const {
+    const Domain& D = pd;
+    using R4 = Domain::R4;
+    const int NX = D.nx, NY = D.ny, NG = N_GHOST;
+    int xi[2], yi[2];
+    if (bi == 0) {
+      xi[0] = lower ? 0 : NX - NG; xi[1] = lower ? NG - 1 : NX - 1;
+      yi[0] = 0; yi[1] = NY - 1;
+    } else {
+      yi[0] = lower ? 0 : NY - NG; yi[1] = lower ? NG - 1 : NY - 1;
+      xi[0] = 0; xi[1] = NX - 1;
+    }
+    int xin[2] = {xi[0], xi[1]}, yin[2] = {yi[0], yi[1]}, xpd[2] = {xi[0], xi[1]}, ypd[2] = {yi[0], yi[1]};
+    int xlc[2] = {xi[0], xi[1]}, xuc[2] = {xi[0], xi[1]}, ylc[2] = {yi[0], yi[1]}, yuc[2] = {yi[0], yi[1]};
+    if (bi == 0 && D.bc[2] != Periodic) {
+      yin[0] = yi[0] + NG; yin[1] = yi[1] - NG;
+      ypd[0] = yi[0] + 1; ypd[1] = yi[1] - 1;
+      if (D.bc[2] != OpenMoC) { yi[0] = yin[0]; ypd[0] = yin[0]; }
+      if (D.bc[3] != OpenMoC) { yi[1] = yin[1]; ypd[1] = yin[1]; }
+      ylc[0] = yi[0]; ylc[1] = ypd[0] - 1;
+      yuc[0] = ypd[1] + 1; yuc[1] = yi[1];
+    } else if (bi == 1 && D.bc[0] != Periodic) {
+      xin[0] = xi[0] + NG; xin[1] = xi[1] - NG;
+      xpd[0] = xi[0] + 1; xpd[1] = xi[1] - 1;
+      if (D.bc[0] != OpenMoC) { xi[0] = xin[0]; xpd[0] = xin[0]; }
+      if (D.bc[1] != OpenMoC) { xi[1] = xin[1]; xpd[1] = xin[1]; }
+      xlc[0] = xi[0]; xlc[1] = xpd[0] - 1;
+      xuc[0] = xpd[1] + 1; xuc[1] = xi[1];
+    }
+    const R4 RB{xi[0], yi[0], xi[1], yi[1]}, RI{xin[0], yin[0], xin[1], yin[1]}, RP{xpd[0], ypd[0], xpd[1], ypd[1]};
+    const R4 RU{xuc[0], yuc[0], xuc[1], yuc[1]}, RL{xlc[0], ylc[0], xlc[1], ylc[1]};
+    A imask(NX, NY, 0.0);
+    for (int i = RI.il; i <= RI.iu; ++i) for (int j = RI.jl; j <= RI.ju; ++j) imask(i, j) = 1.0;
+
+    const int vperp = bi == 0 ? v_x : v_y, vpara = bi == 0 ? v_y : v_x, vgd = v_z;
+    const A& be_perp = D.pd[bi == 0 ? be_x : be_y];
+    const A& be_para = D.pd[bi == 0 ? be_y : be_x];
+    const A& be_gd = D.pd[be_z];
+    const A& bi_perp = G[bi == 0 ? bi_x : bi_y];
+    const A& bi_para = G[bi == 0 ? bi_y : bi_x];
+    const A& bi_gd = G[bi_z];
+    const A& g_perp = G[bi == 0 ? grav_x : grav_y];
+    const A& g_para = G[bi == 1 ? grav_x : grav_y];
+    const bool pf = !lower;
+    const A b_perp = be_perp + bi_perp, b_para = be_para + bi_para, b_gd = be_gd + bi_gd;
+    const A bh = vsqrt(sq(b_para) + sq(b_gd));
+
+    A s_perp(NX, NY, 0.0), R_para = s_perp, R_gd = s_perp, c_perp = s_perp, c_plus = s_perp, c_minus = s_perp;
+    A al_p = s_perp, al_m = s_perp;
+    A c_s(NX, NY, 1.0), c_s_sq(NX, NY, 1.0);
+    for (int i = RB.il; i <= RB.iu; ++i)
+      for (int j = RB.jl; j <= RB.ju; ++j) {
+        s_perp(i, j) = b_perp(i, j) > 0.0 ? 1.0 : -1.0;
+        R_para(i, j) = b_para(i, j) / bh(i, j);
+        R_gd(i, j) = b_gd(i, j) / bh(i, j);
+        const double r = G[rho](i, j);
+        const double css = pd.gamma * G[press](i, j) / r;
+        const double cas = G[b_mag](i, j) * G[b_mag](i, j) / (4.0 * PI * r);
+        const double cps = b_perp(i, j) * b_perp(i, j) / (4.0 * PI * r);
+        const double sum = cas + css;
+        const double root = std::sqrt(std::fabs(sum * sum / 4.0 - cps * css));
+        const double cpl = sum / 2.0 + root, cmi = sum / 2.0 - root;
+        const double aps = std::max((css - cmi) / (cpl - cmi), 0.0), ams = std::max((cpl - css) / (cpl - cmi), 0.0);
+        c_s_sq(i, j) = css;
+        c_s(i, j) = std::sqrt(css);
+        c_perp(i, j) = std::sqrt(cps);
+        c_plus(i, j) = std::sqrt(cpl);
+        c_minus(i, j) = std::sqrt(cmi);
+        al_p(i, j) = std::sqrt(aps);
+        al_m(i, j) = std::sqrt(ams);
+      }
+    for (size_t c = 0; c < s_perp.size(); ++c) if (b_perp.v[c] == 0.0) s_perp.v[c] = 0.0;
+
+    auto cb = [&](const A& q) { return D.char_barton(q, pf, bi, RB); };
+    const A b_para_grad = cb(be_para) + cb(bi_para);
+    const A b_gd_grad = cb(be_gd) + cb(bi_gd);
+    const A v_para_grad = cb(G[vpara]), v_perp_grad = cb(G[vperp]), v_gd_grad = cb(G[vgd]), p_grad = cb(G[press]);
+    const A& vp = G[vperp];
+    const A& RH = G[rho];
+    const double sg = bi == 0 ? 1.0 : -1.0;
+    const A sq4 = vsqrt(4.0 * PI * RH);
+    A d1 = vp * cb(b_perp);
+    A d2 = vp * ((G[thermal_energy] + G[press]) * cb(RH) - RH * cb(G[thermal_energy]));
+    A d3 = s_perp * sg * (vp + c_perp) *
+           (-s_perp * R_gd * v_para_grad + s_perp * R_para * v_gd_grad + R_gd / sq4 * b_para_grad - R_para / sq4 * b_gd_grad);
+    A d4 = s_perp * sg * (vp - c_perp) *
+           (s_perp * R_gd * v_para_grad - s_perp * R_para * v_gd_grad + R_gd / sq4 * b_para_grad - R_para / sq4 * b_gd_grad);
+    A d5 = (vp + c_plus) * (al_p / RH * p_grad - s_perp * R_para * c_minus * al_m * v_para_grad
+                            - s_perp * R_gd * c_minus * al_m * v_gd_grad + c_plus * al_p * v_perp_grad
+                            + R_para * c_s * al_m / sq4 * b_para_grad + R_gd * c_s * al_m / sq4 * b_gd_grad);
+    A d6 = (vp - c_plus) * (al_p / RH * p_grad + s_perp * R_para * c_minus * al_m * v_para_grad
+                            + s_perp * R_gd * c_minus * al_m * v_gd_grad - c_plus * al_p * v_perp_grad
+                            + R_para * c_s * al_m / sq4 * b_para_grad + R_gd * c_s * al_m / sq4 * b_gd_grad);
+    A d7 = (vp + c_minus) * (al_m / RH * p_grad + s_perp * R_para * c_plus * al_p * v_para_grad
+                             + s_perp * R_gd * c_plus * al_p * v_gd_grad + c_minus * al_m * v_perp_grad
+                             - R_para * c_s * al_p / sq4 * b_para_grad - R_gd * c_s * al_p / sq4 * b_gd_grad);
+    A d8 = (vp - c_minus) * (al_m / RH * p_grad - s_perp * R_para * c_plus * al_p * v_para_grad
+                             - s_perp * R_gd * c_plus * al_p * v_gd_grad - c_minus * al_m * v_perp_grad
+                             - R_para * c_s * al_p / sq4 * b_para_grad - R_gd * c_s * al_p / sq4 * b_gd_grad);
+    const int pa = bi == 0 ? 1 : 0;
+    // central derivative on the padded range plus the one-sided ones on the corner ranges
+    auto alt = [&](const A& q) { return D.deriv_r(q, pa, RP) + D.back1(q, true, pa, RU) + D.back1(q, false, pa, RL); };
+    auto alt2 = [&](const A& e, const A& in) {
+      return D.deriv_r(e, pa, RP) + D.back1(e, true, pa, RU) + D.back1(e, false, pa, RL) + D.deriv_r(in, pa, RP) +
+             D.back1(in, true, pa, RU) + D.back1(in, false, pa, RL);
+    };
+    const A bps_alt = 2.0 * b_perp * alt2(be_perp, bi_perp);
+    const A p_alt = alt(G[press]);
+    const A bp_alt = alt2(be_perp, bi_perp);
+    const A bg_alt = alt2(be_gd, bi_gd);
+    const A d3a = s_perp * s_perp * sg *
+                  ((-1.0 * imask * R_gd * g_para + (R_gd * (p_alt + bps_alt / (8.0 * PI)) + bh * (bg_alt) / (4.0 * PI)) / RH));
+    const A d5a = c_plus * al_p * (imask * g_perp + b_para * bp_alt / (4.0 * PI * RH)) +
+                  c_minus * al_m * s_perp * (-imask * R_para * g_para + R_para * (p_alt + bps_alt / (8.0 * PI)) / RH);
+    const A d7a = c_minus * al_m * (imask * g_perp + b_para * bp_alt / (4.0 * PI * RH)) -
+                  c_plus * al_p * s_perp * (-imask * R_para * g_para + R_para * (p_alt + bps_alt / (8.0 * PI)) / RH);
+    const int in_dir = lower ? 1 : -1;
+    for (int i = RB.il; i <= RB.iu; ++i)
+      for (int j = RB.jl; j <= RB.ju; ++j) {
+        const double v = vp(i, j);
+        if (in_dir * v > 0.0) { d1(i, j) = 0.0; d2(i, j) = 0.0; }
+        if (in_dir * (v + c_perp(i, j)) > 0.0) d3(i, j) = d3a(i, j);
+        if (in_dir * (v - c_perp(i, j)) > 0.0) d4(i, j) = -d3a(i, j);
+        if (in_dir * (v + c_plus(i, j)) > 0.0) d5(i, j) = d5a(i, j);
+        if (in_dir * (v - c_plus(i, j)) > 0.0) d6(i, j) = -d5a(i, j);
+        if (in_dir * (v + c_minus(i, j)) > 0.0) d7(i, j) = d7a(i, j);
+        if (in_dir * (v - c_minus(i, j)) > 0.0) d8(i, j) = -d7a(i, j);
+      }
+    const A& TE = G[thermal_energy];
+    const A& P = G[press];
+    const A sqp = vsqrt(PI * RH);
+    std::array<A, 2> rho_t, vpara_t, vgd_t, vperp_t, te_t, bpara_t, bgd_t, bperp_t;
+    rho_t[0] = -(pd.gamma / RH * d2 + 0.5 * RH * al_p * (d5 + d6) + 0.5 * RH * al_m * (d7 + d8)) / c_s_sq;
+    vpara_t[0] = -0.5 * s_perp * (sg * R_gd * (-d3 + d4) + (c_minus * al_m / c_s_sq * R_para * (-d5 + d6) +
+                                                           c_plus * al_p / c_s_sq * R_para * (d7 - d8))) +
+                 vc * D.second_back(G[vpara], pf, bi, RB);
+    vgd_t[0] = -0.5 * s_perp * (sg * R_para * (d3 - d4) + (c_minus * al_m / c_s_sq * R_gd * (-d5 + d6) +
+                                                          c_plus * al_p / c_s_sq * R_gd * (d7 - d8))) +
+               vc * D.second_back(G[vgd], pf, bi, RB);
+    vperp_t[0] = -0.5 / c_s_sq * (c_plus * al_p * (d5 - d6) + c_minus * al_m * (d7 - d8)) +
+                 vc * D.second_back(G[vperp], pf, bi, RB);
+    te_t[0] = -(0.5 * (TE + P) * al_p * (d5 + d6) + 0.5 * (TE + P) * al_m * (d7 + d8)) / c_s_sq;
+    bpara_t[0] = -sqp * (sg * R_gd * (d3 + d4) + al_m / c_s * R_para * (d5 + d6) - al_p / c_s * R_para * (d7 + d8));
+    bgd_t[0] = -sqp * (sg * -1.0 * R_para * (d3 + d4) + al_m / c_s * R_gd * (d5 + d6) - al_p / c_s * R_gd * (d7 + d8));
+    bperp_t[0] = -d1;
+
+    const A& VA = G[vpara];
+    auto dI = [&](const A& q) { return D.deriv_r(q, pa, RI); };
+    auto tI = [&](const A& q) { return D.tderiv_r(q, VA, pa, RI); };
+    rho_t[1] = -tI(RH);
+    vpara_t[1] = -1.0 / RH * (dI(P) + 2.0 * b_para / (8.0 * PI) * (dI(be_para) + dI(bi_para)) +
+                              2.0 * b_perp / (8.0 * PI) * (dI(be_perp) + dI(bi_perp)) +
+                              2.0 * b_gd / (8.0 * PI) * (dI(be_gd) + dI(bi_gd))) -
+                 (tI(RH * VA) - VA * tI(RH)) / RH + b_para / (4.0 * PI * RH) * (dI(be_para) + dI(bi_para)) +
+                 imask * g_para + vc * D.second_r(VA, pa, RI);
+    vgd_t[1] = -(tI(RH * G[vgd]) - G[vgd] * tI(RH)) / RH + b_para / (4.0 * PI * RH) * (dI(be_gd) + dI(bi_gd)) +
+               vc * D.second_r(G[vgd], pa, RI);
+    vperp_t[1] = -(tI(RH * vp) - vp * tI(RH)) / RH + b_para / (4.0 * PI * RH) * (dI(be_perp) + dI(bi_perp)) +
+                 imask * g_perp + vc * D.second_r(vp, pa, RI);
+    te_t[1] = -tI(TE) - P * dI(VA);
+    bpara_t[1] = -(tI(be_para) + tI(bi_para)) + b_para * dI(VA);
+    bgd_t[1] = -(tI(be_gd) + tI(bi_gd)) + b_para * dI(G[vgd]);
+    bperp_t[1] = -(tI(be_perp) + tI(bi_perp)) + b_para * dI(vp);
+    if (bi == 0) return {rho_t, vperp_t, vpara_t, vgd_t, te_t, bperp_t, bpara_t, bgd_t};
+    return {rho_t, vpara_t, vperp_t, vgd_t, te_t, bpara_t, bperp_t, bgd_t};
+  }
+  // applyMomThresholdingMoC / applyBThresholdingMoC (idealmhd.cpp:107-224): clamp the open_moc ghost
+  // cells and the first interior cell to [lo, hi] times the next interior cell, side by side
+  static double clamp_ref(double x, double fl, double ce, double ref) {
+    return ref >= 0.0 ? std::min(std::max(x, fl), ce) : std::max(std::min(x, fl), ce);
+  }
+  void moc_threshold(Grids& G) const {
+    const int NX = pd.nx, NY = pd.ny, NG = N_GHOST;
+    if (moc_mom) {
+      for (int v : {mom_x, mom_y, mom_z}) {
+        A& q = G[v];
+        if (pd.bc[0] == OpenMoC)
+          for (int j = 0; j < NY; ++j) for (int k = 0; k < NG + 1; ++k) {
+            const double r = q(NG + 1, j);
+            q(k, j) = clamp_ref(q(k, j), moc_mom_lo * r, moc_mom_hi * r, r);
+          }
+        if (pd.bc[1] == OpenMoC)
+          for (int j = 0; j < NY; ++j) for (int k = 0; k < NG + 1; ++k) {
+            const double r = q(NX - 2 - NG, j);
+            q(NX - 1 - k, j) = clamp_ref(q(NX - 1 - k, j), moc_mom_lo * r, moc_mom_hi * r, r);
+          }
+        if (pd.bc[2] == OpenMoC)
+          for (int i = 0; i < NX; ++i) for (int k = 0; k < NG + 1; ++k) {
+            const double r = q(i, NG + 1);
+            q(i, k) = clamp_ref(q(i, k), moc_mom_lo * r, moc_mom_hi * r, r);
+          }
+        if (pd.bc[3] == OpenMoC)
+          for (int i = 0; i < NX; ++i) for (int k = 0; k < NG + 1; ++k) {
+            const double r = q(i, NY - 2 - NG);
+            q(i, NY - 1 - k) = clamp_ref(q(i, NY - 1 - k), moc_mom_lo * r, moc_mom_hi * r, r);
+          }
+      }
+    }
+    if (moc_b) {
+      const int bes[3] = {be_x, be_y, be_z}, bis[3] = {bi_x, bi_y, bi_z};
+      for (int c = 0; c < 3; ++c) {
+        const A& e = pd.pd[bes[c]];
+        A& q = G[bis[c]];
+        if (pd.bc[0] == OpenMoC)
+          for (int j = 0; j < NY; ++j) for (int k = 0; k < NG + 1; ++k) {
+            const double r = e(NG + 1, j) + q(NG + 1, j);
+            q(k, j) = clamp_ref(q(k, j), moc_b_lo * r - e(k, j), moc_b_hi * r - e(k, j), r);
+          }
+        if (pd.bc[1] == OpenMoC)
+          for (int j = 0; j < NY; ++j) for (int k = 0; k < NG + 1; ++k) {
+            const double r = e(NX - 2 - NG, j) + q(NX - 2 - NG, j);
+            q(NX - 1 - k, j) = clamp_ref(q(NX - 1 - k, j), moc_b_lo * r - e(NX - 1 - k, j), moc_b_hi * r - e(NX - 1 - k, j), r);
+          }
+        if (pd.bc[2] == OpenMoC)
+          for (int i = 0; i < NX; ++i) for (int k = 0; k < NG + 1; ++k) {
+            const double r = e(i, NG + 1) + q(i, NG + 1);
+            q(i, k) = clamp_ref(q(i, k), moc_b_lo * r - e(i, k), moc_b_hi * r - e(i, k), r);
+          }
+        if (pd.bc[3] == OpenMoC) {
+          // the reference indexes the reference column with the x dimension (idealmhd.cpp:158)
+          require(NX - 2 - NG < NY, "open_moc y_bound_2 with moc_b_limiting needs xdim - 4 < ydim (the reference reads outside the grid)");
+          for (int i = 0; i < NX; ++i) for (int k = 0; k < NG + 1; ++k) {
+            const double r = e(i, NX - 2 - NG) + q(i, NX - 2 - NG);
+            q(i, NY - 1 - k) = clamp_ref(q(i, NY - 1 - k), moc_b_lo * r - e(i, NY - 1 - k), moc_b_hi * r - e(i, NY - 1 - k), r);
+          }
+        }
+      }
+    }
   }
   void evolved_from_state(Grids& G) const override {  // idealmhd.cpp:226-232
     G[n] = vmax(G[rho] / pd.m_ion, pd.nmin);
@@ -545,6 +928,7 @@ struct IdealMHD : EqSet {
     G[thermal_energy] = vmax(G[thermal_energy], pd.temin);
   }
   void derived(Grids& G) const override {  // idealmhd.cpp:241-277
+    moc_threshold(G);
     G[n] = vmax(G[rho] / pd.m_ion, pd.nmin);
     G[rho] = G[n] * pd.m_ion;
     G[v_x] = G[mom_x] / G[rho];

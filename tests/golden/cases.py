@@ -302,6 +302,20 @@ def _cases() -> Dict[str, Tuple[Callable[[], StateFile], Callable[[], ConfigFile
                                                ("max_accel", "2e8"), ("falloff_length", "3e7"),
                                                ("boundary", "x_bound_2"), ("falloff_shape", "exp"),
                                                ("feather_length", "1.5e7"), ("field_aligned_mode", "true")])]))
+    # open_moc characteristic boundaries (idealmhd.cpp:107-615): one axis of them, one side with a
+    # closed neighbour axis, all four sides with the ghost-cell limiters and the global viscosity
+    c["moc_x_periodic_rk2"] = (lambda: random_mhd_state(22, 20, 111),
+                               lambda: _mhd(("open_moc", "open_moc", "periodic", "periodic"), "rk2", 5))
+    c["moc_y2_closed_euler"] = (lambda: random_mhd_state(20, 22, 112),
+                                lambda: _mhd(("fixed", "reflect", "open", "open_moc"), "euler", 5))
+    c["moc_all_limits_rk4"] = (lambda: random_mhd_state(20, 24, 113),
+                               lambda: _mhd(("open_moc",) * 4, "rk4", 4,
+                                            equation_set_options=[("moc_b_limiting", "true"), ("moc_mom_limiting", "true"),
+                                                     ("moc_b_lower_lim", "0.5"), ("moc_b_upper_lim", "1.5"),
+                                                     ("moc_mom_lower_lim", "0.2"), ("moc_mom_upper_lim", "3.0"),
+                                                     ("global_viscosity", "0.05")]))
+    c["moc_mixed_stretch_rk2"] = (lambda: random_mhd_state(22, 24, 114, stretch=True),
+                                  lambda: _mhd(("open_moc", "fixed", "open_moc", "reflect"), "rk2", 5))
     # single-fluid variants: two energies, and the conservative total-energy form
     E2 = ["rho", "mom_x", "mom_y", "i_thermal_energy", "e_thermal_energy", "bi_x", "bi_y", "i_temp", "e_temp",
           "press", "b_hat_x", "dt"]

@@ -8,7 +8,7 @@ For each case the reference executable runs in input mode with ``write_precision
 round-trips every double), and the frames after setup, after step 1 and after the last step are
 kept for the variables listed in the case's ``output_flags``.  The step sizes of every step are
 reconstructed exactly from the ``dt`` grid of the preceding frame,
-``dt_step = epsilon * min(dt over the interior)`` (evolution.cpp:62; grid.cpp:71-82).
+``dt_step = epsilon * min(dt over the dt bounds)`` (the interior, widened by the ghost cells of open_moc sides) (evolution.cpp:62; grid.cpp:71-82).
 
 A successful reference run ends in SIGABRT after its success message (evolution.cpp:54-56); that
 exit is accepted.  Outputs: ``tests/golden/<case>.npz`` holding the input texts and the vectors.
@@ -31,12 +31,17 @@ from spruce_amd.stateio import parse_config, read_out  # noqa: E402
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "run")
 
 
-def interior_bounds(cfg, xdim, ydim):
+def dt_bounds(cfg, xdim, ydim):
     bc = [cfg.settings[k] for k in ("x_bound_1", "x_bound_2", "y_bound_1", "y_bound_2")]
     xl = 0 if bc[0] == "periodic" else 2
     xu = xdim - 1 if bc[1] == "periodic" else xdim - 3
     yl = 0 if bc[2] == "periodic" else 2
     yu = ydim - 1 if bc[3] == "periodic" else ydim - 3
+    # open_moc sides take part in the step size (plasmadomain.cpp:155-159)
+    xl -= 2 if bc[0] == "open_moc" else 0
+    xu += 2 if bc[1] == "open_moc" else 0
+    yl -= 2 if bc[2] == "open_moc" else 0
+    yu += 2 if bc[3] == "open_moc" else 0
     return xl, xu, yl, yu
 
 
@@ -63,7 +68,7 @@ def run_case(name: str) -> str:
         out = read_out(os.path.join(out_dir, "mhd.out"))
     parsed = parse_config(config_text)
     eps = float(parsed.settings["epsilon"])
-    xl, xu, yl, yu = interior_bounds(parsed, st.xdim, st.ydim)
+    xl, xu, yl, yu = dt_bounds(parsed, st.xdim, st.ydim)
     nframes = len(out.frames)
     names = list(parsed.output_flags)
     dts = []
