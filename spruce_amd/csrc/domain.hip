@@ -27,6 +27,7 @@
 #include "sources.hpp"
 #include "adjoint.hpp"
 #include "adj_modules.hpp"
+#include "moc.hpp"
 #include "crpow.hpp"
 #include "tf_host.hpp"
 #include "e2_host.hpp"
@@ -180,13 +181,52 @@ class Domain : public DomainBase {
     // IdealMHD::parseEquationSetConfigs (idealmhd.cpp:12-40): keys other than the MoC/viscosity
     // options are an error; global_viscosity only acts on open_moc boundaries.
     for (size_t i = 0; i < k.size(); ++i) {
-      static const char* ok[] = {"global_viscosity", "viscosity_opt", "moc_b_limiting", "moc_mom_limiting",
-                                 "moc_b_lower_lim", "moc_b_upper_lim", "moc_mom_lower_lim", "moc_mom_upper_lim"};
-      bool found = false;
-      for (auto* s : ok) found = found || k[i] == s;
-      require(found, SPRUCE_E_ARG, k[i] + " is not recognized for this equation set.");
-      (void)v;
+      if (k[i] == "global_viscosity") gvisc = std::stod(v[i]);
+      else if (k[i] == "viscosity_opt") {}
+      else if (k[i] == "moc_b_limiting") lim.b = v[i] == "true" ? 1 : 0;
+      else if (k[i] == "moc_mom_limiting") lim.mom = v[i] == "true" ? 1 : 0;
+      else if (k[i] == "moc_b_lower_lim") {
+        lim.b_lo = std::stod(v[i]);
+        require(lim.b_lo <= 1.0, SPRUCE_E_ARG, "MoC B field lower threshold should be <=1.0");
+      } else if (k[i] == "moc_b_upper_lim") {
+        lim.b_hi = std::stod(v[i]);
+        require(lim.b_hi >= 1.0, SPRUCE_E_ARG, "MoC B field upper threshold should be >=1.0");
+      } else if (k[i] == "moc_mom_lower_lim") {
+        lim.mom_lo = std::stod(v[i]);
+        require(lim.mom_lo <= 1.0, SPRUCE_E_ARG, "MoC momentum threshold should be <=1.0");
+      } else if (k[i] == "moc_mom_upper_lim") {
+        lim.mom_hi = std::stod(v[i]);
+        require(lim.mom_hi >= 1.0, SPRUCE_E_ARG, "MoC momentum threshold should be >=1.0");
+      } else {
+        throw Error(SPRUCE_E_ARG, k[i] + " is not recognized for this equation set.");
+      }
     }
+  }
+  // open_moc boundaries (idealmhd.hpp:48, 59-64)
+  double gvisc = 0.0;
+  MocLimits lim{0, 0, 0.1, 10.0, 0.1, 10.0};
+  bool any_moc = false;
+  int moc_on[4] = {0, 0, 0, 0};
+  MocSide moc_sides[4];
+  DBuf mocbuf, moc_vc, moc_partial;
+  // the characteristic terms of set S into mocbuf (and the global viscosity coefficient first)
+  const double* moc_terms(const MhdSet& S) {
+    if (!any_moc) return nullptr;
+    MocArgs m{};
+    m.S = S; m.st = stat(); m.K = kc; m.M = mesh;
+    for (int b = 0; b < 4; ++b) m.side[b] = moc_sides[b];
+    m.out = mocbuf.p;
+    m.vc = nullptr;
+    if (gvisc != 0.0) {
+      timed("moc_visc", [&] { launch_moc_visc(S, stat(), kc, mesh, gvisc, moc_partial.p, moc_vc.p, err_flag, stream); });
+      m.vc = moc_vc.p;
+    }
+    timed("moc_rhs", [&] { launch_moc_rhs(m, stream); });
+    return mocbuf.p;
+  }
+  // the ghost-cell limiters at the start of recomputeDerivedVarsFromEvolvedVars (idealmhd.cpp:244-245)
+  void moc_limit(const MhdSet& G) {
+    if (any_moc && (lim.mom || lim.b)) timed("moc_limit", [&] { launch_moc_threshold(G, stat(), mesh, moc_on, lim, stream); });
   }
 
   std::vector<std::unique_ptr<GpuModule>> mods;
@@ -220,7 +260,16 @@ class Domain : public DomainBase {
     // computeIterationBounds (plasmadomain.cpp:138-161)
     require(nx > 2 * N_GHOST && ny > 2 * N_GHOST, SPRUCE_E_ARG, "Grid too small for ghost zones");
     const int* bc = desc.bc;
-    for (int k = 0; k < 4; ++k) require(bc[k] != SPRUCE_BC_OPEN_MOC, SPRUCE_E_UNSUPPORTED, "open_moc boundaries are not available in this build");
+    for (int k = 0; k < 4; ++k) {
+      moc_on[k] = bc[k] == SPRUCE_BC_OPEN_MOC ? 1 : 0;
+      any_moc = any_moc || moc_on[k];
+    }
+    require(!any_moc || mods.empty(), SPRUCE_E_UNSUPPORTED, "modules with open_moc boundaries are not available in this build");
+    require(!any_moc || !(lim.mom || lim.b) || (nx >= 8 && ny >= 8), SPRUCE_E_UNSUPPORTED,
+            "the open_moc limiters need xdim >= 8 and ydim >= 8 in this build");
+    // the y_bound_2 field limiter reads column xdim-4 (idealmhd.cpp:158): outside the grid unless xdim-4 < ydim
+    require(!(moc_on[3] && lim.b) || nx - 4 < ny, SPRUCE_E_ARG,
+            "moc_b_limiting on y_bound_2 reads column xdim-4, outside the grid when xdim-4 >= ydim");
     require((bc[0] == 0) == (bc[1] == 0) && (bc[2] == 0) == (bc[3] == 0), SPRUCE_E_ARG,
             "an axis must be periodic on both sides or on neither");
     xl = bc[0] == 0 ? 0 : N_GHOST;
@@ -259,6 +308,12 @@ class Domain : public DomainBase {
     mesh.y = ay.view(ny, yl, yu, yper);
     mesh.nx = nx;
     mesh.ny = ny;
+    // open_moc sides take part in the step size (plasmadomain.cpp:155-159)
+    mesh.x.dlo = xl - (moc_on[0] ? N_GHOST : 0);
+    mesh.x.dhi = xu + (moc_on[1] ? N_GHOST : 0);
+    mesh.y.dlo = yl - (moc_on[2] ? N_GHOST : 0);
+    mesh.y.dhi = yu + (moc_on[3] ? N_GHOST : 0);
+    for (int b = 0; b < 4; ++b) moc_sides[b] = moc_side(b, bc, nx, ny);
 
     kc.m = desc.ion_mass;
     kc.rm = 1.0 / desc.ion_mass;
@@ -322,9 +377,15 @@ class Domain : public DomainBase {
     hip_check(hipMalloc(&err_flag, sizeof(int)), "hipMalloc");
     hip_check(hipMemset(err_flag, 0, sizeof(int)), "hipMemset");
 
+    if (any_moc) {
+      mocbuf.alloc(8 * N);   // zero: cells outside the open_moc regions keep a zero term
+      moc_vc.alloc(1);
+      moc_partial.alloc(dt_blocks);
+    }
     // populateVariablesFromState (equationset.cpp:96-104)
     launch_mhd_populate(L.view(), scratch.p, kc, nx, ny, stream);
     launch_mhd_ghost(L.view(), kc, mesh, kinds_all, s1, s2, GHOST_NORMAL, stream);
+    moc_limit(L.view());
     finish_step(-1);
     sync_and_check();
     if (!mods.empty()) {
@@ -417,6 +478,7 @@ class Domain : public DomainBase {
     a.step = step_ptr ? step_ptr : step.p; a.step_mult = mult; a.mode = mode;
     a.write_pre = any_open ? 1 : 0;
     a.write_n = 1;
+    a.moc = moc_terms(S);
     if (mode == STAGE_STORE_K) a.kbuf = kout ? kout : kbuf[kslot].p;
     if (mode == STAGE_RK4_FINAL) { a.kbuf = kbuf[0].p; a.k2buf = kbuf[1].p; a.k3buf = kbuf[2].p; }
     return a;
@@ -463,22 +525,30 @@ class Domain : public DomainBase {
         launch_stage(stage_args(rec->W, L.view(), L.view(), 1.0, STAGE_STORE_K, 0, nullptr, rec->k2));
       } else {
         stage(L, L, W, 0.5, STAGE_PLAIN, 0);
-        ghost(W, kinds_ucnp, GHOST_NORMAL);            // live-grid ghost writes here are overwritten unread
+        ghost(W, kinds_ucnp, GHOST_NORMAL);
+        // the midpoint's live-grid ghost writes are overwritten unread, except where a fixed side
+        // zeroes momenta inside an open_moc side's ghost cells, which the final stage reads as its base
+        if (any_moc) ghost(L, kinds_live, GHOST_LIVE_RAW);
+        moc_limit(W.view());
         stage(W, L, L, 1.0, STAGE_PLAIN, 0);
       }
     } else {                                           // evolution.cpp:103-124
       stage(L, L, W, 0.5, STAGE_STORE_K, 0);
       ghost(W, kinds_ucnp, GHOST_NORMAL);
       ghost(L, kinds_live, GHOST_LIVE_RAW);            // open/fixed/reflect write the live grids
+      moc_limit(W.view());
       stage(W, L, W2, 0.5, STAGE_STORE_K, 1);
       ghost(W2, kinds_ucnp, GHOST_NORMAL);
       ghost(L, kinds_live, GHOST_LIVE_RAW);
+      moc_limit(W2.view());
       stage(W2, L, W, 1.0, STAGE_STORE_K, 2);
       ghost(W, kinds_ucnp, GHOST_NORMAL);
       ghost(L, kinds_live, GHOST_LIVE_RAW);
+      moc_limit(W.view());
       stage(W, L, L, 1.0, STAGE_RK4_FINAL, 0);
     }
     ghost(L, kinds_all, GHOST_NORMAL);
+    moc_limit(L.view());
     if (rec) copy_set(L.view(), rec->A3, N, stream);
     size_t np = 0;
     for (size_t k = 0; k < mods.size(); ++k) {
@@ -561,7 +631,7 @@ class Domain : public DomainBase {
     require(count == (int64_t)(8 * N), SPRUCE_E_ARG, "rhs output must hold nevolved*xdim*ydim doubles");
     DBuf tmp;
     tmp.alloc(8 * N);
-    launch_mhd_rhs_full(L.view(), stat(), kc, mesh, tmp.p, stream);
+    launch_mhd_rhs_full(L.view(), stat(), kc, mesh, tmp.p, stream, moc_terms(L.view()));
     for (auto& m : mods) m->before_integrator(*this);
     for (auto& m : mods) m->add_rhs(*this, L, tmp.p);
     hip_check(hipMemcpyAsync(out, tmp.p, 8 * N * 8, hipMemcpyDeviceToHost, stream), "download rhs");

@@ -53,7 +53,7 @@ struct AxisBufs {
   DBuf d, rd, h, hsq, rhsq, fden, rfden, gup, gdn;
   std::vector<double> hd;  // host copy of the cell sizes
   Axis view(int n, int lo, int hi, int per) const {
-    return Axis{d.p, rd.p, h.p, hsq.p, rhsq.p, fden.p, rfden.p, gup.p, gdn.p, n, lo, hi, per};
+    return Axis{d.p, rd.p, h.p, hsq.p, rhsq.p, fden.p, rfden.p, gup.p, gdn.p, n, lo, hi, per, lo, hi};
   }
   // Precomputes the geometry the stencils divide by; every reciprocal is a correctly rounded
   // IEEE division on the host.

@@ -27,6 +27,9 @@ __global__ void __launch_bounds__(256) k_mhd_stage(StageArgs a) {
   double k[8];
   if (interior(M, i, j)) {
     mhd_rhs_cell(a.S, a.st, a.k, M, i, j, k);
+  } else if (a.moc) {  // open_moc ghost cells: masked RHS (0) + characteristic terms (idealmhd.cpp:99-103)
+#pragma unroll
+    for (int v = 0; v < 8; ++v) k[v] = 0.0 + a.moc[v * N + c];
   } else {
 #pragma unroll
     for (int v = 0; v < 8; ++v) k[v] = 0.0;
@@ -226,10 +229,40 @@ __global__ void __launch_bounds__(256) k_mhd_dt(MhdSet L, MhdStatic st, MhdConst
   if (i < M.nx && j < M.ny) {
     const double dt = mhd_dt_cell(L, st, K, M, i, j);
     if (dt_grid) dt_grid[(size_t)i * M.ny + j] = dt;
-    if (interior(M, i, j)) v = (dt < v) ? dt : v;
+    if (i >= M.x.dlo && i <= M.x.dhi && j >= M.y.dlo && j <= M.y.dhi) v = (dt < v) ? dt : v;
   }
   v = block_min(v);
   if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(256) k_moc_visc_part(MhdSet S, MhdStatic st, MhdConst K, Mesh M,
+                                                       double* partial) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  double v = DBL_MAX;
+  if (i < M.nx && j < M.ny && i >= M.x.dlo && i <= M.x.dhi && j >= M.y.dlo && j <= M.y.dhi) {
+    const double dx = M.x.d[i], dy = M.y.d[j];
+    const double t = 1.0 / (1.0 / (dx * dx) + 1.0 / (dy * dy)) / mhd_dt_cell(S, st, K, M, i, j);
+    v = (t < v) ? t : v;
+  }
+  v = block_min(v);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(1024) k_moc_visc_final(const double* partial, int n, double gv, double* vc,
+                                                         int* err) {
+  double v = DBL_MAX;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) { const double o = partial[k]; v = (o < v) ? o : v; }
+  for (int off = 32; off > 0; off >>= 1) { const double o = __shfl_xor(v, off, 64); v = (o < v) ? o : v; }
+  __shared__ double part[16];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = part[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = (part[k] < m) ? part[k] : m;
+    if (!(m < DBL_MAX)) *err = 1;   // Grid::min asserts (grid.cpp:80)
+    *vc = gv * 0.5 * m;
+  }
 }
 
 __global__ void __launch_bounds__(1024) k_reduce_step(const double* partial, int n, double eps, double* step,
@@ -250,13 +283,15 @@ __global__ void __launch_bounds__(1024) k_reduce_step(const double* partial, int
   }
 }
 
-__global__ void __launch_bounds__(256) k_mhd_rhs_full(MhdSet S, MhdStatic st, MhdConst K, Mesh M, double* out) {
+__global__ void __launch_bounds__(256) k_mhd_rhs_full(MhdSet S, MhdStatic st, MhdConst K, Mesh M, double* out,
+                                                     const double* moc) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= M.nx || j >= M.ny) return;
   const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
   double k[8];
   if (interior(M, i, j)) mhd_rhs_cell(S, st, K, M, i, j, k);
+  else if (moc) for (int v = 0; v < 8; ++v) k[v] = 0.0 + moc[v * N + c];
   else for (int v = 0; v < 8; ++v) k[v] = 0.0;
   for (int v = 0; v < 8; ++v) out[v * N + c] = k[v];
 }
@@ -338,6 +373,14 @@ void launch_mhd_dt(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double* dt_gri
   hipLaunchKernelGGL(k_mhd_dt, grid2d(mesh.nx, mesh.ny, b), b, 0, s, L, st, k, mesh, dt_grid, partial);
 }
 
+void launch_moc_visc(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, double gv, double* partial, double* vc, int* err,
+                     hipStream_t s) {
+  dim3 b(64, 4);
+  const dim3 g = grid2d(mesh.nx, mesh.ny, b);
+  hipLaunchKernelGGL(k_moc_visc_part, g, b, 0, s, S, st, k, mesh, partial);
+  hipLaunchKernelGGL(k_moc_visc_final, dim3(1), dim3(1024), 0, s, (const double*)partial, (int)(g.x * g.y), gv, vc, err);
+}
+
 void launch_gather_scalars(const double* scal, int n, const double* step, const int* err, double* host_out,
                            hipStream_t s) {
   hipLaunchKernelGGL(k_gather_scalars, dim3(1), dim3(((n + 2 + 63) / 64) * 64), 0, s, scal, n, step, err, host_out);
@@ -348,9 +391,10 @@ void launch_reduce_step(const double* partial, int n, double eps, double* step, 
   hipLaunchKernelGGL(k_reduce_step, dim3(1), dim3(1024), 0, s, partial, n, eps, step, history, idx, err);
 }
 
-void launch_mhd_rhs_full(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, double* out, hipStream_t s) {
+void launch_mhd_rhs_full(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, double* out, hipStream_t s,
+                         const double* moc) {
   dim3 b(64, 4);
-  hipLaunchKernelGGL(k_mhd_rhs_full, grid2d(mesh.nx, mesh.ny, b), b, 0, s, S, st, k, mesh, out);
+  hipLaunchKernelGGL(k_mhd_rhs_full, grid2d(mesh.nx, mesh.ny, b), b, 0, s, S, st, k, mesh, out, moc);
 }
 
 void launch_mhd_derived(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, int which, double* out, hipStream_t s) {

@@ -18,6 +18,7 @@ struct Axis {
   const double* gup;    // face c: h[c-2] + 2 h[c-1]   (derivs.cpp:499, extrapolation from below)
   const double* gdn;    // face c: h[c+1] + 2 h[c]     (extrapolation from above)
   int n, lo, hi, per;   // cells, interior bounds, periodic on both sides
+  int dlo, dhi;         // bounds of the step-size minimum (plasmadomain.cpp:155-159)
 };
 
 // Per-cell state of one IdealMHD grid set.  rho and n are both kept because after
@@ -77,6 +78,9 @@ struct StageArgs {
   const double* extra[4];
   int extra_var[4];
   int n_extra;
+  // open_moc characteristic terms (8 * nx*ny, evolved order), added to the ghost cells' RHS; null
+  // without open_moc sides
+  const double* moc;
 };
 constexpr int MAX_RHS_TERMS = 4;
 
@@ -95,12 +99,19 @@ void launch_mhd_ghost(MhdSet G, MhdConst k, Mesh mesh, const int* kinds, const d
 // per block (NaN dropped as std::min does); mhd_dt_blocks() blocks are launched.
 int mhd_dt_blocks(const Mesh& m);
 void launch_mhd_dt(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double* dt_grid, double* partial, hipStream_t s);
+// global_visc_coeff of IdealMHD::computeTimeDerivativesDerived (idealmhd.cpp:90):
+// *vc = (global_viscosity * 0.5) * min over the dt bounds of (1/(1/dx^2 + 1/dy^2)) / dt(S); *err = 1
+// when the minimum is not finite.  partial holds mhd_dt_blocks() values.
+void launch_moc_visc(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, double gv, double* partial, double* vc, int* err,
+                     hipStream_t s);
 // out[0..n) = scal, out[n] = *step, out[n+1] = *err (out: host memory from hipHostMalloc)
 void launch_gather_scalars(const double* scal, int n, const double* step, const int* err, double* host_out,
                            hipStream_t s);
 void launch_reduce_step(const double* partial, int n, double eps, double* step, double* history, int idx,
                         int* err, hipStream_t s);
-void launch_mhd_rhs_full(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, double* out, hipStream_t s);
+// with moc non-null the open_moc characteristic terms are added to the ghost cells (idealmhd.cpp:99-103)
+void launch_mhd_rhs_full(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, double* out, hipStream_t s,
+                         const double* moc = nullptr);
 // Derived variable `which` (see kDerived in domain.cpp) of a set, written to out.
 void launch_mhd_derived(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, int which, double* out, hipStream_t s);
 

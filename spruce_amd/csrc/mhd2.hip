@@ -219,6 +219,9 @@ k_mhd_stage_strip(StageArgs a, int R, int nstrips) {
       k8[5] = (-tdiv[Q_BIX] - tdiv[Q_BEX] + (bix + bex) * dvx_x + (biy + bey) * dvx_y) + 0.0;
       k8[6] = (-tdiv[Q_BIY] - tdiv[Q_BEY] + (bix + bex) * dvy_x + (biy + bey) * dvy_y) + 0.0;
       k8[7] = (-tdiv[Q_BIZ] - tdiv[Q_BEZ] + (bix + bex) * dvz_x + (biy + bey) * dvz_y) + 0.0;
+    } else if (a.moc) {  // open_moc ghost cells: masked RHS (0) + characteristic terms (idealmhd.cpp:99-103)
+#pragma unroll
+      for (int w = 0; w < 8; ++w) k8[w] = 0.0 + a.moc[w * N + c];
     } else {
 #pragma unroll
       for (int w = 0; w < 8; ++w) k8[w] = 0.0;

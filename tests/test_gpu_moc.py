@@ -1,0 +1,118 @@
+"""GPU parity for open_moc characteristic boundaries (idealmhd.cpp:107-615): the ghost cells of
+open_moc sides evolve by the characteristic equations instead of being filled from the interior,
+and take part in the step size.
+
+Every case is bit-identical to the reference golden vectors (grids after set-up, after step 1 and
+after the last step, and every step size), and to the CPU restatement step by step, with both
+stage kernels; on 133x197 stretched meshes too.  The characteristic terms need no power of a grid,
+so no tolerance is involved.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from golden_util import load_golden
+from oracle.oracle import OracleDomain
+from spruce_amd import workloads as W
+from spruce_amd.gpu import GpuDomain, SpruceError
+from spruce_amd.spec import spec_from_files
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+from cases import random_mhd_state  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["moc_x_periodic_rk2", "moc_y2_closed_euler", "moc_all_limits_rk4", "moc_mixed_stretch_rk2"]
+VARS = ["rho", "temp", "mom_x", "mom_y", "mom_z", "bi_x", "bi_y", "bi_z", "n", "thermal_energy", "press", "dt"]
+
+
+def same_bits(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return a.shape == b.shape and (np.array_equal(a.view(np.int64), b.view(np.int64)) or np.array_equal(a, b, equal_nan=True))
+
+
+def rel_linf(a, b):
+    scale = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / scale) if scale > 0 else float(np.max(np.abs(a - b)))
+
+
+@pytest.mark.parametrize("kernel", ["strip", "cell"])
+@pytest.mark.parametrize("case", CASES)
+def test_moc_matches_reference_golden_bitwise(case, kernel):
+    g = load_golden(case)
+    dom = GpuDomain(g.spec(), options={"stage_kernel": kernel})
+    try:
+        for name, ref in g.frame(0).items():
+            assert same_bits(dom.grid(name), ref), "%s setup %s" % (case, name)
+        step = 0
+        for k in g.frames[1:]:
+            dts = dom.step(k - step)
+            assert same_bits(dts, g.dt_steps[step:k]), "%s dt %s vs %s" % (case, dts, g.dt_steps[step:k])
+            step = k
+            for name, ref in g.frame(k).items():
+                got = dom.grid(name)
+                assert same_bits(got, ref), "%s frame %d %s: rel %.3e" % (case, k, name, rel_linf(got, ref))
+    finally:
+        dom.close()
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_moc_matches_restatement_each_step(case):
+    spec = load_golden(case).spec()
+    gpu = GpuDomain(spec)
+    ora = OracleDomain(spec)
+    try:
+        assert same_bits(gpu.rhs(), ora.rhs()), "rhs at set-up"
+        for s in range(5):
+            dg = gpu.step(1)
+            do = ora.step(1)
+            assert same_bits(dg, do), "step %d dt %r vs %r" % (s, dg, do)
+            for v in VARS:
+                assert same_bits(gpu.grid(v), ora.grid(v)), "step %d %s: rel %.3e" % (s, v, rel_linf(gpu.grid(v), ora.grid(v)))
+    finally:
+        gpu.close()
+        ora.close()
+
+
+def _big_spec(bc, integ, seed, eq_opts=()):
+    st = random_mhd_state(133, 197, seed, stretch=True)
+    cfg = W.base_config("ideal_mhd", bc, integ, epsilon=0.2, density_min=1.0, temp_min=1.0,
+                        thermal_energy_min=1e-12, max_iterations=3, equation_set_options=list(eq_opts))
+    return spec_from_files(cfg, st)
+
+
+BIG = [
+    ("all_limits_rk2", ("open_moc",) * 4, "rk2",
+     [("moc_b_limiting", "true"), ("moc_mom_limiting", "true"), ("global_viscosity", "0.1")]),
+    ("x_moc_y_fixed_rk4", ("open_moc", "open_moc", "fixed", "open"), "rk4", []),
+    ("y_moc_x_periodic_euler", ("periodic", "periodic", "open_moc", "open_moc"), "euler",
+     [("moc_mom_limiting", "true"), ("moc_mom_lower_lim", "0.5")]),
+]
+
+
+@pytest.mark.parametrize("name,bc,integ,opts", BIG, ids=[b[0] for b in BIG])
+def test_moc_multi_block_matches_restatement(name, bc, integ, opts):
+    spec = _big_spec(bc, integ, 95, opts)
+    gpu = GpuDomain(spec)
+    ora = OracleDomain(spec)
+    try:
+        for s in range(3):
+            dg = gpu.step(1)
+            do = ora.step(1)
+            assert same_bits(dg, do), "step %d dt %r vs %r" % (s, dg, do)
+        for v in VARS:
+            assert same_bits(gpu.grid(v), ora.grid(v)), "%s %s: rel %.3e" % (name, v, rel_linf(gpu.grid(v), ora.grid(v)))
+    finally:
+        gpu.close()
+        ora.close()
+
+
+def test_moc_with_modules_is_refused():
+    g = load_golden("moc_x_periodic_rk2")
+    spec = g.spec()
+    spec.modules = [("ambient_heating", [("heating_rate", "1.0")])]
+    with pytest.raises(SpruceError):
+        GpuDomain(spec)

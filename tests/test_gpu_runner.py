@@ -24,7 +24,8 @@ RUN = os.path.join(ROOT, "spruce_amd", "spruce_run")
 pytestmark = pytest.mark.gpu
 
 EXACT = ["mhd_periodic_rk2", "mhd_open_rk4", "mhd_mixed_euler", "mhd_reflect_rk2", "tf_ucnp_rk2",
-         "tf_curl_periodic_rk4", "visc_hyper_rk2", "heat_const_euler", "e2_open_rk4"]
+         "tf_curl_periodic_rk4", "visc_hyper_rk2", "heat_const_euler", "e2_open_rk4", "moc_mixed_stretch_rk2",
+         "moc_all_limits_rk4"]
 
 
 def _inputs(tmp, name, **settings):
