@@ -264,7 +264,6 @@ class Domain : public DomainBase {
       moc_on[k] = bc[k] == SPRUCE_BC_OPEN_MOC ? 1 : 0;
       any_moc = any_moc || moc_on[k];
     }
-    require(!any_moc || mods.empty(), SPRUCE_E_UNSUPPORTED, "modules with open_moc boundaries are not available in this build");
     require(!any_moc || !(lim.mom || lim.b) || (nx >= 8 && ny >= 8), SPRUCE_E_UNSUPPORTED,
             "the open_moc limiters need xdim >= 8 and ydim >= 8 in this build");
     // the y_bound_2 field limiter reads column xdim-4 (idealmhd.cpp:158): outside the grid unless xdim-4 < ydim
@@ -413,6 +412,7 @@ class Domain : public DomainBase {
   void propagate_live() {
     timed("propagate_live", [&] { launch_propagate_live(L.view(), kc, nx, ny, any_open ? 1 : 0, stream); });
     ghost(L, kinds_all, GHOST_NORMAL);
+    moc_limit(L.view());
   }
 
   // ModuleHandler pre/iterate loops (modulehandler.cpp:34-50; evolution.cpp:65-66).
@@ -1879,7 +1879,8 @@ struct ResistModule : GpuModule {
       compute_diffusion(d, work[0].p, work[0].p, work[2].p);   // the reference passes bi_x for bi_y (:107)
       launch_ar_coeff(tmpl.p, diff.p, d.mesh, coeff.p, td.p, d.stream);
       // computeNumSubcycles (:105-122)
-      launch_ar_scales(td.p, d.mesh, d.xl, d.xu, d.yl, d.yu, part.p, scal.p, d.stream);
+      // the time scale's minimum runs over the dt bounds (anomalousresistivity.cpp:191)
+      launch_ar_scales(td.p, d.mesh, d.mesh.x.dlo, d.mesh.x.dhi, d.mesh.y.dlo, d.mesh.y.dhi, part.p, scal.p, d.stream);
       launch_mhd_dt(d.L.view(), d.stat(), d.kc, d.mesh, nullptr, d.partial.p, d.stream);
       launch_reduce_step(d.partial.p, d.dt_blocks, d.kc.eps, scal.p + 2, nullptr, 0, d.err_flag, d.stream);
     });
@@ -1963,6 +1964,14 @@ struct OutflowModule : PostSourceModule {
   }
   bool needs_host_dt() const override { return false; }  // the step size is read on the device
   bool xside() const { return p.side == OF_X1 || p.side == OF_X2; }
+  // the interior, extended to the dt bound on the outflow side (:80-86, :144-149)
+  void bounds(const Domain& d, int& xl, int& xu, int& yl, int& yu) const {
+    xl = d.xl; xu = d.xu; yl = d.yl; yu = d.yu;
+    if (p.side == OF_X1) xl = d.mesh.x.dlo;
+    else if (p.side == OF_X2) xu = d.mesh.x.dhi;
+    else if (p.side == OF_Y2) yu = d.mesh.y.dhi;
+    else yl = d.mesh.y.dlo;
+  }
   // constructBoundaryAccel (:78-138): depends on positions only, built once with the C library's exp
   std::vector<double> build(const Domain& d) const {
     const auto& x = d.host_grids.at("pos_x");
@@ -1970,12 +1979,14 @@ struct OutflowModule : PostSourceModule {
     auto lo = [](const std::vector<double>& a) { double m = a[0]; for (double e : a) m = (e < m) ? e : m; return m; };
     auto hi = [](const std::vector<double>& a) { double m = a[0]; for (double e : a) m = (m < e) ? e : m; return m; };
     const double xmin = lo(x), xmax = hi(x), ymin = lo(y), ymax = hi(y), len = fall, fe = feather;
+    int bxl, bxu, byl, byu;
+    bounds(d, bxl, bxu, byl, byu);
     std::vector<double> r(d.N, 0.0);
     for (int i = 0; i < d.nx; ++i)
       for (int j = 0; j < d.ny; ++j) {
         const size_t c = (size_t)i * d.ny + j;
         const double X = x[c], Y = y[c];
-        const bool in = i >= d.xl && i <= d.xu && j >= d.yl && j <= d.yu;
+        const bool in = i >= bxl && i <= bxu && j >= byl && j <= byu;
         double t = 0.0;
         if (shape == "exp") {
           if (p.side == OF_X1) t = std::exp(-2.3 * (X - xmin) / len);
@@ -2007,7 +2018,8 @@ struct OutflowModule : PostSourceModule {
     const auto& y = d.host_grids.at("pos_y");
     auto X = [&](int i) { return x[(size_t)i * d.ny]; };
     auto Y = [&](int j) { return y[(size_t)j]; };
-    int xl = d.xl, xu = d.xu, yl = d.yl, yu = d.yu;
+    int xl, xu, yl, yu;
+    bounds(d, xl, xu, yl, yu);
     if (xside()) {
       for (int j = yl; j <= yu; ++j) if (Y(j) - Y(yl) >= feather) { yl = j; break; }
       for (int j = yu; j >= yl; --j) if (Y(yu) - Y(j) >= feather) { yu = j; break; }

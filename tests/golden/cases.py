@@ -316,6 +316,23 @@ def _cases() -> Dict[str, Tuple[Callable[[], StateFile], Callable[[], ConfigFile
                                                      ("global_viscosity", "0.05")]))
     c["moc_mixed_stretch_rk2"] = (lambda: random_mhd_state(22, 24, 114, stretch=True),
                                   lambda: _mhd(("open_moc", "fixed", "open_moc", "reflect"), "rk2", 5))
+    # open_moc sides with modules: conduction and radiation on an arcade whose top evolves by the
+    # characteristic equations; artificial viscosity and boundary outflow (both read the dt bounds);
+    # anomalous resistivity (its time scale is a minimum over the dt bounds)
+    c["moc_solar_tc_rl_rk2"] = (lambda: mini_arcade(20, 28, 31),
+                                lambda: _solar(("periodic", "periodic", "fixed", "open_moc"), "rk2", 4,
+                                               [_tc(wf="30"), _rl(eps="1e-3")]))
+    c["moc_visc_outflow_rk2"] = (lambda: random_mhd_state(20, 24, 121),
+                                 lambda: _mhd(("open", "open", "fixed", "open_moc"), "rk2", 5, modules=[
+                                     _visc("global", "0.1", "v_x", "mom_x", "0", "i"),
+                                     ("boundary_outflow", [("max_accel", "5e7"), ("falloff_length", "4e7"),
+                                                           ("boundary", "y_bound_2"), ("falloff_shape", "flat")])]))
+    c["moc_ar_frob_euler"] = (lambda: random_mhd_state(22, 20, 122),
+                              lambda: _mhd(("open_moc", "open_moc", "periodic", "periodic"), "euler", 4, modules=[
+                                  ("anomalous_resistivity", [
+                                      ("resistivity_model", "time_scale"), ("time_scale", "2e-2"),
+                                      ("template_mode", "frobenius"), ("frobenius_metric_coeff", "1e41"),
+                                      ("smoothing_sigma", "0.6")])]))
     # single-fluid variants: two energies, and the conservative total-energy form
     E2 = ["rho", "mom_x", "mom_y", "i_thermal_energy", "e_thermal_energy", "bi_x", "bi_y", "i_temp", "e_temp",
           "press", "b_hat_x", "dt"]

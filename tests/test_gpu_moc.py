@@ -5,7 +5,9 @@ and take part in the step size.
 Every case is bit-identical to the reference golden vectors (grids after set-up, after step 1 and
 after the last step, and every step size), and to the CPU restatement step by step, with both
 stage kernels; on 133x197 stretched meshes too.  The characteristic terms need no power of a grid,
-so no tolerance is involved.
+so no tolerance is involved; modules beside open_moc sides are checked the same way, those that
+raise grids to powers within 1e-10 of the reference and bit for bit against the restatement with
+correctly rounded powers.
 """
 import os
 import sys
@@ -14,7 +16,7 @@ import numpy as np
 import pytest
 
 from golden_util import load_golden
-from oracle.oracle import OracleDomain
+from oracle.oracle import OracleDomain, set_pow_mode
 from spruce_amd import workloads as W
 from spruce_amd.gpu import GpuDomain, SpruceError
 from spruce_amd.spec import spec_from_files
@@ -110,9 +112,64 @@ def test_moc_multi_block_matches_restatement(name, bc, integ, opts):
         ora.close()
 
 
-def test_moc_with_modules_is_refused():
-    g = load_golden("moc_x_periodic_rk2")
-    spec = g.spec()
-    spec.modules = [("ambient_heating", [("heating_rate", "1.0")])]
+def test_moc_field_limiter_outside_the_grid_is_refused():
+    """The y_bound_2 field limiter reads column xdim-4 (idealmhd.cpp:158): with xdim-4 >= ydim the
+    reference reads outside the grid, so set-up refuses the case."""
+    st = random_mhd_state(30, 20, 96)
+    cfg = W.base_config("ideal_mhd", ("open_moc",) * 4, "rk2", epsilon=0.2, density_min=1.0, temp_min=1.0,
+                        thermal_energy_min=1e-12, max_iterations=1,
+                        equation_set_options=[("moc_b_limiting", "true")])
     with pytest.raises(SpruceError):
-        GpuDomain(spec)
+        GpuDomain(spec_from_files(cfg, st))
+
+
+def test_moc_with_viscosity_and_outflow_matches_reference_golden_bitwise():
+    """Artificial viscosity (global option: the minimum of dt over the dt bounds) and boundary
+    outflow (its window and mask reach the dt bound on its side) beside an open_moc side."""
+    test_moc_matches_reference_golden_bitwise("moc_visc_outflow_rk2", "strip")
+    spec = load_golden("moc_visc_outflow_rk2").spec()
+    gpu = GpuDomain(spec)
+    ora = OracleDomain(spec)
+    try:
+        for s in range(5):
+            assert same_bits(gpu.step(1), ora.step(1)), s
+        for v in VARS:
+            assert same_bits(gpu.grid(v), ora.grid(v)), v
+    finally:
+        gpu.close()
+        ora.close()
+
+
+@pytest.mark.parametrize("case", ["moc_solar_tc_rl_rk2", "moc_ar_frob_euler"])
+def test_moc_with_pow_modules(case):
+    """Conduction and radiation (pow of T), anomalous resistivity (x^1.5) beside open_moc sides:
+    within 1e-10 of the reference (libm pow), bit for bit against the restatement with correctly
+    rounded powers, subcycle counts included."""
+    g = load_golden(case)
+    dom = GpuDomain(g.spec())
+    try:
+        step = 0
+        for k in g.frames[1:]:
+            dts = dom.step(k - step)
+            ref = g.dt_steps[step:k]
+            assert np.max(np.abs(dts - ref) / ref) <= 1e-12, (dts, ref)
+            step = k
+            for name, want in g.frame(k).items():
+                assert rel_linf(dom.grid(name), want) <= 1e-10, "frame %d %s: %.3e" % (k, name, rel_linf(dom.grid(name), want))
+    finally:
+        dom.close()
+    set_pow_mode(1)
+    gpu = GpuDomain(g.spec())
+    ora = OracleDomain(g.spec())
+    try:
+        for s in range(4):
+            dg, sg = gpu.step(1, subcycles=True)
+            do = ora.step(1)
+            assert list(sg[0]) == ora.subcycles(), (s, sg[0], ora.subcycles())
+            assert same_bits(dg, do), (s, dg, do)
+        for v in VARS:
+            assert same_bits(gpu.grid(v), ora.grid(v)), v
+    finally:
+        set_pow_mode(0)
+        gpu.close()
+        ora.close()

@@ -97,7 +97,8 @@ def test_runner_output_schedule_and_continue_mode(tmp_path):
 
 
 SOURCE_MODULES = ["lheat_ramp_rk2", "mass_inj_rk4", "mom_inj_osc_euler", "sink_exp_heat_rk2", "divclean_rk2",
-                  "ar_flood_time_euler", "ar_flood_masks_rk2", "outflow_gauss_x1_dyn_euler", "outflow_flat_y1_rk4"]
+                  "ar_flood_time_euler", "ar_flood_masks_rk2", "outflow_gauss_x1_dyn_euler", "outflow_flat_y1_rk4",
+                  "moc_visc_outflow_rk2"]
 
 
 @pytest.mark.parametrize("name", SOURCE_MODULES)
