@@ -816,8 +816,9 @@ struct TcModule : GpuModule {
       }
     }
     if (inactive) return;
-    if (cur != d.L.te.p)
-      hip_check(hipMemcpyAsync(d.L.te.p, cur, d.N * 8, hipMemcpyDeviceToDevice, d.stream), "copy te");
+    // the last subcycle's buffer becomes the live thermal energy (no copy)
+    for (DBuf* b : {&ta, &tb, &tc})
+      if (b->p == cur && cur != d.L.te.p) { d.L.te.swap(*b); break; }
     d.propagate_live();
   }
 };
@@ -870,7 +871,7 @@ struct RlModule : GpuModule {
     const int ig = integ == "euler" ? 0 : (integ == "rk2" ? 1 : 2);
     d.timed("rl_iterate", [&] { launch_rl_iterate(d.L.view(), d.dtgrid.p, p, d.kc, d.mesh, nsub, h, ig, tout.p, d.stream); });
     if (inactive) return;
-    hip_check(hipMemcpyAsync(d.L.te.p, tout.p, d.N * 8, hipMemcpyDeviceToDevice, d.stream), "copy te");
+    d.L.te.swap(tout);   // the result buffer becomes the live thermal energy (no copy)
     d.propagate_live();
   }
 };

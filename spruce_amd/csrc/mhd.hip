@@ -172,9 +172,18 @@ struct GhostArgs {
   int mode;
 };
 
-__global__ void __launch_bounds__(256) k_mhd_ghost(GhostArgs a, int xsides) {
+// With lines > 0 each thread fills one line of one of the two sides (the sides touch disjoint cells
+// and read disjoint cells once the axis has at least 6 cells); otherwise one thread fills a line of
+// both sides in order.
+__global__ void __launch_bounds__(256) k_mhd_ghost(GhostArgs a, int xsides, int lines) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int s0 = xsides ? 0 : 2;
+  if (lines > 0) {
+    const int s = s0 + (t >= lines ? 1 : 0), line = t >= lines ? t - lines : t;
+    if (t < 2 * lines && a.kind[s] != 0 && a.kind[s] != 4)
+      ghost_side(a.G, a.k, a.mesh, s, a.kind[s], line, a.s1[s], a.s2[s], a.mode);
+    return;
+  }
   for (int s = s0; s < s0 + 2; ++s)
     if (a.kind[s] != 0 && a.kind[s] != 4) ghost_side(a.G, a.k, a.mesh, s, a.kind[s], t, a.s1[s], a.s2[s], a.mode);
 }
@@ -356,10 +365,15 @@ void launch_mhd_ghost(MhdSet G, MhdConst k, Mesh mesh, const int* kinds, const d
   GhostArgs a;
   a.G = G; a.k = k; a.mesh = mesh; a.mode = mode;
   for (int q = 0; q < 4; ++q) { a.kind[q] = kinds[q]; a.s1[q] = s1[q]; a.s2[q] = s2[q]; }
-  if (a.kind[0] || a.kind[1])
-    hipLaunchKernelGGL(k_mhd_ghost, dim3((mesh.ny + 255) / 256), dim3(256), 0, s, a, 1);
-  if (a.kind[2] || a.kind[3])
-    hipLaunchKernelGGL(k_mhd_ghost, dim3((mesh.nx + 255) / 256), dim3(256), 0, s, a, 0);
+  // x sides fill rows {0,1,2} and {nx-3..nx-1} from rows 2 and nx-3: independent when nx >= 6
+  if (a.kind[0] || a.kind[1]) {
+    const int lines = mesh.nx >= 6 ? mesh.ny : 0;
+    hipLaunchKernelGGL(k_mhd_ghost, dim3((2 * mesh.ny + 255) / 256), dim3(256), 0, s, a, 1, lines);
+  }
+  if (a.kind[2] || a.kind[3]) {
+    const int lines = mesh.ny >= 6 ? mesh.nx : 0;
+    hipLaunchKernelGGL(k_mhd_ghost, dim3((2 * mesh.nx + 255) / 256), dim3(256), 0, s, a, 0, lines);
+  }
 }
 
 int mhd_dt_blocks(const Mesh& m) {
