@@ -4,6 +4,7 @@
 #include <cfloat>
 
 #include "arith.hpp"
+#include "crpow.hpp"
 #include "e2.hpp"
 
 namespace spruce {
@@ -133,6 +134,25 @@ __device__ void e2_rhs_cell(const E2Set& S, const E2Static& st, const MhdConst& 
   for (int v = 0; v < 7; ++v) k[v] *= mask;
 }
 
+// EICThermalization::computeTimeDerivativesModule (eic_thermalization.cpp:27-43) at one cell:
+// electron-ion energy exchange at the collision rate of the electron coupling parameter, moved
+// from the electron to the ion thermal energy (times the ghost-zone mask).  x^(1/3) and x^(3/2) are
+// the correctly rounded powers; log is the device logarithm.
+__device__ __forceinline__ void eic_apply(const E2Set& S, const MhdConst& K, double nu_c, size_t c, double mask,
+                                          double k[7]) {
+  const double n = S.n[c];
+  const double te_e = smax(S.tee[c] * K.gm1 / (K_B * n), K.tmin);           // e_temp (idealmhd2E.cpp:93)
+  const double a = pow_third(3.0 / 4.0 / PI / n);
+  const double w_pe = sqrt(4.0 * PI * E_CHARGE * E_CHARGE / M_ELECTRON * n);
+  const double gam = E_CHARGE * E_CHARGE / K_B / te_e / a;
+  const double g15 = pow_1p5(gam);
+  const double lam = 1.0 / sqrt(3.0) / g15;
+  const double gam_ei = sqrt(2.0 / 3.0 / PI) * g15 * w_pe * log(lam);
+  const double dedt = nu_c * gam_ei * (S.tee[c] - S.tei[c]);
+  k[4] = k[4] - dedt * mask;
+  k[3] = k[3] + dedt * mask;
+}
+
 __global__ void __launch_bounds__(256) k_e2_stage(E2StageArgs a) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y * blockDim.y + threadIdx.y;
@@ -141,6 +161,7 @@ __global__ void __launch_bounds__(256) k_e2_stage(E2StageArgs a) {
   const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
   double k[7];
   e2_rhs_cell(a.S, a.st, a.k, M, i, j, k);
+  for (int e = 0; e < a.eic; ++e) eic_apply(a.S, a.k, a.eic_nu, c, inside(M, i, j) ? 1.0 : 0.0, k);
   if (a.mode == STAGE_STORE_K) {
 #pragma unroll
     for (int v = 0; v < 7; ++v) a.kbuf[v * N + c] = k[v];
@@ -344,13 +365,15 @@ __global__ void __launch_bounds__(256) k_e2_derived(E2Set S, E2Static st, MhdCon
   out[c] = r;
 }
 
-__global__ void __launch_bounds__(256) k_e2_rhs(E2Set S, E2Static st, MhdConst K, Mesh M, double* out) {
+__global__ void __launch_bounds__(256) k_e2_rhs(E2Set S, E2Static st, MhdConst K, Mesh M, double* out, int eic,
+                                                double eic_nu) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= M.nx || j >= M.ny) return;
   const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
   double k[7];
   e2_rhs_cell(S, st, K, M, i, j, k);
+  for (int e = 0; e < eic; ++e) eic_apply(S, K, eic_nu, c, inside(M, i, j) ? 1.0 : 0.0, k);
   for (int v = 0; v < 7; ++v) out[v * N + c] = k[v];
 }
 
@@ -491,8 +514,8 @@ void launch_e2_dt(E2Set L, E2Static st, MhdConst k, Mesh mesh, double* partial, 
 void launch_e2_derived(E2Set S, E2Static st, MhdConst k, Mesh mesh, int which, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_e2_derived, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, S, st, k, mesh, which, out);
 }
-void launch_e2_rhs(E2Set S, E2Static st, MhdConst k, Mesh mesh, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_e2_rhs, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, S, st, k, mesh, out);
+void launch_e2_rhs(E2Set S, E2Static st, MhdConst k, Mesh mesh, double* out, hipStream_t s, int eic, double eic_nu) {
+  hipLaunchKernelGGL(k_e2_rhs, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, S, st, k, mesh, out, eic, eic_nu);
 }
 void launch_cons_populate(ConsGrids g, MhdConst k, int nx, int ny, hipStream_t s) {
   const size_t N = (size_t)nx * ny;

@@ -29,6 +29,9 @@ struct E2StageArgs {
   const double* k3buf;
   int mode;                // StageMode
   int write_pre;
+  // eic_thermalization modules (eic_thermalization.cpp:27-43): how many, and (2 m_e / m_ion)
+  int eic;
+  double eic_nu;
 };
 
 void launch_e2_stage(const E2StageArgs& a, hipStream_t s);
@@ -38,7 +41,8 @@ void launch_e2_ghost(E2Set G, MhdConst k, Mesh mesh, const int* kinds, const dou
 void launch_e2_dt(E2Set L, E2Static st, MhdConst k, Mesh mesh, double* partial, hipStream_t s);
 // `which` indexes E2_DERIVED of e2_host.hip
 void launch_e2_derived(E2Set S, E2Static st, MhdConst k, Mesh mesh, int which, double* out, hipStream_t s);
-void launch_e2_rhs(E2Set S, E2Static st, MhdConst k, Mesh mesh, double* out, hipStream_t s);
+void launch_e2_rhs(E2Set S, E2Static st, MhdConst k, Mesh mesh, double* out, hipStream_t s, int eic = 0,
+                   double eic_nu = 0.0);
 
 // IdealMHDCons set-up: recomputeEvolvedVarsFromStateVars + enforceMinimums (idealmhdcons.cpp:89-109),
 // then after the ghost fill recomputeDerivedVarsFromEvolvedVars (:111-128).  Grids in the order

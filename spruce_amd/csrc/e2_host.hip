@@ -100,6 +100,9 @@ E2Domain::~E2Domain() {
 }
 
 void E2Domain::add_module(const std::string& name, const std::vector<std::pair<std::string, std::string>>&) {
+  require(!ready, SPRUCE_E_STATE, "modules must be added before setup");
+  // EICThermalization takes no options (eic_thermalization.cpp:7-10)
+  if (name == "eic_thermalization") { ++eic; return; }
   throw Error(SPRUCE_E_UNSUPPORTED, "module " + name + " is not available with ideal_mhd_2E in this build");
 }
 
@@ -124,6 +127,7 @@ void E2Domain::setup() {
   for (auto* s : PD_NAMES_E2) require(host_grids.count(s), SPRUCE_E_ARG, std::string("PlasmaDomain grid ") + s + " not set");
   for (auto* s : STATE) require(host_grids.count(s), SPRUCE_E_ARG, std::string("state variable ") + s + " not set");
   build_mesh(desc, nx, ny, host_grids, ax, ay, mesh, kc, s1, s2);
+  eic_nu = 2.0 * M_ELECTRON / desc.ion_mass;
   for (int k = 0; k < 4; ++k) {
     kinds_all[k] = desc.bc[k];
     kinds_ucnp[k] = desc.bc[k] == SPRUCE_BC_OPEN_UCNP ? desc.bc[k] : 0;
@@ -169,6 +173,8 @@ void E2Domain::stage(Set& S, Set& B, Set& T, double mult, int mode, int kslot) {
   a.st = E2Static{bex.p, bey.p, gx.p, gy.p};
   a.k = kc; a.mesh = mesh; a.step = step.p; a.step_mult = mult; a.mode = mode;
   a.write_pre = 1;
+  a.eic = eic;
+  a.eic_nu = eic_nu;
   if (mode == STAGE_STORE_K) a.kbuf = kbuf[kslot].p;
   if (mode == STAGE_RK4_FINAL) { a.kbuf = kbuf[0].p; a.k2buf = kbuf[1].p; a.k3buf = kbuf[2].p; }
   timed("e2_stage", [&] { launch_e2_stage(a, stream); });
@@ -217,7 +223,8 @@ void E2Domain::advance(int s) {
 void E2Domain::run(int nsteps, double* dt_out, int32_t* sub_out) {
   require(ready, SPRUCE_E_STATE, "setup must precede step");
   require(nsteps >= 0, SPRUCE_E_ARG, "nsteps must be non-negative");
-  (void)sub_out;
+  if (sub_out)   // no module of this equation set subcycles
+    for (int k = 0; k < nsteps * eic; ++k) sub_out[k] = -1;
   if (nsteps == 0) return;
   if ((int)history.n < nsteps + 1) history.alloc(nsteps + 1);
   hip_check(hipMemcpyAsync(history.p, step.p, sizeof(double), hipMemcpyDeviceToDevice, stream), "copy step");
@@ -281,7 +288,7 @@ void E2Domain::rhs(double* out, int64_t count) {
   require(count == (int64_t)(7 * N), SPRUCE_E_ARG, "rhs output must hold nevolved*xdim*ydim doubles");
   DBuf tmp;
   tmp.alloc(7 * N);
-  launch_e2_rhs(L.view(), E2Static{bex.p, bey.p, gx.p, gy.p}, kc, mesh, tmp.p, stream);
+  launch_e2_rhs(L.view(), E2Static{bex.p, bey.p, gx.p, gy.p}, kc, mesh, tmp.p, stream, eic, eic_nu);
   hip_check(hipMemcpyAsync(out, tmp.p, 7 * N * 8, hipMemcpyDeviceToHost, stream), "download rhs");
   hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }

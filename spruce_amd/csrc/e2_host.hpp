@@ -56,6 +56,8 @@ class E2Domain : public DomainBase {
   DBuf kbuf[3];
   DBuf bex, bey, gx, gy, partial, step, history, scratch;
   int* err_flag = nullptr;
+  int eic = 0;            // eic_thermalization modules, applied in order to every RHS
+  double eic_nu = 0.0;    // 2 m_e / m_ion (eic_thermalization.cpp:40)
 };
 
 class ConsDomain : public DomainBase {

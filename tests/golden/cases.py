@@ -349,6 +349,12 @@ def _cases() -> Dict[str, Tuple[Callable[[], StateFile], Callable[[], ConfigFile
                                      open_boundary_strength=0.4, open_boundary_decay_base=0.7))
     c["e2_ucnp_euler"] = (lambda: W.mhd_variant_state("ideal_mhd_2E", 18, 24, 43),
                           lambda: _var("ideal_mhd_2E", ("open_ucnp",) * 4, "euler", 6, E2))
+    # electron-ion thermalization (eic_thermalization.cpp) on the two-energy set
+    c["e2_eic_periodic_rk2"] = (lambda: W.mhd_variant_state("ideal_mhd_2E", 20, 26, 47),
+                                lambda: _var("ideal_mhd_2E", P4, "rk2", 5, E2, modules=[("eic_thermalization", [])]))
+    c["e2_eic_open_euler"] = (lambda: W.mhd_variant_state("ideal_mhd_2E", 22, 18, 48, stretch=True),
+                              lambda: _var("ideal_mhd_2E", ("open", "reflect", "fixed", "open"), "euler", 5, E2,
+                                           modules=[("eic_thermalization", [])]))
     # ideal_mhd_cons aborts at its first time derivative in the reference (idealmhdcons.cpp:48-49),
     # so its cases hold the set-up state only (max_iterations = 0)
     c["cons_periodic_setup"] = (lambda: W.mhd_variant_state("ideal_mhd_cons", 24, 20, 44),
