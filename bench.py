@@ -1,6 +1,6 @@
 """Time-step throughput of the MI355X path: cell-updates/s and the HBM roofline of the stage kernel.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload arcade|periodic|periodic2048|arcade2048|batch2f|adjoint|adjoint_arcade|copy]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload arcade|periodic|periodic2048|moc2048|arcade2048|batch2f|adjoint|adjoint_arcade|copy]
 
 One "step" is one PlasmaDomain::advanceTime (evolution.cpp:59-82) over the whole grid, ghost cells
 included: every integrator stage, every module subcycle, the ghost fill and the dt reduction.
@@ -78,6 +78,13 @@ def make_workload(name: str):
         st = W.periodic_wave_state(2048, 2048, d=1.25e6, seed=0)
         cfg = W.periodic_wave_config(steps=1000)
         desc = {"workload": "periodic ideal-MHD 2048x2048 (RK2, FP64)", "grid": [2048, 2048],
+                "equation_set": "ideal_mhd", "integrator": "rk2", "modules": []}
+    elif name == "moc2048":
+        # the same wave with open characteristic (open_moc) boundaries on both y sides
+        st = W.periodic_wave_state(2048, 2048, d=1.25e6, seed=0)
+        cfg = W.base_config("ideal_mhd", ("periodic", "periodic", "open_moc", "open_moc"), "rk2", epsilon=0.2,
+                            max_iterations=1000, density_min=1.0, temp_min=1.0, thermal_energy_min=1e-12)
+        desc = {"workload": "ideal-MHD 2048x2048 wave, periodic x, open_moc y (RK2, FP64)", "grid": [2048, 2048],
                 "equation_set": "ideal_mhd", "integrator": "rk2", "modules": []}
     elif name in ("arcade", "arcade2048"):
         # 2048 rows stretched by 1.002 per row would reach 1.5e11 cm: the explicit conduction

@@ -208,7 +208,8 @@ class Domain : public DomainBase {
   bool any_moc = false;
   int moc_on[4] = {0, 0, 0, 0};
   MocSide moc_sides[4];
-  DBuf mocbuf, moc_vc, moc_partial;
+  DBuf mocbuf, moc_vc, moc_partial, moc_ops;
+  int moc_base[5] = {0, 0, 0, 0, 0};
   // the characteristic terms of set S into mocbuf (and the global viscosity coefficient first)
   const double* moc_terms(const MhdSet& S) {
     if (!any_moc) return nullptr;
@@ -216,6 +217,8 @@ class Domain : public DomainBase {
     m.S = S; m.st = stat(); m.K = kc; m.M = mesh;
     for (int b = 0; b < 4; ++b) m.side[b] = moc_sides[b];
     m.out = mocbuf.p;
+    m.ops = moc_ops.p;
+    for (int b = 0; b < 5; ++b) m.slot_base[b] = moc_base[b];
     m.vc = nullptr;
     if (gvisc != 0.0) {
       timed("moc_visc", [&] { launch_moc_visc(S, stat(), kc, mesh, gvisc, moc_partial.p, moc_vc.p, err_flag, stream); });
@@ -378,6 +381,7 @@ class Domain : public DomainBase {
 
     if (any_moc) {
       mocbuf.alloc(8 * N);   // zero: cells outside the open_moc regions keep a zero term
+      moc_ops.alloc((size_t)moc_slot_bases(moc_sides, moc_base) * moc_ops_per_slot());
       moc_vc.alloc(1);
       moc_partial.alloc(dt_blocks);
     }

@@ -169,9 +169,54 @@ __device__ double char_barton(const Ctx& c, int f, int a, bool pf, int i, int j)
   return (hi - lo) / A.d[k];
 }
 
+// The derivatives one side's terms take at one cell of its region, in a fixed order (MOC_NOPS of
+// them): evaluated by one thread each in k_moc_ops, then combined per cell in side_terms.
+enum MocOp : int {
+  OP_CB = 0,        // 11 characteristic Barton derivatives across the side
+  OP_ALT = 11,      // 5 fields x {central on the padded range, backward at the upper/lower corner} along it
+  OP_SB = 26,       // 3 one-sided second derivatives across the side
+  OP_DI = 29,       // 10 central derivatives along the side (parallel range)
+  OP_TI = 39,       // 11 transport derivatives along the side (parallel range, velocity v_para)
+  OP_SI = 50,       // 3 second derivatives along the side (parallel range)
+  MOC_NOPS = 53
+};
+__device__ double eval_op(const Ctx& c, const MocSide& sd, int k, int i, int j) {
+  const int a = sd.axis, pa = 1 - a;
+  const bool pf = sd.lower == 0;
+  const int Fvperp = a == 0 ? F_VX : F_VY, Fvpara = a == 0 ? F_VY : F_VX;
+  const int Fbe_perp = a == 0 ? F_BEX : F_BEY, Fbe_para = a == 0 ? F_BEY : F_BEX;
+  const int Fbi_perp = a == 0 ? F_BIX : F_BIY, Fbi_para = a == 0 ? F_BIY : F_BIX;
+  const int Fb_perp = a == 0 ? F_BX : F_BY;
+  const int Frv_perp = a == 0 ? F_RVX : F_RVY, Frv_para = a == 0 ? F_RVY : F_RVX;
+  if (k < OP_ALT) {
+    const int f[11] = {Fbe_para, Fbi_para, F_BEZ, F_BIZ, Fvpara, Fvperp, F_VZ, F_P, Fb_perp, F_RHO, F_TE};
+    return char_barton(c, f[k - OP_CB], a, pf, i, j);
+  }
+  if (k < OP_SB) {
+    const int f[5] = {F_P, Fbe_perp, Fbi_perp, F_BEZ, F_BIZ};
+    const int q = k - OP_ALT, fld = f[q / 3], w = q % 3;
+    if (w == 0) return deriv(c, fld, pa, sd.rp, i, j);
+    return w == 1 ? back1(c, fld, pa, true, sd.ru, i, j) : back1(c, fld, pa, false, sd.rl, i, j);
+  }
+  if (k < OP_DI) {
+    const int f[3] = {Fvpara, F_VZ, Fvperp};
+    return second_back(c, f[k - OP_SB], a, pf, sd.rb, i, j);
+  }
+  if (k < OP_TI) {
+    const int f[10] = {F_P, Fbe_para, Fbi_para, Fbe_perp, Fbi_perp, F_BEZ, F_BIZ, Fvpara, F_VZ, Fvperp};
+    return deriv(c, f[k - OP_DI], pa, sd.ri, i, j);
+  }
+  if (k < OP_SI) {
+    const int f[11] = {F_RHO, Frv_para, F_RVZ, Frv_perp, F_TE, Fbe_para, Fbi_para, F_BEZ, F_BIZ, Fbe_perp, Fbi_perp};
+    return tderiv(c, f[k - OP_TI], Fvpara, pa, sd.ri, i, j);
+  }
+  const int f[3] = {Fvpara, F_VZ, Fvperp};
+  return second(c, f[k - OP_SI], pa, sd.ri, i, j);
+}
+
 // normal + parallel terms of one open_moc side at a cell of its region, in the order
 // {rho, v_x, v_y, v_z, thermal_energy, b_x, b_y, b_z} (idealmhd.cpp:336-615)
-__device__ void side_terms(const Ctx& c, const MocSide& sd, double vc, int i, int j, double* out) {
+__device__ void side_terms(const Ctx& c, const MocSide& sd, double vc, int i, int j, const double* op, double* out) {
   const int a = sd.axis, pa = 1 - a;
   const bool lower = sd.lower != 0, pf = !lower;
   const size_t cell = (size_t)i * c.M.ny + j;
@@ -201,15 +246,15 @@ __device__ void side_terms(const Ctx& c, const MocSide& sd, double vc, int i, in
   const double c_s = sqrt(css), c_perp = sqrt(cps), c_plus = sqrt(cpl), c_minus = sqrt(cmi);
   const double al_p = sqrt(aps), al_m = sqrt(ams);
 
-  auto cb = [&](int f) { return char_barton(c, f, a, pf, i, j); };
-  const double bpg = cb(Fbe_para) + cb(Fbi_para);
-  const double bgg = cb(Fbe_gd) + cb(Fbi_gd);
-  const double vpg = cb(Fvpara), vperpg = cb(Fvperp), vgg = cb(Fvgd), pg = cb(F_P);
+  const double* cb = op + OP_CB;   // be_para, bi_para, be_gd, bi_gd, v_para, v_perp, v_gd, press, b_perp, rho, te
+  const double bpg = cb[0] + cb[1];
+  const double bgg = cb[2] + cb[3];
+  const double vpg = cb[4], vperpg = cb[5], vgg = cb[6], pg = cb[7];
   const double vp = fv(c, Fvperp, i, j);
   const double sg = a == 0 ? 1.0 : -1.0;
   const double sq4 = sqrt(4.0 * PI * r);
-  double d1 = vp * cb(Fb_perp);
-  double d2 = vp * ((te + p) * cb(F_RHO) - r * cb(F_TE));
+  double d1 = vp * cb[8];
+  double d2 = vp * ((te + p) * cb[9] - r * cb[10]);
   double d3 = s_perp * sg * (vp + c_perp) *
               (-s_perp * R_gd * vpg + s_perp * R_para * vgg + R_gd / sq4 * bpg - R_para / sq4 * bgg);
   double d4 = s_perp * sg * (vp - c_perp) *
@@ -226,17 +271,11 @@ __device__ void side_terms(const Ctx& c, const MocSide& sd, double vc, int i, in
   double d8 = (vp - c_minus) * (A2 - Bpp - Bgp - Cm - Epp - Egp);
 
   // alternate terms for inflowing characteristics: derivatives along the side
-  auto alt1 = [&](int f) {
-    return deriv(c, f, pa, sd.rp, i, j) + back1(c, f, pa, true, sd.ru, i, j) + back1(c, f, pa, false, sd.rl, i, j);
-  };
-  auto alt2 = [&](int fe, int fi) {
-    return deriv(c, fe, pa, sd.rp, i, j) + back1(c, fe, pa, true, sd.ru, i, j) + back1(c, fe, pa, false, sd.rl, i, j) +
-           deriv(c, fi, pa, sd.rp, i, j) + back1(c, fi, pa, true, sd.ru, i, j) + back1(c, fi, pa, false, sd.rl, i, j);
-  };
-  const double bps_alt = 2.0 * bperp * alt2(Fbe_perp, Fbi_perp);
-  const double p_alt = alt1(F_P);
-  const double bp_alt = alt2(Fbe_perp, Fbi_perp);
-  const double bg_alt = alt2(Fbe_gd, Fbi_gd);
+  const double* al = op + OP_ALT;   // press, be_perp, bi_perp, be_gd, bi_gd: central, upper, lower
+  const double p_alt = al[0] + al[1] + al[2];
+  const double bp_alt = al[3] + al[4] + al[5] + al[6] + al[7] + al[8];
+  const double bg_alt = al[9] + al[10] + al[11] + al[12] + al[13] + al[14];
+  const double bps_alt = 2.0 * bperp * bp_alt;
   const double imask = in_range(sd.ri, i, j) ? 1.0 : 0.0;
   const double d3a =
       s_perp * s_perp * sg * ((-1.0 * imask * R_gd * gpara + (R_gd * (p_alt + bps_alt / (8.0 * PI)) + bh * (bg_alt) / (4.0 * PI)) / r));
@@ -258,35 +297,37 @@ __device__ void side_terms(const Ctx& c, const MocSide& sd, double vc, int i, in
   const double rho_n = -(c.K.gamma / r * d2 + 0.5 * r * al_p * (d5 + d6) + 0.5 * r * al_m * (d7 + d8)) / css;
   const double vpara_n = -0.5 * s_perp * (sg * R_gd * (-d3 + d4) + (c_minus * al_m / css * R_para * (-d5 + d6) +
                                                                       c_plus * al_p / css * R_para * (d7 - d8))) +
-                         vc * second_back(c, Fvpara, a, pf, sd.rb, i, j);
+                         vc * op[OP_SB + 0];
   const double vgd_n = -0.5 * s_perp * (sg * R_para * (d3 - d4) + (c_minus * al_m / css * R_gd * (-d5 + d6) +
                                                                      c_plus * al_p / css * R_gd * (d7 - d8))) +
-                       vc * second_back(c, Fvgd, a, pf, sd.rb, i, j);
+                       vc * op[OP_SB + 1];
   const double vperp_n = -0.5 / css * (c_plus * al_p * (d5 - d6) + c_minus * al_m * (d7 - d8)) +
-                         vc * second_back(c, Fvperp, a, pf, sd.rb, i, j);
+                         vc * op[OP_SB + 2];
   const double te_n = -(0.5 * (te + p) * al_p * (d5 + d6) + 0.5 * (te + p) * al_m * (d7 + d8)) / css;
   const double bpara_n = -sqp * (sg * R_gd * (d3 + d4) + al_m / c_s * R_para * (d5 + d6) - al_p / c_s * R_para * (d7 + d8));
   const double bgd_n = -sqp * (sg * -1.0 * R_para * (d3 + d4) + al_m / c_s * R_gd * (d5 + d6) - al_p / c_s * R_gd * (d7 + d8));
   const double bperp_n = -d1;
 
   // parallel terms (transport along the side)
-  auto dI = [&](int f) { return deriv(c, f, pa, sd.ri, i, j); };
-  auto tI = [&](int f) { return tderiv(c, f, Fvpara, pa, sd.ri, i, j); };
+  // dI: press, be_para, bi_para, be_perp, bi_perp, be_gd, bi_gd, v_para, v_gd, v_perp
+  // tI: rho, rho v_para, rho v_gd, rho v_perp, te, be_para, bi_para, be_gd, bi_gd, be_perp, bi_perp
+  const double* dI = op + OP_DI;
+  const double* tI = op + OP_TI;
+  const double* sI = op + OP_SI;   // v_para, v_gd, v_perp
   const double vpara = fv(c, Fvpara, i, j), vgd = fv(c, Fvgd, i, j);
-  const double rho_p = -tI(F_RHO);
-  const double vpara_p = -1.0 / r * (dI(F_P) + 2.0 * bpara / (8.0 * PI) * (dI(Fbe_para) + dI(Fbi_para)) +
-                                     2.0 * bperp / (8.0 * PI) * (dI(Fbe_perp) + dI(Fbi_perp)) +
-                                     2.0 * bgd / (8.0 * PI) * (dI(Fbe_gd) + dI(Fbi_gd))) -
-                         (tI(Frv_para) - vpara * tI(F_RHO)) / r + bpara / (4.0 * PI * r) * (dI(Fbe_para) + dI(Fbi_para)) +
-                         imask * gpara + vc * second(c, Fvpara, pa, sd.ri, i, j);
-  const double vgd_p = -(tI(Frv_gd) - vgd * tI(F_RHO)) / r + bpara / (4.0 * PI * r) * (dI(Fbe_gd) + dI(Fbi_gd)) +
-                       vc * second(c, Fvgd, pa, sd.ri, i, j);
-  const double vperp_p = -(tI(Frv_perp) - vp * tI(F_RHO)) / r + bpara / (4.0 * PI * r) * (dI(Fbe_perp) + dI(Fbi_perp)) +
-                         imask * gperp + vc * second(c, Fvperp, pa, sd.ri, i, j);
-  const double te_p = -tI(F_TE) - p * dI(Fvpara);
-  const double bpara_p = -(tI(Fbe_para) + tI(Fbi_para)) + bpara * dI(Fvpara);
-  const double bgd_p = -(tI(Fbe_gd) + tI(Fbi_gd)) + bpara * dI(Fvgd);
-  const double bperp_p = -(tI(Fbe_perp) + tI(Fbi_perp)) + bpara * dI(Fvperp);
+  const double rho_p = -tI[0];
+  const double vpara_p = -1.0 / r * (dI[0] + 2.0 * bpara / (8.0 * PI) * (dI[1] + dI[2]) +
+                                     2.0 * bperp / (8.0 * PI) * (dI[3] + dI[4]) +
+                                     2.0 * bgd / (8.0 * PI) * (dI[5] + dI[6])) -
+                         (tI[1] - vpara * tI[0]) / r + bpara / (4.0 * PI * r) * (dI[1] + dI[2]) +
+                         imask * gpara + vc * sI[0];
+  const double vgd_p = -(tI[2] - vgd * tI[0]) / r + bpara / (4.0 * PI * r) * (dI[5] + dI[6]) + vc * sI[1];
+  const double vperp_p = -(tI[3] - vp * tI[0]) / r + bpara / (4.0 * PI * r) * (dI[3] + dI[4]) + imask * gperp +
+                         vc * sI[2];
+  const double te_p = -tI[4] - p * dI[7];
+  const double bpara_p = -(tI[5] + tI[6]) + bpara * dI[7];
+  const double bgd_p = -(tI[7] + tI[8]) + bpara * dI[8];
+  const double bperp_p = -(tI[9] + tI[10]) + bpara * dI[9];
 
   out[0] = rho_n + rho_p;
   out[3] = vgd_n + vgd_p;
@@ -320,6 +361,27 @@ __device__ __forceinline__ bool strip_cell(int t, int nx, int ny, int& i, int& j
   return !(i < 2 || i >= nx - 2);
 }
 
+__device__ __forceinline__ int slot_of(const MocArgs& a, int b, int i, int j) {
+  const int* r = a.side[b].rb;
+  return a.slot_base[b] + (i - r[0]) * (r[3] - r[1] + 1) + (j - r[1]);
+}
+
+// one thread per (derivative, slot), derivative-major so that a wavefront evaluates one derivative
+// at 64 neighbouring cells (no divergence between the derivative kinds)
+__global__ void __launch_bounds__(256) k_moc_ops(MocArgs a) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int total = a.slot_base[4];
+  if (t >= (long long)total * MOC_NOPS) return;
+  const int k = (int)(t / total), slot = (int)(t % total);
+  int b = 0;
+  while (b < 3 && slot >= a.slot_base[b + 1]) ++b;
+  const int* r = a.side[b].rb;
+  const int w = r[3] - r[1] + 1, loc = slot - a.slot_base[b];
+  const int i = r[0] + loc / w, j = r[1] + loc % w;
+  const Ctx c{a.S, a.st, a.K, a.M};
+  a.ops[(size_t)slot * MOC_NOPS + k] = eval_op(c, a.side[b], k, i, j);
+}
+
 __global__ void __launch_bounds__(64) k_moc_rhs(MocArgs a) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int nx = a.M.nx, ny = a.M.ny;
@@ -337,7 +399,7 @@ __global__ void __launch_bounds__(64) k_moc_rhs(MocArgs a) {
   for (int b = 0; b < 4; ++b) {
     if (!a.side[b].on || !in_range(a.side[b].rb, i, j)) continue;
     double t8[8];
-    side_terms(c, a.side[b], vc, i, j, t8);
+    side_terms(c, a.side[b], vc, i, j, a.ops + (size_t)slot_of(a, b, i, j) * MOC_NOPS, t8);
 #pragma unroll
     for (int w = 0; w < 8; ++w) res[w] = res[w] + t8[w];
   }
@@ -461,7 +523,24 @@ MocSide moc_side(int b, const int* bc, int nx, int ny) {
   return s;
 }
 
+int moc_ops_per_slot() { return MOC_NOPS; }
+
+int moc_slot_bases(const MocSide* sides, int* base) {
+  int tot = 0;
+  for (int b = 0; b < 4; ++b) {
+    base[b] = tot;
+    if (sides[b].on) {
+      const int* r = sides[b].rb;
+      tot += (r[2] - r[0] + 1) * (r[3] - r[1] + 1);
+    }
+  }
+  base[4] = tot;
+  return tot;
+}
+
 void launch_moc_rhs(const MocArgs& a, hipStream_t s) {
+  const long long nops = (long long)a.slot_base[4] * MOC_NOPS;
+  hipLaunchKernelGGL(k_moc_ops, dim3((unsigned)((nops + 255) / 256)), dim3(256), 0, s, a);
   const int n = 4 * a.M.nx + 4 * a.M.ny;
   hipLaunchKernelGGL(k_moc_rhs, dim3((n + 63) / 64), dim3(64), 0, s, a);
 }

@@ -32,12 +32,20 @@ struct MocArgs {
   MocSide side[4];        // x_bound_1, x_bound_2, y_bound_1, y_bound_2
   const double* vc;       // global viscosity coefficient on the device (null: 0)
   double* out;            // 8 * nx*ny, evolved order; only region cells are written
+  // one slot per (open_moc side, cell of its region), sides in order: side b's slots start at
+  // slot_base[b], slot_base[4] is the total; ops holds moc_ops_per_slot() doubles per slot
+  int slot_base[5];
+  double* ops;
 };
+int moc_ops_per_slot();
+// the slot bases of sides[4] (an off side has no slots); returns the total
+int moc_slot_bases(const MocSide* sides, int* base);
 
 // The ranges of singleBoundaryTermsMOC (idealmhd.cpp:338-382) for side b (0..3) of an nx x ny grid
 // with boundary kinds bc[4].
 MocSide moc_side(int b, const int* bc, int nx, int ny);
-// computeTimeDerivativesCharacteristicBoundary (idealmhd.cpp:308-332) on the region cells.
+// computeTimeDerivativesCharacteristicBoundary (idealmhd.cpp:308-332) on the region cells: every
+// derivative the terms take, one thread each, then the terms of each cell.
 void launch_moc_rhs(const MocArgs& a, hipStream_t s);
 // the ghost-cell limiters of recomputeDerivedVarsFromEvolvedVars (idealmhd.cpp:107-224, 244-245):
 // momenta then fields, x sides before y sides, on set G
