@@ -101,7 +101,7 @@ void TfDomain::fill_descriptor() {
   for (int k = 0; k < 3; ++k) D.k[k] = K[k].p;
   D.bex = bex.p; D.bey = bey.p; D.gx = gx.p; D.gy = gy.p;
   D.kx3 = kx3.p; D.kxv = kxv.p; D.ky3 = ky3.p; D.kyv = kyv.p;
-  D.step = step.p; D.history = history.p; D.counter = counter; D.partial = partial.p;
+  D.step = step.p; D.history = history.p; D.hist_cap = (int)history.n; D.counter = counter; D.partial = partial.p;
 }
 
 void TfDomain::setup() {
@@ -311,6 +311,7 @@ void TfBatchRun::rebuild(TfShape sh) {
     if ((int)d->partial.n < nt) { d->partial.alloc(nt); d->dom.partial = d->partial.p; }
     D.partial = d->partial.p;
     D.history = d->history.p;
+    D.hist_cap = (int)d->history.n;
     d->dom.tiles_x = D.tiles_x; d->dom.tiles_y = D.tiles_y;
     for (int t = 0; t < nt; ++t) tile_dom.push_back((int)k);
     base += nt;
@@ -387,9 +388,11 @@ void TfBatchRun::run(int nsteps, bool use_graph, std::vector<std::vector<double>
     if ((int)d->history.n < nsteps + 1) {
       d->history.alloc(nsteps + 1);
       d->dom.history = d->history.p;
+      d->dom.hist_cap = (int)d->history.n;
     }
     // another batch holding this domain may have replaced its history or partial buffers
-    realloc = realloc || uploaded[k].history != d->history.p || uploaded[k].partial != d->partial.p;
+    realloc = realloc || uploaded[k].history != d->history.p || uploaded[k].hist_cap != (int)d->history.n ||
+              uploaded[k].partial != d->partial.p;
   }
   if (realloc) rebuild(shape);
   launch_tf_reset_counters(b, s);
@@ -406,6 +409,7 @@ void TfBatchRun::run(int nsteps, bool use_graph, std::vector<std::vector<double>
   }
   hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   if (clock.enabled) clock.collect();
+  check_history();
   if (dts) dts->clear();
   for (auto* d : members) {
     std::vector<double> h(nsteps + 1);
@@ -414,6 +418,18 @@ void TfBatchRun::run(int nsteps, bool use_graph, std::vector<std::vector<double>
       require(!std::isnan(h[k]), SPRUCE_E_NUMERIC, "Something went wrong in the min calculation (no finite dt)");
     for (int k = 0; k < nsteps; ++k) { d->time += h[k]; d->iter++; }
     if (dts) dts->emplace_back(h.begin(), h.begin() + nsteps);
+  }
+}
+
+// The step-size reduction stores a step size only while its counter is below the history's
+// capacity; a run that stepped further than the history holds is an error of this class.
+void TfBatchRun::check_history() {
+  for (size_t k = 0; k < members.size(); ++k) {
+    int cnt = 0;
+    hip_check(hipMemcpy(&cnt, members[k]->counter, sizeof(int), hipMemcpyDeviceToHost), "read step counter");
+    const int cap = std::min(uploaded[k].hist_cap, (int)members[k]->history.n);
+    require(cnt <= cap, SPRUCE_E_STATE,
+            "step-size history overflow: " + std::to_string(cnt) + " steps, capacity " + std::to_string(cap));
   }
 }
 
@@ -454,13 +470,23 @@ double TfBatchRun::autotune(int trial_steps, std::string* chosen) {
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "hipEventCreate");
   hip_check(hipEventCreate(&e1), "hipEventCreate");
+  // every timed trial stores trial_steps step sizes after the reset below
+  for (auto* d : members) {
+    if ((int)d->history.n < trial_steps + 1) {
+      d->history.alloc(trial_steps + 1);
+      d->dom.history = d->history.p;
+      d->dom.hist_cap = (int)d->history.n;
+    }
+  }
   for (auto& sh : shapes) {
     rebuild(sh);
     run(1, false, nullptr);  // warm
+    launch_tf_reset_counters(b, s);
     hip_check(hipEventRecord(e0, s), "record");
     for (int k = 0; k < trial_steps; ++k) enqueue_step(s);
     hip_check(hipEventRecord(e1, s), "record");
     hip_check(hipEventSynchronize(e1), "sync");
+    check_history();
     float ms = 0.f;
     hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
     if (ms < best) { best = ms; bestsh = sh; }

@@ -60,6 +60,7 @@ class TfBatchRun {
   TfBatchRun(const std::vector<TfDomain*>& members, TfShape shape);
   ~TfBatchRun();
   void rebuild(TfShape shape);    // new launch shape (autotuning)
+  void check_history();           // error if a member stepped past its step-size history
   void enqueue_step(hipStream_t s);
   void run(int nsteps, bool graph, std::vector<std::vector<double>>* dts);
   double autotune(int trial_steps, std::string* chosen);

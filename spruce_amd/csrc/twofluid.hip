@@ -414,7 +414,7 @@ __global__ void k_tf_reduce(TfBatch b) {
     const double s = D.c.eps * m;
     *D.step = s;
     const int idx = *D.counter;
-    D.history[idx] = (m < DBL_MAX) ? s : __builtin_nan("");
+    if (idx < D.hist_cap) D.history[idx] = (m < DBL_MAX) ? s : __builtin_nan("");
     *D.counter = idx + 1;
   }
 }

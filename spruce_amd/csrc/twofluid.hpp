@@ -42,6 +42,7 @@ struct TfDom {
   double* step;            // device step size of this domain
   double* history;         // step sizes, indexed by *counter
   int* counter;
+  int hist_cap;            // doubles history holds; later step sizes are not stored
   double* partial;         // one dt minimum per tile
   int tile_base, tiles_x, tiles_y;
   int strip_base, strips_x;  // strip kernel: first strip tile of this domain, strips per column tile

@@ -189,3 +189,43 @@ def test_domain_in_batch_cannot_step_alone_and_recovers():
     d2.close()
     for d in doms:
         d.close()
+
+
+def test_autotune_with_more_trial_steps_than_the_history_holds():
+    """Fresh domains hold two step sizes; autotuning times 12 steps per launch shape.  The
+    library grows every member's step-size history first and reports an overflow as an error, so
+    this passes only if no step size was dropped or written past the history."""
+    print("autotune 12 trial steps", flush=True)
+    specs = _members("rk2", False, sizes=[(64, 64), (48, 80)])
+    alone, dts_a = _run_alone(specs, [3])
+    doms = [GpuDomain(s) for s in specs]
+    b = GpuBatch(doms, graph=True)
+    try:
+        b.autotune(12)
+        dts_b = b.step(3)
+        _check_same(doms, alone, dts_b, dts_a, "after autotune(12)")
+    finally:
+        b.close()
+        for d in doms:
+            d.close()
+
+
+def test_batch_of_64_expansions_matches_individual_runs():
+    """The batched-mode workload itself: 64 two-fluid expansions with edges from 128 to 512, one
+    batch with graph replay and the autotuned launch shape, 3 steps against each domain stepped
+    alone."""
+    print("batch of 64", flush=True)
+    members = W.batch_2f_members(64)
+    specs = [spec_from_files(c, s) for s, c in members]
+    alone, dts_a = _run_alone(specs, [3])
+    doms = [GpuDomain(s) for s in specs]
+    b = GpuBatch(doms, graph=True)
+    try:
+        shape, _ = b.autotune(2)
+        print("batch of 64: shape", shape, flush=True)
+        dts_b = b.step(3)
+        _check_same(doms, alone, dts_b, dts_a, "64-domain batch")
+    finally:
+        b.close()
+        for d in doms:
+            d.close()
