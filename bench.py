@@ -1,24 +1,27 @@
-"""Time-step throughput of the MI355X path: cell-updates/s and the HBM roofline of the stage kernel.
+"""Time-step throughput of the MI355X path: cell-updates/s and the HBM roofline of the dominant kernel.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload arcade|periodic|periodic2048|moc2048|arcade2048|batch2f|adjoint|adjoint_arcade|copy]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload arcade2048|arcade|periodic|periodic2048|moc2048|batch2f|adjoint|adjoint_arcade|copy]
 
 One "step" is one PlasmaDomain::advanceTime (evolution.cpp:59-82) over the whole grid, ghost cells
 included: every integrator stage, every module subcycle, the ghost fill and the dt reduction.
 Inputs are synthetic (spruce_amd.workloads) and resident in HBM before timing starts; the timed
-region is K steps bracketed by device synchronisation.  Rank 0 prints one JSON line.
+region is steps W..W+K-1, bracketed by device synchronisation.  Rank 0 prints one JSON line.
 
-The CPU baseline is the reference executable compiled from its own sources (oracle/_ref/run),
-timed on this host over a bounded number of steps of the same workload, with a zero-step run
-subtracted to remove setup and file output.
+The default workload is the 2048x2048 solar arcade (ideal MHD + thermal conduction + radiative
+losses, RK2, FP64).  The CPU baseline is the reference executable compiled from its own sources
+(oracle/_ref/run), started from the GPU's state after W steps and timed over step W+1 between its
+own per-step output lines (setup and file output excluded).
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import queue
 import subprocess
 import sys
 import tempfile
+import threading
 import time
 
 import numpy as np
@@ -66,6 +69,19 @@ def stage_bytes_per_cell(integrator: str) -> float:
     raise ValueError(integrator)
 
 
+def kernel_bytes_per_cell(name: str, integrator: str):
+    """Algorithmic bytes per cell of one launch of a kernel of the IdealMHD step (FP64), or None."""
+    table = {
+        "mhd_stage": stage_bytes_per_cell(integrator),
+        "tc_step": 40.0,          # one Euler conduction subcycle: te, n, b_hat_x, b_hat_y read, te written
+        "rl_iterate": 24.0,       # all radiative subcycles of a step: te, n read, te written
+        "mhd_dt": 88.0,           # 8 evolved + be_x/y/z read
+        "bhat": 64.0,             # be_x/y/z, bi_x/y/z read, b_hat_x/y written
+        "propagate_live": 48.0,   # rho, te read; n, rho, te written
+    }
+    return table.get(name)
+
+
 def make_workload(name: str):
     from spruce_amd import workloads as W
     from spruce_amd.spec import spec_from_files
@@ -106,40 +122,130 @@ def make_workload(name: str):
     return st, cfg, spec_from_files(cfg, st), desc
 
 
-def cpu_baseline(st, cfg, steps: int, threads: int) -> dict:
-    """Time the reference executable: (t(steps) - t(0)) over the same input, OMP_NUM_THREADS=threads."""
+def host_cpu() -> dict:
+    """CPU model, logical CPUs this process may run on, and threads per core, from lscpu."""
+    info = {"model": "unknown", "cpus_available": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Model name":
+                info["model"] = v
+            elif k == "Thread(s) per core":
+                info["threads_per_core"] = int(v)
+            elif k == "Core(s) per socket":
+                info["cores_per_socket"] = int(v)
+            elif k == "Socket(s)":
+                info["sockets"] = int(v)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_threads() -> int:
+    """OpenMP threads of the CPU leg: SPRUCE_CPU_THREADS, else OMP_NUM_THREADS, else the CPUs this
+    process may use (at most 16)."""
+    for k in ("SPRUCE_CPU_THREADS", "OMP_NUM_THREADS"):
+        if os.environ.get(k, "").isdigit() and int(os.environ[k]) > 0:
+            return int(os.environ[k])
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+# glibc serves each 32 MiB grid of a 2048x2048 run with a fresh mmap by default, so every whole-grid
+# operation of the reference first takes page faults on 32 MiB; these tunables keep freed grids in
+# the heap instead.  Both settings are the reference binary unchanged.
+MALLOC_HEAP_ENV = {"MALLOC_MMAP_MAX_": "0", "MALLOC_TRIM_THRESHOLD_": "137438953472", "MALLOC_TOP_PAD_": "1073741824"}
+
+
+def reference_step_times(st, cfg, nsteps: int, threads: int, env_extra=None, line_timeout: float = 170.0):
+    """Runs the reference executable on (st, cfg) for nsteps + 1 steps with one progress line per step
+    and stops it after the last one.  Returns [(seconds, line)] for steps 1..nsteps: the wall time
+    between consecutive per-step lines, i.e. one advanceTime each, without set-up or file output
+    (evolution.cpp:26-77 prints the line at the end of every step).  A run that prints nothing for
+    line_timeout seconds is stopped and reported as an error."""
     exe = os.path.join(ROOT, "oracle", "_ref", "run")
     if not os.path.exists(exe):
-        return None
+        raise RuntimeError("reference executable oracle/_ref/run is not built")
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
-    times = {}
+    env.update(env_extra or {})
+    c = cfg
+    c.settings["max_iterations"] = str(nsteps + 1)
+    c.settings["iter_output_interval"] = "-1"
+    c.settings["time_output_interval"] = "-1"
+    c.settings["write_interval"] = "-1"
+    c.settings["std_out_interval"] = "1"
     with tempfile.TemporaryDirectory() as tmp:
-        sp = os.path.join(tmp, "w.state")
-        st.write(sp)
-        for n in (0, steps):
-            c = cfg
-            c.settings["max_iterations"] = str(n)
-            c.settings["iter_output_interval"] = "-1"
-            c.settings["write_interval"] = "-1"
-            # one line per reference step on stderr: a sample of a large grid runs for minutes
-            c.settings["std_out_interval"] = "1"
-            cp = os.path.join(tmp, "w%d.config" % n)
-            c.write(cp)
-            print("cpu baseline: reference run of %d steps" % n, file=sys.stderr, flush=True)
-            t0 = time.perf_counter()
-            p = subprocess.Popen([exe, "-m", "input", "-o", os.path.join(tmp, "o%d" % n), "-s", sp, "-c", cp],
-                                 stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True, env=env, cwd=tmp)
-            for line in p.stdout:
-                print("  reference: " + line.rstrip()[:160], file=sys.stderr, flush=True)
-            rc = p.wait()
-            times[n] = time.perf_counter() - t0
-            if rc not in (0, -6, 134):
-                return {"error": "reference run failed rc=%d" % rc}
-    dt = times[steps] - times[0]
-    return {"value": st.xdim * st.ydim * steps / dt, "unit": "cell-updates/s", "cores": threads,
-            "kind": "reference",
-            "sample": "%d steps of the same workload, reference executable (g++ -O3 -fopenmp), "
-                      "t(%d steps) - t(0 steps) = %.2f s" % (steps, steps, dt)}
+        # the state goes in as <output>/init.state, which input mode reads without rewriting it
+        # (main.cpp:44-56; plasmadomain.cpp:62-65)
+        out = os.path.join(tmp, "o")
+        os.makedirs(out)
+        print("cpu baseline: writing the %dx%d state" % (st.xdim, st.ydim), file=sys.stderr, flush=True)
+        st.write(os.path.join(out, "init.state"))
+        cp = os.path.join(tmp, "w.config")
+        c.write(cp)
+        print("cpu baseline: reference run of %d steps, %d threads" % (nsteps + 1, threads), file=sys.stderr,
+              flush=True)
+        p = subprocess.Popen([exe, "-m", "input", "-o", out, "-c", cp],
+                             stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, env=env, cwd=tmp)
+        lines = queue.Queue()
+
+        def reader():
+            for ln in p.stdout:
+                lines.put((time.perf_counter(), ln))
+            lines.put((time.perf_counter(), None))
+
+        threading.Thread(target=reader, daemon=True).start()
+        stamps, tail = [], []
+        try:
+            while True:
+                try:
+                    now, line = lines.get(timeout=line_timeout)
+                except queue.Empty:
+                    raise RuntimeError("the reference printed nothing for %d s (step %d); stopped"
+                                       % (line_timeout, len(stamps)))
+                if line is None:
+                    break
+                line = line.rstrip()
+                tail = (tail + [line])[-5:]
+                print("  reference: " + line[:200], file=sys.stderr, flush=True)
+                if line.startswith("Iter: "):
+                    stamps.append((now, line))
+                    if len(stamps) == nsteps + 1:
+                        break
+        finally:
+            if p.poll() is None:
+                p.kill()
+            p.wait()
+    if len(stamps) < nsteps + 1:
+        raise RuntimeError("the reference stopped after %d of %d steps (rc %s): %s"
+                           % (len(stamps), nsteps + 1, p.returncode, " | ".join(tail)))
+    for k, (_, line) in enumerate(stamps):
+        if not line.startswith("Iter: %d/" % k):
+            raise RuntimeError("unexpected reference progress line %r" % line)
+    return [(stamps[k][0] - stamps[k - 1][0], stamps[k][1]) for k in range(1, nsteps + 1)]
+
+
+def module_count(line: str, label: str):
+    """The integer after `label: ` in a reference progress line (e.g. Thermal Subcycles), or None."""
+    for part in line.split("|"):
+        k, _, v = part.partition(":")
+        if k.strip() == label:
+            try:
+                return int(v.strip())
+            except ValueError:
+                return None
+    return None
+
+
+def state_of(dom, base):
+    """The GPU domain's current state as a .state file on the mesh and statics of `base`."""
+    from spruce_amd.stateio import StateFile
+    grids = dict(base.grids)
+    for v in ("rho", "temp", "mom_x", "mom_y", "mom_z", "bi_x", "bi_y", "bi_z"):
+        grids[v] = dom.grid(v)
+    return StateFile(base.xdim, base.ydim, base.ion_mass, base.adiabatic_index, dom.time, grids,
+                     list(base.comments))
 
 
 def tf_stage_bytes_per_cell(curl: bool) -> float:
@@ -217,16 +323,20 @@ def bench_batch(args) -> dict:
         "kernels_ms": {k: v[0] for k, v in kt.items()},
     }
     if not args.no_cpu:
-        threads = int(os.environ.get("SPRUCE_CPU_THREADS", str(min(16, os.cpu_count() or 1))))
+        threads = cpu_threads()
         k = next(i for i, s in enumerate(specs) if s.xdim == 256)
         st, cfg = members[k]
-        n = args.cpu_steps or max(2, int(1.5e6 / (st.xdim * st.ydim)))
-        cb = cpu_baseline(st, cfg, n, threads)
-        if cb and "sample" in cb:
-            cb["sample"] = "member %d (256x256) only: " % k + cb["sample"]
-        out["cpu_baseline"] = cb
-        if cb and "value" in cb:
-            out["speedup_vs_cpu"] = value / cb["value"]
+        n = args.cpu_steps or 10
+        times = reference_step_times(st, cfg, n, threads)
+        secs = sum(t for t, _ in times)
+        cpu = host_cpu()
+        out["cpu_baseline"] = {
+            "value": st.xdim * st.ydim * n / secs, "unit": "cell-updates/s", "cores": threads, "kind": "reference",
+            "sample": "member %d (256x256) only, steps 1..%d of its own run, reference executable (g++ -O3 -fopenmp), "
+                      "timed between its per-step output lines (%.2f s); %d OpenMP threads on %s"
+                      % (k, n, secs, threads, cpu["model"]),
+            "host": cpu}
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
     return out
 
 
@@ -237,7 +347,7 @@ def bench_adjoint(args) -> dict:
     import numpy as np
     from spruce_amd.adjoint import PeriodicWaveProblem
 
-    steps = args.steps if args.steps != 200 else 500
+    steps = args.steps or 500
     prob = PeriodicWaveProblem(256, 256, nsteps=steps, checkpoint=50)
     prob.forward(prob.temp0, prob.v0)                       # warm-up (module load, first allocations)
     loss, rho, t_fwd = prob.forward(prob.temp0, prob.v0)
@@ -273,7 +383,7 @@ def bench_adjoint_arcade(args) -> dict:
     import numpy as np
     from spruce_amd.adjoint import ArcadeHeatingProblem
 
-    steps = args.steps if args.steps != 200 else 1000
+    steps = args.steps or 1000
     n = args.grid or 2048
     prob = ArcadeHeatingProblem(n, n, nsteps=steps, checkpoint=50)
     th = prob.theta0
@@ -350,14 +460,19 @@ def bench_copy(args) -> dict:
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="arcade")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: 20; adjoint 500, "
+                    "adjoint_arcade 1000, batch2f 200)")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="arcade2048")
     ap.add_argument("--cpu-steps", type=int, default=0, help="reference steps for the CPU baseline (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-kernel", default="auto", choices=["auto", "strip", "cell"])
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0, help="edge of the square arcade of the adjoint_arcade workload")
+    ap.add_argument("--tc-kernel", default="strip", choices=["strip", "cell"])
+    ap.add_argument("--tc-rows", type=int, default=0)
+    ap.add_argument("--cpu-malloc", default="heap", choices=["heap", "default"],
+                    help="glibc malloc tunables of the reference CPU leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -366,6 +481,9 @@ def main() -> None:
 
     from spruce_amd.gpu import GpuDomain
 
+    if args.steps is None:
+        args.steps = 200 if args.workload == "batch2f" else (20 if args.workload not in ("adjoint", "adjoint_arcade")
+                                                             else 0)
     if args.workload == "batch2f":
         print(json.dumps(bench_batch(args)))
         return
@@ -381,11 +499,16 @@ def main() -> None:
     st, cfg, spec, desc = make_workload(args.workload)
     cells = spec.xdim * spec.ydim
     kernel = resolved_stage_kernel(args.stage_kernel, spec.xdim, spec.ydim)
-    opts = {"stage_kernel": args.stage_kernel}
+    opts = {"stage_kernel": args.stage_kernel, "tc_kernel": args.tc_kernel}
     if args.strip_rows:
         opts["strip_rows"] = args.strip_rows
+    if args.tc_rows:
+        opts["tc_rows"] = args.tc_rows
     dom = GpuDomain(spec, options=opts)
     dom.step(args.warmup)
+    snapshot = None
+    if not args.no_cpu:
+        snapshot = state_of(dom, st)          # the CPU leg starts from the state of step W
     t0 = time.perf_counter()
     _, subs = dom.step(args.steps, subcycles=True)   # returns after a device synchronisation
     wall = time.perf_counter() - t0
@@ -393,21 +516,28 @@ def main() -> None:
     mods = desc["modules"]
     mean_sub = {m: float(np.mean(subs[:, k])) for k, m in enumerate(mods)}
 
-    # Per-kernel durations from HIP events recorded on the domain's own stream.
+    # Per-kernel durations from HIP events recorded on the domain's own stream, over as many steps
+    # as were timed (at most 50), starting where the timed steps ended.
     dom.set_profiling(True)
     prof_steps = max(5, min(args.steps, 50))
-    dom.step(prof_steps)
+    _, psubs = dom.step(prof_steps, subcycles=True)
     kt = dom.kernel_times()
     dom.set_profiling(False)
     integ = desc["integrator"]
-    stage_ms = kt.get("mhd_stage", (float("nan"), 0))[0]
-    stage_bytes = stage_bytes_per_cell(integ) * cells
-    achieved = stage_bytes / (stage_ms * 1e-3) / 1e9
-    # algorithmic bytes of one whole step: RK2 stages + 40 B per conduction subcycle + 24 B per
+    per_step = {k: v[0] * v[1] / prof_steps for k, v in kt.items()}
+    dominant = max(per_step, key=per_step.get)
+    kms = kt[dominant][0]
+    bpc = kernel_bytes_per_cell(dominant, integ)
+    achieved = bpc * cells / (kms * 1e-3) / 1e9 if bpc else None
+    rows = str(args.tc_rows or "auto") if dominant == "tc_step" else str(args.strip_rows or "auto")
+    traffic = measured_traffic(args.workload, dominant, rows,
+                               "hbm_traffic_tc_step.md" if dominant == "tc_step" else "hbm_traffic_mhd_stage.md")
+    # algorithmic bytes of one whole step: the stages + 40 B per conduction subcycle + 24 B per
     # radiative subcycle + 64 B of subcycle-count passes when those modules are present
-    b_alg = 8.0 * (5 * E_MHD + 2 * S_MHD)
+    b_alg = 8.0 * (5 * E_MHD + 2 * S_MHD) if integ == "rk2" else stage_bytes_per_cell(integ)
     if mods:
         b_alg += 40.0 * mean_sub.get("thermal_conduction", 0.0) + 24.0 * mean_sub.get("radiative_losses", 0.0) + 64.0
+    ms_per_step = wall / args.steps * 1e3
 
     out = {
         "metric": "cell-updates/s and % HBM roofline (FP64); adjoint/forward cost ratio",
@@ -416,29 +546,48 @@ def main() -> None:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": dict(desc, stage_kernel=kernel, strip_rows=args.strip_rows or "auto"),
+        "config": dict(desc, stage_kernel=kernel, strip_rows=args.strip_rows or "auto", tc_kernel=args.tc_kernel,
+                       tc_rows=args.tc_rows or "auto", timed_steps=[args.warmup, args.warmup + args.steps - 1]),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.workload, kernel, str(args.strip_rows or "auto")),
-                     "kernel": "mhd_stage", "kernel_ms": stage_ms,
-                     "stage_bytes_per_cell": stage_bytes_per_cell(integ),
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                     "kernel": dominant, "kernel_ms": kms, "bytes_per_cell": bpc,
+                     "kernel_share_of_step": per_step[dominant] * prof_steps / sum(per_step.values()) / prof_steps
+                     if per_step else None,
                      "step_bytes_per_cell_update": b_alg,
                      "step_frac": b_alg * value / 1e9 / HBM_PEAK_GBS},
         "mean_subcycles": mean_sub,
         "kernels_ms": {k: v[0] for k, v in kt.items()},
+        "kernels_ms_per_step": per_step,
+        "profiled_steps": [args.warmup + args.steps, args.warmup + args.steps + prof_steps - 1],
+        "profiled_mean_subcycles": {m: float(np.mean(psubs[:, k])) for k, m in enumerate(mods)},
     }
     if not args.no_cpu:
-        threads = int(os.environ.get("SPRUCE_CPU_THREADS", str(min(16, os.cpu_count() or 1))))
-        n = args.cpu_steps or max(2, int(1.5e6 / cells))
-        out["cpu_baseline"] = cpu_baseline(st, cfg, n, threads)
-        if out["cpu_baseline"] and "value" in out["cpu_baseline"]:
-            out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+        threads = cpu_threads()
+        n = args.cpu_steps or (1 if cells >= 1 << 20 else 5)
+        env = MALLOC_HEAP_ENV if args.cpu_malloc == "heap" else {}
+        times = reference_step_times(snapshot, cfg, n, threads, env)
+        secs = sum(t for t, _ in times)
+        cpu = host_cpu()
+        first = args.warmup + 1
+        counts = [module_count(line, "Thermal Subcycles") for _, line in times]
+        gpu_counts = [int(c) for c in subs[1:1 + n, 0]] if "thermal_conduction" in mods and args.steps > n else []
+        out["cpu_baseline"] = {
+            "value": cells * n / secs, "unit": "cell-updates/s", "cores": threads, "kind": "reference",
+            "sample": "reference executable (g++ -O3 -fopenmp, its own sources), steps %d..%d of this workload, "
+                      "started from the GPU's state after %d steps and timed between its own per-step output lines "
+                      "(%.2f s, set-up and file output excluded); %d OpenMP threads (OMP_PLACES=cores) on %s; "
+                      "malloc: %s" % (first, first + n - 1, args.warmup, secs, threads, cpu["model"],
+                                      "freed grids kept in the heap" if env else "glibc default"),
+            "step_seconds": [t for t, _ in times], "steps": [first, first + n - 1],
+            "conduction_subcycles": counts, "gpu_conduction_subcycles_same_steps": gpu_counts,
+            "host": cpu}
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out))
 
 

@@ -106,6 +106,20 @@ int spruce_gpu_rhs(spruce_gpu_domain* d, double* host_out, int64_t count);
 int spruce_gpu_dt_min(spruce_gpu_domain* d, double* step_out);
 int spruce_gpu_num_evolved(spruce_gpu_domain* d);
 
+/* put_grid: overwrites one evolved variable of the live state (rho, mom_x/y/z, thermal_energy,
+ *      bi_x/y/z) from host memory without propagating, as a plugin assigning m_eqs->grid(v)
+ *      does (e.g. thermalconduction.cpp:153); derived variables keep their values until the next
+ *      propagate.  count = xdim*ydim.  ideal_mhd only.
+ * ghost_fill: PlasmaDomain::updateGhostZones on the live grids (evolution.cpp:126-152): the ghost
+ *      cells' evolved variables for every non-periodic side.  ideal_mhd only.
+ * propagate: EquationSet::propagateChanges on the live grids (equationset.cpp:212-220): minimums,
+ *      ghost fill, derived variables and the dt grid; the next step's size epsilon * min dt
+ *      (evolution.cpp:62) becomes the domain's and is written to step_out (may be NULL).
+ *      ideal_mhd only. */
+int spruce_gpu_put_grid(spruce_gpu_domain* d, const char* name, const double* host, int64_t count);
+int spruce_gpu_ghost_fill(spruce_gpu_domain* d);
+int spruce_gpu_propagate(spruce_gpu_domain* d, double* step_out);
+
 /* Device-resident access for zero-copy callers: the HIP stream the handle launches on and the
  * device address of a live grid (valid until the next step call). */
 int spruce_gpu_stream(spruce_gpu_domain* d, void** hip_stream);

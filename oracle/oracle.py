@@ -33,6 +33,8 @@ def lib() -> ctypes.CDLL:
         L.oracle_get_grid.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(D)]
         L.oracle_rhs.argtypes = [P, ctypes.POINTER(D)]
         L.oracle_num_evolved.argtypes = [P]
+        L.oracle_propagate.argtypes = [P, ctypes.POINTER(D)]
+        L.oracle_ghost_fill.argtypes = [P]
         L.oracle_time.restype = D
         L.oracle_time.argtypes = [P]
         L.oracle_subcycles.argtypes = [P, ctypes.POINTER(I), I]
@@ -108,6 +110,19 @@ class OracleDomain:
         out = np.empty((ne, self.spec.xdim, self.spec.ydim))
         self._chk(lib().oracle_rhs(self.h, _dptr(out)))
         return out
+
+    def put_grid(self, name: str, g: np.ndarray) -> None:
+        """Overwrite one grid of the live state (no propagate)."""
+        a = np.ascontiguousarray(g, dtype=np.float64)
+        self._chk(lib().oracle_set_grid(self.h, name.encode(), _dptr(a)))
+
+    def ghost_fill(self) -> None:
+        self._chk(lib().oracle_ghost_fill(self.h))
+
+    def propagate(self) -> float:
+        v = ctypes.c_double()
+        self._chk(lib().oracle_propagate(self.h, ctypes.byref(v)))
+        return v.value
 
     def subcycles(self) -> List[int]:
         buf = (ctypes.c_int * 16)()

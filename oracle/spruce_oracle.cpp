@@ -1403,6 +1403,26 @@ int oracle_rhs(void* h, double* out) {
   GUARD_END(-1)
 }
 
+// EquationSet::propagateChanges on the live grids (equationset.cpp:212-220); *step_out = the next
+// step's size, epsilon * min dt over the dt bounds (evolution.cpp:62).
+int oracle_propagate(void* h, double* step_out) {
+  GUARD_BEGIN
+  auto* d = static_cast<Domain*>(h);
+  d->eqs->propagate(d->eqs->g);
+  if (step_out) *step_out = d->eps * range_min(d->eqs->g[d->eqs->idx("dt")], d->xl_dt, d->yl_dt, d->xu_dt, d->yu_dt);
+  return 0;
+  GUARD_END(-1)
+}
+
+// PlasmaDomain::updateGhostZones on the live grids (evolution.cpp:126-152).
+int oracle_ghost_fill(void* h) {
+  GUARD_BEGIN
+  auto* d = static_cast<Domain*>(h);
+  d->ghost(d->eqs->g);
+  return 0;
+  GUARD_END(-1)
+}
+
 int oracle_num_evolved(void* h) { return (int)static_cast<Domain*>(h)->eqs->evolved().size(); }
 double oracle_time(void* h) { return static_cast<Domain*>(h)->time; }
 

@@ -144,6 +144,8 @@ class Domain : public DomainBase {
   bool any_open = false;
   int stage_kernel = -1;  // 0: wavefront strips (mhd2.hip), 1: one thread per cell (mhd.hip), -1: by size
   int strip_rows = 0;     // rows per wavefront strip; 0 = chosen from the grid size
+  int tc_kernel = 0;      // conduction update: 0 wavefront strips, 1 one thread per cell
+  int tc_rows = 0;        // rows per conduction strip; 0 = chosen from the grid size
 
   void set_option(const std::string& k, const std::string& v) override {
     if (k == "stage_kernel") {
@@ -153,6 +155,13 @@ class Domain : public DomainBase {
       const int r = std::stoi(v);
       require(r >= 4 && r <= 256 && r % 4 == 0, SPRUCE_E_ARG, "strip_rows must be a multiple of 4 in [4, 256]");
       strip_rows = r;
+    } else if (k == "tc_kernel") {
+      require(v == "strip" || v == "cell", SPRUCE_E_ARG, "tc_kernel must be strip or cell");
+      tc_kernel = v == "cell" ? 1 : 0;
+    } else if (k == "tc_rows") {
+      const int r = std::stoi(v);
+      require(r >= 1 && r <= 256, SPRUCE_E_ARG, "tc_rows must be in [1, 256]");
+      tc_rows = r;
     } else {
       throw Error(SPRUCE_E_ARG, "unknown option " + k);
     }
@@ -371,6 +380,7 @@ class Domain : public DomainBase {
     upload(scratch, host_grids["temp"]);
 
     if (strip_rows == 0) strip_rows = mhd_strip_rows_default(nx, ny);
+    if (tc_rows == 0) tc_rows = tc_strip_rows_default(nx, ny);
     if (stage_kernel < 0) stage_kernel = mhd_stage_kernel_default(nx, ny);
     dt_blocks = mhd_dt_blocks(mesh);
     partial.alloc(dt_blocks);
@@ -705,6 +715,45 @@ class Domain : public DomainBase {
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
   }
 
+  // Overwrites one evolved grid of the live state without propagating (a plugin writing
+  // m_eqs->grid(v), e.g. thermalconduction.cpp:153); derived grids keep their values until the
+  // next propagate.
+  void put_grid(const std::string& name, const double* p, int64_t count) override {
+    require(ready, SPRUCE_E_STATE, "setup must precede put_grid");
+    require(count == (int64_t)N, SPRUCE_E_ARG, "grid " + name + ": count must equal xdim*ydim");
+    DBuf* b = nullptr;
+    if (name == "rho") b = &L.rho;
+    else if (name == "mom_x") b = &L.mx;
+    else if (name == "mom_y") b = &L.my;
+    else if (name == "mom_z") b = &L.mz;
+    else if (name == "thermal_energy") b = &L.te;
+    else if (name == "bi_x") b = &L.bx;
+    else if (name == "bi_y") b = &L.by;
+    else if (name == "bi_z") b = &L.bz;
+    require(b != nullptr, SPRUCE_E_ARG, "put_grid takes an evolved variable of ideal_mhd, not <" + name + ">");
+    hip_check(hipMemcpyAsync(b->p, p, N * 8, hipMemcpyHostToDevice, stream), "upload grid");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  }
+
+  // PlasmaDomain::updateGhostZones on the live grids (evolution.cpp:126-152): the ghost cells'
+  // evolved variables only (derived ones follow at the next propagate).
+  void ghost_fill() override {
+    require(ready, SPRUCE_E_STATE, "setup must precede ghost_fill");
+    ghost(L, kinds_all, GHOST_LIVE_RAW);
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  }
+
+  // EquationSet::propagateChanges on the live grids (equationset.cpp:212-220): minimums, ghost
+  // fill, derived variables, dt; the next step's size epsilon * min dt (evolution.cpp:62) becomes
+  // the domain's and is returned.
+  double propagate() override {
+    require(ready, SPRUCE_E_STATE, "setup must precede propagate");
+    propagate_live();
+    finish_step(-1);
+    sync_and_check();
+    return dt_min();
+  }
+
   double dt_min() override {
     require(ready, SPRUCE_E_STATE, "setup must precede dt_min");
     double v = 0.0;
@@ -756,7 +805,11 @@ struct TcModule : GpuModule {
     if (p.saturation) { rx.alloc(d.N); ry.alloc(d.N); coef.alloc(d.N); }
   }
   void pre_launch(Domain& d, double* scal) override {
-    d.timed("bhat_derivs", [&] { launch_bhat_derivs(d.bhx.p, d.bhy.p, d.mesh, dbxx.p, dbxy.p, dbyx.p, dbyy.p, d.stream); });
+    // the strip form of the update forms the b_hat derivatives itself; the grids serve the cell
+    // form and the adjoint's replay
+    if (d.tc_kernel == 1 || recording)
+      d.timed("bhat_derivs",
+              [&] { launch_bhat_derivs(d.bhx.p, d.bhy.p, d.mesh, dbxx.p, dbxy.p, dbyx.p, dbyy.p, d.stream); });
     d.timed("tc_pre", [&] {
       launch_tc_pre(d.L.view(), d.bhx.p, d.bhy.p, p, d.kc, d.mesh, d.mod_partial.p, solar_blocks(d.mesh), scal,
                     d.stream);
@@ -780,6 +833,7 @@ struct TcModule : GpuModule {
     a.k1 = k1.p; a.k2 = k2.p; a.k3 = k3.p; a.h = h; a.mult = mult; a.mode = mode; a.p = p; a.k = d.kc;
     a.dbxx = dbxx.p; a.dbxy = dbxy.p; a.dbyx = dbyx.p; a.dbyy = dbyy.p;
     a.mesh = d.mesh;
+    a.rows = d.tc_kernel == 1 ? 0 : d.tc_rows;
     d.timed("tc_step", [&] { launch_tc_step(a, d.stream); });
   }
   void iterate(Domain& d, double dt) override {  // thermalconduction.cpp:91-156
@@ -1666,6 +1720,11 @@ struct ParamHeatModule : PostSourceModule {
   bool needs_host_dt() const override { return false; }
   void setup(Domain& d) override {
     const auto& y = d.host_grids.at("pos_y");
+    // the basis (and dJ/dtheta) is a function of the row height: every column must share it
+    for (int i = 1; i < d.nx; ++i)
+      for (int j = 0; j < d.ny; ++j)
+        require(y[(size_t)i * d.ny + j] == y[j], SPRUCE_E_ARG,
+                "parametric_heating: pos_y must not vary along x (the heating profile is a function of height)");
     double lo = y[0], hi = y[0];
     for (double v : y) { lo = std::min(lo, v); hi = std::max(hi, v); }
     const double sp = K > 1 ? (hi - lo) / (double)(K - 1) : hi - lo;
@@ -2524,6 +2583,26 @@ int spruce_gpu_rhs(spruce_gpu_domain* h, double* host_out, int64_t count) {
   API_BEGIN
   require(host_out, SPRUCE_E_ARG, "null argument");
   dom(h).rhs(host_out, count);
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_put_grid(spruce_gpu_domain* h, const char* name, const double* host, int64_t count) {
+  API_BEGIN
+  require(name && host, SPRUCE_E_ARG, "null argument");
+  dom(h).put_grid(name, host, count);
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_ghost_fill(spruce_gpu_domain* h) {
+  API_BEGIN
+  dom(h).ghost_fill();
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_propagate(spruce_gpu_domain* h, double* step_out) {
+  API_BEGIN
+  const double v = dom(h).propagate();
+  if (step_out) *step_out = v;
   API_END(SPRUCE_OK)
 }
 

@@ -156,6 +156,14 @@ class DomainBase {
   virtual double* device_grid(const std::string& name) = 0;
   virtual void set_option(const std::string& k, const std::string& v) = 0;
   virtual int num_evolved() const = 0;
+  // unit entry points on the live grids: overwrite an evolved grid (no propagate), the ghost fill
+  // alone, and a whole propagate returning the next step size
+  virtual void put_grid(const std::string& name, const double* p, int64_t count) {
+    (void)name; (void)p; (void)count;
+    throw Error(SPRUCE_E_UNSUPPORTED, "put_grid is available for ideal_mhd only");
+  }
+  virtual void ghost_fill() { throw Error(SPRUCE_E_UNSUPPORTED, "ghost_fill is available for ideal_mhd only"); }
+  virtual double propagate() { throw Error(SPRUCE_E_UNSUPPORTED, "propagate is available for ideal_mhd only"); }
   // discrete adjoint of nsteps steps (only some equation sets provide it)
   virtual void adjoint(int nsteps, int checkpoint, const double* seed, double* grad, double* dt_out) {
     (void)nsteps; (void)checkpoint; (void)seed; (void)grad; (void)dt_out;

@@ -156,7 +156,10 @@ __device__ void ghost_side(MhdSet G, const MhdConst& K, const Mesh& M, int side,
     const size_t ci = at(ein);
     for (int g = 0; g < N_GHOST; ++g) {
       const size_t cb = at(upper ? ax.hi + 1 + g : g);
-      G.rho[cb] = G.rho[ci]; G.n[cb] = G.n[ci]; G.te[cb] = G.te[ci];
+      // the reference copies rho (densities(), idealmhd.hpp:37); a propagate's derived step then
+      // gives the ghost the interior's n, which the normal mode copies directly
+      G.rho[cb] = G.rho[ci]; G.te[cb] = G.te[ci];
+      if (mode == GHOST_NORMAL) G.n[cb] = G.n[ci];
       G.bx[cb] = G.bx[ci]; G.by[cb] = G.by[ci]; G.bz[cb] = G.bz[ci];
       G.mx[cb] = G.mx[ci]; G.my[cb] = G.my[ci]; G.mz[cb] = G.mz[ci];
     }

@@ -6,6 +6,7 @@
 #include "arith.hpp"
 #include "crpow.hpp"
 #include "solar.hpp"
+#include "strip.hpp"
 #include "tc_cell.hpp"
 
 namespace spruce {
@@ -248,6 +249,186 @@ __global__ void __launch_bounds__(256) k_tc_step(TcStepArgs a) {
   a.te_out[c] = smax(v, a.k.temin);
 }
 
+// Strip form of one conduction update (the production path of launch_tc_step).  One 64-lane
+// wavefront owns 64 consecutive columns (y) and marches down R rows (x): lane l holds column
+// 62*tile + l - 1, lanes 1..62 write outputs and lanes 0, 63 are the y-halo.  Every face value the
+// energy derivative takes -- the interpolants of T, of dT/dx along y, of dT/dy along x and of b_hat
+// along both axes -- is computed once and shared by the two cells of the face: along y through DPP
+// lane shifts, along x by keeping the previous row's faces in registers.  The derivatives of the
+// frozen b_hat are formed here from b_hat, so a subcycle reads te, n, b_hat_x, b_hat_y and writes
+// te.  Each value is the same sequence of IEEE operations as tc_core's (tc_cell.hpp), so the result
+// is bit-identical to the one-thread-per-cell form.
+namespace tcs {
+
+// one row of the strip at this lane: cell values and the quantities that need only y-neighbours
+struct Row {
+  double T, te, gy, s1, bx, by, dbxy, dbyy;
+  double coef, dcy;          // flux saturation: coefficient and its y-derivative
+  int i;                     // wrapped row index (clamped into [0, nx) when the row lies outside)
+  bool val, inx;             // row exists; row inside the interior bounds
+};
+struct Raw { double te, n, bx, by, coef; };
+struct XFaces { double T, gy, bx, by, coef; };
+
+struct Lane {
+  int j;                     // wrapped column (clamped into [0, ny) when it lies outside)
+  bool val, iny, out;        // column exists; inside the interior bounds; writes an output
+  double hc, hn, fd, rfd;    // upper y-face: weight of this lane's value, of the next lane's, denominator
+  double dy, rdy, hsq, rhsq;
+};
+
+__device__ __forceinline__ int row_index(int k, int n, int per, bool& val) {
+  if (per) { k %= n; if (k < 0) k += n; val = true; return k; }
+  val = k >= 0 && k < n;
+  return k < 0 ? 0 : (k >= n ? n - 1 : k);
+}
+
+// boundaryInterpolate across this lane's upper y-face (derivs.cpp:477-487), from its own value and
+// the next lane's
+__device__ __forceinline__ double yface(const Lane& L, double q) {
+  const double up = strip::shl1(q);
+  return qdiv(q * L.hc + up * L.hn, L.fd, L.rfd);
+}
+// derivative1D along y at this lane from the upper faces of this lane and of the previous one
+__device__ __forceinline__ double yderiv(const Lane& L, double fup) {
+  return qdiv(fup - strip::shr1(fup), L.dy, L.rdy);
+}
+
+template <bool SAT>
+__device__ __forceinline__ Raw load_raw(const TcStepArgs& a, const Lane& L, int k) {
+  Raw r{0.0, 0.0, 0.0, 0.0, 0.0};
+  bool v;
+  const int i = row_index(k, a.mesh.nx, a.mesh.x.per, v);
+  if (v && L.val) {
+    const size_t c = (size_t)i * a.mesh.ny + L.j;
+    r.te = a.te_src[c];
+    r.n = a.n[c];
+    r.bx = a.bhx[c];
+    r.by = a.bhy[c];
+    if (SAT) r.coef = a.coef[c];
+  }
+  return r;
+}
+
+template <bool SAT>
+__device__ __forceinline__ Row make_row(const TcStepArgs& a, const Lane& L, int k, const Raw& r) {
+  const Mesh& M = a.mesh;
+  Row w;
+  w.i = row_index(k, M.nx, M.x.per, w.val);
+  w.inx = w.val && w.i >= M.x.lo && w.i <= M.x.hi;
+  const bool ok = w.val && L.val;
+  w.te = r.te;
+  w.T = ok ? temp_of(r.te, r.n, a.k) : 0.0;
+  w.bx = r.bx;
+  w.by = r.by;
+  const double fT = yface(L, w.T);
+  const double fTlo = strip::shr1(fT);
+  // dtemp_dy is zero outside the interior (derivs.cpp:229-233)
+  w.gy = (w.inx && L.iny) ? qdiv(fT - fTlo, L.dy, L.rdy) : 0.0;
+  w.s1 = qdiv(fT - 2.0 * w.T + fTlo, L.hsq, L.rhsq);      // secondDerivative1D (derivs.cpp:450)
+  w.dbxy = yderiv(L, yface(L, w.bx));
+  w.dbyy = yderiv(L, yface(L, w.by));
+  if (SAT) {
+    w.coef = r.coef;
+    w.dcy = yderiv(L, yface(L, w.coef));
+  }
+  return w;
+}
+
+// boundaryInterpolate across the x-face between rows A (lower) and B (upper)
+template <bool SAT>
+__device__ __forceinline__ XFaces xfaces(const Mesh& M, const Row& A, const Row& B) {
+  const double ha = M.x.h[A.i], hb = M.x.h[B.i], fd = M.x.fden[B.i], rfd = M.x.rfden[B.i];
+  XFaces f;
+  f.T = qdiv(A.T * hb + B.T * ha, fd, rfd);
+  f.gy = qdiv(A.gy * hb + B.gy * ha, fd, rfd);
+  f.bx = qdiv(A.bx * hb + B.bx * ha, fd, rfd);
+  f.by = qdiv(A.by * hb + B.by * ha, fd, rfd);
+  if (SAT) f.coef = qdiv(A.coef * hb + B.coef * ha, fd, rfd);
+  return f;
+}
+
+}  // namespace tcs
+
+template <int MODE, bool SAT, bool SEP>
+__global__ void __launch_bounds__(64) k_tc_strip(TcStepArgs a, int R, int nstrips, int total) {
+  using namespace tcs;
+  const Mesh& M = a.mesh;
+  const int t = strip::xcd_tile(blockIdx.x, total);
+  const int tile = t / nstrips, sidx = t - tile * nstrips;
+  const int lane = threadIdx.x;
+  Lane L;
+  {
+    const int jc = tile * 62 + lane - 1;
+    L.j = row_index(jc, M.ny, M.y.per, L.val);
+    L.iny = L.val && L.j >= M.y.lo && L.j <= M.y.hi;
+    L.out = lane >= 1 && lane <= 62 && jc < M.ny;
+    bool vp;
+    const int jp = row_index(L.j + 1, M.ny, M.y.per, vp);
+    L.hc = M.y.h[jp];
+    L.hn = M.y.h[L.j];
+    L.fd = M.y.fden[jp];
+    L.rfd = M.y.rfden[jp];
+    L.dy = M.y.d[L.j];
+    L.rdy = M.y.rd[L.j];
+    L.hsq = M.y.hsq[L.j];
+    L.rhsq = M.y.rhsq[L.j];
+  }
+  const int r0 = sidx * R, r1 = min(r0 + R, M.nx);
+  Row cur = make_row<SAT>(a, L, r0, load_raw<SAT>(a, L, r0));
+  XFaces lo;
+  {
+    const Row prev = make_row<SAT>(a, L, r0 - 1, load_raw<SAT>(a, L, r0 - 1));
+    lo = xfaces<SAT>(M, prev, cur);
+  }
+  Raw ahead = load_raw<SAT>(a, L, r0 + 1);
+  for (int r = r0; r < r1; ++r) {
+    const Raw raw = ahead;
+    if (r + 1 < r1) ahead = load_raw<SAT>(a, L, r + 2);
+    const Row nxt = make_row<SAT>(a, L, r + 1, raw);
+    const XFaces hi = xfaces<SAT>(M, cur, nxt);
+    // cell values at row r along x
+    const int i = cur.i;
+    const double dx = M.x.d[i], rdx = M.x.rd[i];
+    const bool in = cur.inx && L.iny;
+    const double gx = qdiv(hi.T - lo.T, dx, rdx);
+    // dtemp_dx is zero outside the interior; its y-derivative takes the neighbouring lanes' values
+    const double dgx_y = yderiv(L, yface(L, in ? gx : 0.0));
+    if (L.out && cur.val) {
+      const size_t c = (size_t)i * M.ny + L.j;
+      const double base = SEP ? a.te_base[c] : cur.te;
+      if (!in) {
+        // mask = 0: te + 0*dE; every derivative is zero there, so dE = 0
+        a.te_out[c] = smax(base + 0.0, a.k.temin);
+        if (MODE == TC_STORE_K) a.k_out[c] = 0.0;
+      } else {
+        const double s0 = qdiv(hi.T - 2.0 * cur.T + lo.T, M.x.hsq[i], M.x.rhsq[i]);
+        const double dgy_x = qdiv(hi.gy - lo.gy, dx, rdx);
+        const double dbxx = qdiv(hi.bx - lo.bx, dx, rdx);
+        const double dbyx = qdiv(hi.by - lo.by, dx, rdx);
+        double dE = tcc::tc_combine<double>(cur.T, gx, cur.gy, s0, cur.s1, dgx_y, dgy_x, cur.bx, cur.by, dbxx,
+                                            cur.dbxy, dbyx, cur.dbyy, a.p.k0);
+        if (SAT) {                                              // thermalconduction.cpp:171-174, 255-271
+          const double dcx = qdiv(hi.coef - lo.coef, dx, rdx);
+          const double add = -1.0 * (dcx * a.rx[c] + cur.dcy * a.ry[c]);
+          dE = cur.coef * dE + add;
+        }
+        double v;
+        if (MODE == TC_RK4_FINAL) {
+          const double sum = a.k1[c] + 2.0 * a.k2[c] + 2.0 * a.k3[c] + dE;
+          v = base + qdiv(1.0 * a.h * sum, 6.0, a.k.r6);
+        } else {
+          if (MODE == TC_STORE_K) a.k_out[c] = dE;
+          v = base + 1.0 * (a.mult * a.h) * dE;
+        }
+        a.te_out[c] = smax(v, a.k.temin);
+      }
+    }
+    cur = nxt;
+    lo = hi;
+  }
+}
+
 // RadiativeLosses::computeLosses at one interior cell (radiativelosses.cpp:149-197).
 __device__ __forceinline__ double rl_loss(double t, double n, double te_live, double dt_live, const RlParams& p) {
   if (t < p.cut) return 0.0;
@@ -372,8 +553,36 @@ void launch_tc_sat(const double* te_src, const double* n, const double* rho, con
                      coef);
 }
 
+int tc_strip_rows_default(int nx, int ny) {
+  const int tiles = (ny + 61) / 62;
+  for (int R : {16, 8, 4})
+    if (tiles * ((nx + R - 1) / R) >= 2048) return R;
+  return 2;
+}
+
+template <int MODE, bool SAT>
+static void launch_tc_strip_m(const TcStepArgs& a, int R, hipStream_t s) {
+  const int tiles = (a.mesh.ny + 61) / 62, nstrips = (a.mesh.nx + R - 1) / R, total = tiles * nstrips;
+  if (a.te_base != a.te_src)
+    hipLaunchKernelGGL((k_tc_strip<MODE, SAT, true>), dim3(total), dim3(64), 0, s, a, R, nstrips, total);
+  else
+    hipLaunchKernelGGL((k_tc_strip<MODE, SAT, false>), dim3(total), dim3(64), 0, s, a, R, nstrips, total);
+}
+
 void launch_tc_step(const TcStepArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_tc_step, g2(a.mesh, dim3(64, 4)), dim3(64, 4), 0, s, a);
+  if (a.rows <= 0) {
+    hipLaunchKernelGGL(k_tc_step, g2(a.mesh, dim3(64, 4)), dim3(64, 4), 0, s, a);
+    return;
+  }
+  if (a.p.saturation) {
+    if (a.mode == TC_STORE_K) launch_tc_strip_m<TC_STORE_K, true>(a, a.rows, s);
+    else if (a.mode == TC_RK4_FINAL) launch_tc_strip_m<TC_RK4_FINAL, true>(a, a.rows, s);
+    else launch_tc_strip_m<TC_UPDATE, true>(a, a.rows, s);
+  } else {
+    if (a.mode == TC_STORE_K) launch_tc_strip_m<TC_STORE_K, false>(a, a.rows, s);
+    else if (a.mode == TC_RK4_FINAL) launch_tc_strip_m<TC_RK4_FINAL, false>(a, a.rows, s);
+    else launch_tc_strip_m<TC_UPDATE, false>(a, a.rows, s);
+  }
 }
 
 void launch_rl_pre(MhdSet L, const double* dt_grid, RlParams p, MhdConst k, Mesh mesh, double* partial, int nblocks,

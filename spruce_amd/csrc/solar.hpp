@@ -47,6 +47,7 @@ struct TcStepArgs {
   const double* dbyy;
   double h, mult;
   int mode;
+  int rows;               // rows per wavefront strip (tc_strip form); 0: one thread per cell
   TcParams p;
   MhdConst k;
   Mesh mesh;
@@ -64,7 +65,11 @@ void launch_tc_pre(MhdSet L, const double* bhx, const double* bhy, TcParams p, M
                    double* partial, int nblocks, double* out, hipStream_t s);
 void launch_tc_sat(const double* te_src, const double* n, const double* rho, const double* bhx, const double* bhy,
                    TcParams p, MhdConst k, Mesh mesh, double* rx, double* ry, double* coef, hipStream_t s);
+// One conduction update: the wavefront-strip form with a.rows rows per strip, or one thread per cell
+// when a.rows is 0 (the b_hat derivative grids are then read, else they are formed in the kernel).
 void launch_tc_step(const TcStepArgs& a, hipStream_t s);
+// rows per strip giving at least 2048 wavefronts where the grid allows
+int tc_strip_rows_default(int nx, int ny);
 // RL subcycle count inputs: out[0] = max losses, out[1] = min |te / losses| (radiativelosses.cpp:200-205).
 void launch_rl_pre(MhdSet L, const double* dt_grid, RlParams p, MhdConst k, Mesh mesh, double* partial, int nblocks,
                    double* out, hipStream_t s);

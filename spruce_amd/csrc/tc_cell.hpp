@@ -43,6 +43,27 @@ template <> struct Zero<double> { __host__ __device__ static double get() { retu
 template <int K> struct Zero<Dn<K>> { __host__ __device__ static Dn<K> get() { return dn_const<K>(0.0); } };
 template <class S> __host__ __device__ __forceinline__ S zero() { return Zero<S>::get(); }
 
+// The energy derivative from the cell's temperature Tc, its gradient (gx, gy), second derivatives
+// (s0 = d2T/dx2, s1 = d2T/dy2), the cross derivatives of the gradient (dgx_y = d(dT/dx)/dy,
+// dgy_x = d(dT/dy)/dx), b_hat and the four derivatives of b_hat (thermalconduction.cpp:158-176,
+// operation for operation).
+template <class S>
+__host__ __device__ __forceinline__ S tc_combine(const S& Tc, const S& gx, const S& gy, const S& s0, const S& s1,
+                                                 const S& dgx_y, const S& dgy_x, const S& bx, const S& by,
+                                                 const S& dbx_x, const S& dbx_y, const S& dby_x, const S& dby_y,
+                                                 double k0) {
+  const S bdg = bx * gx + by * gy;
+  const S t1x = bx * s0 + by * dgx_y, t1y = bx * dgy_x + by * s1;
+  const S t2x = gx * dbx_x + gy * dbx_y, t2y = gx * dby_x + gy * dby_y;
+  const S curl = dby_x - dbx_y;
+  const S t3x = (-(-1.0 * curl)) * gy, t3y = (-1.0 * curl) * gx;
+  const S q15 = p15(Tc), q25 = p25(Tc);
+  const S ttx = 5.0 / 2.0 * q15 * gx * bdg + q25 * (t1x + t2x + t3x);
+  const S tty = 5.0 / 2.0 * q15 * gy * bdg + q25 * (t1y + t2y + t3y);
+  const S divb = dbx_x + dby_y;
+  return ((q25 * bdg * divb) + (bx * ttx + by * tty)) * k0;
+}
+
 // T[a][b] is the temperature of cell (i-1+a, j-1+b) (periodic neighbours wrapped).
 template <class S>
 __host__ __device__ inline S tc_core(const Mesh& M, int i, int j, const S T[3][3], const S& bx, const S& by,
@@ -74,16 +95,7 @@ __host__ __device__ inline S tc_core(const Mesh& M, int i, int j, const S T[3][3
   const S s0 = dv(Txu - 2.0 * Tc + Txl, M.x.hsq[i], M.x.rhsq[i]);
   const S s1 = dv(Tyu - 2.0 * Tc + Tyl, M.y.hsq[j], M.y.rhsq[j]);
 
-  const S bdg = bx * gx + by * gy;
-  const S t1x = bx * s0 + by * dgx_y, t1y = bx * dgy_x + by * s1;
-  const S t2x = gx * dbx_x + gy * dbx_y, t2y = gx * dby_x + gy * dby_y;
-  const S curl = dby_x - dbx_y;
-  const S t3x = (-(-1.0 * curl)) * gy, t3y = (-1.0 * curl) * gx;
-  const S q15 = p15(Tc), q25 = p25(Tc);
-  const S ttx = 5.0 / 2.0 * q15 * gx * bdg + q25 * (t1x + t2x + t3x);
-  const S tty = 5.0 / 2.0 * q15 * gy * bdg + q25 * (t1y + t2y + t3y);
-  const S divb = dbx_x + dby_y;
-  return ((q25 * bdg * divb) + (bx * ttx + by * tty)) * k0;
+  return tc_combine<S>(Tc, gx, gy, s0, s1, dgx_y, dgy_x, bx, by, dbx_x, dbx_y, dby_x, dby_y, k0);
 }
 
 // Reverse mode of tc_core<double> at interior cell (i, j): returns the value and adds ld * dR/dT to

@@ -62,6 +62,9 @@ def lib() -> ctypes.CDLL:
     L.spruce_gpu_time.argtypes = [P, DP, ctypes.POINTER(I32)]
     L.spruce_gpu_rhs.argtypes = [P, DP, I64]
     L.spruce_gpu_dt_min.argtypes = [P, DP]
+    L.spruce_gpu_put_grid.argtypes = [P, ctypes.c_char_p, DP, I64]
+    L.spruce_gpu_ghost_fill.argtypes = [P]
+    L.spruce_gpu_propagate.argtypes = [P, DP]
     L.spruce_gpu_num_evolved.argtypes = [P]
     L.spruce_gpu_stream.argtypes = [P, ctypes.POINTER(P)]
     L.spruce_gpu_device_grid.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(DP)]
@@ -184,6 +187,20 @@ class GpuDomain:
     def dt_min(self) -> float:
         v = ctypes.c_double()
         _chk(lib().spruce_gpu_dt_min(self.h, ctypes.byref(v)))
+        return v.value
+
+    def put_grid(self, name: str, g: np.ndarray) -> None:
+        """Overwrite one evolved grid of the live state without propagating."""
+        a = np.ascontiguousarray(g, dtype=np.float64)
+        _chk(lib().spruce_gpu_put_grid(self.h, name.encode(), _dp(a), a.size))
+
+    def ghost_fill(self) -> None:
+        _chk(lib().spruce_gpu_ghost_fill(self.h))
+
+    def propagate(self) -> float:
+        """propagateChanges on the live grids; returns the next step size."""
+        v = ctypes.c_double()
+        _chk(lib().spruce_gpu_propagate(self.h, ctypes.byref(v)))
         return v.value
 
     @property

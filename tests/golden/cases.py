@@ -362,8 +362,17 @@ def _cases() -> Dict[str, Tuple[Callable[[], StateFile], Callable[[], ConfigFile
     c["cons_open_setup"] = (lambda: W.mhd_variant_state("ideal_mhd_cons", 18, 22, 45, stretch=True),
                             lambda: _var("ideal_mhd_cons", ("fixed", "open", "reflect", "open_ucnp"), "rk4", 0, CO,
                                          open_boundary_strength=0.3, open_boundary_decay_base=0.6))
+    # the arcade workload's own physics and module settings (workloads.arcade_config: conduction and
+    # radiative losses, Euler subcycles, RK2) on a 64x128 arcade for 200 steps, long enough for the
+    # conduction subcycle count to settle; frames every 50 steps (KEEP_EVERY)
+    c["solar_arcade_c1_64x128"] = (lambda: W.arcade_state(64, 128),
+                                   lambda: W.arcade_config(steps=200, output_flags=LONG_OUT))
     return c
 
+
+LONG_OUT = ["rho", "temp", "mom_x", "mom_y", "bi_x", "bi_y", "thermal_energy", "dt"]
+# cases whose golden file keeps every k-th frame (and the last) instead of frames 0, 1 and the last
+KEEP_EVERY = {"solar_arcade_c1_64x128": 50}
 
 CASES = _cases()
 

@@ -6,15 +6,22 @@ Usage (in a container that has /root/reference):
 
 For each case the reference executable runs in input mode with ``write_precision = 17`` (which
 round-trips every double), and the frames after setup, after step 1 and after the last step are
-kept for the variables listed in the case's ``output_flags``.  The step sizes of every step are
+kept for the variables listed in the case's ``output_flags`` (every k-th frame for the long cases of
+``KEEP_EVERY``).  The step sizes of every step are
 reconstructed exactly from the ``dt`` grid of the preceding frame,
 ``dt_step = epsilon * min(dt over the dt bounds)`` (the interior, widened by the ghost cells of open_moc sides) (evolution.cpp:62; grid.cpp:71-82).
 
 A successful reference run ends in SIGABRT after its success message (evolution.cpp:54-56); that
 exit is accepted.  Outputs: ``tests/golden/<case>.npz`` holding the input texts and the vectors.
+
+Each case also runs once more with ``std_out_interval = 1`` (nothing else changed): the reference's
+progress line of every step (PlasmaDomain::printUpdate, fileio.cpp:283-295), which carries the
+modules' integer subcycle counts, goes to ``tests/golden/progress.json``.
+``python tests/golden/make_golden.py --progress [case ...]`` refreshes only those lines.
 """
 from __future__ import annotations
 
+import json
 import os
 import subprocess
 import sys
@@ -24,7 +31,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
-from cases import CASES, ROOT, build_case  # noqa: E402
+from cases import CASES, KEEP_EVERY, ROOT, build_case  # noqa: E402
 
 from spruce_amd.stateio import parse_config, read_out  # noqa: E402
 
@@ -76,7 +83,10 @@ def run_case(name: str) -> str:
         g = out.frames[k]["dt"][xl:xu + 1, yl:yu + 1]
         g = g[~np.isnan(g)]
         dts.append(eps * float(g.min()))
-    keep = sorted({0, min(1, nframes - 1), nframes - 1})
+    if name in KEEP_EVERY:
+        keep = sorted(set(range(0, nframes, KEEP_EVERY[name])) | {nframes - 1})
+    else:
+        keep = sorted({0, min(1, nframes - 1), nframes - 1})
     data = np.stack([np.stack([out.frames[k][v] for v in names]) for k in keep])
     path = os.path.join(HERE, name + ".npz")
     np.savez_compressed(path,
@@ -87,13 +97,44 @@ def run_case(name: str) -> str:
     return path
 
 
+PROGRESS = os.path.join(HERE, "progress.json")
+
+
+def run_progress(name: str):
+    """The reference's per-step progress lines of a case (the case's config with std_out_interval = 1)."""
+    st, cfg = build_case(name)
+    cfg.settings["std_out_interval"] = "1"
+    with tempfile.TemporaryDirectory() as tmp:
+        sp = os.path.join(tmp, "case.state")
+        cp = os.path.join(tmp, "case.config")
+        with open(sp, "w") as f:
+            f.write(st.to_text(17))
+        with open(cp, "w") as f:
+            f.write(cfg.to_text())
+        r = subprocess.run([REF_BIN, "-m", "input", "-o", os.path.join(tmp, "out"), "-s", sp, "-c", cp],
+                           capture_output=True, text=True, timeout=600, cwd=tmp)
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("Iter: ")]
+    steps = int(cfg.settings["max_iterations"])
+    if len(lines) != steps:
+        raise RuntimeError("%s: %d progress lines for %d steps (rc=%d)" % (name, len(lines), steps, r.returncode))
+    return lines
+
+
 def main(argv):
     if not os.path.exists(REF_BIN):
         sys.exit("build the reference first: make -C oracle ref")
-    names = argv[1:] or sorted(CASES)
+    progress_only = "--progress" in argv
+    names = [a for a in argv[1:] if not a.startswith("--")] or sorted(CASES)
+    prog = json.load(open(PROGRESS)) if os.path.exists(PROGRESS) else {}
     for n in names:
-        p = run_case(n)
-        print("%-28s -> %s (%d bytes)" % (n, os.path.relpath(p, ROOT), os.path.getsize(p)))
+        if not progress_only:
+            p = run_case(n)
+            print("%-28s -> %s (%d bytes)" % (n, os.path.relpath(p, ROOT), os.path.getsize(p)))
+        prog[n] = run_progress(n)
+        print("%-28s progress: %d lines" % (n, len(prog[n])))
+    with open(PROGRESS, "w") as f:
+        json.dump(dict(sorted(prog.items())), f, indent=0)
+        f.write("\n")
 
 
 if __name__ == "__main__":

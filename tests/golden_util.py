@@ -2,9 +2,11 @@
 from __future__ import annotations
 
 import glob
+import json
 import os
+import re
 from dataclasses import dataclass
-from typing import Dict, List
+from typing import Dict, List, Optional
 
 import numpy as np
 
@@ -45,3 +47,41 @@ def load_golden(name: str) -> Golden:
     z = np.load(os.path.join(GOLDEN_DIR, name + ".npz"), allow_pickle=False)
     return Golden(name, bytes(z["state"]).decode(), bytes(z["config"]).decode(), [str(s) for s in z["names"]],
                   [int(f) for f in z["frames"]], z["data"], z["dt_steps"], z["bounds"])
+
+
+# Modules whose progress message carries their integer subcycle count (thermalconduction.cpp:244,
+# radiativelosses.cpp:171, anomalousresistivity.cpp:317, physicalviscosity.cpp:286).
+COUNTED_MODULES = ("thermal_conduction", "radiative_losses", "anomalous_resistivity", "physical_viscosity")
+_COUNT = re.compile(r"(?:Subcycles: (\d+)|, (\d+) Subcycle\(s\))")
+
+
+def load_progress(name: str) -> Optional[List[str]]:
+    """The reference's per-step progress lines of a golden case (tests/golden/progress.json)."""
+    path = os.path.join(GOLDEN_DIR, "progress.json")
+    if not os.path.exists(path):
+        return None
+    return json.load(open(path)).get(name)
+
+
+def progress_counts(line: str) -> List[int]:
+    """The subcycle counts of one progress line, in module order."""
+    out = []
+    for part in line.split("|")[3:]:
+        m = _COUNT.search(part)
+        if m:
+            out.append(int(m.group(1) or m.group(2)))
+    return out
+
+
+def counted_cases() -> List[str]:
+    """Golden cases whose progress lines carry subcycle counts."""
+    path = os.path.join(GOLDEN_DIR, "progress.json")
+    if not os.path.exists(path):
+        return []
+    prog = json.load(open(path))
+    return sorted(k for k, v in prog.items() if v and any(progress_counts(x) for x in v))
+
+
+def counted_module_indices(spec) -> List[int]:
+    """Indices (config order) of the modules whose counts the progress lines print."""
+    return [k for k, (name, _) in enumerate(spec.modules) if name in COUNTED_MODULES]

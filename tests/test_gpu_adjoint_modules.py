@@ -257,3 +257,80 @@ def test_module_adjoint_refuses_what_it_does_not_cover():
                 d.adjoint(2, 1, np.zeros((8, 24, 40)))
         finally:
             d.close()
+
+
+@pytest.mark.parametrize("active", [[5], [0, 11]])
+def test_parametric_heating_closed_form(active):
+    """On a static uniform plasma the ideal-MHD step changes nothing, so one step of a heating
+    profile with coefficients theta_k raises the thermal energy of every interior cell by
+    dt * sum_k theta_k phi_k(y), phi_k(y) = exp(-(y - y_k)^2 / (2 w^2)) with centres y_k spread evenly
+    from the lowest to the highest cell centre and w = basis_width * spacing, and leaves the ghost
+    cells alone."""
+    nx, ny, K = 12, 40, 16
+    dy = 4e6 * 1.01 ** np.arange(ny)
+    z = np.zeros((nx, ny))
+    st = W.mhd_state(nx, ny, [5e6] * nx, dy, np.full((nx, ny), 1e-15), np.full((nx, ny), 1e6), (z, z, z),
+                     (z, z, z), be=(np.full((nx, ny), 2.0), z, z))
+    te0 = 2.0 * (1e-15 / W.M_PROTON) * W.K_B * 1e6 / (W.GAMMA - 1.0)
+    theta = np.zeros(K)
+    for k in active:
+        theta[k] = 10.0 * te0 * (1 + k)
+    cfg = W.base_config("ideal_mhd", ("periodic", "periodic", "fixed", "fixed"), "euler", epsilon=0.2,
+                        max_iterations=1, density_min=1.0, temp_min=1.0, thermal_energy_min=1e-12,
+                        modules=[("parametric_heating", [("num_basis", str(K)), ("basis_width", "1.5")] +
+                                  [("coeff_%d" % k, repr(float(theta[k]))) for k in active])])
+    gpu = GpuDomain(spec_from_files(cfg, st))
+    try:
+        before = gpu.grid("thermal_energy")
+        dt = float(gpu.step(1)[0])
+        after = gpu.grid("thermal_energy")
+    finally:
+        gpu.close()
+    y = st.grids["pos_y"][0]
+    lo, hi = float(np.min(st.grids["pos_y"])), float(np.max(st.grids["pos_y"]))
+    sp = (hi - lo) / (K - 1)
+    w = 1.5 * sp
+    phi = np.exp(-0.5 * ((y[None, :] - (lo + sp * np.arange(K)[:, None])) / w) ** 2)   # [K, ny]
+    want = dt * (theta @ phi)
+    interior = slice(2, ny - 2)
+    got = after - before
+    # the difference of two thermal energies carries their rounding: 1e-13 of the largest increment
+    np.testing.assert_allclose(got[:, interior], np.broadcast_to(want[interior], (nx, ny - 4)), rtol=1e-12,
+                               atol=1e-13 * float(np.max(np.abs(want))))
+    assert float(np.max(np.abs(want))) > 0.05 * te0
+    assert np.all(got[:, :2] == 0.0) and np.all(got[:, -2:] == 0.0)
+
+
+def test_parametric_heating_rejects_heights_that_vary_along_x():
+    """The heating basis is a function of the height of each row; a mesh whose cell heights differ
+    between columns is refused at set-up."""
+    st = _arcade()
+    st.grids["pos_y"] = st.grids["pos_y"] * (1.0 + 1e-3 * np.arange(st.xdim)[:, None])
+    cfg = W.base_config("ideal_mhd", ("periodic", "periodic", "fixed", "open"), "rk2", epsilon=0.2,
+                        max_iterations=1, density_min=1.0, temp_min=1e4, thermal_energy_min=1e-8,
+                        modules=[HEAT(THETA)])
+    with pytest.raises(SpruceError):
+        GpuDomain(spec_from_files(cfg, st))
+
+
+def test_arcade_2048_heating_gradient_matches_finite_differences():
+    """The gradient workload at its full size: the 2048x2048 arcade with conduction, radiative
+    losses and the 16-coefficient heating, dJ/dtheta from the adjoint against central differences
+    (relative step 1e-5) on 4 coefficients, over 5 steps, a window in which the conduction subcycle
+    count does not change with theta (a change makes the loss jump)."""
+    from spruce_amd.adjoint import ArcadeHeatingProblem
+
+    prob = ArcadeHeatingProblem(2048, 2048, nsteps=5, checkpoint=50)
+    th = prob.theta0
+    print("arcade 2048: forward", flush=True)
+    _, final, _ = prob.forward(th)
+    g, _, _ = prob.gradient(th, final)
+    for k in (0, 5, 10, 15):
+        e = 1e-5 * th[k]
+        tp, tm = th.copy(), th.copy()
+        tp[k] += e
+        tm[k] -= e
+        fd = (prob.forward(tp)[0] - prob.forward(tm)[0]) / (tp[k] - tm[k])
+        rel = abs(g[k] - fd) / (abs(g[k]) + abs(fd))
+        print("arcade 2048: theta_%d adjoint %.9e fd %.9e rel %.2e" % (k, g[k], fd, rel), flush=True)
+        assert rel <= 1e-6, (k, g[k], fd)

@@ -11,7 +11,7 @@ uses correctly rounded powers.  Hence:
 import numpy as np
 import pytest
 
-from golden_util import golden_names, load_golden
+from golden_util import counted_cases, counted_module_indices, golden_names, load_golden, load_progress, progress_counts
 from oracle.oracle import OracleDomain, cr_pow, set_pow_mode
 from spruce_amd import workloads as W
 from spruce_amd.gpu import POW_EXPONENTS, GpuDomain, pow_probe
@@ -112,3 +112,21 @@ def test_arcade_multi_block_matches_cr_restatement(cr_oracle):
     finally:
         gpu.close()
         ora.close()
+
+
+@pytest.mark.parametrize("case", counted_cases())
+def test_subcycle_counts_match_reference_progress(case):
+    """The GPU's integer subcycle counts of every step equal the ones the reference printed for that
+    step (fileio.cpp:283-295): conduction, radiation, anomalous resistivity, physical viscosity."""
+    g = load_golden(case)
+    lines = load_progress(case)
+    spec = g.spec()
+    idx = counted_module_indices(spec)
+    gpu = GpuDomain(spec)
+    try:
+        for s, line in enumerate(lines):
+            _, sub = gpu.step(1, subcycles=True)
+            got = [int(sub[0][k]) for k in idx]
+            assert got == progress_counts(line), "%s step %d: %s vs reference %r" % (case, s, got, line)
+    finally:
+        gpu.close()

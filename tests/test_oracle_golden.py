@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from golden_util import golden_names, load_golden
+from golden_util import counted_cases, counted_module_indices, golden_names, load_golden, load_progress, progress_counts
 from oracle.oracle import OracleDomain
 
 CASES = golden_names()
@@ -30,5 +30,24 @@ def test_oracle_matches_reference(case):
                 if not _bits_equal(got, ref):
                     diff = np.nanmax(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300))
                     pytest.fail("%s: frame %d grid %s differs (max rel %.3e)" % (case, k, name, diff))
+    finally:
+        dom.close()
+
+
+@pytest.mark.parametrize("case", counted_cases())
+def test_oracle_subcycle_counts_match_reference_progress(case):
+    """The integer subcycle counts of every step equal the ones the reference printed for it
+    (fileio.cpp:283-295), module by module."""
+    g = load_golden(case)
+    lines = load_progress(case)
+    assert len(lines) == g.nsteps
+    spec = g.spec()
+    idx = counted_module_indices(spec)
+    dom = OracleDomain(spec)
+    try:
+        for s, line in enumerate(lines):
+            dom.step(1)
+            got = [dom.subcycles()[k] for k in idx]
+            assert got == progress_counts(line), "%s step %d: %s vs reference %r" % (case, s, got, line)
     finally:
         dom.close()
