@@ -469,8 +469,10 @@ def main() -> None:
     ap.add_argument("--stage-kernel", default="auto", choices=["auto", "strip", "cell"])
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0, help="edge of the square arcade of the adjoint_arcade workload")
-    ap.add_argument("--tc-kernel", default="strip", choices=["strip", "cell"])
+    ap.add_argument("--tc-kernel", default="auto", choices=["auto", "strip", "cell"])
     ap.add_argument("--tc-rows", type=int, default=0)
+    ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
+                    help="execution option of the GPU domain (spruce_gpu_set_option), repeatable")
     ap.add_argument("--cpu-malloc", default="heap", choices=["heap", "default"],
                     help="glibc malloc tunables of the reference CPU leg")
     args = ap.parse_args()
@@ -504,6 +506,9 @@ def main() -> None:
         opts["strip_rows"] = args.strip_rows
     if args.tc_rows:
         opts["tc_rows"] = args.tc_rows
+    for kv in args.option:
+        k, _, v = kv.partition("=")
+        opts[k] = v
     dom = GpuDomain(spec, options=opts)
     dom.step(args.warmup)
     snapshot = None
@@ -553,7 +558,8 @@ def main() -> None:
         "dtype": "f64",
         "data": "synthetic",
         "config": dict(desc, stage_kernel=kernel, strip_rows=args.strip_rows or "auto", tc_kernel=args.tc_kernel,
-                       tc_rows=args.tc_rows or "auto", timed_steps=[args.warmup, args.warmup + args.steps - 1]),
+                       tc_rows=args.tc_rows or "auto", options=args.option,
+                       timed_steps=[args.warmup, args.warmup + args.steps - 1]),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "kernel": dominant, "kernel_ms": kms, "bytes_per_cell": bpc,

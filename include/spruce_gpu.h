@@ -130,6 +130,10 @@ int spruce_gpu_device_grid(spruce_gpu_domain* d, const char* name, double** dev_
  *                    wavefronts, else one thread per cell) | "strip" (wavefront strips marching
  *                    along x) | "cell" (one thread per cell, faces recomputed per cell);
  *   "strip_rows"   = rows per wavefront strip, a multiple of 4 (default: chosen from the grid size);
+ *   "tc_kernel"    = thermal-conduction update: "auto" (default: strips when 8-row strips give at
+ *                    least 4096 wavefronts, else one thread per cell) | "strip" (wavefront strips,
+ *                    every face computed once, b_hat derivatives formed in the kernel) | "cell";
+ *   "tc_rows"      = rows per conduction strip, 1..256 (default: chosen from the grid size);
  *   "block"        = ideal_2F launch shape: "<columns>x<rows>" thread block of every kernel, stage
  *                    one thread per cell (default 64x4), or "strip<rows>" (rows a multiple of 4
  *                    up to 64): the stage kernel in wavefront strips, the other kernels in 64x4. */

@@ -144,7 +144,7 @@ class Domain : public DomainBase {
   bool any_open = false;
   int stage_kernel = -1;  // 0: wavefront strips (mhd2.hip), 1: one thread per cell (mhd.hip), -1: by size
   int strip_rows = 0;     // rows per wavefront strip; 0 = chosen from the grid size
-  int tc_kernel = 0;      // conduction update: 0 wavefront strips, 1 one thread per cell
+  int tc_kernel = -1;     // conduction update: 0 wavefront strips, 1 one thread per cell, -1 by size
   int tc_rows = 0;        // rows per conduction strip; 0 = chosen from the grid size
 
   void set_option(const std::string& k, const std::string& v) override {
@@ -156,8 +156,8 @@ class Domain : public DomainBase {
       require(r >= 4 && r <= 256 && r % 4 == 0, SPRUCE_E_ARG, "strip_rows must be a multiple of 4 in [4, 256]");
       strip_rows = r;
     } else if (k == "tc_kernel") {
-      require(v == "strip" || v == "cell", SPRUCE_E_ARG, "tc_kernel must be strip or cell");
-      tc_kernel = v == "cell" ? 1 : 0;
+      require(v == "strip" || v == "cell" || v == "auto", SPRUCE_E_ARG, "tc_kernel must be strip, cell or auto");
+      tc_kernel = v == "cell" ? 1 : (v == "strip" ? 0 : -1);
     } else if (k == "tc_rows") {
       const int r = std::stoi(v);
       require(r >= 1 && r <= 256, SPRUCE_E_ARG, "tc_rows must be in [1, 256]");
@@ -381,6 +381,7 @@ class Domain : public DomainBase {
 
     if (strip_rows == 0) strip_rows = mhd_strip_rows_default(nx, ny);
     if (tc_rows == 0) tc_rows = tc_strip_rows_default(nx, ny);
+    if (tc_kernel < 0) tc_kernel = tc_kernel_default(nx, ny);
     if (stage_kernel < 0) stage_kernel = mhd_stage_kernel_default(nx, ny);
     dt_blocks = mhd_dt_blocks(mesh);
     partial.alloc(dt_blocks);

@@ -2,10 +2,12 @@
 // source/equationsets/idealmhd.cpp:107-224, 308-615; source/mhd/derivs.cpp:10-73, 77-117,
 // 167-214, 223-264, 317-404, 417-455, 477-499).
 //
-// Only the ghost cells of open_moc sides are evaluated, one thread per cell, straight from the
-// stage's input set: every whole-grid expression of singleBoundaryTermsMOC is evaluated at the
-// cell with the reference's operation order, and every derivative the reference takes over a
-// sub-range is zero outside that range, as the reference's zero-initialised result grids are.
+// Only the ghost cells of open_moc sides are evaluated, straight from the stage's input set, in
+// two passes: k_moc_ops evaluates every derivative the characteristic terms take, one thread per
+// (derivative, ghost cell) in derivative-major order, and k_moc_rhs combines them, one thread per
+// ghost cell.  Every whole-grid expression of singleBoundaryTermsMOC is evaluated at the cell with
+// the reference's operation order, and every derivative the reference takes over a sub-range is
+// zero outside that range, as the reference's zero-initialised result grids are.
 #include <cfloat>
 
 #include "../../include/spruce_gpu.h"

@@ -350,8 +350,14 @@ __device__ __forceinline__ XFaces xfaces(const Mesh& M, const Row& A, const Row&
 
 }  // namespace tcs
 
+// Four wavefronts per SIMD (at most 128 VGPRs; the compiler spills about 16 bytes per lane, outside
+// the row loop): 79.9 us per subcycle on the 2048x2048 arcade against 84.3 us at 152 VGPRs and three
+// wavefronts per SIMD.  Reading the four b_hat derivatives from grids written once per step instead
+// of forming them here was slower (102 us): the kernel is bound by latency and issue, not by the
+// VALU work those derivatives take.
 template <int MODE, bool SAT, bool SEP>
-__global__ void __launch_bounds__(64) k_tc_strip(TcStepArgs a, int R, int nstrips, int total) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+k_tc_strip(TcStepArgs a, int R, int nstrips, int total) {
   using namespace tcs;
   const Mesh& M = a.mesh;
   const int t = strip::xcd_tile(blockIdx.x, total);
@@ -553,11 +559,20 @@ void launch_tc_sat(const double* te_src, const double* n, const double* rho, con
                      coef);
 }
 
+// Measured on one MI355X: on the 2048x2048 arcade strips of 8 rows (8704 wavefronts) take 80 us
+// per subcycle, against 85 us with 4 rows, 92 us with 16 and 124 us with one thread per cell; on the
+// 512x1024 arcade the strips give too few wavefronts (2176 with 4 rows) and one thread per cell is
+// faster (20.3 us against 22.0 us).
 int tc_strip_rows_default(int nx, int ny) {
   const int tiles = (ny + 61) / 62;
-  for (int R : {16, 8, 4})
-    if (tiles * ((nx + R - 1) / R) >= 2048) return R;
+  for (int R : {8, 4})
+    if (tiles * ((nx + R - 1) / R) >= 4096) return R;
   return 2;
+}
+
+int tc_kernel_default(int nx, int ny) {
+  const int tiles = (ny + 61) / 62;
+  return tiles * ((nx + 7) / 8) >= 4096 ? 0 : 1;
 }
 
 template <int MODE, bool SAT>

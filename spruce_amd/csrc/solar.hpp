@@ -68,8 +68,10 @@ void launch_tc_sat(const double* te_src, const double* n, const double* rho, con
 // One conduction update: the wavefront-strip form with a.rows rows per strip, or one thread per cell
 // when a.rows is 0 (the b_hat derivative grids are then read, else they are formed in the kernel).
 void launch_tc_step(const TcStepArgs& a, hipStream_t s);
-// rows per strip giving at least 2048 wavefronts where the grid allows
+// rows per strip giving at least 4096 wavefronts where the grid allows
 int tc_strip_rows_default(int nx, int ny);
+// 0 (strips) when 8-row strips give at least 4096 wavefronts, else 1 (one thread per cell)
+int tc_kernel_default(int nx, int ny);
 // RL subcycle count inputs: out[0] = max losses, out[1] = min |te / losses| (radiativelosses.cpp:200-205).
 void launch_rl_pre(MhdSet L, const double* dt_grid, RlParams p, MhdConst k, Mesh mesh, double* partial, int nblocks,
                    double* out, hipStream_t s);

@@ -16,7 +16,8 @@ import numpy as np
 from .spec import DomainSpec
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libspruce_gpu.so")
+# the in-tree build; SPRUCE_GPU_LIB names another build of the same library
+LIB_PATH = os.environ.get("SPRUCE_GPU_LIB") or os.path.join(_HERE, "libspruce_gpu.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "spruce_gpu.h")
 _LIB: Optional[ctypes.CDLL] = None
 

@@ -26,7 +26,9 @@ def test_library_exports_every_declared_symbol():
 def test_reciprocal_corrected_division_is_exact():
     with tempfile.TemporaryDirectory() as tmp:
         exe = os.path.join(tmp, "arith_check")
-        subprocess.run(["gcc", "-O2", "-o", exe, os.path.join(HERE, "arith_check.c"), "-lm"], check=True)
+        csrc = os.path.join(os.path.dirname(HERE), "spruce_amd", "csrc")
+        subprocess.run(["g++", "-O2", "-ffp-contract=off", "-I" + csrc, "-o", exe, os.path.join(HERE, "arith_check.cpp"),
+                        "-lm"], check=True)
         r = subprocess.run([exe, "20000000"], capture_output=True, text=True)
         assert r.returncode == 0, r.stdout
 
@@ -57,3 +59,15 @@ def test_format_matches_ostream():
     assert stateio.format_double(0.1, 17) == "0.10000000000000001"
     assert stateio.format_double(1e-15, 4) == "1e-15"
     assert stateio.format_double(202020202.020202, -1) == "202020202.020202"
+
+
+def test_correctly_rounded_powers_match_quad_precision():
+    """The powers the conduction, radiation and viscosity kernels use (crpow.hpp: x^1.5, x^2.5,
+    x^-1.5, x^(1/3)) equal libquadmath's powq rounded to double, on 3e5 samples per exponent over
+    2^-60..2^100 and 8..8e9."""
+    with tempfile.TemporaryDirectory() as tmp:
+        exe = os.path.join(tmp, "crpow_check")
+        subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-o", exe, os.path.join(HERE, "crpow_check.cpp"),
+                        "-lquadmath", "-lm"], check=True)
+        r = subprocess.run([exe, "300000"], capture_output=True, text=True)
+        assert r.returncode == 0, r.stdout

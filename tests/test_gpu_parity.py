@@ -37,6 +37,12 @@ def test_device_division_and_sqrt_are_ieee():
     d, s, q = arith_probe(a, b)
     assert same_bits(d, a / b)
     assert same_bits(q, a / b)
+    # exponents on both sides of the thresholds where the corrected-reciprocal quotient hands over to
+    # the IEEE division (|a| near 2^+-700, |a/b| near 2^+-850)
+    a2 = np.ldexp(rng.uniform(1, 2, n), rng.integers(-720, 720, n)) * rng.choice([-1, 1], n)
+    b2 = np.ldexp(rng.uniform(1, 2, n), rng.integers(-170, 170, n))
+    _, _, q2 = arith_probe(a2, b2)
+    assert same_bits(q2, a2 / b2)
     pos = np.abs(a)
     _, s2, _ = arith_probe(pos, b)
     assert same_bits(s2, np.sqrt(pos))
