@@ -117,6 +117,15 @@ int spruce_gpu_num_evolved(spruce_gpu_domain* d);
  *      (evolution.cpp:62) becomes the domain's and is written to step_out (may be NULL).
  *      ideal_mhd only. */
 int spruce_gpu_put_grid(spruce_gpu_domain* d, const char* name, const double* host, int64_t count);
+
+/* Allocation-free variant of spruce_gpu_rhs; it has no counterpart in the reference.  It is for
+ * callers that cannot allocate device memory while they run (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the eight derivative grids are staged in a
+ * device array the library holds for the life of the process, one per GPU, instead of in memory
+ * allocated per call.  Limit: grids of at most 65,536 cells (the array is 4 MiB); larger grids
+ * return SPRUCE_E_ARG.  Every call on one GPU uses the same array, so calls that overlap on one GPU
+ * (from two host threads, or two handles on two streams) race.  ideal_mhd only. */
+int spruce_gpu_rhs_noalloc(spruce_gpu_domain* d, double* host_out, int64_t count);
 int spruce_gpu_ghost_fill(spruce_gpu_domain* d);
 int spruce_gpu_propagate(spruce_gpu_domain* d, double* step_out);
 

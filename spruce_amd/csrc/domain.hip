@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <sstream>
 #include <string>
@@ -35,6 +36,31 @@
 namespace spruce {
 
 thread_local std::string g_err;
+
+// Staging array of spruce_gpu_rhs_noalloc: the eight derivative grids of a grid of up to
+// NOALLOC_CELLS cells (4 MiB).  Every GPU the library runs on holds its own copy until the process
+// ends.  In host code the array's name is not a device address: the kernels get the address
+// hipGetSymbolAddress returns for the device, looked up once per device with that device current.
+constexpr size_t NOALLOC_CELLS = 65536;
+__device__ __align__(256) double scratch[8 * NOALLOC_CELLS];
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[64] = {};
+
+static double* noalloc_scratch(int dev) {
+  require(dev >= 0 && dev < 64, SPRUCE_E_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    hip_check(hipGetDevice(&cur), "hipGetDevice");
+    if (cur != dev) hip_check(hipSetDevice(dev), "hipSetDevice");
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    hip_check(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
 
 static const char* PD_NAMES[7] = {"d_x", "d_y", "pos_x", "pos_y", "be_x", "be_y", "be_z"};
 
@@ -178,6 +204,7 @@ class Domain : public DomainBase {
     ny = d.ydim;
     N = (size_t)nx * ny;
     time = d.time;
+    hip_check(hipGetDevice(&device), "hipGetDevice");
     hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
   }
   ~Domain() {
@@ -646,11 +673,27 @@ class Domain : public DomainBase {
     require(count == (int64_t)(8 * N), SPRUCE_E_ARG, "rhs output must hold nevolved*xdim*ydim doubles");
     DBuf tmp;
     tmp.alloc(8 * N);
-    launch_mhd_rhs_full(L.view(), stat(), kc, mesh, tmp.p, stream, moc_terms(L.view()));
+    rhs_staged(tmp.p, out);
+  }
+
+  // The eight derivative grids of the live state, computed into dev (8*N doubles of device memory,
+  // every element written) and copied to out.  The kernels and module hooks use buffers allocated
+  // at set-up only.
+  void rhs_staged(double* dev, double* out) {
+    launch_mhd_rhs_full(L.view(), stat(), kc, mesh, dev, stream, moc_terms(L.view()));
     for (auto& m : mods) m->before_integrator(*this);
-    for (auto& m : mods) m->add_rhs(*this, L, tmp.p);
-    hip_check(hipMemcpyAsync(out, tmp.p, 8 * N * 8, hipMemcpyDeviceToHost, stream), "download rhs");
+    for (auto& m : mods) m->add_rhs(*this, L, dev);
+    hip_check(hipMemcpyAsync(out, dev, 8 * N * 8, hipMemcpyDeviceToHost, stream), "download rhs");
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  }
+
+  void rhs_noalloc(double* out, int64_t count) override {
+    require(ready, SPRUCE_E_STATE, "setup must precede rhs");
+    require(count == (int64_t)(8 * N), SPRUCE_E_ARG, "rhs output must hold nevolved*xdim*ydim doubles");
+    require(N <= NOALLOC_CELLS, SPRUCE_E_ARG,
+            "spruce_gpu_rhs_noalloc handles grids of at most " + std::to_string(NOALLOC_CELLS) + " cells (this one has " +
+                std::to_string(N) + "); use spruce_gpu_rhs");
+    rhs_staged(noalloc_scratch(device), out);
   }
 
   static void copy_set(SetBufs& from, SetBufs& to, size_t N, hipStream_t s) {
@@ -2604,6 +2647,13 @@ int spruce_gpu_propagate(spruce_gpu_domain* h, double* step_out) {
   API_BEGIN
   const double v = dom(h).propagate();
   if (step_out) *step_out = v;
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_rhs_noalloc(spruce_gpu_domain* h, double* host_out, int64_t count) {
+  API_BEGIN
+  require(host_out, SPRUCE_E_ARG, "null argument");
+  dom(h).rhs_noalloc(host_out, count);
   API_END(SPRUCE_OK)
 }
 

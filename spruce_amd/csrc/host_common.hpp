@@ -164,6 +164,11 @@ class DomainBase {
   }
   virtual void ghost_fill() { throw Error(SPRUCE_E_UNSUPPORTED, "ghost_fill is available for ideal_mhd only"); }
   virtual double propagate() { throw Error(SPRUCE_E_UNSUPPORTED, "propagate is available for ideal_mhd only"); }
+  // rhs() staged through the library's static device array instead of memory allocated per call
+  virtual void rhs_noalloc(double* out, int64_t count) {
+    (void)out; (void)count;
+    throw Error(SPRUCE_E_UNSUPPORTED, "rhs_noalloc is available for ideal_mhd only");
+  }
   // discrete adjoint of nsteps steps (only some equation sets provide it)
   virtual void adjoint(int nsteps, int checkpoint, const double* seed, double* grad, double* dt_out) {
     (void)nsteps; (void)checkpoint; (void)seed; (void)grad; (void)dt_out;
@@ -182,6 +187,7 @@ class DomainBase {
 
   double time = 0.0;
   int iter = 0;
+  int device = 0;                 // the GPU the handle's memory and stream live on
   hipStream_t stream = nullptr;
   KernelClock clock;
 

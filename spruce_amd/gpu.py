@@ -64,6 +64,7 @@ def lib() -> ctypes.CDLL:
     L.spruce_gpu_rhs.argtypes = [P, DP, I64]
     L.spruce_gpu_dt_min.argtypes = [P, DP]
     L.spruce_gpu_put_grid.argtypes = [P, ctypes.c_char_p, DP, I64]
+    L.spruce_gpu_rhs_noalloc.argtypes = [P, DP, I64]
     L.spruce_gpu_ghost_fill.argtypes = [P]
     L.spruce_gpu_propagate.argtypes = [P, DP]
     L.spruce_gpu_num_evolved.argtypes = [P]
@@ -189,6 +190,13 @@ class GpuDomain:
         v = ctypes.c_double()
         _chk(lib().spruce_gpu_dt_min(self.h, ctypes.byref(v)))
         return v.value
+
+    def rhs_noalloc(self) -> np.ndarray:
+        """rhs() staged through the library's static device array (no device allocation)."""
+        ne = _chk(lib().spruce_gpu_num_evolved(self.h))
+        out = np.empty((ne, self.spec.xdim, self.spec.ydim))
+        _chk(lib().spruce_gpu_rhs_noalloc(self.h, _dp(out), out.size))
+        return out
 
     def put_grid(self, name: str, g: np.ndarray) -> None:
         """Overwrite one evolved grid of the live state without propagating."""
