@@ -96,6 +96,22 @@ int spruce_gpu_time(spruce_gpu_domain* d, double* t, int32_t* iter);
  * count >= that many doubles. */
 int spruce_gpu_module_report(spruce_gpu_domain* d, int32_t module, double* out, int32_t count, int32_t* count_out);
 
+/* The grids module `module` adds to every stored frame (Module::fileOutput, called by
+ * PlasmaDomain::storeGrids, fileio.cpp:186-194), as of the last step: thermal_conduction,
+ * radiative_losses, physical_viscosity, field_heating and anomalous_resistivity with
+ * output_to_file = true, artificial_viscosity with visc_output_*.  count_out (may be NULL)
+ * receives how many; grid k's name goes to name (name_len bytes, NUL-terminated; may be NULL) and
+ * its xdim*ydim values to host (count = xdim*ydim; may be NULL). */
+int spruce_gpu_module_output(spruce_gpu_domain* d, int32_t module, int32_t k, char* name, int32_t name_len,
+                             double* host, int64_t count, int32_t* count_out);
+
+/* Multispecies mode (set_option "multispecies_mode" = "true" before setup; PlasmaDomain,
+ * plasmadomain.cpp:55-58): the cumulative electron (which = 0), ion (1) and Joule (2) heating
+ * grids that storeGrids writes after the variables (fileio.cpp:164-184), and their reset to zero
+ * after each store (evolution.cpp:35-41). */
+int spruce_gpu_multispecies(spruce_gpu_domain* d, int32_t which, double* host, int64_t count);
+int spruce_gpu_multispecies_reset(spruce_gpu_domain* d);
+
 /* Unit entry points for parity tests.
  * rhs: EquationSet::computeTimeDerivatives on the live grids (equationset.cpp:204-210), written as
  *      nevolved consecutive xdim*ydim blocks in evolved-variable order.
@@ -143,6 +159,8 @@ int spruce_gpu_device_grid(spruce_gpu_domain* d, const char* name, double** dev_
  *                    least 4096 wavefronts, else one thread per cell) | "strip" (wavefront strips,
  *                    every face computed once, b_hat derivatives formed in the kernel) | "cell";
  *   "tc_rows"      = rows per conduction strip, 1..256 (default: chosen from the grid size);
+ *   "multispecies_mode" = "true" | "false" (the config setting of that name, fileio.cpp:322):
+ *                    accumulate the cumulative heating grids (before setup);
  *   "block"        = ideal_2F launch shape: "<columns>x<rows>" thread block of every kernel, stage
  *                    one thread per cell (default 64x4), or "strip<rows>" (rows a multiple of 4
  *                    up to 64): the stage kernel in wavefront strips, the other kernels in 64x4. */

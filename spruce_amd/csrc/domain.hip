@@ -28,6 +28,7 @@
 #include "sources.hpp"
 #include "adjoint.hpp"
 #include "adj_modules.hpp"
+#include "outputs.hpp"
 #include "moc.hpp"
 #include "crpow.hpp"
 #include "tf_host.hpp"
@@ -116,7 +117,16 @@ struct GpuModule {
   virtual void post(Domain&) {}
   // the values commandLineMessage prints that live on the device (empty for most modules)
   virtual std::vector<double> report(Domain&) { return {}; }
+  // Module::fileOutput (modulehandler.cpp:123-130): the grids this module appends to every stored
+  // frame, name and device address, in the reference's order
+  virtual void file_output(std::vector<std::pair<std::string, const double*>>&) {}
 };
+
+using OutList = std::vector<std::pair<std::string, const double*>>;
+static void alloc_zero(DBuf& b, size_t n, hipStream_t s) {
+  b.alloc(n);
+  launch_fill(b.p, 0.0, n, s);
+}
 
 static double opt_d(const std::vector<std::pair<std::string, std::string>>& o, const std::string& k, bool& found) {
   for (auto& kv : o) if (kv.first == k) { found = true; return std::stod(kv.second); }
@@ -172,6 +182,9 @@ class Domain : public DomainBase {
   int strip_rows = 0;     // rows per wavefront strip; 0 = chosen from the grid size
   int tc_kernel = -1;     // conduction update: 0 wavefront strips, 1 one thread per cell, -1 by size
   int tc_rows = 0;        // rows per conduction strip; 0 = chosen from the grid size
+  // multispecies mode (plasmadomain.hpp:133-137): cumulative electron, ion and Joule heating since
+  // the last store, which the modules add to
+  DBuf ms_grid[3];
 
   void set_option(const std::string& k, const std::string& v) override {
     if (k == "stage_kernel") {
@@ -184,6 +197,9 @@ class Domain : public DomainBase {
     } else if (k == "tc_kernel") {
       require(v == "strip" || v == "cell" || v == "auto", SPRUCE_E_ARG, "tc_kernel must be strip, cell or auto");
       tc_kernel = v == "cell" ? 1 : (v == "strip" ? 0 : -1);
+    } else if (k == "multispecies_mode") {
+      require(!ready, SPRUCE_E_STATE, "multispecies_mode must be set before setup");
+      multispecies_on = v == "true";
     } else if (k == "tc_rows") {
       const int r = std::stoi(v);
       require(r >= 1 && r <= 256, SPRUCE_E_ARG, "tc_rows must be in [1, 256]");
@@ -441,7 +457,45 @@ class Domain : public DomainBase {
       }
       if (need_dt_grid) dtgrid.alloc(N);
     }
+    if (multispecies_on)
+      for (auto& g : ms_grid) alloc_zero(g, N, stream);
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     ready = true;
+  }
+
+  size_t num_cells() const override { return N; }
+  // Module::fileOutput of module k: the grids it writes to every stored frame
+  int module_output(int k, int idx, std::string* name, double* host, int64_t count) override {
+    require(ready, SPRUCE_E_STATE, "setup must precede module_output");
+    require(k >= 0 && k < (int)mods.size(), SPRUCE_E_ARG, "module index out of range");
+    OutList out;
+    mods[k]->file_output(out);
+    if (idx < 0) return (int)out.size();
+    require(idx < (int)out.size(), SPRUCE_E_ARG, "module output index out of range");
+    if (name) *name = out[idx].first;
+    if (host) {
+      require(count == (int64_t)N, SPRUCE_E_ARG, "count must equal xdim*ydim");
+      hip_check(hipMemcpyAsync(host, out[idx].second, N * 8, hipMemcpyDeviceToHost, stream), "download output");
+      hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+    return (int)out.size();
+  }
+  // the multispecies accumulators (0 electron, 1 ion, 2 Joule heating); null unless multispecies_mode
+  double* ms(int w) { return multispecies_on ? ms_grid[w].p : nullptr; }
+  void multispecies(int which, double* host, int64_t count) override {
+    require(ready, SPRUCE_E_STATE, "setup must precede multispecies");
+    require(multispecies_on, SPRUCE_E_STATE, "multispecies_mode is off");
+    require(which >= 0 && which < 3, SPRUCE_E_ARG, "multispecies grid index must be 0, 1 or 2");
+    require(count == (int64_t)N, SPRUCE_E_ARG, "count must equal xdim*ydim");
+    hip_check(hipMemcpyAsync(host, ms_grid[which].p, N * 8, hipMemcpyDeviceToHost, stream), "download multispecies");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  }
+  // cumulative heating restarts after every store (evolution.cpp:35-41)
+  void multispecies_reset() override {
+    require(ready, SPRUCE_E_STATE, "setup must precede multispecies_reset");
+    require(multispecies_on, SPRUCE_E_STATE, "multispecies_mode is off");
+    for (auto& g : ms_grid) launch_fill(g.p, 0.0, N, stream);
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
   }
 
   // The live dt grid (recomputeDT after the latest propagate), for modules that read it.
@@ -816,6 +870,9 @@ struct TcModule : GpuModule {
   std::string integ = "euler";
   bool inactive = false;
   DBuf ta, tb, tc, k1, k2, k3, rx, ry, coef, dbxx, dbxy, dbyx, dbyy;
+  bool to_file = false;           // output_to_file: the average heating rate and the saturation factor
+  double msf = 1.0;               // ms_electron_heating_fraction (thermalconduction.hpp:34)
+  DBuf avg, sat;
   // while recording (the adjoint's replay of a step): te after each Euler subcycle
   bool recording = false;
   std::vector<DBuf> tape;
@@ -834,6 +891,12 @@ struct TcModule : GpuModule {
     integ = opt_s(o, "time_integrator", "euler");
     if (integ.empty()) integ = "euler";
     inactive = opt_s(o, "inactive_mode", "false") == "true";
+    to_file = opt_s(o, "output_to_file", "false") == "true";
+    bool f4;
+    const double m = opt_d(o, "ms_electron_heating_fraction", f4);
+    if (f4) msf = m;
+    require(msf >= 0.0 && msf <= 1.0, SPRUCE_E_ARG,
+            "Thermal Conduction MS electron heating fraction must be between 0 and 1");
     require(integ == "euler" || integ == "rk2" || integ == "rk4", SPRUCE_E_ARG,
             "Invalid time integrator given for Thermal Conduction module");
     p.k0 = wf * KAPPA_0;
@@ -847,6 +910,12 @@ struct TcModule : GpuModule {
     dbxx.alloc(d.N); dbxy.alloc(d.N); dbyx.alloc(d.N); dbyy.alloc(d.N);
     if (integ == "rk4") { k1.alloc(d.N); k2.alloc(d.N); k3.alloc(d.N); }
     if (p.saturation) { rx.alloc(d.N); ry.alloc(d.N); coef.alloc(d.N); }
+    if (to_file) { alloc_zero(avg, d.N, d.stream); alloc_zero(sat, d.N, d.stream); }
+  }
+  void file_output(OutList& o) override {  // thermalconduction.cpp:229-239
+    if (!to_file) return;
+    o.emplace_back("thermal_conduction", avg.p);
+    o.emplace_back("flux_saturation", sat.p);
   }
   void pre_launch(Domain& d, double* scal) override {
     // the strip form of the update forms the b_hat derivatives itself; the grids serve the cell
@@ -883,6 +952,11 @@ struct TcModule : GpuModule {
   void iterate(Domain& d, double dt) override {  // thermalconduction.cpp:91-156
     const double h = dt / (double)nsub;
     h_last = h;
+    const double* te0 = d.L.te.p;
+    if (to_file && p.saturation) {   // saturationTerms(temp) at the start of the step (:51-57)
+      launch_tc_sat(te0, d.L.n.p, d.L.rho.p, d.bhx.p, d.bhy.p, p, d.kc, d.mesh, rx.p, ry.p, coef.p, d.stream);
+      hip_check(hipMemcpyAsync(sat.p, coef.p, d.N * 8, hipMemcpyDeviceToDevice, d.stream), "copy saturation");
+    }
     const double* cur = d.L.te.p;
     double* bufs[3] = {ta.p, tb.p, tc.p};
     auto other = [&](const double* a, const double* b) {
@@ -917,6 +991,11 @@ struct TcModule : GpuModule {
         cur = i2;
       }
     }
+    if (to_file) launch_out_rate(cur, te0, dt, d.N, avg.p, d.stream);   // (:101-103)
+    if (d.multispecies_on) {                                            // (:105-108)
+      if (msf < 1.0) launch_ms_change(d.ms(1), cur, te0, 1.0 - msf, true, d.N, d.stream);
+      if (msf > 0.0) launch_ms_change(d.ms(0), cur, te0, msf, true, d.N, d.stream);
+    }
     if (inactive) return;
     // the last subcycle's buffer becomes the live thermal energy (no copy)
     for (DBuf* b : {&ta, &tb, &tc})
@@ -932,7 +1011,9 @@ struct RlModule : GpuModule {
   RlParams p{};
   std::string integ = "euler";
   bool inactive = false;
-  DBuf tout;
+  bool to_file = false;           // output_to_file: the average loss rate ("rad")
+  double msf = 1.0;               // ms_electron_heating_fraction (radiativelosses.hpp:31)
+  DBuf tout, avg;
   explicit RlModule(const std::vector<std::pair<std::string, std::string>>& o) {
     name = "radiative_losses";
     bool f1, f2, f3;
@@ -946,6 +1027,12 @@ struct RlModule : GpuModule {
     integ = opt_s(o, "time_integrator", "euler");
     if (integ.empty()) integ = "euler";
     inactive = opt_s(o, "inactive_mode", "false") == "true";
+    to_file = opt_s(o, "output_to_file", "false") == "true";
+    bool f4;
+    const double m = opt_d(o, "ms_electron_heating_fraction", f4);
+    if (f4) msf = m;
+    require(msf >= 0.0 && msf <= 1.0, SPRUCE_E_ARG,
+            "Radiative Losses MS electron heating fraction must be between 0 and 1");
     require(integ == "euler" || integ == "rk2" || integ == "rk4", SPRUCE_E_ARG,
             "Invalid time integrator given for radiative losses module");
   }
@@ -953,6 +1040,10 @@ struct RlModule : GpuModule {
   void setup(Domain& d) override {
     tout.alloc(d.N);
     p.pd_eps = d.kc.eps;
+    if (to_file) alloc_zero(avg, d.N, d.stream);
+  }
+  void file_output(OutList& o) override {  // radiativelosses.cpp:174-181
+    if (to_file) o.emplace_back("rad", avg.p);
   }
   void pre_launch(Domain& d, double* scal) override {
     if (p.prevent) d.refresh_dt_grid();
@@ -972,6 +1063,11 @@ struct RlModule : GpuModule {
     if (p.prevent) d.refresh_dt_grid();
     const int ig = integ == "euler" ? 0 : (integ == "rk2" ? 1 : 2);
     d.timed("rl_iterate", [&] { launch_rl_iterate(d.L.view(), d.dtgrid.p, p, d.kc, d.mesh, nsub, h, ig, tout.p, d.stream); });
+    if (to_file) launch_out_rate(tout.p, d.L.te.p, dt, d.N, avg.p, d.stream);   // (:93)
+    if (d.multispecies_on) {                                                   // (:94-97)
+      if (msf < 1.0) launch_ms_change(d.ms(1), tout.p, d.L.te.p, 1.0 - msf, true, d.N, d.stream);
+      if (msf > 0.0) launch_ms_change(d.ms(0), tout.p, d.L.te.p, msf, true, d.N, d.stream);
+    }
     if (inactive) return;
     d.L.te.swap(tout);   // the result buffer becomes the live thermal energy (no copy)
     d.propagate_live();
@@ -1007,8 +1103,10 @@ struct ViscModule : GpuModule {
     double scale_c = 1.0;
     std::vector<double> sgrid_host;
     DBuf sgrid, out;
+    DBuf o_dq, o_lap, o_str, o_dt;   // file output of the last evaluation (hyperviscous dqdt: last subcycle)
     bool hyper() const { return strength > 1.0; }
   };
+  bool out_visc = false, out_lap = false, out_str = false, out_dt = false;   // visc_output_*
   std::string in_opt, in_strength, in_diff, in_evol, in_length, in_species, shape, hv_integ;
   bool grad = false;
   double hv_eps = 1.0;  // viscosity.hpp:68
@@ -1031,7 +1129,11 @@ struct ViscModule : GpuModule {
       else if (k == "visc_vars_to_evol") in_evol = v;
       else if (k == "visc_length") in_length = v;
       else if (k == "visc_species") in_species = v;
-      // visc_output_* select file output only; other keys only warn in the reference
+      else if (k == "visc_output_visc") out_visc = v == "true";
+      else if (k == "visc_output_lap") out_lap = v == "true";
+      else if (k == "visc_output_strength") out_str = v == "true";
+      else if (k == "visc_output_timescale") out_dt = v == "true";
+      // other keys only warn in the reference
     }
   }
   bool needs_pre_scalars() const override { return false; }
@@ -1154,7 +1256,12 @@ struct ViscModule : GpuModule {
         hip_check(hipMemcpy(t.sgrid.p, t.sgrid_host.data(), d.N * 8, hipMemcpyHostToDevice), "upload strength");
       }
       if (t.hyper()) any_hyper = true;
-      else { any_rhs = true; t.out.alloc(d.N); ++n_rhs; }
+      else { any_rhs = true; alloc_zero(t.out, d.N, d.stream); ++n_rhs; }
+      // the output grids start at zero (:96-99); a term's RHS result is its masked dqdt (:117)
+      if (out_visc && t.hyper()) alloc_zero(t.o_dq, d.N, d.stream);
+      if (out_lap) alloc_zero(t.o_lap, d.N, d.stream);
+      if (out_str) alloc_zero(t.o_str, d.N, d.stream);
+      if (out_dt) alloc_zero(t.o_dt, d.N, d.stream);
     }
     require(n_rhs <= MAX_RHS_TERMS, SPRUCE_E_UNSUPPORTED, "at most 4 artificial_viscosity RHS terms");
     dtg.alloc(d.N);
@@ -1183,10 +1290,28 @@ struct ViscModule : GpuModule {
     a.dtmin = dtmin.p;
     a.var = t.var; a.scale = t.scale; a.scale_c = t.scale_c; a.grad = grad ? 1 : 0; a.masked = masked ? 1 : 0;
     a.out = out;
+    a.lap_out = t.o_lap.p; a.str_out = t.o_str.p; a.dt_out = t.o_dt.p;
     d.timed("visc_term", [&] { launch_visc_term(a, d.stream); });
   }
-  void before_integrator(Domain& d) override { if (any_rhs) refresh(d); }
+  void file_output(OutList& o) override {  // viscosity.cpp:351-376
+    if (out_visc) for (auto& t : terms) o.emplace_back(t.evol + "_dqdt", t.hyper() ? t.o_dq.p : t.out.p);
+    if (out_lap) for (auto& t : terms) o.emplace_back(t.evol + "_lap", t.o_lap.p);
+    if (out_str) for (auto& t : terms) o.emplace_back(t.evol + "_str", t.o_str.p);
+    if (out_dt) for (auto& t : terms) o.emplace_back(t.evol + "_dt", t.o_dt.p);
+  }
+  // The reference evaluates every term, hyperviscous ones included, in each right-hand side
+  // (constructViscosityGrids, viscosity.cpp:112-121) and adds only the others: a hyperviscous
+  // term's Laplacian, strength and timescale outputs come from that evaluation when the step's
+  // integrator runs after the module's subcycles.
+  bool side_out() const { return out_lap || out_str || out_dt; }
+  void hyper_side(Domain& d, SetBufs& S) {
+    if (!side_out()) return;
+    for (auto& t : terms)
+      if (t.hyper()) term(d, t, S, d1.p, true);   // the dqdt itself is discarded
+  }
+  void before_integrator(Domain& d) override { if (any_rhs || (any_hyper && side_out())) refresh(d); }
   void rhs_terms(Domain& d, SetBufs& S, StageArgs& a) override {
+    hyper_side(d, S);
     for (auto& t : terms) {
       if (t.hyper()) continue;
       term(d, t, S, t.out.p, true);
@@ -1196,6 +1321,7 @@ struct ViscModule : GpuModule {
     }
   }
   void add_rhs(Domain& d, SetBufs& S, double* out) override {
+    hyper_side(d, S);
     for (auto& t : terms) {
       if (t.hyper()) continue;
       term(d, t, S, t.out.p, true);
@@ -1244,6 +1370,11 @@ struct ViscModule : GpuModule {
           }
         }
       }
+      // the stored dqdt: the last subcycle's final evaluation (rk2) or combination (rk4); Euler
+      // subcycles leave it unchanged (:133-173)
+      if (out_visc && hv_integ != "euler")
+        hip_check(hipMemcpyAsync(t.o_dq.p, hv_integ == "rk2" ? d1.p : dq.p, d.N * 8, hipMemcpyDeviceToDevice, d.stream),
+                  "copy dqdt");
       d.propagate_live();
     }
   }
@@ -1254,6 +1385,7 @@ struct ViscModule : GpuModule {
 // ---------------------------------------------------------------------------------------------
 struct HeatModule : GpuModule {
   double rate = 0.0, exp_base = 0.0, exp_h = 0.0, split_h = 0.0, split_start = 0.0;
+  double msf = 0.5;   // ms_electron_heating_fraction (ambientheating.hpp:28)
   bool exp_mode = false, split_mode = false;
   DBuf heating;
   explicit HeatModule(const std::vector<std::pair<std::string, std::string>>& o) {  // :12-26
@@ -1268,6 +1400,7 @@ struct HeatModule : GpuModule {
       else if (k == "split_exp_mode") split_mode = v == "true";
       else if (k == "split_exp_scale_height") split_h = std::stod(v);
       else if (k == "split_exp_start_height") split_start = std::stod(v);
+      else if (k == "ms_electron_heating_fraction") msf = std::stod(v);
     }
   }
   bool needs_pre_scalars() const override { return false; }
@@ -1296,12 +1429,18 @@ struct HeatModule : GpuModule {
           h[c] = mask * rate;
         }
       }
+    require(msf >= 0.0 && msf <= 1.0, SPRUCE_E_ARG,
+            "Ambient Heating MS electron heating fraction must be between 0 and 1");
     heating.alloc(d.N);
     hip_check(hipMemcpy(heating.p, h.data(), d.N * 8, hipMemcpyHostToDevice), "upload heating");
   }
   void post(Domain& d) override {  // postIterateModule (:42-49)
     d.timed("heating", [&] { launch_add_step_times(d.L.te.p, heating.p, d.step.p, d.N, d.stream); });
     d.propagate_live();
+    if (d.multispecies_on) {
+      if (msf < 1.0) launch_ms_source(d.ms(1), heating.p, 0.0, d.step.p, 1.0 - msf, MS_SRC_RATE, d.mesh, d.stream);
+      if (msf > 0.0) launch_ms_source(d.ms(0), heating.p, 0.0, d.step.p, msf, MS_SRC_RATE, d.mesh, d.stream);
+    }
   }
 };
 
@@ -1310,9 +1449,11 @@ struct HeatModule : GpuModule {
 // ---------------------------------------------------------------------------------------------
 struct PvModule : GpuModule {
   double coeff = 0.0, ramp = 0.0, epsilon = 1.0;
-  bool heating_on = true, force_on = true, inactive = false, grad_corr = false;
+  double msf = 0.0;   // ms_electron_heating_fraction (physicalviscosity.hpp:33)
+  bool heating_on = true, force_on = true, inactive = false, grad_corr = false, to_file = false;
   std::string integ;
   DBuf coef, bhx, bhy, bhz, part, tsmin;
+  DBuf avg_heat, avg_f[3];   // output_to_file: subcycle averages of the heating and the force
   DBuf sa[8], sb[8], sh[8], dv[7];
   explicit PvModule(const std::vector<std::pair<std::string, std::string>>& o) {  // :17-34
     name = "physical_viscosity";
@@ -1328,9 +1469,13 @@ struct PvModule : GpuModule {
       else if (k == "inactive_mode") inactive = v == "true";
       else if (k == "gradient_correction") grad_corr = v == "true";
       else if (k == "time_integrator") integ = v;
+      else if (k == "output_to_file") to_file = v == "true";
+      else if (k == "ms_electron_heating_fraction") msf = std::stod(v);
     }
     if (integ.empty()) integ = "euler";
     require(integ == "euler" || integ == "rk2", SPRUCE_E_ARG, "Invalid time integrator given for Physical Viscosity module");
+    require(msf >= 0.0 && msf <= 1.0, SPRUCE_E_ARG,
+            "Physical Viscosity MS electron heating fraction must be between 0 and 1");
   }
   bool needs_pre_scalars() const override { return false; }
   void pre_launch(Domain&, double*) override {}
@@ -1368,6 +1513,19 @@ struct PvModule : GpuModule {
     for (auto& b : dv) b.alloc(d.N);
     part.alloc(pv_blocks(d.mesh));
     tsmin.alloc(1);
+    if (to_file) {
+      alloc_zero(avg_heat, d.N, d.stream);
+      for (auto& b : avg_f) alloc_zero(b, d.N, d.stream);
+    }
+  }
+  void file_output(OutList& o) override {  // :292-310
+    if (!to_file) return;
+    if (heating_on) o.emplace_back("viscous_heating", avg_heat.p);
+    if (force_on) {
+      o.emplace_back("viscous_force_x", avg_f[0].p);
+      o.emplace_back("viscous_force_y", avg_f[1].p);
+      o.emplace_back("viscous_force_z", avg_f[2].p);
+    }
   }
   void iterate(Domain& d, double dt) override {  // :148-250
     PvFrozen fz{bhx.p, bhy.p, bhz.p, coef.p, d.L.n.p, d.L.rho.p};
@@ -1388,6 +1546,13 @@ struct PvModule : GpuModule {
     a.d = dd; a.f = fz; a.k = d.kc; a.mesh = d.mesh; a.h = h;
     a.heating_on = heating_on ? 1 : 0; a.force_on = force_on ? 1 : 0; a.apply = inactive ? 0 : 1;
     a.coeff_zero = coeff == 0.0 ? 1 : 0; a.grad_corr = grad_corr ? 1 : 0;
+    a.nsd = (double)nsub;
+    if (to_file) {   // the averages restart every step (:151-152)
+      launch_fill(avg_heat.p, 0.0, d.N, d.stream);
+      for (int q = 0; q < 3; ++q) { launch_fill(avg_f[q].p, 0.0, d.N, d.stream); a.avg_f[q] = avg_f[q].p; }
+      a.avg_heat = avg_heat.p;
+    }
+    if (d.multispecies_on) { a.ms_ion = d.ms(1); a.ms_el = d.ms(0); a.msf = msf; }
     DBuf* cur = sa;
     DBuf* nxt = sb;
     for (int s = 0; s < nsub; ++s) {
@@ -1520,6 +1685,10 @@ struct LocalHeatModule : PostSourceModule {
     const double a = d.host_dt * rf;
     d.timed("source", [&] { launch_source(d.L.te.p, tmpl.p, nullptr, a, 0.0, nullptr, SRC_ADD, d.mesh, d.stream); });
     d.propagate_live();
+    if (d.multispecies_on) {   // (:63-65): the template without the ramp factor
+      if (msf < 1.0) launch_ms_source(d.ms(1), tmpl.p, d.host_dt, nullptr, 1.0 - msf, MS_SRC_TEMPLATE, d.mesh, d.stream);
+      if (msf > 0.0) launch_ms_source(d.ms(0), tmpl.p, d.host_dt, nullptr, msf, MS_SRC_TEMPLATE, d.mesh, d.stream);
+    }
   }
 };
 
@@ -1614,7 +1783,7 @@ struct MomInjModule : PostSourceModule {
 
 // AmbientHeatingSink (ambientheatingsink.cpp)
 struct HeatSinkModule : PostSourceModule {
-  double rate = 0.0, exp_base = 0.0, exp_h = 1.0, cx = 0.0, hw = 1.0, msf = 0.0;
+  double rate = 0.0, exp_base = 0.0, exp_h = 1.0, cx = 0.0, hw = 1.0, msf = 0.5;   // msf: ambientheatingsink.hpp:28
   bool exp_mode = false;
   DBuf red;
   explicit HeatSinkModule(const Opts& o) {  // :12-25
@@ -1655,21 +1824,25 @@ struct HeatSinkModule : PostSourceModule {
   void post(Domain& d) override {  // :35-37
     d.timed("source", [&] { launch_source(d.L.te.p, red.p, nullptr, 0.0, 0.0, d.step.p, SRC_SUB, d.mesh, d.stream); });
     d.propagate_live();
+    if (d.multispecies_on) {   // (:38-41)
+      if (msf < 1.0) launch_ms_source(d.ms(1), red.p, 0.0, d.step.p, 1.0 - msf, MS_SRC_SINK, d.mesh, d.stream);
+      if (msf > 0.0) launch_ms_source(d.ms(0), red.p, 0.0, d.step.p, msf, MS_SRC_SINK, d.mesh, d.stream);
+    }
   }
 };
 
 // FieldHeating (fieldheating.cpp)
 struct FieldHeatModule : GpuModule {
   FheatParams p{};
-  bool inactive = false;
-  DBuf heat;
+  bool inactive = false, to_file = false;
+  DBuf heat;   // after a step: mask * (dt * heating), the energy the step added
   explicit FieldHeatModule(const Opts& o) {  // :15-28
     name = "field_heating";
     p.cc = C_LIGHT / (4.0 * PI);
     for (auto& kv : o) {
       const std::string& k = kv.first;
       const std::string& v = kv.second;
-      if (k == "output_to_file") require(v != "true", SPRUCE_E_UNSUPPORTED, "field_heating file output is not available in this build");
+      if (k == "output_to_file") to_file = v == "true";
       else if (k == "coeff") p.coeff = std::stod(v);
       else if (k == "current_pow") p.cpow = std::stod(v);
       else if (k == "b_pow") p.bpow = std::stod(v);
@@ -1681,7 +1854,10 @@ struct FieldHeatModule : GpuModule {
   bool needs_pre_scalars() const override { return false; }
   bool needs_host_dt() const override { return false; }   // the step size is read on the device
   bool needs_bhat() const override { return p.coeff != 0.0 && p.rpow != 0.0; }
-  void setup(Domain& d) override { heat.alloc(d.N); }
+  void setup(Domain& d) override { alloc_zero(heat, d.N, d.stream); }   // :30-32
+  void file_output(OutList& o) override {                                  // :73-80
+    if (to_file) o.emplace_back("field_heating", heat.p);
+  }
   void pre_launch(Domain& d, double*) override {  // preIterateModule -> computeHeating (:34-54)
     d.timed("fheat", [&] { launch_fheat_pre(d.L.view(), d.stat(), d.bhx.p, d.bhy.p, p, d.mesh, heat.p, d.stream); });
   }
@@ -1803,8 +1979,9 @@ struct ResistModule : GpuModule {
   std::string integ = "euler", tmode = "flood_fill", model = "time_scale";
   double safety = 1.0, frob_coeff = 1.0e50, ff_threshold = 1.0, ff_ramp = 1.0e-5, ff_argmin_radius = 5.0e9;
   double sigma = 3.0, ts = -1.0, ff_max_radius = -1.0, ff_min_current = -1.0;
-  bool smoothing = true, grad_corr = false;
+  bool smoothing = true, grad_corr = false, to_file = false;
   std::vector<double> params;
+  DBuf avg;                                             // output_to_file: the Joule heating rate of the last step
   int null_i = 0, null_j = 0, kr = 0;
   int nx = 0, ny = 0;
   std::vector<double> px, py;                           // host positions (the argmin box)
@@ -1818,7 +1995,7 @@ struct ResistModule : GpuModule {
       const std::string& a = kv.first;
       const std::string& v = kv.second;
       if (a == "time_scale") ts = std::stod(v);
-      else if (a == "output_to_file") require(v != "true", SPRUCE_E_UNSUPPORTED, "anomalous_resistivity file output is not available in this build");
+      else if (a == "output_to_file") to_file = v == "true";
       else if (a == "frobenius_metric_coeff") frob_coeff = std::stod(v);
       else if (a == "smoothing_sigma") sigma = std::stod(v);
       else if (a == "safety_factor") safety = std::stod(v);
@@ -1939,7 +2116,9 @@ struct ResistModule : GpuModule {
     py = d.host_grids.at("pos_y");
     upload_vec(dpx, px);
     upload_vec(dpy, py);
-    for (DBuf* b : {&J, &bx, &by, &bm, &ratio, &tmpl, &tmp, &diff, &coeff, &td}) b->alloc(d.N);
+    for (DBuf* b : {&J, &bx, &by, &bm, &ratio, &tmpl, &tmp, &coeff, &td}) b->alloc(d.N);
+    alloc_zero(diff, d.N, d.stream);                    // diffusivity is zero until the first step (:20)
+    if (to_file) alloc_zero(avg, d.N, d.stream);
     for (int k = 0; k < 3; ++k) { lap[k].alloc(d.N); mid[k].alloc(d.N); }
     for (int k = 0; k < 4; ++k) {
       work[k].alloc(d.N); d1[k].alloc(d.N);
@@ -1974,7 +2153,16 @@ struct ResistModule : GpuModule {
     }
     launch_ar_current(d.L.bx.p, d.L.by.p, d.L.bz.p, CC, d.mesh, J.p, d.stream);
     compute_template(d, d.L.bx.p, d.L.by.p, J.p);
+    launch_ar_coeff(tmpl.p, diff.p, d.mesh, coeff.p, td.p, d.stream);   // td = template * diffusivity
     hip_check(hipStreamSynchronize(d.stream), "hipStreamSynchronize");
+  }
+  // fileOutput (:320-329): template * diffusivity and the template of the last derivative
+  // evaluation, and the Joule heating rate of the last step
+  void file_output(OutList& o) override {
+    if (!to_file) return;
+    o.emplace_back("anomalous_diffusivity", td.p);
+    o.emplace_back("anomalous_template", tmpl.p);
+    o.emplace_back("joule_heating", avg.p);
   }
   void iterate(Domain& d, double dt) override {  // :104-122, 170-220
     const size_t bytes = d.N * 8;
@@ -2041,6 +2229,8 @@ struct ResistModule : GpuModule {
         }
       }
     });
+    if (to_file) launch_out_rate(work[3].p, d.L.te.p, dt, d.N, avg.p, d.stream);          // (:167)
+    if (d.multispecies_on) launch_ms_change(d.ms(2), work[3].p, d.L.te.p, 1.0, false, d.N, d.stream);  // (:168-170)
     for (int k = 0; k < 4; ++k)
       hip_check(hipMemcpyAsync(L4[k], work[k].p, bytes, hipMemcpyDeviceToDevice, d.stream), "copy field");
     d.propagate_live();
@@ -2847,6 +3037,37 @@ int spruce_gpu_module_report(spruce_gpu_domain* h, int32_t module, double* out, 
     require(count >= (int32_t)v.size(), SPRUCE_E_ARG, "out holds fewer values than the module reports");
     std::memcpy(out, v.data(), v.size() * sizeof(double));
   }
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_module_output(spruce_gpu_domain* h, int32_t module, int32_t k, char* name, int32_t name_len,
+                             double* host, int64_t count, int32_t* count_out) {
+  API_BEGIN
+  require(count_out || name || host, SPRUCE_E_ARG, "null argument");
+  DomainBase& d = dom(h);
+  const int n = d.module_output(module, -1, nullptr, nullptr, 0);
+  if (count_out) *count_out = n;
+  if (name || host) {
+    std::string nm;
+    d.module_output(module, k, &nm, host, count);
+    if (name) {
+      require(name_len > (int32_t)nm.size(), SPRUCE_E_ARG, "name buffer too small for " + nm);
+      std::memcpy(name, nm.c_str(), nm.size() + 1);
+    }
+  }
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_multispecies(spruce_gpu_domain* h, int32_t which, double* host, int64_t count) {
+  API_BEGIN
+  require(host != nullptr, SPRUCE_E_ARG, "null argument");
+  dom(h).multispecies(which, host, count);
+  API_END(SPRUCE_OK)
+}
+
+int spruce_gpu_multispecies_reset(spruce_gpu_domain* h) {
+  API_BEGIN
+  dom(h).multispecies_reset();
   API_END(SPRUCE_OK)
 }
 

@@ -113,8 +113,10 @@ void E2Domain::set_grid(const std::string& name, const double* p, int64_t count)
   host_grids[name].assign(p, p + N);
 }
 
-void E2Domain::set_option(const std::string& k, const std::string&) {
-  require(k == "stage_kernel" || k == "strip_rows" || k == "block", SPRUCE_E_ARG, "unknown option " + k);
+void E2Domain::set_option(const std::string& k, const std::string& v) {
+  require(k == "stage_kernel" || k == "strip_rows" || k == "block" || k == "multispecies_mode", SPRUCE_E_ARG,
+          "unknown option " + k);
+  if (k == "multispecies_mode") multispecies_on = v == "true";
 }
 
 E2Set E2Domain::Set::view() { return E2Set{rho.p, n.p, mx.p, my.p, tei.p, tee.p, bx.p, by.p, rho_pre.p}; }
@@ -331,8 +333,10 @@ void ConsDomain::set_grid(const std::string& name, const double* p, int64_t coun
   host_grids[name].assign(p, p + N);
 }
 
-void ConsDomain::set_option(const std::string& k, const std::string&) {
-  require(k == "stage_kernel" || k == "strip_rows" || k == "block", SPRUCE_E_ARG, "unknown option " + k);
+void ConsDomain::set_option(const std::string& k, const std::string& v) {
+  require(k == "stage_kernel" || k == "strip_rows" || k == "block" || k == "multispecies_mode", SPRUCE_E_ARG,
+          "unknown option " + k);
+  if (k == "multispecies_mode") multispecies_on = v == "true";
 }
 
 void ConsDomain::setup() {

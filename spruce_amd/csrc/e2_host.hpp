@@ -24,6 +24,7 @@ class E2Domain : public DomainBase {
   double dt_min() override;
   double* device_grid(const std::string& name) override;
   void set_option(const std::string& k, const std::string& v) override;
+  size_t num_cells() const override { return N; }
   int num_evolved() const override { return 7; }
 
  private:
@@ -74,6 +75,7 @@ class ConsDomain : public DomainBase {
   double dt_min() override { return step_h; }
   double* device_grid(const std::string& name) override;
   void set_option(const std::string& k, const std::string& v) override;
+  size_t num_cells() const override { return N; }
   int num_evolved() const override { return 6; }
 
  private:

@@ -184,6 +184,26 @@ class DomainBase {
   virtual int64_t adjoint_bytes() const { return 0; }
   // the device values module k's command-line message reports (empty when it reports none)
   virtual std::vector<double> module_report(int k) { (void)k; return {}; }
+  // xdim * ydim (0 before the grids are known)
+  virtual size_t num_cells() const = 0;
+  // Module::fileOutput of module k (fileio.cpp:186-194): the number of grids it writes to every
+  // stored frame; with idx >= 0 also that grid's name and, when host is not null, its values
+  virtual int module_output(int k, int idx, std::string* name, double* host, int64_t count) {
+    (void)k; (void)name; (void)host; (void)count;
+    require(idx < 0, SPRUCE_E_ARG, "module output index out of range");
+    return 0;
+  }
+  // multispecies mode (plasmadomain.cpp:55-58, evolution.cpp:35-41): the cumulative electron (0),
+  // ion (1) and Joule (2) heating since the last store.  Only the ideal_mhd modules add to them;
+  // the other equation sets keep them at zero.
+  bool multispecies_on = false;
+  virtual void multispecies(int which, double* host, int64_t count) {
+    require(multispecies_on, SPRUCE_E_STATE, "multispecies_mode is off");
+    require(which >= 0 && which < 3, SPRUCE_E_ARG, "multispecies grid index must be 0, 1 or 2");
+    require(count == (int64_t)num_cells() && count > 0, SPRUCE_E_ARG, "count must equal xdim*ydim");
+    for (int64_t c = 0; c < count; ++c) host[c] = 0.0;
+  }
+  virtual void multispecies_reset() { require(multispecies_on, SPRUCE_E_STATE, "multispecies_mode is off"); }
 
   double time = 0.0;
   int iter = 0;

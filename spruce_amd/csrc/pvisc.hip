@@ -200,6 +200,17 @@ __global__ void __launch_bounds__(256) k_pv_apply(PvArgs a) {
       fr[jj] = r * (a.grad_corr ? -3.0 * mask : -3.0 * coef * mask);
     }
   }
+  if (a.mode == PV_FULL) {
+    if (a.avg_heat && a.heating_on) a.avg_heat[c] = a.avg_heat[c] + heat / a.nsd;
+    if (a.avg_f[0] && a.force_on) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) a.avg_f[q][c] = a.avg_f[q][c] + fr[q] / a.nsd;
+    }
+    if (a.ms_ion && a.heating_on && a.apply) {
+      if (a.msf < 1.0) a.ms_ion[c] = a.ms_ion[c] + ((1.0 - a.msf) * heat) * a.h;
+      if (a.msf > 0.0) a.ms_el[c] = a.ms_el[c] + (a.msf * heat) * a.h;
+    }
+  }
   // subcycle update (euler :163-188; rk2 half step :194-210 and full step :220-245)
   double te = a.base.te[c], T = a.base.T[c];
   double m[3] = {a.base.mx[c], a.base.my[c], a.base.mz[c]};

@@ -31,6 +31,14 @@ struct PvArgs {
   double h;                    // subcycle step
   int mode;                    // PV_FULL: out = base + h*rate; PV_HALF: out = base + 0.5*h*rate
   int heating_on, force_on, apply, coeff_zero, grad_corr;
+  // file output and multispecies accumulators, added to in PV_FULL evaluations only; null when off
+  // (physicalviscosity.cpp:164-178, :218-230)
+  double* avg_heat;            // += heating / nsd
+  double* avg_f[3];            // += force / nsd
+  double nsd;                  // the subcycle count
+  double* ms_ion;              // += ((1 - msf) * heating) * h when heating is applied, msf < 1
+  double* ms_el;               // += (msf * heating) * h when heating is applied, msf > 0
+  double msf;
 };
 
 // b_hat, velocities, temperature and the copies of te and momenta that start the iteration

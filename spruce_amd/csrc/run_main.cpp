@@ -11,8 +11,7 @@
 //
 // Differences, all on paths where the reference asserts or hangs: invalid input is reported on stderr
 // with exit status 1 instead of an abort; an empty or comment line inside an equation-set block is
-// skipped (the reference loops forever there, equationset.cpp:72-73); module file output
-// (output_to_file and the like) and multispecies mode are refused.  When the run reaches its time
+// skipped (the reference loops forever there, equationset.cpp:72-73).  When the run reaches its time
 // or iteration limit the reference ends on a failed assert "to end recursive scripts"
 // (evolution.cpp:54-56); this program prints the same message and exits with status 134, the status
 // a shell reports for that abort.
@@ -152,9 +151,6 @@ const std::vector<std::string> MODULE_NAMES = {
     "eic_thermalization", "artificial_viscosity", "global_temperature", "mass_injection", "ambient_heating_sink",
     "physical_viscosity", "div_cleaning", "boundary_outflow",
     "parametric_heating"};  // this build's parametrised heating profile (no reference module)
-// module options that ask for file output (fileOutput hooks)
-const std::vector<std::string> OUTPUT_KEYS = {"output_to_file", "visc_output_visc", "visc_output_lap",
-                                              "visc_output_strength", "visc_output_timescale"};
 
 struct Module {
   std::string name;
@@ -362,8 +358,7 @@ class Runner {
     std::cout << "Reading state file...\n";
     st_ = read_state(state_path, continue_mode);
     std::cout << "Validating input data...\n";
-    check(!(cfg_.set.count("multispecies_mode") && cfg_.set.at("multispecies_mode") == "true"),
-          "multispecies_mode output is not available in this build");
+    multispecies_ = cfg_.set.count("multispecies_mode") && cfg_.set.at("multispecies_mode") == "true";
     max_iterations_ = (int)num(cfg_, "max_iterations", 100);
     iter_out_ = (int)num(cfg_, "iter_output_interval", 1);
     time_out_ = num(cfg_, "time_output_interval", -1.0);
@@ -375,9 +370,6 @@ class Runner {
     for (auto& f : cfg_.output_flags)
       check(std::find(names_.vars.begin(), names_.vars.end(), f) != names_.vars.end(),
             "output flag " + f + " is not a variable of " + cfg_.eqset);
-    for (auto& m : cfg_.modules)
-      for (auto& k : OUTPUT_KEYS)
-        check(m.opt(k) != "true", "module file output (" + m.name + ": " + k + ") is not available in this build");
 
     spruce_domain_desc d{};
     d.xdim = st_.nx;
@@ -409,6 +401,7 @@ class Runner {
       gpu(spruce_gpu_add_module(dom_, m.name.c_str(), (int32_t)mk.size(), mk.data(), mv.data()),
           "module " + m.name);
     }
+    if (multispecies_) gpu(spruce_gpu_set_option(dom_, "multispecies_mode", "true"), "multispecies_mode");
     for (auto& g : st_.grids) {
       gpu(spruce_gpu_set_grid(dom_, g.first.c_str(), g.second.data(), (int64_t)g.second.size()), "grid " + g.first);
       pd_[g.first] = g.second;
@@ -454,7 +447,11 @@ class Runner {
       const int new_t = (int)(time_ / time_out_);
       const bool c1 = iter_out_ > 0 && iter_ % iter_out_ == 0;
       const bool c2 = time_out_ > 0.0 && new_t > old_t;
-      if (c1 || c2) store();
+      if (c1 || c2) {
+        store();
+        // cumulative multispecies heating restarts after every store (evolution.cpp:35-41)
+        if (multispecies_) gpu(spruce_gpu_multispecies_reset(dom_), "multispecies reset");
+      }
       if (write_interval_ > 0 && stores_ > 0 && stores_ % write_interval_ == 0) {
         flush_out();
         write_state("end");
@@ -494,13 +491,35 @@ class Runner {
       f << fmt(n, 4);
     }
   }
-  // PlasmaDomain::storeGrids (fileio.cpp:142-200), output-flagged variables in index order
+  // PlasmaDomain::storeGrids (fileio.cpp:142-200): output-flagged variables in index order, the
+  // multispecies heating grids, then every module's fileOutput grids in module order
   void store() {
     lines_.push_back("t=" + num2str(time_) + '\n');
     for (auto& v : names_.vars) {
       if (std::find(cfg_.output_flags.begin(), cfg_.output_flags.end(), v) == cfg_.output_flags.end()) continue;
       lines_.push_back(v + '\n');
       lines_.push_back(fmt(v, write_precision_));
+    }
+    const size_t n = (size_t)st_.nx * st_.ny;
+    std::vector<double> g(n);
+    if (multispecies_) {
+      const char* ms[3] = {"cumulative_electron_heating", "cumulative_ion_heating", "cumulative_joule_heating"};
+      for (int w = 0; w < 3; ++w) {
+        gpu(spruce_gpu_multispecies(dom_, w, g.data(), (int64_t)n), ms[w]);
+        lines_.push_back(std::string(ms[w]) + '\n');
+        lines_.push_back(format_grid(g, st_.nx, st_.ny, write_precision_));
+      }
+    }
+    for (int m = 0; m < (int)cfg_.modules.size(); ++m) {
+      int32_t count = 0;
+      gpu(spruce_gpu_module_output(dom_, m, 0, nullptr, 0, nullptr, 0, &count), "module output");
+      for (int k = 0; k < count; ++k) {
+        char name[128] = {0};
+        gpu(spruce_gpu_module_output(dom_, m, k, name, (int32_t)sizeof name, g.data(), (int64_t)n, nullptr),
+            "module output");
+        lines_.push_back(std::string(name) + '\n');
+        lines_.push_back(format_grid(g, st_.nx, st_.ny, write_precision_));
+      }
     }
     ++stores_;
   }
@@ -607,6 +626,7 @@ class Runner {
   std::map<std::string, std::vector<double>> pd_;
   std::vector<std::string> lines_;
   int stores_ = 0, iter_ = 0;
+  bool multispecies_ = false;
   double time_ = 0.0, max_time_ = -1.0, duration_ = -1.0, time_out_ = -1.0;
   int max_iterations_ = 100, iter_out_ = 1, std_out_ = 1, write_precision_ = 4, write_interval_ = 1;
 };

@@ -90,6 +90,8 @@ void TfDomain::set_option(const std::string& k, const std::string& v) {
     if (self) self->rebuild(shape);
   } else if (k == "stage_kernel" || k == "strip_rows") {
     // IdealMHD options; the two-fluid path has a single stage kernel
+  } else if (k == "multispecies_mode") {
+    multispecies_on = v == "true";
   } else {
     throw Error(SPRUCE_E_ARG, "unknown option " + k);
   }

@@ -34,6 +34,7 @@ class TfDomain : public DomainBase {
   double dt_min() override;
   double* device_grid(const std::string& name) override;
   void set_option(const std::string& k, const std::string& v) override;
+  size_t num_cells() const override { return N; }
   int num_evolved() const override { return TF_NE; }
 
   // state shared with batches

@@ -113,6 +113,9 @@ __global__ void __launch_bounds__(256) k_visc_term(ViscTermArgs a) {
   double r = nu * lap * sc;
   if (a.grad) r = r + fx * gx + fy * gy;
   a.out[c] = a.masked ? r * (in ? 1.0 : 0.0) : r;
+  if (a.lap_out) a.lap_out[c] = lap;   // zero outside the interior, as laplacian() leaves it
+  if (a.str_out) a.str_out[c] = a.strength ? a.strength[c] : a.strength_c;
+  if (a.dt_out) a.dt_out[c] = a.dtgrid ? a.dtgrid[c] : *a.dtmin;
 }
 
 __global__ void __launch_bounds__(256) k_visc_update(double* target, const double* init, const double* d, double h,

@@ -27,6 +27,11 @@ struct ViscTermArgs {
   int grad;                 // gradient_correction
   int masked;               // multiply by the ghost-zone mask
   double* out;
+  // file output (viscosity.cpp:351-376), null when off: the Laplacian, strength and timescale the
+  // evaluation used, on every cell
+  double* lap_out;
+  double* str_out;
+  double* dt_out;
 };
 
 // dqdt of one term on every cell (constructSingleViscosityGrid, viscosity.cpp:185-267).

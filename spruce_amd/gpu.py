@@ -87,6 +87,10 @@ def lib() -> ctypes.CDLL:
     L.spruce_gpu_adjoint_state.argtypes = [P, DP, DP]
     L.spruce_gpu_adjoint_params.argtypes = [P, DP, I32, ctypes.POINTER(I32)]
     L.spruce_gpu_module_report.argtypes = [P, I32, DP, I32, ctypes.POINTER(I32)]
+    L.spruce_gpu_module_output.argtypes = [P, I32, I32, ctypes.c_char_p, I32, DP, ctypes.c_int64,
+                                           ctypes.POINTER(I32)]
+    L.spruce_gpu_multispecies.argtypes = [P, I32, DP, ctypes.c_int64]
+    L.spruce_gpu_multispecies_reset.argtypes = [P]
     L.spruce_gpu_adjoint_device_bytes.argtypes = [P, ctypes.POINTER(ctypes.c_int64)]
     L.spruce_gpu_batch_set_profiling.argtypes = [P, I32]
     L.spruce_gpu_batch_kernel_times.argtypes = [P, ctypes.c_char_p, I32, DP, ctypes.POINTER(I32), I32]
@@ -260,6 +264,32 @@ class GpuDomain:
         if n.value:
             _chk(lib().spruce_gpu_module_report(self.h, k, _dp(out), n.value, None))
         return out
+
+    def module_output(self, k: int) -> dict:
+        """The grids module k writes to every stored frame (its fileOutput), by name, as of the
+        latest step."""
+        n = ctypes.c_int32(0)
+        _chk(lib().spruce_gpu_module_output(self.h, k, 0, None, 0, None, 0, ctypes.byref(n)))
+        out = {}
+        for i in range(n.value):
+            name = ctypes.create_string_buffer(128)
+            g = np.empty((self.spec.xdim, self.spec.ydim))
+            _chk(lib().spruce_gpu_module_output(self.h, k, i, name, 128, _dp(g), g.size, None))
+            out[name.value.decode()] = g
+        return out
+
+    def multispecies(self) -> dict:
+        """The cumulative electron, ion and Joule heating grids of multispecies mode."""
+        out = {}
+        for w, name in enumerate(("cumulative_electron_heating", "cumulative_ion_heating",
+                                  "cumulative_joule_heating")):
+            g = np.empty((self.spec.xdim, self.spec.ydim))
+            _chk(lib().spruce_gpu_multispecies(self.h, w, _dp(g), g.size))
+            out[name] = g
+        return out
+
+    def multispecies_reset(self) -> None:
+        _chk(lib().spruce_gpu_multispecies_reset(self.h))
 
     def adjoint_device_bytes(self) -> int:
         """Device memory the last adjoint call allocated beyond the domain's own grids."""

@@ -64,8 +64,22 @@ def _same_files(o):
     assert _files(da) == _files(db)
     for f in _files(da):
         a, b = open(os.path.join(da, f), "rb").read(), open(os.path.join(db, f), "rb").read()
-        assert a == b, "%s differs (%d vs %d bytes)" % (f, len(a), len(b))
+        if a != b:
+            pytest.fail("%s differs (%d vs %d bytes): %s" % (f, len(a), len(b), _first_difference(a, b)))
     return _files(da)
+
+
+def _first_difference(a, b):
+    """Where two output files first differ: the line, the grid name above it and both texts."""
+    k = next((i for i in range(min(len(a), len(b))) if a[i] != b[i]), min(len(a), len(b)))
+    la, lb = a[:k].split(b"\n"), b[:k].split(b"\n")
+    at = next((i for i in range(len(la) - 1, -1, -1)
+               if la[i] and b"," not in la[i] and not la[i][:1].isdigit() and la[i][:1] != b"-"), -1)
+    name = la[at] if at >= 0 else b"?"
+    end_a, end_b = a.find(b"\n", k), b.find(b"\n", k)
+    return "line %d (grid %s, %d lines below its name): %r vs %r" % (
+        len(la), name.decode(errors="replace"), len(la) - 1 - at,
+        (la[-1] + a[k:end_a if end_a >= 0 else None])[:120], (lb[-1] + b[k:end_b if end_b >= 0 else None])[:120])
 
 
 @pytest.mark.parametrize("name", EXACT)

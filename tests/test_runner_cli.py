@@ -47,10 +47,3 @@ def test_unknown_config_name(tmp_path):
     sp, cp = _inputs(str(tmp_path), extra_lines=["not_a_setting = 3"])
     rc, _, err = _run(["-m", "input", "-o", os.path.join(str(tmp_path), "o"), "-s", sp, "-c", cp], str(tmp_path))
     assert rc == 1 and "not_a_setting is not a valid config name." in err
-
-
-def test_module_file_output_is_refused(tmp_path):
-    sp, cp = _inputs(str(tmp_path), module_opts=[("flux_saturation", "false"), ("epsilon", "0.5"),
-                                                 ("dt_subcycle_min", "1e-6"), ("output_to_file", "true")])
-    rc, _, err = _run(["-m", "input", "-o", os.path.join(str(tmp_path), "o"), "-s", sp, "-c", cp], str(tmp_path))
-    assert rc == 1 and "module file output" in err
