@@ -54,10 +54,10 @@ def measured_traffic(workload: str, kernel: str, rows: str, summary: str = "hbm_
 
 def resolved_stage_kernel(option: str, nx: int, ny: int) -> str:
     """The stage kernel the library runs for --stage-kernel auto: strips when 4-row strips of 60
-    output columns give at least 1024 wavefronts, else one thread per cell."""
+    output columns give at least 1024 wavefronts, else 8x8 tiles."""
     if option != "auto":
         return option
-    return "strip" if ((ny + 59) // 60) * ((nx + 3) // 4) >= 1024 else "cell"
+    return "strip" if ((ny + 59) // 60) * ((nx + 3) // 4) >= 1024 else "tile"
 
 
 def stage_bytes_per_cell(integrator: str) -> float:
@@ -466,7 +466,7 @@ def main() -> None:
     ap.add_argument("--workload", default="arcade2048")
     ap.add_argument("--cpu-steps", type=int, default=0, help="reference steps for the CPU baseline (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--stage-kernel", default="auto", choices=["auto", "strip", "cell"])
+    ap.add_argument("--stage-kernel", default="auto", choices=["auto", "strip", "cell", "tile"])
     ap.add_argument("--strip-rows", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0, help="edge of the square arcade of the adjoint_arcade workload")
     ap.add_argument("--tc-kernel", default="auto", choices=["auto", "strip", "cell"])

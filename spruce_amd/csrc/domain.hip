@@ -178,7 +178,10 @@ class Domain : public DomainBase {
   int kinds_all[4] = {0, 0, 0, 0}, kinds_ucnp[4] = {0, 0, 0, 0}, kinds_live[4] = {0, 0, 0, 0};
   double s1[4] = {1, 1, 1, 1}, s2[4] = {1, 1, 1, 1};
   bool any_open = false;
-  int stage_kernel = -1;  // 0: wavefront strips (mhd2.hip), 1: one thread per cell (mhd.hip), -1: by size
+  int stage_kernel = -1;  // 0: wavefront strips (mhd2.hip), 1: one thread per cell, 2: tiles (mhd.hip), -1: by size
+  // the final stage of a step also forms the next step size's per-block minima (tile form, no ghost
+  // fill, limiter or module after the integrator)
+  bool fuse_dt = false;
   int strip_rows = 0;     // rows per wavefront strip; 0 = chosen from the grid size
   int tc_kernel = -1;     // conduction update: 0 wavefront strips, 1 one thread per cell, -1 by size
   int tc_rows = 0;        // rows per conduction strip; 0 = chosen from the grid size
@@ -188,8 +191,9 @@ class Domain : public DomainBase {
 
   void set_option(const std::string& k, const std::string& v) override {
     if (k == "stage_kernel") {
-      require(v == "strip" || v == "cell" || v == "auto", SPRUCE_E_ARG, "stage_kernel must be strip, cell or auto");
-      stage_kernel = v == "cell" ? 1 : (v == "strip" ? 0 : -1);
+      require(v == "strip" || v == "cell" || v == "tile" || v == "auto", SPRUCE_E_ARG,
+              "stage_kernel must be strip, cell, tile or auto");
+      stage_kernel = v == "cell" ? 1 : (v == "strip" ? 0 : (v == "tile" ? 2 : -1));
     } else if (k == "strip_rows") {
       const int r = std::stoi(v);
       require(r >= 4 && r <= 256 && r % 4 == 0, SPRUCE_E_ARG, "strip_rows must be a multiple of 4 in [4, 256]");
@@ -426,8 +430,10 @@ class Domain : public DomainBase {
     if (tc_rows == 0) tc_rows = tc_strip_rows_default(nx, ny);
     if (tc_kernel < 0) tc_kernel = tc_kernel_default(nx, ny);
     if (stage_kernel < 0) stage_kernel = mhd_stage_kernel_default(nx, ny);
+    fuse_dt = stage_kernel == 2 && !(kinds_all[0] || kinds_all[1] || kinds_all[2] || kinds_all[3]) && !any_moc &&
+              mods.empty();
     dt_blocks = mhd_dt_blocks(mesh);
-    partial.alloc(dt_blocks);
+    partial.alloc(std::max(dt_blocks, mhd_tile_blocks(mesh)));
     step.alloc(1);
     history.alloc(1);
     hip_check(hipMalloc(&err_flag, sizeof(int)), "hipMalloc");
@@ -555,8 +561,10 @@ class Domain : public DomainBase {
 
   MhdStatic stat() { return MhdStatic{bex.p, bey.p, bez.p, gx.p, gy.p}; }
 
-  void stage(SetBufs& S, SetBufs& B, SetBufs& T, double mult, int mode, int kslot, const double* step_ptr = nullptr) {
+  void stage(SetBufs& S, SetBufs& B, SetBufs& T, double mult, int mode, int kslot, const double* step_ptr = nullptr,
+             double* dt_partial = nullptr) {
     StageArgs a = stage_args(S.view(), B.view(), T.view(), mult, mode, kslot, step_ptr, nullptr);
+    a.dt_partial = dt_partial;
     for (auto& m : mods) m->rhs_terms(*this, S, a);
     launch_stage(a);
   }
@@ -581,6 +589,7 @@ class Domain : public DomainBase {
   }
   void launch_stage(const StageArgs& a) {
     if (stage_kernel == 1) timed("mhd_stage", [&] { launch_mhd_stage(a, stream); });
+    else if (stage_kernel == 2) timed("mhd_stage", [&] { launch_mhd_stage_tile(a, stream); });
     else timed("mhd_stage", [&] { launch_mhd_stage_strip(a, strip_rows, stream); });
   }
 
@@ -591,10 +600,11 @@ class Domain : public DomainBase {
 
   // End of a propagate on the live grids: per-cell dt, then the step size of the next step
   // (evolution.cpp:62) written on the device.
-  void finish_step(int idx) {
-    timed("mhd_dt", [&] { launch_mhd_dt(L.view(), stat(), kc, mesh, nullptr, partial.p, stream); });
+  void finish_step(int idx, bool fused = false) {
+    if (!fused) timed("mhd_dt", [&] { launch_mhd_dt(L.view(), stat(), kc, mesh, nullptr, partial.p, stream); });
+    const int nb = fused ? mhd_tile_blocks(mesh) : dt_blocks;
     timed("reduce_step", [&] {
-      launch_reduce_step(partial.p, dt_blocks, kc.eps, step.p, idx >= 0 ? history.p : nullptr, idx, err_flag, stream);
+      launch_reduce_step(partial.p, nb, kc.eps, step.p, idx >= 0 ? history.p : nullptr, idx, err_flag, stream);
     });
   }
 
@@ -611,9 +621,11 @@ class Domain : public DomainBase {
   void advance(int s) {
     if (!mods.empty()) run_modules(s);
     const int integ = desc.time_integrator;
+    const bool fused = fuse_dt && !rec;
+    double* dtp = fused ? partial.p : nullptr;
     if (integ == SPRUCE_EULER) {                       // evolution.cpp:84-88
       if (rec) launch_stage(stage_args(L.view(), L.view(), W.view(), 1.0, STAGE_STORE_K, 0, nullptr, rec->k1));
-      else stage(L, L, W, 1.0, STAGE_PLAIN, 0);
+      else stage(L, L, W, 1.0, STAGE_PLAIN, 0, nullptr, dtp);
       L.swap(W);
     } else if (integ == SPRUCE_RK2) {                  // evolution.cpp:90-101
       if (rec) {                                       // the same stages, keeping k1, k2 and the midpoint
@@ -626,7 +638,7 @@ class Domain : public DomainBase {
         // zeroes momenta inside an open_moc side's ghost cells, which the final stage reads as its base
         if (any_moc) ghost(L, kinds_live, GHOST_LIVE_RAW);
         moc_limit(W.view());
-        stage(W, L, L, 1.0, STAGE_PLAIN, 0);
+        stage(W, L, L, 1.0, STAGE_PLAIN, 0, nullptr, dtp);
       }
     } else {                                           // evolution.cpp:103-124
       stage(L, L, W, 0.5, STAGE_STORE_K, 0);
@@ -641,7 +653,7 @@ class Domain : public DomainBase {
       ghost(W, kinds_ucnp, GHOST_NORMAL);
       ghost(L, kinds_live, GHOST_LIVE_RAW);
       moc_limit(W.view());
-      stage(W, L, L, 1.0, STAGE_RK4_FINAL, 0);
+      stage(W, L, L, 1.0, STAGE_RK4_FINAL, 0, nullptr, dtp);
     }
     ghost(L, kinds_all, GHOST_NORMAL);
     moc_limit(L.view());
@@ -652,7 +664,7 @@ class Domain : public DomainBase {
       if (rec && k < rec_post.size() && rec_post[k]) copy_set(L.view(), rec->after_post[np++], N, stream);
     }
     if (have_host_dt) tstep += host_dt;               // evolution.cpp:80, after the post-iterate hooks
-    finish_step(s + 1);
+    finish_step(s + 1, fused);
   }
 
   void run(int nsteps, double* dt_out, int32_t* sub_out) override {

@@ -10,30 +10,32 @@
 #include "arith.hpp"
 #include "mhd.hpp"
 #include "mhd_cell.hpp"
+#include "strip.hpp"
 
 namespace spruce {
 
 namespace {
 
-// Stage: T = propagate-minimums-derived(B + step * k(S)) (equationset.cpp:222-230, 212-220;
-// idealmhd.cpp:234-263).  Ghost cells have k = +0.
-__global__ void __launch_bounds__(256) k_mhd_stage(StageArgs a) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y * blockDim.y + threadIdx.y;
-  const Mesh& M = a.mesh;
-  if (i >= M.nx || j >= M.ny) return;
-  const size_t N = (size_t)M.nx * M.ny;
-  const size_t c = (size_t)i * M.ny + j;
-  double k[8];
-  if (interior(M, i, j)) {
-    mhd_rhs_cell(a.S, a.st, a.k, M, i, j, k);
-  } else if (a.moc) {  // open_moc ghost cells: masked RHS (0) + characteristic terms (idealmhd.cpp:99-103)
+// The right-hand side of a ghost cell: the masked derivative (0) plus, with open_moc sides, the
+// characteristic terms (idealmhd.cpp:99-103).
+__device__ __forceinline__ void ghost_rhs(const StageArgs& a, size_t N, size_t c, double k[8]) {
+  if (a.moc) {
 #pragma unroll
     for (int v = 0; v < 8; ++v) k[v] = 0.0 + a.moc[v * N + c];
   } else {
 #pragma unroll
     for (int v = 0; v < 8; ++v) k[v] = 0.0;
   }
+}
+
+// The values a stage writes that the step size reads.
+struct StageOut {
+  double rho, mx, my, te, bx, by, bz;
+};
+
+// The rest of a stage at cell c once the equation set's k is known: module terms, the RK4
+// combination, the update, enforceMinimums and the derived density.
+__device__ __forceinline__ StageOut stage_finish(const StageArgs& a, size_t N, size_t c, double k[8]) {
   for (int e = 0; e < a.n_extra; ++e) {
 #pragma unroll
     for (int v = 0; v < 8; ++v)
@@ -65,16 +67,36 @@ __global__ void __launch_bounds__(256) k_mhd_stage(StageArgs a) {
   const double te1 = smax(te_new, K.temin);
   // recomputeDerivedVarsFromEvolvedVars (idealmhd.cpp:246-252)
   const double n = smax(qdiv(rho1, K.m, K.rm), K.nmin);
-  a.T.rho[c] = n * K.m;
+  const StageOut o{n * K.m, mx, my, smax(te1, K.temin), bx, by, bz};
+  a.T.rho[c] = o.rho;
   if (a.write_n) a.T.n[c] = n;
   if (a.write_pre) a.T.rho_pre[c] = rho1;
   a.T.mx[c] = mx;
   a.T.my[c] = my;
   a.T.mz[c] = mz;
-  a.T.te[c] = smax(te1, K.temin);
+  a.T.te[c] = o.te;
   a.T.bx[c] = bx;
   a.T.by[c] = by;
   a.T.bz[c] = bz;
+  return o;
+}
+
+// Stage: T = propagate-minimums-derived(B + step * k(S)) (equationset.cpp:222-230, 212-220;
+// idealmhd.cpp:234-263).  Ghost cells have k = +0.
+__global__ void __launch_bounds__(256) k_mhd_stage(StageArgs a) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  const Mesh& M = a.mesh;
+  if (i >= M.nx || j >= M.ny) return;
+  const size_t N = (size_t)M.nx * M.ny;
+  const size_t c = (size_t)i * M.ny + j;
+  double k[8];
+  if (interior(M, i, j)) {
+    mhd_rhs_cell(a.S, a.st, a.k, M, i, j, k);
+  } else {
+    ghost_rhs(a, N, c, k);
+  }
+  (void)stage_finish(a, N, c, k);
 }
 
 // populateVariablesFromState up to the ghost fill (equationset.cpp:96-104; idealmhd.cpp:226-252).
@@ -192,14 +214,12 @@ __global__ void __launch_bounds__(256) k_mhd_ghost(GhostArgs a, int xsides, int 
 }
 
 // IdealMHD::recomputeDT at one cell (idealmhd.cpp:279-304).
-__device__ __forceinline__ double mhd_dt_cell(const MhdSet& L, const MhdStatic& st, const MhdConst& K,
-                                              const Mesh& M, int i, int j) {
-  const size_t c = (size_t)i * M.ny + j;
-  const double rho = L.rho[c];
-  const double press = L.te[c] * K.gm1;
-  const double bx = st.bex[c] + L.bx[c], by = st.bey[c] + L.by[c], bz = st.bez[c] + L.bz[c];
+// recomputeDT at one cell from its density, thermal energy, momenta and total field
+__device__ __forceinline__ double mhd_dt_vals(double rho, double te, double mxc, double myc, double bx, double by,
+                                              double bz, const MhdConst& K, const Mesh& M, int i, int j) {
+  const double press = te * K.gm1;
   const double bmag = sqrt(bx * bx + by * by + bz * bz);
-  const double vx = L.mx[c] / rho, vy = L.my[c] / rho;
+  const double vx = mxc / rho, vy = myc / rho;
   const double cs = sqrt(press * K.gamma / rho);
   const double cs2 = cs * cs;
   const double va = bmag / sqrt(rho * (4.0 * PI));
@@ -211,6 +231,13 @@ __device__ __forceinline__ double mhd_dt_cell(const MhdSet& L, const MhdStatic& 
   const double vmx = sqrt(vx * vx), vmy = sqrt(vy * vy);
   const double cmax = smax(smax(smax(cs, va), vfast), vslow);
   return 1.0 / (qdiv(vmx + cmax, M.x.d[i], M.x.rd[i]) + qdiv(vmy + cmax, M.y.d[j], M.y.rd[j]));
+}
+
+__device__ __forceinline__ double mhd_dt_cell(const MhdSet& L, const MhdStatic& st, const MhdConst& K,
+                                              const Mesh& M, int i, int j) {
+  const size_t c = (size_t)i * M.ny + j;
+  return mhd_dt_vals(L.rho[c], L.te[c], L.mx[c], L.my[c], st.bex[c] + L.bx[c], st.bey[c] + L.by[c],
+                     st.bez[c] + L.bz[c], K, M, i, j);
 }
 
 __device__ __forceinline__ double block_min(double v) {
@@ -245,6 +272,74 @@ __global__ void __launch_bounds__(256) k_mhd_dt(MhdSet L, MhdStatic st, MhdConst
   }
   v = block_min(v);
   if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = v;
+}
+
+// Tile form of the stage.  A block owns cells i0..i0+TX-1 x j0..j0+TY-1 (tiles row-major, handed
+// to the XCDs in contiguous bands).  Face f of the x
+// list lies between rows i0+f/TY-1 and i0+f/TY (column j0+f%TY); face g of the y list between
+// columns j0+g%(TY+1)-1 and j0+g%(TY+1) (row i0+g/(TY+1)).  A face is computed when a cell on
+// either side of it is interior, exactly as the one-thread-per-cell form computes it for that
+// cell, so every value is bit-identical to that form; the faces shared by two cells are computed
+// once instead of twice.
+constexpr int TILE_X = 8, TILE_Y = 8, TILE_THREADS = 192;
+
+__global__ void __launch_bounds__(TILE_THREADS) k_mhd_stage_tile(StageArgs a) {
+  constexpr int NFX = (TILE_X + 1) * TILE_Y, NFY = TILE_X * (TILE_Y + 1);
+  static_assert(NFX + NFY <= 2 * TILE_THREADS && TILE_X * TILE_Y == 64, "tile shape");
+  __shared__ Face fx[NFX];
+  __shared__ Face fy[NFY];
+  const Mesh& M = a.mesh;
+  const int ntj = (M.ny + TILE_Y - 1) / TILE_Y, nti = (M.nx + TILE_X - 1) / TILE_X;
+  const int tile = strip::xcd_tile(blockIdx.x, ntj * nti);
+  const int i0 = (tile / ntj) * TILE_X, j0 = (tile % ntj) * TILE_Y;
+  const int t = threadIdx.x;
+  for (int f = t; f < NFX + NFY; f += TILE_THREADS) {
+    if (f < NFX) {
+      const int i = i0 + f / TILE_Y, j = j0 + f % TILE_Y;
+      if (j < M.ny && i <= M.nx) {
+        const bool lower_of = i < M.nx && interior(M, i, j);   // the lower x-face of cell i
+        const bool upper_of = i >= 1 && interior(M, i - 1, j);  // the upper x-face of cell i-1
+        if (lower_of || upper_of) compute_face<0>(a.S, a.st, a.k, M, wrapi(i, M.x.n, M.x.per), j, fx[f]);
+      }
+    } else {
+      const int g = f - NFX;
+      const int i = i0 + g / (TILE_Y + 1), j = j0 + g % (TILE_Y + 1);
+      if (i < M.nx && j <= M.ny) {
+        const bool lower_of = j < M.ny && interior(M, i, j);
+        const bool upper_of = j >= 1 && interior(M, i, j - 1);
+        if (lower_of || upper_of) compute_face<1>(a.S, a.st, a.k, M, i, wrapi(j, M.y.n, M.y.per), fy[g]);
+      }
+    }
+  }
+  __syncthreads();
+  if (t >= TILE_X * TILE_Y) return;   // one wavefront combines
+  const int ti = t / TILE_Y, tj = t % TILE_Y;
+  const int i = i0 + ti, j = j0 + tj;
+  double v = DBL_MAX;
+  if (i < M.nx && j < M.ny) {
+    const size_t N = (size_t)M.nx * M.ny;
+    const size_t c = (size_t)i * M.ny + j;
+    double k[8];
+    if (interior(M, i, j)) {
+      mhd_rhs_faces(a.S, a.st, a.k, M, i, j, fx[ti * TILE_Y + tj], fx[(ti + 1) * TILE_Y + tj],
+                    fy[ti * (TILE_Y + 1) + tj], fy[ti * (TILE_Y + 1) + tj + 1], k);
+    } else {
+      ghost_rhs(a, N, c, k);
+    }
+    const StageOut o = stage_finish(a, N, c, k);
+    if (a.dt_partial && i >= M.x.dlo && i <= M.x.dhi && j >= M.y.dlo && j <= M.y.dhi) {
+      const double dt = mhd_dt_vals(o.rho, o.te, o.mx, o.my, a.st.bex[c] + o.bx, a.st.bey[c] + o.by,
+                                    a.st.bez[c] + o.bz, a.k, M, i, j);
+      v = (dt < v) ? dt : v;
+    }
+  }
+  if (a.dt_partial) {
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(v, off, 64);
+      v = (o < v) ? o : v;
+    }
+    if (t == 0) a.dt_partial[blockIdx.x] = v;
+  }
 }
 
 __global__ void __launch_bounds__(256) k_moc_visc_part(MhdSet S, MhdStatic st, MhdConst K, Mesh M,
@@ -356,6 +451,14 @@ __global__ void k_gather_scalars(const double* scal, int n, const double* step, 
 void launch_mhd_stage(const StageArgs& a, hipStream_t s) {
   dim3 b(64, 4);
   hipLaunchKernelGGL(k_mhd_stage, grid2d(a.mesh.nx, a.mesh.ny, b), b, 0, s, a);
+}
+
+void launch_mhd_stage_tile(const StageArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_mhd_stage_tile, dim3(mhd_tile_blocks(a.mesh)), dim3(TILE_THREADS), 0, s, a);
+}
+
+int mhd_tile_blocks(const Mesh& m) {
+  return ((m.ny + TILE_Y - 1) / TILE_Y) * ((m.nx + TILE_X - 1) / TILE_X);
 }
 
 void launch_mhd_populate(MhdSet L, const double* temp, MhdConst k, int nx, int ny, hipStream_t s) {

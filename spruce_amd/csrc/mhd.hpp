@@ -81,6 +81,9 @@ struct StageArgs {
   // open_moc characteristic terms (8 * nx*ny, evolved order), added to the ghost cells' RHS; null
   // without open_moc sides
   const double* moc;
+  // tile form only, null otherwise: one minimum per block of the target's per-cell dt over the dt
+  // bounds (the final stage of a step that no ghost fill, limiter or module follows)
+  double* dt_partial;
 };
 constexpr int MAX_RHS_TERMS = 4;
 
@@ -88,8 +91,13 @@ void launch_mhd_stage(const StageArgs& a, hipStream_t s);
 // Wavefront-strip form of the same stage (mhd2.hip); R rows per wavefront, a multiple of 4.
 void launch_mhd_stage_strip(const StageArgs& a, int R, hipStream_t s);
 int mhd_strip_rows_default(int nx, int ny);
-// 0 (strips) when the grid gives the strip kernel enough wavefronts, else 1 (one thread per cell)
+// 0 (strips) when the grid gives the strip kernel enough wavefronts, else 2 (tiles)
 int mhd_stage_kernel_default(int nx, int ny);
+// Tile form (mhd.hip): a block of 192 threads owns an 8x8 tile; each face of the tile is computed
+// once, by one thread, into LDS, then one thread per cell combines its four faces.  For grids too
+// small to fill the GPU with strips.  mhd_tile_blocks() blocks (and dt_partial entries).
+void launch_mhd_stage_tile(const StageArgs& a, hipStream_t s);
+int mhd_tile_blocks(const Mesh& m);
 void launch_mhd_populate(MhdSet L, const double* temp, MhdConst k, int nx, int ny, hipStream_t s);
 // Ghost fill of the sides whose kinds[] entry is non-zero; s1/s2 are the open-boundary decay
 // factors of each side (evolution.cpp:166-167), computed on the host with the C library's pow.

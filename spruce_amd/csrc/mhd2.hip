@@ -288,9 +288,10 @@ int mhd_strip_rows_default(int nx, int ny) {
 
 int mhd_stage_kernel_default(int nx, int ny) {
   // with 4-row strips a 256x256 grid gives 320 wavefronts, too few to fill 256 CUs (strips 49 us,
-  // one thread per cell 32 us); from about four wavefronts per CU on, the strips are faster
+  // one thread per cell 30 us, 8x8 tiles 21 us per stage); from about four wavefronts per CU on
+  // (512x512: strips 127 us per step, tiles 139 us), the strips are faster
   const long waves = (long)((ny + OUT_COLS - 1) / OUT_COLS) * ((nx + 3) / 4);
-  return waves >= 1024 ? 0 : 1;
+  return waves >= 1024 ? 0 : 2;
 }
 
 void launch_mhd_stage_strip(const StageArgs& a, int R, hipStream_t s) {

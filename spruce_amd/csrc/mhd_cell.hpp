@@ -78,14 +78,11 @@ __host__ __device__ __forceinline__ void compute_face(const MhdSet& S, const Mhd
   }
 }
 
-// IdealMHD::computeTimeDerivativesDerived at one interior cell (idealmhd.cpp:42-105).
-__host__ __device__ __forceinline__ void mhd_rhs_cell(const MhdSet& S, const MhdStatic& st, const MhdConst& K,
-                                             const Mesh& M, int i, int j, double out[8]) {
-  Face xl, xu, yl, yu;
-  compute_face<0>(S, st, K, M, i, j, xl);
-  compute_face<0>(S, st, K, M, wrapi(i + 1, M.x.n, M.x.per), j, xu);
-  compute_face<1>(S, st, K, M, i, j, yl);
-  compute_face<1>(S, st, K, M, i, wrapi(j + 1, M.y.n, M.y.per), yu);
+// IdealMHD::computeTimeDerivativesDerived at one interior cell (idealmhd.cpp:42-105) from its four
+// faces: xl/yl are the faces computed at (i, j), xu/yu those at (i+1, j) and (i, j+1), wrapped.
+__host__ __device__ __forceinline__ void mhd_rhs_faces(const MhdSet& S, const MhdStatic& st, const MhdConst& K,
+                                              const Mesh& M, int i, int j, const Face& xl, const Face& xu,
+                                              const Face& yl, const Face& yu, double out[8]) {
   const size_t c = (size_t)i * M.ny + j;
   const double dx = M.x.d[i], rdx = M.x.rd[i], dy = M.y.d[j], rdy = M.y.rd[j];
 
@@ -142,6 +139,16 @@ __host__ __device__ __forceinline__ void mhd_rhs_cell(const MhdSet& S, const Mhd
   out[5] = k[5] + 0.0;
   out[6] = k[6] + 0.0;
   out[7] = k[7] + 0.0;
+}
+
+__host__ __device__ __forceinline__ void mhd_rhs_cell(const MhdSet& S, const MhdStatic& st, const MhdConst& K,
+                                             const Mesh& M, int i, int j, double out[8]) {
+  Face xl, xu, yl, yu;
+  compute_face<0>(S, st, K, M, i, j, xl);
+  compute_face<0>(S, st, K, M, wrapi(i + 1, M.x.n, M.x.per), j, xu);
+  compute_face<1>(S, st, K, M, i, j, yl);
+  compute_face<1>(S, st, K, M, i, wrapi(j + 1, M.y.n, M.y.per), yu);
+  mhd_rhs_faces(S, st, K, M, i, j, xl, xu, yl, yu, out);
 }
 
 __host__ __device__ __forceinline__ bool interior(const Mesh& M, int i, int j) {

@@ -41,7 +41,7 @@ def rel_linf(a, b):
     return float(np.max(np.abs(a - b)) / scale) if scale > 0 else float(np.max(np.abs(a - b)))
 
 
-@pytest.mark.parametrize("kernel", ["strip", "cell"])
+@pytest.mark.parametrize("kernel", ["strip", "cell", "tile"])
 @pytest.mark.parametrize("case", CASES)
 def test_moc_matches_reference_golden_bitwise(case, kernel):
     g = load_golden(case)

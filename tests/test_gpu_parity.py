@@ -85,13 +85,16 @@ BIG = [
     ("reflect_rk2", ("reflect",) * 4, "rk2", dict(stretch=True)),
     ("ucnp_rk4", ("open_ucnp",) * 4, "rk4", {}),
     ("mixed_rk2", ("periodic", "periodic", "fixed", "open"), "rk2", dict(open_boundary_strength=0.3)),
+    # all sides periodic: the tile form also forms the next step size in its final stage
+    ("periodic_euler", ("periodic",) * 4, "euler", dict(stretch=True)),
+    ("periodic_rk4", ("periodic",) * 4, "rk4", {}),
 ]
 
 
-KERNELS = [("strip", None), ("strip", 4), ("strip", 12), ("cell", None)]
+KERNELS = [("strip", None), ("strip", 4), ("strip", 12), ("cell", None), ("tile", None)]
 
 
-@pytest.mark.parametrize("kernel,rows", KERNELS, ids=["strip", "strip4", "strip12", "cell"])
+@pytest.mark.parametrize("kernel,rows", KERNELS, ids=["strip", "strip4", "strip12", "cell", "tile"])
 @pytest.mark.parametrize("name,bc,integ,kw", BIG, ids=[b[0] for b in BIG])
 def test_gpu_matches_oracle_multi_block(name, bc, integ, kw, kernel, rows):
     spec = _random_case(133, 197, 7, bc, integ, **dict(kw))
@@ -112,11 +115,12 @@ def test_gpu_matches_oracle_multi_block(name, bc, integ, kw, kernel, rows):
         ora.close()
 
 
-def test_periodic_workload_full_size_matches_oracle():
+@pytest.mark.parametrize("kernel", ["auto", "cell", "tile"])
+def test_periodic_workload_full_size_matches_oracle(kernel):
     st = W.periodic_wave_state(256, 256, seed=0)
     spec = spec_from_files(W.periodic_wave_config(steps=1000), st)
     ora = OracleDomain(spec)
-    gpu = GpuDomain(spec)
+    gpu = GpuDomain(spec, options={"stage_kernel": kernel})
     try:
         dg, do = gpu.step(5), ora.step(5)
         assert same_bits(dg, do)

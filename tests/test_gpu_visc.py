@@ -19,7 +19,7 @@ VISC = [c for c in golden_names() if c.startswith(("visc_", "heat_")) and "_tc_"
 # cases through a libm power in the reference: conduction, physical viscosity (T^2.5)
 VISC_TC = [c for c in golden_names() if (c.startswith(("visc_", "heat_")) and "_tc_" in c) or c.startswith("pv_")]
 VARS = ["rho", "temp", "mom_x", "mom_y", "mom_z", "bi_x", "bi_y", "bi_z", "n", "thermal_energy", "press", "dt"]
-KERNELS = ["cell", "strip"]
+KERNELS = ["cell", "strip", "tile"]
 
 
 def same_bits(a, b):
