@@ -40,13 +40,14 @@ struct AdjGhostArgs {
   Mesh mesh;
   int kind[4];
   double s1[4], s2[4];
+  int raw;   // GHOST_LIVE_RAW: open sides read rho (not rho_pre) and write without floors
 };
 
 // Open boundary ghost values of one boundary cell row (mesh.hip ghost_side, kind 1, GHOST_NORMAL)
 // from x = (rho_pre, te, mom_x, mom_y) of the first interior cell: out = rho, te, mom_x, mom_y of
 // the two ghost cells (order c1, c2).
 __device__ void open_ghost_dual(const MhdConst& K, const Axis& ax, bool xside, bool upper, double s1, double s2,
-                                const double x[4], Dn<4> out[8]) {
+                                const double x[4], bool raw, Dn<4> out[8]) {
   const int n = ax.n;
   const int e2 = upper ? n - 2 : 1, e3 = upper ? n - 3 : 2;
   const Dn<4> rho3 = dn_var<4>(x[0], 0), te3 = dn_var<4>(x[1], 1), mx3 = dn_var<4>(x[2], 2), my3 = dn_var<4>(x[3], 3);
@@ -61,11 +62,18 @@ __device__ void open_ghost_dual(const MhdConst& K, const Axis& ax, bool xside, b
   const Dn<4> vn = xside ? vx : vy;
   const Dn<4> bv = upper ? dn_max(dn_const<4>(0.0), vn + boost) : dn_min(dn_const<4>(0.0), vn + boost);
   const Dn<4> gv = (dist * bv - (0.5 * ax.d[e2]) * vn) / (0.5 * ax.d[e3]);
-  const Dn<4> n1 = dn_max(rg1 / K.m, K.nmin), n2 = dn_max(rg2 / K.m, K.nmin);
-  out[0] = n1 * K.m;
-  out[1] = n2 * K.m;
-  out[2] = dn_max(tg1, K.temin);
-  out[3] = dn_max(tg2, K.temin);
+  if (raw) {
+    out[0] = rg1;
+    out[1] = rg2;
+    out[2] = tg1;
+    out[3] = tg2;
+  } else {
+    const Dn<4> n1 = dn_max(rg1 / K.m, K.nmin), n2 = dn_max(rg2 / K.m, K.nmin);
+    out[0] = n1 * K.m;
+    out[1] = n2 * K.m;
+    out[2] = dn_max(tg1, K.temin);
+    out[3] = dn_max(tg2, K.temin);
+  }
   out[4] = rg1 * (xside ? gv : vx);
   out[5] = rg2 * (xside ? gv : vx);
   out[6] = rg1 * (xside ? vy : gv);
@@ -104,16 +112,17 @@ __device__ void adj_ghost_side(const AdjGhostArgs& a, int side, int t, double* l
     lmy[c1] = 0.0; lmy[c2] = 0.0; lmy[c3] = 0.0;
     lmz[c1] = 0.0; lmz[c2] = 0.0; lmz[c3] = 0.0;
   } else {                         // open
-    const double x[4] = {a.P.rho_pre[c3], a.P.te[c3], a.P.mx[c3], a.P.my[c3]};
+    const double x[4] = {a.raw ? a.P.rho[c3] : a.P.rho_pre[c3], a.P.te[c3], a.P.mx[c3], a.P.my[c3]};
     Dn<4> o[8];
-    open_ghost_dual(a.k, ax, xside, upper, a.s1[side], a.s2[side], x, o);
+    open_ghost_dual(a.k, ax, xside, upper, a.s1[side], a.s2[side], x, a.raw != 0, o);
     const double lo[8] = {lr[c1], lr[c2], lte[c1], lte[c2], lmx[c1], lmx[c2], lmy[c1], lmy[c2]};
     double g[4] = {0.0, 0.0, 0.0, 0.0};
     for (int q = 0; q < 8; ++q)
       for (int k = 0; k < 4; ++k) g[k] += lo[q] * o[q].d[k];
     lr[c1] = 0.0; lr[c2] = 0.0; lte[c1] = 0.0; lte[c2] = 0.0;
     lmx[c1] = 0.0; lmx[c2] = 0.0; lmy[c1] = 0.0; lmy[c2] = 0.0;
-    // rho_pre of c3 and its rho come from the same pre-update density with the same floors
+    // normal fill: rho_pre of c3 and its rho come from the same pre-update density with the same
+    // floors; raw fill: rho of c3 itself
     lr[c3] += g[0];
     lte[c3] += g[1];
     lmx[c3] += g[2];
@@ -368,9 +377,9 @@ int adj_mod_blocks(const Mesh& m) {
 }
 
 void launch_adj_ghost_axis(MhdSet P, MhdConst k, Mesh mesh, const int kinds[4], const double s1[4],
-                           const double s2[4], int axis, double* lam, hipStream_t s) {
+                           const double s2[4], int axis, double* lam, hipStream_t s, int raw) {
   AdjGhostArgs a;
-  a.P = P; a.k = k; a.mesh = mesh;
+  a.P = P; a.k = k; a.mesh = mesh; a.raw = raw;
   for (int q = 0; q < 4; ++q) { a.kind[q] = kinds[q]; a.s1[q] = s1[q]; a.s2[q] = s2[q]; }
   const int s0 = axis == 0 ? 0 : 2;
   if (!(a.kind[s0] || a.kind[s0 + 1])) return;

@@ -12,11 +12,12 @@
 namespace spruce {
 
 int adj_mod_blocks(const Mesh& m);
-// Reverse of the ghost fill (evolution.cpp:126-334, GHOST_NORMAL) on the two sides of one axis
-// (0: x, 1: y), given the filled state P.  The ghost values a side writes lose their adjoint to the
-// interior cells they were computed from.  Sides with an open boundary need the other axis periodic.
+// Reverse of the ghost fill (evolution.cpp:126-334, GHOST_NORMAL, or GHOST_LIVE_RAW with raw = 1)
+// on the two sides of one axis (0: x, 1: y), given the filled state P (raw: its interior).  The
+// ghost values a side writes lose their adjoint to the interior cells they were computed from.
+// Sides with an open boundary need the other axis periodic.
 void launch_adj_ghost_axis(MhdSet P, MhdConst k, Mesh mesh, const int kinds[4], const double s1[4],
-                           const double s2[4], int axis, double* lam, hipStream_t s);
+                           const double s2[4], int axis, double* lam, hipStream_t s, int raw = 0);
 // Reverse of enforceMinimums + recomputeDerivedVarsFromEvolvedVars (idealmhd.cpp:234-252) applied
 // to rho_in and te_in = te_base + (*step) * add (add may be null: te_in = te_base): the adjoint of
 // rho and thermal_energy is kept where no floor is active.  With add given, partial[block] receives

@@ -76,6 +76,56 @@ __global__ void __launch_bounds__(256) k_adj_stage_cells(MhdSet B, const double*
   if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
 }
 
+// Reverse of the pointwise part of one stage, T = minimums/derived(B + mult*h*kv), alone:
+// lam_B (+)= mu = dT/dpre^T lam_T, lamk = mult*h*mu (written), partial: block sums of mult*mu.kv.
+__global__ void __launch_bounds__(256) k_adj_update(MhdSet B, const double* kv, const double* step, double mult,
+                                                    MhdConst K, Mesh M, const double* lam_T, double* lam_B,
+                                                    int assign_B, double* lamk, double* partial) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  double dot = 0.0;
+  if (i < M.nx && j < M.ny) {
+    const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+    const double h = mult * (*step);
+    const double rho_new = B.rho[c] + kv[c] * h;
+    const double x = rho_new / K.m;
+    const double rho1 = smax(x, K.nmin) * K.m;
+    const double drho = (x < K.nmin || rho1 / K.m < K.nmin) ? 0.0 : 1.0;
+    const double te_new = B.te[c] + kv[4 * N + c] * h;
+    const double dte = te_new < K.temin ? 0.0 : 1.0;
+    for (int v = 0; v < 8; ++v) {
+      const double d = v == 0 ? drho : (v == 4 ? dte : 1.0);
+      const double mu = lam_T[v * N + c] * d;
+      lam_B[v * N + c] = assign_B ? mu : lam_B[v * N + c] + mu;
+      lamk[v * N + c] = h * mu;
+      dot += mu * kv[v * N + c];
+    }
+    dot *= mult;
+  }
+  dot = block_sum(dot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
+}
+
+// The per-cell part of the transposed time derivative at S applied to lamk = sa*ka + kb (kb may be
+// null): its own part to lam_S, its face coefficients to cx, cy (then k_adj_faces).
+__global__ void __launch_bounds__(256) k_adj_rhs_cells(MhdSet S, MhdStatic st, MhdConst K, Mesh M, const double* ka,
+                                                       double sa, const double* kb, double* lam_S, int assign_S,
+                                                       double* cx, double* cy) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= M.nx || j >= M.ny) return;
+  const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+  double fx[ADJ_NCOEF], fy[ADJ_NCOEF];
+  if (interior_cell(M, i, j)) {
+    double lamk[8], own[8];
+    for (int v = 0; v < 8; ++v) lamk[v] = kb ? sa * ka[v * N + c] + kb[v * N + c] : sa * ka[v * N + c];
+    adj_cell_coeffs(S, st, K, M, i, j, lamk, own, fx, fy);
+    for (int v = 0; v < 8; ++v) lam_S[v * N + c] = assign_S ? own[v] : lam_S[v * N + c] + own[v];
+  } else {
+    for (int q = 0; q < ADJ_NCOEF; ++q) { fx[q] = 0.0; fy[q] = 0.0; }
+    if (assign_S) for (int v = 0; v < 8; ++v) lam_S[v * N + c] = 0.0;
+  }
+  for (int q = 0; q < ADJ_NCOEF; ++q) { cx[q * N + c] = fx[q]; cy[q * N + c] = fy[q]; }
+}
+
 // One thread per face along AX; the face at cell (i, j) lies between (i, j) and its lower neighbour.
 template <int AX>
 __device__ __forceinline__ void adj_faces_axis(const MhdSet& S, const MhdStatic& st, const MhdConst& K, const Mesh& M,
@@ -230,6 +280,20 @@ void launch_adj_populate(const double* rho_s, const double* temp_s, const double
 
 void launch_fill(double* p, double v, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
+}
+
+
+void launch_adj_update(MhdSet B, const double* kv, const double* step, double mult, MhdConst k, Mesh mesh,
+                       const double* lam_T, double* lam_B, int assign_B, double* lamk, double* partial,
+                       hipStream_t s) {
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(k_adj_update, g2(mesh, b), b, 0, s, B, kv, step, mult, k, mesh, lam_T, lam_B, assign_B, lamk, partial);
+}
+
+void launch_adj_rhs_cells(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, const double* ka, double sa,
+                          const double* kb, double* lam_S, int assign_S, double* cx, double* cy, hipStream_t s) {
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(k_adj_rhs_cells, g2(mesh, b), b, 0, s, S, st, k, mesh, ka, sa, kb, lam_S, assign_S, cx, cy);
 }
 
 }  // namespace spruce

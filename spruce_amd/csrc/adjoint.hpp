@@ -18,6 +18,14 @@ int adj_blocks(const Mesh& m);
 void launch_adj_stage_cells(MhdSet B, const double* kv, const double* step, double mult, MhdSet S, MhdStatic st,
                             MhdConst k, Mesh mesh, const double* lam_T, double* lam_B, int assign_B, double* lam_S,
                             int assign_S, double* cx, double* cy, double* partial, hipStream_t s);
+// The two halves of launch_adj_stage_cells, for stages whose derivative also feeds a later
+// combination (RK4): the pointwise reverse (lam_B (+)= mu, lamk = mult*h*mu, partial of mult*mu.kv)
+// and the cell part of the transposed time derivative at S applied to sa*ka + kb (kb may be null).
+void launch_adj_update(MhdSet B, const double* kv, const double* step, double mult, MhdConst k, Mesh mesh,
+                       const double* lam_T, double* lam_B, int assign_B, double* lamk, double* partial,
+                       hipStream_t s);
+void launch_adj_rhs_cells(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, const double* ka, double sa,
+                          const double* kb, double* lam_S, int assign_S, double* cx, double* cy, hipStream_t s);
 // The face part of the transposed time derivative at S (both axes, one launch): FP64 atomics into lam_S.
 void launch_adj_faces(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, const double* cx, const double* cy,
                       double* lam_S, hipStream_t s);

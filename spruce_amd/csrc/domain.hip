@@ -148,10 +148,14 @@ static int subcycle_count(double dt, double m) {
 // The state snapshots the adjoint keeps while it replays one step, for that step's reverse.
 struct StepTape {
   std::vector<MhdSet> after_it;    // after each iterating module (its propagate included)
-  MhdSet W{}, A3{};                // RK2 midpoint; after the integrator's final ghost fill
+  MhdSet W{}, A3{};                // RK2 midpoint / RK4 first stage; after the integrator's final ghost fill
+  MhdSet W2{}, W3{}, Lm{};         // RK4: second and third stages, and the live set they and the final
+                                   // stage use as base (its ghost cells refilled from its interior)
   std::vector<MhdSet> after_post;  // after each post-iterate module
   double* k1 = nullptr;
   double* k2 = nullptr;
+  double* k3 = nullptr;            // RK4
+  double* kf = nullptr;            // RK4: (k1 + k4)/6 + (k2 + k3)/3
 };
 
 class Domain : public DomainBase {
@@ -640,6 +644,17 @@ class Domain : public DomainBase {
         moc_limit(W.view());
         stage(W, L, L, 1.0, STAGE_PLAIN, 0, nullptr, dtp);
       }
+    } else if (rec) {                                  // RK4, keeping every stage and derivative
+      launch_stage(stage_args(L.view(), L.view(), rec->W, 0.5, STAGE_STORE_K, 0, nullptr, rec->k1));
+      ghost(L, kinds_live, GHOST_LIVE_RAW);
+      launch_stage(stage_args(rec->W, L.view(), rec->W2, 0.5, STAGE_STORE_K, 0, nullptr, rec->k2));
+      ghost(L, kinds_live, GHOST_LIVE_RAW);
+      launch_stage(stage_args(rec->W2, L.view(), rec->W3, 1.0, STAGE_STORE_K, 0, nullptr, rec->k3));
+      ghost(L, kinds_live, GHOST_LIVE_RAW);
+      copy_set(L.view(), rec->Lm, N, stream);
+      StageArgs a = stage_args(rec->W3, L.view(), L.view(), 1.0, STAGE_RK4_FINAL, 0, nullptr, nullptr);
+      a.kbuf = rec->k1; a.k2buf = rec->k2; a.k3buf = rec->k3; a.kf_out = rec->kf;
+      launch_stage(a);
     } else {                                           // evolution.cpp:103-124
       stage(L, L, W, 0.5, STAGE_STORE_K, 0);
       ghost(W, kinds_ucnp, GHOST_NORMAL);
@@ -2404,7 +2419,8 @@ std::vector<double> Domain::param_grad() {
 void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, double* dt_out) {
   require(ready, SPRUCE_E_STATE, "setup must precede adjoint");
   const int integ = desc.time_integrator;
-  require(integ == SPRUCE_EULER || integ == SPRUCE_RK2, SPRUCE_E_UNSUPPORTED, "the adjoint covers euler and rk2");
+  require(integ == SPRUCE_EULER || integ == SPRUCE_RK2 || integ == SPRUCE_RK4, SPRUCE_E_UNSUPPORTED,
+          "the adjoint covers euler, rk2 and rk4");
   const int* bc = desc.bc;
   for (int k = 0; k < 4; ++k)
     require(bc[k] == SPRUCE_BC_PERIODIC || bc[k] == SPRUCE_BC_OPEN || bc[k] == SPRUCE_BC_FIXED ||
@@ -2437,6 +2453,10 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   require(nsteps >= 1 && ckpt >= 1, SPRUCE_E_ARG, "nsteps and checkpoint_interval must be positive");
   require(seed && grad, SPRUCE_E_ARG, "null argument");
   const bool with_mods = !mods.empty();
+  const bool rk4 = integ == SPRUCE_RK4;
+  // modules and RK4 replay each step of a segment keeping its intermediate states; the plain
+  // Euler/RK2 step keeps its stages for the whole segment instead
+  const bool replay = with_mods || rk4;
   const int nseg = (nsteps + ckpt - 1) / ckpt;
   const int sl = std::min(ckpt, nsteps);
   const bool rk2 = integ == SPRUCE_RK2;
@@ -2447,11 +2467,12 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   // snapshots of one replayed step; the three adjoint vectors, face coefficients, module scratch
   // (dJ/dT, the six frozen conduction fields, column sums) and the reduction partials.
   const size_t SET = 10 * N, EV = 8 * N, CO = (size_t)ADJ_COEFS * N;
-  const size_t seg_store = with_mods ? (size_t)(nit + npost + 2) * SET + 2 * EV
-                                     : (rk2 ? (size_t)sl * SET : 0) + (size_t)sl * EV * (rk2 ? 2 : 1);
-  const size_t modscratch = with_mods ? 7 * N + (size_t)ny + (size_t)nb + 1 : 0;
+  const size_t seg_store = replay ? (size_t)(nit + npost + 2 + (rk4 ? 3 : 0)) * SET + (rk4 ? 4 : 2) * EV
+                                  : (rk2 ? (size_t)sl * SET : 0) + (size_t)sl * EV * (rk2 ? 2 : 1);
+  const size_t modscratch = replay ? 7 * N + (size_t)ny + (size_t)nb + 1 : 0;
+  const size_t rk4scratch = rk4 ? 4 * EV + 2 * (size_t)nb : 0;
   const size_t total = (size_t)nseg * SET + (size_t)(sl + 1) * SET + seg_store + 3 * EV + 2 * CO + 4 * (size_t)nb +
-                       modscratch;
+                       modscratch + rk4scratch;
   DBuf slab;
   slab.alloc(total);
   adj_bytes = (int64_t)(total * sizeof(double));
@@ -2462,7 +2483,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   for (auto& c : seg) c = set_at(take(SET), N);
   std::vector<double*> k1s, k2s;
   StepTape tape;
-  if (with_mods) {
+  if (replay) {
     tape.after_it.resize(nit);
     tape.after_post.resize(npost);
     for (auto& c : tape.after_it) c = set_at(take(SET), N);
@@ -2471,6 +2492,13 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
     tape.A3 = set_at(take(SET), N);
     tape.k1 = take(EV);
     tape.k2 = take(EV);
+    if (rk4) {
+      tape.W2 = set_at(take(SET), N);
+      tape.W3 = set_at(take(SET), N);
+      tape.Lm = set_at(take(SET), N);
+      tape.k3 = take(EV);
+      tape.kf = take(EV);
+    }
   } else {
     mid.resize(rk2 ? sl : 0);
     for (auto& c : mid) c = set_at(take(SET), N);
@@ -2489,12 +2517,23 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   double* pmin = take(nb);
   int* pidx = reinterpret_cast<int*>(take(nb));
   double *lamT = nullptr, *lamfz = nullptr, *colsum = nullptr, *partm = nullptr, *gh = nullptr;
-  if (with_mods) {
+  if (replay) {
     lamT = take(N);
     lamfz = take(6 * N);
     colsum = take(ny);
     partm = take(nb);
     gh = take(1);
+  }
+  // RK4: the adjoint of the stages' common base, the final combination's h*mu, one stage's
+  // mult*h*mu, a second stage adjoint, and the partials of the two middle stages
+  double *lamM = nullptr, *lkf = nullptr, *lka = nullptr, *lamX = nullptr, *part3 = nullptr, *part4 = nullptr;
+  if (rk4) {
+    lamM = take(EV);
+    lkf = take(EV);
+    lka = take(EV);
+    lamX = take(EV);
+    part3 = take(nb);
+    part4 = take(nb);
   }
   if ((int)history.n < nsteps + 1) history.alloc(nsteps + 1);
   hip_check(hipMemcpyAsync(history.p, step.p, sizeof(double), hipMemcpyDeviceToDevice, stream), "copy step");
@@ -2520,7 +2559,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   // live grids, so a copy is restored at the end
   DBuf keep;
   MhdSet kept{};
-  if (with_mods) {
+  if (replay) {
     keep.alloc(SET);
     kept = set_at(keep.p, N);
     copy_set(L.view(), kept, N, stream);
@@ -2534,7 +2573,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int n0 = sg * ckpt, n1 = std::min(nsteps, n0 + ckpt);
     copy_set(cps[sg], seg[0], N, stream);
-    if (!with_mods) {
+    if (!replay) {
       // recompute the segment: states, k1, W, k2 (and the final ghost fill of each step)
       for (int n = n0; n < n1; ++n) {
         const int i = n - n0;
@@ -2559,7 +2598,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
       const int i = n - n0;
       const MhdSet& Ln = seg[i];
       const double* hp = history.p + n;
-      if (!with_mods) {
+      if (!replay) {
         adj_ghost(seg[i + 1], lam);
         // lamN and lamW are written first by the stage-cell kernels (assign), so they need no clear
         if (rk2) {
@@ -2602,7 +2641,39 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
       }
       adj_ghost(tape.A3, lam);                             // the integrator's final ghost fill
       const MhdSet& A2 = nit ? tape.after_it[nit - 1] : Ln;
-      if (rk2) {
+      if (rk4) {
+        // final: L = F(Lm + h kf), kf = (k1 + k4)/6 + (k2 + k3)/3 with k4 = k(W3); lkf = h mu
+        timed("adj_stage", [&] {
+          launch_adj_update(tape.Lm, tape.kf, hp, 1.0, kc, mesh, lam, lamM, 1, lkf, part, stream);
+          launch_adj_rhs_cells(tape.W3, st, kc, mesh, lkf, 1.0 / 6.0, nullptr, lamW, 1, cx, cy, stream);
+        });
+        timed("adj_faces", [&] { launch_adj_faces(tape.W3, st, kc, mesh, cx, cy, lamW, stream); });
+        // W3 = F(Lm + h k3), k3 = k(W2): lambda_k3 = h mu3 + lkf/3
+        timed("adj_stage", [&] {
+          launch_adj_update(tape.Lm, tape.k3, hp, 1.0, kc, mesh, lamW, lamM, 0, lka, part2, stream);
+          launch_adj_rhs_cells(tape.W2, st, kc, mesh, lkf, 1.0 / 3.0, lka, lamX, 1, cx, cy, stream);
+        });
+        timed("adj_faces", [&] { launch_adj_faces(tape.W2, st, kc, mesh, cx, cy, lamX, stream); });
+        // W2 = F(Lm + h/2 k2), k2 = k(W): lambda_k2 = h/2 mu2 + lkf/3
+        timed("adj_stage", [&] {
+          launch_adj_update(tape.Lm, tape.k2, hp, 0.5, kc, mesh, lamX, lamM, 0, lka, part3, stream);
+          launch_adj_rhs_cells(tape.W, st, kc, mesh, lkf, 1.0 / 3.0, lka, lamW, 1, cx, cy, stream);
+        });
+        timed("adj_faces", [&] { launch_adj_faces(tape.W, st, kc, mesh, cx, cy, lamW, stream); });
+        // W = F(A2 + h/2 k1), k1 = k(A2): lambda_k1 = h/2 mu1 + lkf/6
+        timed("adj_stage", [&] {
+          launch_adj_update(A2, tape.k1, hp, 0.5, kc, mesh, lamW, lamN, 1, lka, part4, stream);
+          launch_adj_rhs_cells(A2, st, kc, mesh, lkf, 1.0 / 6.0, lka, lamN, 0, cx, cy, stream);
+        });
+        // (its faces follow below, as for the other integrators); Lm is A2 with its ghost cells refilled from A2's interior without floors
+        // (GHOST_LIVE_RAW, evolution.cpp:126-152 writing the live grids): its ghost adjoint goes to
+        // that interior, the rest is A2's own
+        launch_adj_ghost_axis(A2, kc, mesh, kinds_live, s1, s2, 1, lamM, stream, 1);
+        launch_adj_ghost_axis(A2, kc, mesh, kinds_live, s1, s2, 0, lamM, stream, 1);
+        launch_add(lamN, lamM, 8 * N, stream);
+        launch_adj_accum(part3, nb, 1.0, gh, stream);
+        launch_adj_accum(part4, nb, 1.0, gh, stream);
+      } else if (rk2) {
         timed("adj_stage", [&] {
           launch_adj_stage_cells(A2, tape.k2, hp, 1.0, tape.W, st, kc, mesh, lam, lamN, 1, lamW, 1, cx, cy, part,
                                  stream);
@@ -2649,12 +2720,12 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
         }
       }
       timed("adj_dt", [&] {
-        launch_adj_dt(Ln, st, kc, mesh, kc.eps, part, rk2 ? part2 : nullptr, gh, pmin, pidx, lam, stream);
+        launch_adj_dt(Ln, st, kc, mesh, kc.eps, part, (rk2 || rk4) ? part2 : nullptr, gh, pmin, pidx, lam, stream);
       });
     }
   }
   hip_check(hipMemcpyAsync(grad, lam, 8 * N * 8, hipMemcpyDeviceToHost, stream), "download gradient");
-  if (with_mods) {                                      // the state after the steps and its step size
+  if (replay) {                                         // the state after the steps and its step size
     copy_set(kept, L.view(), N, stream);
     finish_step(-1);
     for (auto& m : mods)                                  // conduction's subcycle record

@@ -51,6 +51,10 @@ __device__ __forceinline__ StageOut stage_finish(const StageArgs& a, size_t N, s
       const double k1 = a.kbuf[v * N + c], k2 = a.k2buf[v * N + c], k3 = a.k3buf[v * N + c];
       k[v] = qdiv(k1 + k[v], 6.0, a.k.r6) + qdiv(k2 + k3, 3.0, a.k.r3);
     }
+    if (a.kf_out) {
+#pragma unroll
+      for (int v = 0; v < 8; ++v) a.kf_out[v * N + c] = k[v];
+    }
   }
   const double step = a.step_mult * (*a.step);
   const MhdConst& K = a.k;

@@ -70,6 +70,7 @@ struct StageArgs {
   double* kbuf;          // 8 * nx*ny: k store (STORE_K) or k1 (RK4_FINAL)
   const double* k2buf;   // RK4_FINAL: k2
   const double* k3buf;   // RK4_FINAL: k3
+  double* kf_out;        // RK4_FINAL, when not null: the combined (k1 + k4)/6 + (k2 + k3)/3 (the adjoint's record)
   int mode;
   int write_pre;         // write rho_pre (open boundaries present)
   int write_n;

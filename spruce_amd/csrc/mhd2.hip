@@ -240,6 +240,10 @@ k_mhd_stage_strip(StageArgs a, int R, int nstrips) {
         const double k1 = a.kbuf[w * N + c], k2 = a.k2buf[w * N + c], k3 = a.k3buf[w * N + c];
         k8[w] = qdiv(k1 + k8[w], 6.0, K.r6) + qdiv(k2 + k3, 3.0, K.r3);
       }
+      if (a.kf_out) {
+#pragma unroll
+        for (int w = 0; w < 8; ++w) a.kf_out[w * N + c] = k8[w];
+      }
     }
     double b[8];
     if (a.B.rho == a.S.rho) {

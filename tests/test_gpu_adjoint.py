@@ -31,7 +31,7 @@ def _loss(grids, base, cfg, nsteps, w):
     return sum(float(np.sum(wk * o)) for wk, o in zip(w, out))
 
 
-@pytest.mark.parametrize("integ,nsteps,ckpt", [("rk2", 24, 5), ("euler", 16, 16), ("rk2", 9, 1)])
+@pytest.mark.parametrize("integ,nsteps,ckpt", [("rk2", 24, 5), ("euler", 16, 16), ("rk2", 9, 1), ("rk4", 12, 5)])
 def test_adjoint_matches_finite_differences(integ, nsteps, ckpt):
     """dJ/dstate . d against central differences at two step sizes (relative 1e-7 and 1e-8 along
     seeded random directions).  The loss is only piecewise smooth: the limiter's min/max selections
@@ -75,9 +75,10 @@ def test_adjoint_matches_finite_differences(integ, nsteps, ckpt):
     assert worst < 1e-6
 
 
-def test_adjoint_leaves_the_same_final_state_as_stepping():
+@pytest.mark.parametrize("integ", ["rk2", "rk4"])
+def test_adjoint_leaves_the_same_final_state_as_stepping(integ):
     base = W.periodic_wave_state(32, 40, seed=2)
-    cfg = W.periodic_wave_config(steps=10)
+    cfg = W.periodic_wave_config(steps=10, integrator=integ)
     a = GpuDomain(spec_from_files(cfg, base))
     b = GpuDomain(spec_from_files(cfg, base))
     try:

@@ -127,6 +127,9 @@ BOUNDARIES = [
     ("fixed_reflect_rk2", ("periodic", "periodic", "fixed", "reflect"), "rk2", 12, 5),
     ("reflect_fixed_x_euler", ("reflect", "fixed", "periodic", "periodic"), "euler", 10, 4),
     ("fixed_open_rk2", ("periodic", "periodic", "fixed", "open"), "rk2", 12, 4),
+    # RK4: stages 2-4 read the live grids whose ghost cells the intermediate fills rewrote
+    ("fixed_reflect_rk4", ("periodic", "periodic", "fixed", "reflect"), "rk4", 10, 4),
+    ("open_fixed_x_rk4", ("open", "fixed", "periodic", "periodic"), "rk4", 10, 4),
 ]
 
 
@@ -175,6 +178,7 @@ MODULE_CASES = [
     ("tc_rl_heat_rk2", ("periodic", "periodic", "fixed", "open"), "rk2", None, 6, 4),
     ("tc_rl2_ambient_euler", ("periodic", "periodic", "reflect", "fixed"), "euler",
      [TC, RL("rk2"), ("ambient_heating", [("heating_rate", "3e-3")]), HEAT(THETA[:8])], 6, 3),
+    ("tc_rl_heat_rk4", ("periodic", "periodic", "fixed", "open"), "rk4", None, 5, 3),
 ]
 
 
