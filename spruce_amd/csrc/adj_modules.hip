@@ -211,6 +211,107 @@ __global__ void __launch_bounds__(256) k_adj_tc_sub(const double* te_s, const do
   if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
 }
 
+// Generalised subcycle stage reverse: the energy derivative dE is evaluated at the temperatures of
+// `eval`; with cbase given the stage output was max(cbase + mask*(mult*h)*dE, te_min), whose
+// adjoint lam_out is reversed through the clamp into lam_base (assigned or added) and seeds dE with
+// (mult*h)*mu; seed_add (may be null) adds add_coef * seed_add[c] to that seed (the weights a later
+// RK4 combination gives this stage's dE).
+__global__ void __launch_bounds__(256) k_adj_tc_stage(const double* eval, const double* n, const double* bhx,
+                                                      const double* bhy, const double* dbxx, const double* dbxy,
+                                                      const double* dbyx, const double* dbyy, double h, double mult,
+                                                      double k0, MhdConst K, Mesh M, const double* cbase,
+                                                      const double* lam_out, double* lam_base, int assign_base,
+                                                      const double* seed_add, double add_coef, double* lam_T,
+                                                      double* lam_fz, double* partial) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  double dot = 0.0;
+  if (i < M.nx && j < M.ny) {
+    const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+    if (!inside(M, i, j)) {
+      if (cbase) {
+        const double mu = (cbase[c] + 0.0 < K.temin) ? 0.0 : lam_out[c];
+        lam_base[c] = assign_base ? mu : lam_base[c] + mu;
+      }
+    } else {
+      size_t q[3][3];
+      double T[3][3];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          q[a][b] = (size_t)wrp(i - 1 + a, M.nx, M.x.per) * M.ny + wrp(j - 1 + b, M.ny, M.y.per);
+          T[a][b] = temp_of(eval[q[a][b]], n[q[a][b]], K);
+        }
+      double gT[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+      double gfz[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      const double dE = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0,
+                                             1.0, gT, gfz);
+      double ld = 0.0;
+      if (cbase) {
+        const double v = cbase[c] + 1.0 * (mult * h) * dE;
+        const double mu = (v < K.temin) ? 0.0 : lam_out[c];
+        lam_base[c] = assign_base ? mu : lam_base[c] + mu;
+        dot = mu * mult * dE;
+        ld = mu * (mult * h);
+      }
+      if (seed_add) ld += add_coef * seed_add[c];
+      if (ld != 0.0) {
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) atomicAdd(&lam_T[q[a][b]], ld * gT[a][b]);
+        for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += ld * gfz[k];
+      }
+    }
+  }
+  dot = block_sum(dot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
+}
+
+// Reverse of an RK4 subcycle's final combination out = max(te + (1.0*h*S)/6, te_min) with
+// S = k1 + 2 k2 + 2 k3 + k4, k4 = dE at the temperatures of i3 (thermalconduction.cpp:122-124):
+// lam_base = mu (assigned), lamS = mu*h/6 (the seed of each k, times its weight), k4's transpose
+// applied here, partial = mu*S/6.
+__global__ void __launch_bounds__(256) k_adj_tc_final4(const double* te, const double* k1, const double* k2,
+                                                       const double* k3, const double* i3, const double* n,
+                                                       const double* bhx, const double* bhy, const double* dbxx,
+                                                       const double* dbxy, const double* dbyx, const double* dbyy,
+                                                       double h, double k0, MhdConst K, Mesh M,
+                                                       const double* lam_out, double* lam_base, double* lamS,
+                                                       double* lam_T, double* lam_fz, double* partial) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  double dot = 0.0;
+  if (i < M.nx && j < M.ny) {
+    const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+    if (!inside(M, i, j)) {
+      lam_base[c] = (te[c] + 0.0 < K.temin) ? 0.0 : lam_out[c];
+      lamS[c] = 0.0;
+    } else {
+      size_t q[3][3];
+      double T[3][3];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          q[a][b] = (size_t)wrp(i - 1 + a, M.nx, M.x.per) * M.ny + wrp(j - 1 + b, M.ny, M.y.per);
+          T[a][b] = temp_of(i3[q[a][b]], n[q[a][b]], K);
+        }
+      double gT[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+      double gfz[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      const double dE = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0,
+                                             1.0, gT, gfz);
+      const double sum = k1[c] + 2.0 * k2[c] + 2.0 * k3[c] + dE;
+      const double v = te[c] + qdiv(1.0 * h * sum, 6.0, K.r6);
+      const double mu = (v < K.temin) ? 0.0 : lam_out[c];
+      lam_base[c] = mu;
+      const double ls = mu * h / 6.0;
+      lamS[c] = ls;
+      dot = mu * sum / 6.0;
+      if (ls != 0.0) {
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) atomicAdd(&lam_T[q[a][b]], ls * gT[a][b]);
+        for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += ls * gfz[k];
+      }
+    }
+  }
+  dot = block_sum(dot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
+}
+
 __global__ void __launch_bounds__(256) k_adj_tc_temp(const double* te_s, const double* n, MhdConst K, size_t N,
                                                      double* lam_T, double* lam_te, double* lam_rho) {
   const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -399,6 +500,25 @@ void launch_adj_tc_sub(const double* te_s, const double* n, const double* bhx, c
                        double* partial, hipStream_t s) {
   hipLaunchKernelGGL(k_adj_tc_sub, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, te_s, n, bhx, bhy, dbxx, dbxy, dbyx,
                      dbyy, h, k0, k, mesh, lam_te, lam_T, lam_fz, partial);
+}
+
+void launch_adj_tc_stage(const double* eval, const double* n, const double* bhx, const double* bhy,
+                         const double* dbxx, const double* dbxy, const double* dbyx, const double* dbyy, double h,
+                         double mult, double k0, MhdConst k, Mesh mesh, const double* cbase, const double* lam_out,
+                         double* lam_base, int assign_base, const double* seed_add, double add_coef, double* lam_T,
+                         double* lam_fz, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_tc_stage, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, eval, n, bhx, bhy, dbxx, dbxy, dbyx,
+                     dbyy, h, mult, k0, k, mesh, cbase, lam_out, lam_base, assign_base, seed_add, add_coef, lam_T,
+                     lam_fz, partial);
+}
+
+void launch_adj_tc_final4(const double* te, const double* k1, const double* k2, const double* k3, const double* i3,
+                          const double* n, const double* bhx, const double* bhy, const double* dbxx,
+                          const double* dbxy, const double* dbyx, const double* dbyy, double h, double k0, MhdConst k,
+                          Mesh mesh, const double* lam_out, double* lam_base, double* lamS, double* lam_T,
+                          double* lam_fz, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_tc_final4, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, te, k1, k2, k3, i3, n, bhx, bhy,
+                     dbxx, dbxy, dbyx, dbyy, h, k0, k, mesh, lam_out, lam_base, lamS, lam_T, lam_fz, partial);
 }
 
 void launch_adj_tc_temp(const double* te_s, const double* n, MhdConst k, size_t N, double* lam_T, double* lam_te,

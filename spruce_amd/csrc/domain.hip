@@ -900,10 +900,15 @@ struct TcModule : GpuModule {
   bool to_file = false;           // output_to_file: the average heating rate and the saturation factor
   double msf = 1.0;               // ms_electron_heating_fraction (thermalconduction.hpp:34)
   DBuf avg, sat;
-  // while recording (the adjoint's replay of a step): te after each Euler subcycle
+  // while recording (the adjoint's replay of a step), per subcycle: Euler: te after it; RK2: the
+  // midpoint, te after it; RK4: i1, i2, i3, k1, k2, k3, te after it
   bool recording = false;
   std::vector<DBuf> tape;
   double h_last = 0.0;
+  int rec_per_sub() const { return integ == "euler" ? 1 : (integ == "rk2" ? 2 : 7); }
+  void record(Domain& d, int s, int k, const double* src);
+  const double* rec_at(int s, int k) const { return tape[(size_t)s * rec_per_sub() + k].p; }
+  const double* rec_out(int s) const { return rec_at(s, rec_per_sub() - 1); }
   explicit TcModule(const std::vector<std::pair<std::string, std::string>>& o) {
     name = "thermal_conduction";
     bool f1, f2, f3;
@@ -995,26 +1000,25 @@ struct TcModule : GpuModule {
         double* out = other(cur, nullptr);
         step(d, cur, cur, out, 1.0, TC_UPDATE, nullptr, h);
         cur = out;
-        if (recording) {
-          if ((int)tape.size() <= s) {
-            tape.emplace_back();
-            tape.back().alloc(d.N);
-          }
-          hip_check(hipMemcpyAsync(tape[s].p, out, d.N * 8, hipMemcpyDeviceToDevice, d.stream), "record te");
-        }
+        if (recording) record(d, s, 0, out);
       } else if (integ == "rk2") {
         double* half = other(cur, nullptr);
         step(d, cur, cur, half, 0.5, TC_UPDATE, nullptr, h);
         double* out = other(cur, half);
         step(d, half, cur, out, 1.0, TC_UPDATE, nullptr, h);
+        if (recording) { record(d, s, 0, half); record(d, s, 1, out); }
         cur = out;
       } else {
         double* i1 = other(cur, nullptr);
         step(d, cur, cur, i1, 0.5, TC_STORE_K, k1.p, h);
+        if (recording) { record(d, s, 0, i1); record(d, s, 3, k1.p); }
         double* i2 = other(cur, i1);
         step(d, i1, cur, i2, 0.5, TC_STORE_K, k2.p, h);
+        if (recording) { record(d, s, 1, i2); record(d, s, 4, k2.p); }
         step(d, i2, cur, i1, 1.0, TC_STORE_K, k3.p, h);
+        if (recording) { record(d, s, 2, i1); record(d, s, 5, k3.p); }
         step(d, i1, cur, i2, 1.0, TC_RK4_FINAL, nullptr, h);
+        if (recording) record(d, s, 6, i2);
         cur = i2;
       }
     }
@@ -1030,6 +1034,15 @@ struct TcModule : GpuModule {
     d.propagate_live();
   }
 };
+
+void TcModule::record(Domain& d, int s, int k, const double* src) {
+  const size_t idx = (size_t)s * rec_per_sub() + k;
+  while (tape.size() <= idx) {
+    tape.emplace_back();
+    tape.back().alloc(d.N);
+  }
+  hip_check(hipMemcpyAsync(tape[idx].p, src, d.N * 8, hipMemcpyDeviceToDevice, d.stream), "record conduction");
+}
 
 // ---------------------------------------------------------------------------------------------
 // RadiativeLosses (source/modules/solar/radiativelosses.cpp)
@@ -2436,8 +2449,8 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   for (size_t k = 0; k < mods.size(); ++k) {
     GpuModule* m = mods[k].get();
     if (auto* tc = dynamic_cast<TcModule*>(m)) {
-      require(tc->integ == "euler" && !tc->p.saturation && !tc->inactive, SPRUCE_E_UNSUPPORTED,
-              "the adjoint covers thermal conduction with euler subcycles and no flux saturation");
+      require(!tc->p.saturation && !tc->inactive, SPRUCE_E_UNSUPPORTED,
+              "the adjoint covers thermal conduction without flux saturation");
       it_mods.push_back((int)k);
     } else if (auto* rl = dynamic_cast<RlModule*>(m)) {
       require(!rl->p.prevent && !rl->inactive, SPRUCE_E_UNSUPPORTED,
@@ -2469,7 +2482,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   const size_t SET = 10 * N, EV = 8 * N, CO = (size_t)ADJ_COEFS * N;
   const size_t seg_store = replay ? (size_t)(nit + npost + 2 + (rk4 ? 3 : 0)) * SET + (rk4 ? 4 : 2) * EV
                                   : (rk2 ? (size_t)sl * SET : 0) + (size_t)sl * EV * (rk2 ? 2 : 1);
-  const size_t modscratch = replay ? 7 * N + (size_t)ny + (size_t)nb + 1 : 0;
+  const size_t modscratch = replay ? 10 * N + (size_t)ny + (size_t)nb + 1 : 0;
   const size_t rk4scratch = rk4 ? 4 * EV + 2 * (size_t)nb : 0;
   const size_t total = (size_t)nseg * SET + (size_t)(sl + 1) * SET + seg_store + 3 * EV + 2 * CO + 4 * (size_t)nb +
                        modscratch + rk4scratch;
@@ -2517,9 +2530,13 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   double* pmin = take(nb);
   int* pidx = reinterpret_cast<int*>(take(nb));
   double *lamT = nullptr, *lamfz = nullptr, *colsum = nullptr, *partm = nullptr, *gh = nullptr;
+  double *lacc = nullptr, *lx = nullptr, *lamS = nullptr;   // conduction RK2/RK4 subcycle reverse
   if (replay) {
     lamT = take(N);
     lamfz = take(6 * N);
+    lacc = take(N);
+    lx = take(N);
+    lamS = take(N);
     colsum = take(ny);
     partm = take(nb);
     gh = take(1);
@@ -2697,16 +2714,54 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
         adj_ghost(post, lam);
         if (auto* tc = dynamic_cast<TcModule*>(m)) {
           const int ns = tc->nsub;
-          const double* te_last = ns ? tc->tape[ns - 1].p : pre.te;
+          const double* te_last = ns ? tc->rec_out(ns - 1) : pre.te;
+          const double hs = tc->h_last, k0 = tc->p.k0;
+          double* lte = lam + 4 * N;
+          // one stage reverse: dE at `ev`, clamp base te_s (or none), seed extra
+          auto stage_rev = [&](const double* ev, double mult, const double* cb, const double* lout, int assign,
+                               const double* sadd, double coef) {
+            launch_adj_tc_stage(ev, pre.n, bhx.p, bhy.p, tc->dbxx.p, tc->dbxy.p, tc->dbyx.p, tc->dbyy.p, hs, mult, k0,
+                                kc, mesh, cb, lout, lacc, assign, sadd, coef, lamT, lamfz, partm, stream);
+            launch_adj_accum(partm, nb, 1.0 / (double)ns, gh, stream);
+          };
+          // dJ/dT at the temperatures of `ev` onto dJ/dte (into `into`, cleared first unless it
+          // accumulates) and dJ/drho
+          auto temp_rev = [&](const double* ev, double* into, bool clear) {
+            if (clear) launch_fill(into, 0.0, N, stream);
+            launch_adj_tc_temp(ev, pre.n, kc, N, lamT, into, lam, stream);
+          };
           timed("adj_tc", [&] {
             launch_adj_minmask(pre.rho, te_last, nullptr, nullptr, kc, mesh, lam, nullptr, stream);
             launch_fill(lamfz, 0.0, 6 * N, stream);
             for (int q = ns - 1; q >= 0; --q) {
-              const double* te_s = q ? tc->tape[q - 1].p : pre.te;
-              launch_adj_tc_sub(te_s, pre.n, bhx.p, bhy.p, tc->dbxx.p, tc->dbxy.p, tc->dbyx.p, tc->dbyy.p, tc->h_last,
-                                tc->p.k0, kc, mesh, lam + 4 * N, lamT, lamfz, partm, stream);
-              launch_adj_accum(partm, nb, 1.0 / (double)ns, gh, stream);
-              launch_adj_tc_temp(te_s, pre.n, kc, N, lamT, lam + 4 * N, lam, stream);
+              const double* te_s = q ? tc->rec_out(q - 1) : pre.te;
+              if (tc->integ == "euler") {
+                launch_adj_tc_sub(te_s, pre.n, bhx.p, bhy.p, tc->dbxx.p, tc->dbxy.p, tc->dbyx.p, tc->dbyy.p, hs, k0,
+                                  kc, mesh, lte, lamT, lamfz, partm, stream);
+                launch_adj_accum(partm, nb, 1.0 / (double)ns, gh, stream);
+                launch_adj_tc_temp(te_s, pre.n, kc, N, lamT, lte, lam, stream);
+                continue;
+              }
+              if (tc->integ == "rk2") {                 // thermalconduction.cpp:67-78
+                const double* half = tc->rec_at(q, 0);
+                stage_rev(half, 1.0, te_s, lte, 1, nullptr, 0.0);
+                temp_rev(half, lx, true);
+                stage_rev(te_s, 0.5, te_s, lx, 0, nullptr, 0.0);
+              } else {                                  // thermalconduction.cpp:79-98
+                const double *i1 = tc->rec_at(q, 0), *i2 = tc->rec_at(q, 1), *i3 = tc->rec_at(q, 2);
+                launch_adj_tc_final4(te_s, tc->rec_at(q, 3), tc->rec_at(q, 4), tc->rec_at(q, 5), i3, pre.n, bhx.p,
+                                     bhy.p, tc->dbxx.p, tc->dbxy.p, tc->dbyx.p, tc->dbyy.p, hs, k0, kc, mesh, lte, lacc,
+                                     lamS, lamT, lamfz, partm, stream);
+                launch_adj_accum(partm, nb, 1.0 / (double)ns, gh, stream);
+                temp_rev(i3, lx, true);
+                stage_rev(i2, 1.0, te_s, lx, 0, lamS, 2.0);
+                temp_rev(i2, lx, true);
+                stage_rev(i1, 0.5, te_s, lx, 0, lamS, 2.0);
+                temp_rev(i1, lx, true);
+                stage_rev(te_s, 0.5, te_s, lx, 0, lamS, 1.0);
+              }
+              temp_rev(te_s, lacc, false);
+              hip_check(hipMemcpyAsync(lte, lacc, N * 8, hipMemcpyDeviceToDevice, stream), "copy conduction adjoint");
             }
             launch_adj_bhat(Ln, st, mesh, lamfz, lam, stream);
           });
