@@ -191,12 +191,12 @@ void spruce_gpu_destroy(spruce_gpu_domain* d);
  * backward through the hand-written reverse of every stage, minimum/derived update and step-size
  * reduction, recomputing each segment from its checkpoint.  seed = dJ/d(evolved grids after the
  * steps); grad_out = dJ/d(evolved grids before them); both 8*xdim*ydim doubles in evolved order
- * (rho, mom_x, mom_y, mom_z, thermal_energy, bi_x, bi_y, bi_z).  Covers ideal_mhd, euler or rk2,
- * with periodic, fixed, reflect and open boundaries (open only with the other axis periodic) and the
- * modules thermal_conduction (euler subcycles, no flux saturation), radiative_losses (no
- * prevent_subcycling), ambient_heating and parametric_heating; SPRUCE_E_UNSUPPORTED otherwise.
- * With modules each step of the backward sweep is replayed once more to keep its intermediate
- * states (subcycle temperatures, module outputs). */
+ * (rho, mom_x, mom_y, mom_z, thermal_energy, bi_x, bi_y, bi_z).  Covers ideal_mhd, euler, rk2 or
+ * rk4, with periodic, fixed, reflect and open boundaries (open only with the other axis periodic)
+ * and the modules thermal_conduction (any subcycle integrator, no flux saturation),
+ * radiative_losses (no prevent_subcycling), ambient_heating and parametric_heating;
+ * SPRUCE_E_UNSUPPORTED otherwise.  With modules or rk4 each step of the backward sweep is replayed
+ * once more to keep its intermediate states (stages, subcycle temperatures, module outputs). */
 int spruce_gpu_adjoint(spruce_gpu_domain* d, int32_t nsteps, int32_t checkpoint_interval, const double* seed,
                        double* grad_out, double* dt_out);
 /* Gradient of the last spruce_gpu_adjoint call with respect to module parameters (no reference
