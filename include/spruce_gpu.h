@@ -192,7 +192,7 @@ void spruce_gpu_destroy(spruce_gpu_domain* d);
  * reduction, recomputing each segment from its checkpoint.  seed = dJ/d(evolved grids after the
  * steps); grad_out = dJ/d(evolved grids before them); both 8*xdim*ydim doubles in evolved order
  * (rho, mom_x, mom_y, mom_z, thermal_energy, bi_x, bi_y, bi_z).  Covers ideal_mhd, euler, rk2 or
- * rk4, with periodic, fixed, reflect and open boundaries (open only with the other axis periodic)
+ * rk4, with periodic, fixed, reflect, open (only with the other axis periodic) and open_ucnp boundaries
  * and the modules thermal_conduction (any subcycle integrator, no flux saturation),
  * radiative_losses (no prevent_subcycling), ambient_heating and parametric_heating;
  * SPRUCE_E_UNSUPPORTED otherwise.  With modules or rk4 each step of the backward sweep is replayed

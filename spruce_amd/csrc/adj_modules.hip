@@ -82,7 +82,7 @@ __device__ void open_ghost_dual(const MhdConst& K, const Axis& ax, bool xside, b
 
 __device__ void adj_ghost_side(const AdjGhostArgs& a, int side, int t, double* lam) {
   const int kind = a.kind[side];
-  if (kind != 1 && kind != 2 && kind != 3) return;
+  if (kind != 1 && kind != 2 && kind != 3 && kind != 5) return;
   const Mesh& M = a.mesh;
   const bool xside = side < 2, upper = side & 1;
   const Axis& ax = xside ? M.x : M.y;
@@ -99,6 +99,17 @@ __device__ void adj_ghost_side(const AdjGhostArgs& a, int side, int t, double* l
   const size_t N = (size_t)M.nx * M.ny;
   double* lr = lam + 0 * N; double* lmx = lam + 1 * N; double* lmy = lam + 2 * N; double* lmz = lam + 3 * N;
   double* lte = lam + 4 * N;
+  if (kind == 5) {                 // open_ucnp: both ghost cells copy the first interior cell
+    const size_t ci = at(upper ? ax.hi : ax.lo);
+    for (int g = N_GHOST - 1; g >= 0; --g) {
+      const size_t cb = at(upper ? ax.hi + 1 + g : g);
+      for (int v = 0; v < 8; ++v) {
+        lam[v * N + ci] += lam[v * N + cb];
+        lam[v * N + cb] = 0.0;
+      }
+    }
+    return;
+  }
   if (kind == 2) {                 // fixed: momenta of c1, c2, c3 set to zero
     lmx[c1] = 0.0; lmx[c2] = 0.0; lmx[c3] = 0.0;
     lmy[c1] = 0.0; lmy[c2] = 0.0; lmy[c3] = 0.0;

@@ -597,9 +597,10 @@ class Domain : public DomainBase {
     else timed("mhd_stage", [&] { launch_mhd_stage_strip(a, strip_rows, stream); });
   }
 
-  void ghost(SetBufs& G, const int* kinds, int mode) {
+  void ghost(SetBufs& G, const int* kinds, int mode) { ghost_set(G.view(), kinds, mode); }
+  void ghost_set(const MhdSet& G, const int* kinds, int mode) {
     if (!(kinds[0] || kinds[1] || kinds[2] || kinds[3])) return;
-    timed("mhd_ghost", [&] { launch_mhd_ghost(G.view(), kc, mesh, kinds, s1, s2, mode, stream); });
+    timed("mhd_ghost", [&] { launch_mhd_ghost(G, kc, mesh, kinds, s1, s2, mode, stream); });
   }
 
   // End of a propagate on the live grids: per-cell dt, then the step size of the next step
@@ -634,6 +635,7 @@ class Domain : public DomainBase {
     } else if (integ == SPRUCE_RK2) {                  // evolution.cpp:90-101
       if (rec) {                                       // the same stages, keeping k1, k2 and the midpoint
         launch_stage(stage_args(L.view(), L.view(), rec->W, 0.5, STAGE_STORE_K, 0, nullptr, rec->k1));
+        ghost_set(rec->W, kinds_ucnp, GHOST_NORMAL);
         launch_stage(stage_args(rec->W, L.view(), L.view(), 1.0, STAGE_STORE_K, 0, nullptr, rec->k2));
       } else {
         stage(L, L, W, 0.5, STAGE_PLAIN, 0);
@@ -646,10 +648,13 @@ class Domain : public DomainBase {
       }
     } else if (rec) {                                  // RK4, keeping every stage and derivative
       launch_stage(stage_args(L.view(), L.view(), rec->W, 0.5, STAGE_STORE_K, 0, nullptr, rec->k1));
+      ghost_set(rec->W, kinds_ucnp, GHOST_NORMAL);
       ghost(L, kinds_live, GHOST_LIVE_RAW);
       launch_stage(stage_args(rec->W, L.view(), rec->W2, 0.5, STAGE_STORE_K, 0, nullptr, rec->k2));
+      ghost_set(rec->W2, kinds_ucnp, GHOST_NORMAL);
       ghost(L, kinds_live, GHOST_LIVE_RAW);
       launch_stage(stage_args(rec->W2, L.view(), rec->W3, 1.0, STAGE_STORE_K, 0, nullptr, rec->k3));
+      ghost_set(rec->W3, kinds_ucnp, GHOST_NORMAL);
       ghost(L, kinds_live, GHOST_LIVE_RAW);
       copy_set(L.view(), rec->Lm, N, stream);
       StageArgs a = stage_args(rec->W3, L.view(), L.view(), 1.0, STAGE_RK4_FINAL, 0, nullptr, nullptr);
@@ -798,9 +803,10 @@ class Domain : public DomainBase {
   // segment recomputed from its checkpoint during the backward sweep.  Defined after the modules.
   void adjoint(int nsteps, int ckpt, const double* seed, double* grad, double* dt_out) override;
   void replay_step(int n, const MhdSet& start, StepTape& tp, const std::vector<double>& times);
-  void adj_ghost(const MhdSet& P, double* lam) {
-    launch_adj_ghost_axis(P, kc, mesh, kinds_all, s1, s2, 1, lam, stream);   // y sides were filled last
-    launch_adj_ghost_axis(P, kc, mesh, kinds_all, s1, s2, 0, lam, stream);
+  void adj_ghost(const MhdSet& P, double* lam, const int* kinds = nullptr) {
+    const int* k = kinds ? kinds : kinds_all;
+    launch_adj_ghost_axis(P, kc, mesh, k, s1, s2, 1, lam, stream);   // y sides were filled last
+    launch_adj_ghost_axis(P, kc, mesh, k, s1, s2, 0, lam, stream);
   }
   // dJ/dtheta of the parametrised heating modules after the last adjoint call
   std::vector<double> param_grad() override;
@@ -2437,8 +2443,8 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   const int* bc = desc.bc;
   for (int k = 0; k < 4; ++k)
     require(bc[k] == SPRUCE_BC_PERIODIC || bc[k] == SPRUCE_BC_OPEN || bc[k] == SPRUCE_BC_FIXED ||
-                bc[k] == SPRUCE_BC_REFLECT, SPRUCE_E_UNSUPPORTED,
-            "the adjoint covers periodic, open, fixed and reflect boundaries");
+                bc[k] == SPRUCE_BC_REFLECT || bc[k] == SPRUCE_BC_OPEN_UCNP, SPRUCE_E_UNSUPPORTED,
+            "the adjoint covers periodic, open, fixed, reflect and open_ucnp boundaries");
   require(!((bc[0] == SPRUCE_BC_OPEN || bc[1] == SPRUCE_BC_OPEN) && bc[2] != SPRUCE_BC_PERIODIC) &&
               !((bc[2] == SPRUCE_BC_OPEN || bc[3] == SPRUCE_BC_OPEN) && bc[0] != SPRUCE_BC_PERIODIC),
           SPRUCE_E_UNSUPPORTED, "the adjoint of an open boundary needs the other axis periodic");
@@ -2598,6 +2604,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
           stage_view(seg[i], seg[i], seg[i + 1], 1.0, STAGE_STORE_K, history.p + n, k1s[i]);
         } else {
           stage_view(seg[i], seg[i], mid[i], 0.5, STAGE_STORE_K, history.p + n, k1s[i]);
+          ghost_set(mid[i], kinds_ucnp, GHOST_NORMAL);
           stage_view(mid[i], seg[i], seg[i + 1], 1.0, STAGE_STORE_K, history.p + n, k2s[i]);
         }
         launch_mhd_ghost(seg[i + 1], kc, mesh, kinds_all, s1, s2, GHOST_NORMAL, stream);
@@ -2624,6 +2631,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
                                    stream);
           });
           timed("adj_faces", [&] { launch_adj_faces(mid[i], st, kc, mesh, cx, cy, lamW, stream); });
+          adj_ghost(mid[i], lamW, kinds_ucnp);             // the midpoint's open_ucnp ghost fill
           timed("adj_stage", [&] {
             launch_adj_stage_cells(Ln, k1s[i], hp, 0.5, Ln, st, kc, mesh, lamW, lamN, 0, lamN, 0, cx, cy, part2,
                                    stream);
@@ -2665,18 +2673,21 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
           launch_adj_rhs_cells(tape.W3, st, kc, mesh, lkf, 1.0 / 6.0, nullptr, lamW, 1, cx, cy, stream);
         });
         timed("adj_faces", [&] { launch_adj_faces(tape.W3, st, kc, mesh, cx, cy, lamW, stream); });
+        adj_ghost(tape.W3, lamW, kinds_ucnp);              // each stage state's open_ucnp ghost fill
         // W3 = F(Lm + h k3), k3 = k(W2): lambda_k3 = h mu3 + lkf/3
         timed("adj_stage", [&] {
           launch_adj_update(tape.Lm, tape.k3, hp, 1.0, kc, mesh, lamW, lamM, 0, lka, part2, stream);
           launch_adj_rhs_cells(tape.W2, st, kc, mesh, lkf, 1.0 / 3.0, lka, lamX, 1, cx, cy, stream);
         });
         timed("adj_faces", [&] { launch_adj_faces(tape.W2, st, kc, mesh, cx, cy, lamX, stream); });
+        adj_ghost(tape.W2, lamX, kinds_ucnp);
         // W2 = F(Lm + h/2 k2), k2 = k(W): lambda_k2 = h/2 mu2 + lkf/3
         timed("adj_stage", [&] {
           launch_adj_update(tape.Lm, tape.k2, hp, 0.5, kc, mesh, lamX, lamM, 0, lka, part3, stream);
           launch_adj_rhs_cells(tape.W, st, kc, mesh, lkf, 1.0 / 3.0, lka, lamW, 1, cx, cy, stream);
         });
         timed("adj_faces", [&] { launch_adj_faces(tape.W, st, kc, mesh, cx, cy, lamW, stream); });
+        adj_ghost(tape.W, lamW, kinds_ucnp);
         // W = F(A2 + h/2 k1), k1 = k(A2): lambda_k1 = h/2 mu1 + lkf/6
         timed("adj_stage", [&] {
           launch_adj_update(A2, tape.k1, hp, 0.5, kc, mesh, lamW, lamN, 1, lka, part4, stream);
@@ -2696,6 +2707,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
                                  stream);
         });
         timed("adj_faces", [&] { launch_adj_faces(tape.W, st, kc, mesh, cx, cy, lamW, stream); });
+        adj_ghost(tape.W, lamW, kinds_ucnp);
         timed("adj_stage", [&] {
           launch_adj_stage_cells(A2, tape.k1, hp, 0.5, A2, st, kc, mesh, lamW, lamN, 0, lamN, 0, cx, cy, part2,
                                  stream);

@@ -1,7 +1,7 @@
 """GPU discrete adjoint with non-periodic boundaries and modules, against central finite
 differences of the GPU forward.
 
-Covered: the ghost-fill reverse of fixed, reflect and open boundaries; thermal conduction (Euler,
+Covered: the ghost-fill reverse of fixed, reflect, open and open_ucnp boundaries; thermal conduction (Euler,
 RK2 and RK4 subcycles, no flux saturation), radiative losses and the parametrised heating profile of the
 gradient workload, whose coefficients get their own gradient (spruce_gpu_adjoint_params).  The loss
 is a seeded linear functional of the final evolved grids.  It is only piecewise smooth (limiter,
@@ -134,6 +134,10 @@ BOUNDARIES = [
     # RK4: stages 2-4 read the live grids whose ghost cells the intermediate fills rewrote
     ("fixed_reflect_rk4", ("periodic", "periodic", "fixed", "reflect"), "rk4", 10, 4),
     ("open_fixed_x_rk4", ("open", "fixed", "periodic", "periodic"), "rk4", 10, 4),
+    # open_ucnp: the final fill and every intermediate stage state's fill copy the first interior cell
+    ("ucnp_all_rk2", ("open_ucnp",) * 4, "rk2", 10, 4),
+    ("ucnp_x_fixed_reflect_rk4", ("open_ucnp", "open_ucnp", "fixed", "reflect"), "rk4", 8, 3),
+    ("ucnp_y_euler", ("periodic", "periodic", "open_ucnp", "open_ucnp"), "euler", 10, 5),
 ]
 
 
