@@ -180,46 +180,99 @@ __device__ __forceinline__ double temp_of(double te, double n, const MhdConst& K
   return smax(te * K.gm1 / (n * (2.0 * K_B)), K.tmin);
 }
 
+// The conduction reverses scatter dJ/dT onto the 3x3 neighbourhood of every cell.  A block of 256
+// threads (64 x 4, one cell each) owns a tile of 64 columns x 4 rows; the scatter goes into an LDS
+// accumulator over the tile plus a one-cell ring, which is then added to lam_T: plainly for the
+// cells whose whole neighbourhood lies in the tile (no other tile reaches them), atomically on the
+// tile's border and ring (about one global atomic per cell instead of nine).  Every cell's
+// contributions are the same; only their summation order differs from a scatter of global atomics.
+constexpr int TCR_COLS = 64, TCR_ROWS = 4, TCR_W = TCR_COLS + 2, TCR_N = (TCR_ROWS + 2) * TCR_W;
+
+__device__ __forceinline__ void tcr_zero(double* acc) {
+  for (int t = threadIdx.y * 64 + threadIdx.x; t < TCR_N; t += 256) acc[t] = 0.0;
+}
+// cell at tile row li, column lj: its neighbour (i-1+a, j-1+b) is accumulator entry (li+a, lj+b)
+__device__ __forceinline__ void tcr_add(double* acc, int li, int lj, const double gT[3][3], double ld) {
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) atomicAdd(&acc[(li + a) * TCR_W + lj + b], ld * gT[a][b]);
+}
+__device__ __forceinline__ void tcr_flush(const double* acc, const Mesh& M, int i0, int j0, double* lam_T) {
+  const int vr = min(TCR_ROWS, M.nx - i0), vc = min(TCR_COLS, M.ny - j0);
+  for (int t = threadIdx.y * 64 + threadIdx.x; t < TCR_N; t += 256) {
+    const double v = acc[t];
+    if (v == 0.0) continue;
+    const int la = t / TCR_W, lb = t % TCR_W;
+    int gi = i0 - 1 + la, gj = j0 - 1 + lb;
+    if (gi < 0 || gi >= M.nx) {
+      if (!M.x.per) continue;
+      gi = gi < 0 ? gi + M.nx : gi - M.nx;
+    }
+    if (gj < 0 || gj >= M.ny) {
+      if (!M.y.per) continue;
+      gj = gj < 0 ? gj + M.ny : gj - M.ny;
+    }
+    const size_t g = (size_t)gi * M.ny + gj;
+    if (la >= 2 && la <= vr - 1 && lb >= 2 && lb <= vc - 1) lam_T[g] += v;
+    else atomicAdd(&lam_T[g], v);
+  }
+}
+__device__ __forceinline__ void tcr_temps(const double* te, const double* n, const MhdConst& K, const Mesh& M, int i,
+                                          int j, double T[3][3]) {
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      const size_t q = (size_t)wrp(i - 1 + a, M.nx, M.x.per) * M.ny + wrp(j - 1 + b, M.ny, M.y.per);
+      T[a][b] = temp_of(te[q], n[q], K);
+    }
+}
+__device__ __forceinline__ void tcr_partial(double dot, double* partial, int accumulate) {
+  dot = block_sum(dot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const int b = blockIdx.y * gridDim.x + blockIdx.x;
+    partial[b] = accumulate ? partial[b] + dot : dot;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_adj_tc_sub(const double* te_s, const double* n, const double* bhx,
                                                     const double* bhy, const double* dbxx, const double* dbxy,
                                                     const double* dbyx, const double* dbyy, double h, double k0,
                                                     MhdConst K, Mesh M, double* lam_te, double* lam_T,
-                                                    double* lam_fz, double* partial) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+                                                    double* lam_fz, double* partial, int accumulate) {
+  __shared__ double acc[TCR_N];
+  const int i0 = blockIdx.y * TCR_ROWS, j0 = blockIdx.x * TCR_COLS;
+  tcr_zero(acc);
+  __syncthreads();
   double dot = 0.0;
-  if (i < M.nx && j < M.ny) {
+  do {
+    const int li = threadIdx.y, lj = threadIdx.x;
+    const int i = i0 + li, j = j0 + lj;
+    if (i >= M.nx || j >= M.ny) break;
     const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
     const double base = te_s[c];
     if (!inside(M, i, j)) {
       lam_te[c] *= (base + 0.0 < K.temin) ? 0.0 : 1.0;
-    } else {
-      size_t q[3][3];
-      double T[3][3];
-      for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) {
-          q[a][b] = (size_t)wrp(i - 1 + a, M.nx, M.x.per) * M.ny + wrp(j - 1 + b, M.ny, M.y.per);
-          T[a][b] = temp_of(te_s[q[a][b]], n[q[a][b]], K);
-        }
-      // the clamp of the update needs dE first; its reverse is linear in the seed, so it runs with
-      // a unit seed and is scaled afterwards
-      double gT[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-      double gfz[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      const double dE = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0,
-                                             1.0, gT, gfz);
-      const double v = base + 1.0 * (1.0 * h) * dE;
-      const double lv = (v < K.temin) ? 0.0 : lam_te[c];
-      lam_te[c] = lv;
-      dot = lv * dE;
-      const double ld = lv * h;
-      if (ld != 0.0) {
-        for (int a = 0; a < 3; ++a)
-          for (int b = 0; b < 3; ++b) atomicAdd(&lam_T[q[a][b]], ld * gT[a][b]);
-        for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += ld * gfz[k];
-      }
+      continue;
     }
-  }
-  dot = block_sum(dot);
-  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
+    double T[3][3];
+    tcr_temps(te_s, n, K, M, i, j, T);
+    // the clamp of the update needs dE first; its reverse is linear in the seed, so it runs with
+    // a unit seed and is scaled afterwards
+    double gT[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    double gfz[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double dE = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0, 1.0,
+                                           gT, gfz);
+    const double v = base + 1.0 * (1.0 * h) * dE;
+    const double lv = (v < K.temin) ? 0.0 : lam_te[c];
+    lam_te[c] = lv;
+    dot += lv * dE;
+    const double ld = lv * h;
+    if (ld != 0.0) {
+      tcr_add(acc, li, lj, gT, ld);
+      for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += ld * gfz[k];
+    }
+  } while (false);   // one cell per thread; `continue`/`break` leave the cell
+  __syncthreads();
+  tcr_flush(acc, M, i0, j0, lam_T);
+  tcr_partial(dot, partial, accumulate);
 }
 
 // Generalised subcycle stage reverse: the energy derivative dE is evaluated at the temperatures of
@@ -233,46 +286,47 @@ __global__ void __launch_bounds__(256) k_adj_tc_stage(const double* eval, const 
                                                       double k0, MhdConst K, Mesh M, const double* cbase,
                                                       const double* lam_out, double* lam_base, int assign_base,
                                                       const double* seed_add, double add_coef, double* lam_T,
-                                                      double* lam_fz, double* partial) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+                                                      double* lam_fz, double* partial, int accumulate) {
+  __shared__ double acc[TCR_N];
+  const int i0 = blockIdx.y * TCR_ROWS, j0 = blockIdx.x * TCR_COLS;
+  tcr_zero(acc);
+  __syncthreads();
   double dot = 0.0;
-  if (i < M.nx && j < M.ny) {
+  do {
+    const int li = threadIdx.y, lj = threadIdx.x;
+    const int i = i0 + li, j = j0 + lj;
+    if (i >= M.nx || j >= M.ny) break;
     const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
     if (!inside(M, i, j)) {
       if (cbase) {
         const double mu = (cbase[c] + 0.0 < K.temin) ? 0.0 : lam_out[c];
         lam_base[c] = assign_base ? mu : lam_base[c] + mu;
       }
-    } else {
-      size_t q[3][3];
-      double T[3][3];
-      for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) {
-          q[a][b] = (size_t)wrp(i - 1 + a, M.nx, M.x.per) * M.ny + wrp(j - 1 + b, M.ny, M.y.per);
-          T[a][b] = temp_of(eval[q[a][b]], n[q[a][b]], K);
-        }
-      double gT[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-      double gfz[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      const double dE = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0,
-                                             1.0, gT, gfz);
-      double ld = 0.0;
-      if (cbase) {
-        const double v = cbase[c] + 1.0 * (mult * h) * dE;
-        const double mu = (v < K.temin) ? 0.0 : lam_out[c];
-        lam_base[c] = assign_base ? mu : lam_base[c] + mu;
-        dot = mu * mult * dE;
-        ld = mu * (mult * h);
-      }
-      if (seed_add) ld += add_coef * seed_add[c];
-      if (ld != 0.0) {
-        for (int a = 0; a < 3; ++a)
-          for (int b = 0; b < 3; ++b) atomicAdd(&lam_T[q[a][b]], ld * gT[a][b]);
-        for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += ld * gfz[k];
-      }
+      continue;
     }
-  }
-  dot = block_sum(dot);
-  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
+    double T[3][3];
+    tcr_temps(eval, n, K, M, i, j, T);
+    double gT[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    double gfz[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double dE = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0, 1.0,
+                                           gT, gfz);
+    double ld = 0.0;
+    if (cbase) {
+      const double v = cbase[c] + 1.0 * (mult * h) * dE;
+      const double mu = (v < K.temin) ? 0.0 : lam_out[c];
+      lam_base[c] = assign_base ? mu : lam_base[c] + mu;
+      dot += mu * mult * dE;
+      ld = mu * (mult * h);
+    }
+    if (seed_add) ld += add_coef * seed_add[c];
+    if (ld != 0.0) {
+      tcr_add(acc, li, lj, gT, ld);
+      for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += ld * gfz[k];
+    }
+  } while (false);   // one cell per thread; `continue`/`break` leave the cell
+  __syncthreads();
+  tcr_flush(acc, M, i0, j0, lam_T);
+  tcr_partial(dot, partial, accumulate);
 }
 
 // Reverse of an RK4 subcycle's final combination out = max(te + (1.0*h*S)/6, te_min) with
@@ -285,42 +339,44 @@ __global__ void __launch_bounds__(256) k_adj_tc_final4(const double* te, const d
                                                        const double* dbxy, const double* dbyx, const double* dbyy,
                                                        double h, double k0, MhdConst K, Mesh M,
                                                        const double* lam_out, double* lam_base, double* lamS,
-                                                       double* lam_T, double* lam_fz, double* partial) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+                                                       double* lam_T, double* lam_fz, double* partial,
+                                                       int accumulate) {
+  __shared__ double acc[TCR_N];
+  const int i0 = blockIdx.y * TCR_ROWS, j0 = blockIdx.x * TCR_COLS;
+  tcr_zero(acc);
+  __syncthreads();
   double dot = 0.0;
-  if (i < M.nx && j < M.ny) {
+  do {
+    const int li = threadIdx.y, lj = threadIdx.x;
+    const int i = i0 + li, j = j0 + lj;
+    if (i >= M.nx || j >= M.ny) break;
     const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
     if (!inside(M, i, j)) {
       lam_base[c] = (te[c] + 0.0 < K.temin) ? 0.0 : lam_out[c];
       lamS[c] = 0.0;
-    } else {
-      size_t q[3][3];
-      double T[3][3];
-      for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) {
-          q[a][b] = (size_t)wrp(i - 1 + a, M.nx, M.x.per) * M.ny + wrp(j - 1 + b, M.ny, M.y.per);
-          T[a][b] = temp_of(i3[q[a][b]], n[q[a][b]], K);
-        }
-      double gT[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-      double gfz[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      const double dE = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0,
-                                             1.0, gT, gfz);
-      const double sum = k1[c] + 2.0 * k2[c] + 2.0 * k3[c] + dE;
-      const double v = te[c] + qdiv(1.0 * h * sum, 6.0, K.r6);
-      const double mu = (v < K.temin) ? 0.0 : lam_out[c];
-      lam_base[c] = mu;
-      const double ls = mu * h / 6.0;
-      lamS[c] = ls;
-      dot = mu * sum / 6.0;
-      if (ls != 0.0) {
-        for (int a = 0; a < 3; ++a)
-          for (int b = 0; b < 3; ++b) atomicAdd(&lam_T[q[a][b]], ls * gT[a][b]);
-        for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += ls * gfz[k];
-      }
+      continue;
     }
-  }
-  dot = block_sum(dot);
-  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
+    double T[3][3];
+    tcr_temps(i3, n, K, M, i, j, T);
+    double gT[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    double gfz[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double dE = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0, 1.0,
+                                           gT, gfz);
+    const double sum = k1[c] + 2.0 * k2[c] + 2.0 * k3[c] + dE;
+    const double v = te[c] + qdiv(1.0 * h * sum, 6.0, K.r6);
+    const double mu = (v < K.temin) ? 0.0 : lam_out[c];
+    lam_base[c] = mu;
+    const double ls = mu * h / 6.0;
+    lamS[c] = ls;
+    dot += mu * sum / 6.0;
+    if (ls != 0.0) {
+      tcr_add(acc, li, lj, gT, ls);
+      for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += ls * gfz[k];
+    }
+  } while (false);   // one cell per thread; `continue`/`break` leave the cell
+  __syncthreads();
+  tcr_flush(acc, M, i0, j0, lam_T);
+  tcr_partial(dot, partial, accumulate);
 }
 
 __global__ void __launch_bounds__(256) k_adj_tc_temp(const double* te_s, const double* n, MhdConst K, size_t N,
@@ -480,6 +536,7 @@ __global__ void __launch_bounds__(256) k_adj_accum(const double* partial, int n,
 }
 
 inline dim3 g2(const Mesh& m, dim3 b) { return dim3((m.ny + b.x - 1) / b.x, (m.nx + b.y - 1) / b.y); }
+inline dim3 tcr_grid(const Mesh& m) { return dim3((m.ny + TCR_COLS - 1) / TCR_COLS, (m.nx + TCR_ROWS - 1) / TCR_ROWS); }
 
 }  // namespace
 
@@ -508,28 +565,33 @@ void launch_adj_minmask(const double* rho_in, const double* te_base, const doubl
 void launch_adj_tc_sub(const double* te_s, const double* n, const double* bhx, const double* bhy,
                        const double* dbxx, const double* dbxy, const double* dbyx, const double* dbyy, double h,
                        double k0, MhdConst k, Mesh mesh, double* lam_te, double* lam_T, double* lam_fz,
-                       double* partial, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_tc_sub, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, te_s, n, bhx, bhy, dbxx, dbxy, dbyx,
-                     dbyy, h, k0, k, mesh, lam_te, lam_T, lam_fz, partial);
+                       double* partial, int accumulate, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_tc_sub, tcr_grid(mesh), dim3(64, 4), 0, s, te_s, n, bhx, bhy, dbxx, dbxy, dbyx, dbyy,
+                     h, k0, k, mesh, lam_te, lam_T, lam_fz, partial, accumulate);
+}
+
+int adj_tc_blocks(const Mesh& m) {
+  const dim3 g = tcr_grid(m);
+  return (int)(g.x * g.y);
 }
 
 void launch_adj_tc_stage(const double* eval, const double* n, const double* bhx, const double* bhy,
                          const double* dbxx, const double* dbxy, const double* dbyx, const double* dbyy, double h,
                          double mult, double k0, MhdConst k, Mesh mesh, const double* cbase, const double* lam_out,
                          double* lam_base, int assign_base, const double* seed_add, double add_coef, double* lam_T,
-                         double* lam_fz, double* partial, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_tc_stage, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, eval, n, bhx, bhy, dbxx, dbxy, dbyx,
-                     dbyy, h, mult, k0, k, mesh, cbase, lam_out, lam_base, assign_base, seed_add, add_coef, lam_T,
-                     lam_fz, partial);
+                         double* lam_fz, double* partial, int accumulate, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_tc_stage, tcr_grid(mesh), dim3(64, 4), 0, s, eval, n, bhx, bhy, dbxx, dbxy, dbyx, dbyy,
+                     h, mult, k0, k, mesh, cbase, lam_out, lam_base, assign_base, seed_add, add_coef, lam_T, lam_fz,
+                     partial, accumulate);
 }
 
 void launch_adj_tc_final4(const double* te, const double* k1, const double* k2, const double* k3, const double* i3,
                           const double* n, const double* bhx, const double* bhy, const double* dbxx,
                           const double* dbxy, const double* dbyx, const double* dbyy, double h, double k0, MhdConst k,
                           Mesh mesh, const double* lam_out, double* lam_base, double* lamS, double* lam_T,
-                          double* lam_fz, double* partial, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_tc_final4, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, te, k1, k2, k3, i3, n, bhx, bhy,
-                     dbxx, dbxy, dbyx, dbyy, h, k0, k, mesh, lam_out, lam_base, lamS, lam_T, lam_fz, partial);
+                          double* lam_fz, double* partial, int accumulate, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_tc_final4, tcr_grid(mesh), dim3(64, 4), 0, s, te, k1, k2, k3, i3, n, bhx, bhy, dbxx,
+                     dbxy, dbyx, dbyy, h, k0, k, mesh, lam_out, lam_base, lamS, lam_T, lam_fz, partial, accumulate);
 }
 
 void launch_adj_tc_temp(const double* te_s, const double* n, MhdConst k, size_t N, double* lam_T, double* lam_te,

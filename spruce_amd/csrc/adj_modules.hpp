@@ -28,11 +28,13 @@ void launch_adj_minmask(const double* rho_in, const double* te_base, const doubl
 // n, b_hat and b_hat derivatives (thermalconduction.cpp:94-97, 158-176): lam_te (in: dJ/dte_out,
 // out: the direct part of dJ/dte_s), lam_T += dJ/dT on the 3x3 neighbourhoods (atomics), lam_fz
 // (6*N: b_hat_x, b_hat_y, d b_hat_x/dx, /dy, d b_hat_y/dx, /dy) += their adjoints, partial[block]
-// = block sums of lam * dE (the adjoint of h).
+// = block sums of lam * dE (the adjoint of h), added to the entries already there when accumulate.
+// The conduction reverses run adj_tc_blocks() blocks (16 x 64 tiles).
+int adj_tc_blocks(const Mesh& m);
 void launch_adj_tc_sub(const double* te_s, const double* n, const double* bhx, const double* bhy,
                        const double* dbxx, const double* dbxy, const double* dbyx, const double* dbyy, double h,
                        double k0, MhdConst k, Mesh mesh, double* lam_te, double* lam_T, double* lam_fz,
-                       double* partial, hipStream_t s);
+                       double* partial, int accumulate, hipStream_t s);
 // A subcycle stage of the RK2/RK4 subcycle integrators reversed (thermalconduction.cpp:67-124):
 // dE is taken at the temperatures of `eval`; with cbase given, the stage output was
 // max(cbase + mask*(mult*h)*dE, te_min): lam_out goes through the clamp into lam_base (assigned
@@ -43,7 +45,7 @@ void launch_adj_tc_stage(const double* eval, const double* n, const double* bhx,
                          const double* dbxx, const double* dbxy, const double* dbyx, const double* dbyy, double h,
                          double mult, double k0, MhdConst k, Mesh mesh, const double* cbase, const double* lam_out,
                          double* lam_base, int assign_base, const double* seed_add, double add_coef, double* lam_T,
-                         double* lam_fz, double* partial, hipStream_t s);
+                         double* lam_fz, double* partial, int accumulate, hipStream_t s);
 // The RK4 subcycle's final combination max(te + (h*(k1 + 2k2 + 2k3 + k4))/6, te_min), k4 at the
 // temperatures of i3, reversed: lam_base = mu, lamS = mu*h/6 (k_j's seed is its weight times
 // lamS), k4's transpose applied, partial = block sums of mu*S/6.
@@ -51,7 +53,7 @@ void launch_adj_tc_final4(const double* te, const double* k1, const double* k2, 
                           const double* n, const double* bhx, const double* bhy, const double* dbxx,
                           const double* dbxy, const double* dbyx, const double* dbyy, double h, double k0, MhdConst k,
                           Mesh mesh, const double* lam_out, double* lam_base, double* lamS, double* lam_T,
-                          double* lam_fz, double* partial, hipStream_t s);
+                          double* lam_fz, double* partial, int accumulate, hipStream_t s);
 // T = max((gamma-1) te / (2 K_B n), T_min) reversed: lam_te += lam_T dT/dte, lam_rho += lam_T dT/dn / m
 // (n = rho / m), and lam_T is cleared for the next subcycle.
 void launch_adj_tc_temp(const double* te_s, const double* n, MhdConst k, size_t N, double* lam_T, double* lam_te,

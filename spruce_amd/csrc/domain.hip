@@ -2730,11 +2730,13 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
           const double hs = tc->h_last, k0 = tc->p.k0;
           double* lte = lam + 4 * N;
           // one stage reverse: dE at `ev`, clamp base te_s (or none), seed extra
+          // the step-size partials of every subcycle kernel are summed per block, reduced once
+          int acc = 0;
           auto stage_rev = [&](const double* ev, double mult, const double* cb, const double* lout, int assign,
                                const double* sadd, double coef) {
             launch_adj_tc_stage(ev, pre.n, bhx.p, bhy.p, tc->dbxx.p, tc->dbxy.p, tc->dbyx.p, tc->dbyy.p, hs, mult, k0,
-                                kc, mesh, cb, lout, lacc, assign, sadd, coef, lamT, lamfz, partm, stream);
-            launch_adj_accum(partm, nb, 1.0 / (double)ns, gh, stream);
+                                kc, mesh, cb, lout, lacc, assign, sadd, coef, lamT, lamfz, partm, acc, stream);
+            acc = 1;
           };
           // dJ/dT at the temperatures of `ev` onto dJ/dte (into `into`, cleared first unless it
           // accumulates) and dJ/drho
@@ -2749,8 +2751,8 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
               const double* te_s = q ? tc->rec_out(q - 1) : pre.te;
               if (tc->integ == "euler") {
                 launch_adj_tc_sub(te_s, pre.n, bhx.p, bhy.p, tc->dbxx.p, tc->dbxy.p, tc->dbyx.p, tc->dbyy.p, hs, k0,
-                                  kc, mesh, lte, lamT, lamfz, partm, stream);
-                launch_adj_accum(partm, nb, 1.0 / (double)ns, gh, stream);
+                                  kc, mesh, lte, lamT, lamfz, partm, acc, stream);
+                acc = 1;
                 launch_adj_tc_temp(te_s, pre.n, kc, N, lamT, lte, lam, stream);
                 continue;
               }
@@ -2763,8 +2765,8 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
                 const double *i1 = tc->rec_at(q, 0), *i2 = tc->rec_at(q, 1), *i3 = tc->rec_at(q, 2);
                 launch_adj_tc_final4(te_s, tc->rec_at(q, 3), tc->rec_at(q, 4), tc->rec_at(q, 5), i3, pre.n, bhx.p,
                                      bhy.p, tc->dbxx.p, tc->dbxy.p, tc->dbyx.p, tc->dbyy.p, hs, k0, kc, mesh, lte, lacc,
-                                     lamS, lamT, lamfz, partm, stream);
-                launch_adj_accum(partm, nb, 1.0 / (double)ns, gh, stream);
+                                     lamS, lamT, lamfz, partm, acc, stream);
+                acc = 1;
                 temp_rev(i3, lx, true);
                 stage_rev(i2, 1.0, te_s, lx, 0, lamS, 2.0);
                 temp_rev(i2, lx, true);
@@ -2775,6 +2777,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
               temp_rev(te_s, lacc, false);
               hip_check(hipMemcpyAsync(lte, lacc, N * 8, hipMemcpyDeviceToDevice, stream), "copy conduction adjoint");
             }
+            if (acc) launch_adj_accum(partm, adj_tc_blocks(mesh), 1.0 / (double)ns, gh, stream);
             launch_adj_bhat(Ln, st, mesh, lamfz, lam, stream);
           });
         } else {
