@@ -278,6 +278,13 @@ def bench_batch(args) -> dict:
     b.step(max(5, min(args.steps, 20)))
     kt = b.kernel_times()
     b.set_profiling(False)
+
+    # the same batch with all steps of the call in one cooperative launch
+    b.set_option("persistent", "true")
+    b.step(args.warmup)
+    t0 = time.perf_counter()
+    b.step(args.steps)
+    pwall = time.perf_counter() - t0
     b.close()
 
     # sequential: each domain advanced alone (one launch sequence per domain and step), with the
@@ -319,6 +326,8 @@ def bench_batch(args) -> dict:
                      "kernel_ms": stage_ms, "stage_bytes_per_cell": bpc},
         "sequential": {"value": seq_value, "steps": seq_steps, "by_shape": seq,
                        "batched_over_sequential": value / seq_value},
+        "persistent": {"value": cells * args.steps / pwall, "ms_per_step": pwall / args.steps * 1e3,
+                       "over_graph": (cells * args.steps / pwall) / value},
         "autotune_ms_per_step": tune_ms,
         "kernels_ms": {k: v[0] for k, v in kt.items()},
     }

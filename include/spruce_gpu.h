@@ -220,7 +220,10 @@ int spruce_gpu_adjoint_state(spruce_gpu_domain* d, const double* grad_evolved, d
 typedef struct spruce_gpu_batch spruce_gpu_batch;
 spruce_gpu_batch* spruce_gpu_batch_create(spruce_gpu_domain* const* domains, int32_t n);
 /* "graph" = "true" (default: one step is captured once into a HIP graph and replayed) | "false";
- * "block" = launch shape, as the domain option "block" ("<columns>x<rows>" or "strip<rows>"). */
+ * "block" = launch shape, as the domain option "block" ("<columns>x<rows>" or "strip<rows>");
+ * "persistent" = "true": every spruce_gpu_batch_step call is one cooperative launch that runs all
+ * its steps for every member, phase by phase with grid-wide barriers (strip stage; a cell shape is
+ * replaced by strips of 4 rows) | "false" (default). */
 int spruce_gpu_batch_set_option(spruce_gpu_batch* b, const char* key, const char* value);
 /* Times trial_steps steps for each candidate block shape, keeps the fastest and restores the state
  * of every member, so tuning does not advance the simulation.  chosen receives "<cols>x<rows>". */

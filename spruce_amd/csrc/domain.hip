@@ -3113,6 +3113,9 @@ int spruce_gpu_batch_set_option(spruce_gpu_batch* h, const char* key, const char
     b.use_graph = v == "true";
   } else if (k == "block") {
     b.rebuild(parse_tf_shape(v));
+  } else if (k == "persistent") {
+    require(v == "true" || v == "false", SPRUCE_E_ARG, "persistent must be true or false");
+    b.persistent = v == "true";
   } else {
     throw Error(SPRUCE_E_ARG, "unknown batch option " + k);
   }

@@ -397,8 +397,25 @@ void TfBatchRun::run(int nsteps, bool use_graph, std::vector<std::vector<double>
               uploaded[k].partial != d->partial.p;
   }
   if (realloc) rebuild(shape);
+  if (persistent && !(shape.R > 0 && shape.bx == 64 && shape.by == 4)) {
+    TfShape sh;                                      // the persistent form runs strips, 64x4 cell tiles
+    sh.R = shape.R > 0 ? shape.R : 4;
+    rebuild(sh);
+  }
   launch_tf_reset_counters(b, s);
-  if (use_graph && !clock.enabled) {
+  if (persistent) {
+    if (persistent_blocks == 0) persistent_blocks = tf_persistent_blocks();
+    // every phase loops over its work items with the grid's stride, so any grid size is correct;
+    // a multiple of 8 keeps each block's strip tiles on one XCD band
+    const int work = std::max(b.nstrip_tiles, std::max(b.ntiles, b.ndom));
+    int blocks = std::min(persistent_blocks, ((work + 7) / 8) * 8);
+    blocks = std::max(8, blocks - blocks % 8);
+    blocks = std::min(blocks, persistent_blocks);
+    const int ucnp_mask = any_ucnp ? 1 << SPRUCE_BC_OPEN_UCNP : 0, final_mask = (any_ucnp || any_other) ? 0x3F : 0;
+    clock.timed(s, "tf_persistent", [&] {
+      launch_tf_persistent(b, nsteps, integrator, ucnp_mask, final_mask, maxn, blocks, s);
+    });
+  } else if (use_graph && !clock.enabled) {
     if (!gexec) {
       hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeGlobal), "hipStreamBeginCapture");
       enqueue_step(s);

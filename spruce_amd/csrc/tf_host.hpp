@@ -71,6 +71,8 @@ class TfBatchRun {
   int integrator = 1;
   bool any_ucnp = false, any_other = false;
   bool use_graph = true;              // replay a captured step (the batch API default)
+  bool persistent = false;            // all steps of a call in one cooperative launch (strip shapes)
+  int persistent_blocks = 0;          // co-resident one-wavefront blocks of that launch
   bool broken = false;                // a member was destroyed while in the batch
   std::vector<TfDom> uploaded;        // host copy of d_doms, to notice reallocated member buffers
   KernelClock clock;                  // per-kernel event timing (profiling runs skip the graph)

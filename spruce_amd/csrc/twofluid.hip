@@ -11,6 +11,9 @@
 
 #include "arith.hpp"
 #include "strip.hpp"
+#include <hip/hip_cooperative_groups.h>
+
+#include "host_common.hpp"
 #include "twofluid.hpp"
 
 namespace spruce {
@@ -572,8 +575,7 @@ __device__ __forceinline__ void tf_derive(const double* q, const TfConst& K, dou
   dv[TFD_P1] = q[7] * K.gm1;
 }
 
-__device__ __forceinline__ void tf_strip_of(const TfBatch& b, int& d, int& col, int& strip) {
-  const int t = strip::xcd_tile(blockIdx.x, b.nstrip_tiles);
+__device__ __forceinline__ void tf_strip_of(const TfBatch& b, int t, int& d, int& col, int& strip) {
   d = b.strip_dom[t];
   const TfDom& D = b.doms[d];
   const int lt = t - D.strip_base;
@@ -581,11 +583,13 @@ __device__ __forceinline__ void tf_strip_of(const TfBatch& b, int& d, int& col, 
   strip = lt - col * D.strips_x;
 }
 
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
-k_tf_stage_strip(TfBatch b, int Ssel, int Bsel, int Tsel, double mult, int mode, int kslot) {
+// One strip tile t (a 64-lane wavefront); qs: the wavefront's 16 KiB of LDS for the rotating rows.
+using TfRows = double[4][TQ][64];
+__device__ __forceinline__ void tf_stage_strip_tile(const TfBatch& b, int t, int Ssel, int Bsel, int Tsel,
+                                                    double mult, int mode, int kslot, TfRows& qs) {
   using namespace strip;
   int d, col, sidx;
-  tf_strip_of(b, d, col, sidx);
+  tf_strip_of(b, t, d, col, sidx);
   const TfDom& D = b.doms[d];
   const TfSet& S = pick(D, Ssel);
   const TfSet& B = pick(D, Bsel);
@@ -613,7 +617,6 @@ k_tf_stage_strip(TfBatch b, int Ssel, int Bsel, int Tsel, double mult, int mode,
 
   // rotating rows r-1 .. r+2 of the transported quantities, one column per lane: kept in LDS
   // (16 KiB per wavefront) rather than registers, which leaves room for two wavefronts per SIMD
-  __shared__ double qs[4][TQ][64];
   auto Q = [&](int slot, int k) -> double& { return qs[slot][k][lane]; };
   auto load_q = [&](size_t c, bool ok, int slot) {
     double t[TQ];
@@ -831,6 +834,124 @@ k_tf_stage_strip(TfBatch b, int Ssel, int Bsel, int Tsel, double mult, int mode,
     step(std::integral_constant<int, 3>{}, r + 3);
   }
 }
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+k_tf_stage_strip(TfBatch b, int Ssel, int Bsel, int Tsel, double mult, int mode, int kslot) {
+  __shared__ TfRows qs;
+  tf_stage_strip_tile(b, strip::xcd_tile(blockIdx.x, b.nstrip_tiles), Ssel, Bsel, Tsel, mult, mode, kslot, qs);
+}
+
+// ---- persistent form: every phase of nsteps steps of the whole batch in one cooperative launch,
+// one 64-lane wavefront per block, phases separated by grid barriers --------------------------
+
+// dt minimum of 64x4 cell tile t (the batch's tile list) by one wavefront: lane = column, 4 rows
+__device__ void tf_dt_tile_wave(const TfBatch& b, int t) {
+  const int d = b.tile_dom[t];
+  const TfDom& D = b.doms[d];
+  const int lt = t - D.tile_base;
+  const int ty = lt / D.tiles_x, tx = lt - ty * D.tiles_x;
+  const int j = tx * 64 + (int)threadIdx.x;
+  double v = DBL_MAX;
+  for (int r = 0; r < 4; ++r) {
+    const int i = ty * 4 + r;
+    if (i < D.mesh.nx && j < D.mesh.ny && inside(D.mesh, i, j)) {
+      const double dt = tf_dt_cell(D, D.L, i, j);
+      v = (dt < v) ? dt : v;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(v, off, 64);
+    v = (o < v) ? o : v;
+  }
+  if (threadIdx.x == 0) D.partial[lt] = v;
+}
+
+// the step size of domain d from its tiles' minima (as k_tf_reduce), by one wavefront
+__device__ void tf_reduce_wave(const TfBatch& b, int d) {
+  const TfDom& D = b.doms[d];
+  double v = DBL_MAX;
+  const int n = D.tiles_x * D.tiles_y;
+  for (int k = threadIdx.x; k < n; k += 64) { const double o = D.partial[k]; v = (o < v) ? o : v; }
+  for (int off = 32; off > 0; off >>= 1) { const double o = __shfl_xor(v, off, 64); v = (o < v) ? o : v; }
+  if (threadIdx.x == 0) {
+    const double s = D.c.eps * v;
+    *D.step = s;
+    const int idx = *D.counter;
+    if (idx < D.hist_cap) D.history[idx] = (v < DBL_MAX) ? s : __builtin_nan("");
+    *D.counter = idx + 1;
+  }
+}
+
+// the copy of set `from` into `to` over 64x4 tile t, by one wavefront
+__device__ void tf_copy_tile_wave(const TfBatch& b, int t, int from, int to) {
+  const int d = b.tile_dom[t];
+  const TfDom& D = b.doms[d];
+  const int lt = t - D.tile_base;
+  const int ty = lt / D.tiles_x, tx = lt - ty * D.tiles_x;
+  const int j = tx * 64 + (int)threadIdx.x;
+  const TfSet& F = pick(D, from);
+  const TfSet& T = pick(D, to);
+  for (int r = 0; r < 4; ++r) {
+    const int i = ty * 4 + r;
+    if (i >= D.mesh.nx || j >= D.mesh.ny) continue;
+    const size_t c = (size_t)i * D.mesh.ny + j;
+#pragma unroll
+    for (int v = 0; v < TF_NE; ++v) T.v[v][c] = F.v[v][c];
+  }
+}
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2)))
+k_tf_persistent(TfBatch b, int nsteps, int integrator, int ucnp_mask, int final_mask, int maxn) {
+  __shared__ TfRows qs;
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const int G = (int)gridDim.x, bid = (int)blockIdx.x;
+  // block bid takes strip tiles bid, bid + G, ...: with G a multiple of 8 the block's XCD (bid % 8)
+  // is the XCD band of every tile it takes (strip::xcd_tile)
+  auto stage = [&](int S, int B, int T, double mult, int mode, int kslot) {
+    for (int t = bid; t < b.nstrip_tiles; t += G)
+      tf_stage_strip_tile(b, strip::xcd_tile(t, b.nstrip_tiles), S, B, T, mult, mode, kslot, qs);
+    grid.sync();
+  };
+  auto ghost = [&](int Gsel, int mask) {   // x sides, then y sides (launch_tf_ghost)
+    if (!mask) return;
+    for (int xs = 1; xs >= 0; --xs) {
+      for (int w = bid * 64 + (int)threadIdx.x; w < b.ndom * maxn; w += G * 64) {
+        const int d = w / maxn, t = w - d * maxn;
+        const TfDom& D = b.doms[d];
+        const TfSet& Gs = pick(D, Gsel);
+        for (int s = xs ? 0 : 2; s < (xs ? 2 : 4); ++s) {
+          const int kind = D.bc[s];
+          if (kind != 0 && kind != 4 && ((mask >> kind) & 1)) tf_ghost_side(D, Gs, s, kind, t);
+        }
+      }
+      grid.sync();
+    }
+  };
+  // the stages of TfBatchRun::enqueue_step (one call site, so the kernel holds one copy of the
+  // stage): Euler L->W; RK2 L->W, W->L; RK4 L->W, W->W2, W2->W, W->L (base L throughout)
+  const int nops = integrator == SPRUCE_EULER ? 1 : (integrator == SPRUCE_RK2 ? 2 : 4);
+  for (int n = 0; n < nsteps; ++n) {      // TfBatchRun::enqueue_step, phase by phase
+    for (int o = 0; o < nops; ++o) {
+      const bool last = o == nops - 1;
+      const int S = o == 0 ? TF_SEL_L : (integrator == SPRUCE_RK4 && o == 2 ? TF_SEL_W2 : TF_SEL_W);
+      const int T = integrator == SPRUCE_EULER ? TF_SEL_W
+                  : (last ? TF_SEL_L : (integrator == SPRUCE_RK4 && o == 1 ? TF_SEL_W2 : TF_SEL_W));
+      const double mult = (integrator == SPRUCE_EULER || o >= nops / 2) ? 1.0 : 0.5;
+      const int mode = integrator != SPRUCE_RK4 ? TF_STAGE_PLAIN : (last ? TF_STAGE_RK4_FINAL : TF_STAGE_STORE_K);
+      stage(S, TF_SEL_L, T, mult, mode, last ? 0 : o);
+      if (!last) ghost(T, ucnp_mask);
+    }
+    if (integrator == SPRUCE_EULER) {
+      for (int t = bid; t < b.ntiles; t += G) tf_copy_tile_wave(b, t, TF_SEL_W, TF_SEL_L);
+      grid.sync();
+    }
+    ghost(TF_SEL_L, final_mask);
+    for (int t = bid; t < b.ntiles; t += G) tf_dt_tile_wave(b, t);
+    grid.sync();
+    for (int d = bid; d < b.ndom; d += G) tf_reduce_wave(b, d);
+    grid.sync();
+  }
+}
 }  // namespace
 
 void launch_tf_stage(const TfBatch& b, int S, int B, int T, double mult, int mode, int kslot, hipStream_t s) {
@@ -862,6 +983,28 @@ void launch_tf_derived(const TfBatch& b, int d, int which, double* out, int nx, 
 void launch_tf_rhs(const TfBatch& b, int d, double* out, int nx, int ny, hipStream_t s) {
   dim3 blk(64, 4), g((ny + 63) / 64, (nx + 3) / 4);
   hipLaunchKernelGGL(k_tf_rhs, g, blk, 0, s, b, d, out);
+}
+
+int tf_persistent_blocks() {
+  int dev = 0, per_cu = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  hipDeviceProp_t prop;
+  hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+  hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_tf_persistent),
+                                                          64, 0),
+            "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+  return per_cu * prop.multiProcessorCount;
+}
+
+void launch_tf_persistent(const TfBatch& b, int nsteps, int integrator, int ucnp_mask, int final_mask, int maxn,
+                          int blocks, hipStream_t s) {
+  require(b.strip_rows > 0 && b.bx == 64 && b.by == 4, SPRUCE_E_STATE,
+          "the persistent batch step needs a strip launch shape");
+  require(blocks > 0, SPRUCE_E_STATE, "no co-resident blocks for the persistent batch step");
+  void* args[] = {(void*)&b, (void*)&nsteps, (void*)&integrator, (void*)&ucnp_mask, (void*)&final_mask, (void*)&maxn};
+  hip_check(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_tf_persistent), dim3(blocks), dim3(64), args, 0,
+                                       s),
+            "hipLaunchCooperativeKernel");
 }
 
 void launch_tf_reset_counters(const TfBatch& b, hipStream_t s) {

@@ -73,6 +73,13 @@ void launch_tf_populate(const TfBatch& b, const double* const* itemp, const doub
 // (14 grids, evolved order); nx, ny are that domain's sizes.
 void launch_tf_derived(const TfBatch& b, int d, int which, double* out, int nx, int ny, hipStream_t s);
 void launch_tf_rhs(const TfBatch& b, int d, double* out, int nx, int ny, hipStream_t s);
+// nsteps steps of the whole batch in one cooperative launch of `blocks` one-wavefront blocks (the
+// strip stage, ghost fills, dt minima and step sizes, separated by grid barriers); needs a strip
+// launch shape.  ucnp_mask / final_mask: the boundary-kind masks of the intermediate and final ghost
+// fills (0: none).  tf_persistent_blocks(): how many such blocks the GPU holds at once.
+int tf_persistent_blocks();
+void launch_tf_persistent(const TfBatch& b, int nsteps, int integrator, int ucnp_mask, int final_mask, int maxn,
+                          int blocks, hipStream_t s);
 // history[0] = current step size and counter = 1 for every domain (start of a run of steps).
 void launch_tf_reset_counters(const TfBatch& b, hipStream_t s);
 // set `to` = set `from` for every domain (the Euler update, which cannot write the set it reads).

@@ -229,3 +229,65 @@ def test_batch_of_64_expansions_matches_individual_runs():
         b.close()
         for d in doms:
             d.close()
+
+
+@pytest.mark.parametrize("integ,curl", [("rk2", False), ("rk4", True), ("euler", False)])
+def test_persistent_batch_matches_individual_runs(integ, curl):
+    """Every step of a call in one cooperative launch (grid barriers between the phases): bit for
+    bit the members stepped alone, with eager work between calls."""
+    print("persistent batch", integ, curl, flush=True)
+    specs = _members(integ, curl)
+    chunks = [3, 1, 4]
+    alone, dts_a = _run_alone(specs, chunks)
+    doms = [GpuDomain(s) for s in specs]
+    b = GpuBatch(doms, graph=False)
+    try:
+        b.set_option("persistent", "true")
+        dts_b = []
+        for c in chunks:
+            dts_b.append(b.step(c))
+            for d in doms:
+                d.grid("e_rho")
+        _check_same(doms, alone, np.concatenate(dts_b, axis=1), dts_a, "persistent batch %s" % integ)
+    finally:
+        b.close()
+        for d in doms:
+            d.close()
+
+
+def test_persistent_batch_mixed_boundaries_and_shapes():
+    """Every boundary kind beside each other; a strip shape is kept, a cell shape gives 4-row strips."""
+    print("persistent batch mixed", flush=True)
+    specs = [tf_spec(64, 80, 7, ("periodic", "periodic", "open", "open_ucnp"), "rk2", False),
+             tf_spec(56, 56, 8, ("reflect",) * 4, "rk2", False),
+             tf_spec(72, 48, 9, ("fixed",) * 4, "rk2", True),
+             tf_spec(40, 96, 10, ("open",) * 4, "rk2", False, open_boundary_strength=0.4)]
+    alone, dts_a = _run_alone(specs, [5])
+    for block in ("strip8", "32x8"):
+        doms = [GpuDomain(s) for s in specs]
+        b = GpuBatch(doms, graph=True, block=block)
+        try:
+            b.set_option("persistent", "true")
+            dts_b = b.step(5)
+            _check_same(doms, alone, dts_b, dts_a, "persistent block %s" % block)
+        finally:
+            b.close()
+            for d in doms:
+                d.close()
+
+
+def test_persistent_batch_of_64_expansions_matches_individual_runs():
+    print("persistent batch of 64", flush=True)
+    members = W.batch_2f_members(64)
+    specs = [spec_from_files(c, s) for s, c in members]
+    alone, dts_a = _run_alone(specs, [3])
+    doms = [GpuDomain(s) for s in specs]
+    b = GpuBatch(doms, graph=True, block="strip4")
+    try:
+        b.set_option("persistent", "true")
+        dts_b = b.step(3)
+        _check_same(doms, alone, dts_b, dts_a, "persistent 64-domain batch")
+    finally:
+        b.close()
+        for d in doms:
+            d.close()
