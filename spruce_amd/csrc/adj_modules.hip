@@ -6,6 +6,7 @@
 // reproducible to rounding.
 #include <cfloat>
 
+#include "adj_cell.hpp"
 #include "adj_modules.hpp"
 #include "arith.hpp"
 #include "dual.hpp"
@@ -441,7 +442,8 @@ __global__ void __launch_bounds__(256) k_adj_bhat_field(MhdSet L, MhdStatic st, 
 // ---- radiative losses --------------------------------------------------------------------------
 
 template <int K>
-__device__ Dn<K> rl_loss_dn(const Dn<K>& t, const Dn<K>& n, const RlParams& p) {
+__device__ Dn<K> rl_loss_dn(const Dn<K>& t, const Dn<K>& n, const Dn<K>& te_live, const Dn<K>& dt_live,
+                           const RlParams& p) {
   if (t.v < p.cut) return dn_const<K>(0.0);
   const double lt = log10(t.v);
   double chi;
@@ -455,11 +457,17 @@ __device__ Dn<K> rl_loss_dn(const Dn<K>& t, const Dn<K>& n, const RlParams& p) {
   else { chi = 1.96e-27; pw = dn_sqrt(t); }
   Dn<K> v = n * n * chi * pw;
   if (t.v < p.cut_plus_ramp) v = v * ((t - p.cut) / p.ramp);
+  // prevent_subcycling: the cap by the live thermal energy over the live dt grid (:148-152)
+  if (p.prevent && 0.1 * p.eps * (te_live.v / v.v) < p.pd_eps * dt_live.v)
+    v = 0.1 * p.eps * te_live / (p.pd_eps * dt_live);
   return v;
 }
 
-__global__ void __launch_bounds__(256) k_adj_rl(MhdSet A, RlParams p, MhdConst K, Mesh M, int nsub, double h,
-                                                int integ, double* lam, double* partial) {
+// The subcycles of one cell in a 4-component forward dual: the cell's thermal energy (which is also
+// the live energy prevent_subcycling reads), n, the subcycle step and, with prevent_subcycling,
+// the cell's live dt, whose own gradient (dt_cell_dual, of the input state A) carries that part on.
+__global__ void __launch_bounds__(256) k_adj_rl(MhdSet A, MhdStatic st, RlParams p, MhdConst K, Mesh M, int nsub,
+                                                double h, int integ, double* lam, double* partial) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
   double dot = 0.0;
   if (i < M.nx && j < M.ny) {
@@ -473,27 +481,34 @@ __global__ void __launch_bounds__(256) k_adj_rl(MhdSet A, RlParams p, MhdConst K
     if (!inside(M, i, j)) {
       lam[4 * N + c] *= (te0 < K.temin) ? 0.0 : 1.0;
     } else {
-      const Dn<3> nn = dn_var<3>(A.n[c], 1), hv = dn_var<3>(h, 2);
-      Dn<3> te = dn_var<3>(te0, 0);
-      const Dn<3> hh = 1.0 * hv, half = 1.0 * (0.5 * hv);
-      auto loss = [&](const Dn<3>& e) {
-        const Dn<3> T = dn_max(e * K.gm1 / (nn * (2.0 * K_B)), K.tmin);
-        return rl_loss_dn(T, nn, p);
+      const Dn<4> nn = dn_var<4>(A.n[c], 1), hv = dn_var<4>(h, 2);
+      Dn<4> te = dn_var<4>(te0, 0);
+      const Dn<4> te_live = te;
+      Dual8 dtd = dconst(0.0);
+      Dn<4> dtl = dn_const<4>(0.0);
+      if (p.prevent) {
+        dtd = dt_cell_dual(A, st, K, M, i, j);
+        dtl = dn_var<4>(dtd.v, 3);
+      }
+      const Dn<4> hh = 1.0 * hv, half = 1.0 * (0.5 * hv);
+      auto loss = [&](const Dn<4>& e) {
+        const Dn<4> T = dn_max(e * K.gm1 / (nn * (2.0 * K_B)), K.tmin);
+        return rl_loss_dn(T, nn, te_live, dtl, p);
       };
       for (int s = 0; s < nsub; ++s) {
         if (integ == 0) {
           te = dn_max(te - hh * loss(te), K.temin);
         } else if (integ == 1) {
-          const Dn<3> hs = dn_max(te - half * loss(te), K.temin);
+          const Dn<4> hs = dn_max(te - half * loss(te), K.temin);
           te = dn_max(te - hh * loss(hs), K.temin);
         } else {
-          const Dn<3> k1 = -loss(te);
-          Dn<3> im = dn_max(te + half * k1, K.temin);
-          const Dn<3> k2 = -loss(im);
+          const Dn<4> k1 = -loss(te);
+          Dn<4> im = dn_max(te + half * k1, K.temin);
+          const Dn<4> k2 = -loss(im);
           im = dn_max(te + half * k2, K.temin);
-          const Dn<3> k3 = -loss(im);
+          const Dn<4> k3 = -loss(im);
           im = dn_max(te + hh * k3, K.temin);
-          const Dn<3> k4 = -loss(im);
+          const Dn<4> k4 = -loss(im);
           te = dn_max(te + (hh * (k1 + 2.0 * k2 + 2.0 * k3 + k4)) / 6.0, K.temin);
         }
       }
@@ -501,6 +516,11 @@ __global__ void __launch_bounds__(256) k_adj_rl(MhdSet A, RlParams p, MhdConst K
       lam[4 * N + c] = lo * te.d[0];
       lrho += lo * te.d[1] * K.rm;
       dot = lo * te.d[2];
+      if (p.prevent) {
+        const double ldt = lo * te.d[3];
+        lrho += ldt * dtd.d[AV_RHO];
+        for (int k = 1; k < 8; ++k) lam[(size_t)k * N + c] += ldt * dtd.d[k];
+      }
     }
     lam[c] = lrho;
   }
@@ -606,9 +626,10 @@ void launch_adj_bhat(MhdSet L, MhdStatic st, Mesh mesh, double* lam_fz, double* 
                      lam);
 }
 
-void launch_adj_rl(MhdSet A, RlParams p, MhdConst k, Mesh mesh, int nsub, double h, int integ, double* lam,
-                   double* partial, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_rl, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, A, p, k, mesh, nsub, h, integ, lam, partial);
+void launch_adj_rl(MhdSet A, MhdStatic st, RlParams p, MhdConst k, Mesh mesh, int nsub, double h, int integ,
+                   double* lam, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_rl, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, A, st, p, k, mesh, nsub, h, integ, lam,
+                     partial);
 }
 
 void launch_adj_heat_params(const double* lam_te, const double* basis, int K, const double* step, Mesh mesh,

@@ -62,10 +62,10 @@ void launch_adj_tc_temp(const double* te_s, const double* n, MhdConst k, size_t 
 // reversed onto bi_x, bi_y, bi_z of the state L the conduction step froze them from.
 void launch_adj_bhat(MhdSet L, MhdStatic st, Mesh mesh, double* lam_fz, double* lam, hipStream_t s);
 // Reverse of RadiativeLosses::iterateModule on the input state A (radiativelosses.cpp:84-140,
-// without prevent_subcycling) fused with the live propagate's minimum/derived reverse: lam updated
-// in place, partial[block] = block sums of dJ/dh.
-void launch_adj_rl(MhdSet A, RlParams p, MhdConst k, Mesh mesh, int nsub, double h, int integ, double* lam,
-                   double* partial, hipStream_t s);
+// with prevent_subcycling through the live dt of A, st its static fields) fused with the live
+// propagate's minimum/derived reverse: lam updated in place, partial[block] = block sums of dJ/dh.
+void launch_adj_rl(MhdSet A, MhdStatic st, RlParams p, MhdConst k, Mesh mesh, int nsub, double h, int integ,
+                   double* lam, double* partial, hipStream_t s);
 // dJ/dtheta_k += (*step) * sum over interior cells of lam_te * phi_k(y) for the parametrised heating
 // profile (basis: K x ny values of phi_k at row j); colsum is ny doubles of scratch.
 void launch_adj_heat_params(const double* lam_te, const double* basis, int K, const double* step, Mesh mesh,

@@ -2459,8 +2459,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
               "the adjoint covers thermal conduction without flux saturation");
       it_mods.push_back((int)k);
     } else if (auto* rl = dynamic_cast<RlModule*>(m)) {
-      require(!rl->p.prevent && !rl->inactive, SPRUCE_E_UNSUPPORTED,
-              "the adjoint covers radiative losses without prevent_subcycling");
+      require(!rl->inactive, SPRUCE_E_UNSUPPORTED, "the adjoint covers radiative losses that are applied");
       it_mods.push_back((int)k);
     } else if (dynamic_cast<ParamHeatModule*>(m) || dynamic_cast<HeatModule*>(m)) {
       post_mods.push_back((int)k);
@@ -2784,7 +2783,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
           auto* rl = static_cast<RlModule*>(m);
           const int ig = rl->integ == "euler" ? 0 : (rl->integ == "rk2" ? 1 : 2);
           timed("adj_rl", [&] {
-            launch_adj_rl(pre, rl->p, kc, mesh, rl->nsub, rl->h_last, ig, lam, partm, stream);
+            launch_adj_rl(pre, st, rl->p, kc, mesh, rl->nsub, rl->h_last, ig, lam, partm, stream);
             if (rl->nsub > 0) launch_adj_accum(partm, nb, 1.0 / (double)rl->nsub, gh, stream);
           });
         }

@@ -2,7 +2,8 @@
 differences of the GPU forward.
 
 Covered: the ghost-fill reverse of fixed, reflect, open and open_ucnp boundaries; thermal conduction (Euler,
-RK2 and RK4 subcycles, no flux saturation), radiative losses and the parametrised heating profile of the
+RK2 and RK4 subcycles, no flux saturation), radiative losses (with prevent_subcycling too) and the
+parametrised heating profile of the
 gradient workload, whose coefficients get their own gradient (spruce_gpu_adjoint_params).  The loss
 is a seeded linear functional of the final evolved grids.  It is only piecewise smooth (limiter,
 min/max and floor selections, the argmin of the step size, the integer subcycle counts), so every
@@ -37,9 +38,9 @@ def TCI(integ):
 TC = TCI("euler")
 
 
-def RL(integ="euler"):
+def RL(integ="euler", prevent=False):
     return ("radiative_losses", [("cutoff_ramp", "2e4"), ("cutoff_temp", "3e4"), ("epsilon", "0.5"),
-                                 ("time_integrator", integ)])
+                                 ("time_integrator", integ)] + ([("prevent_subcycling", "true")] if prevent else []))
 
 
 def HEAT(theta):
@@ -191,6 +192,12 @@ MODULE_CASES = [
     ("tc2_rl_heat_rk2", ("periodic", "periodic", "fixed", "open"), "rk2", [TCI("rk2"), RL(), HEAT(THETA[:8])], 6, 3),
     ("tc4_heat_euler", ("periodic", "periodic", "reflect", "fixed"), "euler", [TCI("rk4"), HEAT(THETA[:8])], 6, 3),
     ("tc4_rl4_heat_rk4", ("periodic", "periodic", "fixed", "open"), "rk4", [TCI("rk4"), RL("rk4"), HEAT(THETA[:8])],
+     4, 2),
+    # prevent_subcycling caps the losses of 65 of the 720 interior cells at step 0 by the live energy
+    # over the live dt grid, whose gradient then enters
+    ("tc_rlprev_heat_rk2", ("periodic", "periodic", "fixed", "open"), "rk2",
+     [TC, RL(prevent=True), HEAT(THETA[:8])], 6, 3),
+    ("rlprev2_heat_rk4", ("periodic", "periodic", "reflect", "fixed"), "rk4", [RL("rk2", prevent=True), HEAT(THETA[:8])],
      4, 2),
 ]
 
