@@ -10,6 +10,7 @@
 #include "adj_modules.hpp"
 #include "arith.hpp"
 #include "dual.hpp"
+#include "sources.hpp"
 #include "tc_cell.hpp"
 
 namespace spruce {
@@ -528,6 +529,59 @@ __global__ void __launch_bounds__(256) k_adj_rl(MhdSet A, MhdStatic st, RlParams
   if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
 }
 
+// ---- source-term modules after the integrator ------------------------------------------------
+
+// Reverse of one source module's post-integrator update and the live propagate's floors
+// (sources.hip k_source; localizedheating.cpp:52-66, massinjection.cpp:49-53,
+// momentuminjection.cpp:70-76, ambientheatingsink.cpp:35-37), P the state before it.  The added
+// term is mask * ((h * g) * t) [* m_i | * rho] for ADD / ADD_SCALED / ADD_WEIGHTED (the last on
+// both momenta, t and t2), or -(*step) * t for SUB; g depends on the time at the step (dg/dtime =
+// gp).  lam is updated in place; ph / pt: block sums of dJ/dh and dJ/dtime.
+__global__ void __launch_bounds__(256) k_adj_source(MhdSet P, int mode, const double* t, const double* t2,
+                                                    const double* step, double g, double gp, MhdConst K, Mesh M,
+                                                    double* lam, double* ph, double* pt) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  double dh = 0.0, dtm = 0.0;
+  if (i < M.nx && j < M.ny) {
+    const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+    const double mask = inside(M, i, j) ? 1.0 : 0.0;
+    const double h = *step;
+    double rho_in = P.rho[c], te_in = P.te[c];
+    if (mode == SRC_ADD) te_in = P.te[c] + mask * ((h * g) * t[c]);
+    else if (mode == SRC_ADD_SCALED) rho_in = P.rho[c] + mask * (((h * g) * t[c]) * K.m);
+    else if (mode == SRC_SUB) te_in = P.te[c] - h * t[c];
+    const double x = rho_in / K.m;
+    const double rho1 = smax(x, K.nmin) * K.m;
+    const double drho = (x < K.nmin || rho1 / K.m < K.nmin) ? 0.0 : 1.0;
+    const double dte = te_in < K.temin ? 0.0 : 1.0;
+    const double lr = lam[c] * drho, lt = lam[4 * N + c] * dte;
+    lam[c] = lr;
+    lam[4 * N + c] = lt;
+    if (mode == SRC_ADD) {
+      dh = lt * mask * (g * t[c]);
+      dtm = lt * mask * ((h * gp) * t[c]);
+    } else if (mode == SRC_ADD_SCALED) {
+      dh = lr * mask * ((g * t[c]) * K.m);
+      dtm = lr * mask * (((h * gp) * t[c]) * K.m);
+    } else if (mode == SRC_ADD_WEIGHTED) {
+      const double lmx = lam[N + c], lmy = lam[2 * N + c], rho = P.rho[c];
+      const double sx = lmx * t[c] + lmy * t2[c];
+      lam[c] = lr + mask * ((h * g) * sx);
+      dh = mask * (g * sx) * rho;
+      dtm = mask * ((h * gp) * sx) * rho;
+    } else {
+      dh = -lt * t[c];
+    }
+  }
+  dh = block_sum(dh);
+  dtm = block_sum(dtm);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const int b = blockIdx.y * gridDim.x + blockIdx.x;
+    ph[b] = dh;
+    pt[b] = dtm;
+  }
+}
+
 // ---- parametrised heating ----------------------------------------------------------------------
 
 __global__ void __launch_bounds__(256) k_adj_colsum(const double* lam_te, Mesh M, double* colsum) {
@@ -630,6 +684,12 @@ void launch_adj_rl(MhdSet A, MhdStatic st, RlParams p, MhdConst k, Mesh mesh, in
                    double* lam, double* partial, hipStream_t s) {
   hipLaunchKernelGGL(k_adj_rl, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, A, st, p, k, mesh, nsub, h, integ, lam,
                      partial);
+}
+
+void launch_adj_source(MhdSet P, int mode, const double* t, const double* t2, const double* step, double g, double gp,
+                       MhdConst k, Mesh mesh, double* lam, double* ph, double* pt, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_source, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, P, mode, t, t2, step, g, gp, k, mesh, lam,
+                     ph, pt);
 }
 
 void launch_adj_heat_params(const double* lam_te, const double* basis, int K, const double* step, Mesh mesh,

@@ -66,6 +66,13 @@ void launch_adj_bhat(MhdSet L, MhdStatic st, Mesh mesh, double* lam_fz, double* 
 // propagate's minimum/derived reverse: lam updated in place, partial[block] = block sums of dJ/dh.
 void launch_adj_rl(MhdSet A, MhdStatic st, RlParams p, MhdConst k, Mesh mesh, int nsub, double h, int integ,
                    double* lam, double* partial, hipStream_t s);
+// Reverse of a source module's update after the integrator and of the propagate that follows it,
+// from the state P before it (modes of launch_source, sources.hpp: ADD te, ADD_SCALED rho by m_i,
+// ADD_WEIGHTED both momenta by rho with templates t, t2, SUB te by (*step) * t): the added term is
+// mask * ((h * g) * t) ... with h = *step and g a factor of the time at the step (dg/dtime = gp).
+// lam updated in place; ph[block] / pt[block]: block sums of dJ/dh and dJ/dtime.
+void launch_adj_source(MhdSet P, int mode, const double* t, const double* t2, const double* step, double g, double gp,
+                       MhdConst k, Mesh mesh, double* lam, double* ph, double* pt, hipStream_t s);
 // dJ/dtheta_k += (*step) * sum over interior cells of lam_te * phi_k(y) for the parametrised heating
 // profile (basis: K x ny values of phi_k at row j); colsum is ny doubles of scratch.
 void launch_adj_heat_params(const double* lam_te, const double* basis, int K, const double* step, Mesh mesh,

@@ -1684,6 +1684,15 @@ static void upload_vec(DBuf& b, const std::vector<double>& v) {
   hip_check(hipMemcpy(b.p, v.data(), v.size() * 8, hipMemcpyHostToDevice), "upload template");
 }
 
+// What the adjoint needs of one source module's post-integrator update at simulation time `now`:
+// the launch_source mode and templates, and the factor g(now) of the step size in its amplitude
+// with its time derivative gp.
+struct SourceRev {
+  int mode = SRC_ADD;
+  const double *t = nullptr, *t2 = nullptr;
+  double g = 1.0, gp = 0.0;
+};
+
 // Base of the modules that act only after the integrator, at a simulation time window.
 struct PostSourceModule : GpuModule {
   bool needs_pre_scalars() const override { return false; }
@@ -1691,6 +1700,8 @@ struct PostSourceModule : GpuModule {
   void pre_launch(Domain&, double*) override {}
   void pre_finish(double, const double*) override {}
   void iterate(Domain&, double) override {}
+  // false when post() at `now` leaves the state alone (outside the module's time window)
+  virtual bool source_rev(double now, SourceRev& r) const { (void)now; (void)r; return false; }
 };
 
 // LocalizedHeating (localizedheating.cpp)
@@ -1721,14 +1732,31 @@ struct LocalHeatModule : PostSourceModule {
             "Localized Heating MS electron heating fraction must be between 0 and 1");
     upload_vec(tmpl, injection_template(d, rate, sx, sy, cx, cy));
   }
+  // the ramp (:56-59) and its derivative in time
+  double ramp_factor(double now, double* deriv) const {
+    const double t = now - start;
+    double rf = 1.0, dr = 0.0;
+    if (t < ramp && t <= 0.5 * dur) {
+      rf = t / ramp;
+      dr = 1.0 / ramp;
+    } else if (t > dur - ramp && t > 0.5 * dur) {
+      rf = (dur - t) / ramp;
+      dr = -1.0 / ramp;
+    }
+    if (deriv) *deriv = dr;
+    return rf;
+  }
+  bool source_rev(double now, SourceRev& r) const override {
+    if (now < start || now > start + dur) return false;
+    r.mode = SRC_ADD;
+    r.t = tmpl.p;
+    r.g = ramp_factor(now, &r.gp);
+    return true;
+  }
   void post(Domain& d) override {  // :52-66
     const double now = d.tstep;
     if (now < start || now > start + dur) return;
-    const double t = now - start;
-    double rf = 1.0;
-    if (t < ramp && t <= 0.5 * dur) rf = t / ramp;
-    else if (t > dur - ramp && t > 0.5 * dur) rf = (dur - t) / ramp;
-    const double a = d.host_dt * rf;
+    const double a = d.host_dt * ramp_factor(now, nullptr);
     d.timed("source", [&] { launch_source(d.L.te.p, tmpl.p, nullptr, a, 0.0, nullptr, SRC_ADD, d.mesh, d.stream); });
     d.propagate_live();
     if (d.multispecies_on) {   // (:63-65): the template without the ramp factor
@@ -1757,6 +1785,12 @@ struct MassInjModule : PostSourceModule {
   }
   bool needs_host_dt() const override { return true; }
   void setup(Domain& d) override { upload_vec(tmpl, injection_template(d, rate, sx, sy, cx, cy)); }  // :29-48
+  bool source_rev(double now, SourceRev& r) const override {
+    if (now < start || now > start + dur) return false;
+    r.mode = SRC_ADD_SCALED;
+    r.t = tmpl.p;
+    return true;
+  }
   void post(Domain& d) override {  // :49-53
     if (d.tstep < start || d.tstep > start + dur) return;
     d.timed("source", [&] {
@@ -1815,6 +1849,15 @@ struct MomInjModule : PostSourceModule {
       upload_vec(tmpl[c], t);
     }
   }
+  bool source_rev(double now, SourceRev& r) const override {
+    if (now < start || now > start + dur) return false;
+    r.mode = SRC_ADD_WEIGHTED;
+    r.t = tmpl[0].p;
+    r.t2 = tmpl[1].p;
+    r.g = (osc ? std::sin(2.0 * PI * (now - start) / period) : 1.0) * amax;   // as post() forms it
+    r.gp = osc ? amax * (2.0 * PI / period) * std::cos(2.0 * PI * (now - start) / period) : 0.0;
+    return true;
+  }
   void post(Domain& d) override {  // :70-76
     if (d.tstep < start || d.tstep > start + dur) return;
     const double of = osc ? std::sin(2.0 * PI * (d.tstep - start) / period) : 1.0;
@@ -1866,6 +1909,11 @@ struct HeatSinkModule : PostSourceModule {
     require(msf >= 0.0 && msf <= 1.0, SPRUCE_E_ARG,
             "Ambient Heating Sink MS electron heating fraction must be between 0 and 1");
     upload_vec(red, r);
+  }
+  bool source_rev(double, SourceRev& r) const override {
+    r.mode = SRC_SUB;
+    r.t = red.p;
+    return true;
   }
   void post(Domain& d) override {  // :35-37
     d.timed("source", [&] { launch_source(d.L.te.p, red.p, nullptr, 0.0, 0.0, d.step.p, SRC_SUB, d.mesh, d.stream); });
@@ -2448,8 +2496,8 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   require(!((bc[0] == SPRUCE_BC_OPEN || bc[1] == SPRUCE_BC_OPEN) && bc[2] != SPRUCE_BC_PERIODIC) &&
               !((bc[2] == SPRUCE_BC_OPEN || bc[3] == SPRUCE_BC_OPEN) && bc[0] != SPRUCE_BC_PERIODIC),
           SPRUCE_E_UNSUPPORTED, "the adjoint of an open boundary needs the other axis periodic");
-  // modules: conduction (euler subcycles, no saturation) and radiative losses (no
-  // prevent_subcycling) act in iterateModule; ambient and parametrised heating after the integrator
+  // modules: conduction (no saturation) and radiative losses act in iterateModule; ambient and
+  // parametrised heating and the source modules after the integrator
   std::vector<int> it_mods, post_mods;
   rec_post.assign(mods.size(), false);
   for (size_t k = 0; k < mods.size(); ++k) {
@@ -2461,7 +2509,9 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
     } else if (auto* rl = dynamic_cast<RlModule*>(m)) {
       require(!rl->inactive, SPRUCE_E_UNSUPPORTED, "the adjoint covers radiative losses that are applied");
       it_mods.push_back((int)k);
-    } else if (dynamic_cast<ParamHeatModule*>(m) || dynamic_cast<HeatModule*>(m)) {
+    } else if (dynamic_cast<ParamHeatModule*>(m) || dynamic_cast<HeatModule*>(m) ||
+               dynamic_cast<LocalHeatModule*>(m) || dynamic_cast<MassInjModule*>(m) ||
+               dynamic_cast<MomInjModule*>(m) || dynamic_cast<HeatSinkModule*>(m)) {
       post_mods.push_back((int)k);
       rec_post[k] = true;
     } else {
@@ -2487,7 +2537,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   const size_t SET = 10 * N, EV = 8 * N, CO = (size_t)ADJ_COEFS * N;
   const size_t seg_store = replay ? (size_t)(nit + npost + 2 + (rk4 ? 3 : 0)) * SET + (rk4 ? 4 : 2) * EV
                                   : (rk2 ? (size_t)sl * SET : 0) + (size_t)sl * EV * (rk2 ? 2 : 1);
-  const size_t modscratch = replay ? 10 * N + (size_t)ny + (size_t)nb + 1 : 0;
+  const size_t modscratch = replay ? 10 * N + (size_t)ny + 2 * (size_t)nb + 2 : 0;
   const size_t rk4scratch = rk4 ? 4 * EV + 2 * (size_t)nb : 0;
   const size_t total = (size_t)nseg * SET + (size_t)(sl + 1) * SET + seg_store + 3 * EV + 2 * CO + 4 * (size_t)nb +
                        modscratch + rk4scratch;
@@ -2535,6 +2585,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   double* pmin = take(nb);
   int* pidx = reinterpret_cast<int*>(take(nb));
   double *lamT = nullptr, *lamfz = nullptr, *colsum = nullptr, *partm = nullptr, *gh = nullptr;
+  double *partt = nullptr, *gtime = nullptr;   // dJ/d(simulation time) of the time-dependent sources
   double *lacc = nullptr, *lx = nullptr, *lamS = nullptr;   // conduction RK2/RK4 subcycle reverse
   if (replay) {
     lamT = take(N);
@@ -2545,6 +2596,8 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
     colsum = take(ny);
     partm = take(nb);
     gh = take(1);
+    partt = take(nb);
+    gtime = take(1);
   }
   // RK4: the adjoint of the stages' common base, the final combination's h*mu, one stage's
   // mult*h*mu, a second stage adjoint, and the partials of the two middle stages
@@ -2590,6 +2643,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   for (auto& m : mods)
     if (auto* ph = dynamic_cast<ParamHeatModule*>(m.get())) launch_fill(ph->g_theta.p, 0.0, ph->K, stream);
   if (lamT) launch_fill(lamT, 0.0, N, stream);
+  if (gtime) launch_fill(gtime, 0.0, 1, stream);
   hip_check(hipMemcpyAsync(lam, seed, 8 * N * 8, hipMemcpyHostToDevice, stream), "upload seed");
   const MhdStatic st = stat();
   for (int sg = nseg - 1; sg >= 0; --sg) {
@@ -2649,13 +2703,26 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
       }
       // with modules: replay the step keeping its intermediate states, then reverse it
       replay_step(n, Ln, tape, times);
-      launch_fill(gh, 0.0, 1, stream);
-      for (int p = npost - 1; p >= 0; --p) {              // post-iterate modules (heating)
+      // dJ/dh of this step starts with dJ/d(time) of every later step: the simulation time a
+      // time-dependent source reads at step m is the initial time plus the step sizes before m
+      hip_check(hipMemcpyAsync(gh, gtime, sizeof(double), hipMemcpyDeviceToDevice, stream), "copy time adjoint");
+      for (int p = npost - 1; p >= 0; --p) {              // post-iterate modules (heating, sources)
         const MhdSet& post = tape.after_post[p];
         const MhdSet& pre = p ? tape.after_post[p - 1] : tape.A3;
         GpuModule* m = mods[post_mods[p]].get();
-        adj_ghost(post, lam);
         auto* ph = dynamic_cast<ParamHeatModule*>(m);
+        if (!ph && !dynamic_cast<HeatModule*>(m)) {
+          SourceRev r;
+          if (!static_cast<PostSourceModule*>(m)->source_rev(times[n], r)) continue;   // the module did nothing
+          adj_ghost(post, lam);
+          timed("adj_source", [&] {
+            launch_adj_source(pre, r.mode, r.t, r.t2, hp, r.g, r.gp, kc, mesh, lam, partm, partt, stream);
+            launch_adj_accum(partm, nb, 1.0, gh, stream);
+            if (r.gp != 0.0) launch_adj_accum(partt, nb, 1.0, gtime, stream);
+          });
+          continue;
+        }
+        adj_ghost(post, lam);
         const double* heat = ph ? ph->heating.p : static_cast<HeatModule*>(m)->heating.p;
         timed("adj_heating", [&] {
           launch_adj_minmask(pre.rho, pre.te, heat, hp, kc, mesh, lam, partm, stream);

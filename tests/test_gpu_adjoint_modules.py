@@ -2,9 +2,11 @@
 differences of the GPU forward.
 
 Covered: the ghost-fill reverse of fixed, reflect, open and open_ucnp boundaries; thermal conduction (Euler,
-RK2 and RK4 subcycles, no flux saturation), radiative losses (with prevent_subcycling too) and the
-parametrised heating profile of the
-gradient workload, whose coefficients get their own gradient (spruce_gpu_adjoint_params).  The loss
+RK2 and RK4 subcycles, no flux saturation), radiative losses (with prevent_subcycling too), the
+source modules after the integrator (localized heating with its time ramp and window, mass and
+momentum injection, oscillating too, the ambient heating sink) and the parametrised heating
+profile of the gradient workload, whose coefficients get their own gradient
+(spruce_gpu_adjoint_params).  The loss
 is a seeded linear functional of the final evolved grids.  It is only piecewise smooth (limiter,
 min/max and floor selections, the argmin of the step size, the integer subcycle counts), so every
 directional check takes two difference steps and leaves out directions whose two differences
@@ -256,9 +258,52 @@ def test_module_adjoint_state_matches_finite_differences(name, bc, integ, mods, 
     _check_directions(st, cfg, nsteps, ckpt, w, rng, scale=_relative_scale(st))
 
 
-def test_module_adjoint_leaves_the_same_final_state_as_stepping():
+# Source modules after the integrator, in index coordinates of the 24 x 40 arcade (time step about
+# 0.04 s).  The localized heating is in its ramp (heating factor (t - start) / ramp_time, 0.125 to
+# 0.8 over the steps) and the momentum injection oscillates with a 0.5 s period: both depend on
+# the simulation time, the sum of the earlier step sizes, so the step sizes' own dependence on the
+# state carries their time derivative back.
+LHEAT = ("localized_heating", [("start_time", "-0.05"), ("duration", "1.0"), ("max_heating_rate", "3.0"),
+                               ("stddev_x", "4"), ("stddev_y", "6"), ("center_x", "10"), ("center_y", "15"),
+                               ("ramp_time", "0.4")])
+MASSINJ = ("mass_injection", [("start_time", "0"), ("duration", "10"), ("max_injection_rate", "1e10"),
+                              ("stddev_x", "3"), ("stddev_y", "5"), ("center_x", "12"), ("center_y", "20")])
+MOMINJ = ("momentum_injection", [("start_time", "0"), ("duration", "10"), ("max_accel", "1e8"), ("dir_x", "1"),
+                                 ("dir_y", "0.5"), ("template_angle", "20"), ("stddev_x", "4"), ("stddev_y", "6"),
+                                 ("center_x", "12"), ("center_y", "10"), ("oscillatory", "true"),
+                                 ("oscillation_period", "0.5")])
+SINK = ("ambient_heating_sink", [("exp_mode", "true"), ("exp_base_heating_rate", "2.0"),
+                                 ("exp_scale_height", "1e8"), ("center_x", "6e7"), ("half_width", "5e7")])
+
+SOURCE_CASES = [
+    ("lheat_sink_tc_rk2", ("periodic", "periodic", "fixed", "open"), "rk2", [TC, LHEAT, SINK], 6, 3),
+    ("massinj_heat_euler", ("periodic", "periodic", "reflect", "fixed"), "euler", [MASSINJ, HEAT(THETA[:8])], 6, 3),
+    ("mominj_rl_rk4", ("periodic", "periodic", "fixed", "open"), "rk4", [MOMINJ, RL()], 5, 2),
+    ("all_sources_rk2", ("periodic", "periodic", "fixed", "open"), "rk2", [LHEAT, MASSINJ, MOMINJ, SINK], 6, 3),
+    # a localized heating window that opens after the first steps and closes before the last
+    ("lheat_window_euler", ("periodic", "periodic", "fixed", "open"), "euler",
+     [("localized_heating", [("start_time", "0.07"), ("duration", "0.11"), ("max_heating_rate", "3.0"),
+                             ("stddev_x", "4"), ("stddev_y", "6"), ("center_x", "10"), ("center_y", "15"),
+                             ("ramp_time", "0.05")])], 7, 3),
+]
+
+
+@pytest.mark.parametrize("name,bc,integ,mods,nsteps,ckpt", SOURCE_CASES, ids=[m[0] for m in SOURCE_CASES])
+def test_source_adjoint_state_matches_finite_differences(name, bc, integ, mods, nsteps, ckpt):
+    print("adjoint sources", name, flush=True)
     st = _arcade()
-    spec = spec_from_files(_arcade_cfg(THETA), st)
+    rng = np.random.default_rng(41)
+    w = _weights(rng, (24, 40), (1e13, 1e6, 1.0, 1.0))
+
+    def cfg():
+        return _arcade_cfg(THETA, bc, integ, mods)
+    _check_directions(st, cfg, nsteps, ckpt, w, rng, scale=_relative_scale(st))
+
+
+@pytest.mark.parametrize("mods", [None, [TC, LHEAT, MASSINJ, MOMINJ, SINK]], ids=["heat", "sources"])
+def test_module_adjoint_leaves_the_same_final_state_as_stepping(mods):
+    st = _arcade()
+    spec = spec_from_files(_arcade_cfg(THETA, mods=mods), st)
     a = GpuDomain(spec)
     b = GpuDomain(spec)
     try:
