@@ -234,11 +234,40 @@ __device__ __forceinline__ void tcr_partial(double dot, double* partial, int acc
   }
 }
 
+// The saturated energy derivative from the unsaturated R at interior cell (i, j) (as the forward
+// forms it, thermalconduction.cpp:171-174); cc receives the coefficient, dcx/dcy its derivatives.
+__device__ __forceinline__ double tc_sat_apply(const TcSatRev& sr, const Mesh& M, int i, int j, double R, double& cc,
+                                               double& dcx, double& dcy) {
+  const int ny = M.ny;
+  const int im = wrp(i - 1, M.nx, M.x.per), ip = wrp(i + 1, M.nx, M.x.per);
+  const int jm = wrp(j - 1, M.ny, M.y.per), jp = wrp(j + 1, M.ny, M.y.per);
+  const size_t c = (size_t)i * ny + j;
+  cc = sr.coef[c];
+  dcx = qdiv(tcc::fint(M.x, ip, i, cc, sr.coef[(size_t)ip * ny + j]) - tcc::fint(M.x, i, im, sr.coef[(size_t)im * ny + j], cc),
+             M.x.d[i], M.x.rd[i]);
+  dcy = qdiv(tcc::fint(M.y, jp, j, cc, sr.coef[(size_t)i * ny + jp]) - tcc::fint(M.y, j, jm, sr.coef[(size_t)i * ny + jm], cc),
+             M.y.d[j], M.y.rd[j]);
+  const double add = -1.0 * (dcx * sr.rx[c] + dcy * sr.ry[c]);
+  return cc * R + add;
+}
+// the saturation outputs of one cell for the seed ld of its saturated derivative (zero seed: zeros)
+__device__ __forceinline__ void tc_sat_out(const TcSatRev& sr, size_t c, double ld, double R, double dcx,
+                                           double dcy) {
+  sr.lcoef[c] = ld * R;
+  sr.wx[c] = -ld * sr.rx[c];
+  sr.wy[c] = -ld * sr.ry[c];
+  sr.lfx[c] = -ld * dcx;
+  sr.lfy[c] = -ld * dcy;
+}
+__device__ __forceinline__ void tc_sat_zero(const TcSatRev& sr, size_t c) {
+  if (sr.coef) tc_sat_out(sr, c, 0.0, 0.0, 0.0, 0.0);
+}
+
 __global__ void __launch_bounds__(256) k_adj_tc_sub(const double* te_s, const double* n, const double* bhx,
                                                     const double* bhy, const double* dbxx, const double* dbxy,
                                                     const double* dbyx, const double* dbyy, double h, double k0,
                                                     MhdConst K, Mesh M, double* lam_te, double* lam_T,
-                                                    double* lam_fz, double* partial, int accumulate) {
+                                                    double* lam_fz, double* partial, int accumulate, TcSatRev sr) {
   __shared__ double acc[TCR_N];
   const int i0 = blockIdx.y * TCR_ROWS, j0 = blockIdx.x * TCR_COLS;
   tcr_zero(acc);
@@ -252,6 +281,7 @@ __global__ void __launch_bounds__(256) k_adj_tc_sub(const double* te_s, const do
     const double base = te_s[c];
     if (!inside(M, i, j)) {
       lam_te[c] *= (base + 0.0 < K.temin) ? 0.0 : 1.0;
+      tc_sat_zero(sr, c);
       continue;
     }
     double T[3][3];
@@ -260,16 +290,19 @@ __global__ void __launch_bounds__(256) k_adj_tc_sub(const double* te_s, const do
     // a unit seed and is scaled afterwards
     double gT[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
     double gfz[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const double dE = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0, 1.0,
-                                           gT, gfz);
+    const double R = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0, 1.0,
+                                          gT, gfz);
+    double cc = 1.0, dcx = 0.0, dcy = 0.0;
+    const double dE = sr.coef ? tc_sat_apply(sr, M, i, j, R, cc, dcx, dcy) : R;
     const double v = base + 1.0 * (1.0 * h) * dE;
     const double lv = (v < K.temin) ? 0.0 : lam_te[c];
     lam_te[c] = lv;
     dot += lv * dE;
     const double ld = lv * h;
+    if (sr.coef) tc_sat_out(sr, c, ld, R, dcx, dcy);
     if (ld != 0.0) {
-      tcr_add(acc, li, lj, gT, ld);
-      for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += ld * gfz[k];
+      tcr_add(acc, li, lj, gT, ld * cc);
+      for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += (ld * cc) * gfz[k];
     }
   } while (false);   // one cell per thread; `continue`/`break` leave the cell
   __syncthreads();
@@ -288,7 +321,7 @@ __global__ void __launch_bounds__(256) k_adj_tc_stage(const double* eval, const 
                                                       double k0, MhdConst K, Mesh M, const double* cbase,
                                                       const double* lam_out, double* lam_base, int assign_base,
                                                       const double* seed_add, double add_coef, double* lam_T,
-                                                      double* lam_fz, double* partial, int accumulate) {
+                                                      double* lam_fz, double* partial, int accumulate, TcSatRev sr) {
   __shared__ double acc[TCR_N];
   const int i0 = blockIdx.y * TCR_ROWS, j0 = blockIdx.x * TCR_COLS;
   tcr_zero(acc);
@@ -304,14 +337,17 @@ __global__ void __launch_bounds__(256) k_adj_tc_stage(const double* eval, const 
         const double mu = (cbase[c] + 0.0 < K.temin) ? 0.0 : lam_out[c];
         lam_base[c] = assign_base ? mu : lam_base[c] + mu;
       }
+      tc_sat_zero(sr, c);
       continue;
     }
     double T[3][3];
     tcr_temps(eval, n, K, M, i, j, T);
     double gT[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
     double gfz[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const double dE = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0, 1.0,
-                                           gT, gfz);
+    const double R = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0, 1.0,
+                                          gT, gfz);
+    double cc = 1.0, dcx = 0.0, dcy = 0.0;
+    const double dE = sr.coef ? tc_sat_apply(sr, M, i, j, R, cc, dcx, dcy) : R;
     double ld = 0.0;
     if (cbase) {
       const double v = cbase[c] + 1.0 * (mult * h) * dE;
@@ -321,9 +357,10 @@ __global__ void __launch_bounds__(256) k_adj_tc_stage(const double* eval, const 
       ld = mu * (mult * h);
     }
     if (seed_add) ld += add_coef * seed_add[c];
+    if (sr.coef) tc_sat_out(sr, c, ld, R, dcx, dcy);
     if (ld != 0.0) {
-      tcr_add(acc, li, lj, gT, ld);
-      for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += ld * gfz[k];
+      tcr_add(acc, li, lj, gT, ld * cc);
+      for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += (ld * cc) * gfz[k];
     }
   } while (false);   // one cell per thread; `continue`/`break` leave the cell
   __syncthreads();
@@ -342,7 +379,7 @@ __global__ void __launch_bounds__(256) k_adj_tc_final4(const double* te, const d
                                                        double h, double k0, MhdConst K, Mesh M,
                                                        const double* lam_out, double* lam_base, double* lamS,
                                                        double* lam_T, double* lam_fz, double* partial,
-                                                       int accumulate) {
+                                                       int accumulate, TcSatRev sr) {
   __shared__ double acc[TCR_N];
   const int i0 = blockIdx.y * TCR_ROWS, j0 = blockIdx.x * TCR_COLS;
   tcr_zero(acc);
@@ -356,14 +393,17 @@ __global__ void __launch_bounds__(256) k_adj_tc_final4(const double* te, const d
     if (!inside(M, i, j)) {
       lam_base[c] = (te[c] + 0.0 < K.temin) ? 0.0 : lam_out[c];
       lamS[c] = 0.0;
+      tc_sat_zero(sr, c);
       continue;
     }
     double T[3][3];
     tcr_temps(i3, n, K, M, i, j, T);
     double gT[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
     double gfz[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const double dE = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0, 1.0,
-                                           gT, gfz);
+    const double R = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0, 1.0,
+                                          gT, gfz);
+    double cc = 1.0, dcx = 0.0, dcy = 0.0;
+    const double dE = sr.coef ? tc_sat_apply(sr, M, i, j, R, cc, dcx, dcy) : R;
     const double sum = k1[c] + 2.0 * k2[c] + 2.0 * k3[c] + dE;
     const double v = te[c] + qdiv(1.0 * h * sum, 6.0, K.r6);
     const double mu = (v < K.temin) ? 0.0 : lam_out[c];
@@ -371,9 +411,10 @@ __global__ void __launch_bounds__(256) k_adj_tc_final4(const double* te, const d
     const double ls = mu * h / 6.0;
     lamS[c] = ls;
     dot += mu * sum / 6.0;
+    if (sr.coef) tc_sat_out(sr, c, ls, R, dcx, dcy);
     if (ls != 0.0) {
-      tcr_add(acc, li, lj, gT, ls);
-      for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += ls * gfz[k];
+      tcr_add(acc, li, lj, gT, ls * cc);
+      for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += (ls * cc) * gfz[k];
     }
   } while (false);   // one cell per thread; `continue`/`break` leave the cell
   __syncthreads();
@@ -407,6 +448,76 @@ __device__ __forceinline__ void scatter_deriv(const Mesh& M, int axis, int i, in
   atomicAdd(&lq[at(cp)], r * (ax.h[c] / ax.fden[cp]));
   atomicAdd(&lq[at(c)], r * (ax.h[cp] / ax.fden[cp] - ax.h[cm] / ax.fden[c]));
   atomicAdd(&lq[at(cm)], -r * (ax.h[c] / ax.fden[c]));
+}
+
+
+// d derivative1D(q, axis)(at cell k) / d q(c) for the three cells k = c-1, c, c+1 along the axis
+// (derivs.cpp:229-233, 477-487): the weights the coefficient's adjoint gathers
+__device__ __forceinline__ double gather_deriv(const Mesh& M, int axis, int i, int j, const double* w) {
+  const Axis& ax = axis == 0 ? M.x : M.y;
+  const int c = axis == 0 ? i : j;
+  const int cm = wrp(c - 1, ax.n, ax.per), cp = wrp(c + 1, ax.n, ax.per);
+  auto at = [&](int k) -> size_t { return axis == 0 ? (size_t)k * M.ny + j : (size_t)i * M.ny + k; };
+  double g = w[at(c)] * ((ax.h[cp] / ax.fden[cp] - ax.h[cm] / ax.fden[c]) / ax.d[c]);
+  if (cm >= 0 && cm < ax.n) g += w[at(cm)] * (ax.h[cm] / ax.fden[c] / ax.d[cm]);
+  if (cp >= 0 && cp < ax.n) g -= w[at(cp)] * (ax.h[cp] / ax.fden[cp] / ax.d[cp]);
+  return g;
+}
+
+__global__ void __launch_bounds__(256) k_adj_tc_satrev(const double* ev, const double* n, const double* rho,
+                                                       const double* bhx, const double* bhy, TcParams p, MhdConst K,
+                                                       Mesh M, TcSatRev sr, double* lam_T, double* lam_fz,
+                                                       double* lam_rho) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= M.nx || j >= M.ny || !inside(M, i, j)) return;   // the coefficient is 1 off the interior
+  const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+  // the coefficient's adjoint: its own seed and the add terms of the cell and its four neighbours
+  const double lc = sr.lcoef[c] + gather_deriv(M, 0, i, j, sr.wx) + gather_deriv(M, 1, i, j, sr.wy);
+  double lfx = sr.lfx[c], lfy = sr.lfy[c];
+  if (lc == 0.0 && lfx == 0.0 && lfy == 0.0) return;
+  // the forward quantities of the cell (saturateConductiveFlux / fieldAlignedConductiveFlux)
+  auto Tat = [&](int a, int b) {
+    const size_t q = (size_t)wrp(a, M.nx, M.x.per) * M.ny + wrp(b, M.ny, M.y.per);
+    return temp_of(ev[q], n[q], K);
+  };
+  const double T = Tat(i, j);
+  const int ip = wrp(i + 1, M.nx, M.x.per), im = wrp(i - 1, M.nx, M.x.per);
+  const int jp = wrp(j + 1, M.ny, M.y.per), jm = wrp(j - 1, M.ny, M.y.per);
+  const double gx = qdiv(tcc::fint(M.x, ip, i, T, Tat(i + 1, j)) - tcc::fint(M.x, i, im, Tat(i - 1, j), T), M.x.d[i], M.x.rd[i]);
+  const double gy = qdiv(tcc::fint(M.y, jp, j, T, Tat(i, j + 1)) - tcc::fint(M.y, j, jm, Tat(i, j - 1), T), M.y.d[j], M.y.rd[j]);
+  const double r = rho[c], bx = bhx[c], by = bhy[c];
+  const double kmax = M.x.d[i] * M.y.d[j] * K_B * (r / K.m) / p.dsm;
+  const double kt = pow_2p5(T) * p.k0;
+  const bool capped = kmax < kt;
+  const double kap = capped ? kmax : kt;
+  const double bdg = gx * bx + gy * by;
+  const double fmag = -kap * bdg;
+  const double fx = fmag * bx, fy = fmag * by;
+  const double sat = (r / K.m) * p.c_sat * pow_1p5(T * K_B) / p.sqrt_me;
+  const double fm2 = fx * fx + fy * fy;
+  double lT = 0.0, lr = 0.0;
+  if (fm2 > 0.0) {
+    // coef = sat / sqrt(sat^2 + |F|^2)
+    const double D = sat * sat + fm2, D32 = D * sqrt(D);
+    const double lsat = lc * (fm2 / D32);
+    lfx += lc * (-sat * fx / D32);
+    lfy += lc * (-sat * fy / D32);
+    lT += lsat * 1.5 * sat / T;
+    lr += lsat * sat / r;
+  }
+  // F = fmag b_hat, fmag = -kap (b_hat . grad T)
+  const double lfm = lfx * bx + lfy * by;
+  const double lbx = lfx * fmag + lfm * (-kap * gx), lby = lfy * fmag + lfm * (-kap * gy);
+  const double lkap = -lfm * bdg;
+  if (capped) lr += lkap * kmax / r;
+  else lT += lkap * p.k0 * 2.5 * pow_1p5(T);
+  const double lgx = -lfm * kap * bx, lgy = -lfm * kap * by;
+  if (lgx != 0.0) scatter_deriv(M, 0, i, j, lgx, lam_T);
+  if (lgy != 0.0) scatter_deriv(M, 1, i, j, lgy, lam_T);
+  if (lT != 0.0) atomicAdd(&lam_T[c], lT);
+  lam_fz[c] += lbx;
+  lam_fz[N + c] += lby;
+  lam_rho[c] += lr;
 }
 
 __global__ void __launch_bounds__(256) k_adj_bhat_derivs(Mesh M, double* lam_fz) {
@@ -639,9 +750,9 @@ void launch_adj_minmask(const double* rho_in, const double* te_base, const doubl
 void launch_adj_tc_sub(const double* te_s, const double* n, const double* bhx, const double* bhy,
                        const double* dbxx, const double* dbxy, const double* dbyx, const double* dbyy, double h,
                        double k0, MhdConst k, Mesh mesh, double* lam_te, double* lam_T, double* lam_fz,
-                       double* partial, int accumulate, hipStream_t s) {
+                       double* partial, int accumulate, hipStream_t s, TcSatRev sr) {
   hipLaunchKernelGGL(k_adj_tc_sub, tcr_grid(mesh), dim3(64, 4), 0, s, te_s, n, bhx, bhy, dbxx, dbxy, dbyx, dbyy,
-                     h, k0, k, mesh, lam_te, lam_T, lam_fz, partial, accumulate);
+                     h, k0, k, mesh, lam_te, lam_T, lam_fz, partial, accumulate, sr);
 }
 
 int adj_tc_blocks(const Mesh& m) {
@@ -653,19 +764,27 @@ void launch_adj_tc_stage(const double* eval, const double* n, const double* bhx,
                          const double* dbxx, const double* dbxy, const double* dbyx, const double* dbyy, double h,
                          double mult, double k0, MhdConst k, Mesh mesh, const double* cbase, const double* lam_out,
                          double* lam_base, int assign_base, const double* seed_add, double add_coef, double* lam_T,
-                         double* lam_fz, double* partial, int accumulate, hipStream_t s) {
+                         double* lam_fz, double* partial, int accumulate, hipStream_t s, TcSatRev sr) {
   hipLaunchKernelGGL(k_adj_tc_stage, tcr_grid(mesh), dim3(64, 4), 0, s, eval, n, bhx, bhy, dbxx, dbxy, dbyx, dbyy,
                      h, mult, k0, k, mesh, cbase, lam_out, lam_base, assign_base, seed_add, add_coef, lam_T, lam_fz,
-                     partial, accumulate);
+                     partial, accumulate, sr);
 }
 
 void launch_adj_tc_final4(const double* te, const double* k1, const double* k2, const double* k3, const double* i3,
                           const double* n, const double* bhx, const double* bhy, const double* dbxx,
                           const double* dbxy, const double* dbyx, const double* dbyy, double h, double k0, MhdConst k,
                           Mesh mesh, const double* lam_out, double* lam_base, double* lamS, double* lam_T,
-                          double* lam_fz, double* partial, int accumulate, hipStream_t s) {
+                          double* lam_fz, double* partial, int accumulate, hipStream_t s, TcSatRev sr) {
   hipLaunchKernelGGL(k_adj_tc_final4, tcr_grid(mesh), dim3(64, 4), 0, s, te, k1, k2, k3, i3, n, bhx, bhy, dbxx,
-                     dbxy, dbyx, dbyy, h, k0, k, mesh, lam_out, lam_base, lamS, lam_T, lam_fz, partial, accumulate);
+                     dbxy, dbyx, dbyy, h, k0, k, mesh, lam_out, lam_base, lamS, lam_T, lam_fz, partial, accumulate,
+                     sr);
+}
+
+void launch_adj_tc_satrev(const double* eval, const double* n, const double* rho, const double* bhx,
+                          const double* bhy, TcParams p, MhdConst k, Mesh mesh, TcSatRev sr, double* lam_T,
+                          double* lam_fz, double* lam_rho, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_tc_satrev, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, eval, n, rho, bhx, bhy, p, k, mesh,
+                     sr, lam_T, lam_fz, lam_rho);
 }
 
 void launch_adj_tc_temp(const double* te_s, const double* n, MhdConst k, size_t N, double* lam_T, double* lam_te,

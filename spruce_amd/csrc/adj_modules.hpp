@@ -1,7 +1,7 @@
 // Discrete-adjoint kernels for the parts of the step outside the IdealMHD stages (adj_modules.hip):
 // the ghost fill of open, fixed and reflect boundaries, the minimum/derived update of a live
-// propagate, thermal conduction (Euler subcycles, no flux saturation), radiative losses, and the
-// heating of the parametrised profile.  Adjoint vectors are 8*N doubles in evolved order
+// propagate, thermal conduction (any subcycle integrator, flux saturation too), radiative losses,
+// the source modules after the integrator, and the heating of the parametrised profile.  Adjoint vectors are 8*N doubles in evolved order
 // (rho, mom_x, mom_y, mom_z, thermal_energy, bi_x, bi_y, bi_z), updated in place.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -31,10 +31,21 @@ void launch_adj_minmask(const double* rho_in, const double* te_base, const doubl
 // = block sums of lam * dE (the adjoint of h), added to the entries already there when accumulate.
 // The conduction reverses run adj_tc_blocks() blocks (16 x 64 tiles).
 int adj_tc_blocks(const Mesh& m);
+// Flux saturation in a conduction reverse (thermalconduction.cpp:171-174, 255-271): the energy
+// derivative is coef * R + add with add = -(d coef/dx * rx + d coef/dy * ry).  coef, rx, ry are the
+// forward grids at the evaluated temperatures (launch_tc_sat); a stage reverse given them seeds R
+// with coef times its seed and writes, per cell, the seed times R (lcoef), minus the seed times rx
+// and ry (wx, wy: the weights of the coefficient's derivatives) and minus the seed times the
+// coefficient's derivatives (lfx, lfy: the adjoint of the raw flux); launch_adj_tc_satrev then
+// carries them to the temperatures, the density and b_hat.  coef == nullptr: no saturation.
+struct TcSatRev {
+  const double *coef = nullptr, *rx = nullptr, *ry = nullptr;
+  double *lcoef = nullptr, *wx = nullptr, *wy = nullptr, *lfx = nullptr, *lfy = nullptr;
+};
 void launch_adj_tc_sub(const double* te_s, const double* n, const double* bhx, const double* bhy,
                        const double* dbxx, const double* dbxy, const double* dbyx, const double* dbyy, double h,
                        double k0, MhdConst k, Mesh mesh, double* lam_te, double* lam_T, double* lam_fz,
-                       double* partial, int accumulate, hipStream_t s);
+                       double* partial, int accumulate, hipStream_t s, TcSatRev sr = TcSatRev());
 // A subcycle stage of the RK2/RK4 subcycle integrators reversed (thermalconduction.cpp:67-124):
 // dE is taken at the temperatures of `eval`; with cbase given, the stage output was
 // max(cbase + mask*(mult*h)*dE, te_min): lam_out goes through the clamp into lam_base (assigned
@@ -45,7 +56,7 @@ void launch_adj_tc_stage(const double* eval, const double* n, const double* bhx,
                          const double* dbxx, const double* dbxy, const double* dbyx, const double* dbyy, double h,
                          double mult, double k0, MhdConst k, Mesh mesh, const double* cbase, const double* lam_out,
                          double* lam_base, int assign_base, const double* seed_add, double add_coef, double* lam_T,
-                         double* lam_fz, double* partial, int accumulate, hipStream_t s);
+                         double* lam_fz, double* partial, int accumulate, hipStream_t s, TcSatRev sr = TcSatRev());
 // The RK4 subcycle's final combination max(te + (h*(k1 + 2k2 + 2k3 + k4))/6, te_min), k4 at the
 // temperatures of i3, reversed: lam_base = mu, lamS = mu*h/6 (k_j's seed is its weight times
 // lamS), k4's transpose applied, partial = block sums of mu*S/6.
@@ -53,7 +64,16 @@ void launch_adj_tc_final4(const double* te, const double* k1, const double* k2, 
                           const double* n, const double* bhx, const double* bhy, const double* dbxx,
                           const double* dbxy, const double* dbyx, const double* dbyy, double h, double k0, MhdConst k,
                           Mesh mesh, const double* lam_out, double* lam_base, double* lamS, double* lam_T,
-                          double* lam_fz, double* partial, int accumulate, hipStream_t s);
+                          double* lam_fz, double* partial, int accumulate, hipStream_t s, TcSatRev sr = TcSatRev());
+// The saturation outputs of a stage reverse (TcSatRev) carried back: the coefficient's adjoint
+// (its own seed plus the transposed derivative1D of the neighbours' weights) through
+// coef = sat / sqrt(sat^2 + |F|^2), sat = (rho/m_i) (1/6)(3/2) (K_B T)^1.5 / sqrt(m_e), and the raw flux
+// F = -min(k0 T^2.5, kappa_max) (b_hat . grad T) b_hat (thermalconduction.cpp:196-227) onto lam_T
+// (atomics; grad T transposed onto the four neighbours), lam_fz (b_hat_x, b_hat_y) and lam_rho,
+// at the temperatures of `eval`.
+void launch_adj_tc_satrev(const double* eval, const double* n, const double* rho, const double* bhx,
+                          const double* bhy, TcParams p, MhdConst k, Mesh mesh, TcSatRev sr, double* lam_T,
+                          double* lam_fz, double* lam_rho, hipStream_t s);
 // T = max((gamma-1) te / (2 K_B n), T_min) reversed: lam_te += lam_T dT/dte, lam_rho += lam_T dT/dn / m
 // (n = rho / m), and lam_T is cleared for the next subcycle.
 void launch_adj_tc_temp(const double* te_s, const double* n, MhdConst k, size_t N, double* lam_T, double* lam_te,

@@ -2499,12 +2499,13 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   // modules: conduction (no saturation) and radiative losses act in iterateModule; ambient and
   // parametrised heating and the source modules after the integrator
   std::vector<int> it_mods, post_mods;
+  bool any_sat = false;
   rec_post.assign(mods.size(), false);
   for (size_t k = 0; k < mods.size(); ++k) {
     GpuModule* m = mods[k].get();
     if (auto* tc = dynamic_cast<TcModule*>(m)) {
-      require(!tc->p.saturation && !tc->inactive, SPRUCE_E_UNSUPPORTED,
-              "the adjoint covers thermal conduction without flux saturation");
+      require(!tc->inactive, SPRUCE_E_UNSUPPORTED, "the adjoint covers thermal conduction that is applied");
+      any_sat = any_sat || tc->p.saturation;
       it_mods.push_back((int)k);
     } else if (auto* rl = dynamic_cast<RlModule*>(m)) {
       require(!rl->inactive, SPRUCE_E_UNSUPPORTED, "the adjoint covers radiative losses that are applied");
@@ -2537,7 +2538,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   const size_t SET = 10 * N, EV = 8 * N, CO = (size_t)ADJ_COEFS * N;
   const size_t seg_store = replay ? (size_t)(nit + npost + 2 + (rk4 ? 3 : 0)) * SET + (rk4 ? 4 : 2) * EV
                                   : (rk2 ? (size_t)sl * SET : 0) + (size_t)sl * EV * (rk2 ? 2 : 1);
-  const size_t modscratch = replay ? 10 * N + (size_t)ny + 2 * (size_t)nb + 2 : 0;
+  const size_t modscratch = replay ? 10 * N + (size_t)ny + 2 * (size_t)nb + 2 + (any_sat ? 8 * N : 0) : 0;
   const size_t rk4scratch = rk4 ? 4 * EV + 2 * (size_t)nb : 0;
   const size_t total = (size_t)nseg * SET + (size_t)(sl + 1) * SET + seg_store + 3 * EV + 2 * CO + 4 * (size_t)nb +
                        modscratch + rk4scratch;
@@ -2598,6 +2599,17 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
     gh = take(1);
     partt = take(nb);
     gtime = take(1);
+  }
+  // flux saturation: the forward coefficient and raw flux at the evaluated temperatures, and the
+  // stage reverses' outputs for them
+  TcSatRev satr;
+  double *sat_coef = nullptr, *sat_rx = nullptr, *sat_ry = nullptr;
+  if (replay && any_sat) {
+    sat_coef = take(N);
+    sat_rx = take(N);
+    sat_ry = take(N);
+    satr.coef = sat_coef; satr.rx = sat_rx; satr.ry = sat_ry;
+    satr.lcoef = take(N); satr.wx = take(N); satr.wy = take(N); satr.lfx = take(N); satr.lfy = take(N);
   }
   // RK4: the adjoint of the stages' common base, the final combination's h*mu, one stage's
   // mult*h*mu, a second stage adjoint, and the partials of the two middle stages
@@ -2795,13 +2807,25 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
           const double* te_last = ns ? tc->rec_out(ns - 1) : pre.te;
           const double hs = tc->h_last, k0 = tc->p.k0;
           double* lte = lam + 4 * N;
+          // with flux saturation every evaluation at temperatures `ev` first recomputes the forward
+          // coefficient and raw flux there, and its stage reverse is followed by their reverse
+          const bool sat = tc->p.saturation != 0;
+          const TcSatRev sr = sat ? satr : TcSatRev();
+          auto sat_pre = [&](const double* ev) {
+            if (sat) launch_tc_sat(ev, pre.n, pre.rho, bhx.p, bhy.p, tc->p, kc, mesh, sat_rx, sat_ry, sat_coef, stream);
+          };
+          auto sat_post = [&](const double* ev) {
+            if (sat) launch_adj_tc_satrev(ev, pre.n, pre.rho, bhx.p, bhy.p, tc->p, kc, mesh, sr, lamT, lamfz, lam, stream);
+          };
           // one stage reverse: dE at `ev`, clamp base te_s (or none), seed extra
           // the step-size partials of every subcycle kernel are summed per block, reduced once
           int acc = 0;
           auto stage_rev = [&](const double* ev, double mult, const double* cb, const double* lout, int assign,
                                const double* sadd, double coef) {
+            sat_pre(ev);
             launch_adj_tc_stage(ev, pre.n, bhx.p, bhy.p, tc->dbxx.p, tc->dbxy.p, tc->dbyx.p, tc->dbyy.p, hs, mult, k0,
-                                kc, mesh, cb, lout, lacc, assign, sadd, coef, lamT, lamfz, partm, acc, stream);
+                                kc, mesh, cb, lout, lacc, assign, sadd, coef, lamT, lamfz, partm, acc, stream, sr);
+            sat_post(ev);
             acc = 1;
           };
           // dJ/dT at the temperatures of `ev` onto dJ/dte (into `into`, cleared first unless it
@@ -2816,8 +2840,10 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
             for (int q = ns - 1; q >= 0; --q) {
               const double* te_s = q ? tc->rec_out(q - 1) : pre.te;
               if (tc->integ == "euler") {
+                sat_pre(te_s);
                 launch_adj_tc_sub(te_s, pre.n, bhx.p, bhy.p, tc->dbxx.p, tc->dbxy.p, tc->dbyx.p, tc->dbyy.p, hs, k0,
-                                  kc, mesh, lte, lamT, lamfz, partm, acc, stream);
+                                  kc, mesh, lte, lamT, lamfz, partm, acc, stream, sr);
+                sat_post(te_s);
                 acc = 1;
                 launch_adj_tc_temp(te_s, pre.n, kc, N, lamT, lte, lam, stream);
                 continue;
@@ -2829,9 +2855,11 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
                 stage_rev(te_s, 0.5, te_s, lx, 0, nullptr, 0.0);
               } else {                                  // thermalconduction.cpp:79-98
                 const double *i1 = tc->rec_at(q, 0), *i2 = tc->rec_at(q, 1), *i3 = tc->rec_at(q, 2);
+                sat_pre(i3);
                 launch_adj_tc_final4(te_s, tc->rec_at(q, 3), tc->rec_at(q, 4), tc->rec_at(q, 5), i3, pre.n, bhx.p,
                                      bhy.p, tc->dbxx.p, tc->dbxy.p, tc->dbyx.p, tc->dbyy.p, hs, k0, kc, mesh, lte, lacc,
-                                     lamS, lamT, lamfz, partm, acc, stream);
+                                     lamS, lamT, lamfz, partm, acc, stream, sr);
+                sat_post(i3);
                 acc = 1;
                 temp_rev(i3, lx, true);
                 stage_rev(i2, 1.0, te_s, lx, 0, lamS, 2.0);

@@ -2,7 +2,7 @@
 differences of the GPU forward.
 
 Covered: the ghost-fill reverse of fixed, reflect, open and open_ucnp boundaries; thermal conduction (Euler,
-RK2 and RK4 subcycles, no flux saturation), radiative losses (with prevent_subcycling too), the
+RK2 and RK4 subcycles, with and without flux saturation), radiative losses (with prevent_subcycling too), the
 source modules after the integrator (localized heating with its time ramp and window, mass and
 momentum injection, oscillating too, the ambient heating sink) and the parametrised heating
 profile of the gradient workload, whose coefficients get their own gradient
@@ -38,6 +38,14 @@ def TCI(integ):
 
 
 TC = TCI("euler")
+
+
+def TCS(integ):
+    """Conduction with flux saturation: on the arcade below a quarter of the interior cells carry a
+    coefficient under 0.99 at step 0 (down to 0.64), so the coefficient, its derivatives and the
+    raw flux all enter the gradient."""
+    return ("thermal_conduction", [("flux_saturation", "true"), ("epsilon", "0.5"), ("dt_subcycle_min", "1e-6"),
+                                   ("time_integrator", integ)])
 
 
 def RL(integ="euler", prevent=False):
@@ -201,6 +209,10 @@ MODULE_CASES = [
      [TC, RL(prevent=True), HEAT(THETA[:8])], 6, 3),
     ("rlprev2_heat_rk4", ("periodic", "periodic", "reflect", "fixed"), "rk4", [RL("rk2", prevent=True), HEAT(THETA[:8])],
      4, 2),
+    # flux saturation under each subcycle integrator
+    ("tcsat_rl_heat_rk2", ("periodic", "periodic", "fixed", "open"), "rk2", [TCS("euler"), RL(), HEAT(THETA[:8])], 6, 3),
+    ("tcsat2_heat_euler", ("periodic", "periodic", "reflect", "fixed"), "euler", [TCS("rk2"), HEAT(THETA[:8])], 6, 3),
+    ("tcsat4_heat_rk4", ("periodic", "periodic", "fixed", "open"), "rk4", [TCS("rk4"), HEAT(THETA[:8])], 4, 2),
 ]
 
 
@@ -320,8 +332,9 @@ def test_module_adjoint_leaves_the_same_final_state_as_stepping(mods):
 
 def test_module_adjoint_refuses_what_it_does_not_cover():
     st = _arcade()
-    sat = ("thermal_conduction", [("flux_saturation", "true"), ("epsilon", "0.5"), ("dt_subcycle_min", "1e-6")])
-    for mods in ([sat], [("physical_viscosity", [("coeff", "0.1")])]):
+    visc = ("artificial_viscosity", [("visc_opt", "global"), ("visc_strength", "0.1"), ("visc_vars_to_diff", "v_x"),
+                                     ("visc_vars_to_evol", "mom_x"), ("visc_length", "0"), ("visc_species", "i")])
+    for mods in ([visc], [("physical_viscosity", [("coeff", "0.1")])]):
         d = GpuDomain(spec_from_files(_arcade_cfg(THETA, mods=mods), st))
         try:
             with pytest.raises(SpruceError):
