@@ -720,6 +720,175 @@ __global__ void __launch_bounds__(256) k_adj_accum(const double* partial, int n,
   if (threadIdx.x == 0) gh[0] += scale * s;
 }
 
+// ---- artificial viscosity ----------------------------------------------------------------------
+
+// the differentiated variable at one cell, as the forward term reads it (idealmhd.cpp:241-277)
+__device__ double visc_var(const MhdSet& S, const MhdStatic& st, const MhdConst& K, int var, size_t c) {
+  switch (var) {
+    case VV_RHO: return S.rho[c];
+    case VV_N: return S.n[c];
+    case VV_MX: return S.mx[c];
+    case VV_MY: return S.my[c];
+    case VV_MZ: return S.mz[c];
+    case VV_TE: return S.te[c];
+    case VV_BIX: return S.bx[c];
+    case VV_BIY: return S.by[c];
+    case VV_BIZ: return S.bz[c];
+    case VV_GX: return st.gx[c];
+    case VV_GY: return st.gy[c];
+    case VV_TEMP: return smax((S.te[c] * K.gm1) / (S.n[c] * (2.0 * K_B)), K.tmin);
+    case VV_PRESS: return S.te[c] * K.gm1;
+    case VV_VX: return S.mx[c] / S.rho[c];
+    case VV_VY: return S.my[c] / S.rho[c];
+    case VV_VZ: return S.mz[c] / S.rho[c];
+    case VV_KE: {
+      const double r = S.rho[c], vx = S.mx[c] / r, vy = S.my[c] / r;
+      return r * 0.5 * (vx * vx + vy * vy);
+    }
+    case VV_BX: return st.bex[c] + S.bx[c];
+    case VV_BY: return st.bey[c] + S.by[c];
+    case VV_BZ: return st.bez[c] + S.bz[c];
+    default: {
+      const double bx = st.bex[c] + S.bx[c], by = st.bey[c] + S.by[c], bz = st.bez[c] + S.bz[c];
+      const double bm = sqrt(bx * bx + by * by + bz * bz);
+      if (var == VV_BMAG) return bm;
+      if (bm == 0.0) return 0.0;
+      return var == VV_BHX ? bx / bm : (var == VV_BHY ? by / bm : bz / bm);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_adj_visc_term(ViscTermArgs a, int evol, const double* ka, double sa,
+                                                       const double* kb, double* lq, double* lam_S, double* ldt,
+                                                       double* partial) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  const Mesh& M = a.mesh;
+  double dot = 0.0;
+  if (i < M.nx && j < M.ny && inside(M, i, j)) {
+    const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+    const double lo = sa * ka[evol * N + c] + (kb ? kb[evol * N + c] : 0.0);
+    if (lo != 0.0) {
+      const Axis& X = M.x;
+      const Axis& Y = M.y;
+      const double dx = X.d[i], dy = Y.d[j];
+      const double st = a.strength ? a.strength[c] : a.strength_c;
+      const double dt = a.dtgrid ? a.dtgrid[c] : *a.dtmin;
+      const double nu = st / (1.0 / (dx * dx) + 1.0 / (dy * dy)) / 2. / dt;
+      const double sc = a.scale == 0 ? 1.0 : a.S.n[c] * a.scale_c;
+      const int im = wrp(i - 1, M.nx, X.per), ip = wrp(i + 1, M.nx, X.per);
+      const int jm = wrp(j - 1, M.ny, Y.per), jp = wrp(j + 1, M.ny, Y.per);
+      const size_t cxm = (size_t)im * M.ny + j, cxp = (size_t)ip * M.ny + j;
+      const size_t cym = (size_t)i * M.ny + jm, cyp = (size_t)i * M.ny + jp;
+      const double q = visc_var(a.S, a.st, a.k, a.var, c);
+      const double qxm = visc_var(a.S, a.st, a.k, a.var, cxm), qxp = visc_var(a.S, a.st, a.k, a.var, cxp);
+      const double qym = visc_var(a.S, a.st, a.k, a.var, cym), qyp = visc_var(a.S, a.st, a.k, a.var, cyp);
+      // second derivatives through the face interpolants (derivs.cpp:450-462, 477-487)
+      const double xu_c = X.h[ip] / X.fden[ip], xu_p = X.h[i] / X.fden[ip];
+      const double xl_m = X.h[i] / X.fden[i], xl_c = X.h[im] / X.fden[i];
+      const double yu_c = Y.h[jp] / Y.fden[jp], yu_p = Y.h[j] / Y.fden[jp];
+      const double yl_m = Y.h[j] / Y.fden[j], yl_c = Y.h[jm] / Y.fden[j];
+      const double sx = ((q * xu_c + qxp * xu_p) - 2.0 * q + (qxm * xl_m + q * xl_c)) / X.hsq[i];
+      const double sy = ((q * yu_c + qyp * yu_p) - 2.0 * q + (qym * yl_m + q * yl_c)) / Y.hsq[j];
+      const double lap = sx + sy;
+      // r = nu * lap * sc: the Laplacian's stencil
+      const double ll = lo * nu * sc;
+      const double rx = ll / X.hsq[i], ry = ll / Y.hsq[j];
+      atomicAdd(&lq[c], rx * (xu_c - 2.0 + xl_c) + ry * (yu_c - 2.0 + yl_c));
+      atomicAdd(&lq[cxp], rx * xu_p);
+      atomicAdd(&lq[cxm], rx * xl_m);
+      atomicAdd(&lq[cyp], ry * yu_p);
+      atomicAdd(&lq[cym], ry * yl_m);
+      // the scale n * c (n = rho / m_i)
+      if (a.scale != 0) lam_S[c] += lo * nu * lap * a.scale_c * a.k.rm;
+      // the coefficient's timescale: d nu / d dt = -nu / dt
+      const double ld = -(lo * lap * sc) * nu / dt;
+      if (a.dtgrid) ldt[c] += ld;
+      else dot = ld;
+    }
+  }
+  if (partial) {
+    dot = block_sum(dot);
+    if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_adj_visc_vars(ViscTermArgs a, double* lq, double* lam_S) {
+  const size_t N = (size_t)a.mesh.nx * a.mesh.ny;
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const double l = lq[c];
+  if (l == 0.0) return;
+  lq[c] = 0.0;
+  const MhdSet& S = a.S;
+  const MhdConst& K = a.k;
+  double* lr = lam_S;
+  double* lmx = lam_S + N;
+  double* lmy = lam_S + 2 * N;
+  double* lmz = lam_S + 3 * N;
+  double* lte = lam_S + 4 * N;
+  double* lb[3] = {lam_S + 5 * N, lam_S + 6 * N, lam_S + 7 * N};
+  switch (a.var) {
+    case VV_RHO: lr[c] += l; break;
+    case VV_N: lr[c] += l * K.rm; break;
+    case VV_MX: lmx[c] += l; break;
+    case VV_MY: lmy[c] += l; break;
+    case VV_MZ: lmz[c] += l; break;
+    case VV_TE: lte[c] += l; break;
+    case VV_BIX: case VV_BX: lb[0][c] += l; break;
+    case VV_BIY: case VV_BY: lb[1][c] += l; break;
+    case VV_BIZ: case VV_BZ: lb[2][c] += l; break;
+    case VV_GX: case VV_GY: break;
+    case VV_TEMP: {
+      const double n = S.n[c], den = n * (2.0 * K_B);
+      const double x = (S.te[c] * K.gm1) / den;
+      if (x < K.tmin) break;
+      lte[c] += l * (K.gm1 / den);
+      lr[c] += l * (-x / n) * K.rm;
+      break;
+    }
+    case VV_PRESS: lte[c] += l * K.gm1; break;
+    case VV_VX: case VV_VY: case VV_VZ: {
+      const double r = S.rho[c];
+      const double m = a.var == VV_VX ? S.mx[c] : (a.var == VV_VY ? S.my[c] : S.mz[c]);
+      double* lm = a.var == VV_VX ? lmx : (a.var == VV_VY ? lmy : lmz);
+      lm[c] += l / r;
+      lr[c] -= l * m / (r * r);
+      break;
+    }
+    case VV_KE: {
+      const double r = S.rho[c], vx = S.mx[c] / r, vy = S.my[c] / r;
+      lmx[c] += l * vx;
+      lmy[c] += l * vy;
+      lr[c] -= l * 0.5 * (vx * vx + vy * vy);
+      break;
+    }
+    default: {   // |b| and b_hat
+      const double b[3] = {a.st.bex[c] + S.bx[c], a.st.bey[c] + S.by[c], a.st.bez[c] + S.bz[c]};
+      const double bm = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+      if (bm == 0.0) break;
+      if (a.var == VV_BMAG) {
+        for (int k = 0; k < 3; ++k) lb[k][c] += l * b[k] / bm;
+      } else {
+        const int w = a.var == VV_BHX ? 0 : (a.var == VV_BHY ? 1 : 2);
+        const double bm3 = bm * bm * bm;
+        for (int k = 0; k < 3; ++k) lb[k][c] += l * ((k == w ? 1.0 / bm : 0.0) - b[w] * b[k] / bm3);
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_adj_dt_grid(MhdSet L, MhdStatic st, MhdConst K, Mesh M, double* ldt,
+                                                     double* lam_L) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= M.nx || j >= M.ny) return;
+  const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+  const double g = ldt[c];
+  if (g == 0.0) return;
+  ldt[c] = 0.0;
+  const Dual8 dt = dt_cell_dual(L, st, K, M, i, j);
+  for (int v = 0; v < 8; ++v) lam_L[v * N + c] += g * dt.d[v];
+}
+
 inline dim3 g2(const Mesh& m, dim3 b) { return dim3((m.ny + b.x - 1) / b.x, (m.nx + b.y - 1) / b.y); }
 inline dim3 tcr_grid(const Mesh& m) { return dim3((m.ny + TCR_COLS - 1) / TCR_COLS, (m.nx + TCR_ROWS - 1) / TCR_ROWS); }
 
@@ -778,6 +947,21 @@ void launch_adj_tc_final4(const double* te, const double* k1, const double* k2, 
   hipLaunchKernelGGL(k_adj_tc_final4, tcr_grid(mesh), dim3(64, 4), 0, s, te, k1, k2, k3, i3, n, bhx, bhy, dbxx,
                      dbxy, dbyx, dbyy, h, k0, k, mesh, lam_out, lam_base, lamS, lam_T, lam_fz, partial, accumulate,
                      sr);
+}
+
+void launch_adj_visc_term(const ViscTermArgs& a, int evol, const double* ka, double sa, const double* kb, double* lq,
+                          double* lam_S, double* ldt, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_visc_term, g2(a.mesh, dim3(64, 4)), dim3(64, 4), 0, s, a, evol, ka, sa, kb, lq, lam_S, ldt,
+                     partial);
+}
+
+void launch_adj_visc_vars(const ViscTermArgs& a, double* lq, double* lam_S, hipStream_t s) {
+  const size_t N = (size_t)a.mesh.nx * a.mesh.ny;
+  hipLaunchKernelGGL(k_adj_visc_vars, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a, lq, lam_S);
+}
+
+void launch_adj_dt_grid(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double* ldt, double* lam_L, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_dt_grid, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, L, st, k, mesh, ldt, lam_L);
 }
 
 void launch_adj_tc_satrev(const double* eval, const double* n, const double* rho, const double* bhx,

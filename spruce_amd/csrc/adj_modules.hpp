@@ -1,13 +1,15 @@
 // Discrete-adjoint kernels for the parts of the step outside the IdealMHD stages (adj_modules.hip):
 // the ghost fill of open, fixed and reflect boundaries, the minimum/derived update of a live
 // propagate, thermal conduction (any subcycle integrator, flux saturation too), radiative losses,
-// the source modules after the integrator, and the heating of the parametrised profile.  Adjoint vectors are 8*N doubles in evolved order
+// artificial viscosity in the right-hand side, the source modules after the integrator, and the
+// heating of the parametrised profile.  Adjoint vectors are 8*N doubles in evolved order
 // (rho, mom_x, mom_y, mom_z, thermal_energy, bi_x, bi_y, bi_z), updated in place.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include "mhd.hpp"
 #include "solar.hpp"
+#include "visc.hpp"
 
 namespace spruce {
 
@@ -81,6 +83,20 @@ void launch_adj_tc_temp(const double* te_s, const double* n, MhdConst k, size_t 
 // The b_hat derivatives (derivative1D, zero off the interior) and b_hat = b/|b| (idealmhd.cpp:259-277)
 // reversed onto bi_x, bi_y, bi_z of the state L the conduction step froze them from.
 void launch_adj_bhat(MhdSet L, MhdStatic st, Mesh mesh, double* lam_fz, double* lam, hipStream_t s);
+// Artificial viscosity as a right-hand-side term (strength <= 1, no gradient correction;
+// viscosity.cpp:112-121, 185-267), reversed at the stage state a.S for the stage derivative's adjoint
+// lamk = sa*ka + kb (kb may be null), its component `evol`: the Laplacian's stencil to lq (atomics;
+// cleared by launch_adj_visc_vars), the scale n*c to lam_S's density, and the coefficient's
+// timescale: a local dt grid's adjoint to ldt (per cell, added), a global minimum's to partial (one
+// block sum each; null for a local term).
+void launch_adj_visc_term(const ViscTermArgs& a, int evol, const double* ka, double sa, const double* kb, double* lq,
+                          double* lam_S, double* ldt, double* partial, hipStream_t s);
+// The Laplacian's adjoint lq of the differentiated variable a.var taken to the evolved grids of a.S
+// (idealmhd.cpp:241-277 reversed) on every cell, into lam_S; lq cleared.
+void launch_adj_visc_vars(const ViscTermArgs& a, double* lq, double* lam_S, hipStream_t s);
+// The adjoint of the per-cell dt grid of L (idealmhd.cpp:279-304) added to lam_L on the interior;
+// ldt cleared.
+void launch_adj_dt_grid(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double* ldt, double* lam_L, hipStream_t s);
 // Reverse of RadiativeLosses::iterateModule on the input state A (radiativelosses.cpp:84-140,
 // with prevent_subcycling through the live dt of A, st its static fields) fused with the live
 // propagate's minimum/derived reverse: lam updated in place, partial[block] = block sums of dJ/dh.

@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(64) k_adj_dt_apply(MhdSet L, MhdStatic st, Mhd
                                                      const double* pmin, const int* pidx, int nb, double* lam_L) {
   // the adjoint of the step size: the sum of the stage kernels' block partials, plus the modules'
   double g = 0.0;
-  for (int k = threadIdx.x; k < nb; k += 64) g += p1[k] + (p2 ? p2[k] : 0.0);
+  for (int k = threadIdx.x; k < nb; k += 64) g += (p1 ? p1[k] : 0.0) + (p2 ? p2[k] : 0.0);
   for (int off = 32; off > 0; off >>= 1) g += __shfl_xor(g, off, 64);
   if (gh) g += gh[0];
   // (min, lowest index) over the block results: one wavefront, then a shuffle reduction; the rule

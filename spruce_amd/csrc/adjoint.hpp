@@ -29,7 +29,7 @@ void launch_adj_rhs_cells(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, const d
 // The face part of the transposed time derivative at S (both axes, one launch): FP64 atomics into lam_S.
 void launch_adj_faces(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, const double* cx, const double* cy,
                       double* lam_S, hipStream_t s);
-// step-size adjoint: g = sum of the partials p1 (+ p2 when not null) (+ gh[0] when not null, the
+// step-size adjoint: g = sum of the partials p1 and p2 (each may be null) (+ gh[0] when not null, the
 // modules' part); h = eps * min dt(L) over the interior; lam_L(c*) += eps * g * grad dt(c*)
 void launch_adj_dt(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double eps, const double* p1, const double* p2,
                    const double* gh, double* pmin, int* pidx, double* lam_L, hipStream_t s);

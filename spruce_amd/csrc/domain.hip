@@ -110,9 +110,9 @@ struct GpuModule {
   // after every module's iterate, before the integrator (the live grids are final for this step)
   virtual void before_integrator(Domain&) {}
   // RHS terms of this module on grid set S (computeTimeDerivativesModule), appended to a stage
-  virtual void rhs_terms(Domain&, SetBufs&, StageArgs&) {}
+  virtual void rhs_terms(Domain&, const MhdSet&, StageArgs&) {}
   // the same terms added to a full RHS array (nevolved consecutive grids)
-  virtual void add_rhs(Domain&, SetBufs&, double*) {}
+  virtual void add_rhs(Domain&, const MhdSet&, double*) {}
   // postIterateModule: after the integrator's final propagate, before the next step size
   virtual void post(Domain&) {}
   // the values commandLineMessage prints that live on the device (empty for most modules)
@@ -569,7 +569,11 @@ class Domain : public DomainBase {
              double* dt_partial = nullptr) {
     StageArgs a = stage_args(S.view(), B.view(), T.view(), mult, mode, kslot, step_ptr, nullptr);
     a.dt_partial = dt_partial;
-    for (auto& m : mods) m->rhs_terms(*this, S, a);
+    launch_stage_mods(a);
+  }
+  // a stage with the modules' right-hand-side terms at its state a.S
+  void launch_stage_mods(StageArgs a) {
+    for (auto& m : mods) m->rhs_terms(*this, a.S, a);
     launch_stage(a);
   }
   // a stage on plain set views, for sets that live in a caller's buffer (no module terms)
@@ -629,14 +633,14 @@ class Domain : public DomainBase {
     const bool fused = fuse_dt && !rec;
     double* dtp = fused ? partial.p : nullptr;
     if (integ == SPRUCE_EULER) {                       // evolution.cpp:84-88
-      if (rec) launch_stage(stage_args(L.view(), L.view(), W.view(), 1.0, STAGE_STORE_K, 0, nullptr, rec->k1));
+      if (rec) launch_stage_mods(stage_args(L.view(), L.view(), W.view(), 1.0, STAGE_STORE_K, 0, nullptr, rec->k1));
       else stage(L, L, W, 1.0, STAGE_PLAIN, 0, nullptr, dtp);
       L.swap(W);
     } else if (integ == SPRUCE_RK2) {                  // evolution.cpp:90-101
       if (rec) {                                       // the same stages, keeping k1, k2 and the midpoint
-        launch_stage(stage_args(L.view(), L.view(), rec->W, 0.5, STAGE_STORE_K, 0, nullptr, rec->k1));
+        launch_stage_mods(stage_args(L.view(), L.view(), rec->W, 0.5, STAGE_STORE_K, 0, nullptr, rec->k1));
         ghost_set(rec->W, kinds_ucnp, GHOST_NORMAL);
-        launch_stage(stage_args(rec->W, L.view(), L.view(), 1.0, STAGE_STORE_K, 0, nullptr, rec->k2));
+        launch_stage_mods(stage_args(rec->W, L.view(), L.view(), 1.0, STAGE_STORE_K, 0, nullptr, rec->k2));
       } else {
         stage(L, L, W, 0.5, STAGE_PLAIN, 0);
         ghost(W, kinds_ucnp, GHOST_NORMAL);
@@ -647,19 +651,19 @@ class Domain : public DomainBase {
         stage(W, L, L, 1.0, STAGE_PLAIN, 0, nullptr, dtp);
       }
     } else if (rec) {                                  // RK4, keeping every stage and derivative
-      launch_stage(stage_args(L.view(), L.view(), rec->W, 0.5, STAGE_STORE_K, 0, nullptr, rec->k1));
+      launch_stage_mods(stage_args(L.view(), L.view(), rec->W, 0.5, STAGE_STORE_K, 0, nullptr, rec->k1));
       ghost_set(rec->W, kinds_ucnp, GHOST_NORMAL);
       ghost(L, kinds_live, GHOST_LIVE_RAW);
-      launch_stage(stage_args(rec->W, L.view(), rec->W2, 0.5, STAGE_STORE_K, 0, nullptr, rec->k2));
+      launch_stage_mods(stage_args(rec->W, L.view(), rec->W2, 0.5, STAGE_STORE_K, 0, nullptr, rec->k2));
       ghost_set(rec->W2, kinds_ucnp, GHOST_NORMAL);
       ghost(L, kinds_live, GHOST_LIVE_RAW);
-      launch_stage(stage_args(rec->W2, L.view(), rec->W3, 1.0, STAGE_STORE_K, 0, nullptr, rec->k3));
+      launch_stage_mods(stage_args(rec->W2, L.view(), rec->W3, 1.0, STAGE_STORE_K, 0, nullptr, rec->k3));
       ghost_set(rec->W3, kinds_ucnp, GHOST_NORMAL);
       ghost(L, kinds_live, GHOST_LIVE_RAW);
       copy_set(L.view(), rec->Lm, N, stream);
       StageArgs a = stage_args(rec->W3, L.view(), L.view(), 1.0, STAGE_RK4_FINAL, 0, nullptr, nullptr);
       a.kbuf = rec->k1; a.k2buf = rec->k2; a.k3buf = rec->k3; a.kf_out = rec->kf;
-      launch_stage(a);
+      launch_stage_mods(a);
     } else {                                           // evolution.cpp:103-124
       stage(L, L, W, 0.5, STAGE_STORE_K, 0);
       ghost(W, kinds_ucnp, GHOST_NORMAL);
@@ -768,7 +772,7 @@ class Domain : public DomainBase {
   void rhs_staged(double* dev, double* out) {
     launch_mhd_rhs_full(L.view(), stat(), kc, mesh, dev, stream, moc_terms(L.view()));
     for (auto& m : mods) m->before_integrator(*this);
-    for (auto& m : mods) m->add_rhs(*this, L, dev);
+    for (auto& m : mods) m->add_rhs(*this, L.view(), dev);
     hip_check(hipMemcpyAsync(out, dev, 8 * N * 8, hipMemcpyDeviceToHost, stream), "download rhs");
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
   }
@@ -1326,15 +1330,21 @@ struct ViscModule : GpuModule {
       launch_reduce_step(part.p, d.dt_blocks, 1.0, dtmin.p, nullptr, 0, d.err_flag, d.stream);
     });
   }
-  void term(Domain& d, const Term& t, SetBufs& S, double* out, bool masked) {
+  // the arguments of one term's evaluation on S (the adjoint reuses them)
+  ViscTermArgs term_args(Domain& d, const Term& t, const MhdSet& S) {
     ViscTermArgs a{};
-    a.S = S.view(); a.st = d.stat(); a.k = d.kc; a.mesh = d.mesh;
+    a.S = S; a.st = d.stat(); a.k = d.kc; a.mesh = d.mesh;
     a.strength = t.sgrid.p;
     a.strength_c = t.strength;
     const bool local = t.opt == "local" || t.opt == "boundary";
     a.dtgrid = local ? dtg.p : nullptr;
     a.dtmin = dtmin.p;
-    a.var = t.var; a.scale = t.scale; a.scale_c = t.scale_c; a.grad = grad ? 1 : 0; a.masked = masked ? 1 : 0;
+    a.var = t.var; a.scale = t.scale; a.scale_c = t.scale_c; a.grad = grad ? 1 : 0;
+    return a;
+  }
+  void term(Domain& d, const Term& t, const MhdSet& S, double* out, bool masked) {
+    ViscTermArgs a = term_args(d, t, S);
+    a.masked = masked ? 1 : 0;
     a.out = out;
     a.lap_out = t.o_lap.p; a.str_out = t.o_str.p; a.dt_out = t.o_dt.p;
     d.timed("visc_term", [&] { launch_visc_term(a, d.stream); });
@@ -1350,13 +1360,13 @@ struct ViscModule : GpuModule {
   // term's Laplacian, strength and timescale outputs come from that evaluation when the step's
   // integrator runs after the module's subcycles.
   bool side_out() const { return out_lap || out_str || out_dt; }
-  void hyper_side(Domain& d, SetBufs& S) {
+  void hyper_side(Domain& d, const MhdSet& S) {
     if (!side_out()) return;
     for (auto& t : terms)
       if (t.hyper()) term(d, t, S, d1.p, true);   // the dqdt itself is discarded
   }
   void before_integrator(Domain& d) override { if (any_rhs || (any_hyper && side_out())) refresh(d); }
-  void rhs_terms(Domain& d, SetBufs& S, StageArgs& a) override {
+  void rhs_terms(Domain& d, const MhdSet& S, StageArgs& a) override {
     hyper_side(d, S);
     for (auto& t : terms) {
       if (t.hyper()) continue;
@@ -1366,7 +1376,7 @@ struct ViscModule : GpuModule {
       a.n_extra++;
     }
   }
-  void add_rhs(Domain& d, SetBufs& S, double* out) override {
+  void add_rhs(Domain& d, const MhdSet& S, double* out) override {
     hyper_side(d, S);
     for (auto& t : terms) {
       if (t.hyper()) continue;
@@ -1387,7 +1397,7 @@ struct ViscModule : GpuModule {
       if (nsub < 0) nsub = n;
       const double h = dt / (double)n;
       double* g = evolved_ptr(d.L, t.evol_idx);
-      auto eval = [&](double* out) { refresh(d); term(d, t, d.L, out, false); };
+      auto eval = [&](double* out) { refresh(d); term(d, t, d.L.view(), out, false); };
       auto upd = [&](const double* base, const double* dd, double hh) {
         d.timed("visc_update", [&] { launch_visc_update(g, base, dd, hh, d.mesh, d.stream); });
         d.propagate_live();
@@ -2499,6 +2509,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   // modules: conduction (no saturation) and radiative losses act in iterateModule; ambient and
   // parametrised heating and the source modules after the integrator
   std::vector<int> it_mods, post_mods;
+  std::vector<ViscModule*> visc_mods;   // right-hand-side terms of every stage
   bool any_sat = false;
   rec_post.assign(mods.size(), false);
   for (size_t k = 0; k < mods.size(); ++k) {
@@ -2510,6 +2521,11 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
     } else if (auto* rl = dynamic_cast<RlModule*>(m)) {
       require(!rl->inactive, SPRUCE_E_UNSUPPORTED, "the adjoint covers radiative losses that are applied");
       it_mods.push_back((int)k);
+    } else if (auto* vm = dynamic_cast<ViscModule*>(m)) {
+      require(!vm->any_hyper, SPRUCE_E_UNSUPPORTED,
+              "the adjoint covers artificial viscosity without hyperviscous terms (strength above 1)");
+      require(!vm->grad, SPRUCE_E_UNSUPPORTED, "the adjoint covers artificial viscosity without gradient_correction");
+      visc_mods.push_back(vm);
     } else if (dynamic_cast<ParamHeatModule*>(m) || dynamic_cast<HeatModule*>(m) ||
                dynamic_cast<LocalHeatModule*>(m) || dynamic_cast<MassInjModule*>(m) ||
                dynamic_cast<MomInjModule*>(m) || dynamic_cast<HeatSinkModule*>(m)) {
@@ -2538,10 +2554,12 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   const size_t SET = 10 * N, EV = 8 * N, CO = (size_t)ADJ_COEFS * N;
   const size_t seg_store = replay ? (size_t)(nit + npost + 2 + (rk4 ? 3 : 0)) * SET + (rk4 ? 4 : 2) * EV
                                   : (rk2 ? (size_t)sl * SET : 0) + (size_t)sl * EV * (rk2 ? 2 : 1);
+  const bool visc = !visc_mods.empty();
   const size_t modscratch = replay ? 10 * N + (size_t)ny + 2 * (size_t)nb + 2 + (any_sat ? 8 * N : 0) : 0;
   const size_t rk4scratch = rk4 ? 4 * EV + 2 * (size_t)nb : 0;
+  const size_t viscscratch = visc ? (rk4 ? 0 : EV) + 2 * N + (size_t)adj_mod_blocks(mesh) + 1 : 0;
   const size_t total = (size_t)nseg * SET + (size_t)(sl + 1) * SET + seg_store + 3 * EV + 2 * CO + 4 * (size_t)nb +
-                       modscratch + rk4scratch;
+                       modscratch + rk4scratch + viscscratch;
   DBuf slab;
   slab.alloc(total);
   adj_bytes = (int64_t)(total * sizeof(double));
@@ -2622,6 +2640,30 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
     part3 = take(nb);
     part4 = take(nb);
   }
+  // artificial viscosity: the stage derivatives' adjoint (the RK4 buffer otherwise), the Laplacian's
+  // scatter, the dt grid's adjoint, the global minimum's block sums and its adjoint
+  double *vlq = nullptr, *vldt = nullptr, *vpart = nullptr, *vg = nullptr;
+  if (visc) {
+    if (!rk4) lka = take(EV);
+    vlq = take(N);
+    vldt = take(N);
+    vpart = take((size_t)adj_mod_blocks(mesh));
+    vg = take(1);
+    launch_fill(vlq, 0.0, N, stream);
+    launch_fill(vldt, 0.0, N, stream);
+    launch_fill(vg, 0.0, 1, stream);
+  }
+  auto visc_rev = [&](const MhdSet& S, const double* ka, double sa, const double* kb, double* lamS) {
+    for (ViscModule* vm : visc_mods)
+      for (auto& t : vm->terms) {
+        const ViscTermArgs a = vm->term_args(*this, t, S);
+        timed("adj_visc", [&] {
+          launch_adj_visc_term(a, t.evol_idx, ka, sa, kb, vlq, lamS, vldt, a.dtgrid ? nullptr : vpart, stream);
+          if (!a.dtgrid) launch_adj_accum(vpart, adj_mod_blocks(mesh), 1.0, vg, stream);
+          launch_adj_visc_vars(a, vlq, lamS, stream);
+        });
+      }
+  };
   if ((int)history.n < nsteps + 1) history.alloc(nsteps + 1);
   hip_check(hipMemcpyAsync(history.p, step.p, sizeof(double), hipMemcpyDeviceToDevice, stream), "copy step");
   sub_hist.assign((size_t)nsteps * std::max<size_t>(1, mods.size()), -1);
@@ -2750,6 +2792,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
           launch_adj_update(tape.Lm, tape.kf, hp, 1.0, kc, mesh, lam, lamM, 1, lkf, part, stream);
           launch_adj_rhs_cells(tape.W3, st, kc, mesh, lkf, 1.0 / 6.0, nullptr, lamW, 1, cx, cy, stream);
         });
+        visc_rev(tape.W3, lkf, 1.0 / 6.0, nullptr, lamW);
         timed("adj_faces", [&] { launch_adj_faces(tape.W3, st, kc, mesh, cx, cy, lamW, stream); });
         adj_ghost(tape.W3, lamW, kinds_ucnp);              // each stage state's open_ucnp ghost fill
         // W3 = F(Lm + h k3), k3 = k(W2): lambda_k3 = h mu3 + lkf/3
@@ -2757,6 +2800,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
           launch_adj_update(tape.Lm, tape.k3, hp, 1.0, kc, mesh, lamW, lamM, 0, lka, part2, stream);
           launch_adj_rhs_cells(tape.W2, st, kc, mesh, lkf, 1.0 / 3.0, lka, lamX, 1, cx, cy, stream);
         });
+        visc_rev(tape.W2, lkf, 1.0 / 3.0, lka, lamX);
         timed("adj_faces", [&] { launch_adj_faces(tape.W2, st, kc, mesh, cx, cy, lamX, stream); });
         adj_ghost(tape.W2, lamX, kinds_ucnp);
         // W2 = F(Lm + h/2 k2), k2 = k(W): lambda_k2 = h/2 mu2 + lkf/3
@@ -2764,6 +2808,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
           launch_adj_update(tape.Lm, tape.k2, hp, 0.5, kc, mesh, lamX, lamM, 0, lka, part3, stream);
           launch_adj_rhs_cells(tape.W, st, kc, mesh, lkf, 1.0 / 3.0, lka, lamW, 1, cx, cy, stream);
         });
+        visc_rev(tape.W, lkf, 1.0 / 3.0, lka, lamW);
         timed("adj_faces", [&] { launch_adj_faces(tape.W, st, kc, mesh, cx, cy, lamW, stream); });
         adj_ghost(tape.W, lamW, kinds_ucnp);
         // W = F(A2 + h/2 k1), k1 = k(A2): lambda_k1 = h/2 mu1 + lkf/6
@@ -2771,6 +2816,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
           launch_adj_update(A2, tape.k1, hp, 0.5, kc, mesh, lamW, lamN, 1, lka, part4, stream);
           launch_adj_rhs_cells(A2, st, kc, mesh, lkf, 1.0 / 6.0, lka, lamN, 0, cx, cy, stream);
         });
+        visc_rev(A2, lkf, 1.0 / 6.0, lka, lamN);
         // (its faces follow below, as for the other integrators); Lm is A2 with its ghost cells refilled from A2's interior without floors
         // (GHOST_LIVE_RAW, evolution.cpp:126-152 writing the live grids): its ghost adjoint goes to
         // that interior, the rest is A2's own
@@ -2779,23 +2825,39 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
         launch_add(lamN, lamM, 8 * N, stream);
         launch_adj_accum(part3, nb, 1.0, gh, stream);
         launch_adj_accum(part4, nb, 1.0, gh, stream);
-      } else if (rk2) {
-        timed("adj_stage", [&] {
-          launch_adj_stage_cells(A2, tape.k2, hp, 1.0, tape.W, st, kc, mesh, lam, lamN, 1, lamW, 1, cx, cy, part,
-                                 stream);
-        });
-        timed("adj_faces", [&] { launch_adj_faces(tape.W, st, kc, mesh, cx, cy, lamW, stream); });
-        adj_ghost(tape.W, lamW, kinds_ucnp);
-        timed("adj_stage", [&] {
-          launch_adj_stage_cells(A2, tape.k1, hp, 0.5, A2, st, kc, mesh, lamW, lamN, 0, lamN, 0, cx, cy, part2,
-                                 stream);
-        });
       } else {
-        timed("adj_stage", [&] {
-          launch_adj_stage_cells(A2, tape.k1, hp, 1.0, A2, st, kc, mesh, lam, lamN, 1, lamN, 0, cx, cy, part, stream);
-        });
+        // one stage reverse: T = F(B + mult*h*k(S)) with lam_T in, lam_B and lam_S out; with viscosity
+        // terms the stage derivative's adjoint is kept for them
+        auto stage_rev = [&](const double* kv, double mult, const MhdSet& S, const double* lam_T, double* lam_B,
+                             int assign_B, double* lam_S, int assign_S, double* prt) {
+          timed("adj_stage", [&] {
+            if (!visc) {
+              launch_adj_stage_cells(A2, kv, hp, mult, S, st, kc, mesh, lam_T, lam_B, assign_B, lam_S, assign_S, cx,
+                                     cy, prt, stream);
+            } else {
+              launch_adj_update(A2, kv, hp, mult, kc, mesh, lam_T, lam_B, assign_B, lka, prt, stream);
+              launch_adj_rhs_cells(S, st, kc, mesh, lka, 1.0, nullptr, lam_S, assign_S, cx, cy, stream);
+            }
+          });
+          if (visc) visc_rev(S, lka, 1.0, nullptr, lam_S);
+        };
+        if (rk2) {
+          stage_rev(tape.k2, 1.0, tape.W, lam, lamN, 1, lamW, 1, part);
+          timed("adj_faces", [&] { launch_adj_faces(tape.W, st, kc, mesh, cx, cy, lamW, stream); });
+          adj_ghost(tape.W, lamW, kinds_ucnp);
+          stage_rev(tape.k1, 0.5, A2, lamW, lamN, 0, lamN, 0, part2);
+        } else {
+          stage_rev(tape.k1, 1.0, A2, lam, lamN, 1, lamN, 0, part);
+        }
       }
       timed("adj_faces", [&] { launch_adj_faces(A2, st, kc, mesh, cx, cy, lamN, stream); });
+      if (visc) {   // the viscosity coefficient's timescale: the dt grid of A2 (local) or its minimum
+        timed("adj_visc", [&] {
+          launch_adj_dt_grid(A2, st, kc, mesh, vldt, lamN, stream);
+          launch_adj_dt(A2, st, kc, mesh, 1.0, nullptr, nullptr, vg, pmin, pidx, lamN, stream);
+          launch_fill(vg, 0.0, 1, stream);
+        });
+      }
       std::swap(lam, lamN);
       for (int k = nit - 1; k >= 0; --k) {                 // iterating modules, last first
         const MhdSet& post = tape.after_it[k];

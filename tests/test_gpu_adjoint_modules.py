@@ -3,7 +3,8 @@ differences of the GPU forward.
 
 Covered: the ghost-fill reverse of fixed, reflect, open and open_ucnp boundaries; thermal conduction (Euler,
 RK2 and RK4 subcycles, with and without flux saturation), radiative losses (with prevent_subcycling too), the
-source modules after the integrator (localized heating with its time ramp and window, mass and
+artificial viscosity as a right-hand-side term (local, global and boundary strengths, any
+differentiated variable), the source modules after the integrator (localized heating with its time ramp and window, mass and
 momentum injection, oscillating too, the ambient heating sink) and the parametrised heating
 profile of the gradient workload, whose coefficients get their own gradient
 (spruce_gpu_adjoint_params).  The loss
@@ -165,6 +166,40 @@ def test_boundary_adjoint_matches_finite_differences(name, bc, integ, nsteps, ck
     _check_directions(base, cfg, nsteps, ckpt, w, rng)
 
 
+def VISC(opt, strength, diff, evol, length=None, species=None):
+    n = len(opt.split(","))
+    return ("artificial_viscosity", [("visc_opt", opt), ("visc_strength", strength), ("visc_vars_to_diff", diff),
+                                     ("visc_vars_to_evol", evol), ("visc_length", length or ",".join(["0"] * n)),
+                                     ("visc_species", species or ",".join(["i"] * n))])
+
+
+VISC_CASES = [
+    # artificial viscosity as a right-hand-side term: local terms read the dt grid of the state the
+    # integrator starts from, global ones its minimum; velocity and temperature are scaled by n
+    ("local_global_rk2", ("periodic", "periodic", "fixed", "reflect"), "rk2",
+     VISC("local,global", "0.1,0.05", "v_x,temp", "mom_x,thermal_energy"), 8, 3),
+    ("boundary_euler", ("periodic", "periodic", "fixed", "open"), "euler",
+     VISC("boundary,boundary_global", "0.3,0.2", "rho,v_y", "rho,mom_y", "2e7,3e7"), 8, 4),
+    ("fields_rk4", ("periodic", "periodic", "reflect", "fixed"), "rk4",
+     VISC("local,global,local", "0.2,0.1,0.1", "bi_x,b_hat_y,press", "bi_x,bi_y,thermal_energy"), 6, 2),
+    ("ke_bmag_ucnp_rk2", ("open_ucnp",) * 4, "rk2",
+     VISC("local,global", "0.1,0.1", "kinetic_energy,b_mag", "thermal_energy,bi_z"), 6, 3),
+]
+
+
+@pytest.mark.parametrize("name,bc,integ,visc,nsteps,ckpt", VISC_CASES, ids=[v[0] for v in VISC_CASES])
+def test_viscosity_adjoint_matches_finite_differences(name, bc, integ, visc, nsteps, ckpt):
+    print("adjoint viscosity", name, flush=True)
+    base = random_mhd_state(40, 48, 97, stretch=True)
+    rng = np.random.default_rng(13)
+    w = _weights(rng, (40, 48), (1e15, 1e8, 10.0, 1.0))
+
+    def cfg():
+        return W.base_config("ideal_mhd", bc, integ, epsilon=0.2, density_min=1.0, temp_min=1.0,
+                             thermal_energy_min=1e-12, modules=[visc])
+    _check_directions(base, cfg, nsteps, ckpt, w, rng)
+
+
 def _arcade_cfg(theta, bc=("periodic", "periodic", "fixed", "open"), integ="rk2", mods=None):
     mods = mods if mods is not None else [TC, RL(), HEAT(theta)]
     return W.base_config("ideal_mhd", bc, integ, epsilon=0.2, density_min=1.0, temp_min=1e4,
@@ -213,6 +248,9 @@ MODULE_CASES = [
     ("tcsat_rl_heat_rk2", ("periodic", "periodic", "fixed", "open"), "rk2", [TCS("euler"), RL(), HEAT(THETA[:8])], 6, 3),
     ("tcsat2_heat_euler", ("periodic", "periodic", "reflect", "fixed"), "euler", [TCS("rk2"), HEAT(THETA[:8])], 6, 3),
     ("tcsat4_heat_rk4", ("periodic", "periodic", "fixed", "open"), "rk4", [TCS("rk4"), HEAT(THETA[:8])], 4, 2),
+    # artificial viscosity beside the iterating modules (its dt grid is that of the state after them)
+    ("tc_visc_heat_rk2", ("periodic", "periodic", "fixed", "open"), "rk2",
+     [TC, RL(), VISC("local,global", "0.1,0.05", "v_y,temp", "mom_y,thermal_energy"), HEAT(THETA[:8])], 6, 3),
 ]
 
 
@@ -332,8 +370,7 @@ def test_module_adjoint_leaves_the_same_final_state_as_stepping(mods):
 
 def test_module_adjoint_refuses_what_it_does_not_cover():
     st = _arcade()
-    visc = ("artificial_viscosity", [("visc_opt", "global"), ("visc_strength", "0.1"), ("visc_vars_to_diff", "v_x"),
-                                     ("visc_vars_to_evol", "mom_x"), ("visc_length", "0"), ("visc_species", "i")])
+    visc = VISC("local", "2.5", "v_x", "mom_x")      # hyperviscous: subcycled, not covered
     for mods in ([visc], [("physical_viscosity", [("coeff", "0.1")])]):
         d = GpuDomain(spec_from_files(_arcade_cfg(THETA, mods=mods), st))
         try:
