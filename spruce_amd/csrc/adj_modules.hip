@@ -758,9 +758,17 @@ __device__ double visc_var(const MhdSet& S, const MhdStatic& st, const MhdConst&
   }
 }
 
+// nu at one cell (viscosity.cpp:212): ((s * 1) / (1/d_x^2 + 1/d_y^2)) / 2 / dt
+__device__ __forceinline__ double visc_nu(const ViscTermArgs& a, int i, int j, size_t c, double& dt) {
+  const double dx = a.mesh.x.d[i], dy = a.mesh.y.d[j];
+  const double st = a.strength ? a.strength[c] : a.strength_c;
+  dt = a.dtgrid ? a.dtgrid[c] : *a.dtmin;
+  return st / (1.0 / (dx * dx) + 1.0 / (dy * dy)) / 2. / dt;
+}
+
 __global__ void __launch_bounds__(256) k_adj_visc_term(ViscTermArgs a, int evol, const double* ka, double sa,
-                                                       const double* kb, double* lq, double* lam_S, double* ldt,
-                                                       double* partial) {
+                                                       const double* kb, double* lq, double* lf, double* lam_S,
+                                                       double* ldt, double* partial) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
   const Mesh& M = a.mesh;
   double dot = 0.0;
@@ -770,10 +778,8 @@ __global__ void __launch_bounds__(256) k_adj_visc_term(ViscTermArgs a, int evol,
     if (lo != 0.0) {
       const Axis& X = M.x;
       const Axis& Y = M.y;
-      const double dx = X.d[i], dy = Y.d[j];
-      const double st = a.strength ? a.strength[c] : a.strength_c;
-      const double dt = a.dtgrid ? a.dtgrid[c] : *a.dtmin;
-      const double nu = st / (1.0 / (dx * dx) + 1.0 / (dy * dy)) / 2. / dt;
+      double dt;
+      const double nu = visc_nu(a, i, j, c, dt);
       const double sc = a.scale == 0 ? 1.0 : a.S.n[c] * a.scale_c;
       const int im = wrp(i - 1, M.nx, X.per), ip = wrp(i + 1, M.nx, X.per);
       const int jm = wrp(j - 1, M.ny, Y.per), jp = wrp(j + 1, M.ny, Y.per);
@@ -802,6 +808,57 @@ __global__ void __launch_bounds__(256) k_adj_visc_term(ViscTermArgs a, int evol,
       if (a.scale != 0) lam_S[c] += lo * nu * lap * a.scale_c * a.k.rm;
       // the coefficient's timescale: d nu / d dt = -nu / dt
       const double ld = -(lo * lap * sc) * nu / dt;
+      if (a.dtgrid) ldt[c] += ld;
+      else dot = ld;
+      if (a.grad) {
+        // + d(nu*sc)/dx * dq/dx + d(nu*sc)/dy * dq/dy (derivative1D of both, derivs.cpp:223-264)
+        auto fof = [&](int ii, int jj) {
+          const size_t cc = (size_t)ii * M.ny + jj;
+          double t;
+          return visc_nu(a, ii, jj, cc, t) * (a.scale == 0 ? 1.0 : a.S.n[cc] * a.scale_c);
+        };
+        const double f = nu * sc, fxm = fof(im, j), fxp = fof(ip, j), fym = fof(i, jm), fyp = fof(i, jp);
+        const double gx = ((q * xu_c + qxp * xu_p) - (qxm * xl_m + q * xl_c)) / X.d[i];
+        const double gy = ((q * yu_c + qyp * yu_p) - (qym * yl_m + q * yl_c)) / Y.d[j];
+        const double fx = ((f * xu_c + fxp * xu_p) - (fxm * xl_m + f * xl_c)) / X.d[i];
+        const double fy = ((f * yu_c + fyp * yu_p) - (fym * yl_m + f * yl_c)) / Y.d[j];
+        // derivative1D transposed onto the three cells of each axis, for q (seed fx, fy) and f (gx, gy)
+        auto spread = [&](double* to, double sx_, double sy_) {
+          const double ax = sx_ / X.d[i], ay = sy_ / Y.d[j];
+          atomicAdd(&to[c], ax * (xu_c - xl_c) + ay * (yu_c - yl_c));
+          atomicAdd(&to[cxp], ax * xu_p);
+          atomicAdd(&to[cxm], -ax * xl_m);
+          atomicAdd(&to[cyp], ay * yu_p);
+          atomicAdd(&to[cym], -ay * yl_m);
+        };
+        spread(lq, lo * fx, lo * fy);
+        spread(lf, lo * gx, lo * gy);
+      }
+    }
+  }
+  if (partial) {
+    dot = block_sum(dot);
+    if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
+  }
+}
+
+// the gradient correction's adjoint of f = nu * sc on every cell: the scale onto the density, nu
+// onto its timescale (per cell, or block sums for a global minimum); lf cleared
+__global__ void __launch_bounds__(256) k_adj_visc_f(ViscTermArgs a, double* lf, double* lam_S, double* ldt,
+                                                    double* partial) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  const Mesh& M = a.mesh;
+  double dot = 0.0;
+  if (i < M.nx && j < M.ny) {
+    const size_t c = (size_t)i * M.ny + j;
+    const double l = lf[c];
+    if (l != 0.0) {
+      lf[c] = 0.0;
+      double dt;
+      const double nu = visc_nu(a, i, j, c, dt);
+      const double sc = a.scale == 0 ? 1.0 : a.S.n[c] * a.scale_c;
+      if (a.scale != 0) lam_S[c] += l * nu * a.scale_c * a.k.rm;
+      const double ld = -(l * sc) * nu / dt;
       if (a.dtgrid) ldt[c] += ld;
       else dot = ld;
     }
@@ -950,9 +1007,13 @@ void launch_adj_tc_final4(const double* te, const double* k1, const double* k2, 
 }
 
 void launch_adj_visc_term(const ViscTermArgs& a, int evol, const double* ka, double sa, const double* kb, double* lq,
-                          double* lam_S, double* ldt, double* partial, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_visc_term, g2(a.mesh, dim3(64, 4)), dim3(64, 4), 0, s, a, evol, ka, sa, kb, lq, lam_S, ldt,
-                     partial);
+                          double* lf, double* lam_S, double* ldt, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_visc_term, g2(a.mesh, dim3(64, 4)), dim3(64, 4), 0, s, a, evol, ka, sa, kb, lq, lf, lam_S,
+                     ldt, partial);
+}
+
+void launch_adj_visc_f(const ViscTermArgs& a, double* lf, double* lam_S, double* ldt, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_visc_f, g2(a.mesh, dim3(64, 4)), dim3(64, 4), 0, s, a, lf, lam_S, ldt, partial);
 }
 
 void launch_adj_visc_vars(const ViscTermArgs& a, double* lq, double* lam_S, hipStream_t s) {

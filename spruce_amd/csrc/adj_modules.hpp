@@ -83,14 +83,18 @@ void launch_adj_tc_temp(const double* te_s, const double* n, MhdConst k, size_t 
 // The b_hat derivatives (derivative1D, zero off the interior) and b_hat = b/|b| (idealmhd.cpp:259-277)
 // reversed onto bi_x, bi_y, bi_z of the state L the conduction step froze them from.
 void launch_adj_bhat(MhdSet L, MhdStatic st, Mesh mesh, double* lam_fz, double* lam, hipStream_t s);
-// Artificial viscosity as a right-hand-side term (strength <= 1, no gradient correction;
-// viscosity.cpp:112-121, 185-267), reversed at the stage state a.S for the stage derivative's adjoint
-// lamk = sa*ka + kb (kb may be null), its component `evol`: the Laplacian's stencil to lq (atomics;
-// cleared by launch_adj_visc_vars), the scale n*c to lam_S's density, and the coefficient's
-// timescale: a local dt grid's adjoint to ldt (per cell, added), a global minimum's to partial (one
-// block sum each; null for a local term).
+// Artificial viscosity as a right-hand-side term (strength <= 1; viscosity.cpp:112-121, 185-267),
+// reversed at the stage state a.S for the stage derivative's adjoint lamk = sa*ka + kb (kb may be
+// null), its component `evol`: the Laplacian's stencil to lq (atomics; cleared by
+// launch_adj_visc_vars), the scale n*c to lam_S's density, and the coefficient's timescale: a local
+// dt grid's adjoint to ldt (per cell, added), a global minimum's to partial (one block sum each;
+// null for a local term).  With gradient_correction the derivatives of q go to lq too and those of
+// f = nu*sc to lf (atomics; then launch_adj_visc_f).
 void launch_adj_visc_term(const ViscTermArgs& a, int evol, const double* ka, double sa, const double* kb, double* lq,
-                          double* lam_S, double* ldt, double* partial, hipStream_t s);
+                          double* lf, double* lam_S, double* ldt, double* partial, hipStream_t s);
+// The gradient correction's adjoint lf of f = nu*sc on every cell taken to the density (the scale)
+// and to the timescale (ldt, or block sums in partial for a global term); lf cleared.
+void launch_adj_visc_f(const ViscTermArgs& a, double* lf, double* lam_S, double* ldt, double* partial, hipStream_t s);
 // The Laplacian's adjoint lq of the differentiated variable a.var taken to the evolved grids of a.S
 // (idealmhd.cpp:241-277 reversed) on every cell, into lam_S; lq cleared.
 void launch_adj_visc_vars(const ViscTermArgs& a, double* lq, double* lam_S, hipStream_t s);

@@ -2524,7 +2524,6 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
     } else if (auto* vm = dynamic_cast<ViscModule*>(m)) {
       require(!vm->any_hyper, SPRUCE_E_UNSUPPORTED,
               "the adjoint covers artificial viscosity without hyperviscous terms (strength above 1)");
-      require(!vm->grad, SPRUCE_E_UNSUPPORTED, "the adjoint covers artificial viscosity without gradient_correction");
       visc_mods.push_back(vm);
     } else if (dynamic_cast<ParamHeatModule*>(m) || dynamic_cast<HeatModule*>(m) ||
                dynamic_cast<LocalHeatModule*>(m) || dynamic_cast<MassInjModule*>(m) ||
@@ -2557,7 +2556,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   const bool visc = !visc_mods.empty();
   const size_t modscratch = replay ? 10 * N + (size_t)ny + 2 * (size_t)nb + 2 + (any_sat ? 8 * N : 0) : 0;
   const size_t rk4scratch = rk4 ? 4 * EV + 2 * (size_t)nb : 0;
-  const size_t viscscratch = visc ? (rk4 ? 0 : EV) + 2 * N + (size_t)adj_mod_blocks(mesh) + 1 : 0;
+  const size_t viscscratch = visc ? (rk4 ? 0 : EV) + 3 * N + (size_t)adj_mod_blocks(mesh) + 1 : 0;
   const size_t total = (size_t)nseg * SET + (size_t)(sl + 1) * SET + seg_store + 3 * EV + 2 * CO + 4 * (size_t)nb +
                        modscratch + rk4scratch + viscscratch;
   DBuf slab;
@@ -2642,10 +2641,12 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   }
   // artificial viscosity: the stage derivatives' adjoint (the RK4 buffer otherwise), the Laplacian's
   // scatter, the dt grid's adjoint, the global minimum's block sums and its adjoint
-  double *vlq = nullptr, *vldt = nullptr, *vpart = nullptr, *vg = nullptr;
+  double *vlq = nullptr, *vlf = nullptr, *vldt = nullptr, *vpart = nullptr, *vg = nullptr;
   if (visc) {
     if (!rk4) lka = take(EV);
     vlq = take(N);
+    vlf = take(N);
+    launch_fill(vlf, 0.0, N, stream);
     vldt = take(N);
     vpart = take((size_t)adj_mod_blocks(mesh));
     vg = take(1);
@@ -2658,8 +2659,12 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
       for (auto& t : vm->terms) {
         const ViscTermArgs a = vm->term_args(*this, t, S);
         timed("adj_visc", [&] {
-          launch_adj_visc_term(a, t.evol_idx, ka, sa, kb, vlq, lamS, vldt, a.dtgrid ? nullptr : vpart, stream);
+          launch_adj_visc_term(a, t.evol_idx, ka, sa, kb, vlq, vlf, lamS, vldt, a.dtgrid ? nullptr : vpart, stream);
           if (!a.dtgrid) launch_adj_accum(vpart, adj_mod_blocks(mesh), 1.0, vg, stream);
+          if (a.grad) {
+            launch_adj_visc_f(a, vlf, lamS, vldt, a.dtgrid ? nullptr : vpart, stream);
+            if (!a.dtgrid) launch_adj_accum(vpart, adj_mod_blocks(mesh), 1.0, vg, stream);
+          }
           launch_adj_visc_vars(a, vlq, lamS, stream);
         });
       }

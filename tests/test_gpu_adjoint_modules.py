@@ -4,7 +4,7 @@ differences of the GPU forward.
 Covered: the ghost-fill reverse of fixed, reflect, open and open_ucnp boundaries; thermal conduction (Euler,
 RK2 and RK4 subcycles, with and without flux saturation), radiative losses (with prevent_subcycling too), the
 artificial viscosity as a right-hand-side term (local, global and boundary strengths, any
-differentiated variable), the source modules after the integrator (localized heating with its time ramp and window, mass and
+differentiated variable, gradient correction), the source modules after the integrator (localized heating with its time ramp and window, mass and
 momentum injection, oscillating too, the ambient heating sink) and the parametrised heating
 profile of the gradient workload, whose coefficients get their own gradient
 (spruce_gpu_adjoint_params).  The loss
@@ -166,11 +166,11 @@ def test_boundary_adjoint_matches_finite_differences(name, bc, integ, nsteps, ck
     _check_directions(base, cfg, nsteps, ckpt, w, rng)
 
 
-def VISC(opt, strength, diff, evol, length=None, species=None):
+def VISC(opt, strength, diff, evol, length=None, species=None, **extra):
     n = len(opt.split(","))
     return ("artificial_viscosity", [("visc_opt", opt), ("visc_strength", strength), ("visc_vars_to_diff", diff),
                                      ("visc_vars_to_evol", evol), ("visc_length", length or ",".join(["0"] * n)),
-                                     ("visc_species", species or ",".join(["i"] * n))])
+                                     ("visc_species", species or ",".join(["i"] * n))] + list(extra.items()))
 
 
 VISC_CASES = [
@@ -184,6 +184,12 @@ VISC_CASES = [
      VISC("local,global,local", "0.2,0.1,0.1", "bi_x,b_hat_y,press", "bi_x,bi_y,thermal_energy"), 6, 2),
     ("ke_bmag_ucnp_rk2", ("open_ucnp",) * 4, "rk2",
      VISC("local,global", "0.1,0.1", "kinetic_energy,b_mag", "thermal_energy,bi_z"), 6, 3),
+    # gradient correction: the derivatives of nu*scale take the neighbours' dt grid, strength and n
+    ("gradcorr_rk2", ("periodic", "periodic", "fixed", "open"), "rk2",
+     VISC("boundary,local,global", "0.3,0.1,0.1", "rho,temp,v_x", "rho,thermal_energy,mom_x", "2e7,0,0",
+          gradient_correction="true"), 8, 3),
+    ("gradcorr_rk4", ("reflect", "fixed", "periodic", "periodic"), "rk4",
+     VISC("boundary_global,local", "0.2,0.1", "v_y,bi_y", "mom_y,bi_y", "3e7,0", gradient_correction="true"), 6, 2),
 ]
 
 
