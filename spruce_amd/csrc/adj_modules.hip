@@ -934,6 +934,40 @@ __global__ void __launch_bounds__(256) k_adj_visc_vars(ViscTermArgs a, double* l
   }
 }
 
+struct HvTargets {
+  double* ld[4];
+  double w[4];
+  int n;
+};
+
+__global__ void __launch_bounds__(256) k_adj_hv_update(MhdSet P, int g, const double* init, const double* D, double cf,
+                                                       double hh, MhdConst K, Mesh M, double* lam, double* lam_init,
+                                                       HvTargets tg, double* partial, int accumulate) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  double dot = 0.0;
+  if (i < M.nx && j < M.ny) {
+    const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+    const double mask = inside(M, i, j) ? 1.0 : 0.0;
+    const double gn = init[c] + mask * hh * D[c];   // the update as k_visc_update forms it
+    const double rho_in = g == 0 ? gn : P.rho[c], te_in = g == 4 ? gn : P.te[c];
+    const double x = rho_in / K.m;
+    const double rho1 = smax(x, K.nmin) * K.m;
+    lam[c] *= (x < K.nmin || rho1 / K.m < K.nmin) ? 0.0 : 1.0;
+    lam[4 * N + c] *= te_in < K.temin ? 0.0 : 1.0;
+    const double lg = lam[(size_t)g * N + c];
+    lam[(size_t)g * N + c] = 0.0;
+    lam_init[c] += lg;
+    const double ldk = mask * hh * lg;
+    for (int t = 0; t < tg.n; ++t) tg.ld[t][c] += tg.w[t] * ldk;
+    dot = cf * mask * lg * D[c];
+  }
+  dot = block_sum(dot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const int b = blockIdx.y * gridDim.x + blockIdx.x;
+    partial[b] = accumulate ? partial[b] + dot : dot;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_adj_dt_grid(MhdSet L, MhdStatic st, MhdConst K, Mesh M, double* ldt,
                                                      double* lam_L) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
@@ -1019,6 +1053,16 @@ void launch_adj_visc_f(const ViscTermArgs& a, double* lf, double* lam_S, double*
 void launch_adj_visc_vars(const ViscTermArgs& a, double* lq, double* lam_S, hipStream_t s) {
   const size_t N = (size_t)a.mesh.nx * a.mesh.ny;
   hipLaunchKernelGGL(k_adj_visc_vars, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, a, lq, lam_S);
+}
+
+void launch_adj_hv_update(MhdSet P, int g, const double* init, const double* D, double c, double hh, MhdConst k,
+                          Mesh mesh, double* lam, double* lam_init, int nt, double* const ld[4], const double w[4],
+                          double* partial, int accumulate, hipStream_t s) {
+  HvTargets tg{};
+  tg.n = nt;
+  for (int t = 0; t < nt; ++t) { tg.ld[t] = ld[t]; tg.w[t] = w[t]; }
+  hipLaunchKernelGGL(k_adj_hv_update, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, P, g, init, D, c, hh, k, mesh, lam,
+                     lam_init, tg, partial, accumulate);
 }
 
 void launch_adj_dt_grid(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double* ldt, double* lam_L, hipStream_t s) {

@@ -98,6 +98,15 @@ void launch_adj_visc_f(const ViscTermArgs& a, double* lf, double* lam_S, double*
 // The Laplacian's adjoint lq of the differentiated variable a.var taken to the evolved grids of a.S
 // (idealmhd.cpp:241-277 reversed) on every cell, into lam_S; lq cleared.
 void launch_adj_visc_vars(const ViscTermArgs& a, double* lq, double* lam_S, hipStream_t s);
+// One update of a hyperviscous subcycle, g = init + (mask*hh)*D followed by a live propagate
+// (viscosity.cpp:133-173), reversed: P is the state before the update (its other grids and the
+// floors' inputs), g the evolved index updated, hh = c*h.  lam (adjoint of the propagated state)
+// goes through the density and energy floors; its g component moves to lam_init (added) and is
+// cleared; mask*hh times it is added, weighted by w[t], to each of the nt derivative adjoints ld[t];
+// partial (one per block; accumulated when `accumulate`) gets the sum of c * mask * lam_g * D.
+void launch_adj_hv_update(MhdSet P, int g, const double* init, const double* D, double c, double hh, MhdConst k,
+                          Mesh mesh, double* lam, double* lam_init, int nt, double* const ld[4], const double w[4],
+                          double* partial, int accumulate, hipStream_t s);
 // The adjoint of the per-cell dt grid of L (idealmhd.cpp:279-304) added to lam_L on the interior;
 // ldt cleared.
 void launch_adj_dt_grid(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double* ldt, double* lam_L, hipStream_t s);

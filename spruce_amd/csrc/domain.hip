@@ -801,6 +801,11 @@ class Domain : public DomainBase {
   static MhdSet set_at(double* p, size_t N) {
     return MhdSet{p, p + N, p + 2 * N, p + 3 * N, p + 4 * N, p + 5 * N, p + 6 * N, p + 7 * N, p + 8 * N, p + 9 * N};
   }
+  // evolved grid e (rho, mom_x, mom_y, mom_z, thermal_energy, bi_x, bi_y, bi_z) of a set
+  static double* evolved_of(const MhdSet& S, int e) {
+    double* const g[8] = {S.rho, S.mx, S.my, S.mz, S.te, S.bx, S.by, S.bz};
+    return g[e];
+  }
 
   // Discrete adjoint of nsteps steps (reverse of advance(): modules, stages, ghost fills,
   // minimums/derived, step size), with the evolved grids checkpointed every `ckpt` steps and each
@@ -1388,16 +1393,43 @@ struct ViscModule : GpuModule {
     DBuf* b[8] = {&L.rho, &L.mx, &L.my, &L.mz, &L.te, &L.bx, &L.by, &L.bz};
     return b[e]->p;
   }
+  // while recording (the adjoint's replay of a step), per hyperviscous term and subcycle, for each
+  // evaluation the live state it read (10 grids) and its dqdt, and with RK4 the combination after
+  // the fourth; hv_rec: per hyperviscous term, where its entries start, its subcycle count and size
+  bool recording = false;
+  std::vector<DBuf> tape;
+  size_t tape_n = 0;
+  struct HvRec { size_t start; int n; double h; };
+  std::vector<HvRec> hv_rec;
+  void record(Domain& d, const double* src, size_t count) {
+    if (tape_n >= tape.size()) tape.emplace_back();
+    if (tape[tape_n].n != count) tape[tape_n].alloc(count);
+    hip_check(hipMemcpyAsync(tape[tape_n].p, src, count * 8, hipMemcpyDeviceToDevice, d.stream), "record viscosity");
+    ++tape_n;
+  }
+  void record_state(Domain& d) {
+    if (tape_n >= tape.size()) tape.emplace_back();
+    if (tape[tape_n].n != 10 * d.N) tape[tape_n].alloc(10 * d.N);
+    Domain::copy_set(d.L.view(), Domain::set_at(tape[tape_n].p, d.N), d.N, d.stream);
+    ++tape_n;
+  }
   void iterate(Domain& d, double dt) override {  // viscosity.cpp:125-180
     nsub = -1;
+    if (recording) { tape_n = 0; hv_rec.clear(); }
     for (auto& t : terms) {
       if (!t.hyper()) continue;
       const int n = (int)(std::ceil(t.strength / hv_eps) + 0.1);
       require(n > 0, SPRUCE_E_ARG, "hyperviscous subcycle count must be positive");
       if (nsub < 0) nsub = n;
       const double h = dt / (double)n;
+      if (recording) hv_rec.push_back(HvRec{tape_n, n, h});
       double* g = evolved_ptr(d.L, t.evol_idx);
-      auto eval = [&](double* out) { refresh(d); term(d, t, d.L.view(), out, false); };
+      auto eval = [&](double* out) {
+        refresh(d);
+        if (recording) record_state(d);
+        term(d, t, d.L.view(), out, false);
+        if (recording) record(d, out, d.N);
+      };
       auto upd = [&](const double* base, const double* dd, double hh) {
         d.timed("visc_update", [&] { launch_visc_update(g, base, dd, hh, d.mesh, d.stream); });
         d.propagate_live();
@@ -1422,6 +1454,7 @@ struct ViscModule : GpuModule {
             upd(init.p, d3.p, h);
             eval(d4.p);
             launch_visc_rk4(d1.p, d2.p, d3.p, d4.p, dq.p, d.N, d.stream);
+            if (recording) record(d, dq.p, d.N);
             upd(init.p, dq.p, h);
           }
         }
@@ -2473,13 +2506,17 @@ void Domain::replay_step(int n, const MhdSet& start, StepTape& tp, const std::ve
   copy_set(start, L.view(), N, stream);
   hip_check(hipMemcpyAsync(step.p, history.p + n, sizeof(double), hipMemcpyDeviceToDevice, stream), "copy step");
   tstep = times[n];
-  for (auto& m : mods)
-    if (auto* tc = dynamic_cast<TcModule*>(m.get())) tc->recording = true;
+  auto set_recording = [&](bool on) {
+    for (auto& m : mods) {
+      if (auto* tc = dynamic_cast<TcModule*>(m.get())) tc->recording = on;
+      if (auto* vm = dynamic_cast<ViscModule*>(m.get())) vm->recording = on;
+    }
+  };
+  set_recording(true);
   rec = &tp;
   advance(n);
   rec = nullptr;
-  for (auto& m : mods)
-    if (auto* tc = dynamic_cast<TcModule*>(m.get())) tc->recording = false;
+  set_recording(false);
 }
 
 std::vector<double> Domain::param_grad() {
@@ -2522,9 +2559,8 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
       require(!rl->inactive, SPRUCE_E_UNSUPPORTED, "the adjoint covers radiative losses that are applied");
       it_mods.push_back((int)k);
     } else if (auto* vm = dynamic_cast<ViscModule*>(m)) {
-      require(!vm->any_hyper, SPRUCE_E_UNSUPPORTED,
-              "the adjoint covers artificial viscosity without hyperviscous terms (strength above 1)");
-      visc_mods.push_back(vm);
+      visc_mods.push_back(vm);                     // its right-hand-side terms
+      if (vm->any_hyper) it_mods.push_back((int)k);   // its subcycled hyperviscous terms
     } else if (dynamic_cast<ParamHeatModule*>(m) || dynamic_cast<HeatModule*>(m) ||
                dynamic_cast<LocalHeatModule*>(m) || dynamic_cast<MassInjModule*>(m) ||
                dynamic_cast<MomInjModule*>(m) || dynamic_cast<HeatSinkModule*>(m)) {
@@ -2556,7 +2592,9 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   const bool visc = !visc_mods.empty();
   const size_t modscratch = replay ? 10 * N + (size_t)ny + 2 * (size_t)nb + 2 + (any_sat ? 8 * N : 0) : 0;
   const size_t rk4scratch = rk4 ? 4 * EV + 2 * (size_t)nb : 0;
-  const size_t viscscratch = visc ? (rk4 ? 0 : EV) + 3 * N + (size_t)adj_mod_blocks(mesh) + 1 : 0;
+  bool any_hyper = false;
+  for (ViscModule* vm : visc_mods) any_hyper = any_hyper || vm->any_hyper;
+  const size_t viscscratch = visc ? (rk4 ? 0 : EV) + 3 * N + (size_t)adj_mod_blocks(mesh) + 1 + (any_hyper ? 5 * N : 0) : 0;
   const size_t total = (size_t)nseg * SET + (size_t)(sl + 1) * SET + seg_store + 3 * EV + 2 * CO + 4 * (size_t)nb +
                        modscratch + rk4scratch + viscscratch;
   DBuf slab;
@@ -2642,6 +2680,8 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   // artificial viscosity: the stage derivatives' adjoint (the RK4 buffer otherwise), the Laplacian's
   // scatter, the dt grid's adjoint, the global minimum's block sums and its adjoint
   double *vlq = nullptr, *vlf = nullptr, *vldt = nullptr, *vpart = nullptr, *vg = nullptr;
+  double* hv_init = nullptr;            // hyperviscous subcycles: the adjoint of the subcycle's start
+  double* hv_d[4] = {nullptr, nullptr, nullptr, nullptr};   // and of its evaluations
   if (visc) {
     if (!rk4) lka = take(EV);
     vlq = take(N);
@@ -2650,23 +2690,37 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
     vldt = take(N);
     vpart = take((size_t)adj_mod_blocks(mesh));
     vg = take(1);
+    if (any_hyper) {
+      hv_init = take(N);
+      for (auto& q : hv_d) q = take(N);
+    }
     launch_fill(vlq, 0.0, N, stream);
     launch_fill(vldt, 0.0, N, stream);
     launch_fill(vg, 0.0, 1, stream);
   }
+  // one term's reverse at S for the seed sa*ka + kb of component evol
+  auto visc_term_rev = [&](const ViscTermArgs& a, int evol, const double* ka, double sa, const double* kb,
+                           double* lamS) {
+    launch_adj_visc_term(a, evol, ka, sa, kb, vlq, vlf, lamS, vldt, a.dtgrid ? nullptr : vpart, stream);
+    if (!a.dtgrid) launch_adj_accum(vpart, adj_mod_blocks(mesh), 1.0, vg, stream);
+    if (a.grad) {
+      launch_adj_visc_f(a, vlf, lamS, vldt, a.dtgrid ? nullptr : vpart, stream);
+      if (!a.dtgrid) launch_adj_accum(vpart, adj_mod_blocks(mesh), 1.0, vg, stream);
+    }
+    launch_adj_visc_vars(a, vlq, lamS, stream);
+  };
+  // the timescales' adjoint (dt grid of S, per cell, and its minimum) taken to S
+  auto visc_dt_rev = [&](const MhdSet& S, double* lamS) {
+    launch_adj_dt_grid(S, stat(), kc, mesh, vldt, lamS, stream);
+    launch_adj_dt(S, stat(), kc, mesh, 1.0, nullptr, nullptr, vg, pmin, pidx, lamS, stream);
+    launch_fill(vg, 0.0, 1, stream);
+  };
   auto visc_rev = [&](const MhdSet& S, const double* ka, double sa, const double* kb, double* lamS) {
     for (ViscModule* vm : visc_mods)
       for (auto& t : vm->terms) {
+        if (t.hyper()) continue;
         const ViscTermArgs a = vm->term_args(*this, t, S);
-        timed("adj_visc", [&] {
-          launch_adj_visc_term(a, t.evol_idx, ka, sa, kb, vlq, vlf, lamS, vldt, a.dtgrid ? nullptr : vpart, stream);
-          if (!a.dtgrid) launch_adj_accum(vpart, adj_mod_blocks(mesh), 1.0, vg, stream);
-          if (a.grad) {
-            launch_adj_visc_f(a, vlf, lamS, vldt, a.dtgrid ? nullptr : vpart, stream);
-            if (!a.dtgrid) launch_adj_accum(vpart, adj_mod_blocks(mesh), 1.0, vg, stream);
-          }
-          launch_adj_visc_vars(a, vlq, lamS, stream);
-        });
+        timed("adj_visc", [&] { visc_term_rev(a, t.evol_idx, ka, sa, kb, lamS); });
       }
   };
   if ((int)history.n < nsteps + 1) history.alloc(nsteps + 1);
@@ -2857,11 +2911,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
       }
       timed("adj_faces", [&] { launch_adj_faces(A2, st, kc, mesh, cx, cy, lamN, stream); });
       if (visc) {   // the viscosity coefficient's timescale: the dt grid of A2 (local) or its minimum
-        timed("adj_visc", [&] {
-          launch_adj_dt_grid(A2, st, kc, mesh, vldt, lamN, stream);
-          launch_adj_dt(A2, st, kc, mesh, 1.0, nullptr, nullptr, vg, pmin, pidx, lamN, stream);
-          launch_fill(vg, 0.0, 1, stream);
-        });
+        timed("adj_visc", [&] { visc_dt_rev(A2, lamN); });
       }
       std::swap(lam, lamN);
       for (int k = nit - 1; k >= 0; --k) {                 // iterating modules, last first
@@ -2940,6 +2990,57 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
             }
             if (acc) launch_adj_accum(partm, adj_tc_blocks(mesh), 1.0 / (double)ns, gh, stream);
             launch_adj_bhat(Ln, st, mesh, lamfz, lam, stream);
+          });
+        } else if (auto* vm = dynamic_cast<ViscModule*>(m)) {
+          // hyperviscous subcycles (viscosity.cpp:125-180), last term and last subcycle first: each
+          // update g = init + mask*(c*h)*D and its propagate reversed (launch_adj_hv_update), then
+          // the evaluation that produced D's newest derivative at the state it read, with that
+          // state's timescales recomputed
+          const int K = vm->hv_integ == "euler" ? 1 : (vm->hv_integ == "rk2" ? 2 : 4);
+          const size_t per = vm->hv_integ == "rk4" ? 9 : (size_t)(2 * K);
+          const double cs[4] = {K == 1 ? 1.0 : 0.5, K == 2 ? 1.0 : 0.5, 1.0, 1.0};
+          timed("adj_visc", [&] {
+            int slot = (int)vm->hv_rec.size();
+            for (int ti = (int)vm->terms.size() - 1; ti >= 0; --ti) {
+              auto& t = vm->terms[ti];
+              if (!t.hyper()) continue;
+              const auto& hr = vm->hv_rec[--slot];
+              const int g = t.evol_idx;
+              // the state after this term's subcycles: the next term's first recorded state, or the module's output
+              const MhdSet out_state =
+                  slot + 1 < (int)vm->hv_rec.size() ? set_at(vm->tape[vm->hv_rec[slot + 1].start].p, N) : post;
+              int acc = 0;
+              for (int sc = hr.n - 1; sc >= 0; --sc) {
+                const size_t b = hr.start + (size_t)sc * per;
+                auto S_at = [&](int k) { return set_at(vm->tape[b + 2 * k].p, N); };   // read by evaluation k+1
+                const MhdSet Sq = S_at(0);
+                const double* init = evolved_of(Sq, g);
+                launch_fill(hv_init, 0.0, N, stream);
+                for (int q = 0; q < K; ++q) launch_fill(hv_d[q], 0.0, N, stream);
+                for (int k = K; k >= 1; --k) {
+                  const MhdSet after = k < K ? S_at(k) : (sc + 1 < hr.n ? set_at(vm->tape[b + per].p, N) : out_state);
+                  const MhdSet before = S_at(k - 1);
+                  adj_ghost(after, lam);
+                  const bool comb = K == 4 && k == 4;        // RK4: (d1 + 2 d2 + 2 d3 + d4) / 6
+                  const double* D = comb ? vm->tape[b + 8].p : vm->tape[b + 2 * (k - 1) + 1].p;
+                  double* tg[4] = {hv_d[0], hv_d[1], hv_d[2], hv_d[3]};
+                  const double w4[4] = {1.0 / 6.0, 2.0 / 6.0, 2.0 / 6.0, 1.0 / 6.0};
+                  double* one[4] = {hv_d[k - 1], nullptr, nullptr, nullptr};
+                  const double w1[4] = {1.0, 0.0, 0.0, 0.0};
+                  const double c = cs[k - 1];
+                  launch_adj_hv_update(before, g, init, D, c, c == 1.0 ? hr.h : 0.5 * hr.h, kc, mesh, lam, hv_init,
+                                       comb ? 4 : 1, comb ? tg : one, comb ? w4 : w1, partm, acc, stream);
+                  acc = 1;
+                  // evaluation k at `before`: its timescales, then the term's reverse for hv_d[k-1]
+                  launch_mhd_dt(before, st, kc, mesh, vm->dtg.p, vm->part.p, stream);
+                  launch_reduce_step(vm->part.p, dt_blocks, 1.0, vm->dtmin.p, nullptr, 0, err_flag, stream);
+                  visc_term_rev(vm->term_args(*this, t, before), 0, hv_d[k - 1], 1.0, nullptr, lam);
+                  visc_dt_rev(before, lam);
+                }
+                launch_add(lam + (size_t)g * N, hv_init, N, stream);   // init is the subcycle start's g
+              }
+              if (acc) launch_adj_accum(partm, nb, 1.0 / (double)hr.n, gh, stream);
+            }
           });
         } else {
           auto* rl = static_cast<RlModule*>(m);

@@ -3,8 +3,9 @@ differences of the GPU forward.
 
 Covered: the ghost-fill reverse of fixed, reflect, open and open_ucnp boundaries; thermal conduction (Euler,
 RK2 and RK4 subcycles, with and without flux saturation), radiative losses (with prevent_subcycling too), the
-artificial viscosity as a right-hand-side term (local, global and boundary strengths, any
-differentiated variable, gradient correction), the source modules after the integrator (localized heating with its time ramp and window, mass and
+artificial viscosity (right-hand-side terms with local, global and boundary strengths, any
+differentiated variable and the gradient correction; hyperviscous terms under each subcycle
+integrator), the source modules after the integrator (localized heating with its time ramp and window, mass and
 momentum injection, oscillating too, the ambient heating sink) and the parametrised heating
 profile of the gradient workload, whose coefficients get their own gradient
 (spruce_gpu_adjoint_params).  The loss
@@ -190,6 +191,16 @@ VISC_CASES = [
           gradient_correction="true"), 8, 3),
     ("gradcorr_rk4", ("reflect", "fixed", "periodic", "periodic"), "rk4",
      VISC("boundary_global,local", "0.2,0.1", "v_y,bi_y", "mom_y,bi_y", "3e7,0", gradient_correction="true"), 6, 2),
+    # hyperviscous terms (strength above 1): subcycled after the iterating modules, each evaluation
+    # taking the dt grid of the live state it reads, with each subcycle integrator
+    ("hyper_euler", ("periodic", "periodic", "fixed", "reflect"), "rk2",
+     VISC("local,global", "2.5,0.1", "v_y,bi_x", "mom_y,bi_x", hv_time_integrator="euler", hv_epsilon="1.0"), 6, 3),
+    ("hyper_rk2", ("periodic", "periodic", "fixed", "open"), "euler",
+     VISC("local,global", "2.5,1.5", "temp,rho", "thermal_energy,rho", hv_time_integrator="rk2", hv_epsilon="1.0"),
+     6, 3),
+    ("hyper_rk4_grad", ("reflect", "fixed", "periodic", "periodic"), "rk4",
+     VISC("boundary,local", "2.0,1.5", "rho,v_x", "rho,mom_x", "3e7,0", hv_time_integrator="rk4", hv_epsilon="0.8",
+          gradient_correction="true"), 4, 2),
 ]
 
 
@@ -376,8 +387,9 @@ def test_module_adjoint_leaves_the_same_final_state_as_stepping(mods):
 
 def test_module_adjoint_refuses_what_it_does_not_cover():
     st = _arcade()
-    visc = VISC("local", "2.5", "v_x", "mom_x")      # hyperviscous: subcycled, not covered
-    for mods in ([visc], [("physical_viscosity", [("coeff", "0.1")])]):
+    fheat = ("field_heating", [("coeff", "3e3"), ("current_pow", "1.0"), ("b_pow", "0.5"), ("n_pow", "-0.5"),
+                               ("roc_pow", "0.5")])
+    for mods in ([fheat], [("physical_viscosity", [("coeff", "0.1")])]):
         d = GpuDomain(spec_from_files(_arcade_cfg(THETA, mods=mods), st))
         try:
             with pytest.raises(SpruceError):
