@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libspruce_gpu.so")
-SOURCES = ["domain.hip", "outputs.hip", "mhd.hip", "mhd2.hip", "solar.hip", "twofluid.hip", "tf_host.hip", "visc.hip", "e2.hip", "e2_host.hip", "pvisc.hip", "adjoint.hip", "sources.hip", "adj_modules.hip", "moc.hip"]
+SOURCES = ["domain.hip", "outputs.hip", "mhd.hip", "mhd2.hip", "solar.hip", "twofluid.hip", "tf_host.hip", "visc.hip", "e2.hip", "e2_host.hip", "pvisc.hip", "adjoint.hip", "sources.hip", "adj_modules.hip", "adj_pvisc.hip", "moc.hip"]
 
 # -ffp-contract=off: no implicit multiply-add fusion (the reference rounds every product).
 # -packed-fp32-ops off: packed FP32 instructions are not used on this hardware pool.

@@ -28,6 +28,7 @@
 #include "sources.hpp"
 #include "adjoint.hpp"
 #include "adj_modules.hpp"
+#include "adj_pvisc.hpp"
 #include "outputs.hpp"
 #include "moc.hpp"
 #include "crpow.hpp"
@@ -1544,6 +1545,22 @@ struct PvModule : GpuModule {
   DBuf coef, bhx, bhy, bhz, part, tsmin;
   DBuf avg_heat, avg_f[3];   // output_to_file: subcycle averages of the heating and the force
   DBuf sa[8], sb[8], sh[8], dv[7];
+  // while recording (the adjoint's replay of a step): per subcycle the evaluation sets (8 grids
+  // each): Euler: its start; RK2: its start and half step; and the subcycle size
+  bool recording = false;
+  std::vector<DBuf> tape;
+  double h_last = 0.0;
+  void record(Domain& d, size_t idx, const PvSet& src) {
+    while (tape.size() <= idx) tape.emplace_back();
+    if (tape[idx].n != 8 * d.N) tape[idx].alloc(8 * d.N);
+    const double* g[8] = {src.te, src.T, src.mx, src.my, src.mz, src.vx, src.vy, src.vz};
+    for (int k = 0; k < 8; ++k)
+      hip_check(hipMemcpyAsync(tape[idx].p + (size_t)k * d.N, g[k], d.N * 8, hipMemcpyDeviceToDevice, d.stream),
+                "record physical viscosity");
+  }
+  static PvSet set_of(double* p, size_t N) {
+    return PvSet{p, p + N, p + 2 * N, p + 3 * N, p + 4 * N, p + 5 * N, p + 6 * N, p + 7 * N};
+  }
   explicit PvModule(const std::vector<std::pair<std::string, std::string>>& o) {  // :17-34
     name = "physical_viscosity";
     for (auto& kv : o) {
@@ -1630,6 +1647,7 @@ struct PvModule : GpuModule {
     require(m < DBL_MAX, SPRUCE_E_NUMERIC, "Something went wrong in the min calculation (physical viscosity)");
     nsub = subcycle_count(dt, epsilon * m);
     const double h = dt / nsub;
+    h_last = h;
     PvDerivs dd{dv[0].p, dv[1].p, dv[2].p, dv[3].p, dv[4].p, dv[5].p, dv[6].p};
     PvArgs a{};
     a.d = dd; a.f = fz; a.k = d.kc; a.mesh = d.mesh; a.h = h;
@@ -1645,6 +1663,7 @@ struct PvModule : GpuModule {
     DBuf* cur = sa;
     DBuf* nxt = sb;
     for (int s = 0; s < nsub; ++s) {
+      if (recording) record(d, integ == "euler" ? (size_t)s : 2 * (size_t)s, view(cur));
       d.timed("pv_derivs", [&] { launch_pv_derivs(view(cur), d.mesh, dd, d.stream); });
       if (integ == "euler") {
         a.eval = view(cur); a.base = view(cur); a.out = view(nxt); a.mode = PV_FULL;
@@ -1652,6 +1671,7 @@ struct PvModule : GpuModule {
       } else {
         a.eval = view(cur); a.base = view(cur); a.out = view(sh); a.mode = PV_HALF;
         d.timed("pv_apply", [&] { launch_pv_apply(a, d.stream); });
+        if (recording) record(d, 2 * (size_t)s + 1, view(sh));
         d.timed("pv_derivs", [&] { launch_pv_derivs(view(sh), d.mesh, dd, d.stream); });
         a.eval = view(sh); a.base = view(cur); a.out = view(nxt); a.mode = PV_FULL;
         d.timed("pv_apply", [&] { launch_pv_apply(a, d.stream); });
@@ -2510,6 +2530,7 @@ void Domain::replay_step(int n, const MhdSet& start, StepTape& tp, const std::ve
     for (auto& m : mods) {
       if (auto* tc = dynamic_cast<TcModule*>(m.get())) tc->recording = on;
       if (auto* vm = dynamic_cast<ViscModule*>(m.get())) vm->recording = on;
+      if (auto* pv = dynamic_cast<PvModule*>(m.get())) pv->recording = on;
     }
   };
   set_recording(true);
@@ -2547,7 +2568,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   // parametrised heating and the source modules after the integrator
   std::vector<int> it_mods, post_mods;
   std::vector<ViscModule*> visc_mods;   // right-hand-side terms of every stage
-  bool any_sat = false;
+  bool any_sat = false, any_pv = false;
   rec_post.assign(mods.size(), false);
   for (size_t k = 0; k < mods.size(); ++k) {
     GpuModule* m = mods[k].get();
@@ -2558,6 +2579,10 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
     } else if (auto* rl = dynamic_cast<RlModule*>(m)) {
       require(!rl->inactive, SPRUCE_E_UNSUPPORTED, "the adjoint covers radiative losses that are applied");
       it_mods.push_back((int)k);
+    } else if (auto* pv = dynamic_cast<PvModule*>(m)) {
+      it_mods.push_back((int)k);
+      any_pv = true;
+      (void)pv;
     } else if (auto* vm = dynamic_cast<ViscModule*>(m)) {
       visc_mods.push_back(vm);                     // its right-hand-side terms
       if (vm->any_hyper) it_mods.push_back((int)k);   // its subcycled hyperviscous terms
@@ -2595,8 +2620,11 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   bool any_hyper = false;
   for (ViscModule* vm : visc_mods) any_hyper = any_hyper || vm->any_hyper;
   const size_t viscscratch = visc ? (rk4 ? 0 : EV) + 3 * N + (size_t)adj_mod_blocks(mesh) + 1 + (any_hyper ? 5 * N : 0) : 0;
+  // physical viscosity: the forward derivatives of an evaluation set (7), the scatter accumulators
+  // (velocity derivatives 6, T^2.5, b_hat 3, velocities 3, T) and an RK2 subcycle's two adjoint sets (4 + 4)
+  const size_t pvscratch = any_pv ? 29 * N : 0;
   const size_t total = (size_t)nseg * SET + (size_t)(sl + 1) * SET + seg_store + 3 * EV + 2 * CO + 4 * (size_t)nb +
-                       modscratch + rk4scratch + viscscratch;
+                       modscratch + rk4scratch + viscscratch + pvscratch;
   DBuf slab;
   slab.alloc(total);
   adj_bytes = (int64_t)(total * sizeof(double));
@@ -2715,6 +2743,11 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
     launch_adj_dt(S, stat(), kc, mesh, 1.0, nullptr, nullptr, vg, pmin, pidx, lamS, stream);
     launch_fill(vg, 0.0, 1, stream);
   };
+  double* pvs = nullptr;
+  if (any_pv) {
+    pvs = take(29 * N);
+    launch_fill(pvs, 0.0, 29 * N, stream);
+  }
   auto visc_rev = [&](const MhdSet& S, const double* ka, double sa, const double* kb, double* lamS) {
     for (ViscModule* vm : visc_mods)
       for (auto& t : vm->terms) {
@@ -2990,6 +3023,64 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
             }
             if (acc) launch_adj_accum(partm, adj_tc_blocks(mesh), 1.0 / (double)ns, gh, stream);
             launch_adj_bhat(Ln, st, mesh, lamfz, lam, stream);
+          });
+        } else if (auto* pv = dynamic_cast<PvModule*>(m)) {
+          // physical viscosity (physicalviscosity.cpp:148-250): its subcycles reversed, last first;
+          // the thermal energy and momenta of lam carry the subcycle states' adjoint
+          if (!pv->inactive && pv->nsub > 0) timed("adj_pvisc", [&] {
+            const bool e = pv->integ == "euler";
+            const PvFrozen fz{pv->bhx.p, pv->bhy.p, pv->bhz.p, pv->coef.p, pre.n, pre.rho};
+            double* q = pvs;
+            PvDerivs pd{q, q + N, q + 2 * N, q + 3 * N, q + 4 * N, q + 5 * N, q + 6 * N};
+            q += 7 * N;
+            PvLam Lm{};
+            for (int k = 0; k < 6; ++k) Lm.l_d[k] = q + (size_t)k * N;
+            Lm.l_t25 = q + 6 * N;
+            for (int k = 0; k < 3; ++k) Lm.l_bh[k] = q + (size_t)(7 + k) * N;
+            for (int k = 0; k < 3; ++k) Lm.l_v[k] = q + (size_t)(10 + k) * N;
+            Lm.l_T = q + 13 * N;
+            double* lH = q + 14 * N;     // te, mx, my, mz of the half step
+            double* lB = q + 18 * N;     // te, mx, my, mz of the subcycle start (RK2)
+            Lm.partial = partm;
+            double* lte = lam + 4 * N;
+            double* lm3[3] = {lam + N, lam + 2 * N, lam + 3 * N};
+            PvArgs a{};
+            a.d = pd; a.f = fz; a.k = kc; a.mesh = mesh; a.h = pv->h_last;
+            a.heating_on = pv->heating_on ? 1 : 0; a.force_on = pv->force_on ? 1 : 0; a.apply = 1;
+            a.coeff_zero = pv->coeff == 0.0 ? 1 : 0; a.grad_corr = pv->grad_corr ? 1 : 0;
+            int acc = 0;
+            // one update reversed: output adjoint (lo_te, lo_m) into the base's (mode 0/1/2), then the
+            // evaluation's derivatives and derived variables onto (out_te, out_m) (assign or add)
+            auto update_rev = [&](const PvSet& E, const PvSet& B, int mode, const double* lo_te, double* const lo_m[3],
+                                  double* lb_te, double* const lb_m[3], int base_mode, double* out_te,
+                                  double* const out_m[3], int assign) {
+              launch_pv_derivs(E, mesh, pd, stream);
+              a.eval = E; a.base = B; a.mode = mode;
+              Lm.lo_te = lo_te; Lm.lb_te = lb_te; Lm.base_mode = base_mode; Lm.accumulate = acc;
+              for (int k = 0; k < 3; ++k) { Lm.lo_m[k] = lo_m[k]; Lm.lb_m[k] = lb_m[k]; }
+              launch_adj_pv_apply(a, Lm, stream);
+              acc = 1;
+              launch_adj_pv_derivs(E, mesh, Lm, stream);
+              launch_adj_pv_derived(E, fz, kc, N, Lm, out_te, out_m[0], out_m[1], out_m[2], assign, lam, stream);
+            };
+            double* lHm[3] = {lH + N, lH + 2 * N, lH + 3 * N};
+            double* lBm[3] = {lB + N, lB + 2 * N, lB + 3 * N};
+            for (int sc = pv->nsub - 1; sc >= 0; --sc) {
+              if (e) {
+                const PvSet E = PvModule::set_of(pv->tape[(size_t)sc].p, N);
+                update_rev(E, E, PV_FULL, lte, lm3, lte, lm3, 0, lte, lm3, 0);
+              } else {
+                const PvSet E = PvModule::set_of(pv->tape[2 * (size_t)sc].p, N);
+                const PvSet H = PvModule::set_of(pv->tape[2 * (size_t)sc + 1].p, N);
+                update_rev(H, E, PV_FULL, lte, lm3, lB, lBm, 1, lH, lHm, 1);
+                update_rev(E, E, PV_HALF, lH, lHm, lB, lBm, 2, lB, lBm, 0);
+                hip_check(hipMemcpyAsync(lte, lB, N * 8, hipMemcpyDeviceToDevice, stream), "copy adjoint");
+                for (int k = 0; k < 3; ++k)
+                  hip_check(hipMemcpyAsync(lm3[k], lBm[k], N * 8, hipMemcpyDeviceToDevice, stream), "copy adjoint");
+              }
+            }
+            launch_adj_pv_bhat(pre, st, N, Lm, lam, stream);
+            launch_adj_accum(partm, nb, 1.0 / (double)pv->nsub, gh, stream);
           });
         } else if (auto* vm = dynamic_cast<ViscModule*>(m)) {
           // hyperviscous subcycles (viscosity.cpp:125-180), last term and last subcycle first: each

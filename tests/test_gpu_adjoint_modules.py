@@ -5,7 +5,7 @@ Covered: the ghost-fill reverse of fixed, reflect, open and open_ucnp boundaries
 RK2 and RK4 subcycles, with and without flux saturation), radiative losses (with prevent_subcycling too), the
 artificial viscosity (right-hand-side terms with local, global and boundary strengths, any
 differentiated variable and the gradient correction; hyperviscous terms under each subcycle
-integrator), the source modules after the integrator (localized heating with its time ramp and window, mass and
+integrator), physical viscosity (heating and force, Euler and RK2 subcycles, gradient correction), the source modules after the integrator (localized heating with its time ramp and window, mass and
 momentum injection, oscillating too, the ambient heating sink) and the parametrised heating
 profile of the gradient workload, whose coefficients get their own gradient
 (spruce_gpu_adjoint_params).  The loss
@@ -204,6 +204,22 @@ VISC_CASES = [
 ]
 
 
+def PV(coeff, **extra):
+    return ("physical_viscosity", [("coeff", coeff)] + list(extra.items()))
+
+
+VISC_CASES += [
+    # physical (Braginskii) viscosity: heating and force through the velocity derivatives, T^2.5 and
+    # the frozen b_hat of the cell and its neighbours, subcycled with Euler or RK2
+    ("pv_euler", ("periodic", "periodic", "fixed", "reflect"), "rk2", PV("5e-17", epsilon="0.15"), 6, 3),
+    ("pv_rk2_gc_ramp", ("periodic", "periodic", "fixed", "open"), "euler",
+     PV("5e-17", ramp_length="1e8", gradient_correction="true", time_integrator="rk2", epsilon="0.1"), 6, 3),
+    ("pv_force_only_rk4", ("reflect", "fixed", "periodic", "periodic"), "rk4",
+     PV("5e-17", heating_on="false", epsilon="0.3"), 4, 2),
+    ("pv_heat_only_rk2", ("open_ucnp",) * 4, "rk2", PV("1e-16", force_on="false", time_integrator="rk2", epsilon="0.3"), 6, 3),
+]
+
+
 @pytest.mark.parametrize("name,bc,integ,visc,nsteps,ckpt", VISC_CASES, ids=[v[0] for v in VISC_CASES])
 def test_viscosity_adjoint_matches_finite_differences(name, bc, integ, visc, nsteps, ckpt):
     print("adjoint viscosity", name, flush=True)
@@ -389,7 +405,7 @@ def test_module_adjoint_refuses_what_it_does_not_cover():
     st = _arcade()
     fheat = ("field_heating", [("coeff", "3e3"), ("current_pow", "1.0"), ("b_pow", "0.5"), ("n_pow", "-0.5"),
                                ("roc_pow", "0.5")])
-    for mods in ([fheat], [("physical_viscosity", [("coeff", "0.1")])]):
+    for mods in ([fheat], [("div_cleaning", [("epsilon", "0.1"), ("time_scale", "0.01")])]):
         d = GpuDomain(spec_from_files(_arcade_cfg(THETA, mods=mods), st))
         try:
             with pytest.raises(SpruceError):
