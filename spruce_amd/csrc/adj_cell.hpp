@@ -88,10 +88,11 @@ __host__ __device__ inline void adj_cell_coeffs(const MhdSet& S, const MhdStatic
 }
 
 // Reverse of compute_face<AX> for the face between cells c-1 and c ((i, j) is cell c), given the
-// face adjoint fa[ADJ_NCOEF].  add(cell_index, var, value) accumulates into the adjoint state.
+// face adjoint fa[ADJ_NCOEF].  add(slot, var, value) accumulates into the adjoint of cell
+// c-2+slot along the axis (slot 0..3: c-2, c-1, c, c+1).
 template <int AX, class ADD>
-__host__ __device__ inline void adj_face(const MhdSet& S, const MhdStatic& st, const MhdConst& K, const Mesh& M,
-                                         int i, int j, const double fa[ADJ_NCOEF], ADD&& add) {
+__host__ __device__ inline void adj_face_slots(const MhdSet& S, const MhdStatic& st, const MhdConst& K, const Mesh& M,
+                                               int i, int j, const double fa[ADJ_NCOEF], ADD&& add) {
   const Axis& ax = AX == 0 ? M.x : M.y;
   const int c = AX == 0 ? i : j;
   const int cm2 = wrapi(c - 2, ax.n, ax.per), cm1 = wrapi(c - 1, ax.n, ax.per), cp1 = wrapi(c + 1, ax.n, ax.per);
@@ -135,8 +136,8 @@ __host__ __device__ inline void adj_face(const MhdSet& S, const MhdStatic& st, c
     const int v = var_of[q];
     if (v < 0) continue;
     const double gs = fa[q] * vf;
-    const size_t ia = pos ? i0 : i3, ib = pos ? i1 : i2;
-    if (sel == 0) { add(i1, v, gs * wlo); add(i2, v, gs * wup); }
+    const int ia = pos ? 0 : 3, ib = pos ? 1 : 2;
+    if (sel == 0) { add(1, v, gs * wlo); add(2, v, gs * wup); }
     else if (sel == 1) { add(ia, v, gs * (1.0 - t)); add(ib, v, gs * t); }
     else add(ib, v, gs);
   }
@@ -145,18 +146,30 @@ __host__ __device__ inline void adj_face(const MhdSet& S, const MhdStatic& st, c
   giv[AX] += g_vf;
   for (int k = 0; k < 3; ++k) {
     const double g1 = giv[k] * wlo, g2 = giv[k] * wup;
-    add(i1, AV_MX + k, g1 / r1);
-    add(i1, AV_RHO, -g1 * v1[k] / r1);
-    add(i2, AV_MX + k, g2 / r2);
-    add(i2, AV_RHO, -g2 * v2[k] / r2);
+    add(1, AV_MX + k, g1 / r1);
+    add(1, AV_RHO, -g1 * v1[k] / r1);
+    add(2, AV_MX + k, g2 / r2);
+    add(2, AV_RHO, -g2 * v2[k] / r2);
   }
-  add(i1, AV_TE, fa[11] * wlo * K.gm1);
-  add(i2, AV_TE, fa[11] * wup * K.gm1);
+  add(1, AV_TE, fa[11] * wlo * K.gm1);
+  add(2, AV_TE, fa[11] * wup * K.gm1);
   const int bfirst = AX == 0 ? AV_BY : AV_BX;        // x faces carry bi_y, y faces bi_x
-  add(i1, bfirst, fa[15] * wlo);
-  add(i2, bfirst, fa[15] * wup);
-  add(i1, AV_BZ, fa[16] * wlo);
-  add(i2, AV_BZ, fa[16] * wup);
+  add(1, bfirst, fa[15] * wlo);
+  add(2, bfirst, fa[15] * wup);
+  add(1, AV_BZ, fa[16] * wlo);
+  add(2, AV_BZ, fa[16] * wup);
+}
+
+// adj_face_slots with add(cell_index, var, value) on the grid's cell indices.
+template <int AX, class ADD>
+__host__ __device__ inline void adj_face(const MhdSet& S, const MhdStatic& st, const MhdConst& K, const Mesh& M,
+                                         int i, int j, const double fa[ADJ_NCOEF], ADD&& add) {
+  const Axis& ax = AX == 0 ? M.x : M.y;
+  const int c = AX == 0 ? i : j;
+  auto at = [&](int k) -> size_t { return AX == 0 ? (size_t)k * M.ny + j : (size_t)i * M.ny + k; };
+  const size_t cell[4] = {at(wrapi(c - 2, ax.n, ax.per)), at(wrapi(c - 1, ax.n, ax.per)), at(c),
+                          at(wrapi(c + 1, ax.n, ax.per))};
+  adj_face_slots<AX>(S, st, K, M, i, j, fa, [&](int slot, int v, double val) { add(cell[slot], v, val); });
 }
 
 // Forward-mode number for the gradient of the per-cell step size with respect to the cell's 8

@@ -218,13 +218,25 @@ __device__ __forceinline__ void tcr_flush(const double* acc, const Mesh& M, int 
     else atomicAdd(&lam_T[g], v);
   }
 }
-__device__ __forceinline__ void tcr_temps(const double* te, const double* n, const MhdConst& K, const Mesh& M, int i,
-                                          int j, double T[3][3]) {
-  for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b) {
-      const size_t q = (size_t)wrp(i - 1 + a, M.nx, M.x.per) * M.ny + wrp(j - 1 + b, M.ny, M.y.per);
-      T[a][b] = temp_of(te[q], n[q], K);
+// The temperatures of the block's tile and its one-cell ring, each computed once (a division) into
+// LDS; ring cells outside a closed grid are never read (interior cells have all neighbours inside).
+__device__ __forceinline__ void tcr_load_temps(const double* te, const double* n, const MhdConst& K, const Mesh& M,
+                                               int i0, int j0, double* Ts) {
+  for (int t = threadIdx.y * 64 + threadIdx.x; t < TCR_N; t += 256) {
+    const int a = t / TCR_W, b = t - a * TCR_W;
+    const int ii = wrp(i0 - 1 + a, M.nx, M.x.per), jj = wrp(j0 - 1 + b, M.ny, M.y.per);
+    double v = 0.0;
+    if (ii >= 0 && ii < M.nx && jj >= 0 && jj < M.ny) {
+      const size_t q = (size_t)ii * M.ny + jj;
+      v = temp_of(te[q], n[q], K);
     }
+    Ts[t] = v;
+  }
+}
+// the 3x3 temperatures of tile cell (li, lj) from the LDS tile
+__device__ __forceinline__ void tcr_temps(const double* Ts, int li, int lj, double T[3][3]) {
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) T[a][b] = Ts[(li + a) * TCR_W + lj + b];
 }
 __device__ __forceinline__ void tcr_partial(double dot, double* partial, int accumulate) {
   dot = block_sum(dot);
@@ -263,14 +275,17 @@ __device__ __forceinline__ void tc_sat_zero(const TcSatRev& sr, size_t c) {
   if (sr.coef) tc_sat_out(sr, c, 0.0, 0.0, 0.0, 0.0);
 }
 
+template <bool SAT>
 __global__ void __launch_bounds__(256) k_adj_tc_sub(const double* te_s, const double* n, const double* bhx,
                                                     const double* bhy, const double* dbxx, const double* dbxy,
                                                     const double* dbyx, const double* dbyy, double h, double k0,
                                                     MhdConst K, Mesh M, double* lam_te, double* lam_T,
                                                     double* lam_fz, double* partial, int accumulate, TcSatRev sr) {
   __shared__ double acc[TCR_N];
+  __shared__ double Ts[TCR_N];
   const int i0 = blockIdx.y * TCR_ROWS, j0 = blockIdx.x * TCR_COLS;
   tcr_zero(acc);
+  tcr_load_temps(te_s, n, K, M, i0, j0, Ts);
   __syncthreads();
   double dot = 0.0;
   do {
@@ -281,11 +296,11 @@ __global__ void __launch_bounds__(256) k_adj_tc_sub(const double* te_s, const do
     const double base = te_s[c];
     if (!inside(M, i, j)) {
       lam_te[c] *= (base + 0.0 < K.temin) ? 0.0 : 1.0;
-      tc_sat_zero(sr, c);
+      if (SAT) tc_sat_zero(sr, c);
       continue;
     }
     double T[3][3];
-    tcr_temps(te_s, n, K, M, i, j, T);
+    tcr_temps(Ts, li, lj, T);
     // the clamp of the update needs dE first; its reverse is linear in the seed, so it runs with
     // a unit seed and is scaled afterwards
     double gT[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
@@ -293,13 +308,13 @@ __global__ void __launch_bounds__(256) k_adj_tc_sub(const double* te_s, const do
     const double R = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0, 1.0,
                                           gT, gfz);
     double cc = 1.0, dcx = 0.0, dcy = 0.0;
-    const double dE = sr.coef ? tc_sat_apply(sr, M, i, j, R, cc, dcx, dcy) : R;
+    const double dE = SAT ? tc_sat_apply(sr, M, i, j, R, cc, dcx, dcy) : R;
     const double v = base + 1.0 * (1.0 * h) * dE;
     const double lv = (v < K.temin) ? 0.0 : lam_te[c];
     lam_te[c] = lv;
     dot += lv * dE;
     const double ld = lv * h;
-    if (sr.coef) tc_sat_out(sr, c, ld, R, dcx, dcy);
+    if (SAT) tc_sat_out(sr, c, ld, R, dcx, dcy);
     if (ld != 0.0) {
       tcr_add(acc, li, lj, gT, ld * cc);
       for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += (ld * cc) * gfz[k];
@@ -315,6 +330,7 @@ __global__ void __launch_bounds__(256) k_adj_tc_sub(const double* te_s, const do
 // adjoint lam_out is reversed through the clamp into lam_base (assigned or added) and seeds dE with
 // (mult*h)*mu; seed_add (may be null) adds add_coef * seed_add[c] to that seed (the weights a later
 // RK4 combination gives this stage's dE).
+template <bool SAT>
 __global__ void __launch_bounds__(256) k_adj_tc_stage(const double* eval, const double* n, const double* bhx,
                                                       const double* bhy, const double* dbxx, const double* dbxy,
                                                       const double* dbyx, const double* dbyy, double h, double mult,
@@ -323,8 +339,10 @@ __global__ void __launch_bounds__(256) k_adj_tc_stage(const double* eval, const 
                                                       const double* seed_add, double add_coef, double* lam_T,
                                                       double* lam_fz, double* partial, int accumulate, TcSatRev sr) {
   __shared__ double acc[TCR_N];
+  __shared__ double Ts[TCR_N];
   const int i0 = blockIdx.y * TCR_ROWS, j0 = blockIdx.x * TCR_COLS;
   tcr_zero(acc);
+  tcr_load_temps(eval, n, K, M, i0, j0, Ts);
   __syncthreads();
   double dot = 0.0;
   do {
@@ -337,17 +355,17 @@ __global__ void __launch_bounds__(256) k_adj_tc_stage(const double* eval, const 
         const double mu = (cbase[c] + 0.0 < K.temin) ? 0.0 : lam_out[c];
         lam_base[c] = assign_base ? mu : lam_base[c] + mu;
       }
-      tc_sat_zero(sr, c);
+      if (SAT) tc_sat_zero(sr, c);
       continue;
     }
     double T[3][3];
-    tcr_temps(eval, n, K, M, i, j, T);
+    tcr_temps(Ts, li, lj, T);
     double gT[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
     double gfz[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const double R = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0, 1.0,
                                           gT, gfz);
     double cc = 1.0, dcx = 0.0, dcy = 0.0;
-    const double dE = sr.coef ? tc_sat_apply(sr, M, i, j, R, cc, dcx, dcy) : R;
+    const double dE = SAT ? tc_sat_apply(sr, M, i, j, R, cc, dcx, dcy) : R;
     double ld = 0.0;
     if (cbase) {
       const double v = cbase[c] + 1.0 * (mult * h) * dE;
@@ -357,7 +375,7 @@ __global__ void __launch_bounds__(256) k_adj_tc_stage(const double* eval, const 
       ld = mu * (mult * h);
     }
     if (seed_add) ld += add_coef * seed_add[c];
-    if (sr.coef) tc_sat_out(sr, c, ld, R, dcx, dcy);
+    if (SAT) tc_sat_out(sr, c, ld, R, dcx, dcy);
     if (ld != 0.0) {
       tcr_add(acc, li, lj, gT, ld * cc);
       for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += (ld * cc) * gfz[k];
@@ -372,6 +390,7 @@ __global__ void __launch_bounds__(256) k_adj_tc_stage(const double* eval, const 
 // S = k1 + 2 k2 + 2 k3 + k4, k4 = dE at the temperatures of i3 (thermalconduction.cpp:122-124):
 // lam_base = mu (assigned), lamS = mu*h/6 (the seed of each k, times its weight), k4's transpose
 // applied here, partial = mu*S/6.
+template <bool SAT>
 __global__ void __launch_bounds__(256) k_adj_tc_final4(const double* te, const double* k1, const double* k2,
                                                        const double* k3, const double* i3, const double* n,
                                                        const double* bhx, const double* bhy, const double* dbxx,
@@ -381,8 +400,10 @@ __global__ void __launch_bounds__(256) k_adj_tc_final4(const double* te, const d
                                                        double* lam_T, double* lam_fz, double* partial,
                                                        int accumulate, TcSatRev sr) {
   __shared__ double acc[TCR_N];
+  __shared__ double Ts[TCR_N];
   const int i0 = blockIdx.y * TCR_ROWS, j0 = blockIdx.x * TCR_COLS;
   tcr_zero(acc);
+  tcr_load_temps(i3, n, K, M, i0, j0, Ts);
   __syncthreads();
   double dot = 0.0;
   do {
@@ -393,17 +414,17 @@ __global__ void __launch_bounds__(256) k_adj_tc_final4(const double* te, const d
     if (!inside(M, i, j)) {
       lam_base[c] = (te[c] + 0.0 < K.temin) ? 0.0 : lam_out[c];
       lamS[c] = 0.0;
-      tc_sat_zero(sr, c);
+      if (SAT) tc_sat_zero(sr, c);
       continue;
     }
     double T[3][3];
-    tcr_temps(i3, n, K, M, i, j, T);
+    tcr_temps(Ts, li, lj, T);
     double gT[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
     double gfz[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const double R = tcc::tc_core_reverse(M, i, j, T, bhx[c], bhy[c], dbxx[c], dbxy[c], dbyx[c], dbyy[c], k0, 1.0,
                                           gT, gfz);
     double cc = 1.0, dcx = 0.0, dcy = 0.0;
-    const double dE = sr.coef ? tc_sat_apply(sr, M, i, j, R, cc, dcx, dcy) : R;
+    const double dE = SAT ? tc_sat_apply(sr, M, i, j, R, cc, dcx, dcy) : R;
     const double sum = k1[c] + 2.0 * k2[c] + 2.0 * k3[c] + dE;
     const double v = te[c] + qdiv(1.0 * h * sum, 6.0, K.r6);
     const double mu = (v < K.temin) ? 0.0 : lam_out[c];
@@ -411,7 +432,7 @@ __global__ void __launch_bounds__(256) k_adj_tc_final4(const double* te, const d
     const double ls = mu * h / 6.0;
     lamS[c] = ls;
     dot += mu * sum / 6.0;
-    if (sr.coef) tc_sat_out(sr, c, ls, R, dcx, dcy);
+    if (SAT) tc_sat_out(sr, c, ls, R, dcx, dcy);
     if (ls != 0.0) {
       tcr_add(acc, li, lj, gT, ls * cc);
       for (int k = 0; k < 6; ++k) lam_fz[k * N + c] += (ls * cc) * gfz[k];
@@ -1011,8 +1032,12 @@ void launch_adj_tc_sub(const double* te_s, const double* n, const double* bhx, c
                        const double* dbxx, const double* dbxy, const double* dbyx, const double* dbyy, double h,
                        double k0, MhdConst k, Mesh mesh, double* lam_te, double* lam_T, double* lam_fz,
                        double* partial, int accumulate, hipStream_t s, TcSatRev sr) {
-  hipLaunchKernelGGL(k_adj_tc_sub, tcr_grid(mesh), dim3(64, 4), 0, s, te_s, n, bhx, bhy, dbxx, dbxy, dbyx, dbyy,
-                     h, k0, k, mesh, lam_te, lam_T, lam_fz, partial, accumulate, sr);
+  if (sr.coef)
+    hipLaunchKernelGGL(k_adj_tc_sub<true>, tcr_grid(mesh), dim3(64, 4), 0, s, te_s, n, bhx, bhy, dbxx, dbxy, dbyx,
+                       dbyy, h, k0, k, mesh, lam_te, lam_T, lam_fz, partial, accumulate, sr);
+  else
+    hipLaunchKernelGGL(k_adj_tc_sub<false>, tcr_grid(mesh), dim3(64, 4), 0, s, te_s, n, bhx, bhy, dbxx, dbxy, dbyx,
+                       dbyy, h, k0, k, mesh, lam_te, lam_T, lam_fz, partial, accumulate, sr);
 }
 
 int adj_tc_blocks(const Mesh& m) {
@@ -1025,9 +1050,14 @@ void launch_adj_tc_stage(const double* eval, const double* n, const double* bhx,
                          double mult, double k0, MhdConst k, Mesh mesh, const double* cbase, const double* lam_out,
                          double* lam_base, int assign_base, const double* seed_add, double add_coef, double* lam_T,
                          double* lam_fz, double* partial, int accumulate, hipStream_t s, TcSatRev sr) {
-  hipLaunchKernelGGL(k_adj_tc_stage, tcr_grid(mesh), dim3(64, 4), 0, s, eval, n, bhx, bhy, dbxx, dbxy, dbyx, dbyy,
-                     h, mult, k0, k, mesh, cbase, lam_out, lam_base, assign_base, seed_add, add_coef, lam_T, lam_fz,
-                     partial, accumulate, sr);
+  if (sr.coef)
+    hipLaunchKernelGGL(k_adj_tc_stage<true>, tcr_grid(mesh), dim3(64, 4), 0, s, eval, n, bhx, bhy, dbxx, dbxy, dbyx,
+                       dbyy, h, mult, k0, k, mesh, cbase, lam_out, lam_base, assign_base, seed_add, add_coef, lam_T,
+                       lam_fz, partial, accumulate, sr);
+  else
+    hipLaunchKernelGGL(k_adj_tc_stage<false>, tcr_grid(mesh), dim3(64, 4), 0, s, eval, n, bhx, bhy, dbxx, dbxy, dbyx,
+                       dbyy, h, mult, k0, k, mesh, cbase, lam_out, lam_base, assign_base, seed_add, add_coef, lam_T,
+                       lam_fz, partial, accumulate, sr);
 }
 
 void launch_adj_tc_final4(const double* te, const double* k1, const double* k2, const double* k3, const double* i3,
@@ -1035,9 +1065,14 @@ void launch_adj_tc_final4(const double* te, const double* k1, const double* k2, 
                           const double* dbxy, const double* dbyx, const double* dbyy, double h, double k0, MhdConst k,
                           Mesh mesh, const double* lam_out, double* lam_base, double* lamS, double* lam_T,
                           double* lam_fz, double* partial, int accumulate, hipStream_t s, TcSatRev sr) {
-  hipLaunchKernelGGL(k_adj_tc_final4, tcr_grid(mesh), dim3(64, 4), 0, s, te, k1, k2, k3, i3, n, bhx, bhy, dbxx,
-                     dbxy, dbyx, dbyy, h, k0, k, mesh, lam_out, lam_base, lamS, lam_T, lam_fz, partial, accumulate,
-                     sr);
+  if (sr.coef)
+    hipLaunchKernelGGL(k_adj_tc_final4<true>, tcr_grid(mesh), dim3(64, 4), 0, s, te, k1, k2, k3, i3, n, bhx, bhy, dbxx,
+                       dbxy, dbyx, dbyy, h, k0, k, mesh, lam_out, lam_base, lamS, lam_T, lam_fz, partial, accumulate,
+                       sr);
+  else
+    hipLaunchKernelGGL(k_adj_tc_final4<false>, tcr_grid(mesh), dim3(64, 4), 0, s, te, k1, k2, k3, i3, n, bhx, bhy,
+                       dbxx, dbxy, dbyx, dbyy, h, k0, k, mesh, lam_out, lam_base, lamS, lam_T, lam_fz, partial,
+                       accumulate, sr);
 }
 
 void launch_adj_visc_term(const ViscTermArgs& a, int evol, const double* ka, double sa, const double* kb, double* lq,

@@ -44,6 +44,9 @@ struct TcSatRev {
   const double *coef = nullptr, *rx = nullptr, *ry = nullptr;
   double *lcoef = nullptr, *wx = nullptr, *wy = nullptr, *lfx = nullptr, *lfy = nullptr;
 };
+// An Euler conduction subcycle max(te_s + mask*h*dE(T(te_s)), te_min) reversed: lam_te (the
+// output's adjoint) through the clamp in place, dE's transpose scattered onto lam_T (and the frozen
+// fields' adjoint lam_fz), partial the step-size sums.
 void launch_adj_tc_sub(const double* te_s, const double* n, const double* bhx, const double* bhy,
                        const double* dbxx, const double* dbxy, const double* dbyx, const double* dbyy, double h,
                        double k0, MhdConst k, Mesh mesh, double* lam_te, double* lam_T, double* lam_fz,

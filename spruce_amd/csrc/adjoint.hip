@@ -126,34 +126,93 @@ __global__ void __launch_bounds__(256) k_adj_rhs_cells(MhdSet S, MhdStatic st, M
   for (int q = 0; q < ADJ_NCOEF; ++q) { cx[q * N + c] = fx[q]; cy[q * N + c] = fy[q]; }
 }
 
-// One thread per face along AX; the face at cell (i, j) lies between (i, j) and its lower neighbour.
-template <int AX>
-__device__ __forceinline__ void adj_faces_axis(const MhdSet& S, const MhdStatic& st, const MhdConst& K, const Mesh& M,
-                                               const double* coef, double* lam_S, int i, int j) {
-  const Axis& ax = AX == 0 ? M.x : M.y;
-  const int f = AX == 0 ? i : j;
-  if (!ax.per && (f < ax.lo || f > ax.hi + 1)) return;      // faces of interior cells only
-  const int fl = wrapi(f - 1, ax.n, ax.per);
+// The face scatter without a global atomic per contribution.  x faces: one thread per column
+// marches a strip of FX_ROWS faces down the rows; a face adds to the rows f-2 .. f+1 of its column,
+// so the thread keeps those four rows' adjoint in registers and writes a row once no later face of
+// the strip can reach it: plainly where no other strip reaches the row either (rows fs+1 ..
+// fe-3 of the strip [fs, fe)), atomically otherwise.  y faces: a 64 x 4 block sums its faces'
+// contributions per row in LDS and writes each cell once, plainly for the cells only its own faces
+// reach (columns j0+1 .. j0+61), atomically for the rest.  The x and y launches run one after the
+// other, so no cell is written plainly by two threads at once.
+constexpr int FX_ROWS = 16;
+
+__global__ void __launch_bounds__(64) k_adj_faces_x(MhdSet S, MhdStatic st, MhdConst K, Mesh M, const double* cx,
+                                                    double* lam_S) {
+  const int j = blockIdx.x * 64 + threadIdx.x;
+  if (j >= M.ny) return;
+  const Axis& ax = M.x;
   const size_t N = (size_t)M.nx * M.ny;
-  const size_t cu = (size_t)i * M.ny + j;                     // cell above the face
-  const size_t cl = AX == 0 ? (size_t)fl * M.ny + j : (size_t)i * M.ny + fl;
-  double fa[ADJ_NCOEF];
-  bool any = false;
-  for (int q = 0; q < ADJ_NCOEF; ++q) {
-    fa[q] = coef[q * N + cl] - coef[q * N + cu];
-    any = any || fa[q] != 0.0;
+  const int fs = blockIdx.y * FX_ROWS, fe = min(fs + FX_ROWS, M.nx);
+  double acc[4][8];
+  for (int a = 0; a < 4; ++a)
+    for (int v = 0; v < 8; ++v) acc[a][v] = 0.0;
+  auto flush = [&](int row, const double* a8, bool plain) {
+    if (!ax.per && (row < 0 || row >= ax.n)) return;
+    const size_t c = (size_t)wrapi(row, ax.n, ax.per) * M.ny + j;
+    for (int v = 0; v < 8; ++v) {
+      if (a8[v] == 0.0) continue;
+      if (plain) lam_S[v * N + c] += a8[v];
+      else atomicAdd(&lam_S[v * N + c], a8[v]);
+    }
+  };
+  for (int f = fs; f < fe; ++f) {
+    if (ax.per || (f >= ax.lo && f <= ax.hi + 1)) {     // faces of interior cells only
+      const int fl = wrapi(f - 1, ax.n, ax.per);
+      const size_t cu = (size_t)f * M.ny + j, cl = (size_t)fl * M.ny + j;
+      double fa[ADJ_NCOEF];
+      bool any = false;
+      for (int q = 0; q < ADJ_NCOEF; ++q) {
+        fa[q] = cx[q * N + cl] - cx[q * N + cu];
+        any = any || fa[q] != 0.0;
+      }
+      if (any) adj_face_slots<0>(S, st, K, M, f, j, fa, [&](int slot, int v, double val) { acc[slot][v] += val; });
+    }
+    // row f-2 is complete: later faces of the strip reach rows f-1 and above
+    flush(f - 2, acc[0], f - 3 >= fs);
+    for (int a = 0; a < 3; ++a)
+      for (int v = 0; v < 8; ++v) acc[a][v] = acc[a + 1][v];
+    for (int v = 0; v < 8; ++v) acc[3][v] = 0.0;
   }
-  if (!any) return;
-  adj_face<AX>(S, st, K, M, i, j, fa, [&](size_t cell, int v, double val) { atomicAdd(&lam_S[v * N + cell], val); });
+  for (int a = 0; a < 3; ++a) flush(fe - 2 + a, acc[a], false);   // rows the next strip reaches too
 }
 
-// x-faces for blockIdx.z == 0, y-faces for blockIdx.z == 1
-__global__ void __launch_bounds__(256) k_adj_faces(MhdSet S, MhdStatic st, MhdConst K, Mesh M, const double* cx,
-                                                   const double* cy, double* lam_S) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i >= M.nx || j >= M.ny) return;
-  if (blockIdx.z == 0) adj_faces_axis<0>(S, st, K, M, cx, lam_S, i, j);
-  else adj_faces_axis<1>(S, st, K, M, cy, lam_S, i, j);
+constexpr int FY_W = 64 + 3;   // cells j0-2 .. j0+64 of a row
+
+__global__ void __launch_bounds__(256) k_adj_faces_y(MhdSet S, MhdStatic st, MhdConst K, Mesh M, const double* cy,
+                                                     double* lam_S) {
+  __shared__ double acc[4][8][FY_W];
+  const int li = threadIdx.y, lj = threadIdx.x;
+  for (int t = li * 64 + lj; t < 4 * 8 * FY_W; t += 256) (&acc[0][0][0])[t] = 0.0;
+  __syncthreads();
+  const Axis& ax = M.y;
+  const size_t N = (size_t)M.nx * M.ny;
+  const int i = blockIdx.y * 4 + li, j0 = blockIdx.x * 64, j = j0 + lj;
+  if (i < M.nx && j < M.ny && (ax.per || (j >= ax.lo && j <= ax.hi + 1))) {
+    const int fl = wrapi(j - 1, ax.n, ax.per);
+    const size_t cu = (size_t)i * M.ny + j, cl = (size_t)i * M.ny + fl;
+    double fa[ADJ_NCOEF];
+    bool any = false;
+    for (int q = 0; q < ADJ_NCOEF; ++q) {
+      fa[q] = cy[q * N + cl] - cy[q * N + cu];
+      any = any || fa[q] != 0.0;
+    }
+    if (any)
+      adj_face_slots<1>(S, st, K, M, i, j, fa,
+                        [&](int slot, int v, double val) { atomicAdd(&acc[li][v][lj + slot], val); });
+  }
+  __syncthreads();
+  for (int t = li * 64 + lj; t < 4 * 8 * FY_W; t += 256) {
+    const int r = t / (8 * FY_W), rem = t - r * (8 * FY_W), v = rem / FY_W, col = rem - v * FY_W;
+    const double val = acc[r][v][col];
+    const int ii = blockIdx.y * 4 + r;
+    if (val == 0.0 || ii >= M.nx) continue;
+    const int jj = j0 - 2 + col;
+    if (!ax.per && (jj < 0 || jj >= ax.n)) continue;
+    const size_t c = (size_t)ii * M.ny + wrapi(jj, ax.n, ax.per);
+    // owned: the faces jj-1 .. jj+2 are all lanes of this block (none wraps round a periodic axis)
+    if (col >= 3 && col <= 63 && jj + 2 < ax.n) lam_S[v * N + c] += val;
+    else atomicAdd(&lam_S[v * N + c], val);
+  }
 }
 
 // per-block (min dt, index) over the interior, std::min semantics (NaN never selected)
@@ -260,8 +319,11 @@ void launch_adj_stage_cells(MhdSet B, const double* kv, const double* step, doub
 
 void launch_adj_faces(MhdSet S, MhdStatic st, MhdConst k, Mesh mesh, const double* cx, const double* cy,
                       double* lam_S, hipStream_t s) {
-  const dim3 b(64, 4), g = g2(mesh, b);
-  hipLaunchKernelGGL(k_adj_faces, dim3(g.x, g.y, 2), b, 0, s, S, st, k, mesh, cx, cy, lam_S);
+  const unsigned cols = (unsigned)((mesh.ny + 63) / 64);
+  hipLaunchKernelGGL(k_adj_faces_x, dim3(cols, (unsigned)((mesh.nx + FX_ROWS - 1) / FX_ROWS)), dim3(64), 0, s, S, st,
+                     k, mesh, cx, lam_S);
+  hipLaunchKernelGGL(k_adj_faces_y, dim3(cols, (unsigned)((mesh.nx + 3) / 4)), dim3(64, 4), 0, s, S, st, k, mesh, cy,
+                     lam_S);
 }
 
 void launch_adj_dt(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double eps, const double* p1, const double* p2,

@@ -100,8 +100,9 @@ __host__ __device__ inline S tc_core(const Mesh& M, int i, int j, const S T[3][3
 
 // Reverse mode of tc_core<double> at interior cell (i, j): returns the value and adds ld * dR/dT to
 // gT[3][3] and ld * dR/d(b_hat_x, b_hat_y, d b_hat_x/dx, /dy, d b_hat_y/dx, /dy) to gfz[6].  The
-// partial derivatives are those of the exact function (interpolation weights and divisions taken
-// in exact arithmetic); the value is tc_core's.
+// partial derivatives are those of the exact function, with the geometric divisions taken as
+// products with the prepared reciprocals (a relative error of an ulp in a derivative); the value is
+// tc_core's.
 __host__ __device__ inline double tc_core_reverse(const Mesh& M, int i, int j, const double T[3][3], double bx,
                                                   double by, double dbx_x, double dbx_y, double dby_x, double dby_y,
                                                   double k0, double ld, double gT[3][3], double gfz[6]) {
@@ -111,10 +112,10 @@ __host__ __device__ inline double tc_core_reverse(const Mesh& M, int i, int j, c
   const Axis& Y = M.y;
   const double dx = X.d[i], rdx = X.rd[i], dy = Y.d[j], rdy = Y.rd[j];
   // interpolation weights: upper face (c, c+1) and lower face (c-1, c) on each axis
-  const double xu_c = X.h[ip] / X.fden[ip], xu_p = X.h[i] / X.fden[ip];
-  const double xl_m = X.h[i] / X.fden[i], xl_c = X.h[im] / X.fden[i];
-  const double yu_c = Y.h[jp] / Y.fden[jp], yu_p = Y.h[j] / Y.fden[jp];
-  const double yl_m = Y.h[j] / Y.fden[j], yl_c = Y.h[jm] / Y.fden[j];
+  const double xu_c = X.h[ip] * X.rfden[ip], xu_p = X.h[i] * X.rfden[ip];
+  const double xl_m = X.h[i] * X.rfden[i], xl_c = X.h[im] * X.rfden[i];
+  const double yu_c = Y.h[jp] * Y.rfden[jp], yu_p = Y.h[j] * Y.rfden[jp];
+  const double yl_m = Y.h[j] * Y.rfden[j], yl_c = Y.h[jm] * Y.rfden[j];
   // forward, as tc_core
   const double Tc = T[1][1];
   const double Txu = fint(X, ip, i, Tc, T[2][1]), Txl = fint(X, i, im, T[0][1], Tc);
@@ -180,25 +181,26 @@ __host__ __device__ inline double tc_core_reverse(const Mesh& M, int i, int j, c
   // powers of Tc
   double g_Tc = g_q15 * 1.5 * sqrt(Tc) + g_q25 * 2.5 * q15;
   // second derivatives
-  double g_Txu = g_s0 / X.hsq[i], g_Txl = g_s0 / X.hsq[i];
-  g_Tc -= 2.0 * g_s0 / X.hsq[i];
-  double g_Tyu = g_s1 / Y.hsq[j], g_Tyl = g_s1 / Y.hsq[j];
-  g_Tc -= 2.0 * g_s1 / Y.hsq[j];
+  const double sx = g_s0 * X.rhsq[i], sy = g_s1 * Y.rhsq[j];
+  double g_Txu = sx, g_Txl = sx;
+  g_Tc -= 2.0 * sx;
+  double g_Tyu = sy, g_Tyl = sy;
+  g_Tc -= 2.0 * sy;
   // nested derivatives
-  const double ry = g_dgx_y / dy;
+  const double ry = g_dgx_y * rdy;
   g_gx += ry * (yu_c - yl_c);
   const double g_gxp = ry * yu_p, g_gxm = -ry * yl_m;
-  const double rx = g_dgy_x / dx;
+  const double rx = g_dgy_x * rdx;
   g_gy += rx * (xu_c - xl_c);
   const double g_gyp = rx * xu_p, g_gym = -rx * xl_m;
   auto rev_gxc = [&](int b1, double g) {   // gx of column b1 (cells (i-1..i+1, j-1+b1))
-    const double r = g / dx;
+    const double r = g * rdx;
     gT[1][b1] += r * (xu_c - xl_c);
     gT[2][b1] += r * xu_p;
     gT[0][b1] -= r * xl_m;
   };
   auto rev_gyr = [&](int a1, double g) {   // gy of row a1 (cells (i-1+a1, j-1..j+1))
-    const double r = g / dy;
+    const double r = g * rdy;
     gT[a1][1] += r * (yu_c - yl_c);
     gT[a1][2] += r * yu_p;
     gT[a1][0] -= r * yl_m;
@@ -208,10 +210,10 @@ __host__ __device__ inline double tc_core_reverse(const Mesh& M, int i, int j, c
   if (in_im) rev_gyr(0, g_gym);
   if (in_ip) rev_gyr(2, g_gyp);
   // first derivatives at the cell
-  g_Txu += g_gx / dx;
-  g_Txl -= g_gx / dx;
-  g_Tyu += g_gy / dy;
-  g_Tyl -= g_gy / dy;
+  g_Txu += g_gx * rdx;
+  g_Txl -= g_gx * rdx;
+  g_Tyu += g_gy * rdy;
+  g_Tyl -= g_gy * rdy;
   // face interpolants
   g_Tc += g_Txu * xu_c + g_Txl * xl_c + g_Tyu * yu_c + g_Tyl * yl_c;
   gT[2][1] += g_Txu * xu_p;
