@@ -741,6 +741,100 @@ __global__ void __launch_bounds__(256) k_adj_accum(const double* partial, int n,
   if (threadIdx.x == 0) gh[0] += scale * s;
 }
 
+// ---- field heating ---------------------------------------------------------------------------
+
+// FieldHeating (fieldheating.cpp:34-61) reversed: the heating H is taken from the step's start state
+// Ln (b = be + bi, n, and b_hat of Ln), te += mask*(h*H) is applied to the module's input state P and
+// propagated.  lam (the propagated state's adjoint) goes through the propagate's floors in place;
+// H's adjoint lam_te*mask*h is carried to Ln: the current through derivative1D transposed onto bi_y
+// (x neighbours) and bi_x (y neighbours), |b| onto bi of the cell, n onto rho, and the curvature of
+// b_hat onto lfz (b_hat and its four derivatives at the cell, for launch_adj_bhat) -- all into lamLn
+// (atomics).  partial: block sums of lam_te*mask*H (the step-size adjoint).
+__global__ void __launch_bounds__(256) k_adj_fheat(MhdSet Ln, MhdSet P, MhdStatic st, const double* bhx,
+                                                   const double* bhy, FheatParams p, const double* step, MhdConst K,
+                                                   Mesh M, double* lam, double* lamLn, double* lfz, double* partial) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  double dot = 0.0;
+  if (i < M.nx && j < M.ny) {
+    const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+    const bool in = inside(M, i, j);
+    const double h = *step;
+    // the heating as the forward forms it
+    double H = p.coeff, curl = 0.0, A = 1.0, B = 1.0, C = 1.0, D = 1.0, bm = 0.0, kx = 0.0, ky = 0.0, km = 0.0;
+    double dbx_x = 0.0, dbx_y = 0.0, dby_x = 0.0, dby_y = 0.0;
+    const int im = wrp(i - 1, M.nx, M.x.per), ip = wrp(i + 1, M.nx, M.x.per);
+    const int jm = wrp(j - 1, M.ny, M.y.per), jp = wrp(j + 1, M.ny, M.y.per);
+    const size_t cxm = (size_t)im * M.ny + j, cxp = (size_t)ip * M.ny + j;
+    const size_t cym = (size_t)i * M.ny + jm, cyp = (size_t)i * M.ny + jp;
+    auto dx_of = [&](double qm, double qc, double qp) {
+      return qdiv(tcc::fint(M.x, ip, i, qc, qp) - tcc::fint(M.x, i, im, qm, qc), M.x.d[i], M.x.rd[i]);
+    };
+    auto dy_of = [&](double qm, double qc, double qp) {
+      return qdiv(tcc::fint(M.y, jp, j, qc, qp) - tcc::fint(M.y, j, jm, qm, qc), M.y.d[j], M.y.rd[j]);
+    };
+    const double b[3] = {st.bex[c] + Ln.bx[c], st.bey[c] + Ln.by[c], st.bez[c] + Ln.bz[c]};
+    if (p.coeff != 0.0 && in) {
+      if (p.cpow != 0.0) {
+        curl = dx_of(st.bey[cxm] + Ln.by[cxm], b[1], st.bey[cxp] + Ln.by[cxp]) -
+               dy_of(st.bex[cym] + Ln.bx[cym], b[0], st.bex[cyp] + Ln.bx[cyp]);
+        A = pow(p.cc * fabs(curl), p.cpow);
+      }
+      if (p.bpow != 0.0) {
+        bm = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+        B = pow(bm, p.bpow);
+      }
+      if (p.npow != 0.0) C = pow(Ln.n[c], p.npow);
+      if (p.rpow != 0.0) {
+        dbx_x = dx_of(bhx[cxm], bhx[c], bhx[cxp]);
+        dbx_y = dy_of(bhx[cym], bhx[c], bhx[cyp]);
+        dby_x = dx_of(bhy[cxm], bhy[c], bhy[cxp]);
+        dby_y = dy_of(bhy[cym], bhy[c], bhy[cyp]);
+        kx = bhx[c] * dbx_x + bhy[c] * dbx_y;
+        ky = bhx[c] * dby_x + bhy[c] * dby_y;
+        km = sqrt(kx * kx + ky * ky);
+        D = pow(smax(km, 1.0e-16), p.rpow);          // h / roc^rpow = h * max(|k|, 1e-16)^rpow
+      }
+      H = p.coeff * A * B * C * D;
+    } else {
+      H = 0.0;
+    }
+    // the update te + mask*(h*H) and the propagate's floors
+    const double mask = in ? 1.0 : 0.0;
+    const double te_in = P.te[c] + mask * (h * H);
+    const double x = P.rho[c] / K.m;
+    const double rho1 = smax(x, K.nmin) * K.m;
+    lam[c] *= (x < K.nmin || rho1 / K.m < K.nmin) ? 0.0 : 1.0;
+    const double lt = lam[4 * N + c] * (te_in < K.temin ? 0.0 : 1.0);
+    lam[4 * N + c] = lt;
+    dot = lt * mask * H;
+    const double lH = lt * mask * h;
+    if (lH != 0.0 && H != 0.0) {
+      if (p.cpow != 0.0 && curl != 0.0) {
+        const double lc = lH * H * p.cpow / curl;          // d|c|^p/dc = p |c|^p / c
+        scatter_deriv(M, 0, i, j, lc, lamLn + 6 * N);      // curl = d(by)/dx - d(bx)/dy
+        scatter_deriv(M, 1, i, j, -lc, lamLn + 5 * N);
+      }
+      if (p.bpow != 0.0 && bm != 0.0) {
+        const double lb = lH * H * p.bpow / (bm * bm);
+        for (int k = 0; k < 3; ++k) atomicAdd(&lamLn[(5 + k) * N + c], lb * b[k]);
+      }
+      if (p.npow != 0.0) atomicAdd(&lamLn[c], lH * H * p.npow / Ln.n[c] * K.rm);
+      if (p.rpow != 0.0 && km > 1.0e-16) {
+        const double lk = lH * H * p.rpow / (km * km);     // d|k|^p/dk = p |k|^p k / |k|^2
+        const double lkx = lk * kx, lky = lk * ky;
+        lfz[c] += lkx * dbx_x + lky * dby_x;                // b_hat_x
+        lfz[N + c] += lkx * dbx_y + lky * dby_y;            // b_hat_y
+        lfz[2 * N + c] += lkx * bhx[c];                     // d b_hat_x / dx
+        lfz[3 * N + c] += lkx * bhy[c];                     // d b_hat_x / dy
+        lfz[4 * N + c] += lky * bhx[c];                     // d b_hat_y / dx
+        lfz[5 * N + c] += lky * bhy[c];                     // d b_hat_y / dy
+      }
+    }
+  }
+  dot = block_sum(dot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
+}
+
 // ---- artificial viscosity ----------------------------------------------------------------------
 
 // the differentiated variable at one cell, as the forward term reads it (idealmhd.cpp:241-277)
@@ -1073,6 +1167,13 @@ void launch_adj_tc_final4(const double* te, const double* k1, const double* k2, 
     hipLaunchKernelGGL(k_adj_tc_final4<false>, tcr_grid(mesh), dim3(64, 4), 0, s, te, k1, k2, k3, i3, n, bhx, bhy,
                        dbxx, dbxy, dbyx, dbyy, h, k0, k, mesh, lam_out, lam_base, lamS, lam_T, lam_fz, partial,
                        accumulate, sr);
+}
+
+void launch_adj_fheat(MhdSet Ln, MhdSet P, MhdStatic st, const double* bhx, const double* bhy, FheatParams p,
+                      const double* step, MhdConst k, Mesh mesh, double* lam, double* lamLn, double* lfz,
+                      double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_fheat, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, Ln, P, st, bhx, bhy, p, step, k, mesh, lam,
+                     lamLn, lfz, partial);
 }
 
 void launch_adj_visc_term(const ViscTermArgs& a, int evol, const double* ka, double sa, const double* kb, double* lq,

@@ -9,6 +9,7 @@
 
 #include "mhd.hpp"
 #include "solar.hpp"
+#include "sources.hpp"
 #include "visc.hpp"
 
 namespace spruce {
@@ -86,6 +87,14 @@ void launch_adj_tc_temp(const double* te_s, const double* n, MhdConst k, size_t 
 // The b_hat derivatives (derivative1D, zero off the interior) and b_hat = b/|b| (idealmhd.cpp:259-277)
 // reversed onto bi_x, bi_y, bi_z of the state L the conduction step froze them from.
 void launch_adj_bhat(MhdSet L, MhdStatic st, Mesh mesh, double* lam_fz, double* lam, hipStream_t s);
+// FieldHeating's update after its heating (fieldheating.cpp:34-61) reversed: the heating is taken
+// from the step's start state Ln (its b_hat bhx, bhy), the update te += mask*(h*H) from the module's
+// input P; lam (the propagated output's adjoint) through the floors in place, the heating's adjoint
+// onto lamLn (Ln's adjoint, atomics) and onto lfz (b_hat and its four derivatives, for
+// launch_adj_bhat on Ln), partial the block sums of the step-size adjoint.
+void launch_adj_fheat(MhdSet Ln, MhdSet P, MhdStatic st, const double* bhx, const double* bhy, FheatParams p,
+                      const double* step, MhdConst k, Mesh mesh, double* lam, double* lamLn, double* lfz,
+                      double* partial, hipStream_t s);
 // Artificial viscosity as a right-hand-side term (strength <= 1; viscosity.cpp:112-121, 185-267),
 // reversed at the stage state a.S for the stage derivative's adjoint lamk = sa*ka + kb (kb may be
 // null), its component `evol`: the Laplacian's stencil to lq (atomics; cleared by

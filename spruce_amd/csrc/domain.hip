@@ -2568,7 +2568,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   // parametrised heating and the source modules after the integrator
   std::vector<int> it_mods, post_mods;
   std::vector<ViscModule*> visc_mods;   // right-hand-side terms of every stage
-  bool any_sat = false, any_pv = false;
+  bool any_sat = false, any_pv = false, any_fheat = false;
   rec_post.assign(mods.size(), false);
   for (size_t k = 0; k < mods.size(); ++k) {
     GpuModule* m = mods[k].get();
@@ -2583,6 +2583,10 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
       it_mods.push_back((int)k);
       any_pv = true;
       (void)pv;
+    } else if (auto* fm = dynamic_cast<FieldHeatModule*>(m)) {
+      require(!fm->inactive, SPRUCE_E_UNSUPPORTED, "the adjoint covers field heating that is applied");
+      it_mods.push_back((int)k);
+      any_fheat = true;
     } else if (auto* vm = dynamic_cast<ViscModule*>(m)) {
       visc_mods.push_back(vm);                     // its right-hand-side terms
       if (vm->any_hyper) it_mods.push_back((int)k);   // its subcycled hyperviscous terms
@@ -2623,8 +2627,10 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   // physical viscosity: the forward derivatives of an evaluation set (7), the scatter accumulators
   // (velocity derivatives 6, T^2.5, b_hat 3, velocities 3, T) and an RK2 subcycle's two adjoint sets (4 + 4)
   const size_t pvscratch = any_pv ? 29 * N : 0;
+  // field heating: the adjoint of the step's start state its heating reads, and of that state's b_hat
+  const size_t fhscratch = any_fheat ? 14 * N : 0;
   const size_t total = (size_t)nseg * SET + (size_t)(sl + 1) * SET + seg_store + 3 * EV + 2 * CO + 4 * (size_t)nb +
-                       modscratch + rk4scratch + viscscratch + pvscratch;
+                       modscratch + rk4scratch + viscscratch + pvscratch + fhscratch;
   DBuf slab;
   slab.alloc(total);
   adj_bytes = (int64_t)(total * sizeof(double));
@@ -2743,6 +2749,12 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
     launch_adj_dt(S, stat(), kc, mesh, 1.0, nullptr, nullptr, vg, pmin, pidx, lamS, stream);
     launch_fill(vg, 0.0, 1, stream);
   };
+  double *fh_lam = nullptr, *fh_fz = nullptr;
+  if (any_fheat) {
+    fh_lam = take(8 * N);
+    fh_fz = take(6 * N);
+    launch_fill(fh_lam, 0.0, 8 * N, stream);
+  }
   double* pvs = nullptr;
   if (any_pv) {
     pvs = take(29 * N);
@@ -3024,6 +3036,15 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
             if (acc) launch_adj_accum(partm, adj_tc_blocks(mesh), 1.0 / (double)ns, gh, stream);
             launch_adj_bhat(Ln, st, mesh, lamfz, lam, stream);
           });
+        } else if (auto* fm = dynamic_cast<FieldHeatModule*>(m)) {
+          // field heating (fieldheating.cpp:34-61): its heating reads the step's start state Ln, whose
+          // adjoint collects in fh_lam until the modules are reversed
+          timed("adj_fheat", [&] {
+            launch_fill(fh_fz, 0.0, 6 * N, stream);
+            launch_adj_fheat(Ln, pre, st, bhx.p, bhy.p, fm->p, hp, kc, mesh, lam, fh_lam, fh_fz, partm, stream);
+            launch_adj_accum(partm, nb, 1.0, gh, stream);
+            launch_adj_bhat(Ln, st, mesh, fh_fz, fh_lam, stream);
+          });
         } else if (auto* pv = dynamic_cast<PvModule*>(m)) {
           // physical viscosity (physicalviscosity.cpp:148-250): its subcycles reversed, last first;
           // the thermal energy and momenta of lam carry the subcycle states' adjoint
@@ -3141,6 +3162,10 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
             if (rl->nsub > 0) launch_adj_accum(partm, nb, 1.0 / (double)rl->nsub, gh, stream);
           });
         }
+      }
+      if (any_fheat) {   // the field heating's adjoint of the step's start state
+        launch_add(lam, fh_lam, 8 * N, stream);
+        launch_fill(fh_lam, 0.0, 8 * N, stream);
       }
       timed("adj_dt", [&] {
         launch_adj_dt(Ln, st, kc, mesh, kc.eps, part, (rk2 || rk4) ? part2 : nullptr, gh, pmin, pidx, lam, stream);

@@ -5,7 +5,8 @@ Covered: the ghost-fill reverse of fixed, reflect, open and open_ucnp boundaries
 RK2 and RK4 subcycles, with and without flux saturation), radiative losses (with prevent_subcycling too), the
 artificial viscosity (right-hand-side terms with local, global and boundary strengths, any
 differentiated variable and the gradient correction; hyperviscous terms under each subcycle
-integrator), physical viscosity (heating and force, Euler and RK2 subcycles, gradient correction), the source modules after the integrator (localized heating with its time ramp and window, mass and
+integrator), physical viscosity (heating and force, Euler and RK2 subcycles, gradient correction),
+field heating, the source modules after the integrator (localized heating with its time ramp and window, mass and
 momentum injection, oscillating too, the ambient heating sink) and the parametrised heating
 profile of the gradient workload, whose coefficients get their own gradient
 (spruce_gpu_adjoint_params).  The loss
@@ -219,6 +220,19 @@ VISC_CASES += [
     ("pv_heat_only_rk2", ("open_ucnp",) * 4, "rk2", PV("1e-16", force_on="false", time_integrator="rk2", epsilon="0.3"), 6, 3),
 ]
 
+FHEAT = ("field_heating", [("coeff", "3e4"), ("current_pow", "1.0"), ("b_pow", "0.5"), ("n_pow", "-0.5"),
+                           ("roc_pow", "0.5")])
+VISC_CASES += [
+    # field heating: the heating of the step's start state (current, |b|, n and the field lines'
+    # curvature) added to the thermal energy; after a hyperviscous term the state it heats differs
+    # from the one its heating reads
+    ("fheat_rk2", ("periodic", "periodic", "fixed", "reflect"), "rk2", FHEAT, 6, 3),
+    ("fheat_after_hyper_euler", ("periodic", "periodic", "fixed", "open"), "euler",
+     [VISC("local", "2.5", "v_x", "mom_x", hv_time_integrator="rk2"),
+      ("field_heating", [("coeff", "3e-4"), ("current_pow", "0.5"), ("b_pow", "1.0"), ("n_pow", "1.0"),
+                         ("roc_pow", "1.0")])], 6, 3),
+]
+
 
 @pytest.mark.parametrize("name,bc,integ,visc,nsteps,ckpt", VISC_CASES, ids=[v[0] for v in VISC_CASES])
 def test_viscosity_adjoint_matches_finite_differences(name, bc, integ, visc, nsteps, ckpt):
@@ -229,7 +243,7 @@ def test_viscosity_adjoint_matches_finite_differences(name, bc, integ, visc, nst
 
     def cfg():
         return W.base_config("ideal_mhd", bc, integ, epsilon=0.2, density_min=1.0, temp_min=1.0,
-                             thermal_energy_min=1e-12, modules=[visc])
+                             thermal_energy_min=1e-12, modules=visc if isinstance(visc, list) else [visc])
     _check_directions(base, cfg, nsteps, ckpt, w, rng)
 
 
@@ -403,9 +417,8 @@ def test_module_adjoint_leaves_the_same_final_state_as_stepping(mods):
 
 def test_module_adjoint_refuses_what_it_does_not_cover():
     st = _arcade()
-    fheat = ("field_heating", [("coeff", "3e3"), ("current_pow", "1.0"), ("b_pow", "0.5"), ("n_pow", "-0.5"),
-                               ("roc_pow", "0.5")])
-    for mods in ([fheat], [("div_cleaning", [("epsilon", "0.1"), ("time_scale", "0.01")])]):
+    fheat_off = ("field_heating", [("coeff", "3e3"), ("current_pow", "1.0"), ("inactive_mode", "true")])
+    for mods in ([fheat_off], [("div_cleaning", [("epsilon", "0.1"), ("time_scale", "0.01")])]):
         d = GpuDomain(spec_from_files(_arcade_cfg(THETA, mods=mods), st))
         try:
             with pytest.raises(SpruceError):
