@@ -741,6 +741,105 @@ __global__ void __launch_bounds__(256) k_adj_accum(const double* partial, int n,
   if (threadIdx.x == 0) gh[0] += scale * s;
 }
 
+// ---- divergence cleaning ------------------------------------------------------------------------
+
+// weights of derivative1D (d) and secondDerivative1D (s2) at cell c of an axis with respect to the
+// values at c-1, c, c+1 (derivs.cpp:229-264, 450-462, 477-487)
+struct Wm { double m, c, p; };
+__device__ __forceinline__ Wm dw(const Axis& A, int c, int cm, int cp) {
+  const double up_c = A.h[cp] / A.fden[cp], up_p = A.h[c] / A.fden[cp];
+  const double lo_m = A.h[c] / A.fden[c], lo_c = A.h[cm] / A.fden[c];
+  return Wm{-lo_m / A.d[c], (up_c - lo_c) / A.d[c], up_p / A.d[c]};
+}
+__device__ __forceinline__ Wm s2w(const Axis& A, int c, int cm, int cp) {
+  const double up_c = A.h[cp] / A.fden[cp], up_p = A.h[c] / A.fden[cp];
+  const double lo_m = A.h[c] / A.fden[c], lo_c = A.h[cm] / A.fden[c];
+  return Wm{lo_m / A.hsq[c], (up_c - 2.0 + lo_c) / A.hsq[c], up_p / A.hsq[c]};
+}
+
+// One divergence-cleaning subcycle (divcleaning.cpp:28-46), linear in b: bx += mask*h*coeff*
+// (d/dx d(by)/dy + d2(bx)/dx2), by += mask*h*coeff*(d/dy d(bx)/dx + d2(by)/dy2), the inner
+// derivatives zero off the interior.  Reversed in two passes: this one takes g = mask*h*coeff*lam
+// of each interior cell through the second derivatives onto ax, ay and through the outer first
+// derivatives onto ux, uy (interior targets only: the inner derivatives vanish elsewhere), and sums
+// the step-size partial lam . mask*coeff*(the stencil at the subcycle's b, bx_s, by_s).
+__global__ void __launch_bounds__(256) k_adj_dc_a(const double* bx_s, const double* by_s, const double* dxb,
+                                                  const double* dyb, const double* coeff, double h, Mesh M,
+                                                  const double* lbx, const double* lby, double* ax, double* ay,
+                                                  double* ux, double* uy, double* partial) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  double dot = 0.0;
+  if (i < M.nx && j < M.ny && inside(M, i, j)) {
+    const size_t c = (size_t)i * M.ny + j;
+    const int im = wrp(i - 1, M.nx, M.x.per), ip = wrp(i + 1, M.nx, M.x.per);
+    const int jm = wrp(j - 1, M.ny, M.y.per), jp = wrp(j + 1, M.ny, M.y.per);
+    const size_t cxm = (size_t)im * M.ny + j, cxp = (size_t)ip * M.ny + j;
+    const size_t cym = (size_t)i * M.ny + jm, cyp = (size_t)i * M.ny + jp;
+    const Wm wx = dw(M.x, i, im, ip), wy = dw(M.y, j, jm, jp), sx = s2w(M.x, i, im, ip), sy = s2w(M.y, j, jm, jp);
+    const double lx = lbx[c], ly = lby[c];
+    // the subcycle's stencil values (as k_dc_update forms them) for the step-size partial
+    const double stx = (wx.m * dyb[cxm] + wx.c * dyb[c] + wx.p * dyb[cxp]) +
+                       (sx.m * bx_s[cxm] + sx.c * bx_s[c] + sx.p * bx_s[cxp]);
+    const double sty = (wy.m * dxb[cym] + wy.c * dxb[c] + wy.p * dxb[cyp]) +
+                       (sy.m * by_s[cym] + sy.c * by_s[c] + sy.p * by_s[cyp]);
+    dot = lx * coeff[c] * stx + ly * coeff[c] * sty;
+    const double gx = h * coeff[c] * lx, gy = h * coeff[c] * ly;
+    if (gx != 0.0) {
+      atomicAdd(&ax[cxm], gx * sx.m);
+      atomicAdd(&ax[c], gx * sx.c);
+      atomicAdd(&ax[cxp], gx * sx.p);
+      if (inside(M, im, j)) atomicAdd(&uy[cxm], gx * wx.m);
+      atomicAdd(&uy[c], gx * wx.c);
+      if (inside(M, ip, j)) atomicAdd(&uy[cxp], gx * wx.p);
+    }
+    if (gy != 0.0) {
+      atomicAdd(&ay[cym], gy * sy.m);
+      atomicAdd(&ay[c], gy * sy.c);
+      atomicAdd(&ay[cyp], gy * sy.p);
+      if (inside(M, i, jm)) atomicAdd(&ux[cym], gy * wy.m);
+      atomicAdd(&ux[c], gy * wy.c);
+      if (inside(M, i, jp)) atomicAdd(&ux[cyp], gy * wy.p);
+    }
+  }
+  dot = block_sum(dot);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = dot;
+}
+
+// second pass: the inner derivatives transposed (uy through d/dy onto ay, ux through d/dx onto ax)
+__global__ void __launch_bounds__(256) k_adj_dc_b(Mesh M, double* ux, double* uy, double* ax, double* ay) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= M.nx || j >= M.ny) return;
+  const size_t c = (size_t)i * M.ny + j;
+  const double u1 = ux[c], u2 = uy[c];
+  ux[c] = 0.0;
+  uy[c] = 0.0;
+  if (!inside(M, i, j)) return;
+  const int im = wrp(i - 1, M.nx, M.x.per), ip = wrp(i + 1, M.nx, M.x.per);
+  const int jm = wrp(j - 1, M.ny, M.y.per), jp = wrp(j + 1, M.ny, M.y.per);
+  if (u1 != 0.0) {   // ux = adjoint of d(bx)/dx at c
+    const Wm w = dw(M.x, i, im, ip);
+    atomicAdd(&ax[(size_t)im * M.ny + j], u1 * w.m);
+    atomicAdd(&ax[c], u1 * w.c);
+    atomicAdd(&ax[(size_t)ip * M.ny + j], u1 * w.p);
+  }
+  if (u2 != 0.0) {   // uy = adjoint of d(by)/dy at c
+    const Wm w = dw(M.y, j, jm, jp);
+    atomicAdd(&ay[(size_t)i * M.ny + jm], u2 * w.m);
+    atomicAdd(&ay[c], u2 * w.c);
+    atomicAdd(&ay[(size_t)i * M.ny + jp], u2 * w.p);
+  }
+}
+
+// lam_bx += ax, lam_by += ay; ax, ay cleared
+__global__ void __launch_bounds__(256) k_adj_dc_c(size_t N, double* ax, double* ay, double* lbx, double* lby) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  lbx[c] += ax[c];
+  lby[c] += ay[c];
+  ax[c] = 0.0;
+  ay[c] = 0.0;
+}
+
 // ---- field heating ---------------------------------------------------------------------------
 
 // FieldHeating (fieldheating.cpp:34-61) reversed: the heating H is taken from the step's start state
@@ -1167,6 +1266,17 @@ void launch_adj_tc_final4(const double* te, const double* k1, const double* k2, 
     hipLaunchKernelGGL(k_adj_tc_final4<false>, tcr_grid(mesh), dim3(64, 4), 0, s, te, k1, k2, k3, i3, n, bhx, bhy,
                        dbxx, dbxy, dbyx, dbyy, h, k0, k, mesh, lam_out, lam_base, lamS, lam_T, lam_fz, partial,
                        accumulate, sr);
+}
+
+void launch_adj_divclean(const double* bx_s, const double* by_s, const double* dxb, const double* dyb,
+                         const double* coeff, double h, Mesh mesh, double* lbx, double* lby, double* scratch,
+                         double* partial, hipStream_t s) {
+  const size_t N = (size_t)mesh.nx * mesh.ny;
+  double *ax = scratch, *ay = scratch + N, *ux = scratch + 2 * N, *uy = scratch + 3 * N;
+  hipLaunchKernelGGL(k_adj_dc_a, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, bx_s, by_s, dxb, dyb, coeff, h, mesh,
+                     (const double*)lbx, (const double*)lby, ax, ay, ux, uy, partial);
+  hipLaunchKernelGGL(k_adj_dc_b, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, mesh, ux, uy, ax, ay);
+  hipLaunchKernelGGL(k_adj_dc_c, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, ax, ay, lbx, lby);
 }
 
 void launch_adj_fheat(MhdSet Ln, MhdSet P, MhdStatic st, const double* bhx, const double* bhy, FheatParams p,

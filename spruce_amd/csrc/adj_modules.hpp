@@ -87,6 +87,13 @@ void launch_adj_tc_temp(const double* te_s, const double* n, MhdConst k, size_t 
 // The b_hat derivatives (derivative1D, zero off the interior) and b_hat = b/|b| (idealmhd.cpp:259-277)
 // reversed onto bi_x, bi_y, bi_z of the state L the conduction step froze them from.
 void launch_adj_bhat(MhdSet L, MhdStatic st, Mesh mesh, double* lam_fz, double* lam, hipStream_t s);
+// One divergence-cleaning subcycle (divcleaning.cpp:28-46) reversed, given the subcycle's total
+// fields bx_s, by_s and their inner derivatives dxb = d(bx)/dx, dyb = d(by)/dy (zero off the
+// interior; launch_dc_inner): lbx, lby (the adjoints of bi_x, bi_y) updated in place; scratch: 4*N
+// doubles, zero on entry and left zero; partial: block sums of the adjoint of the subcycle size h.
+void launch_adj_divclean(const double* bx_s, const double* by_s, const double* dxb, const double* dyb,
+                         const double* coeff, double h, Mesh mesh, double* lbx, double* lby, double* scratch,
+                         double* partial, hipStream_t s);
 // FieldHeating's update after its heating (fieldheating.cpp:34-61) reversed: the heating is taken
 // from the step's start state Ln (its b_hat bhx, bhy), the update te += mask*(h*H) from the module's
 // input P; lam (the propagated output's adjoint) through the floors in place, the heating's adjoint

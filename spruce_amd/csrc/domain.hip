@@ -2054,11 +2054,21 @@ struct DivCleanModule : GpuModule {
     nsub = subcycle_count(dt, eps * ts);
     h = dt / ((double)nsub);
   }
+  // while recording (the adjoint's replay of a step): the total fields at the start of every
+  // subcycle (bx, by; 2 grids each)
+  bool recording = false;
+  std::vector<DBuf> tape;
   void post(Domain& d) override {  // :28-46
     int cur = 0;
     d.timed("divclean", [&] {
       launch_field_sum(d.L.bx.p, d.L.by.p, d.bex.p, d.bey.p, d.N, bx[0].p, by[0].p, d.stream);
       for (int s = 0; s < nsub; ++s) {
+        if (recording) {
+          while (tape.size() <= (size_t)s) tape.emplace_back();
+          if (tape[s].n != 2 * d.N) tape[s].alloc(2 * d.N);
+          hip_check(hipMemcpyAsync(tape[s].p, bx[cur].p, d.N * 8, hipMemcpyDeviceToDevice, d.stream), "record");
+          hip_check(hipMemcpyAsync(tape[s].p + d.N, by[cur].p, d.N * 8, hipMemcpyDeviceToDevice, d.stream), "record");
+        }
         launch_dc_inner(bx[cur].p, by[cur].p, d.mesh, dxb.p, dyb.p, d.stream);
         launch_dc_update(bx[cur].p, by[cur].p, dxb.p, dyb.p, d.L.bx.p, d.L.by.p, d.bex.p, d.bey.p, coeff.p, h, d.mesh,
                          bx[1 - cur].p, by[1 - cur].p, d.stream);
@@ -2531,6 +2541,7 @@ void Domain::replay_step(int n, const MhdSet& start, StepTape& tp, const std::ve
       if (auto* tc = dynamic_cast<TcModule*>(m.get())) tc->recording = on;
       if (auto* vm = dynamic_cast<ViscModule*>(m.get())) vm->recording = on;
       if (auto* pv = dynamic_cast<PvModule*>(m.get())) pv->recording = on;
+      if (auto* dc = dynamic_cast<DivCleanModule*>(m.get())) dc->recording = on;
     }
   };
   set_recording(true);
@@ -2568,7 +2579,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   // parametrised heating and the source modules after the integrator
   std::vector<int> it_mods, post_mods;
   std::vector<ViscModule*> visc_mods;   // right-hand-side terms of every stage
-  bool any_sat = false, any_pv = false, any_fheat = false;
+  bool any_sat = false, any_pv = false, any_fheat = false, any_dc = false;
   rec_post.assign(mods.size(), false);
   for (size_t k = 0; k < mods.size(); ++k) {
     GpuModule* m = mods[k].get();
@@ -2583,6 +2594,11 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
       it_mods.push_back((int)k);
       any_pv = true;
       (void)pv;
+    } else if (dynamic_cast<DivCleanModule*>(m)) {
+      it_mods.push_back((int)k);   // its iterate only fixes the subcycle count
+      post_mods.push_back((int)k);
+      rec_post[k] = true;
+      any_dc = true;
     } else if (auto* fm = dynamic_cast<FieldHeatModule*>(m)) {
       require(!fm->inactive, SPRUCE_E_UNSUPPORTED, "the adjoint covers field heating that is applied");
       it_mods.push_back((int)k);
@@ -2629,8 +2645,9 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   const size_t pvscratch = any_pv ? 29 * N : 0;
   // field heating: the adjoint of the step's start state its heating reads, and of that state's b_hat
   const size_t fhscratch = any_fheat ? 14 * N : 0;
+  const size_t dcscratch = any_dc ? 4 * N : 0;   // divergence cleaning's reverse passes
   const size_t total = (size_t)nseg * SET + (size_t)(sl + 1) * SET + seg_store + 3 * EV + 2 * CO + 4 * (size_t)nb +
-                       modscratch + rk4scratch + viscscratch + pvscratch + fhscratch;
+                       modscratch + rk4scratch + viscscratch + pvscratch + fhscratch + dcscratch;
   DBuf slab;
   slab.alloc(total);
   adj_bytes = (int64_t)(total * sizeof(double));
@@ -2749,6 +2766,11 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
     launch_adj_dt(S, stat(), kc, mesh, 1.0, nullptr, nullptr, vg, pmin, pidx, lamS, stream);
     launch_fill(vg, 0.0, 1, stream);
   };
+  double* dcs = nullptr;
+  if (any_dc) {
+    dcs = take(4 * N);
+    launch_fill(dcs, 0.0, 4 * N, stream);
+  }
   double *fh_lam = nullptr, *fh_fz = nullptr;
   if (any_fheat) {
     fh_lam = take(8 * N);
@@ -2869,6 +2891,21 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
         const MhdSet& pre = p ? tape.after_post[p - 1] : tape.A3;
         GpuModule* m = mods[post_mods[p]].get();
         auto* ph = dynamic_cast<ParamHeatModule*>(m);
+        if (auto* dc = dynamic_cast<DivCleanModule*>(m)) {   // divcleaning.cpp:28-46, subcycles last first
+          adj_ghost(post, lam);
+          timed("adj_divclean", [&] {
+            launch_adj_minmask(pre.rho, pre.te, nullptr, nullptr, kc, mesh, lam, nullptr, stream);
+            for (int sc = dc->nsub - 1; sc >= 0; --sc) {
+              const double* bxs = dc->tape[(size_t)sc].p;
+              const double* bys = bxs + N;
+              launch_dc_inner(bxs, bys, mesh, dc->dxb.p, dc->dyb.p, stream);
+              launch_adj_divclean(bxs, bys, dc->dxb.p, dc->dyb.p, dc->coeff.p, dc->h, mesh, lam + 5 * N, lam + 6 * N,
+                                  dcs, partm, stream);
+              launch_adj_accum(partm, nb, 1.0 / (double)dc->nsub, gh, stream);
+            }
+          });
+          continue;
+        }
         if (!ph && !dynamic_cast<HeatModule*>(m)) {
           SourceRev r;
           if (!static_cast<PostSourceModule*>(m)->source_rev(times[n], r)) continue;   // the module did nothing
@@ -2963,6 +3000,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
         const MhdSet& post = tape.after_it[k];
         const MhdSet& pre = k ? tape.after_it[k - 1] : Ln;
         GpuModule* m = mods[it_mods[k]].get();
+        if (dynamic_cast<DivCleanModule*>(m)) continue;   // its iterate leaves the state alone
         adj_ghost(post, lam);
         if (auto* tc = dynamic_cast<TcModule*>(m)) {
           const int ns = tc->nsub;

@@ -6,7 +6,7 @@ RK2 and RK4 subcycles, with and without flux saturation), radiative losses (with
 artificial viscosity (right-hand-side terms with local, global and boundary strengths, any
 differentiated variable and the gradient correction; hyperviscous terms under each subcycle
 integrator), physical viscosity (heating and force, Euler and RK2 subcycles, gradient correction),
-field heating, the source modules after the integrator (localized heating with its time ramp and window, mass and
+field heating, divergence cleaning, the source modules after the integrator (localized heating with its time ramp and window, mass and
 momentum injection, oscillating too, the ambient heating sink) and the parametrised heating
 profile of the gradient workload, whose coefficients get their own gradient
 (spruce_gpu_adjoint_params).  The loss
@@ -220,6 +220,14 @@ VISC_CASES += [
     ("pv_heat_only_rk2", ("open_ucnp",) * 4, "rk2", PV("1e-16", force_on="false", time_integrator="rk2", epsilon="0.3"), 6, 3),
 ]
 
+VISC_CASES += [
+    # divergence cleaning after the integrator: subcycles linear in b (about 25 per step here)
+    ("divclean_rk2", ("periodic", "periodic", "reflect", "fixed"), "rk2",
+     ("div_cleaning", [("epsilon", "0.1"), ("time_scale", "0.01")]), 6, 3),
+    ("divclean_visc_rk4", ("fixed", "reflect", "periodic", "periodic"), "rk4",
+     [VISC("local", "0.1", "v_x", "mom_x"), ("div_cleaning", [("epsilon", "0.2"), ("time_scale", "0.03")])], 4, 2),
+]
+
 FHEAT = ("field_heating", [("coeff", "3e4"), ("current_pow", "1.0"), ("b_pow", "0.5"), ("n_pow", "-0.5"),
                            ("roc_pow", "0.5")])
 VISC_CASES += [
@@ -418,8 +426,13 @@ def test_module_adjoint_leaves_the_same_final_state_as_stepping(mods):
 def test_module_adjoint_refuses_what_it_does_not_cover():
     st = _arcade()
     fheat_off = ("field_heating", [("coeff", "3e3"), ("current_pow", "1.0"), ("inactive_mode", "true")])
-    for mods in ([fheat_off], [("div_cleaning", [("epsilon", "0.1"), ("time_scale", "0.01")])]):
-        d = GpuDomain(spec_from_files(_arcade_cfg(THETA, mods=mods), st))
+    resist = ("anomalous_resistivity", [("resistivity_model", "time_scale"), ("time_scale", "1e-2"),
+                                        ("template_mode", "flood_fill"), ("flood_fill_threshold", "1.9"),
+                                        ("smoothing_sigma", "1.0"), ("flood_fill_argmin_radius", "6e7")])
+    # the resistivity's flood-fill argmin box must stay inside a non-periodic x axis
+    for bc, mods in ((("periodic", "periodic", "fixed", "open"), [fheat_off]),
+                     (("reflect", "open", "fixed", "fixed"), [resist])):
+        d = GpuDomain(spec_from_files(_arcade_cfg(THETA, bc=bc, mods=mods), st))
         try:
             with pytest.raises(SpruceError):
                 d.adjoint(2, 1, np.zeros((8, 24, 40)))
