@@ -195,7 +195,7 @@ void spruce_gpu_destroy(spruce_gpu_domain* d);
  * rk4, with periodic, fixed, reflect, open (only with the other axis periodic) and open_ucnp boundaries
  * and the modules thermal_conduction (any subcycle integrator, with or without flux saturation),
  * radiative_losses, artificial_viscosity (right-hand-side and hyperviscous terms), physical_viscosity,
- * field_heating, div_cleaning,
+ * field_heating, div_cleaning, boundary_outflow,
  * ambient_heating, ambient_heating_sink, localized_heating, mass_injection, momentum_injection and
  * parametric_heating;
  * SPRUCE_E_UNSUPPORTED otherwise.  With modules or rk4 each step of the backward sweep is replayed

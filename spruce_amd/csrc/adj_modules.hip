@@ -840,6 +840,109 @@ __global__ void __launch_bounds__(256) k_adj_dc_c(size_t N, double* ax, double* 
   ay[c] = 0.0;
 }
 
+// ---- boundary outflow -------------------------------------------------------------------------
+
+// b_hat_x, b_hat_y = b/|b| (b = be + bi, zero where |b| = 0) of a cell and the map of their adjoint
+// (gx, gy) onto bi_x, bi_y, bi_z of lam
+__device__ __forceinline__ void of_bhat_rev(const MhdSet& P, const MhdStatic& st, size_t c, size_t N, double gx,
+                                            double gy, double* lam) {
+  const double b[3] = {st.bex[c] + P.bx[c], st.bey[c] + P.by[c], st.bez[c] + P.bz[c]};
+  const double bm = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  if (bm == 0.0) return;
+  const double gb = (gx * b[0] + gy * b[1]) / (bm * bm * bm);
+  lam[5 * N + c] += gx / bm - gb * b[0];
+  lam[6 * N + c] += gy / bm - gb * b[1];
+  lam[7 * N + c] -= gb * b[2];
+}
+
+// BoundaryOutflow::postIterateModule (boundaryoutflow.cpp:39-67) reversed on its input state P:
+// mom += step*(a*tmpl)*(b_hat or the boundary normal)*rho.  lam updated in place (density, b_hat
+// onto bi); pa / ph: block sums of the adjoint of the acceleration a and of the step size.
+__global__ void __launch_bounds__(256) k_adj_outflow(MhdSet P, MhdStatic st, const double* tmpl, const double* sc,
+                                                     const double* step, OutflowParams p, Mesh M, double* lam,
+                                                     double* pa, double* ph) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y * blockDim.y + threadIdx.y;
+  double da = 0.0, dh = 0.0;
+  if (i < M.nx && j < M.ny) {
+    const size_t N = (size_t)M.nx * M.ny, c = (size_t)i * M.ny + j;
+    const double a = sc[1], h = *step, t = tmpl[c];
+    const double A = h * (a * t);
+    const double rho = P.rho[c], lmx = lam[N + c], lmy = lam[2 * N + c];
+    double lA = 0.0;
+    if (p.aligned) {
+      const double b[3] = {st.bex[c] + P.bx[c], st.bey[c] + P.by[c], st.bez[c] + P.bz[c]};
+      const double bm = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+      const double hx = bm == 0.0 ? 0.0 : b[0] / bm, hy = bm == 0.0 ? 0.0 : b[1] / bm;
+      lam[c] += lmx * (A * hx) + lmy * (A * hy);
+      lA = lmx * hx * rho + lmy * hy * rho;
+      if (A != 0.0) of_bhat_rev(P, st, c, N, lmx * A * rho, lmy * A * rho, lam);
+    } else {
+      const double lm = (p.side == OF_X1 || p.side == OF_X2) ? lmx : lmy;
+      lam[c] += lm * A;
+      lA = lm * rho;
+    }
+    da = lA * h * t;
+    dh = lA * a * t;
+  }
+  da = block_sum(da);
+  dh = block_sum(dh);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const int b = blockIdx.y * gridDim.x + blockIdx.x;
+    pa[b] = da;
+    ph[b] = dh;
+  }
+}
+
+// dynamic_mode (:41-45): a = clamp((target - m)/dynamic_time, 0, max_accel) with m the largest
+// oriented outflow speed b_hat . v over the window (launch_outflow_max's block results pv, pk);
+// where the clamp does not bind, the acceleration's adjoint reaches the cell that sets m.  One block.
+__global__ void __launch_bounds__(256) k_adj_outflow_dyn(MhdSet P, MhdStatic st, OutflowParams p, Mesh M,
+                                                         const double* pv, const double* pk, int nb, const double* pa,
+                                                         int npa, double* lam) {
+  __shared__ double sv[256], sa[256];
+  __shared__ long long sk[256];
+  double v = -INFINITY, ga = 0.0;
+  long long k = -1;
+  auto pick = [](double& v0, long long& k0, double ov, long long ok) {
+    if (ok >= 0 && (ov > v0 || (ov == v0 && (k0 < 0 || ok < k0)))) { v0 = ov; k0 = ok; }
+  };
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) pick(v, k, pv[b], (long long)pk[b]);
+  for (int b = threadIdx.x; b < npa; b += blockDim.x) ga += pa[b];
+  sv[threadIdx.x] = v;
+  sk[threadIdx.x] = k;
+  sa[threadIdx.x] = ga;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      pick(sv[threadIdx.x], sk[threadIdx.x], sv[threadIdx.x + off], sk[threadIdx.x + off]);
+      sa[threadIdx.x] += sa[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double init = -1.0 * p.target;
+  if (!(sk[0] >= 0 && init < sv[0])) return;          // m is the fold's start value: no cell sets it
+  const double m = sv[0];
+  const double a = (p.target - m) / p.dtime;
+  if (!(a < p.amax) || a < 0.0) return;               // the clamp binds
+  const double lmv = -sa[0] / p.dtime;                // d a / d m = -1/dynamic_time
+  const size_t N = (size_t)M.nx * M.ny, c = (size_t)sk[0];
+  const double b[3] = {st.bex[c] + P.bx[c], st.bey[c] + P.by[c], st.bez[c] + P.bz[c]};
+  const double bm = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  const double hx = bm == 0.0 ? 0.0 : b[0] / bm, hy = bm == 0.0 ? 0.0 : b[1] / bm;
+  double sg = 1.0;
+  if (p.side == OF_X1 && hx > 0.0) sg = -1.0;
+  else if (p.side == OF_X2 && hx < 0.0) sg = -1.0;
+  else if (p.side == OF_Y2 && hy < 0.0) sg = -1.0;
+  else if (p.side == OF_Y1 && hy > 0.0) sg = -1.0;
+  const double rho = P.rho[c], vx = P.mx[c] / rho, vy = P.my[c] / rho;
+  const double g = lmv * sg;
+  of_bhat_rev(P, st, c, N, g * vx, g * vy, lam);
+  lam[N + c] += g * hx / rho;
+  lam[2 * N + c] += g * hy / rho;
+  lam[c] -= g * (hx * vx + hy * vy) / rho;
+}
+
 // ---- field heating ---------------------------------------------------------------------------
 
 // FieldHeating (fieldheating.cpp:34-61) reversed: the heating H is taken from the step's start state
@@ -1277,6 +1380,16 @@ void launch_adj_divclean(const double* bx_s, const double* by_s, const double* d
                      (const double*)lbx, (const double*)lby, ax, ay, ux, uy, partial);
   hipLaunchKernelGGL(k_adj_dc_b, g2(mesh, dim3(64, 4)), dim3(64, 4), 0, s, mesh, ux, uy, ax, ay);
   hipLaunchKernelGGL(k_adj_dc_c, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, N, ax, ay, lbx, lby);
+}
+
+void launch_adj_outflow(MhdSet P, MhdStatic st, const double* tmpl, const double* sc, const double* step,
+                        OutflowParams p, Mesh mesh, const double* pv, const double* pk, int nb, double* lam,
+                        double* pa, double* ph, hipStream_t s) {
+  const dim3 g = g2(mesh, dim3(64, 4));
+  hipLaunchKernelGGL(k_adj_outflow, g, dim3(64, 4), 0, s, P, st, tmpl, sc, step, p, mesh, lam, pa, ph);
+  if (p.dynamic)
+    hipLaunchKernelGGL(k_adj_outflow_dyn, dim3(1), dim3(256), 0, s, P, st, p, mesh, pv, pk, nb, (const double*)pa,
+                       (int)(g.x * g.y), lam);
 }
 
 void launch_adj_fheat(MhdSet Ln, MhdSet P, MhdStatic st, const double* bhx, const double* bhy, FheatParams p,

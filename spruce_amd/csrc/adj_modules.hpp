@@ -94,6 +94,14 @@ void launch_adj_bhat(MhdSet L, MhdStatic st, Mesh mesh, double* lam_fz, double* 
 void launch_adj_divclean(const double* bx_s, const double* by_s, const double* dxb, const double* dyb,
                          const double* coeff, double h, Mesh mesh, double* lbx, double* lby, double* scratch,
                          double* partial, hipStream_t s);
+// BoundaryOutflow after the integrator (boundaryoutflow.cpp:39-67) reversed on its input state P
+// (the propagate's floors and ghost fill reversed beforehand): the acceleration a = sc[1] of the
+// forward, its template, and with dynamic_mode the forward's window maxima pv, pk (nb blocks,
+// launch_outflow_max on P) for the cell that sets the throttle.  lam updated in place; pa: block
+// sums of the acceleration's adjoint (scratch, adj_mod_blocks), ph: of the step size's.
+void launch_adj_outflow(MhdSet P, MhdStatic st, const double* tmpl, const double* sc, const double* step,
+                        OutflowParams p, Mesh mesh, const double* pv, const double* pk, int nb, double* lam,
+                        double* pa, double* ph, hipStream_t s);
 // FieldHeating's update after its heating (fieldheating.cpp:34-61) reversed: the heating is taken
 // from the step's start state Ln (its b_hat bhx, bhy), the update te += mask*(h*H) from the module's
 // input P; lam (the propagated output's adjoint) through the floors in place, the heating's adjoint

@@ -2608,7 +2608,8 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
       if (vm->any_hyper) it_mods.push_back((int)k);   // its subcycled hyperviscous terms
     } else if (dynamic_cast<ParamHeatModule*>(m) || dynamic_cast<HeatModule*>(m) ||
                dynamic_cast<LocalHeatModule*>(m) || dynamic_cast<MassInjModule*>(m) ||
-               dynamic_cast<MomInjModule*>(m) || dynamic_cast<HeatSinkModule*>(m)) {
+               dynamic_cast<MomInjModule*>(m) || dynamic_cast<HeatSinkModule*>(m) ||
+               dynamic_cast<OutflowModule*>(m)) {
       post_mods.push_back((int)k);
       rec_post[k] = true;
     } else {
@@ -2891,6 +2892,17 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
         const MhdSet& pre = p ? tape.after_post[p - 1] : tape.A3;
         GpuModule* m = mods[post_mods[p]].get();
         auto* ph = dynamic_cast<ParamHeatModule*>(m);
+        if (auto* om = dynamic_cast<OutflowModule*>(m)) {     // boundaryoutflow.cpp:39-67
+          adj_ghost(post, lam);
+          timed("adj_outflow", [&] {
+            launch_adj_minmask(pre.rho, pre.te, nullptr, nullptr, kc, mesh, lam, nullptr, stream);
+            const int onb = outflow_blocks(om->p);
+            launch_adj_outflow(pre, st, om->tmpl.p, om->sc.p, hp, om->p, mesh, om->part.p, om->part.p + onb, onb, lam,
+                               partt, partm, stream);
+            launch_adj_accum(partm, nb, 1.0, gh, stream);
+          });
+          continue;
+        }
         if (auto* dc = dynamic_cast<DivCleanModule*>(m)) {   // divcleaning.cpp:28-46, subcycles last first
           adj_ghost(post, lam);
           timed("adj_divclean", [&] {

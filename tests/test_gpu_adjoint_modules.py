@@ -6,7 +6,7 @@ RK2 and RK4 subcycles, with and without flux saturation), radiative losses (with
 artificial viscosity (right-hand-side terms with local, global and boundary strengths, any
 differentiated variable and the gradient correction; hyperviscous terms under each subcycle
 integrator), physical viscosity (heating and force, Euler and RK2 subcycles, gradient correction),
-field heating, divergence cleaning, the source modules after the integrator (localized heating with its time ramp and window, mass and
+field heating, divergence cleaning, boundary outflow (its dynamic throttle too), the source modules after the integrator (localized heating with its time ramp and window, mass and
 momentum injection, oscillating too, the ambient heating sink) and the parametrised heating
 profile of the gradient workload, whose coefficients get their own gradient
 (spruce_gpu_adjoint_params).  The loss
@@ -226,6 +226,24 @@ VISC_CASES += [
      ("div_cleaning", [("epsilon", "0.1"), ("time_scale", "0.01")]), 6, 3),
     ("divclean_visc_rk4", ("fixed", "reflect", "periodic", "periodic"), "rk4",
      [VISC("local", "0.1", "v_x", "mom_x"), ("div_cleaning", [("epsilon", "0.2"), ("time_scale", "0.03")])], 4, 2),
+]
+
+def OUTFLOW(**o):
+    return ("boundary_outflow", list(o.items()))
+
+
+VISC_CASES += [
+    # boundary outflow after the integrator: the acceleration along the boundary normal or the field,
+    # and the dynamic throttle, whose acceleration follows the largest outflow speed of its window
+    ("outflow_exp_y2_rk2", ("periodic", "periodic", "fixed", "open"), "rk2",
+     OUTFLOW(max_accel="1e8", falloff_length="4e7", boundary="y_bound_2", falloff_shape="exp"), 6, 3),
+    ("outflow_dyn_aligned_x2_rk2", ("reflect", "open", "periodic", "periodic"), "rk2",
+     OUTFLOW(max_accel="1e9", falloff_length="5e7", boundary="x_bound_2", falloff_shape="gaussian",
+             feather_length="2e7", field_aligned_mode="true", dynamic_mode="true", dynamic_time="1.0",
+             dynamic_target_speed="3e7"), 6, 3),
+    ("outflow_flat_y1_rk4", ("periodic", "periodic", "open", "fixed"), "rk4",
+     OUTFLOW(max_accel="5e7", falloff_length="3e7", boundary="y_bound_1", falloff_shape="flat",
+             feather_length="2e7"), 4, 2),
 ]
 
 FHEAT = ("field_heating", [("coeff", "3e4"), ("current_pow", "1.0"), ("b_pow", "0.5"), ("n_pow", "-0.5"),
