@@ -394,7 +394,8 @@ def bench_adjoint_arcade(args) -> dict:
 
     steps = args.steps or 1000
     n = args.grid or 2048
-    prob = ArcadeHeatingProblem(n, n, nsteps=steps, checkpoint=50)
+    opts = dict(kv.partition("=")[::2] for kv in args.option)
+    prob = ArcadeHeatingProblem(n, n, nsteps=steps, checkpoint=50, options=opts)
     th = prob.theta0
     print("adjoint_arcade: forward warm-up", file=sys.stderr, flush=True)
     prob.forward(th)
@@ -406,7 +407,7 @@ def bench_adjoint_arcade(args) -> dict:
     # tens of steps this one is not: the conduction subcycle count is an integer set by the
     # smallest time scale over the whole grid, which the cells cooling at the open top move, so the
     # loss jumps wherever that count changes.  The probes use the same grid over 5 steps.
-    short = ArcadeHeatingProblem(n, n, nsteps=5, checkpoint=50)
+    short = ArcadeHeatingProblem(n, n, nsteps=5, checkpoint=50, options=opts)
     _, fin5, _ = short.forward(th)
     g5, _, _ = short.gradient(th, fin5)
     fd = []

@@ -150,7 +150,7 @@ int spruce_gpu_propagate(spruce_gpu_domain* d, double* step_out);
 int spruce_gpu_stream(spruce_gpu_domain* d, void** hip_stream);
 int spruce_gpu_device_grid(spruce_gpu_domain* d, const char* name, double** dev_ptr);
 
-/* Execution options (no reference counterpart; results are identical for every setting):
+/* Execution options (no reference counterpart; step results are identical for every setting):
  *   "stage_kernel" = "auto" (default: strips when the grid gives at least 1024 4-row strip
  *                    wavefronts, else one thread per cell) | "strip" (wavefront strips marching
  *                    along x) | "cell" (one thread per cell, faces recomputed per cell);
@@ -159,6 +159,10 @@ int spruce_gpu_device_grid(spruce_gpu_domain* d, const char* name, double** dev_
  *                    least 4096 wavefronts, else one thread per cell) | "strip" (wavefront strips,
  *                    every face computed once, b_hat derivatives formed in the kernel) | "cell";
  *   "tc_rows"      = rows per conduction strip, 1..256 (default: chosen from the grid size);
+ *   "adjoint_tc_rows" = spruce_gpu_adjoint's reverse of Euler conduction subcycles without flux
+ *                    saturation: rows per wavefront strip, 1..256, 0 (default: 16 where the grid
+ *                    has at least 64 columns and 32 rows), or -1 (tiles of one thread per cell);
+ *                    gradients agree to rounding across settings;
  *   "multispecies_mode" = "true" | "false" (the config setting of that name, fileio.cpp:322):
  *                    accumulate the cumulative heating grids (before setup);
  *   "block"        = ideal_2F launch shape: "<columns>x<rows>" thread block of every kernel, stage

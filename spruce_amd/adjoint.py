@@ -87,8 +87,9 @@ class ArcadeHeatingProblem:
 
     def __init__(self, nx: int = 2048, ny: int = 2048, stretch: float = 1.0005, depth: float = 6.4e8,
                  nsteps: int = 1000, checkpoint: int = 50, K: int = 16, theta_scale: float = 1e-4,
-                 bc=("periodic", "periodic", "fixed", "open")):
+                 bc=("periodic", "periodic", "fixed", "open"), options=None):
         self.nx, self.ny, self.nsteps, self.checkpoint, self.K = nx, ny, nsteps, checkpoint, K
+        self.options = dict(options or {})   # execution options of the domains (spruce_gpu_set_option)
         self.bc = tuple(bc)
         self.base = W.arcade_state(nx, ny, stretch=stretch, depth=depth)
         self.t0 = self.base.grids["temp"].copy()
@@ -110,7 +111,7 @@ class ArcadeHeatingProblem:
 
     def forward(self, theta: np.ndarray):
         """(loss, final (temp, n), seconds of the nsteps steps)."""
-        dom = GpuDomain(self.spec(theta))
+        dom = GpuDomain(self.spec(theta), self.options)
         try:
             t0 = time.perf_counter()
             dom.step(self.nsteps)
@@ -131,7 +132,7 @@ class ArcadeHeatingProblem:
 
     def gradient(self, theta: np.ndarray, final):
         """(dJ/dtheta, seconds of the adjoint call, device bytes the adjoint allocated)."""
-        dom = GpuDomain(self.spec(theta))
+        dom = GpuDomain(self.spec(theta), self.options)
         try:
             t0 = time.perf_counter()
             dom.adjoint(self.nsteps, self.checkpoint, self.seed(final))

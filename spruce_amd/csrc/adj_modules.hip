@@ -8,10 +8,12 @@
 
 #include "adj_cell.hpp"
 #include "adj_modules.hpp"
+#include "host_common.hpp"
 #include "arith.hpp"
 #include "dual.hpp"
 #include "sources.hpp"
 #include "tc_cell.hpp"
+#include "tc_strip.hpp"
 
 namespace spruce {
 namespace {
@@ -456,6 +458,169 @@ __global__ void __launch_bounds__(256) k_adj_tc_temp(const double* te_s, const d
   if (x < K.tmin) return;
   lam_te[c] += lT * (K.gm1 / den);
   lam_rho[c] += lT * (-x / nn) * K.rm;
+}
+
+// Strip form of the reverse of one Euler conduction subcycle without flux saturation,
+// te_out = max(te_s + h dE(te_s), te_min) (thermalconduction.cpp:60-66, 158-176).  A 64-lane
+// wavefront holds columns 60*tile - 2 + lane and marches rows r0-1 .. r1 of its strip, recomputing
+// every cell's face and row values as the forward strip kernel forms them (tc_strip.hpp), with the
+// geometric divisions taken as products with the prepared reciprocals and the powers of T as
+// T sqrt(T), T^2 sqrt(T): within a few ulps of the forward's values, which only the clamp's test
+// (v < te_min) and the step-size sum take; lanes 1..62 reverse their cells.  The reverse of tc_combine at a cell goes to the values it was formed
+// from, and these go to the temperatures and b_hat of the rows and lanes around the cell: along y
+// through the transposes of the forward's DPP lane shifts, along x through the face between two
+// rows, held in registers until both of its cells have added to it.  A row's adjoint is complete
+// once the next row is done; lanes 2..61 of rows r0..r1-1 then own it: they write the adjoint of
+// te_s (the clamp's seed plus dJ/dT through the temperature) and add dJ/drho and dJ/d b_hat.  No
+// atomics, and dJ/dT and the b_hat derivatives' adjoints never reach memory.
+__global__ void __launch_bounds__(64) k_adj_tc_strip(TcStepArgs a, const double* lam_in, double* lam_out,
+                                                     double* lam_rho, double* lam_bx, double* lam_by, double* partial,
+                                                     int accumulate, int R, int nstrips, int total) {
+  const Mesh& M = a.mesh;
+  const MhdConst& K = a.k;
+  const int t = strip::xcd_tile(blockIdx.x, total);
+  const int tile = t / nstrips, sidx = t - tile * nstrips;
+  const int lane = threadIdx.x;
+  const int jc = tile * 60 + lane - 2;
+  tcs::Lane L;
+  L.j = tcs::row_index(jc, M.ny, M.y.per, L.val);
+  L.iny = L.val && L.j >= M.y.lo && L.j <= M.y.hi;
+  L.out = lane >= 2 && lane <= 61 && jc < M.ny;
+  {
+    bool vp;
+    const int jp = tcs::row_index(L.j + 1, M.ny, M.y.per, vp);
+    L.hc = M.y.h[jp];
+    L.hn = M.y.h[L.j];
+    L.fd = M.y.fden[jp];
+    L.rfd = M.y.rfden[jp];
+    L.dy = M.y.d[L.j];
+    L.rdy = M.y.rd[L.j];
+    L.hsq = M.y.hsq[L.j];
+    L.rhsq = M.y.rhsq[L.j];
+  }
+  const bool cmp = lane >= 1 && lane <= 62 && L.val;   // reverses its cells
+  const double wc = L.hc * L.rfd, wn = L.hn * L.rfd;   // d(upper y-face)/d(own value, next lane's)
+  const int r0 = sidx * R, r1 = min(r0 + R, M.nx);
+  tcs::Raw rw = tcs::load_raw<false>(a, L, r0 - 2);
+  const tcs::Row first = tcs::make_row<false, false>(a, L, r0 - 2, rw);
+  rw = tcs::load_raw<false>(a, L, r0 - 1);
+  tcs::Row cur = tcs::make_row<false, false>(a, L, r0 - 1, rw);
+  tcs::XFaces lo = tcs::xfaces<false, false>(M, first, cur);
+  double cn = rw.n;
+  tcs::Raw ahead = tcs::load_raw<false>(a, L, r0);
+  // adjoints of the previous row (p*) and of the face between it and the current row (f*): T, gy
+  // (as masked), b_hat; ps1 = the previous row's seed of secondDerivative1D along y times 1/h^2
+  double pT = 0.0, pgy = 0.0, pbx = 0.0, pby = 0.0, ps1 = 0.0, plv = 0.0, pte = 0.0, pn = 1.0;
+  double fT = 0.0, fgy = 0.0, fbx = 0.0, fby = 0.0;
+  int pi = first.i;
+  bool pown = false, pgm = false;
+  double dot = 0.0;
+  for (int r = r0 - 1; r <= r1; ++r) {
+    const tcs::Raw raw = ahead;
+    if (r < r1) ahead = tcs::load_raw<false>(a, L, r + 2);
+    const tcs::Row nxt = tcs::make_row<false, false>(a, L, r + 1, raw);
+    const tcs::XFaces hi = tcs::xfaces<false, false>(M, cur, nxt);
+    // the cell of row r, as the forward forms it
+    const int i = cur.i;
+    const double rdx = M.x.rd[i];
+    const bool in = cur.inx && L.iny;
+    const double gx = (hi.T - lo.T) * rdx;
+    const double dgx_y = tcs::yderiv<false>(L, tcs::yface<false>(L, in ? gx : 0.0));
+    const bool live = cmp && cur.val;
+    double lv = 0.0;
+    tcc::TcCombAdj G{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (live) {
+      const size_t c = (size_t)i * M.ny + L.j;
+      const double lin = lam_in[c];
+      if (in) {
+        // dE and its partial derivatives for a unit seed of the update (seed h), scaled by the
+        // clamp's seed afterwards
+        const double s0 = (hi.T - 2.0 * cur.T + lo.T) * M.x.rhsq[i];
+        const double dgy_x = (hi.gy - lo.gy) * rdx;
+        const double dbxx = (hi.bx - lo.bx) * rdx;
+        const double dbyx = (hi.by - lo.by) * rdx;
+        const double dE = tcc::tc_combine_reverse<false>(cur.T, gx, cur.gy, s0, cur.s1, dgx_y, dgy_x, cur.bx, cur.by,
+                                                         dbxx, cur.dbxy, dbyx, cur.dbyy, a.p.k0, a.h, G);
+        const double v = cur.te + 1.0 * (1.0 * a.h) * dE;
+        lv = (v < K.temin) ? 0.0 : lin;
+        if (L.out && r >= r0 && r < r1) dot += lv * dE;
+        G.T *= lv; G.gx *= lv; G.gy *= lv; G.s0 *= lv; G.s1 *= lv; G.dgx_y *= lv; G.dgy_x *= lv;
+        G.bx *= lv; G.by *= lv; G.dbx_x *= lv; G.dbx_y *= lv; G.dby_x *= lv; G.dby_y *= lv;
+      } else {
+        lv = (cur.te + 0.0 < K.temin) ? 0.0 : lin;   // mask = 0: dE = 0
+      }
+    }
+    // this row's transposes along y: dgx_y onto gx of lanes l-1..l+1 (as masked), the y-derivatives
+    // of b_hat onto b_hat
+    double lgx;
+    {
+      const double u = G.dgx_y * L.rdy;
+      const double lf = u - strip::shl1(u);
+      const double lg = lf * wc + strip::shr1(lf * wn);
+      lgx = G.gx + (in ? lg : 0.0);
+    }
+    double cT, cgy = G.gy, cbx = G.bx, cby = G.by;
+    {
+      const double ub = G.dbx_y * L.rdy, vb = G.dby_y * L.rdy;
+      const double fb = ub - strip::shl1(ub), gb = vb - strip::shl1(vb);
+      cbx += fb * wc + strip::shr1(fb * wn);
+      cby += gb * wc + strip::shr1(gb * wn);
+    }
+    const double cs1 = G.s1 * L.rhsq;
+    const double a_gx = lgx * rdx, a_s0 = G.s0 * M.x.rhsq[i];
+    cT = G.T - 2.0 * a_s0 - 2.0 * cs1;
+    const double a_gyx = G.dgy_x * rdx, a_bxx = G.dbx_x * rdx, a_byx = G.dby_x * rdx;
+    // the face between rows r-1 and r is complete: xfaces(A = row r-1, B = row r) transposed
+    fT += a_s0 - a_gx;
+    fgy -= a_gyx;
+    fbx -= a_bxx;
+    fby -= a_byx;
+    {
+      const double rf = M.x.rfden[i];
+      const double wa = M.x.h[i] * rf, wb = M.x.h[pi] * rf;
+      pT += fT * wa;
+      cT += fT * wb;
+      pgy += fgy * wa;
+      cgy += fgy * wb;
+      pbx += fbx * wa;
+      cbx += fbx * wb;
+      pby += fby * wa;
+      cby += fby * wb;
+    }
+    // row r-1 is complete: its gy (masked as the forward masks it) and secondDerivative1D along y
+    // onto the temperatures of lanes l-1..l+1, then through the temperature onto te_s and rho
+    {
+      const double u = (pgm ? pgy : 0.0) * L.rdy;
+      const double lf = (u + ps1) + strip::shl1(ps1 - u);
+      pT += lf * wc + strip::shr1(lf * wn);
+      if (pown) {
+        const size_t q = (size_t)pi * M.ny + L.j;
+        const double den = pn * (2.0 * K_B);
+        const double x = pte * K.gm1 / den;
+        double lte = plv;
+        if (!(x < K.tmin)) {
+          lte += pT * (K.gm1 / den);
+          lam_rho[q] += pT * (-x / pn) * K.rm;
+        }
+        lam_out[q] = lte;
+        lam_bx[q] += pbx;
+        lam_by[q] += pby;
+      }
+    }
+    // row r becomes the previous row; the face above it holds row r's part
+    pT = cT; pgy = cgy; pbx = cbx; pby = cby; ps1 = cs1; plv = lv; pte = cur.te; pn = cn; pi = i;
+    pgm = in;
+    pown = L.out && cur.val && r >= r0 && r < r1;
+    fT = a_s0 + a_gx;
+    fgy = a_gyx;
+    fbx = a_bxx;
+    fby = a_byx;
+    cur = nxt;
+    lo = hi;
+    cn = raw.n;
+  }
+  for (int off = 32; off > 0; off >>= 1) dot += __shfl_xor(dot, off);
+  if (lane == 0) partial[blockIdx.x] = accumulate ? partial[blockIdx.x] + dot : dot;
 }
 
 // derivative1D(q, axis) at interior cell (i, j) transposed: d(c) depends on q(c-1), q(c), q(c+1)
@@ -1334,6 +1499,23 @@ void launch_adj_tc_sub(const double* te_s, const double* n, const double* bhx, c
   else
     hipLaunchKernelGGL(k_adj_tc_sub<false>, tcr_grid(mesh), dim3(64, 4), 0, s, te_s, n, bhx, bhy, dbxx, dbxy, dbyx,
                        dbyy, h, k0, k, mesh, lam_te, lam_T, lam_fz, partial, accumulate, sr);
+}
+
+int adj_tc_strip_blocks(const Mesh& m, int R) {
+  if (R < 1 || m.ny < 64 || m.nx < 2 * R) return 0;
+  return ((m.ny + 59) / 60) * ((m.nx + R - 1) / R);
+}
+
+void launch_adj_tc_strip(const double* te_s, const double* n, const double* bhx, const double* bhy, double h, double k0,
+                         MhdConst k, Mesh mesh, const double* lam_in, double* lam_out, double* lam_rho,
+                         double* lam_bx, double* lam_by, double* partial, int accumulate, int R, hipStream_t s) {
+  const int total = adj_tc_strip_blocks(mesh, R);
+  if (!total) throw Error(SPRUCE_E_ARG, "conduction reverse strips: grid too small for the strip height");
+  TcStepArgs a{};
+  a.te_src = te_s; a.te_base = te_s; a.n = n; a.bhx = bhx; a.bhy = bhy;
+  a.h = h; a.mult = 1.0; a.mode = TC_UPDATE; a.p.k0 = k0; a.k = k; a.mesh = mesh;
+  hipLaunchKernelGGL(k_adj_tc_strip, dim3(total), dim3(64), 0, s, a, lam_in, lam_out, lam_rho, lam_bx, lam_by,
+                     partial, accumulate, R, (mesh.nx + R - 1) / R, total);
 }
 
 int adj_tc_blocks(const Mesh& m) {

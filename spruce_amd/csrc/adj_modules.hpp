@@ -52,6 +52,15 @@ void launch_adj_tc_sub(const double* te_s, const double* n, const double* bhx, c
                        const double* dbxx, const double* dbxy, const double* dbyx, const double* dbyy, double h,
                        double k0, MhdConst k, Mesh mesh, double* lam_te, double* lam_T, double* lam_fz,
                        double* partial, int accumulate, hipStream_t s, TcSatRev sr = TcSatRev());
+// The same reverse (Euler, no flux saturation) in wavefront strips of R rows: lam_out (a grid other
+// than lam_in, the adjoint of the subcycle's output) receives the adjoint of te_s whole, dJ/dT taken
+// through the temperature with it; lam_rho += dJ/drho; lam_bx, lam_by += dJ/d(b_hat_x, b_hat_y), the
+// reverse of the b_hat derivatives included.  partial[0 .. adj_tc_strip_blocks) as for
+// launch_adj_tc_sub.  adj_tc_strip_blocks returns 0 where the grid is too small for strips.
+int adj_tc_strip_blocks(const Mesh& m, int R);
+void launch_adj_tc_strip(const double* te_s, const double* n, const double* bhx, const double* bhy, double h, double k0,
+                         MhdConst k, Mesh mesh, const double* lam_in, double* lam_out, double* lam_rho,
+                         double* lam_bx, double* lam_by, double* partial, int accumulate, int R, hipStream_t s);
 // A subcycle stage of the RK2/RK4 subcycle integrators reversed (thermalconduction.cpp:67-124):
 // dE is taken at the temperatures of `eval`; with cbase given, the stage output was
 // max(cbase + mask*(mult*h)*dE, te_min): lam_out goes through the clamp into lam_base (assigned

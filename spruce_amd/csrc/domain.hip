@@ -190,6 +190,9 @@ class Domain : public DomainBase {
   int strip_rows = 0;     // rows per wavefront strip; 0 = chosen from the grid size
   int tc_kernel = -1;     // conduction update: 0 wavefront strips, 1 one thread per cell, -1 by size
   int tc_rows = 0;        // rows per conduction strip; 0 = chosen from the grid size
+  // rows per strip of the adjoint's Euler conduction subcycle reverse; 0 = 16 where the grid allows
+  // strips, -1 = the tiled one-thread-per-cell form
+  int adj_tc_rows = 0;
   // multispecies mode (plasmadomain.hpp:133-137): cumulative electron, ion and Joule heating since
   // the last store, which the modules add to
   DBuf ms_grid[3];
@@ -213,6 +216,10 @@ class Domain : public DomainBase {
       const int r = std::stoi(v);
       require(r >= 1 && r <= 256, SPRUCE_E_ARG, "tc_rows must be in [1, 256]");
       tc_rows = r;
+    } else if (k == "adjoint_tc_rows") {
+      const int r = std::stoi(v);
+      require(r >= -1 && r <= 256, SPRUCE_E_ARG, "adjoint_tc_rows must be -1, 0 or in [1, 256]");
+      adj_tc_rows = r;
     } else {
       throw Error(SPRUCE_E_ARG, "unknown option " + k);
     }
@@ -3046,9 +3053,30 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
             if (clear) launch_fill(into, 0.0, N, stream);
             launch_adj_tc_temp(ev, pre.n, kc, N, lamT, into, lam, stream);
           };
+          // Euler subcycles without saturation: the strip form, alternating the adjoint of te between
+          // lte and lamT (zero outside this reverse)
+          const int srows = adj_tc_rows == 0 ? 16 : adj_tc_rows;
+          const int sblocks = adj_tc_rows >= 0 ? adj_tc_strip_blocks(mesh, srows) : 0;
+          const bool strips = tc->integ == "euler" && !sat && sblocks > 0 && sblocks <= nb;
           timed("adj_tc", [&] {
             launch_adj_minmask(pre.rho, te_last, nullptr, nullptr, kc, mesh, lam, nullptr, stream);
             launch_fill(lamfz, 0.0, 6 * N, stream);
+            if (strips) {
+              double *lin = lte, *lout = lamT;
+              for (int q = ns - 1; q >= 0; --q) {
+                const double* te_s = q ? tc->rec_out(q - 1) : pre.te;
+                launch_adj_tc_strip(te_s, pre.n, bhx.p, bhy.p, hs, k0, kc, mesh, lin, lout, lam, lamfz, lamfz + N,
+                                    partm, acc, srows, stream);
+                acc = 1;
+                std::swap(lin, lout);
+              }
+              if (lin != lte)
+                hip_check(hipMemcpyAsync(lte, lin, N * 8, hipMemcpyDeviceToDevice, stream), "copy conduction adjoint");
+              if (ns) launch_fill(lamT, 0.0, N, stream);
+              if (acc) launch_adj_accum(partm, sblocks, 1.0 / (double)ns, gh, stream);
+              launch_adj_bhat(Ln, st, mesh, lamfz, lam, stream);
+              return;
+            }
             for (int q = ns - 1; q >= 0; --q) {
               const double* te_s = q ? tc->rec_out(q - 1) : pre.te;
               if (tc->integ == "euler") {

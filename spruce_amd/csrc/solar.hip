@@ -8,6 +8,7 @@
 #include "solar.hpp"
 #include "strip.hpp"
 #include "tc_cell.hpp"
+#include "tc_strip.hpp"
 
 namespace spruce {
 namespace {
@@ -258,97 +259,6 @@ __global__ void __launch_bounds__(256) k_tc_step(TcStepArgs a) {
 // frozen b_hat are formed here from b_hat, so a subcycle reads te, n, b_hat_x, b_hat_y and writes
 // te.  Each value is the same sequence of IEEE operations as tc_core's (tc_cell.hpp), so the result
 // is bit-identical to the one-thread-per-cell form.
-namespace tcs {
-
-// one row of the strip at this lane: cell values and the quantities that need only y-neighbours
-struct Row {
-  double T, te, gy, s1, bx, by, dbxy, dbyy;
-  double coef, dcy;          // flux saturation: coefficient and its y-derivative
-  int i;                     // wrapped row index (clamped into [0, nx) when the row lies outside)
-  bool val, inx;             // row exists; row inside the interior bounds
-};
-struct Raw { double te, n, bx, by, coef; };
-struct XFaces { double T, gy, bx, by, coef; };
-
-struct Lane {
-  int j;                     // wrapped column (clamped into [0, ny) when it lies outside)
-  bool val, iny, out;        // column exists; inside the interior bounds; writes an output
-  double hc, hn, fd, rfd;    // upper y-face: weight of this lane's value, of the next lane's, denominator
-  double dy, rdy, hsq, rhsq;
-};
-
-__device__ __forceinline__ int row_index(int k, int n, int per, bool& val) {
-  if (per) { k %= n; if (k < 0) k += n; val = true; return k; }
-  val = k >= 0 && k < n;
-  return k < 0 ? 0 : (k >= n ? n - 1 : k);
-}
-
-// boundaryInterpolate across this lane's upper y-face (derivs.cpp:477-487), from its own value and
-// the next lane's
-__device__ __forceinline__ double yface(const Lane& L, double q) {
-  const double up = strip::shl1(q);
-  return qdiv(q * L.hc + up * L.hn, L.fd, L.rfd);
-}
-// derivative1D along y at this lane from the upper faces of this lane and of the previous one
-__device__ __forceinline__ double yderiv(const Lane& L, double fup) {
-  return qdiv(fup - strip::shr1(fup), L.dy, L.rdy);
-}
-
-template <bool SAT>
-__device__ __forceinline__ Raw load_raw(const TcStepArgs& a, const Lane& L, int k) {
-  Raw r{0.0, 0.0, 0.0, 0.0, 0.0};
-  bool v;
-  const int i = row_index(k, a.mesh.nx, a.mesh.x.per, v);
-  if (v && L.val) {
-    const size_t c = (size_t)i * a.mesh.ny + L.j;
-    r.te = a.te_src[c];
-    r.n = a.n[c];
-    r.bx = a.bhx[c];
-    r.by = a.bhy[c];
-    if (SAT) r.coef = a.coef[c];
-  }
-  return r;
-}
-
-template <bool SAT>
-__device__ __forceinline__ Row make_row(const TcStepArgs& a, const Lane& L, int k, const Raw& r) {
-  const Mesh& M = a.mesh;
-  Row w;
-  w.i = row_index(k, M.nx, M.x.per, w.val);
-  w.inx = w.val && w.i >= M.x.lo && w.i <= M.x.hi;
-  const bool ok = w.val && L.val;
-  w.te = r.te;
-  w.T = ok ? temp_of(r.te, r.n, a.k) : 0.0;
-  w.bx = r.bx;
-  w.by = r.by;
-  const double fT = yface(L, w.T);
-  const double fTlo = strip::shr1(fT);
-  // dtemp_dy is zero outside the interior (derivs.cpp:229-233)
-  w.gy = (w.inx && L.iny) ? qdiv(fT - fTlo, L.dy, L.rdy) : 0.0;
-  w.s1 = qdiv(fT - 2.0 * w.T + fTlo, L.hsq, L.rhsq);      // secondDerivative1D (derivs.cpp:450)
-  w.dbxy = yderiv(L, yface(L, w.bx));
-  w.dbyy = yderiv(L, yface(L, w.by));
-  if (SAT) {
-    w.coef = r.coef;
-    w.dcy = yderiv(L, yface(L, w.coef));
-  }
-  return w;
-}
-
-// boundaryInterpolate across the x-face between rows A (lower) and B (upper)
-template <bool SAT>
-__device__ __forceinline__ XFaces xfaces(const Mesh& M, const Row& A, const Row& B) {
-  const double ha = M.x.h[A.i], hb = M.x.h[B.i], fd = M.x.fden[B.i], rfd = M.x.rfden[B.i];
-  XFaces f;
-  f.T = qdiv(A.T * hb + B.T * ha, fd, rfd);
-  f.gy = qdiv(A.gy * hb + B.gy * ha, fd, rfd);
-  f.bx = qdiv(A.bx * hb + B.bx * ha, fd, rfd);
-  f.by = qdiv(A.by * hb + B.by * ha, fd, rfd);
-  if (SAT) f.coef = qdiv(A.coef * hb + B.coef * ha, fd, rfd);
-  return f;
-}
-
-}  // namespace tcs
 
 // Four wavefronts per SIMD (at most 128 VGPRs; the compiler spills about 16 bytes per lane, outside
 // the row loop): 79.9 us per subcycle on the 2048x2048 arcade against 84.3 us at 152 VGPRs and three

@@ -98,6 +98,80 @@ __host__ __device__ inline S tc_core(const Mesh& M, int i, int j, const S T[3][3
   return tc_combine<S>(Tc, gx, gy, s0, s1, dgx_y, dgy_x, bx, by, dbx_x, dbx_y, dby_x, dby_y, k0);
 }
 
+// Adjoints of tc_combine's arguments for the seed ld of its value.
+struct TcCombAdj {
+  double T, gx, gy, s0, s1, dgx_y, dgy_x, bx, by, dbx_x, dbx_y, dby_x, dby_y;
+};
+
+// Reverse mode of tc_combine: returns its value and sets g to ld times the partial derivatives with
+// respect to its thirteen arguments.  CR: the powers of Tc correctly rounded, as tc_combine takes
+// them (the value is then tc_combine's); otherwise Tc sqrt(Tc) and Tc^2 sqrt(Tc) (within a few ulps).
+template <bool CR = true>
+__host__ __device__ inline double tc_combine_reverse(double Tc, double gx, double gy, double s0, double s1,
+                                                     double dgx_y, double dgy_x, double bx, double by, double dbx_x,
+                                                     double dbx_y, double dby_x, double dby_y, double k0, double ld,
+                                                     TcCombAdj& g) {
+  const double bdg = bx * gx + by * gy;
+  const double t1x = bx * s0 + by * dgx_y, t1y = bx * dgy_x + by * s1;
+  const double t2x = gx * dbx_x + gy * dbx_y, t2y = gx * dby_x + gy * dby_y;
+  const double curl = dby_x - dbx_y;
+  const double t3x = (-(-1.0 * curl)) * gy, t3y = (-1.0 * curl) * gx;
+  const double sq = sqrt(Tc);
+  const double q15 = CR ? pow_1p5(Tc) : Tc * sq;
+  const double q25 = CR ? pow_2p5(Tc) : Tc * q15;
+  const double ttx = 5.0 / 2.0 * q15 * gx * bdg + q25 * (t1x + t2x + t3x);
+  const double tty = 5.0 / 2.0 * q15 * gy * bdg + q25 * (t1y + t2y + t3y);
+  const double divb = dbx_x + dby_y;
+  const double R = ((q25 * bdg * divb) + (bx * ttx + by * tty)) * k0;
+  const double gR = ld * k0;
+  double g_q25 = gR * bdg * divb, g_bdg = gR * q25 * divb;
+  const double g_divb = gR * q25 * bdg;
+  double g_bx = gR * ttx, g_by = gR * tty;
+  const double g_ttx = gR * bx, g_tty = gR * by;
+  const double g_q15 = g_ttx * 2.5 * gx * bdg + g_tty * 2.5 * gy * bdg;
+  double g_gx = g_ttx * 2.5 * q15 * bdg, g_gy = g_tty * 2.5 * q15 * bdg;
+  g_bdg += g_ttx * 2.5 * q15 * gx + g_tty * 2.5 * q15 * gy;
+  g_q25 += g_ttx * (t1x + t2x + t3x) + g_tty * (t1y + t2y + t3y);
+  const double gSx = g_ttx * q25, gSy = g_tty * q25;
+  double g_dbx_x = g_divb, g_dby_y = g_divb, g_dbx_y = 0.0, g_dby_x = 0.0;
+  // t3x = curl gy, t3y = -curl gx; curl = dby_x - dbx_y
+  const double g_curl = gSx * gy - gSy * gx;
+  g_gy += gSx * curl;
+  g_gx -= gSy * curl;
+  g_dby_x += g_curl;
+  g_dbx_y -= g_curl;
+  // t2
+  g_gx += gSx * dbx_x + gSy * dby_x;
+  g_gy += gSx * dbx_y + gSy * dby_y;
+  g_dbx_x += gSx * gx;
+  g_dbx_y += gSx * gy;
+  g_dby_x += gSy * gx;
+  g_dby_y += gSy * gy;
+  // t1
+  g_bx += gSx * s0 + gSy * dgy_x;
+  g_by += gSx * dgx_y + gSy * s1;
+  g.s0 = gSx * bx;
+  g.dgx_y = gSx * by;
+  g.dgy_x = gSy * bx;
+  g.s1 = gSy * by;
+  // bdg
+  g_bx += g_bdg * gx;
+  g_by += g_bdg * gy;
+  g_gx += g_bdg * bx;
+  g_gy += g_bdg * by;
+  // powers of Tc
+  g.T = g_q15 * 1.5 * sq + g_q25 * 2.5 * q15;
+  g.gx = g_gx;
+  g.gy = g_gy;
+  g.bx = g_bx;
+  g.by = g_by;
+  g.dbx_x = g_dbx_x;
+  g.dbx_y = g_dbx_y;
+  g.dby_x = g_dby_x;
+  g.dby_y = g_dby_y;
+  return R;
+}
+
 // Reverse mode of tc_core<double> at interior cell (i, j): returns the value and adds ld * dR/dT to
 // gT[3][3] and ld * dR/d(b_hat_x, b_hat_y, d b_hat_x/dx, /dy, d b_hat_y/dx, /dy) to gfz[6].  The
 // partial derivatives are those of the exact function, with the geometric divisions taken as
@@ -134,52 +208,10 @@ __host__ __device__ inline double tc_core_reverse(const Mesh& M, int i, int j, c
   const double dgy_x = qdiv(fint(X, ip, i, gy, gyp) - fint(X, i, im, gym, gy), dx, rdx);
   const double s0 = qdiv(Txu - 2.0 * Tc + Txl, X.hsq[i], X.rhsq[i]);
   const double s1 = qdiv(Tyu - 2.0 * Tc + Tyl, Y.hsq[j], Y.rhsq[j]);
-  const double bdg = bx * gx + by * gy;
-  const double t1x = bx * s0 + by * dgx_y, t1y = bx * dgy_x + by * s1;
-  const double t2x = gx * dbx_x + gy * dbx_y, t2y = gx * dby_x + gy * dby_y;
-  const double curl = dby_x - dbx_y;
-  const double t3x = (-(-1.0 * curl)) * gy, t3y = (-1.0 * curl) * gx;
-  const double q15 = pow_1p5(Tc), q25 = pow_2p5(Tc);
-  const double ttx = 5.0 / 2.0 * q15 * gx * bdg + q25 * (t1x + t2x + t3x);
-  const double tty = 5.0 / 2.0 * q15 * gy * bdg + q25 * (t1y + t2y + t3y);
-  const double divb = dbx_x + dby_y;
-  const double R = ((q25 * bdg * divb) + (bx * ttx + by * tty)) * k0;
-  // reverse
-  const double gR = ld * k0;
-  double g_q25 = gR * bdg * divb, g_bdg = gR * q25 * divb;
-  const double g_divb = gR * q25 * bdg;
-  double g_bx = gR * ttx, g_by = gR * tty;
-  const double g_ttx = gR * bx, g_tty = gR * by;
-  double g_q15 = g_ttx * 2.5 * gx * bdg + g_tty * 2.5 * gy * bdg;
-  double g_gx = g_ttx * 2.5 * q15 * bdg, g_gy = g_tty * 2.5 * q15 * bdg;
-  g_bdg += g_ttx * 2.5 * q15 * gx + g_tty * 2.5 * q15 * gy;
-  g_q25 += g_ttx * (t1x + t2x + t3x) + g_tty * (t1y + t2y + t3y);
-  const double gSx = g_ttx * q25, gSy = g_tty * q25;
-  double g_dbx_x = g_divb, g_dby_y = g_divb, g_dbx_y = 0.0, g_dby_x = 0.0;
-  // t3x = curl gy, t3y = -curl gx; curl = dby_x - dbx_y
-  const double g_curl = gSx * gy - gSy * gx;
-  g_gy += gSx * curl;
-  g_gx -= gSy * curl;
-  g_dby_x += g_curl;
-  g_dbx_y -= g_curl;
-  // t2
-  g_gx += gSx * dbx_x + gSy * dby_x;
-  g_gy += gSx * dbx_y + gSy * dby_y;
-  g_dbx_x += gSx * gx;
-  g_dbx_y += gSx * gy;
-  g_dby_x += gSy * gx;
-  g_dby_y += gSy * gy;
-  // t1
-  g_bx += gSx * s0 + gSy * dgy_x;
-  g_by += gSx * dgx_y + gSy * s1;
-  const double g_s0 = gSx * bx, g_dgx_y = gSx * by, g_dgy_x = gSy * bx, g_s1 = gSy * by;
-  // bdg
-  g_bx += g_bdg * gx;
-  g_by += g_bdg * gy;
-  g_gx += g_bdg * bx;
-  g_gy += g_bdg * by;
-  // powers of Tc
-  double g_Tc = g_q15 * 1.5 * sqrt(Tc) + g_q25 * 2.5 * q15;
+  TcCombAdj G;
+  const double R = tc_combine_reverse(Tc, gx, gy, s0, s1, dgx_y, dgy_x, bx, by, dbx_x, dbx_y, dby_x, dby_y, k0, ld, G);
+  double g_Tc = G.T, g_gx = G.gx, g_gy = G.gy;
+  const double g_s0 = G.s0, g_s1 = G.s1, g_dgx_y = G.dgx_y, g_dgy_x = G.dgy_x;
   // second derivatives
   const double sx = g_s0 * X.rhsq[i], sy = g_s1 * Y.rhsq[j];
   double g_Txu = sx, g_Txl = sx;
@@ -221,12 +253,12 @@ __host__ __device__ inline double tc_core_reverse(const Mesh& M, int i, int j, c
   gT[1][2] += g_Tyu * yu_p;
   gT[1][0] += g_Tyl * yl_m;
   gT[1][1] += g_Tc;
-  gfz[0] += g_bx;
-  gfz[1] += g_by;
-  gfz[2] += g_dbx_x;
-  gfz[3] += g_dbx_y;
-  gfz[4] += g_dby_x;
-  gfz[5] += g_dby_y;
+  gfz[0] += G.bx;
+  gfz[1] += G.by;
+  gfz[2] += G.dbx_x;
+  gfz[3] += G.dbx_y;
+  gfz[4] += G.dby_x;
+  gfz[5] += G.dby_y;
   return R;
 }
 

@@ -111,10 +111,10 @@ def _relative_scale(base):
 
 
 def _check_directions(base, make_cfg, nsteps, ckpt, w, rng, trials=6, min_smooth=4, steps=(1e-7, 1e-8),
-                      scale=None):
+                      scale=None, options=None):
     shape = (base.xdim, base.ydim)
     scale = scale if scale is not None else {v: SCALE[v] for v in STATE}
-    d = GpuDomain(spec_from_files(make_cfg(), base))
+    d = GpuDomain(spec_from_files(make_cfg(), base), options)
     try:
         g_ev, _ = d.adjoint(nsteps, ckpt, _seed(w, shape))
         g = d.adjoint_state(g_ev)

@@ -1,0 +1,99 @@
+"""The strip form of the adjoint's conduction subcycle reverse (Euler subcycles without flux
+saturation, k_adj_tc_strip: one wavefront per 60 columns and R rows, the adjoint of te, rho and b_hat
+written without atomics) against the tiled one-thread-per-cell form on the same steps, and against
+central differences of the GPU forward.
+
+The two forms differ only in the order their sums are taken, so their gradients agree to rounding;
+the shapes leave partial strips and partial 60-column tiles, with periodic, fixed, open and reflect
+sides along both axes."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from spruce_amd import workloads as W
+from spruce_amd.gpu import GpuDomain
+from spruce_amd.spec import spec_from_files
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.join(HERE, "golden"))
+from cases import random_mhd_state  # noqa: E402
+from test_gpu_adjoint_modules import (HEAT, RL, TC, THETA, _check_directions, _relative_scale,  # noqa: E402
+                                      _seed, _weights)
+
+pytestmark = pytest.mark.gpu
+
+
+def _arcade(nx, ny, seed):
+    """A stratified arcade with seeded flows, field and temperature noise, stretched gently in height."""
+    st = W.arcade_state(nx, ny, dx=5e6, dy0=5e6, stretch=1.01, t_corona=1e6, depth=4e7)
+    u = [W.uniform(seed * 16 + k, (nx, ny)) for k in range(5)]
+    rho = st.grids["rho"] * (1.0 + 0.01 * (2 * u[4] - 1))
+    st.grids["rho"] = rho
+    st.grids["mom_x"] = rho * 2e6 * (2 * u[0] - 1)
+    st.grids["mom_y"] = rho * 2e6 * (2 * u[1] - 1)
+    st.grids["bi_x"] = 0.5 * (2 * u[2] - 1)
+    st.grids["bi_y"] = 0.5 * (2 * u[3] - 1)
+    return st
+
+
+def _cfg(bc, integ, mods, temp_min=1e4, te_min=1e-8):
+    return W.base_config("ideal_mhd", bc, integ, epsilon=0.2, density_min=1.0, temp_min=temp_min,
+                         thermal_energy_min=te_min, modules=mods)
+
+
+CASES = [
+    # name, state, boundaries, integrator, modules, steps, checkpoint interval, strip heights
+    ("arcade_x_periodic_rk2", lambda: _arcade(42, 130, 3), ("periodic", "periodic", "fixed", "open"), "rk2",
+     [TC, RL(), HEAT(THETA[:8])], 5, 3, (4, 7, 16)),
+    ("arcade_reflect_fixed_x_euler", lambda: _arcade(37, 70, 5), ("reflect", "fixed", "periodic", "periodic"),
+     "euler", [TC, HEAT(THETA[:8])], 5, 2, (5, 16)),
+    ("random_periodic_rk4", lambda: random_mhd_state(34, 125, 61, stretch=True), ("periodic",) * 4, "rk4",
+     [TC, RL()], 4, 2, (4, 16)),
+]
+
+
+@pytest.mark.parametrize("name,make,bc,integ,mods,nsteps,ckpt,rows", CASES, ids=[c[0] for c in CASES])
+def test_strip_reverse_matches_tiled_reverse(name, make, bc, integ, mods, nsteps, ckpt, rows):
+    print("conduction reverse strips against tiles", name, flush=True)
+    st = make()
+    shape = (st.xdim, st.ydim)
+    rng = np.random.default_rng(7)
+    w = _weights(rng, shape, (1e13, 1e6, 1.0, 1.0))
+    seed = _seed(w, shape)
+    cfg = _cfg(bc, integ, mods, *((1.0, 1e-12) if name.startswith("random") else ()))
+    out = {}
+    for R in (-1,) + tuple(rows):
+        d = GpuDomain(spec_from_files(cfg, st), {"adjoint_tc_rows": R})
+        try:
+            g, dts = d.adjoint(nsteps, ckpt, seed)
+            p = d.adjoint_params()
+        finally:
+            d.close()
+        out[R] = (g, p, dts)
+    g0, p0, d0 = out[-1]
+    assert np.all(np.isfinite(g0))
+    for R in rows:
+        g, p, dts = out[R]
+        assert np.array_equal(dts, d0), R
+        for k in range(8):
+            scale = float(np.max(np.abs(g0[k])))
+            err = float(np.max(np.abs(g[k] - g0[k])))
+            print("rows", R, "component", k, "max |difference| / max |gradient| %.2e" % (err / max(scale, 1e-300)),
+                  flush=True)
+            assert err <= 1e-10 * scale, (R, k, err, scale)
+        if p0.size:
+            assert np.max(np.abs(p - p0)) <= 1e-10 * np.max(np.abs(p0)), (R, p, p0)
+
+
+def test_strip_reverse_matches_finite_differences():
+    print("conduction reverse strips: finite differences", flush=True)
+    st = _arcade(42, 130, 3)
+    rng = np.random.default_rng(31)
+    w = _weights(rng, (42, 130), (1e13, 1e6, 1.0, 1.0))
+
+    def cfg():
+        return _cfg(("periodic", "periodic", "fixed", "open"), "rk2", [TC, RL(), HEAT(THETA[:8])])
+    _check_directions(st, cfg, 5, 3, w, rng, scale=_relative_scale(st), options={"adjoint_tc_rows": 8})
