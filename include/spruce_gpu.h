@@ -160,9 +160,13 @@ int spruce_gpu_device_grid(spruce_gpu_domain* d, const char* name, double** dev_
  *                    every face computed once, b_hat derivatives formed in the kernel) | "cell";
  *   "tc_rows"      = rows per conduction strip, 1..256 (default: chosen from the grid size);
  *   "adjoint_tc_rows" = spruce_gpu_adjoint's reverse of Euler conduction subcycles without flux
- *                    saturation: rows per wavefront strip, 1..256, 0 (default: 16 where the grid
- *                    has at least 64 columns and 32 rows), or -1 (tiles of one thread per cell);
+ *                    saturation: rows per wavefront strip, 1..256, 0 (default: 8 where the grid
+ *                    has at least 64 columns and 16 rows), or -1 (tiles of one thread per cell);
  *                    gradients agree to rounding across settings;
+ *   "adjoint_tc_records" = "auto" (default: the segment recompute keeps the conduction subcycle
+ *                    results of every step of the segment where they fit in 80% of the free device
+ *                    memory, and the replay of a step takes them instead of running its subcycles)
+ *                    | "step" (each replay runs its subcycles again); same gradients;
  *   "multispecies_mode" = "true" | "false" (the config setting of that name, fileio.cpp:322):
  *                    accumulate the cumulative heating grids (before setup);
  *   "block"        = ideal_2F launch shape: "<columns>x<rows>" thread block of every kernel, stage

@@ -881,13 +881,20 @@ __global__ void __launch_bounds__(256) k_adj_source(MhdSet P, int mode, const do
 
 // ---- parametrised heating ----------------------------------------------------------------------
 
-__global__ void __launch_bounds__(256) k_adj_colsum(const double* lam_te, Mesh M, double* colsum) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= M.ny) return;
+// interior column sums of lam_te: 64 columns per block, 16 rows in flight per column
+__global__ void __launch_bounds__(1024) k_adj_colsum(const double* lam_te, Mesh M, double* colsum) {
+  __shared__ double part[16][64];
+  const int j = blockIdx.x * 64 + threadIdx.x;
   double s = 0.0;
-  if (j >= M.y.lo && j <= M.y.hi)
-    for (int i = M.x.lo; i <= M.x.hi; ++i) s += lam_te[(size_t)i * M.ny + j];
-  colsum[j] = s;
+  if (j < M.ny && j >= M.y.lo && j <= M.y.hi)
+    for (int i = M.x.lo + (int)threadIdx.y; i <= M.x.hi; i += 16) s += lam_te[(size_t)i * M.ny + j];
+  part[threadIdx.y][threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.y == 0 && j < M.ny) {
+    double t = 0.0;
+    for (int r = 0; r < 16; ++r) t += part[r][threadIdx.x];
+    colsum[j] = t;
+  }
 }
 
 __global__ void __launch_bounds__(256) k_adj_theta(const double* colsum, const double* basis, const double* step,
@@ -1643,7 +1650,7 @@ void launch_adj_source(MhdSet P, int mode, const double* t, const double* t2, co
 
 void launch_adj_heat_params(const double* lam_te, const double* basis, int K, const double* step, Mesh mesh,
                             double* colsum, double* g_theta, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_colsum, dim3((mesh.ny + 255) / 256), dim3(256), 0, s, lam_te, mesh, colsum);
+  hipLaunchKernelGGL(k_adj_colsum, dim3((mesh.ny + 63) / 64), dim3(64, 16), 0, s, lam_te, mesh, colsum);
   hipLaunchKernelGGL(k_adj_theta, dim3(K), dim3(256), 0, s, (const double*)colsum, basis, step, mesh, g_theta);
 }
 

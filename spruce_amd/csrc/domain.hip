@@ -190,9 +190,10 @@ class Domain : public DomainBase {
   int strip_rows = 0;     // rows per wavefront strip; 0 = chosen from the grid size
   int tc_kernel = -1;     // conduction update: 0 wavefront strips, 1 one thread per cell, -1 by size
   int tc_rows = 0;        // rows per conduction strip; 0 = chosen from the grid size
-  // rows per strip of the adjoint's Euler conduction subcycle reverse; 0 = 16 where the grid allows
+  // rows per strip of the adjoint's Euler conduction subcycle reverse; 0 = 8 where the grid allows
   // strips, -1 = the tiled one-thread-per-cell form
   int adj_tc_rows = 0;
+  bool adj_tc_slots = true;   // keep the conduction records of whole segments where memory allows
   // multispecies mode (plasmadomain.hpp:133-137): cumulative electron, ion and Joule heating since
   // the last store, which the modules add to
   DBuf ms_grid[3];
@@ -216,6 +217,9 @@ class Domain : public DomainBase {
       const int r = std::stoi(v);
       require(r >= 1 && r <= 256, SPRUCE_E_ARG, "tc_rows must be in [1, 256]");
       tc_rows = r;
+    } else if (k == "adjoint_tc_records") {
+      require(v == "auto" || v == "step", SPRUCE_E_ARG, "adjoint_tc_records must be auto or step");
+      adj_tc_slots = v == "auto";
     } else if (k == "adjoint_tc_rows") {
       const int r = std::stoi(v);
       require(r >= -1 && r <= 256, SPRUCE_E_ARG, "adjoint_tc_rows must be -1, 0 or in [1, 256]");
@@ -928,9 +932,23 @@ struct TcModule : GpuModule {
   bool recording = false;
   std::vector<DBuf> tape;
   double h_last = 0.0;
+  // The adjoint's segment recompute records every step of a segment into a slot of its own (slot >=
+  // 0), and the replay of that step then takes its subcycle results from the slot (reuse) instead of
+  // computing them again; slot_n = the subcycle count a slot holds
+  // (slot_pool: one allocation; slot i holds slot_cap[i] records from record slot_off[i] on)
+  DBuf slot_pool;
+  std::vector<size_t> slot_off, slot_cap;
+  std::vector<int> slot_n;
+  int slot = -1;
+  bool reuse = false;
+  size_t cells = 0;
   int rec_per_sub() const { return integ == "euler" ? 1 : (integ == "rk2" ? 2 : 7); }
   void record(Domain& d, int s, int k, const double* src);
-  const double* rec_at(int s, int k) const { return tape[(size_t)s * rec_per_sub() + k].p; }
+  double* rec_buf(Domain& d, int s, int k);   // the record's buffer (allocated on first use)
+  const double* rec_at(int s, int k) const {
+    const size_t idx = (size_t)s * rec_per_sub() + k;
+    return slot >= 0 ? slot_pool.p + (slot_off[(size_t)slot] + idx) * cells : tape[idx].p;
+  }
   const double* rec_out(int s) const { return rec_at(s, rec_per_sub() - 1); }
   explicit TcModule(const std::vector<std::pair<std::string, std::string>>& o) {
     name = "thermal_conduction";
@@ -961,6 +979,7 @@ struct TcModule : GpuModule {
   }
   bool needs_bhat() const override { return true; }
   void setup(Domain& d) override {
+    cells = d.N;
     ta.alloc(d.N); tb.alloc(d.N); tc.alloc(d.N);
     dbxx.alloc(d.N); dbxy.alloc(d.N); dbyx.alloc(d.N); dbyy.alloc(d.N);
     if (integ == "rk4") { k1.alloc(d.N); k2.alloc(d.N); k3.alloc(d.N); }
@@ -975,7 +994,7 @@ struct TcModule : GpuModule {
   void pre_launch(Domain& d, double* scal) override {
     // the strip form of the update forms the b_hat derivatives itself; the grids serve the cell
     // form and the adjoint's replay
-    if (d.tc_kernel == 1 || recording)
+    if (d.tc_kernel == 1 || (recording && (reuse || slot < 0)))
       d.timed("bhat_derivs",
               [&] { launch_bhat_derivs(d.bhx.p, d.bhy.p, d.mesh, dbxx.p, dbxy.p, dbyx.p, dbyy.p, d.stream); });
     d.timed("tc_pre", [&] {
@@ -1018,12 +1037,17 @@ struct TcModule : GpuModule {
       for (double* q : bufs) if (q != a && q != b) return q;
       return bufs[0];
     };
-    for (int s = 0; s < nsub; ++s) {
+    if (reuse) {   // the adjoint's replay of a step whose subcycles its slot holds
+      require(slot >= 0 && slot_n[(size_t)slot] == nsub, SPRUCE_E_STATE,
+              "conduction replay: the subcycle count differs from the recorded one");
+      if (nsub) cur = rec_out(nsub - 1);
+    }
+    for (int s = 0; s < (reuse ? 0 : nsub); ++s) {
       if (integ == "euler") {
-        double* out = other(cur, nullptr);
+        // while recording, each subcycle writes its output into its record
+        double* out = recording ? rec_buf(d, s, 0) : other(cur, nullptr);
         step(d, cur, cur, out, 1.0, TC_UPDATE, nullptr, h);
         cur = out;
-        if (recording) record(d, s, 0, out);
       } else if (integ == "rk2") {
         double* half = other(cur, nullptr);
         step(d, cur, cur, half, 0.5, TC_UPDATE, nullptr, h);
@@ -1045,26 +1069,36 @@ struct TcModule : GpuModule {
         cur = i2;
       }
     }
+    if (recording && !reuse && slot >= 0) slot_n[(size_t)slot] = nsub;
     if (to_file) launch_out_rate(cur, te0, dt, d.N, avg.p, d.stream);   // (:101-103)
     if (d.multispecies_on) {                                            // (:105-108)
       if (msf < 1.0) launch_ms_change(d.ms(1), cur, te0, 1.0 - msf, true, d.N, d.stream);
       if (msf > 0.0) launch_ms_change(d.ms(0), cur, te0, msf, true, d.N, d.stream);
     }
     if (inactive) return;
-    // the last subcycle's buffer becomes the live thermal energy (no copy)
+    // the last subcycle's buffer becomes the live thermal energy (no copy unless it is a record)
+    bool live = cur == d.L.te.p;
     for (DBuf* b : {&ta, &tb, &tc})
-      if (b->p == cur && cur != d.L.te.p) { d.L.te.swap(*b); break; }
+      if (!live && b->p == cur) { d.L.te.swap(*b); live = true; }
+    if (!live) hip_check(hipMemcpyAsync(d.L.te.p, cur, d.N * 8, hipMemcpyDeviceToDevice, d.stream), "copy conduction");
     d.propagate_live();
   }
 };
 
-void TcModule::record(Domain& d, int s, int k, const double* src) {
+double* TcModule::rec_buf(Domain& d, int s, int k) {
   const size_t idx = (size_t)s * rec_per_sub() + k;
+  if (slot >= 0) {
+    require(idx < slot_cap[(size_t)slot], SPRUCE_E_STATE, "conduction record beyond its slot");
+    return slot_pool.p + (slot_off[(size_t)slot] + idx) * d.N;
+  }
   while (tape.size() <= idx) {
     tape.emplace_back();
     tape.back().alloc(d.N);
   }
-  hip_check(hipMemcpyAsync(tape[idx].p, src, d.N * 8, hipMemcpyDeviceToDevice, d.stream), "record conduction");
+  return tape[idx].p;
+}
+void TcModule::record(Domain& d, int s, int k, const double* src) {
+  hip_check(hipMemcpyAsync(rec_buf(d, s, k), src, d.N * 8, hipMemcpyDeviceToDevice, d.stream), "record conduction");
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -2828,6 +2862,45 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
     copy_set(L.view(), kept, N, stream);
     adj_bytes += (int64_t)(SET * sizeof(double));
   }
+  // Conduction records of whole segments: the segment recompute keeps every step's subcycle
+  // results, so the replay of a step skips its subcycles.  Slot i holds step n0 + i of the segment
+  // being reversed; it is used when the records of the largest counts the forward took fit in 80%
+  // of the free device memory.
+  std::vector<TcModule*> slot_tcs;
+  if (replay) {
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    size_t need = 0;
+    std::vector<std::pair<TcModule*, size_t>> cand;
+    for (size_t k = 0; k < mods.size(); ++k)
+      if (auto* tc = dynamic_cast<TcModule*>(mods[k].get())) {
+        tc->slot = -1;                                  // (an earlier call that failed may have left one)
+        tc->reuse = false;
+        tc->slot_pool = DBuf();
+        tc->slot_off.assign(sl, 0);
+        tc->slot_cap.assign(sl, 0);
+        size_t recs = 0;
+        for (int i = 0; i < sl; ++i) {
+          int mx = 0;
+          for (int sg = 0; sg < nseg; ++sg) {
+            const int n = sg * ckpt + i;
+            if (n < nsteps) mx = std::max(mx, (int)sub_hist[(size_t)n * mods.size() + k]);
+          }
+          tc->slot_off[i] = recs;
+          tc->slot_cap[i] = (size_t)mx * tc->rec_per_sub();
+          recs += tc->slot_cap[i];
+        }
+        cand.emplace_back(tc, recs);
+        need += recs * N * sizeof(double);
+      }
+    if (adj_tc_slots && !cand.empty() && (double)need <= 0.8 * (double)free_b)
+      for (auto& c : cand) {
+        if (c.second) c.first->slot_pool.alloc(c.second * N);
+        c.first->slot_n.assign(sl, -1);
+        slot_tcs.push_back(c.first);
+        adj_bytes += (int64_t)(c.second * N * sizeof(double));
+      }
+  }
   for (auto& m : mods)
     if (auto* ph = dynamic_cast<ParamHeatModule*>(m.get())) launch_fill(ph->g_theta.p, 0.0, ph->K, stream);
   if (lamT) launch_fill(lamT, 0.0, N, stream);
@@ -2855,7 +2928,9 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
       hip_check(hipMemcpyAsync(step.p, history.p + n0, sizeof(double), hipMemcpyDeviceToDevice, stream), "copy step");
       tstep = times[n0];
       for (int n = n0; n < n1; ++n) {
+        for (TcModule* t : slot_tcs) { t->slot = n - n0; t->recording = true; }
         advance(n);
+        for (TcModule* t : slot_tcs) { t->recording = false; t->slot = -1; }
         copy_set(L.view(), seg[n - n0 + 1], N, stream);
       }
     }
@@ -2890,7 +2965,9 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
         continue;
       }
       // with modules: replay the step keeping its intermediate states, then reverse it
+      for (TcModule* t : slot_tcs) { t->slot = i; t->reuse = true; }   // the reverse reads slot i too
       replay_step(n, Ln, tape, times);
+      for (TcModule* t : slot_tcs) t->reuse = false;
       // dJ/dh of this step starts with dJ/d(time) of every later step: the simulation time a
       // time-dependent source reads at step m is the initial time plus the step sizes before m
       hip_check(hipMemcpyAsync(gh, gtime, sizeof(double), hipMemcpyDeviceToDevice, stream), "copy time adjoint");
@@ -3055,7 +3132,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
           };
           // Euler subcycles without saturation: the strip form, alternating the adjoint of te between
           // lte and lamT (zero outside this reverse)
-          const int srows = adj_tc_rows == 0 ? 16 : adj_tc_rows;
+          const int srows = adj_tc_rows == 0 ? 8 : adj_tc_rows;
           const int sblocks = adj_tc_rows >= 0 ? adj_tc_strip_blocks(mesh, srows) : 0;
           const bool strips = tc->integ == "euler" && !sat && sblocks > 0 && sblocks <= nb;
           timed("adj_tc", [&] {
@@ -3258,6 +3335,11 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
       if (auto* tc = dynamic_cast<TcModule*>(m.get())) adj_bytes += (int64_t)(tc->tape.size() * N * sizeof(double));
   }
   hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  for (TcModule* t : slot_tcs) {                        // the segment records are released
+    t->slot = -1;
+    t->slot_pool = DBuf();
+    t->slot_n.clear();
+  }
   tstep = time;
   if (clock.enabled) clock.collect();
 }

@@ -97,3 +97,29 @@ def test_strip_reverse_matches_finite_differences():
     def cfg():
         return _cfg(("periodic", "periodic", "fixed", "open"), "rk2", [TC, RL(), HEAT(THETA[:8])])
     _check_directions(st, cfg, 5, 3, w, rng, scale=_relative_scale(st), options={"adjoint_tc_rows": 8})
+
+
+@pytest.mark.parametrize("integ", ["euler", "rk2", "rk4"])
+def test_segment_records_match_per_step_replay(integ):
+    """The conduction records kept for whole segments (the replay of a step takes its subcycle
+    results from them) against replays that run every subcycle again, for each subcycle integrator:
+    the same gradients."""
+    print("conduction segment records", integ, flush=True)
+    st = _arcade(42, 130, 3)
+    shape = (st.xdim, st.ydim)
+    rng = np.random.default_rng(9)
+    w = _weights(rng, shape, (1e13, 1e6, 1.0, 1.0))
+    tc = ("thermal_conduction", [(k, v) if k != "time_integrator" else (k, integ) for k, v in TC[1]])
+    cfg = _cfg(("periodic", "periodic", "fixed", "open"), "rk2", [tc, RL(), HEAT(THETA[:8])])
+    out = []
+    for mode in ("auto", "step"):
+        d = GpuDomain(spec_from_files(cfg, st), {"adjoint_tc_records": mode})
+        try:
+            g, _ = d.adjoint(6, 4, _seed(w, shape))
+            out.append((g, d.adjoint_params()))
+        finally:
+            d.close()
+    (ga, pa), (gb, pb) = out
+    for k in range(8):
+        assert np.max(np.abs(ga[k] - gb[k])) <= 1e-10 * np.max(np.abs(gb[k])), k
+    assert np.max(np.abs(pa - pb)) <= 1e-10 * np.max(np.abs(pb))
