@@ -2699,7 +2699,8 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   for (auto& c : cps) c = set_at(take(SET), N);
   for (auto& c : seg) c = set_at(take(SET), N);
   std::vector<double*> k1s, k2s;
-  StepTape tape;
+  StepTape tape_one;   // the snapshots of one replayed step
+  StepTape& tape = tape_one;
   if (replay) {
     tape.after_it.resize(nit);
     tape.after_post.resize(npost);
@@ -2901,6 +2902,55 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
         adj_bytes += (int64_t)(c.second * N * sizeof(double));
       }
   }
+  // Whole-segment tapes: with conduction records kept per segment and only modules whose reverse
+  // needs nothing of a step beyond its tape, its step size and subcycle counts (conduction, radiative
+  // losses, the heating and source modules after the integrator), the segment recompute also keeps
+  // every step's StepTape and no step is replayed; b_hat (and its derivatives) are formed again from
+  // the step's start state.  Used where the tapes fit in 80% of the free device memory.
+  DBuf tape_pool;
+  std::vector<StepTape> tapes;
+  bool seg_tapes = false;
+  if (replay && adj_tc_slots) {
+    bool ok = true, any_tc = false;
+    for (auto& m : mods) {
+      GpuModule* g = m.get();
+      if (dynamic_cast<TcModule*>(g)) any_tc = true;
+      else if (!(dynamic_cast<RlModule*>(g) || dynamic_cast<ParamHeatModule*>(g) || dynamic_cast<HeatModule*>(g) ||
+                 dynamic_cast<LocalHeatModule*>(g) || dynamic_cast<MassInjModule*>(g) ||
+                 dynamic_cast<MomInjModule*>(g) || dynamic_cast<HeatSinkModule*>(g)))
+        ok = false;
+    }
+    if (ok && (!any_tc || !slot_tcs.empty())) {
+      const size_t per = (size_t)(nit + npost + 2 + (rk4 ? 3 : 0)) * SET + (rk4 ? 4 : 2) * EV;
+      size_t free_b = 0, total_b = 0;
+      hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+      if ((double)(per * sl * sizeof(double)) <= 0.8 * (double)free_b) {
+        tape_pool.alloc(per * sl);
+        adj_bytes += (int64_t)(per * sl * sizeof(double));
+        tapes.resize(sl);
+        double* q = tape_pool.p;
+        auto tk = [&](size_t n) { double* r = q; q += n; return r; };
+        for (auto& t : tapes) {
+          t.after_it.resize(nit);
+          t.after_post.resize(npost);
+          for (auto& c : t.after_it) c = set_at(tk(SET), N);
+          for (auto& c : t.after_post) c = set_at(tk(SET), N);
+          t.W = set_at(tk(SET), N);
+          t.A3 = set_at(tk(SET), N);
+          t.k1 = tk(EV);
+          t.k2 = tk(EV);
+          if (rk4) {
+            t.W2 = set_at(tk(SET), N);
+            t.W3 = set_at(tk(SET), N);
+            t.Lm = set_at(tk(SET), N);
+            t.k3 = tk(EV);
+            t.kf = tk(EV);
+          }
+        }
+        seg_tapes = true;
+      }
+    }
+  }
   for (auto& m : mods)
     if (auto* ph = dynamic_cast<ParamHeatModule*>(m.get())) launch_fill(ph->g_theta.p, 0.0, ph->K, stream);
   if (lamT) launch_fill(lamT, 0.0, N, stream);
@@ -2929,7 +2979,9 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
       tstep = times[n0];
       for (int n = n0; n < n1; ++n) {
         for (TcModule* t : slot_tcs) { t->slot = n - n0; t->recording = true; }
+        if (seg_tapes) rec = &tapes[n - n0];
         advance(n);
+        rec = nullptr;
         for (TcModule* t : slot_tcs) { t->recording = false; t->slot = -1; }
         copy_set(L.view(), seg[n - n0 + 1], N, stream);
       }
@@ -2965,9 +3017,32 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
         continue;
       }
       // with modules: replay the step keeping its intermediate states, then reverse it
-      for (TcModule* t : slot_tcs) { t->slot = i; t->reuse = true; }   // the reverse reads slot i too
-      replay_step(n, Ln, tape, times);
-      for (TcModule* t : slot_tcs) t->reuse = false;
+      StepTape& tape = seg_tapes ? tapes[i] : tape_one;
+      if (seg_tapes) {
+        // the module state the reverse reads, as the step set it: subcycle counts, their step
+        // sizes (iterate's dt / nsub), b_hat of the step's start state and its derivatives
+        bool bh = false;
+        for (size_t k = 0; k < mods.size(); ++k) {
+          GpuModule* m = mods[k].get();
+          bh = bh || m->needs_bhat();
+          const int ns = sub_hist[(size_t)n * mods.size() + k];
+          if (auto* tc = dynamic_cast<TcModule*>(m)) {
+            tc->nsub = ns;
+            tc->h_last = h[n] / (double)ns;
+            tc->slot = i;
+          } else if (auto* rl = dynamic_cast<RlModule*>(m)) {
+            rl->nsub = ns;
+            rl->h_last = h[n] / (double)ns;
+          }
+        }
+        if (bh) launch_bhat(Ln, st, mesh, bhx.p, bhy.p, stream);
+        for (TcModule* t : slot_tcs)
+          launch_bhat_derivs(bhx.p, bhy.p, mesh, t->dbxx.p, t->dbxy.p, t->dbyx.p, t->dbyy.p, stream);
+      } else {
+        for (TcModule* t : slot_tcs) { t->slot = i; t->reuse = true; }   // the reverse reads slot i too
+        replay_step(n, Ln, tape, times);
+        for (TcModule* t : slot_tcs) t->reuse = false;
+      }
       // dJ/dh of this step starts with dJ/d(time) of every later step: the simulation time a
       // time-dependent source reads at step m is the initial time plus the step sizes before m
       hip_check(hipMemcpyAsync(gh, gtime, sizeof(double), hipMemcpyDeviceToDevice, stream), "copy time adjoint");
