@@ -743,15 +743,14 @@ template <int K>
 __device__ Dn<K> rl_loss_dn(const Dn<K>& t, const Dn<K>& n, const Dn<K>& te_live, const Dn<K>& dt_live,
                            const RlParams& p) {
   if (t.v < p.cut) return dn_const<K>(0.0);
-  const double lt = log10(t.v);
   double chi;
   Dn<K> pw;
-  if (lt <= 4.97) { chi = 1.09e-31; pw = t * t; }
-  else if (lt <= 5.67) { chi = 8.87e-17; pw = 1.0 / t; }
-  else if (lt <= 6.18) { chi = 1.90e-22; pw = dn_const<K>(1.0); }
-  else if (lt <= 6.55) { chi = 3.53e-13; pw = dn_pow_m1p5(t); }
-  else if (lt <= 6.90) { chi = 3.46e-25; pw = dn_pow_third(t); }
-  else if (lt <= 7.63) { chi = 5.49e-16; pw = 1.0 / t; }
+  if (t.v <= p.tb[0]) { chi = 1.09e-31; pw = t * t; }
+  else if (t.v <= p.tb[1]) { chi = 8.87e-17; pw = 1.0 / t; }
+  else if (t.v <= p.tb[2]) { chi = 1.90e-22; pw = dn_const<K>(1.0); }
+  else if (t.v <= p.tb[3]) { chi = 3.53e-13; pw = dn_pow_m1p5(t); }
+  else if (t.v <= p.tb[4]) { chi = 3.46e-25; pw = dn_pow_third(t); }
+  else if (t.v <= p.tb[5]) { chi = 5.49e-16; pw = 1.0 / t; }
   else { chi = 1.96e-27; pw = dn_sqrt(t); }
   Dn<K> v = n * n * chi * pw;
   if (t.v < p.cut_plus_ramp) v = v * ((t - p.cut) / p.ramp);

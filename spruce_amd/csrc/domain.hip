@@ -140,6 +140,16 @@ static std::string opt_s(const std::vector<std::pair<std::string, std::string>>&
   for (auto& kv : o) if (kv.first == k) v = kv.second;
   return v;
 }
+// The largest double t with std::log10(t) <= c (the host C library's log10, which the reference
+// calls): from 10^c, stepped down while log10 exceeds c, then up while the next double still
+// satisfies it.  log10 is monotone there (checked over 2^16 doubles either side of each bound the
+// radiative losses use), so log10(t) <= c exactly when t <= the result.
+static double log10_upper(double c) {
+  double t = std::pow(10.0, c);
+  while (std::log10(t) > c) t = std::nextafter(t, 0.0);
+  while (std::log10(std::nextafter(t, HUGE_VAL)) <= c) t = std::nextafter(t, HUGE_VAL);
+  return t;
+}
 static int subcycle_count(double dt, double m) {
   const double q = dt / m;
   if (!(q < 1.0e8)) throw Error(SPRUCE_E_NUMERIC, "subcycle count out of range");
@@ -1120,6 +1130,8 @@ struct RlModule : GpuModule {
     require(f1 && f2 && f3, SPRUCE_E_ARG,
             "radiative_losses needs cutoff_ramp, cutoff_temp and epsilon (uninitialised otherwise)");
     p.cut_plus_ramp = p.cut + p.ramp;
+    const double lim[6] = {4.97, 5.67, 6.18, 6.55, 6.90, 7.63};
+    for (int k = 0; k < 6; ++k) p.tb[k] = log10_upper(lim[k]);
     p.prevent = opt_s(o, "prevent_subcycling", "false") == "true";
     integ = opt_s(o, "time_integrator", "euler");
     if (integ.empty()) integ = "euler";

@@ -348,14 +348,14 @@ k_tc_strip(TcStepArgs a, int R, int nstrips, int total) {
 // RadiativeLosses::computeLosses at one interior cell (radiativelosses.cpp:149-197).
 __device__ __forceinline__ double rl_loss(double t, double n, double te_live, double dt_live, const RlParams& p) {
   if (t < p.cut) return 0.0;
-  const double lt = log10(t);
+  // log10(t) <= 4.97 etc. as the host's log10 decides it (p.tb)
   double chi, pw;
-  if (lt <= 4.97) { chi = 1.09e-31; pw = t * t; }
-  else if (lt <= 5.67) { chi = 8.87e-17; pw = 1.0 / t; }
-  else if (lt <= 6.18) { chi = 1.90e-22; pw = 1.0; }
-  else if (lt <= 6.55) { chi = 3.53e-13; pw = pow_m1p5(t); }
-  else if (lt <= 6.90) { chi = 3.46e-25; pw = pow_third(t); }
-  else if (lt <= 7.63) { chi = 5.49e-16; pw = 1.0 / t; }
+  if (t <= p.tb[0]) { chi = 1.09e-31; pw = t * t; }
+  else if (t <= p.tb[1]) { chi = 8.87e-17; pw = 1.0 / t; }
+  else if (t <= p.tb[2]) { chi = 1.90e-22; pw = 1.0; }
+  else if (t <= p.tb[3]) { chi = 3.53e-13; pw = pow_m1p5(t); }
+  else if (t <= p.tb[4]) { chi = 3.46e-25; pw = pow_third(t); }
+  else if (t <= p.tb[5]) { chi = 5.49e-16; pw = 1.0 / t; }
   else { chi = 1.96e-27; pw = sqrt(t); }
   double v = n * n * chi * pw;
   if (t < p.cut_plus_ramp) {

@@ -19,6 +19,10 @@ struct TcParams {
 struct RlParams {
   double cut, ramp, cut_plus_ramp, eps, pd_eps;
   int prevent;
+  // the loss curve's branch bounds (radiativelosses.cpp:158-180 compares log10(T) with 4.97, 5.67,
+  // 6.18, 6.55, 6.90, 7.63): tb[k] = the largest T whose log10 under the host C library is at most
+  // the k-th of them, so that T <= tb[k] selects the branch the reference's log10 selects
+  double tb[6];
 };
 
 enum TcMode : int {

@@ -130,3 +130,39 @@ def test_subcycle_counts_match_reference_progress(case):
             assert got == progress_counts(line), "%s step %d: %s vs reference %r" % (case, s, got, line)
     finally:
         gpu.close()
+
+
+def test_radiative_loss_branches_at_their_bounds_match_restatement(cr_oracle):
+    """Temperatures within a few ulps of each log10 branch bound of the loss curve, on both sides:
+    the GPU selects every branch as the restatement's C library log10 does (bit-identical grids
+    after two steps)."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    from cases import random_mhd_state
+    from test_rl_bounds import LIM, _step, upper
+    st = random_mhd_state(24, 32, 77)
+    t = st.grids["temp"].copy()
+    vals = [_step(upper(c), k) for c in LIM for k in range(-6, 7)]
+    flat = t.reshape(-1)
+    for q, v in enumerate(vals):
+        flat[(q * 7) % flat.size] = v
+    st.grids["temp"] = t
+    cfg = W.base_config("ideal_mhd", ("periodic",) * 4, "euler", epsilon=0.1, density_min=1.0, temp_min=1.0,
+                        thermal_energy_min=1e-12,
+                        modules=[("radiative_losses", [("cutoff_ramp", "1e4"), ("cutoff_temp", "2e4"),
+                                                       ("epsilon", "0.5"), ("time_integrator", "euler")])])
+    spec = spec_from_files(cfg, st)
+    gpu = GpuDomain(spec)
+    ora = OracleDomain(spec)
+    try:
+        for s in range(2):
+            dg, sg = gpu.step(1, subcycles=True)
+            do = ora.step(1)
+            assert list(sg[0]) == ora.subcycles(), (s, sg[0], ora.subcycles())
+            assert dg[0] == do[0], (s, dg, do)
+        for v in VARS:
+            assert same_bits(gpu.grid(v), ora.grid(v)), v
+    finally:
+        gpu.close()
+        ora.close()
