@@ -205,7 +205,7 @@ void spruce_gpu_destroy(spruce_gpu_domain* d);
  * radiative_losses, artificial_viscosity (right-hand-side and hyperviscous terms), physical_viscosity,
  * field_heating, div_cleaning, boundary_outflow,
  * ambient_heating, ambient_heating_sink, localized_heating, mass_injection, momentum_injection and
- * parametric_heating;
+ * parametric_heating (any of them in inactive_mode too);
  * SPRUCE_E_UNSUPPORTED otherwise.  With modules or rk4 each step of the backward sweep is replayed
  * once more to keep its intermediate states (stages, subcycle temperatures, module outputs). */
 int spruce_gpu_adjoint(spruce_gpu_domain* d, int32_t nsteps, int32_t checkpoint_interval, const double* seed,

@@ -2636,12 +2636,11 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   rec_post.assign(mods.size(), false);
   for (size_t k = 0; k < mods.size(); ++k) {
     GpuModule* m = mods[k].get();
+    // (an inactive module computes its update without applying it: the step does not depend on it)
     if (auto* tc = dynamic_cast<TcModule*>(m)) {
-      require(!tc->inactive, SPRUCE_E_UNSUPPORTED, "the adjoint covers thermal conduction that is applied");
-      any_sat = any_sat || tc->p.saturation;
+      any_sat = any_sat || (tc->p.saturation && !tc->inactive);
       it_mods.push_back((int)k);
-    } else if (auto* rl = dynamic_cast<RlModule*>(m)) {
-      require(!rl->inactive, SPRUCE_E_UNSUPPORTED, "the adjoint covers radiative losses that are applied");
+    } else if (dynamic_cast<RlModule*>(m)) {
       it_mods.push_back((int)k);
     } else if (auto* pv = dynamic_cast<PvModule*>(m)) {
       it_mods.push_back((int)k);
@@ -2653,9 +2652,8 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
       rec_post[k] = true;
       any_dc = true;
     } else if (auto* fm = dynamic_cast<FieldHeatModule*>(m)) {
-      require(!fm->inactive, SPRUCE_E_UNSUPPORTED, "the adjoint covers field heating that is applied");
       it_mods.push_back((int)k);
-      any_fheat = true;
+      any_fheat = any_fheat || !fm->inactive;
     } else if (auto* vm = dynamic_cast<ViscModule*>(m)) {
       visc_mods.push_back(vm);                     // its right-hand-side terms
       if (vm->any_hyper) it_mods.push_back((int)k);   // its subcycled hyperviscous terms
@@ -3184,6 +3182,12 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
         const MhdSet& pre = k ? tape.after_it[k - 1] : Ln;
         GpuModule* m = mods[it_mods[k]].get();
         if (dynamic_cast<DivCleanModule*>(m)) continue;   // its iterate leaves the state alone
+        {                                                 // inactive modules leave it alone too
+          auto* tci = dynamic_cast<TcModule*>(m);
+          auto* rli = dynamic_cast<RlModule*>(m);
+          auto* fhi = dynamic_cast<FieldHeatModule*>(m);
+          if ((tci && tci->inactive) || (rli && rli->inactive) || (fhi && fhi->inactive)) continue;
+        }
         adj_ghost(post, lam);
         if (auto* tc = dynamic_cast<TcModule*>(m)) {
           const int ns = tc->nsub;

@@ -441,15 +441,37 @@ def test_module_adjoint_leaves_the_same_final_state_as_stepping(mods):
         b.close()
 
 
+def test_inactive_modules_leave_the_gradient_alone():
+    """Modules in inactive_mode compute their update without applying it: the adjoint with inactive
+    conduction, radiative losses and field heating equals the adjoint without them."""
+    st = _arcade()
+    off = [("inactive_mode", "true")]
+    inactive = [(TC[0], TC[1] + off), (RL()[0], RL()[1] + off),
+                ("field_heating", FHEAT[1] + off), HEAT(THETA[:8])]
+    rng = np.random.default_rng(51)
+    w = _weights(rng, (24, 40), (1e13, 1e6, 1.0, 1.0))
+    out = []
+    for mods in (inactive, [HEAT(THETA[:8])]):
+        d = GpuDomain(spec_from_files(_arcade_cfg(THETA[:8], mods=mods), st))
+        try:
+            g, dts = d.adjoint(5, 2, _seed(w, (24, 40)))
+            out.append((g, d.adjoint_params(), dts))
+        finally:
+            d.close()
+    (ga, pa, da), (gb, pb, db) = out
+    assert np.array_equal(da, db)
+    for k in range(8):
+        assert np.max(np.abs(ga[k] - gb[k])) <= 1e-12 * np.max(np.abs(gb[k])), k
+    assert np.max(np.abs(pa - pb)) <= 1e-12 * np.max(np.abs(pb))
+
+
 def test_module_adjoint_refuses_what_it_does_not_cover():
     st = _arcade()
-    fheat_off = ("field_heating", [("coeff", "3e3"), ("current_pow", "1.0"), ("inactive_mode", "true")])
     resist = ("anomalous_resistivity", [("resistivity_model", "time_scale"), ("time_scale", "1e-2"),
                                         ("template_mode", "flood_fill"), ("flood_fill_threshold", "1.9"),
                                         ("smoothing_sigma", "1.0"), ("flood_fill_argmin_radius", "6e7")])
     # the resistivity's flood-fill argmin box must stay inside a non-periodic x axis
-    for bc, mods in ((("periodic", "periodic", "fixed", "open"), [fheat_off]),
-                     (("reflect", "open", "fixed", "fixed"), [resist])):
+    for bc, mods in ((("reflect", "open", "fixed", "fixed"), [resist]),):
         d = GpuDomain(spec_from_files(_arcade_cfg(THETA, bc=bc, mods=mods), st))
         try:
             with pytest.raises(SpruceError):
