@@ -2617,6 +2617,24 @@ std::vector<double> Domain::param_grad() {
 
 void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, double* dt_out) {
   require(ready, SPRUCE_E_STATE, "setup must precede adjoint");
+  // however the call ends (a failure inside a step included), later steps record nothing: the
+  // tapes and records this call points the domain and its modules at are its own
+  struct RecordingReset {
+    Domain* d;
+    ~RecordingReset() {
+      d->rec = nullptr;
+      for (auto& m : d->mods) {
+        if (auto* tc = dynamic_cast<TcModule*>(m.get())) {
+          tc->recording = false;
+          tc->reuse = false;
+          tc->slot = -1;
+        }
+        if (auto* vm = dynamic_cast<ViscModule*>(m.get())) vm->recording = false;
+        if (auto* pv = dynamic_cast<PvModule*>(m.get())) pv->recording = false;
+        if (auto* dc = dynamic_cast<DivCleanModule*>(m.get())) dc->recording = false;
+      }
+    }
+  } recording_reset{this};
   const int integ = desc.time_integrator;
   require(integ == SPRUCE_EULER || integ == SPRUCE_RK2 || integ == SPRUCE_RK4, SPRUCE_E_UNSUPPORTED,
           "the adjoint covers euler, rk2 and rk4");

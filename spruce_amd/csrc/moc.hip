@@ -29,7 +29,7 @@ struct Ctx {
   Mesh M;
 };
 
-__device__ double fv(const Ctx& c, int f, int i, int j) {
+__device__ __forceinline__ double fv(const Ctx& c, int f, int i, int j) {
   const size_t k = (size_t)i * c.M.ny + j;
   switch (f) {
     case F_RHO: return c.S.rho[k];
@@ -53,7 +53,9 @@ __device__ double fv(const Ctx& c, int f, int i, int j) {
   }
 }
 
-__device__ __forceinline__ const Axis& axis_of(const Ctx& c, int a) { return a == 0 ? c.M.x : c.M.y; }
+// the axis by value: a reference to an axis picked at run time would place the context in private
+// memory
+__device__ __forceinline__ Axis axis_of(const Ctx& c, int a) { return a == 0 ? c.M.x : c.M.y; }
 __device__ __forceinline__ int wrapc(const Axis& A, int k) { return A.per ? (k + A.n) % A.n : k; }
 // value of field f at coordinate k along axis a through cell (i, j)
 __device__ __forceinline__ double at(const Ctx& c, int f, int a, int i, int j, int k) {
@@ -64,14 +66,14 @@ __device__ __forceinline__ bool in_range(const int* r, int i, int j) {
 }
 
 // boundaryInterpolate between coordinates k1 and k2 (derivs.cpp:477-487)
-__device__ double interp(const Ctx& c, int f, int a, int i, int j, int k1, int k2) {
+__device__ __forceinline__ double interp(const Ctx& c, int f, int a, int i, int j, int k1, int k2) {
   const Axis& A = axis_of(c, a);
   const double x1 = at(c, f, a, i, j, k1), x2 = at(c, f, a, i, j, k2);
   const double d1 = A.h[k1], d2 = A.h[k2];
   return (x1 * d2 + x2 * d1) / (d2 + d1);
 }
 // boundaryExtrapolate from k1 through k2 (derivs.cpp:490-499)
-__device__ double extrap(const Ctx& c, int f, int a, int i, int j, int k1, int k2) {
+__device__ __forceinline__ double extrap(const Ctx& c, int f, int a, int i, int j, int k1, int k2) {
   const Axis& A = axis_of(c, a);
   const double x1 = at(c, f, a, i, j, k1), x2 = at(c, f, a, i, j, k2);
   const double d1 = A.h[k1], d2 = A.h[k2];
@@ -79,21 +81,21 @@ __device__ double extrap(const Ctx& c, int f, int a, int i, int j, int k1, int k
 }
 
 // derivative1D over range r (derivs.cpp:223-264)
-__device__ double deriv(const Ctx& c, int f, int a, const int* r, int i, int j) {
+__device__ __forceinline__ double deriv(const Ctx& c, int f, int a, const int* r, int i, int j) {
   if (!in_range(r, i, j)) return 0.0;
   const Axis& A = axis_of(c, a);
   const int k = a == 0 ? i : j, m = wrapc(A, k - 1), p = wrapc(A, k + 1);
   return (interp(c, f, a, i, j, k, p) - interp(c, f, a, i, j, m, k)) / A.d[k];
 }
 // secondDerivative1D over range r (derivs.cpp:417-455)
-__device__ double second(const Ctx& c, int f, int a, const int* r, int i, int j) {
+__device__ __forceinline__ double second(const Ctx& c, int f, int a, const int* r, int i, int j) {
   if (!in_range(r, i, j)) return 0.0;
   const Axis& A = axis_of(c, a);
   const int k = a == 0 ? i : j, m = wrapc(A, k - 1), p = wrapc(A, k + 1);
   return (interp(c, f, a, i, j, k, p) - 2.0 * at(c, f, a, i, j, k) + interp(c, f, a, i, j, m, k)) / A.hsq[k];
 }
 // derivative1DBackward over range r (derivs.cpp:317-353)
-__device__ double back1(const Ctx& c, int f, int a, bool pf, const int* r, int i, int j) {
+__device__ __forceinline__ double back1(const Ctx& c, int f, int a, bool pf, const int* r, int i, int j) {
   if (!in_range(r, i, j)) return 0.0;
   const Axis& A = axis_of(c, a);
   const int k = a == 0 ? i : j, k0 = wrapc(A, pf ? k - 1 : k + 1);
@@ -101,7 +103,7 @@ __device__ double back1(const Ctx& c, int f, int a, bool pf, const int* r, int i
   return (pf ? 1.0 : -1.0) * (at(c, f, a, i, j, k) - at(c, f, a, i, j, k0)) / den;
 }
 // secondDerivative1DBackward over range r (derivs.cpp:356-404)
-__device__ double second_back(const Ctx& c, int f, int a, bool pf, const int* r, int i, int j) {
+__device__ __forceinline__ double second_back(const Ctx& c, int f, int a, bool pf, const int* r, int i, int j) {
   if (!in_range(r, i, j)) return 0.0;
   const Axis& A = axis_of(c, a);
   const int k = a == 0 ? i : j;
@@ -114,7 +116,7 @@ __device__ double second_back(const Ctx& c, int f, int a, bool pf, const int* r,
 }
 
 // Barton face value of face k (between k-1 and k) with the upwind side from vel (derivs.cpp:10-73)
-__device__ double upwind_face(const Ctx& c, int f, int vel, int a, int i, int j, int k) {
+__device__ __forceinline__ double upwind_face(const Ctx& c, int f, int vel, int a, int i, int j, int k) {
   const Axis& A = axis_of(c, a);
   const int k1 = wrapc(A, k - 1), k0 = wrapc(A, k - 2), k3 = wrapc(A, k + 1);
   const double d2 = interp(c, f, a, i, j, k1, k);
@@ -131,7 +133,7 @@ __device__ double upwind_face(const Ctx& c, int f, int vel, int a, int i, int j,
   return d2;
 }
 // transportDerivative1D over range r (derivs.cpp:122-162)
-__device__ double tderiv(const Ctx& c, int f, int vel, int a, const int* r, int i, int j) {
+__device__ __forceinline__ double tderiv(const Ctx& c, int f, int vel, int a, const int* r, int i, int j) {
   if (!in_range(r, i, j)) return 0.0;
   const Axis& A = axis_of(c, a);
   const int k = a == 0 ? i : j, m = wrapc(A, k - 1), p = wrapc(A, k + 1);
@@ -139,7 +141,7 @@ __device__ double tderiv(const Ctx& c, int f, int vel, int a, const int* r, int 
   return (sp * interp(c, vel, a, i, j, k, p) - sk * interp(c, vel, a, i, j, m, k)) / A.d[k];
 }
 // upwindSurfaceDirected face computed from cell k (derivs.cpp:77-117): face k for pf, k+1 otherwise
-__device__ double dir_face_of_cell(const Ctx& c, int f, int a, bool pf, int i, int j, int k) {
+__device__ __forceinline__ double dir_face_of_cell(const Ctx& c, int f, int a, bool pf, int i, int j, int k) {
   const int sg = pf ? 1 : -1;
   const int k1 = k - sg, k0 = k - 2 * sg;
   const double d2 = interp(c, f, a, i, j, k1, k), d3 = at(c, f, a, i, j, k1), d1 = extrap(c, f, a, i, j, k0, k1);
@@ -147,7 +149,7 @@ __device__ double dir_face_of_cell(const Ctx& c, int f, int a, bool pf, int i, i
 }
 // characteristicBartonDerivative1D at a cell of the side's ghost rows (derivs.cpp:167-214): the
 // outermost face is extrapolated from the cell centre through the next face
-__device__ double char_barton(const Ctx& c, int f, int a, bool pf, int i, int j) {
+__device__ __forceinline__ double char_barton(const Ctx& c, int f, int a, bool pf, int i, int j) {
   const Axis& A = axis_of(c, a);
   const int k = a == 0 ? i : j, n = A.n;
   double lo, hi;   // faces k and k+1
@@ -182,7 +184,15 @@ enum MocOp : int {
   OP_SI = 50,       // 3 second derivatives along the side (parallel range)
   MOC_NOPS = 53
 };
-__device__ double eval_op(const Ctx& c, const MocSide& sd, int k, int i, int j) {
+// element q of the list, as a chain of selects (a local array indexed at run time would live in
+// private memory)
+template <typename... T>
+__device__ __forceinline__ int pick(int q, T... v) {
+  int r = 0, n = 0;
+  ((r = (n++ == q) ? (int)v : r), ...);
+  return r;
+}
+__device__ __forceinline__ double eval_op(const Ctx& c, const MocSide& sd, int k, int i, int j) {
   const int a = sd.axis, pa = 1 - a;
   const bool pf = sd.lower == 0;
   const int Fvperp = a == 0 ? F_VX : F_VY, Fvpara = a == 0 ? F_VY : F_VX;
@@ -191,29 +201,26 @@ __device__ double eval_op(const Ctx& c, const MocSide& sd, int k, int i, int j) 
   const int Fb_perp = a == 0 ? F_BX : F_BY;
   const int Frv_perp = a == 0 ? F_RVX : F_RVY, Frv_para = a == 0 ? F_RVY : F_RVX;
   if (k < OP_ALT) {
-    const int f[11] = {Fbe_para, Fbi_para, F_BEZ, F_BIZ, Fvpara, Fvperp, F_VZ, F_P, Fb_perp, F_RHO, F_TE};
-    return char_barton(c, f[k - OP_CB], a, pf, i, j);
+    return char_barton(c, pick(k - OP_CB, Fbe_para, Fbi_para, F_BEZ, F_BIZ, Fvpara, Fvperp, F_VZ, F_P, Fb_perp, F_RHO,
+                               F_TE), a, pf, i, j);
   }
   if (k < OP_SB) {
-    const int f[5] = {F_P, Fbe_perp, Fbi_perp, F_BEZ, F_BIZ};
-    const int q = k - OP_ALT, fld = f[q / 3], w = q % 3;
+    const int q = k - OP_ALT, fld = pick(q / 3, F_P, Fbe_perp, Fbi_perp, F_BEZ, F_BIZ), w = q % 3;
     if (w == 0) return deriv(c, fld, pa, sd.rp, i, j);
     return w == 1 ? back1(c, fld, pa, true, sd.ru, i, j) : back1(c, fld, pa, false, sd.rl, i, j);
   }
   if (k < OP_DI) {
-    const int f[3] = {Fvpara, F_VZ, Fvperp};
-    return second_back(c, f[k - OP_SB], a, pf, sd.rb, i, j);
+    return second_back(c, pick(k - OP_SB, Fvpara, F_VZ, Fvperp), a, pf, sd.rb, i, j);
   }
   if (k < OP_TI) {
-    const int f[10] = {F_P, Fbe_para, Fbi_para, Fbe_perp, Fbi_perp, F_BEZ, F_BIZ, Fvpara, F_VZ, Fvperp};
-    return deriv(c, f[k - OP_DI], pa, sd.ri, i, j);
+    return deriv(c, pick(k - OP_DI, F_P, Fbe_para, Fbi_para, Fbe_perp, Fbi_perp, F_BEZ, F_BIZ, Fvpara, F_VZ, Fvperp),
+                 pa, sd.ri, i, j);
   }
   if (k < OP_SI) {
-    const int f[11] = {F_RHO, Frv_para, F_RVZ, Frv_perp, F_TE, Fbe_para, Fbi_para, F_BEZ, F_BIZ, Fbe_perp, Fbi_perp};
-    return tderiv(c, f[k - OP_TI], Fvpara, pa, sd.ri, i, j);
+    return tderiv(c, pick(k - OP_TI, F_RHO, Frv_para, F_RVZ, Frv_perp, F_TE, Fbe_para, Fbi_para, F_BEZ, F_BIZ,
+                          Fbe_perp, Fbi_perp), Fvpara, pa, sd.ri, i, j);
   }
-  const int f[3] = {Fvpara, F_VZ, Fvperp};
-  return second(c, f[k - OP_SI], pa, sd.ri, i, j);
+  return second(c, pick(k - OP_SI, Fvpara, F_VZ, Fvperp), pa, sd.ri, i, j);
 }
 
 // normal + parallel terms of one open_moc side at a cell of its region, in the order
@@ -369,7 +376,8 @@ __device__ __forceinline__ int slot_of(const MocArgs& a, int b, int i, int j) {
 }
 
 // one thread per (derivative, slot), derivative-major so that a wavefront evaluates one derivative
-// at 64 neighbouring cells (no divergence between the derivative kinds)
+// at 64 neighbouring cells (no divergence between the derivative kinds); ops is derivative-major
+// too (ops[k * slots + slot]), so a wavefront's stores are contiguous
 __global__ void __launch_bounds__(256) k_moc_ops(MocArgs a) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int total = a.slot_base[4];
@@ -377,11 +385,27 @@ __global__ void __launch_bounds__(256) k_moc_ops(MocArgs a) {
   const int k = (int)(t / total), slot = (int)(t % total);
   int b = 0;
   while (b < 3 && slot >= a.slot_base[b + 1]) ++b;
-  const int* r = a.side[b].rb;
+  // the side's description by selects from constant indices (indexing the kernel argument's array
+  // at run time would copy it to private memory)
+  MocSide sd;
+  const MocSide &s0 = a.side[0], &s1 = a.side[1], &s2 = a.side[2], &s3 = a.side[3];
+  auto sel = [&](int v0, int v1, int v2, int v3) { return b == 0 ? v0 : b == 1 ? v1 : b == 2 ? v2 : v3; };
+  sd.on = sel(s0.on, s1.on, s2.on, s3.on);
+  sd.axis = sel(s0.axis, s1.axis, s2.axis, s3.axis);
+  sd.lower = sel(s0.lower, s1.lower, s2.lower, s3.lower);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    sd.rb[q] = sel(s0.rb[q], s1.rb[q], s2.rb[q], s3.rb[q]);
+    sd.ri[q] = sel(s0.ri[q], s1.ri[q], s2.ri[q], s3.ri[q]);
+    sd.rp[q] = sel(s0.rp[q], s1.rp[q], s2.rp[q], s3.rp[q]);
+    sd.ru[q] = sel(s0.ru[q], s1.ru[q], s2.ru[q], s3.ru[q]);
+    sd.rl[q] = sel(s0.rl[q], s1.rl[q], s2.rl[q], s3.rl[q]);
+  }
+  const int* r = sd.rb;
   const int w = r[3] - r[1] + 1, loc = slot - a.slot_base[b];
   const int i = r[0] + loc / w, j = r[1] + loc % w;
   const Ctx c{a.S, a.st, a.K, a.M};
-  a.ops[(size_t)slot * MOC_NOPS + k] = eval_op(c, a.side[b], k, i, j);
+  a.ops[(size_t)k * total + slot] = eval_op(c, sd, k, i, j);
 }
 
 __global__ void __launch_bounds__(64) k_moc_rhs(MocArgs a) {
@@ -400,8 +424,10 @@ __global__ void __launch_bounds__(64) k_moc_rhs(MocArgs a) {
   for (int w = 0; w < 8; ++w) res[w] = 0.0;
   for (int b = 0; b < 4; ++b) {
     if (!a.side[b].on || !in_range(a.side[b].rb, i, j)) continue;
-    double t8[8];
-    side_terms(c, a.side[b], vc, i, j, a.ops + (size_t)slot_of(a, b, i, j) * MOC_NOPS, t8);
+    double t8[8], op[MOC_NOPS];
+    const int sl = slot_of(a, b, i, j), total = a.slot_base[4];
+    for (int k = 0; k < MOC_NOPS; ++k) op[k] = a.ops[(size_t)k * total + sl];
+    side_terms(c, a.side[b], vc, i, j, op, t8);
 #pragma unroll
     for (int w = 0; w < 8; ++w) res[w] = res[w] + t8[w];
   }
