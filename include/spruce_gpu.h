@@ -206,8 +206,10 @@ void spruce_gpu_destroy(spruce_gpu_domain* d);
  * field_heating, div_cleaning, boundary_outflow,
  * ambient_heating, ambient_heating_sink, localized_heating, mass_injection, momentum_injection and
  * parametric_heating (any of them in inactive_mode too);
- * SPRUCE_E_UNSUPPORTED otherwise.  With modules or rk4 each step of the backward sweep is replayed
- * once more to keep its intermediate states (stages, subcycle temperatures, module outputs). */
+ * SPRUCE_E_UNSUPPORTED otherwise.  With modules or rk4 the reverse needs each step's intermediate
+ * states (stages, conduction subcycle results, module outputs): the recompute of a segment keeps them
+ * for all its steps where they fit in 80% of the free device memory (see "adjoint_tc_records"), and
+ * each step of the backward sweep is replayed once more otherwise. */
 int spruce_gpu_adjoint(spruce_gpu_domain* d, int32_t nsteps, int32_t checkpoint_interval, const double* seed,
                        double* grad_out, double* dt_out);
 /* Gradient of the last spruce_gpu_adjoint call with respect to module parameters (no reference
