@@ -244,26 +244,39 @@ __global__ void __launch_bounds__(256) k_adj_dt_blocks(MhdSet L, MhdStatic st, M
   }
 }
 
-__global__ void __launch_bounds__(64) k_adj_dt_apply(MhdSet L, MhdStatic st, MhdConst K, Mesh M, double eps,
-                                                     const double* p1, const double* p2, const double* gh,
-                                                     const double* pmin, const int* pidx, int nb, double* lam_L) {
-  // the adjoint of the step size: the sum of the stage kernels' block partials, plus the modules'
-  double g = 0.0;
-  for (int k = threadIdx.x; k < nb; k += 64) g += (p1 ? p1[k] : 0.0) + (p2 ? p2[k] : 0.0);
-  for (int off = 32; off > 0; off >>= 1) g += __shfl_xor(g, off, 64);
-  if (gh) g += gh[0];
-  // (min, lowest index) over the block results: one wavefront, then a shuffle reduction; the rule
-  // is order-independent, so the cell is the same however the blocks are visited
-  double v = DBL_MAX;
+__global__ void __launch_bounds__(1024) k_adj_dt_apply(MhdSet L, MhdStatic st, MhdConst K, Mesh M, double eps,
+                                                       const double* p1, const double* p2, const double* gh,
+                                                       const double* pmin, const int* pidx, int nb, double* lam_L) {
+  // the adjoint of the step size: the sum of the stage kernels' block partials, plus the modules';
+  // and the (min, lowest index) over the block results, a rule independent of the visiting order.
+  // Sixteen wavefronts, each reduced by shuffles, then combined by the first thread.
+  __shared__ double sg[16], sv[16];
+  __shared__ int si[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  double g = 0.0, v = DBL_MAX;
   int idx = -1;
-  for (int k = threadIdx.x; k < nb; k += 64)
+  for (int k = t; k < nb; k += 1024) {
+    g += (p1 ? p1[k] : 0.0) + (p2 ? p2[k] : 0.0);
     if (pmin[k] < v || (pmin[k] == v && pidx[k] >= 0 && (idx < 0 || pidx[k] < idx))) { v = pmin[k]; idx = pidx[k]; }
+  }
   for (int off = 32; off > 0; off >>= 1) {
+    g += __shfl_xor(g, off, 64);
     const double ov = __shfl_xor(v, off, 64);
     const int oi = __shfl_xor(idx, off, 64);
     if (ov < v || (ov == v && oi >= 0 && (idx < 0 || oi < idx))) { v = ov; idx = oi; }
   }
-  if (threadIdx.x != 0 || idx < 0) return;
+  if (lane == 0) { sg[w] = g; sv[w] = v; si[w] = idx; }
+  __syncthreads();
+  if (t != 0) return;
+  g = 0.0;
+  v = DBL_MAX;
+  idx = -1;
+  for (int q = 0; q < 16; ++q) {
+    g += sg[q];
+    if (sv[q] < v || (sv[q] == v && si[q] >= 0 && (idx < 0 || si[q] < idx))) { v = sv[q]; idx = si[q]; }
+  }
+  if (gh) g += gh[0];
+  if (idx < 0) return;
   const int i = idx / M.ny, j = idx - i * M.ny;
   const Dual8 dt = dt_cell_dual(L, st, K, M, i, j);
   const size_t N = (size_t)M.nx * M.ny;
@@ -330,7 +343,7 @@ void launch_adj_dt(MhdSet L, MhdStatic st, MhdConst k, Mesh mesh, double eps, co
                    const double* gh, double* pmin, int* pidx, double* lam_L, hipStream_t s) {
   const dim3 b(64, 4), g = g2(mesh, b);
   hipLaunchKernelGGL(k_adj_dt_blocks, g, b, 0, s, L, st, k, mesh, pmin, pidx);
-  hipLaunchKernelGGL(k_adj_dt_apply, dim3(1), dim3(64), 0, s, L, st, k, mesh, eps, p1, p2, gh, (const double*)pmin,
+  hipLaunchKernelGGL(k_adj_dt_apply, dim3(1), dim3(1024), 0, s, L, st, k, mesh, eps, p1, p2, gh, (const double*)pmin,
                      (const int*)pidx, (int)(g.x * g.y), lam_L);
 }
 
