@@ -2943,6 +2943,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
     for (auto& m : mods) {
       GpuModule* g = m.get();
       if (dynamic_cast<TcModule*>(g)) any_tc = true;
+      else if (auto* vm = dynamic_cast<ViscModule*>(g)) ok = ok && !vm->any_hyper;   // its dt grid is formed again
       else if (!(dynamic_cast<RlModule*>(g) || dynamic_cast<ParamHeatModule*>(g) || dynamic_cast<HeatModule*>(g) ||
                  dynamic_cast<LocalHeatModule*>(g) || dynamic_cast<MassInjModule*>(g) ||
                  dynamic_cast<MomInjModule*>(g) || dynamic_cast<HeatSinkModule*>(g)))
@@ -3066,6 +3067,13 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
         if (bh) launch_bhat(Ln, st, mesh, bhx.p, bhy.p, stream);
         for (TcModule* t : slot_tcs)
           launch_bhat_derivs(bhx.p, bhy.p, mesh, t->dbxx.p, t->dbxy.p, t->dbyx.p, t->dbyy.p, stream);
+        // artificial viscosity's right-hand-side terms read the dt grid of the state the integrator
+        // starts from and its minimum (ViscModule::refresh before the integrator)
+        const MhdSet& Av = nit ? tape.after_it[nit - 1] : Ln;
+        for (ViscModule* vm : visc_mods) {
+          launch_mhd_dt(Av, st, kc, mesh, vm->dtg.p, vm->part.p, stream);
+          launch_reduce_step(vm->part.p, dt_blocks, 1.0, vm->dtmin.p, nullptr, 0, err_flag, stream);
+        }
       } else {
         for (TcModule* t : slot_tcs) { t->slot = i; t->reuse = true; }   // the reverse reads slot i too
         replay_step(n, Ln, tape, times);

@@ -123,3 +123,29 @@ def test_segment_records_match_per_step_replay(integ):
     for k in range(8):
         assert np.max(np.abs(ga[k] - gb[k])) <= 1e-10 * np.max(np.abs(gb[k])), k
     assert np.max(np.abs(pa - pb)) <= 1e-10 * np.max(np.abs(pb))
+
+
+def test_segment_records_with_viscosity_match_per_step_replay():
+    """Artificial viscosity's right-hand-side terms beside conduction: with whole-segment records the
+    reverse forms the terms' dt grid again from the state the integrator starts from; the gradients
+    equal those of per-step replays."""
+    from test_gpu_adjoint_modules import VISC
+    st = _arcade(42, 130, 3)
+    shape = (st.xdim, st.ydim)
+    rng = np.random.default_rng(19)
+    w = _weights(rng, shape, (1e13, 1e6, 1.0, 1.0))
+    mods = [TC, RL(), VISC("local,global", "0.1,0.05", "v_y,temp", "mom_y,thermal_energy"), HEAT(THETA[:8])]
+    cfg = _cfg(("periodic", "periodic", "fixed", "open"), "rk2", mods)
+    out = []
+    for mode in ("auto", "step"):
+        d = GpuDomain(spec_from_files(cfg, st), {"adjoint_tc_records": mode})
+        try:
+            g, _ = d.adjoint(6, 4, _seed(w, shape))
+            out.append((g, d.adjoint_params(), d.adjoint_device_bytes()))
+        finally:
+            d.close()
+    (ga, pa, ba), (gb, pb, bb) = out
+    assert ba > bb   # the whole-segment records were kept
+    for k in range(8):
+        assert np.max(np.abs(ga[k] - gb[k])) <= 1e-10 * np.max(np.abs(gb[k])), k
+    assert np.max(np.abs(pa - pb)) <= 1e-10 * np.max(np.abs(pb))
