@@ -2944,7 +2944,8 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
       GpuModule* g = m.get();
       if (dynamic_cast<TcModule*>(g)) any_tc = true;
       else if (auto* vm = dynamic_cast<ViscModule*>(g)) ok = ok && !vm->any_hyper;   // its dt grid is formed again
-      else if (!(dynamic_cast<RlModule*>(g) || dynamic_cast<ParamHeatModule*>(g) || dynamic_cast<HeatModule*>(g) ||
+      else if (!(dynamic_cast<RlModule*>(g) || dynamic_cast<FieldHeatModule*>(g) ||
+                 dynamic_cast<ParamHeatModule*>(g) || dynamic_cast<HeatModule*>(g) ||
                  dynamic_cast<LocalHeatModule*>(g) || dynamic_cast<MassInjModule*>(g) ||
                  dynamic_cast<MomInjModule*>(g) || dynamic_cast<HeatSinkModule*>(g)))
         ok = false;

@@ -149,3 +149,26 @@ def test_segment_records_with_viscosity_match_per_step_replay():
     for k in range(8):
         assert np.max(np.abs(ga[k] - gb[k])) <= 1e-10 * np.max(np.abs(gb[k])), k
     assert np.max(np.abs(pa - pb)) <= 1e-10 * np.max(np.abs(pb))
+
+
+def test_segment_records_with_field_heating_match_per_step_replay():
+    """Field heating reads b_hat of the step's start state, which the reverse forms again with
+    whole-segment records: the gradients equal those of per-step replays."""
+    from test_gpu_adjoint_modules import FHEAT
+    st = random_mhd_state(40, 48, 97, stretch=True)
+    shape = (st.xdim, st.ydim)
+    rng = np.random.default_rng(13)
+    w = _weights(rng, shape, (1e15, 1e8, 10.0, 1.0))
+    cfg = _cfg(("periodic", "periodic", "fixed", "reflect"), "rk2", [FHEAT], temp_min=1.0, te_min=1e-12)
+    out = []
+    for mode in ("auto", "step"):
+        d = GpuDomain(spec_from_files(cfg, st), {"adjoint_tc_records": mode})
+        try:
+            g, _ = d.adjoint(6, 3, _seed(w, shape))
+            out.append((g, d.adjoint_device_bytes()))
+        finally:
+            d.close()
+    (ga, ba), (gb, bb) = out
+    assert ba > bb   # the whole-segment records were kept
+    for k in range(8):
+        assert np.max(np.abs(ga[k] - gb[k])) <= 1e-10 * np.max(np.abs(gb[k])), k
