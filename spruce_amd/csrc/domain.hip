@@ -2885,11 +2885,19 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   // live grids, so a copy is restored at the end
   DBuf keep;
   MhdSet kept{};
+  std::vector<std::pair<OutflowModule*, DBuf>> kept_sc;   // the outflow scalars the last step left
   if (replay) {
     keep.alloc(SET);
     kept = set_at(keep.p, N);
     copy_set(L.view(), kept, N, stream);
     adj_bytes += (int64_t)(SET * sizeof(double));
+    for (auto& m : mods)
+      if (auto* om = dynamic_cast<OutflowModule*>(m.get())) {
+        kept_sc.emplace_back(om, DBuf());
+        kept_sc.back().second.alloc(2);
+        hip_check(hipMemcpyAsync(kept_sc.back().second.p, om->sc.p, 2 * sizeof(double), hipMemcpyDeviceToDevice,
+                                 stream), "keep outflow scalars");
+      }
   }
   // Conduction records of whole segments: the segment recompute keeps every step's subcycle
   // results, so the replay of a step skips its subcycles.  Slot i holds step n0 + i of the segment
@@ -2944,7 +2952,7 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
       GpuModule* g = m.get();
       if (dynamic_cast<TcModule*>(g)) any_tc = true;
       else if (auto* vm = dynamic_cast<ViscModule*>(g)) ok = ok && !vm->any_hyper;   // its dt grid is formed again
-      else if (!(dynamic_cast<RlModule*>(g) || dynamic_cast<FieldHeatModule*>(g) ||
+      else if (!(dynamic_cast<RlModule*>(g) || dynamic_cast<FieldHeatModule*>(g) || dynamic_cast<OutflowModule*>(g) ||
                  dynamic_cast<ParamHeatModule*>(g) || dynamic_cast<HeatModule*>(g) ||
                  dynamic_cast<LocalHeatModule*>(g) || dynamic_cast<MassInjModule*>(g) ||
                  dynamic_cast<MomInjModule*>(g) || dynamic_cast<HeatSinkModule*>(g)))
@@ -3090,6 +3098,9 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
         auto* ph = dynamic_cast<ParamHeatModule*>(m);
         if (auto* om = dynamic_cast<OutflowModule*>(m)) {     // boundaryoutflow.cpp:39-67
           adj_ghost(post, lam);
+          // without a replay, the window's largest outflow speed, its cell and the acceleration are
+          // formed again from the state the module acted on, as the step formed them
+          if (seg_tapes) launch_outflow_max(pre, st, om->p, mesh, om->part.p, om->sc.p, 1, stream);
           timed("adj_outflow", [&] {
             launch_adj_minmask(pre.rho, pre.te, nullptr, nullptr, kc, mesh, lam, nullptr, stream);
             const int onb = outflow_blocks(om->p);
@@ -3448,6 +3459,9 @@ void Domain::adjoint(int nsteps, int ckpt, const double* seed, double* grad, dou
   hip_check(hipMemcpyAsync(grad, lam, 8 * N * 8, hipMemcpyDeviceToHost, stream), "download gradient");
   if (replay) {                                         // the state after the steps and its step size
     copy_set(kept, L.view(), N, stream);
+    for (auto& o : kept_sc)
+      hip_check(hipMemcpyAsync(o.first->sc.p, o.second.p, 2 * sizeof(double), hipMemcpyDeviceToDevice, stream),
+                "restore outflow scalars");
     finish_step(-1);
     for (auto& m : mods)                                  // conduction's subcycle record
       if (auto* tc = dynamic_cast<TcModule*>(m.get())) adj_bytes += (int64_t)(tc->tape.size() * N * sizeof(double));

@@ -172,3 +172,38 @@ def test_segment_records_with_field_heating_match_per_step_replay():
     assert ba > bb   # the whole-segment records were kept
     for k in range(8):
         assert np.max(np.abs(ga[k] - gb[k])) <= 1e-10 * np.max(np.abs(gb[k])), k
+
+
+def test_segment_records_with_outflow_match_per_step_replay():
+    """Boundary outflow in dynamic mode (its acceleration follows the window's largest outflow speed
+    of each step): with whole-segment records the reverse forms that speed, its cell and the
+    acceleration again from the state the module acted on; the gradients equal those of per-step
+    replays, and after either the module reports the scalars of the last forward step."""
+    from test_gpu_adjoint_modules import OUTFLOW
+    st = random_mhd_state(40, 48, 97, stretch=True)
+    shape = (st.xdim, st.ydim)
+    rng = np.random.default_rng(17)
+    w = _weights(rng, shape, (1e15, 1e8, 10.0, 1.0))
+    mod = OUTFLOW(max_accel="1e9", falloff_length="5e7", boundary="x_bound_2", falloff_shape="gaussian",
+                  feather_length="2e7", field_aligned_mode="true", dynamic_mode="true", dynamic_time="1.0",
+                  dynamic_target_speed="3e7")
+    cfg = _cfg(("reflect", "open", "periodic", "periodic"), "rk2", [mod], temp_min=1.0, te_min=1e-12)
+    ref = GpuDomain(spec_from_files(cfg, st))
+    try:
+        ref.step(6)
+        want = ref.module_report(0)
+    finally:
+        ref.close()
+    out = []
+    for mode in ("auto", "step"):
+        d = GpuDomain(spec_from_files(cfg, st), {"adjoint_tc_records": mode})
+        try:
+            g, _ = d.adjoint(6, 3, _seed(w, shape))
+            out.append((g, d.adjoint_device_bytes()))
+            assert np.array_equal(d.module_report(0), want), mode
+        finally:
+            d.close()
+    (ga, ba), (gb, bb) = out
+    assert ba > bb   # the whole-segment records were kept
+    for k in range(8):
+        assert np.max(np.abs(ga[k] - gb[k])) <= 1e-10 * np.max(np.abs(gb[k])), k
